@@ -1,0 +1,262 @@
+"""Benchmark: streaming cone-beam FDK on one MI355X, full-resolution workload.
+
+Workload (full_cone): 30-ellipsoid phantom, 1200 views over 360 deg, 1536 x 1944
+detector at 0.075 mm, SOD 400 / SDD 600, ram-lak.  Projections are generated on the
+GPU (exact line integrals), filtered into HBM-resident rings stored as float32,
+float16 (per-view scale) and bfloat16.  One step = one 3-slice update: three oblique
+1024^2 slices through the centre back-projected from all 1200 views, the three
+slices in one launch, output left in HBM.
+
+Reported (one JSON line on stdout):
+  value      GUPS of the FP16-stored batched update = in-box slice points x views / s
+  ms_per_step, plus per-form ms / GUPS / rel-L2 (f32 batched, f32 as three separate
+  launches, f16, bf16); rel-L2 and max-abs against the float64 oracle at 3 x 2048
+  sampled in-box pixels (float64 filtering and back-projection of the same float32
+  projections, oracle/liboracle_exact.so).
+  roofline   for the back-projection kernel: compulsory HBM bytes (union of the 2x2 tap
+             footprints over all requested points, per view, x bytes per sample, plus
+             the output) / average kernel time (HIP events), against 8 TB/s.
+  cpu_baseline  the Numba-equivalent CPU restatement (oracle/liboracle_fast.so:
+             fast-math, OpenMP over points, views serial, float64 accumulation),
+             timed on this host on a bounded sample of the same update.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-check] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tomography-local-demo_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def footprint_pixels(geom, pts, views):
+    """Mean number of detector pixels touched by the 2x2 taps of `pts` over `views`."""
+    from minirecast.recon import cone_constants
+    K = cone_constants(geom.vectors[views])
+    R, C = geom.rows, geom.cols
+    counts = []
+    mark = np.zeros(R * C, dtype=bool)
+    for k in range(len(views)):
+        rel = pts - K['s'][k]
+        L = rel @ K['n'][k]
+        X = K['s'][k] + (K['sdd'][k] / L)[:, None] * rel - K['d'][k]
+        col = X @ K['uu'][k] + (C - 1) / 2.0
+        row = X @ K['vv'][k] + (R - 1) / 2.0
+        c0 = np.floor(col).astype(np.int64)
+        r0 = np.floor(row).astype(np.int64)
+        mark[:] = False
+        for dr in (0, 1):
+            for dc in (0, 1):
+                rr, cc = r0 + dr, c0 + dc
+                ok = (rr >= 0) & (rr < R) & (cc >= 0) & (cc < C)
+                mark[rr[ok] * C + cc[ok]] = True
+        counts.append(int(mark.sum()))
+    return float(np.mean(counts))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-check", action="store_true", help="skip the float64 oracle error")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--views", type=int, default=None, help="debug: fewer views")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
+              f"(this build has no multi-GPU path; see DESIGN.md for how it would shard)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from minirecast import _hip, phantom, recon, workloads
+    w = workloads.full_cone()
+    geom = w.geom
+    if args.views:
+        from minirecast.geometry import ConeVecGeometry
+        geom = ConeVecGeometry(geom.vectors[:: max(1, 1200 // args.views)][: args.views],
+                               geom.rows, geom.cols)
+    A, R, C = geom.n_angles, geom.rows, geom.cols
+    box = w.geom.volume_box()
+    log(f"workload full_cone: {A} views, {R}x{C} detector, box {box}")
+
+    # ---------------------------------------------------------------- rings
+    t0 = time.perf_counter()
+    forms = ("f32", "f16", "bf16")
+    ctx = {s: _hip.Context(0) for s in forms}
+    for s in forms:
+        ctx[s].set_cone(geom.vectors, R, C, w.window, s)
+    gen = ctx["f32"]
+    batch = 40
+    d_raw = gen.alloc(R * batch * C * 4)
+    check = not args.no_check
+    # float64 oracle at sampled in-box pixels of the three planes
+    rng = np.random.default_rng(12345)
+    samp_idx, samp_pts, masks = [], [], []
+    for p in w.planes:
+        pts, mask = recon.slice_points(box, p['origin'], p['normal'], p['n'])
+        masks.append(mask)
+        flat = np.flatnonzero(mask.ravel())
+        sel = np.sort(rng.choice(flat, size=2048, replace=False))
+        samp_idx.append(sel)
+        samp_pts.append(pts.reshape(-1, 3)[sel])
+    samp_all = np.concatenate(samp_pts)
+    acc64 = np.zeros(len(samp_all))
+    n_in = int(sum(m.sum() for m in masks))
+    if check:
+        from oracle import oracle_c as OC
+        OC.set_threads(min(16, len(os.sched_getaffinity(0))))
+    host_raw = np.empty((R, batch, C), np.float32)
+    oracle_s = 0.0
+    filt_ms = {s: 0.0 for s in forms}
+    for b0 in range(0, A, batch):
+        k = min(batch, A - b0)
+        idx = np.arange(b0, b0 + k)
+        ph_geom = type(geom)(geom.vectors[idx], R, C)
+        phantom.project_cone_device(w.phantom, ph_geom, gen, d_raw)
+        for s in forms:
+            ts = time.perf_counter()
+            ctx[s].push_device(idx, d_raw)
+            filt_ms[s] += (time.perf_counter() - ts) * 1e3
+        if check:
+            to = time.perf_counter()
+            raw = host_raw[:, :k, :] if k == batch else np.empty((R, k, C), np.float32)
+            gen.d2h(raw, d_raw)
+            for j in range(k):
+                q64 = OC.fdk_filter_view(raw, j, geom.vectors[b0 + j], w.window)
+                OC.bp_cone_acc_view(q64, geom.vectors[b0 + j], 0.0, samp_all, acc64)
+            oracle_s += time.perf_counter() - to
+    gen.free(d_raw)
+    ref = acc64 * (np.pi / A)
+    log(f"ingest done in {time.perf_counter() - t0:.1f}s (oracle {oracle_s:.1f}s); "
+        f"filter+store ms/view: " + ", ".join(f"{s} {filt_ms[s] / A:.3f}" for s in forms))
+
+    # ---------------------------------------------------------------- timed updates
+    tasks = []
+    off = 0
+    for p in w.planes:
+        tasks.append(recon.plane_task(box, p['origin'], p['normal'], p['n'], out_offset=off))
+        off += p['n'] ** 2
+    tasks = np.array(tasks, dtype=_hip.TASK_DTYPE)
+    out_len = off
+    results = {}
+    for s in forms:
+        c = ctx[s]
+        dout = c.alloc(out_len * 4)
+        for _ in range(args.warmup):
+            c.bp_tasks_device(tasks, A, dout, out_len)
+        c.synchronize()
+        kms = []
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            c.bp_tasks_device(tasks, A, dout, out_len)
+            kms.append(c.last_kernel_ms())
+        c.synchronize()
+        wall = (time.perf_counter() - t1) * 1e3 / args.steps
+        img = np.empty(out_len, np.float32)
+        c.d2h(img, dout)
+        res = dict(ms=wall, kernel_ms=float(np.mean(kms)), gups=n_in * A / (wall * 1e-3) / 1e9)
+        if check:
+            got = np.concatenate([img[o:o + p['n'] ** 2][si] for o, p, si in
+                                  zip(np.cumsum([0] + [p['n'] ** 2 for p in w.planes[:-1]]),
+                                      w.planes, samp_idx)]).astype(np.float64)
+            res["rel_l2"] = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+            res["max_abs"] = float(np.max(np.abs(got - ref)))
+        results[s] = res
+        if s == "f32":
+            # plain path: the three slices as three separate launches
+            for _ in range(args.warmup):
+                for t in tasks:
+                    c.bp_tasks_device(t[None], A, dout, out_len)
+            c.synchronize()
+            t1 = time.perf_counter()
+            kms = []
+            for _ in range(args.steps):
+                km = 0.0
+                for t in tasks:
+                    c.bp_tasks_device(t[None], A, dout, out_len)
+                    km += c.last_kernel_ms()
+                kms.append(km)
+            c.synchronize()
+            wall = (time.perf_counter() - t1) * 1e3 / args.steps
+            results["f32_separate"] = dict(ms=wall, kernel_ms=float(np.mean(kms)),
+                                           gups=n_in * A / (wall * 1e-3) / 1e9)
+        c.free(dout)
+        log(f"{s}: {results[s]}")
+    log(f"f32 three launches: {results['f32_separate']}")
+
+    # ---------------------------------------------------------------- roofline
+    all_pts = []
+    for p, m in zip(w.planes, masks):
+        pts, _ = recon.slice_points(box, p['origin'], p['normal'], p['n'])
+        all_pts.append(pts.reshape(-1, 3)[m.ravel()])
+    all_pts = np.concatenate(all_pts)
+    fviews = np.linspace(0, A - 1, 8).astype(int)
+    F = footprint_pixels(geom, all_pts, fviews)
+    head = "f16"
+    hbm_bytes = F * A * 2 + out_len * 4
+    kern_s = results[head]["kernel_ms"] * 1e-3
+    achieved = hbm_bytes / kern_s / 1e9
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    kernel="bp_tiles_kernel<cone, __half>",
+                    algorithmic_bytes=int(hbm_bytes), footprint_px_per_view=round(F),
+                    gather_bytes_l2=int(4 * 2 * n_in * A))
+    log(f"roofline: {roofline}")
+
+    # ---------------------------------------------------------------- CPU baseline
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle_c as OC
+        threads = min(16, len(os.sched_getaffinity(0)))
+        OC.set_threads(threads, "fast")
+        sub = np.arange(0, A, max(1, A // 100))[:100]        # 100 of the views
+        q = np.empty((len(sub), R, C), np.float32)
+        for i, v in enumerate(sub):
+            q[i] = ctx["f32"].read_views(int(v), 1, R, C)[0]
+        order = np.arange(A)            # batch ingest: slot v holds view v
+        K = recon.cone_constants(geom.vectors[order[sub]])
+        tc = time.perf_counter()
+        OC.bp_cone(q, K['s'], K['d'], K['uu'], K['vv'], K['n'], K['sdd'], K['sod'], all_pts,
+                   np.pi / A, kind="fast")
+        dt = time.perf_counter() - tc
+        gups = len(all_pts) * len(sub) / dt / 1e9
+        cpu = dict(value=round(gups, 4), unit="GUPS", cores=threads, kind="port",
+                   sample=f"all {len(all_pts)} in-box points of the 3 slices x {len(sub)} of the "
+                          f"{A} views ({dt:.1f} s); ms per full 3-slice update extrapolated: "
+                          f"{n_in * A / gups / 1e9 * 1e3:.0f}")
+        log(f"cpu baseline: {cpu}")
+
+    h = results[head]
+    line = {
+        "metric": "ms per 3-slice update + GUPS at 1200 views; fp16-store vs fp32 @ rel-L2 err",
+        "value": round(h["gups"], 2), "unit": "GUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(h["ms"], 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded 30-ellipsoid phantom, exact line integrals)",
+        "config": {"workload": "full_cone: 3 oblique 1024^2 slices from 1200 views, "
+                               "1536x1944 detector, ram-lak",
+                   "storage": head, "batched": True, "points_in_box": n_in, "views": A},
+        "forms": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                  for k, v in results.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,170 @@
+/*
+ * minirecast_hip.h -- C ABI of libminirecast_hip.so, the MI355X (gfx950) streaming
+ * FDK/FBP engine behind the minirecast.recon / minirecast.preprocess Python API.
+ *
+ * Plain C types only: pointers, sizes, doubles.  Every function returns MR_OK (0)
+ * or a negative MR_E* code; mr_last_error() then holds a thread-local message.
+ * Host arrays are caller-owned, C-contiguous and only read during the call.
+ * Pointers documented as "device" must come from the same GPU as the context.
+ *
+ * Each entry point cites the reference function it replaces (paths relative to
+ * the minirecast reference package).
+ */
+#ifndef MINIRECAST_HIP_H
+#define MINIRECAST_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes: the Python layer maps them to the reference's exception types */
+#define MR_OK            0
+#define MR_EVALUE       -1   /* ValueError: bad shape, count, window, index        */
+#define MR_ERUNTIME     -2   /* RuntimeError: HIP/device failure, no data          */
+#define MR_ETYPE        -3   /* TypeError: wrong geometry kind for the call        */
+
+/* ---- filter windows (recon.py:45-54) */
+#define MR_WINDOW_RAMLAK        0   /* "ram-lak" / "none" / None */
+#define MR_WINDOW_SHEPP_LOGAN   1
+#define MR_WINDOW_HANN          2
+#define MR_WINDOW_COSINE        3
+
+/* ---- storage of the filtered-projection ring (recon.py:320 stores float32) */
+#define MR_STORE_F32   0
+#define MR_STORE_F16   1   /* IEEE half, one power-of-two scale per view */
+#define MR_STORE_BF16  2   /* bfloat16, round-to-nearest-even           */
+
+/* ---- geometry kinds */
+#define MR_KIND_NONE      0
+#define MR_KIND_CONE      1
+#define MR_KIND_PARALLEL  2
+
+typedef struct mr_ctx mr_ctx;
+
+/* Thread-local message of the last failing call on this thread. */
+const char* mr_last_error(void);
+
+/* ABI version (bumped on any signature change). */
+int mr_abi_version(void);
+
+/* Number of visible HIP devices (0 when none); never fails on a machine without GPUs. */
+int mr_device_count(int* n);
+
+/* Windowed discrete Ram-Lak response: replaces recon.py:34-55 ramp_response.
+ * Writes N = 2^ceil(log2(2n)) to *N and, when H is non-NULL, H[0..N-1] (float64),
+ * computed in float64 on the host.  MR_EVALUE for an unknown window or n < 1. */
+int mr_ramp_response(int n, double tau, int window, double* H, int* N);
+
+/* A reconstruction context owns the device ring, per-view geometry and streams of
+ * one SliceReconstructor (recon.py:298-343).  device = HIP ordinal. */
+int mr_ctx_create(int device, mr_ctx** out);
+int mr_ctx_destroy(mr_ctx* ctx);
+
+/* Cone-beam geometry (recon.py:327-330 via cone_constants, recon.py:70-82).
+ * vectors: [n_total][12] float64 ASTRA cone_vec rows (src, det centre, u, v in mm).
+ * Allocates and zeroes the ring for n_total views of rows x cols. */
+int mr_set_geometry_cone(mr_ctx* ctx, int n_total, int rows, int cols,
+                         const double* vectors, int window, int storage);
+
+/* Parallel-beam geometry (recon.py:322-326): angles [n_total] in radians,
+ * pixel sizes in mm. */
+int mr_set_geometry_parallel(mr_ctx* ctx, int n_total, int rows, int cols,
+                             const double* angles, double pixel_u, double pixel_v,
+                             int window, int storage);
+
+/* Ingest: replaces the filter + store steps of recon.py:346-376 add_projections
+ * (fdk_filter_block recon.py:85-104 / filter_rows recon.py:58-64).
+ * indices[k]: geometry indices of the views (the caller has already dropped the
+ * ones received before, recon.py:353-356).  block: [rows][k][cols] float32, a host
+ * pointer (block_on_device = 0) or a device pointer (1).  The filtered views are
+ * appended in arrival order at slots n..n+k-1; *n_out receives the new count. */
+int mr_push_views(mr_ctx* ctx, const int64_t* indices, int k, const float* block,
+                  int block_on_device, int* n_out);
+
+/* Appends already-filtered views q [k][rows][cols] float32 (host or device pointer)
+ * at slots n..n+k-1 with the geometry of `indices`, quantising for F16/BF16 storage:
+ * the store step of recon.py:363-375 without filtering.  Used by the reference's
+ * module-level bp_cone_* / bp_parallel_* signatures (recon.py:129-219), which take
+ * filtered views, and by the adjoint. */
+int mr_store_views(mr_ctx* ctx, const int64_t* indices, int k, const float* q, int q_on_device,
+                   int* n_out);
+
+/* Views stored so far (recon.py:319 self.n). */
+int mr_num_views(mr_ctx* ctx, int* n);
+
+/* Back-projection of an arbitrary point list: replaces recon.py:405-424
+ * reconstruct_points -> bp_cone_numba (recon.py:193-219) / bp_parallel_numba
+ * (recon.py:175-191).  pts: host [M][3] float64 (mm); uses the first n_views
+ * stored views (a snapshot, recon.py:401-403) with scale pi / n_views.
+ * out: host [M] float64.  n_views = 0 gives zeros. */
+int mr_bp_points(mr_ctx* ctx, const double* pts, int64_t M, double cor_shift,
+                 int n_views, double* out);
+
+/* A slice / grid task: points x(q, p, s) = o + q*c + p*a + s*b (mm, float64),
+ * q < nq, p < np, s < ns, written to
+ * out[out_offset + q*stride_q + p*stride_p + s*stride_s].
+ * mask_box = 1 evaluates the in-box mask of recon.py:253-265 (|x| <= box per axis)
+ * with float64 points built in numpy's operation order pts = (mo + t_s*me1) + t_p*me2,
+ * t_x = (x - mhalf) * mstep, and writes 0 outside (recon.py:434-440).
+ * cor_shift: rotation-centre shift in detector columns (recon.py:412-424). */
+typedef struct mr_task {
+    double o[3], a[3], b[3], c[3];
+    double mo[3], me1[3], me2[3];
+    double box[3];
+    double mhalf, mstep;
+    double cor_shift;
+    int64_t out_offset, stride_q, stride_p, stride_s;
+    int32_t nq, np, ns, mask_box;
+} mr_task;
+
+/* Back-projection of a batch of tasks in one launch: replaces recon.py:426-440
+ * reconstruct_plane (one task per plane), recon.py:442-475 reconstruct_volume /
+ * preview_volume (one task for the whole grid) and the trial slices of
+ * recon.py:498-512 estimate_cor.  out: float32, host (out_on_device = 0) or device
+ * (1), sized by the caller to hold every task's output range. */
+int mr_bp_tasks(mr_ctx* ctx, const mr_task* tasks, int ntasks, int n_views,
+                float* out, int64_t out_len, int out_on_device);
+
+/* Rotation-centre search scores: recon.py:498-513.  Each task is one trial image
+ * (nq = 1, all np x ns equal; out_offset/strides are ignored): it is back-projected
+ * into a device buffer and scored on the device as sum p ln p of a 128-bin
+ * histogram of its non-zero values (np.histogram range min..max).
+ * scores: host [ntasks] float64. */
+int mr_cor_scores(mr_ctx* ctx, const mr_task* tasks, int ntasks, int n_views, double* scores);
+
+/* Parity hook for filter_rows (recon.py:58-64): ramp-filters `rows` host rows of
+ * length n (float32) with pitch tau on the context's device. */
+int mr_filter_rows(mr_ctx* ctx, const float* in, int rows, int n, double tau,
+                   int window, float* out);
+
+/* Reads back stored filtered views [first, first+count) as float32 [count][rows][cols]
+ * (dequantised for F16/BF16 storage).  Parity hook for recon.py:365 (self._q). */
+int mr_read_views(mr_ctx* ctx, int first, int count, float* out);
+
+/* Synthetic acquisition (phantom.py:56-98 line integrals of constant ellipsoids,
+ * float64 per ray, stored float32).  ellipsoids: [K][13] = value, centre[3], M[9]
+ * (row-major; M maps x - centre to the unit sphere).  Cone: vectors [A][12];
+ * parallel: angles [A], pixel sizes, start distance `far`.  out: device pointer
+ * to [rows][A][cols] float32. */
+int mr_phantom_cone(mr_ctx* ctx, const double* ellipsoids, int K, const double* vectors,
+                    int A, int rows, int cols, float* out_device);
+int mr_phantom_parallel(mr_ctx* ctx, const double* ellipsoids, int K, const double* angles,
+                        int A, int rows, int cols, double pixel_u, double pixel_v,
+                        double far, float* out_device);
+
+/* Device allocation helpers for callers without another allocator (bench, tests). */
+int mr_device_alloc(mr_ctx* ctx, int64_t bytes, void** ptr);
+int mr_device_free(mr_ctx* ctx, void* ptr);
+int mr_memcpy(mr_ctx* ctx, void* dst, const void* src, int64_t bytes, int kind /* 0 H2D, 1 D2H, 2 D2D */);
+int mr_synchronize(mr_ctx* ctx);
+
+/* Timing of the last mr_bp_tasks / mr_bp_points kernel on the request stream,
+ * measured with HIP events around the back-projection kernel (milliseconds). */
+int mr_last_kernel_ms(mr_ctx* ctx, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MINIRECAST_HIP_H */

@@ -1,0 +1,284 @@
+"""GPU parity tests: the HIP path against the float64 oracle and the reference's
+composition fixtures, on the same inputs.
+
+Tolerance (floating point): relative L2 <= 1e-5 for the plain FP32 path against the
+reference-equivalent float64 result; FP16 (per-view scale) <= 1e-3 and BF16 <= 5e-3
+against the float64 oracle.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _box_cone(V, R, C):
+    s, d, u, v = np.split(np.asarray(V[0], float), 4)
+    mag = np.linalg.norm(d - s) / np.linalg.norm(s)
+    return np.array([C * np.linalg.norm(u) / mag / 2] * 2 + [R * np.linalg.norm(v) / mag / 2])
+
+
+# ---------------------------------------------------------------- filtering
+@pytest.mark.parametrize("n", [2, 5, 96, 486, 1944, 2048])
+def test_filter_rows_vs_oracle(n):
+    from minirecast import recon
+    rng = np.random.default_rng(n)
+    x = rng.normal(size=(7, n)).astype(np.float32)
+    for window in O.WINDOWS:
+        got = recon.filter_rows(x, 0.3, window)
+        want = O.filter_rows(x, 0.3, window)
+        assert got.dtype == np.float32 and got.shape == x.shape
+        assert rel_l2(got, want) < TOL, (n, window, rel_l2(got, want))
+
+
+def test_fdk_filter_block_vs_oracle():
+    from minirecast import recon
+    V = O.circular_cone_vectors(5, 400.0, 600.0, 0.3, det_shift_u=1.5)
+    K = O.cone_constants(V)
+    rng = np.random.default_rng(7)
+    block = rng.uniform(0, 2, size=(33, 5, 96)).astype(np.float32)   # odd row count
+    got = recon.fdk_filter_block(block, K, "shepp-logan")
+    want = O.fdk_filter_block(block, K, "shepp-logan")
+    assert got.shape == (5, 33, 96)
+    assert rel_l2(got, want) < TOL
+
+
+# ---------------------------------------------------------------- reference composition
+@pytest.fixture(scope="module")
+def golden_cone(golden):
+    from minirecast import recon
+    from minirecast.geometry import ConeVecGeometry
+    R, C = (int(x) for x in golden['cone_shape'])
+    g = ConeVecGeometry(golden['cone_vectors'], R, C)
+    return recon.SliceReconstructor(golden['cone_proj'], g, verbose=False)
+
+
+def test_reference_planes_cone(golden, golden_cone):
+    rec = golden_cone
+    for i in range(4):
+        a = golden[f'cone_plane{i}_args']
+        c = None if np.isnan(a[9]) else a[9:12]
+        img, pts = rec.reconstruct_plane(a[0:3], a[3:6], n=int(a[6]), size=a[7] or None,
+                                         zoom=a[8], center=c)
+        ref = golden[f'cone_plane{i}']
+        assert img.dtype == np.float64 and img.shape == ref.shape
+        np.testing.assert_array_equal(img == 0, ref == 0)            # same in-box mask
+        assert rel_l2(img, ref) < TOL, (i, rel_l2(img, ref))
+    img, _ = rec.reconstruct_plane((0, 0, 0), (1, 0.5, 0.8), n=32, cor_shift=1.25)
+    assert rel_l2(img, golden['cone_plane_cor']) < TOL
+
+
+def test_reference_preview_and_cor_cone(golden, golden_cone):
+    rec = golden_cone
+    vol, origin, h = rec.preview_volume(12)
+    assert vol.dtype == np.float32 and vol.shape == golden['cone_preview'].shape
+    np.testing.assert_array_equal(origin, golden['cone_preview_origin'])
+    assert h == golden['cone_preview_h']
+    assert rel_l2(vol, golden['cone_preview']) < TOL
+    shifts = golden['cone_cor_shifts']
+    best, scores = rec.estimate_cor(shifts, n=32, verbose=False)
+    np.testing.assert_allclose(scores, golden['cone_cor_scores'], rtol=2e-3, atol=2e-3)
+    assert best == golden['cone_cor_best']
+    rec.set_cor_shift(0.0)
+
+
+def test_reference_stream_order_cone(golden):
+    from minirecast import recon
+    from minirecast.geometry import ConeVecGeometry
+    from minirecast.stream import Feeder
+    R, C = (int(x) for x in golden['cone_shape'])
+    g = ConeVecGeometry(golden['cone_vectors'], R, C)
+    live = recon.SliceReconstructor(None, g, verbose=False)
+    Feeder(live, golden['cone_proj'], order="interleaved", group=7, rate=0, verbose=False).run()
+    assert live.n == len(golden['cone_vectors'])
+    img, _ = live.reconstruct_plane((0, 0, 0), (0.2, 0.1, 1.0), n=32)
+    assert rel_l2(img, golden['cone_stream_plane']) < TOL
+
+
+def test_reference_planes_parallel(golden):
+    from minirecast import recon
+    from minirecast.geometry import ParallelGeometry
+    R, C = (int(x) for x in golden['par_shape'])
+    pu, pv = golden['par_pixel']
+    g = ParallelGeometry(golden['par_angles'], R, C, pu, pv)
+    rec = recon.SliceReconstructor(golden['par_proj'], g, window="hann", verbose=False)
+    img, _ = rec.reconstruct_plane((0, 0, 2.0), (0, 0, 1), n=40)
+    assert rel_l2(img, golden['par_plane_axial']) < TOL
+    img, _ = rec.reconstruct_plane((1.0, 0, 0), (1, 0.5, 0.8), n=36, size=100.0)
+    assert rel_l2(img, golden['par_plane_oblique']) < TOL
+    img, _ = rec.reconstruct_plane((0, 0, 0), (0, 0, 1), n=32, cor_shift=-0.75)
+    assert rel_l2(img, golden['par_plane_cor']) < TOL
+    best, scores = rec.estimate_cor(np.arange(-1.0, 1.01, 0.5), n=32, verbose=False)
+    np.testing.assert_allclose(scores, golden['par_cor_scores'], rtol=2e-3, atol=2e-3)
+    assert best == golden['par_cor_best']
+
+
+# ---------------------------------------------------------------- float64 oracle, seeded workload
+@pytest.fixture(scope="module")
+def small_cone():
+    """The 96 x 96, 180-view workload: GPU projections, GPU reconstructor and the
+    float64 oracle on the same float32 projections."""
+    from minirecast import phantom, recon, workloads
+    w = workloads.small_cone()
+    proj = phantom.project_cone(w.phantom, w.geom)
+    rec = recon.SliceReconstructor(proj, w.geom, window=w.window, verbose=False)
+    orc = O.OracleReconstructor("cone", w.geom.rows, w.geom.cols, w.geom.volume_box(),
+                                window=w.window, vectors=w.geom.vectors)
+    orc.add_projections(np.arange(w.geom.n_angles), proj)
+    return w, proj, rec, orc
+
+
+def test_gpu_phantom_matches_oracle_line_integrals(small_cone):
+    w, proj, _, _ = small_cone
+    ph = w.phantom
+    ks = [0, 45, 179]
+    want = O.project_cone(ph.centers, ph.mats, ph.vals, w.geom.vectors[ks], w.geom.rows,
+                          w.geom.cols)
+    np.testing.assert_allclose(proj[:, ks, :], want, rtol=1e-6, atol=1e-6 * want.max())
+
+
+def test_small_cone_slices_vs_float64_oracle(small_cone):
+    w, _, rec, orc = small_cone
+    for p in w.planes:
+        img, _ = rec.reconstruct_plane(p['origin'], p['normal'], n=p['n'])
+        ref, _ = orc.reconstruct_plane(p['origin'], p['normal'], n=p['n'])
+        assert rel_l2(img, ref) < TOL, (p['normal'], rel_l2(img, ref))
+
+
+def test_points_vs_oracle_and_volume_equals_points(small_cone):
+    w, _, rec, orc = small_cone
+    rng = np.random.default_rng(11)
+    pts = rng.uniform(-1, 1, size=(1000, 3)) * w.geom.volume_box()
+    got = rec.reconstruct_points(pts)
+    assert rel_l2(got, orc.reconstruct_points(pts)) < TOL
+    assert rec.reconstruct_points(np.zeros((0, 3))).shape == (0,)
+    vol, origin, h = rec.reconstruct_volume(voxel=0.7)
+    assert vol.shape == tuple(np.maximum(np.ceil(2 * w.geom.volume_box() / 0.7).astype(int), 2))
+    for frac in ([0.3, 0.6, 0.5], [0.0, 0.0, 0.0], [0.99, 0.2, 0.7]):
+        i, j, k = (np.array(vol.shape) * frac).astype(int)
+        p = origin + h * np.array([i, j, k])
+        assert abs(vol[i, j, k] - rec.reconstruct_points(p[None])[0]) < 1e-5
+    ref, _, _ = orc.reconstruct_volume(voxel=0.7)
+    assert rel_l2(vol, ref) < TOL
+
+
+def test_stream_equals_batch(small_cone):
+    from minirecast import recon
+    from minirecast.stream import Feeder
+    w, proj, rec, _ = small_cone
+    live = recon.SliceReconstructor(None, w.geom, window=w.window, verbose=False)
+    Feeder(live, proj, order="interleaved", group=17, rate=0, verbose=False).run()
+    a, _ = rec.reconstruct_plane((0, 0, 3.0), (0.2, 0.1, 1.0), n=64)
+    b, _ = live.reconstruct_plane((0, 0, 3.0), (0.2, 0.1, 1.0), n=64)
+    assert live.n == w.geom.n_angles
+    assert np.max(np.abs(a - b)) < 1e-4 * max(1.0, np.abs(a).max())
+    # re-sent views are ignored
+    v0 = live.version
+    assert live.add_projections([0, 1], proj[:, :2, :]) == live.n
+    assert live.version == v0
+
+
+def test_batched_planes_equal_single_planes(small_cone):
+    w, _, rec, _ = small_cone
+    imgs = rec.reconstruct_planes(w.planes + [dict(origin=(1, 2, 3), normal=(1, 0.5, 0.8), n=47,
+                                                   zoom=2.0, center=(1, 0, 0))])
+    for p, img in zip(w.planes, imgs):
+        one, _ = rec.reconstruct_plane(p['origin'], p['normal'], n=p['n'])
+        np.testing.assert_array_equal(img, one)
+
+
+# ---------------------------------------------------------------- edge cases
+def test_edge_cases_and_errors():
+    from minirecast import recon
+    from minirecast.geometry import circular_cone_vectors
+    g = circular_cone_vectors(12, 300.0, 600.0, 9, 13, 1.0)
+    rec = recon.SliceReconstructor(None, g, verbose=False)
+    img, _ = rec.reconstruct_plane((0, 0, 0), (0, 0, 1), n=8)          # no views yet
+    assert not img.any()
+    assert not rec.reconstruct_points(np.ones((3, 3))).any()
+    vol, _, _ = rec.preview_volume(6)
+    assert not vol.any()
+    with pytest.raises(ValueError):
+        rec.add_projections([0, 1], np.zeros((9, 3, 13), np.float32))
+    rng = np.random.default_rng(0)
+    proj = rng.normal(size=(9, 12, 13)).astype(np.float32)
+    rec.add_projections([3, 3, 5], proj[:, [3, 3, 5], :])               # duplicate within a batch
+    assert rec.n == 3
+    # plane far outside the box: all zeros, no device fault
+    img, _ = rec.reconstruct_plane((0, 0, 1e4), (0, 0, 1), n=16)
+    assert not img.any()
+    # partial tiles
+    orc = O.OracleReconstructor("cone", 9, 13, g.volume_box(), vectors=g.vectors, store_f32=True)
+    orc.add_projections([3, 3, 5], proj[:, [3, 3, 5], :])
+    for n in (1, 31, 33, 70):
+        a, _ = rec.reconstruct_plane((0.1, 0.2, 0.3), (0.3, -1, 0.4), n=n)
+        b, _ = orc.reconstruct_plane((0.1, 0.2, 0.3), (0.3, -1, 0.4), n=n)
+        assert rel_l2(a, b) < TOL if np.any(b) else not np.any(a)
+
+
+def test_module_level_bp_functions_vs_oracle():
+    from minirecast import recon
+    rng = np.random.default_rng(4)
+    V = O.circular_cone_vectors(6, 50.0, 90.0, 0.8, det_shift_u=0.5)
+    K = O.cone_constants(V)
+    q = rng.normal(size=(6, 11, 17)).astype(np.float32)
+    pts = rng.uniform(-5, 5, size=(200, 3))
+    D = K['d'] - 0.4 * K['u']
+    args = (K['s'], D, K['uu'], K['vv'], K['n'], K['sdd'], K['sod'], pts, 0.37)
+    assert rel_l2(recon.bp_cone_numba(q, *args), O.bp_cone(q.astype(np.float64), *args)) < TOL
+    th = rng.uniform(0, np.pi, 6)
+    pargs = (np.cos(th), np.sin(th), 0.9, 1.1, pts, 0.25, -0.6)
+    assert rel_l2(recon.bp_parallel_numba(q, *pargs), O.bp_parallel(q.astype(np.float64), *pargs)) < TOL
+
+
+# ---------------------------------------------------------------- COR search known answer
+def _shepp_logan():
+    from minirecast.phantom import EllipsoidPhantom
+    return EllipsoidPhantom(scale=50.0)
+
+
+def test_cor_search_finds_true_shift():
+    """selftest.py:78-101: data with the axis 3 px off is fixed by estimate_cor."""
+    from minirecast import phantom, recon
+    from minirecast.geometry import ParallelGeometry, circular_cone_vectors
+    ph = _shepp_logan()
+    for kind in ("parallel", "cone"):
+        if kind == "parallel":
+            g = ParallelGeometry(np.arange(180) * np.pi / 180, 32, 160, 0.9, 0.9)
+            proj = np.roll(phantom.project_parallel(ph, g), 3, axis=2)
+        else:
+            moved = circular_cone_vectors(180, 300.0, 600.0, 32, 160, 1.5, det_shift_u=-3.0)
+            proj = phantom.project_cone(ph, moved)
+            g = circular_cone_vectors(180, 300.0, 600.0, 32, 160, 1.5)
+        rec = recon.SliceReconstructor(proj, g, verbose=False)
+        img0, pts = rec.reconstruct_plane((0, 0, 0), (0, 0, 1), n=96, size=110)
+        best, scores = rec.estimate_cor(np.arange(-5.0, 5.01, 0.5), n=96, verbose=False)
+        img1, _ = rec.reconstruct_plane((0, 0, 0), (0, 0, 1), n=96, size=110)
+        truth = ph.evaluate(pts)
+        m = np.linalg.norm(pts, axis=-1) < 45.0
+        e0 = np.median(np.abs(img0[m] - truth[m]))
+        e1 = np.median(np.abs(img1[m] - truth[m]))
+        assert abs(best - 3.0) <= 0.5 and e1 < e0, (kind, best, e0, e1)
+
+
+# ---------------------------------------------------------------- storage forms
+def test_storage_precision_vs_float64_oracle(small_cone):
+    from minirecast import recon
+    w, proj, _, orc = small_cone
+    p = w.planes[0]
+    ref, _ = orc.reconstruct_plane(p['origin'], (1.0, 0.5, 0.8), n=p['n'])
+    errs = {}
+    for storage, tol in (("f32", TOL), ("f16", 1e-3), ("bf16", 5e-3)):
+        rec = recon.SliceReconstructor(proj, w.geom, window=w.window, verbose=False,
+                                       storage=storage)
+        img, _ = rec.reconstruct_plane(p['origin'], (1.0, 0.5, 0.8), n=p['n'])
+        errs[storage] = rel_l2(img, ref)
+        assert errs[storage] < tol, (storage, errs[storage])
+        q = rec.filtered_views(0, 4)
+        qref = orc.q[:4]
+        assert rel_l2(q, qref) < {"f32": TOL, "f16": 1e-3, "bf16": 5e-3}[storage]
+    assert errs["f32"] < errs["f16"] < errs["bf16"]

@@ -1,0 +1,761 @@
+// C ABI of libminirecast_hip.so: contexts, float64 host-side geometry preparation,
+// ingest into the filtered-projection ring, and back-projection requests.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+
+#include "mr_internal.h"
+
+namespace mr {
+
+namespace {
+thread_local std::string g_err;
+constexpr int kAbiVersion = 1;
+
+int fft_length(int n) {
+    // 2 ** ceil(log2(2 n)) (recon.py:37): the smallest power of two >= 2n
+    int N = 1;
+    while (N < 2 * n) N <<= 1;
+    return N;
+}
+
+// In-place iterative radix-2 complex FFT in float64 (forward, e^{-2 pi i k n / N}).
+void fft64(std::vector<std::complex<double>>& a) {
+    const size_t N = a.size();
+    for (size_t i = 1, j = 0; i < N; ++i) {
+        size_t bit = N >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(a[i], a[j]);
+    }
+    for (size_t len = 2; len <= N; len <<= 1) {
+        const size_t half = len / 2;
+        for (size_t k = 0; k < half; ++k) {
+            const double ang = -2.0 * M_PI * static_cast<double>(k) / static_cast<double>(len);
+            const std::complex<double> w(std::cos(ang), std::sin(ang));
+            for (size_t i = 0; i < N; i += len) {
+                const std::complex<double> u = a[i + k], v = a[i + k + half] * w;
+                a[i + k] = u + v;
+                a[i + k + half] = u - v;
+            }
+        }
+    }
+}
+
+double window_at(int window, double f) {
+    switch (window) {
+    case MR_WINDOW_RAMLAK: return 1.0;
+    case MR_WINDOW_SHEPP_LOGAN: {
+        const double y = M_PI * (f == 0.0 ? 1.0e-20 : f);
+        return std::sin(y) / y;
+    }
+    case MR_WINDOW_HANN: return 0.5 * (1.0 + std::cos(2.0 * M_PI * f));
+    case MR_WINDOW_COSINE: return std::cos(M_PI * f);
+    default: return 0.0;
+    }
+}
+
+bool valid_window(int w) { return w >= MR_WINDOW_RAMLAK && w <= MR_WINDOW_COSINE; }
+bool valid_storage(int s) { return s >= MR_STORE_F32 && s <= MR_STORE_BF16; }
+
+struct V3 {
+    double x, y, z;
+};
+V3 v3(const double* p) { return {p[0], p[1], p[2]}; }
+V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+V3 operator*(double s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+double norm(V3 a) { return std::sqrt(dot(a, a)); }
+
+void set_row(double* P, V3 a, double a3) {
+    P[0] = a.x; P[1] = a.y; P[2] = a.z; P[3] = a3;
+}
+
+size_t elem_size(int storage) { return storage == MR_STORE_F32 ? 4 : 2; }
+
+void free_geometry(Ctx* c) {
+    if (c->ring) hipFree(c->ring);
+    if (c->d_geom) hipFree(c->d_geom);
+    if (c->d_vscale) hipFree(c->d_vscale);
+    if (c->d_H) hipFree(c->d_H);
+    if (c->d_tw) hipFree(c->d_tw);
+    c->ring = nullptr; c->d_geom = nullptr; c->d_vscale = nullptr; c->d_H = nullptr; c->d_tw = nullptr;
+    c->kind = MR_KIND_NONE;
+    c->n = 0;
+}
+
+// float32 device tables of the ramp response (tau = 1) and FFT twiddles for length N
+int make_tables(int N, int window, float** d_H, float2** d_tw) {
+    std::vector<double> H;
+    int rc = ramp_response_host(N / 2, 1.0, window, H);
+    if (rc) return rc;
+    std::vector<float> Hf(N);
+    std::vector<float2> tw(N);
+    for (int k = 0; k < N; ++k) {
+        Hf[k] = static_cast<float>(H[k]);
+        const double ang = -2.0 * M_PI * static_cast<double>(k) / static_cast<double>(N);
+        tw[k] = make_float2(static_cast<float>(std::cos(ang)), static_cast<float>(std::sin(ang)));
+    }
+    MR_HIP(hipMalloc(reinterpret_cast<void**>(d_H), sizeof(float) * N));
+    MR_HIP(hipMalloc(reinterpret_cast<void**>(d_tw), sizeof(float2) * N));
+    MR_HIP(hipMemcpy(*d_H, Hf.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+    MR_HIP(hipMemcpy(*d_tw, tw.data(), sizeof(float2) * N, hipMemcpyHostToDevice));
+    return MR_OK;
+}
+
+int alloc_ring(Ctx* c, int n_total, int rows, int cols, int window, int storage) {
+    c->n_total = n_total;
+    c->R = rows;
+    c->C = cols;
+    c->window = window;
+    c->storage = storage;
+    c->N = fft_length(cols);
+    c->pitch = (cols + 3 + 63) / 64 * 64;
+    c->Rp = rows + 3;
+    c->view_stride = static_cast<int64_t>(c->Rp) * c->pitch;
+    const size_t bytes = static_cast<size_t>(n_total) * c->view_stride * elem_size(storage);
+    MR_HIP(hipMalloc(&c->ring, bytes));
+    MR_HIP(hipMemsetAsync(c->ring, 0, bytes, c->s_ingest));
+    MR_HIP(hipMalloc(reinterpret_cast<void**>(&c->d_geom), sizeof(ViewGeom) * n_total));
+    MR_HIP(hipMalloc(reinterpret_cast<void**>(&c->d_vscale), sizeof(float) * n_total));
+    std::vector<float> ones(n_total, 1.0f);
+    MR_HIP(hipMemcpy(c->d_vscale, ones.data(), sizeof(float) * n_total, hipMemcpyHostToDevice));
+    int rc = make_tables(c->N, window, &c->d_H, &c->d_tw);
+    if (rc) return rc;
+    MR_HIP(hipStreamSynchronize(c->s_ingest));
+    c->received.assign(n_total, 0);
+    c->n = 0;
+    return MR_OK;
+}
+
+// conservative test: can any point of the tile lie inside the box?
+bool tile_may_hit_box(const mr_task& T, int q, int p0, int p1, int s0, int s1) {
+    if (!T.mask_box) return true;
+    for (int d = 0; d < 3; ++d) {
+        const double lim = T.box[d] * (1.0 + 1e-9) + 1e-9;
+        bool all_hi = true, all_lo = true;
+        for (int cp = 0; cp < 2; ++cp)
+            for (int cs = 0; cs < 2; ++cs) {
+                const int p = cp ? p1 : p0, s = cs ? s1 : s0;
+                const double x = T.o[d] + q * T.c[d] + p * T.a[d] + s * T.b[d];
+                all_hi = all_hi && (x > lim);
+                all_lo = all_lo && (x < -lim);
+            }
+        if (all_hi || all_lo) return false;
+    }
+    return true;
+}
+
+int validate_tasks(const mr_task* tasks, int ntasks, int64_t out_len) {
+    for (int i = 0; i < ntasks; ++i) {
+        const mr_task& T = tasks[i];
+        if (T.nq < 0 || T.np < 0 || T.ns < 0)
+            return fail(MR_EVALUE, "task " + std::to_string(i) + ": negative grid size");
+        if (T.nq == 0 || T.np == 0 || T.ns == 0) continue;
+        if (T.out_offset < 0 || T.stride_q < 0 || T.stride_p < 0 || T.stride_s < 0)
+            return fail(MR_EVALUE, "task " + std::to_string(i) + ": negative offset or stride");
+        const int64_t last = T.out_offset + (T.nq - 1) * T.stride_q + (T.np - 1) * T.stride_p +
+                             (T.ns - 1) * T.stride_s;
+        if (last >= out_len)
+            return fail(MR_EVALUE, "task " + std::to_string(i) + " writes beyond the output (" +
+                                       std::to_string(last) + " >= " + std::to_string(out_len) + ")");
+    }
+    return MR_OK;
+}
+
+void build_tiles(const mr_task* tasks, int ntasks, std::vector<Tile>& tiles) {
+    tiles.clear();
+    for (int i = 0; i < ntasks; ++i) {
+        const mr_task& T = tasks[i];
+        if (T.nq <= 0 || T.np <= 0 || T.ns <= 0) continue;
+        const int ntp = (T.np + BP_TP - 1) / BP_TP, nts = (T.ns + BP_TS - 1) / BP_TS;
+        for (int q = 0; q < T.nq; ++q)
+            for (int tp = 0; tp < ntp; ++tp)
+                for (int ts = 0; ts < nts; ++ts) {
+                    const int p0 = tp * BP_TP, s0 = ts * BP_TS;
+                    const int p1 = std::min(p0 + BP_TP, T.np) - 1;
+                    const int s1 = std::min(s0 + BP_TS, T.ns) - 1;
+                    if (tile_may_hit_box(T, q, p0, p1, s0, s1)) tiles.push_back({i, q, tp, ts});
+                }
+    }
+}
+
+// Uploads tasks + tiles and launches the tile kernel into d_out (device), timing it.
+int run_tasks(Ctx* c, const mr_task* tasks, int ntasks, int n_views, float* d_out) {
+    std::vector<Tile> tiles;
+    build_tiles(tasks, ntasks, tiles);
+    const size_t tb = sizeof(mr_task) * ntasks;
+    const size_t tl = sizeof(Tile) * tiles.size();
+    const size_t off_tiles = (tb + 255) / 256 * 256;
+    const size_t need = off_tiles + tl + 256;
+    int rc = ensure(&c->d_req, &c->req_bytes, need);
+    if (rc) return rc;
+    rc = ensure_host(&c->h_req, &c->h_req_bytes, need);
+    if (rc) return rc;
+    std::memcpy(c->h_req, tasks, tb);
+    if (tl) std::memcpy(static_cast<char*>(c->h_req) + off_tiles, tiles.data(), tl);
+    MR_HIP(hipMemcpyAsync(c->d_req, c->h_req, off_tiles + tl, hipMemcpyHostToDevice, c->s_req));
+    const mr_task* d_tasks = static_cast<const mr_task*>(c->d_req);
+    const Tile* d_tiles = reinterpret_cast<const Tile*>(static_cast<char*>(c->d_req) + off_tiles);
+    const float scale = n_views > 0 ? static_cast<float>(M_PI / n_views) : 0.0f;
+    MR_HIP(hipEventRecord(c->ev0, c->s_req));
+    rc = launch_bp_tiles(c, d_tasks, d_tiles, static_cast<int>(tiles.size()), n_views, scale, d_out,
+                         c->s_req);
+    if (rc) return rc;
+    MR_HIP(hipEventRecord(c->ev1, c->s_req));
+    return MR_OK;
+}
+
+int finish_timing(Ctx* c) {
+    MR_HIP(hipStreamSynchronize(c->s_req));
+    float ms = 0.0f;
+    MR_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_kernel_ms = ms;
+    return MR_OK;
+}
+
+int check_ctx(mr_ctx* ctx) {
+    if (!ctx) return fail(MR_EVALUE, "null context");
+    return MR_OK;
+}
+
+}  // namespace
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+int ensure(void** p, size_t* have, size_t need) {
+    if (*p && *have >= need) return MR_OK;
+    if (*p) hipFree(*p);
+    *p = nullptr;
+    *have = 0;
+    const size_t bytes = std::max<size_t>((need + (1u << 20) - 1) & ~static_cast<size_t>((1u << 20) - 1),
+                                          1u << 20);
+    MR_HIP(hipMalloc(p, bytes));
+    *have = bytes;
+    return MR_OK;
+}
+
+int ensure_host(void** p, size_t* have, size_t need) {
+    if (*p && *have >= need) return MR_OK;
+    if (*p) hipHostFree(*p);
+    *p = nullptr;
+    *have = 0;
+    const size_t bytes = std::max<size_t>((need + (1u << 16) - 1) & ~static_cast<size_t>((1u << 16) - 1),
+                                          1u << 16);
+    MR_HIP(hipHostMalloc(p, bytes, hipHostMallocDefault));
+    *have = bytes;
+    return MR_OK;
+}
+
+int ramp_response_host(int n, double tau, int window, std::vector<double>& H) {
+    if (n < 1) return fail(MR_EVALUE, "ramp_response: n must be >= 1");
+    if (!valid_window(window)) return fail(MR_EVALUE, "unknown filter window " + std::to_string(window));
+    if (!(tau > 0.0) || !std::isfinite(tau)) return fail(MR_EVALUE, "ramp_response: tau must be > 0");
+    const int N = fft_length(n);
+    std::vector<std::complex<double>> h(N, 0.0);
+    h[0] = 1.0 / (4.0 * tau * tau);
+    for (int i = 1; i < N; ++i) {
+        const int idx = i < N / 2 ? i : i - N;
+        if (idx % 2 != 0) {
+            const double t = M_PI * idx * tau;
+            h[i] = -1.0 / (t * t);
+        }
+    }
+    fft64(h);
+    H.resize(N);
+    const double invN = 1.0 / N;
+    for (int k = 0; k < N; ++k) {
+        const int kk = k < N / 2 ? k : k - N;       // numpy.fft.fftfreq ordering
+        const double f = kk * invN;
+        H[k] = h[k].real() * tau * window_at(window, f);
+    }
+    return MR_OK;
+}
+
+}  // namespace mr
+
+using namespace mr;
+
+struct mr_ctx : Ctx {};
+
+extern "C" {
+
+const char* mr_last_error(void) { return g_err.c_str(); }
+
+int mr_abi_version(void) { return kAbiVersion; }
+
+int mr_device_count(int* n) {
+    if (!n) return fail(MR_EVALUE, "null output");
+    int k = 0;
+    if (hipGetDeviceCount(&k) != hipSuccess) k = 0;
+    *n = k;
+    return MR_OK;
+}
+
+int mr_ramp_response(int n, double tau, int window, double* H, int* N) {
+    std::vector<double> h;
+    int rc = ramp_response_host(n, tau, window, h);
+    if (rc) return rc;
+    if (N) *N = static_cast<int>(h.size());
+    if (H) std::memcpy(H, h.data(), sizeof(double) * h.size());
+    return MR_OK;
+}
+
+int mr_ctx_create(int device, mr_ctx** out) {
+    if (!out) return fail(MR_EVALUE, "null output");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        return fail(MR_ERUNTIME, "no HIP device available");
+    if (device < 0 || device >= count)
+        return fail(MR_EVALUE, "device " + std::to_string(device) + " out of range");
+    MR_HIP(hipSetDevice(device));
+    mr_ctx* c = new mr_ctx();
+    c->device = device;
+    // default-priority, non-blocking streams
+    if (hipStreamCreateWithFlags(&c->s_ingest, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->s_req, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return fail(MR_ERUNTIME, "stream/event creation failed");
+    }
+    *out = c;
+    return MR_OK;
+}
+
+int mr_ctx_destroy(mr_ctx* c) {
+    if (!c) return MR_OK;
+    hipSetDevice(c->device);
+    hipDeviceSynchronize();
+    free_geometry(c);
+    void* bufs[] = {c->d_stage, c->d_fstage, c->d_vmax, c->d_fv, c->d_req, c->d_out, c->d_pts, c->d_misc};
+    for (void* b : bufs)
+        if (b) hipFree(b);
+    if (c->h_req) hipHostFree(c->h_req);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->s_ingest) hipStreamDestroy(c->s_ingest);
+    if (c->s_req) hipStreamDestroy(c->s_req);
+    delete c;
+    return MR_OK;
+}
+
+int mr_set_geometry_cone(mr_ctx* c, int n_total, int rows, int cols, const double* vectors,
+                         int window, int storage) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    MR_CHECK(n_total >= 1 && rows >= 1 && cols >= 1, MR_EVALUE, "empty geometry");
+    MR_CHECK(vectors != nullptr, MR_EVALUE, "null vectors");
+    MR_CHECK(valid_window(window), MR_EVALUE, "unknown filter window " + std::to_string(window));
+    MR_CHECK(valid_storage(storage), MR_EVALUE, "unknown storage " + std::to_string(storage));
+    MR_HIP(hipSetDevice(c->device));
+    MR_HIP(hipDeviceSynchronize());
+    free_geometry(c);
+    const int N = fft_length(cols);
+    c->geom_all.assign(n_total, ViewGeom{});
+    c->filt_all.assign(n_total, FilterView{});
+    const double cc = 0.5 * (cols - 1), cr = 0.5 * (rows - 1);
+    for (int k = 0; k < n_total; ++k) {
+        // cone_constants (recon.py:70-82)
+        const double* V = vectors + 12 * k;
+        const V3 s = v3(V), d = v3(V + 3), u = v3(V + 6), v = v3(V + 9);
+        V3 n = cross(u, v);
+        n = (1.0 / norm(n)) * n;
+        const double sg = dot(d - s, n);
+        n = (sg > 0 ? 1.0 : (sg < 0 ? -1.0 : 0.0)) * n;
+        const double sdd = dot(d - s, n);
+        const double sod = dot(-1.0 * s, n);
+        const V3 uu = (1.0 / dot(u, u)) * u;
+        const V3 vv = (1.0 / dot(v, v)) * v;
+        MR_CHECK(sod > 0 && sdd > 0, MR_EVALUE,
+                 "view " + std::to_string(k) + ": source and detector must lie on opposite sides of the origin");
+        // projective form (see ViewGeom): col - cc = (((S-D).UU) L + SDD (UU.rel)) / L
+        ViewGeom g{};
+        const V3 au = dot(s - d, uu) * n + sdd * uu;
+        const V3 av = dot(s - d, vv) * n + sdd * vv;
+        set_row(g.P[0], (1.0 / sod) * au, -dot(au, s) / sod);
+        set_row(g.P[1], (1.0 / sod) * av, -dot(av, s) / sod);
+        set_row(g.P[2], (1.0 / sod) * n, -dot(n, s) / sod);
+        g.uvv = dot(u, vv);
+        c->geom_all[k] = g;
+        // FDK weight (recon.py:93-102)
+        const V3 p0 = s + sdd * n;
+        const V3 off = d - p0;
+        const double un = norm(u), vn = norm(v);
+        FilterView f{};
+        f.a0 = static_cast<float>(dot(off, u) / un - cc * un);
+        f.da = static_cast<float>(un);
+        f.b0 = static_cast<float>(dot(off, v) / vn - cr * vn);
+        f.db = static_cast<float>(vn);
+        f.sdd = static_cast<float>(sdd);
+        f.sdd2 = static_cast<float>(sdd * sdd);
+        const double tau_iso = un * sod / sdd;
+        f.gain = static_cast<float>(1.0 / (N * tau_iso));
+        f.cone = 1;
+        c->filt_all[k] = f;
+    }
+    rc = alloc_ring(c, n_total, rows, cols, window, storage);
+    if (rc) {
+        free_geometry(c);
+        return rc;
+    }
+    c->kind = MR_KIND_CONE;
+    return MR_OK;
+}
+
+int mr_set_geometry_parallel(mr_ctx* c, int n_total, int rows, int cols, const double* angles,
+                             double pixel_u, double pixel_v, int window, int storage) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    MR_CHECK(n_total >= 1 && rows >= 1 && cols >= 1, MR_EVALUE, "empty geometry");
+    MR_CHECK(angles != nullptr, MR_EVALUE, "null angles");
+    MR_CHECK(pixel_u > 0 && pixel_v > 0, MR_EVALUE, "pixel sizes must be > 0");
+    MR_CHECK(valid_window(window), MR_EVALUE, "unknown filter window " + std::to_string(window));
+    MR_CHECK(valid_storage(storage), MR_EVALUE, "unknown storage " + std::to_string(storage));
+    MR_HIP(hipSetDevice(c->device));
+    MR_HIP(hipDeviceSynchronize());
+    free_geometry(c);
+    const int N = fft_length(cols);
+    c->geom_all.assign(n_total, ViewGeom{});
+    c->filt_all.assign(n_total, FilterView{});
+    for (int k = 0; k < n_total; ++k) {
+        // col - cc = (-x sin t + y cos t) / pu ; row - cr = z / pv  (recon.py:185-188)
+        const double ct = std::cos(angles[k]), st = std::sin(angles[k]);
+        ViewGeom g{};
+        set_row(g.P[0], {-st / pixel_u, ct / pixel_u, 0.0}, 0.0);
+        set_row(g.P[1], {0.0, 0.0, 1.0 / pixel_v}, 0.0);
+        set_row(g.P[2], {0.0, 0.0, 0.0}, 1.0);
+        g.uvv = 0.0;
+        c->geom_all[k] = g;
+        FilterView f{};
+        f.gain = static_cast<float>(1.0 / (N * pixel_u));   // filter_rows(block, pixel_u) (recon.py:359)
+        f.cone = 0;
+        c->filt_all[k] = f;
+    }
+    c->pixel_u = pixel_u;
+    c->pixel_v = pixel_v;
+    rc = alloc_ring(c, n_total, rows, cols, window, storage);
+    if (rc) {
+        free_geometry(c);
+        return rc;
+    }
+    c->kind = MR_KIND_PARALLEL;
+    return MR_OK;
+}
+
+static int append_views(mr_ctx* c, const int64_t* indices, int k, const float* block,
+                        int on_device, int* n_out, bool filter) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    MR_CHECK(c->kind != MR_KIND_NONE, MR_ERUNTIME, "no geometry set");
+    MR_CHECK(k >= 0, MR_EVALUE, "negative view count");
+    if (k == 0) {
+        if (n_out) *n_out = c->n;
+        return MR_OK;
+    }
+    MR_CHECK(indices && block, MR_EVALUE, "null indices or block");
+    MR_CHECK(c->n + k <= c->n_total, MR_EVALUE,
+             "ring full: " + std::to_string(c->n) + " + " + std::to_string(k) + " views > " +
+                 std::to_string(c->n_total));
+    for (int j = 0; j < k; ++j)
+        MR_CHECK(indices[j] >= 0 && indices[j] < c->n_total, MR_EVALUE,
+                 "view index " + std::to_string(indices[j]) + " out of range");
+    MR_HIP(hipSetDevice(c->device));
+    const size_t blk = sizeof(float) * static_cast<size_t>(c->R) * k * c->C;
+    const float* d_block = block;
+    if (!on_device) {
+        rc = ensure(reinterpret_cast<void**>(&c->d_stage), &c->stage_bytes, blk);
+        if (rc) return rc;
+        MR_HIP(hipMemcpyAsync(c->d_stage, block, blk, hipMemcpyHostToDevice, c->s_ingest));
+        d_block = c->d_stage;
+    }
+    std::vector<FilterView> fv(k);
+    std::vector<ViewGeom> gv(k);
+    for (int j = 0; j < k; ++j) {
+        fv[j] = c->filt_all[indices[j]];
+        fv[j].slot = c->n + j;
+        gv[j] = c->geom_all[indices[j]];
+    }
+    rc = ensure(reinterpret_cast<void**>(&c->d_fv), &c->fv_len, sizeof(FilterView) * k);
+    if (rc) return rc;
+    MR_HIP(hipMemcpyAsync(c->d_fv, fv.data(), sizeof(FilterView) * k, hipMemcpyHostToDevice, c->s_ingest));
+    MR_HIP(hipMemcpyAsync(c->d_geom + c->n, gv.data(), sizeof(ViewGeom) * k, hipMemcpyHostToDevice,
+                          c->s_ingest));
+    if (c->storage == MR_STORE_F16) {
+        if (filter) {
+            rc = ensure(reinterpret_cast<void**>(&c->d_fstage), &c->fstage_bytes, blk);
+            if (rc) return rc;
+        }
+        rc = ensure(reinterpret_cast<void**>(&c->d_vmax), &c->vmax_len, sizeof(unsigned) * k);
+        if (rc) return rc;
+    }
+    rc = filter ? launch_filter(c, d_block, k, c->n, c->s_ingest) : launch_store(c, d_block, k, c->s_ingest);
+    if (rc) return rc;
+    MR_HIP(hipStreamSynchronize(c->s_ingest));
+    c->n += k;
+    if (n_out) *n_out = c->n;
+    return MR_OK;
+}
+
+int mr_push_views(mr_ctx* c, const int64_t* indices, int k, const float* block,
+                  int block_on_device, int* n_out) {
+    return append_views(c, indices, k, block, block_on_device, n_out, true);
+}
+
+int mr_store_views(mr_ctx* c, const int64_t* indices, int k, const float* q, int q_on_device,
+                   int* n_out) {
+    return append_views(c, indices, k, q, q_on_device, n_out, false);
+}
+
+int mr_num_views(mr_ctx* c, int* n) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    MR_CHECK(n != nullptr, MR_EVALUE, "null output");
+    std::lock_guard<std::mutex> lk(c->mu);
+    *n = c->n;
+    return MR_OK;
+}
+
+int mr_bp_points(mr_ctx* c, const double* pts, int64_t M, double cor_shift, int n_views,
+                 double* out) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    MR_CHECK(c->kind != MR_KIND_NONE, MR_ERUNTIME, "no geometry set");
+    MR_CHECK(M >= 0, MR_EVALUE, "negative point count");
+    MR_CHECK(n_views >= 0 && n_views <= c->n, MR_EVALUE,
+             "n_views " + std::to_string(n_views) + " outside [0, " + std::to_string(c->n) + "]");
+    if (M == 0) return MR_OK;
+    MR_CHECK(pts && out, MR_EVALUE, "null points or output");
+    if (n_views == 0) {
+        std::fill(out, out + M, 0.0);
+        return MR_OK;
+    }
+    MR_HIP(hipSetDevice(c->device));
+    rc = ensure(reinterpret_cast<void**>(&c->d_pts), &c->pts_bytes, sizeof(double) * 3 * M);
+    if (rc) return rc;
+    rc = ensure(reinterpret_cast<void**>(&c->d_out), &c->out_bytes, sizeof(float) * M);
+    if (rc) return rc;
+    MR_HIP(hipMemcpyAsync(c->d_pts, pts, sizeof(double) * 3 * M, hipMemcpyHostToDevice, c->s_req));
+    const float scale = static_cast<float>(M_PI / n_views);
+    MR_HIP(hipEventRecord(c->ev0, c->s_req));
+    rc = launch_bp_points(c, c->d_pts, M, cor_shift, n_views, scale, c->d_out, c->s_req);
+    if (rc) return rc;
+    MR_HIP(hipEventRecord(c->ev1, c->s_req));
+    std::vector<float> tmp(M);
+    MR_HIP(hipMemcpyAsync(tmp.data(), c->d_out, sizeof(float) * M, hipMemcpyDeviceToHost, c->s_req));
+    rc = finish_timing(c);
+    if (rc) return rc;
+    for (int64_t i = 0; i < M; ++i) out[i] = tmp[i];
+    return MR_OK;
+}
+
+int mr_bp_tasks(mr_ctx* c, const mr_task* tasks, int ntasks, int n_views, float* out,
+                int64_t out_len, int out_on_device) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    MR_CHECK(c->kind != MR_KIND_NONE, MR_ERUNTIME, "no geometry set");
+    MR_CHECK(ntasks >= 0 && out_len >= 0, MR_EVALUE, "negative task count or output length");
+    MR_CHECK(n_views >= 0 && n_views <= c->n, MR_EVALUE,
+             "n_views " + std::to_string(n_views) + " outside [0, " + std::to_string(c->n) + "]");
+    if (ntasks == 0 || out_len == 0) return MR_OK;
+    MR_CHECK(tasks && out, MR_EVALUE, "null tasks or output");
+    rc = validate_tasks(tasks, ntasks, out_len);
+    if (rc) return rc;
+    MR_HIP(hipSetDevice(c->device));
+    float* d_out = out;
+    if (!out_on_device) {
+        rc = ensure(reinterpret_cast<void**>(&c->d_out), &c->out_bytes, sizeof(float) * out_len);
+        if (rc) return rc;
+        d_out = c->d_out;
+        MR_HIP(hipMemsetAsync(d_out, 0, sizeof(float) * out_len, c->s_req));
+    }
+    rc = run_tasks(c, tasks, ntasks, n_views, d_out);
+    if (rc) return rc;
+    if (!out_on_device)
+        MR_HIP(hipMemcpyAsync(out, d_out, sizeof(float) * out_len, hipMemcpyDeviceToHost, c->s_req));
+    return finish_timing(c);
+}
+
+int mr_cor_scores(mr_ctx* c, const mr_task* tasks, int ntasks, int n_views, double* scores) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    MR_CHECK(c->kind != MR_KIND_NONE, MR_ERUNTIME, "no geometry set");
+    MR_CHECK(n_views >= 0 && n_views <= c->n, MR_EVALUE, "n_views outside the stored range");
+    if (ntasks <= 0) return MR_OK;
+    MR_CHECK(tasks && scores, MR_EVALUE, "null tasks or scores");
+    const int64_t npix = static_cast<int64_t>(tasks[0].np) * tasks[0].ns;
+    std::vector<mr_task> T(tasks, tasks + ntasks);
+    for (int i = 0; i < ntasks; ++i) {
+        MR_CHECK(T[i].nq == 1 && static_cast<int64_t>(T[i].np) * T[i].ns == npix, MR_EVALUE,
+                 "score tasks must be single images of equal size");
+        T[i].out_offset = i * npix;
+        T[i].stride_q = npix;
+        T[i].stride_p = T[i].ns;
+        T[i].stride_s = 1;
+    }
+    MR_HIP(hipSetDevice(c->device));
+    const int64_t total = npix * ntasks;
+    rc = ensure(reinterpret_cast<void**>(&c->d_out), &c->out_bytes, sizeof(float) * total);
+    if (rc) return rc;
+    rc = ensure(reinterpret_cast<void**>(&c->d_misc), &c->misc_bytes, sizeof(double) * ntasks);
+    if (rc) return rc;
+    MR_HIP(hipMemsetAsync(c->d_out, 0, sizeof(float) * total, c->s_req));
+    rc = run_tasks(c, T.data(), ntasks, n_views, c->d_out);
+    if (rc) return rc;
+    rc = launch_cor_scores(c, c->d_out, ntasks, npix, c->d_misc, c->s_req);
+    if (rc) return rc;
+    MR_HIP(hipMemcpyAsync(scores, c->d_misc, sizeof(double) * ntasks, hipMemcpyDeviceToHost, c->s_req));
+    return finish_timing(c);
+}
+
+int mr_filter_rows(mr_ctx* c, const float* in, int rows, int n, double tau, int window, float* out) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    MR_CHECK(rows >= 0 && n >= 1, MR_EVALUE, "bad row shape");
+    MR_CHECK(valid_window(window), MR_EVALUE, "unknown filter window " + std::to_string(window));
+    MR_CHECK(tau > 0, MR_EVALUE, "tau must be > 0");
+    if (rows == 0) return MR_OK;
+    MR_CHECK(in && out, MR_EVALUE, "null input or output");
+    MR_HIP(hipSetDevice(c->device));
+    const int N = fft_length(n);
+    float* d_H = nullptr;
+    float2* d_tw = nullptr;
+    rc = make_tables(N, window, &d_H, &d_tw);
+    if (rc) return rc;
+    const size_t bytes = sizeof(float) * static_cast<size_t>(rows) * n;
+    float *d_in = nullptr, *d_o = nullptr;
+    hipError_t e1 = hipMalloc(reinterpret_cast<void**>(&d_in), bytes);
+    hipError_t e2 = hipMalloc(reinterpret_cast<void**>(&d_o), bytes);
+    if (e1 == hipSuccess && e2 == hipSuccess) {
+        hipMemcpy(d_in, in, bytes, hipMemcpyHostToDevice);
+        rc = launch_filter_rows_plain(c, d_in, rows, n, static_cast<float>(1.0 / (N * tau)), d_H, d_tw,
+                                      N, d_o, c->s_req);
+        if (!rc) {
+            hipError_t e = hipStreamSynchronize(c->s_req);
+            if (e == hipSuccess) e = hipMemcpy(out, d_o, bytes, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) rc = fail(MR_ERUNTIME, std::string("filter_rows: ") + hipGetErrorString(e));
+        }
+    } else {
+        rc = fail(MR_ERUNTIME, "filter_rows: device allocation failed");
+    }
+    if (d_in) hipFree(d_in);
+    if (d_o) hipFree(d_o);
+    hipFree(d_H);
+    hipFree(d_tw);
+    return rc;
+}
+
+int mr_read_views(mr_ctx* c, int first, int count, float* out) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    MR_CHECK(c->kind != MR_KIND_NONE, MR_ERUNTIME, "no geometry set");
+    MR_CHECK(first >= 0 && count >= 0 && first + count <= c->n, MR_EVALUE, "view range outside the stored views");
+    if (count == 0) return MR_OK;
+    MR_CHECK(out != nullptr, MR_EVALUE, "null output");
+    MR_HIP(hipSetDevice(c->device));
+    const size_t bytes = sizeof(float) * static_cast<size_t>(count) * c->R * c->C;
+    rc = ensure(reinterpret_cast<void**>(&c->d_out), &c->out_bytes, bytes);
+    if (rc) return rc;
+    rc = launch_read_views(c, first, count, c->d_out, c->s_req);
+    if (rc) return rc;
+    MR_HIP(hipMemcpyAsync(out, c->d_out, bytes, hipMemcpyDeviceToHost, c->s_req));
+    MR_HIP(hipStreamSynchronize(c->s_req));
+    return MR_OK;
+}
+
+static int phantom_common(mr_ctx* c, const double* ell, int K, int kind, const double* geo, int ngeo,
+                          int A, int rows, int cols, double pu, double pv, double far, float* out) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    MR_CHECK(K >= 0 && A >= 0 && rows >= 0 && cols >= 0, MR_EVALUE, "negative phantom size");
+    if (A == 0 || rows == 0 || cols == 0) return MR_OK;
+    MR_CHECK(out && geo && (K == 0 || ell), MR_EVALUE, "null phantom argument");
+    MR_HIP(hipSetDevice(c->device));
+    double *d_ell = nullptr, *d_geo = nullptr;
+    MR_HIP(hipMalloc(reinterpret_cast<void**>(&d_ell), sizeof(double) * (13 * K + 1)));
+    MR_HIP(hipMalloc(reinterpret_cast<void**>(&d_geo), sizeof(double) * ngeo));
+    if (K) MR_HIP(hipMemcpy(d_ell, ell, sizeof(double) * 13 * K, hipMemcpyHostToDevice));
+    MR_HIP(hipMemcpy(d_geo, geo, sizeof(double) * ngeo, hipMemcpyHostToDevice));
+    rc = launch_phantom(d_ell, K, kind, d_geo, A, rows, cols, pu, pv, far, out, c->s_req);
+    hipError_t e = hipStreamSynchronize(c->s_req);
+    hipFree(d_ell);
+    hipFree(d_geo);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(MR_ERUNTIME, std::string("phantom: ") + hipGetErrorString(e));
+    return MR_OK;
+}
+
+int mr_phantom_cone(mr_ctx* c, const double* ell, int K, const double* vectors, int A, int rows,
+                    int cols, float* out_device) {
+    return phantom_common(c, ell, K, MR_KIND_CONE, vectors, 12 * A, A, rows, cols, 0, 0, 0, out_device);
+}
+
+int mr_phantom_parallel(mr_ctx* c, const double* ell, int K, const double* angles, int A, int rows,
+                        int cols, double pixel_u, double pixel_v, double far, float* out_device) {
+    return phantom_common(c, ell, K, MR_KIND_PARALLEL, angles, A, A, rows, cols, pixel_u, pixel_v, far,
+                          out_device);
+}
+
+int mr_device_alloc(mr_ctx* c, int64_t bytes, void** ptr) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    MR_CHECK(ptr && bytes >= 0, MR_EVALUE, "bad allocation request");
+    MR_HIP(hipSetDevice(c->device));
+    MR_HIP(hipMalloc(ptr, bytes > 0 ? bytes : 1));
+    return MR_OK;
+}
+
+int mr_device_free(mr_ctx* c, void* ptr) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    MR_HIP(hipSetDevice(c->device));
+    if (ptr) MR_HIP(hipFree(ptr));
+    return MR_OK;
+}
+
+int mr_memcpy(mr_ctx* c, void* dst, const void* src, int64_t bytes, int kind) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    MR_CHECK(bytes >= 0 && kind >= 0 && kind <= 2, MR_EVALUE, "bad copy request");
+    if (bytes == 0) return MR_OK;
+    MR_HIP(hipSetDevice(c->device));
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                                      : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+    MR_HIP(hipMemcpy(dst, src, bytes, k));
+    return MR_OK;
+}
+
+int mr_synchronize(mr_ctx* c) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    MR_HIP(hipSetDevice(c->device));
+    MR_HIP(hipDeviceSynchronize());
+    return MR_OK;
+}
+
+int mr_last_kernel_ms(mr_ctx* c, double* ms) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    MR_CHECK(ms != nullptr, MR_EVALUE, "null output");
+    *ms = c->last_kernel_ms;
+    return MR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,144 @@
+// Internal declarations shared by the libminirecast_hip translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/minirecast_hip.h"
+
+namespace mr {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define MR_HIP(call)                                                              \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess)                                                     \
+            return ::mr::fail(MR_ERUNTIME, std::string(#call) + ": " +            \
+                                               hipGetErrorString(e_));            \
+    } while (0)
+
+#define MR_CHECK(cond, code, msg)                                                 \
+    do {                                                                          \
+        if (!(cond)) return ::mr::fail(code, msg);                                \
+    } while (0)
+
+// ---------------------------------------------------------------- geometry
+// Projective form of one stored view (float64).  For a point x (mm):
+//   uh = P[0].[x,1], vh = P[1].[x,1], wh = P[2].[x,1]
+//   col = uh / wh + (C-1)/2,  row = vh / wh + (R-1)/2
+// Cone beam: rows are divided by SOD so the FDK distance weight (SOD/L)^2 is 1/wh^2.
+// Parallel beam: wh == 1 and no distance weight.
+// A rotation-centre shift s adds s*wh to uh and s*uvv*wh to vh (the cone detector
+// moved to D - s*u, recon.py:420; the parallel column shift, recon.py:181).
+struct ViewGeom {
+    double P[3][4];
+    double uvv;
+    double pad[3];
+};
+static_assert(sizeof(ViewGeom) == 128, "ViewGeom layout");
+
+// FDK cosine weight / pitch of one arriving view (recon.py:93-102) in float32:
+//   w(r, c) = sdd / sqrt(sdd^2 + (a0 + c*da)^2 + (b0 + r*db)^2)
+// `gain` = 1 / (N * tau_iso): the inverse-FFT normalisation and the ramp
+// response's 1/tau dependence (H(tau) = H(1) / tau).
+struct FilterView {
+    float a0, da, b0, db;
+    float sdd, sdd2, gain;
+    int32_t slot;
+    int32_t cone;
+    int32_t pad[3];
+};
+static_assert(sizeof(FilterView) == 48, "FilterView layout");
+
+// Tile work item of a back-projection launch.
+struct Tile {
+    int32_t task, q, tp, ts;
+};
+
+// BP tile shape: TP x TS points per 256-thread workgroup, 4 points per thread.
+constexpr int BP_TS = 32;
+constexpr int BP_TP = 32;
+constexpr int BP_THREADS = 256;
+constexpr int BP_VPASS = 64;     // views per LDS record pass
+constexpr int BP_REC = 16;       // floats per (tile, view) record
+
+// ---------------------------------------------------------------- context
+struct Ctx {
+    int device = 0;
+    std::mutex mu;
+    hipStream_t s_ingest = nullptr, s_req = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_kernel_ms = 0.0;
+
+    int kind = MR_KIND_NONE;
+    int n_total = 0, R = 0, C = 0;
+    int window = MR_WINDOW_SHEPP_LOGAN, storage = MR_STORE_F32;
+    int N = 0;                       // FFT length
+    int pitch = 0, Rp = 0;           // padded row pitch (elements) and padded rows
+    int64_t view_stride = 0;         // elements per stored view
+    int n = 0;                       // views stored
+
+    // host copies
+    std::vector<ViewGeom> geom_all;   // per geometry index
+    std::vector<FilterView> filt_all; // per geometry index (slot filled at push)
+    std::vector<uint8_t> received;
+    double pixel_u = 1.0, pixel_v = 1.0;
+
+    // device buffers
+    void* ring = nullptr;            // [n_total][Rp][pitch] storage elements
+    ViewGeom* d_geom = nullptr;      // [n_total] per slot
+    float* d_vscale = nullptr;       // [n_total] per slot
+    float* d_H = nullptr;            // [N] ramp response for tau = 1 (windowed)
+    float2* d_tw = nullptr;          // [N] twiddles exp(-2 pi i k / N)
+    float* d_stage = nullptr;        // raw block staging
+    size_t stage_bytes = 0;
+    float* d_fstage = nullptr;       // f32 filtered staging (f16 storage)
+    size_t fstage_bytes = 0;
+    unsigned* d_vmax = nullptr;      // per-batch per-view max |q| (float bits)
+    size_t vmax_len = 0;
+    FilterView* d_fv = nullptr;
+    size_t fv_len = 0;
+
+    // request scratch
+    void* d_req = nullptr;           // tasks + tiles
+    size_t req_bytes = 0;
+    void* h_req = nullptr;           // pinned host staging
+    size_t h_req_bytes = 0;
+    float* d_out = nullptr;          // device output staging
+    size_t out_bytes = 0;
+    double* d_pts = nullptr;
+    size_t pts_bytes = 0;
+    double* d_misc = nullptr;
+    size_t misc_bytes = 0;
+};
+
+int ensure(void** p, size_t* have, size_t need);   // device buffer growth
+int ensure_host(void** p, size_t* have, size_t need);
+
+// ---------------------------------------------------------------- kernels (launchers)
+int launch_filter(Ctx* c, const float* d_block, int k, int n0, hipStream_t s);
+int launch_store(Ctx* c, const float* d_q, int k, hipStream_t s);
+int launch_filter_rows_plain(Ctx* c, const float* d_in, int rows, int n, float gain_tau,
+                             const float* d_H, const float2* d_tw, int N, float* d_out,
+                             hipStream_t s);
+int launch_bp_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles,
+                    int n_views, float scale, float* d_out, hipStream_t s);
+int launch_bp_points(Ctx* c, const double* d_pts, int64_t M, double cor_shift, int n_views,
+                     float scale, float* d_out, hipStream_t s);
+int launch_read_views(Ctx* c, int first, int count, float* d_out, hipStream_t s);
+int launch_cor_scores(Ctx* c, const float* d_img, int nimg, int64_t npix, double* d_scores,
+                      hipStream_t s);
+int launch_phantom(const double* d_ell, int K, int kind, const double* d_geo, int A, int R,
+                   int C, double pu, double pv, double far, float* out, hipStream_t s);
+
+// host float64 helpers
+int ramp_response_host(int n, double tau, int window, std::vector<double>& H);
+
+}  // namespace mr

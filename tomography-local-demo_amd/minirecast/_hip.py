@@ -1,0 +1,277 @@
+"""ctypes binding of libminirecast_hip.so (the C ABI in include/minirecast_hip.h).
+
+The library is built in-tree next to this file.  Loading it never touches the GPU;
+creating a context does.  There is no CPU fallback: when the library is missing,
+every call that needs it raises RuntimeError.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+LIB_NAME = "libminirecast_hip.so"
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+
+MR_OK, MR_EVALUE, MR_ERUNTIME, MR_ETYPE = 0, -1, -2, -3
+WINDOWS = {None: 0, "none": 0, "ram-lak": 0, "shepp-logan": 1, "hann": 2, "cosine": 3}
+STORAGE = {"f32": 0, "float32": 0, "f16": 1, "float16": 1, "bf16": 2, "bfloat16": 2}
+KIND_CONE, KIND_PARALLEL = 1, 2
+
+# every symbol the header declares: (name, restype, argtypes)
+_c = ctypes
+_P = _c.c_void_p
+SIGNATURES = [
+    ("mr_last_error", _c.c_char_p, []),
+    ("mr_abi_version", _c.c_int, []),
+    ("mr_device_count", _c.c_int, [_c.POINTER(_c.c_int)]),
+    ("mr_ramp_response", _c.c_int, [_c.c_int, _c.c_double, _c.c_int, _P, _c.POINTER(_c.c_int)]),
+    ("mr_ctx_create", _c.c_int, [_c.c_int, _c.POINTER(_P)]),
+    ("mr_ctx_destroy", _c.c_int, [_P]),
+    ("mr_set_geometry_cone", _c.c_int, [_P, _c.c_int, _c.c_int, _c.c_int, _P, _c.c_int, _c.c_int]),
+    ("mr_set_geometry_parallel", _c.c_int,
+     [_P, _c.c_int, _c.c_int, _c.c_int, _P, _c.c_double, _c.c_double, _c.c_int, _c.c_int]),
+    ("mr_push_views", _c.c_int, [_P, _P, _c.c_int, _P, _c.c_int, _c.POINTER(_c.c_int)]),
+    ("mr_store_views", _c.c_int, [_P, _P, _c.c_int, _P, _c.c_int, _c.POINTER(_c.c_int)]),
+    ("mr_num_views", _c.c_int, [_P, _c.POINTER(_c.c_int)]),
+    ("mr_bp_points", _c.c_int, [_P, _P, _c.c_int64, _c.c_double, _c.c_int, _P]),
+    ("mr_bp_tasks", _c.c_int, [_P, _P, _c.c_int, _c.c_int, _P, _c.c_int64, _c.c_int]),
+    ("mr_cor_scores", _c.c_int, [_P, _P, _c.c_int, _c.c_int, _P]),
+    ("mr_filter_rows", _c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_double, _c.c_int, _P]),
+    ("mr_read_views", _c.c_int, [_P, _c.c_int, _c.c_int, _P]),
+    ("mr_phantom_cone", _c.c_int, [_P, _P, _c.c_int, _P, _c.c_int, _c.c_int, _c.c_int, _P]),
+    ("mr_phantom_parallel", _c.c_int,
+     [_P, _P, _c.c_int, _P, _c.c_int, _c.c_int, _c.c_int, _c.c_double, _c.c_double,
+      _c.c_double, _P]),
+    ("mr_device_alloc", _c.c_int, [_P, _c.c_int64, _c.POINTER(_P)]),
+    ("mr_device_free", _c.c_int, [_P, _P]),
+    ("mr_memcpy", _c.c_int, [_P, _P, _P, _c.c_int64, _c.c_int]),
+    ("mr_synchronize", _c.c_int, [_P]),
+    ("mr_last_kernel_ms", _c.c_int, [_P, _c.POINTER(_c.c_double)]),
+]
+
+# mr_task (include/minirecast_hip.h)
+TASK_DTYPE = np.dtype([
+    ("o", "<f8", 3), ("a", "<f8", 3), ("b", "<f8", 3), ("c", "<f8", 3),
+    ("mo", "<f8", 3), ("me1", "<f8", 3), ("me2", "<f8", 3), ("box", "<f8", 3),
+    ("mhalf", "<f8"), ("mstep", "<f8"), ("cor_shift", "<f8"),
+    ("out_offset", "<i8"), ("stride_q", "<i8"), ("stride_p", "<i8"), ("stride_s", "<i8"),
+    ("nq", "<i4"), ("np", "<i4"), ("ns", "<i4"), ("mask_box", "<i4"),
+])
+assert TASK_DTYPE.itemsize == 264
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+class HipUnavailable(RuntimeError):
+    """The HIP library or a GPU is not available."""
+
+
+def load(path=None):
+    """Load (once) and return the ctypes library; RuntimeError if it is missing."""
+    global _lib
+    with _lib_lock:
+        if _lib is not None:
+            return _lib
+        p = path or os.environ.get("MINIRECAST_HIP_LIB", LIB_PATH)
+        if not os.path.exists(p):
+            raise HipUnavailable(f"{LIB_NAME} not built ({p}); run tomography-local-demo_amd/build.py")
+        lib = ctypes.CDLL(p)
+        for name, res, args in SIGNATURES:
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+        return lib
+
+
+def lib_path():
+    return os.environ.get("MINIRECAST_HIP_LIB", LIB_PATH)
+
+
+def check(rc):
+    """Map a C return code to the reference's exception types."""
+    if rc == MR_OK:
+        return
+    msg = load().mr_last_error().decode(errors="replace")
+    if rc == MR_EVALUE:
+        raise ValueError(msg)
+    if rc == MR_ETYPE:
+        raise TypeError(msg)
+    raise RuntimeError(msg)
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    check(load().mr_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def window_code(window):
+    if window not in WINDOWS:
+        raise ValueError(f"unknown filter window {window!r}")
+    return WINDOWS[window]
+
+
+def ramp_response(n, tau, window="shepp-logan"):
+    """(H float64 [N], N) from the library's float64 host routine."""
+    code = window_code(window)
+    N = ctypes.c_int(0)
+    check(load().mr_ramp_response(int(n), float(tau), code, None, ctypes.byref(N)))
+    H = np.empty(N.value, dtype=np.float64)
+    check(load().mr_ramp_response(int(n), float(tau), code, ptr(H), ctypes.byref(N)))
+    return H, N.value
+
+
+class Context:
+    """Owns one mr_ctx (device ring + streams) for one reconstructor."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        check(self.lib.mr_ctx_create(int(device), ctypes.byref(h)))
+        self.h = h
+        self.device = int(device)
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.mr_ctx_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- geometry
+    def set_cone(self, vectors, rows, cols, window, storage="f32"):
+        V = np.ascontiguousarray(vectors, dtype=np.float64).reshape(-1, 12)
+        check(self.lib.mr_set_geometry_cone(self.h, len(V), int(rows), int(cols), ptr(V),
+                                            window_code(window), STORAGE[storage]))
+
+    def set_parallel(self, angles, rows, cols, pixel_u, pixel_v, window, storage="f32"):
+        a = np.ascontiguousarray(angles, dtype=np.float64).ravel()
+        check(self.lib.mr_set_geometry_parallel(self.h, len(a), int(rows), int(cols), ptr(a),
+                                                float(pixel_u), float(pixel_v),
+                                                window_code(window), STORAGE[storage]))
+
+    # ---------------------------------------------------------------- ingest
+    def push(self, indices, block):
+        idx = np.ascontiguousarray(indices, dtype=np.int64).ravel()
+        blk = np.ascontiguousarray(block, dtype=np.float32)
+        n = ctypes.c_int(0)
+        check(self.lib.mr_push_views(self.h, ptr(idx), len(idx), ptr(blk), 0, ctypes.byref(n)))
+        return n.value
+
+    def push_device(self, indices, dptr):
+        idx = np.ascontiguousarray(indices, dtype=np.int64).ravel()
+        n = ctypes.c_int(0)
+        check(self.lib.mr_push_views(self.h, ptr(idx), len(idx), ctypes.c_void_p(dptr), 1,
+                                     ctypes.byref(n)))
+        return n.value
+
+    def store(self, indices, q):
+        """Append already-filtered views q [k, rows, cols]."""
+        idx = np.ascontiguousarray(indices, dtype=np.int64).ravel()
+        qq = np.ascontiguousarray(q, dtype=np.float32)
+        n = ctypes.c_int(0)
+        check(self.lib.mr_store_views(self.h, ptr(idx), len(idx), ptr(qq), 0, ctypes.byref(n)))
+        return n.value
+
+    def store_device(self, indices, dptr):
+        idx = np.ascontiguousarray(indices, dtype=np.int64).ravel()
+        n = ctypes.c_int(0)
+        check(self.lib.mr_store_views(self.h, ptr(idx), len(idx), ctypes.c_void_p(dptr), 1,
+                                      ctypes.byref(n)))
+        return n.value
+
+    def num_views(self):
+        n = ctypes.c_int(0)
+        check(self.lib.mr_num_views(self.h, ctypes.byref(n)))
+        return n.value
+
+    # ---------------------------------------------------------------- requests
+    def bp_points(self, pts, cor_shift, n_views):
+        P = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(len(P), dtype=np.float64)
+        if len(P):
+            check(self.lib.mr_bp_points(self.h, ptr(P), len(P), float(cor_shift), int(n_views),
+                                        ptr(out)))
+        return out
+
+    def bp_tasks(self, tasks, n_views, out_len):
+        T = np.ascontiguousarray(tasks, dtype=TASK_DTYPE)
+        out = np.zeros(int(out_len), dtype=np.float32)
+        if len(T) and out_len:
+            check(self.lib.mr_bp_tasks(self.h, ptr(T), len(T), int(n_views), ptr(out),
+                                       int(out_len), 0))
+        return out
+
+    def bp_tasks_device(self, tasks, n_views, dptr, out_len):
+        T = np.ascontiguousarray(tasks, dtype=TASK_DTYPE)
+        check(self.lib.mr_bp_tasks(self.h, ptr(T), len(T), int(n_views), ctypes.c_void_p(dptr),
+                                   int(out_len), 1))
+
+    def cor_scores(self, tasks, n_views):
+        T = np.ascontiguousarray(tasks, dtype=TASK_DTYPE)
+        scores = np.zeros(len(T), dtype=np.float64)
+        if len(T):
+            check(self.lib.mr_cor_scores(self.h, ptr(T), len(T), int(n_views), ptr(scores)))
+        return scores
+
+    def filter_rows(self, rows2d, tau, window):
+        x = np.ascontiguousarray(rows2d, dtype=np.float32)
+        R, n = x.shape
+        out = np.empty_like(x)
+        check(self.lib.mr_filter_rows(self.h, ptr(x), R, n, float(tau), window_code(window),
+                                      ptr(out)))
+        return out
+
+    def read_views(self, first, count, rows, cols):
+        out = np.empty((int(count), int(rows), int(cols)), dtype=np.float32)
+        check(self.lib.mr_read_views(self.h, int(first), int(count), ptr(out)))
+        return out
+
+    def last_kernel_ms(self):
+        ms = ctypes.c_double(0.0)
+        check(self.lib.mr_last_kernel_ms(self.h, ctypes.byref(ms)))
+        return ms.value
+
+    # ---------------------------------------------------------------- device memory
+    def alloc(self, nbytes):
+        p = ctypes.c_void_p()
+        check(self.lib.mr_device_alloc(self.h, int(nbytes), ctypes.byref(p)))
+        return p.value
+
+    def free(self, dptr):
+        check(self.lib.mr_device_free(self.h, ctypes.c_void_p(dptr)))
+
+    def h2d(self, dptr, arr):
+        a = np.ascontiguousarray(arr)
+        check(self.lib.mr_memcpy(self.h, ctypes.c_void_p(dptr), ptr(a), a.nbytes, 0))
+
+    def d2h(self, arr, dptr):
+        check(self.lib.mr_memcpy(self.h, ptr(arr), ctypes.c_void_p(dptr), arr.nbytes, 1))
+        return arr
+
+    def synchronize(self):
+        check(self.lib.mr_synchronize(self.h))
+
+    def phantom_cone(self, ellipsoids, vectors, rows, cols, dptr):
+        E = np.ascontiguousarray(ellipsoids, dtype=np.float64).reshape(-1, 13)
+        V = np.ascontiguousarray(vectors, dtype=np.float64).reshape(-1, 12)
+        check(self.lib.mr_phantom_cone(self.h, ptr(E), len(E), ptr(V), len(V), int(rows),
+                                       int(cols), ctypes.c_void_p(dptr)))
+
+    def phantom_parallel(self, ellipsoids, angles, rows, cols, pu, pv, far, dptr):
+        E = np.ascontiguousarray(ellipsoids, dtype=np.float64).reshape(-1, 13)
+        a = np.ascontiguousarray(angles, dtype=np.float64).ravel()
+        check(self.lib.mr_phantom_parallel(self.h, ptr(E), len(E), ptr(a), len(a), int(rows),
+                                           int(cols), float(pu), float(pv), float(far),
+                                           ctypes.c_void_p(dptr)))
