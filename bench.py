@@ -75,6 +75,7 @@ def main():
     ap.add_argument("--no-check", action="store_true", help="skip the float64 oracle error")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--views", type=int, default=None, help="debug: fewer views")
+    ap.add_argument("--forms", default="f32,f16,bf16", help="storage forms to time")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
@@ -95,7 +96,9 @@ def main():
 
     # ---------------------------------------------------------------- rings
     t0 = time.perf_counter()
-    forms = ("f32", "f16", "bf16")
+    forms = tuple(f for f in args.forms.split(",") if f)
+    if "f32" not in forms:
+        forms = ("f32",) + forms          # the oracle baseline and CPU q come from f32
     ctx = {s: _hip.Context(0) for s in forms}
     for s in forms:
         ctx[s].set_cone(geom.vectors, R, C, w.window, s)
@@ -196,7 +199,7 @@ def main():
                                            gups=n_in * A / (wall * 1e-3) / 1e9)
         c.free(dout)
         log(f"{s}: {results[s]}")
-    log(f"f32 three launches: {results['f32_separate']}")
+    log(f"f32 three launches: {results.get('f32_separate')}")
 
     # ---------------------------------------------------------------- roofline
     all_pts = []
@@ -206,13 +209,13 @@ def main():
     all_pts = np.concatenate(all_pts)
     fviews = np.linspace(0, A - 1, 8).astype(int)
     F = footprint_pixels(geom, all_pts, fviews)
-    head = "f16"
+    head = "f16" if "f16" in forms else "f32"
     hbm_bytes = F * A * 2 + out_len * 4
     kern_s = results[head]["kernel_ms"] * 1e-3
     achieved = hbm_bytes / kern_s / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel="bp_tiles_kernel<cone, __half>",
+                    kernel=f"bp_tiles_kernel (cone, {head})",
                     algorithmic_bytes=int(hbm_bytes), footprint_px_per_view=round(F),
                     gather_bytes_l2=int(4 * 2 * n_in * A))
     log(f"roofline: {roofline}")

@@ -91,6 +91,17 @@ int mr_push_views(mr_ctx* ctx, const int64_t* indices, int k, const float* block
 int mr_store_views(mr_ctx* ctx, const int64_t* indices, int k, const float* q, int q_on_device,
                    int* n_out);
 
+/* Preprocessing applied by mr_push_views to every later block (preprocess.py:59-64
+ * apply, called at recon.py:357): rotation by -tilt degrees about the detector centre,
+ * then a Gaussian low-pass of sigma = gaussian detector pixels; 0 disables each. */
+int mr_set_preprocess(mr_ctx* ctx, double gaussian, double tilt);
+
+/* preprocess.apply on a host block [rows][k][cols] float32 (preprocess.py:59-64;
+ * gaussian_blur :8-28 with tilt = 0, rotate_projections :31-56 with gaussian = 0 and
+ * tilt = -degrees).  out: host [rows][k][cols] float32. */
+int mr_preprocess(mr_ctx* ctx, const float* in, int rows, int k, int cols, double gaussian,
+                  double tilt, float* out);
+
 /* Views stored so far (recon.py:319 self.n). */
 int mr_num_views(mr_ctx* ctx, int* n);
 

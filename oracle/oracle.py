@@ -8,8 +8,9 @@ Every function cites the reference file:line it restates.  Unlike the reference
 (which filters parallel-beam rows in complex64 under numpy >= 2 and stores the
 filtered projections as float32) the oracle keeps everything in float64: it is
 the "float64 NumPy oracle of the same FDK/FBP" against which every GPU result
-states its error.  ``RefCompat`` switches reproduce the reference's float32
-storage where a test wants reference-equivalent inputs.
+states its error.  ``OracleReconstructor(store_f32=True)`` rounds the filtered
+views to float32 as the reference stores them (recon.py:64, 320), giving
+reference-equivalent values.
 
 How each function is pinned (tests/test_oracle.py):
   ramp_response        brute-force O(N^2) DFT of the spatial kernel; ram-lak
@@ -31,8 +32,11 @@ How each function is pinned (tests/test_oracle.py):
   histogram_score      closed form on a hand-made histogram; numpy.histogram.
   arrival_order        known answer [0,8,4,2,6,1,9,5,3,7].
   gaussian_blur / rotate_projections   selftest.py:104-117 invariants.
-  line_integrals       closed-form chord of a sphere; Monte-Carlo free
-                       (exact) agreement with numerical quadrature.
+  line_integrals       closed-form chord of a sphere.
+  OracleReconstructor  the reference's own outputs on its analytic phantom
+                       (tests/golden/reference_composition.npz): bit-exact for
+                       cone beam, 1e-6 for parallel beam (the reference filters
+                       parallel rows in complex64).
 """
 import numpy as np
 
@@ -296,12 +300,14 @@ class OracleReconstructor:
     does (recon.py:64, 320), giving reference-equivalent values."""
 
     def __init__(self, kind, rows, cols, box, window="shepp-logan", vectors=None,
-                 angles=None, pixel_u=1.0, pixel_v=1.0, store_f32=False):
+                 angles=None, pixel_u=1.0, pixel_v=1.0, store_f32=False, gaussian=0.0,
+                 tilt=0.0):
         self.kind = kind
         self.R, self.C = int(rows), int(cols)
         self.box = np.asarray(box, float)
         self.window = window
         self.store_f32 = store_f32
+        self.gaussian, self.tilt = float(gaussian), float(tilt)
         self.pixel_u, self.pixel_v = float(pixel_u), float(pixel_v)
         if kind == "cone":
             self.K_all = cone_constants(vectors)
@@ -316,7 +322,7 @@ class OracleReconstructor:
         self.cor_shift = 0.0
 
     def add_projections(self, indices, block):
-        """recon.py:346-376 (float64 filtering, no preprocessing)."""
+        """recon.py:346-376 (float64 preprocessing and filtering)."""
         indices = np.asarray(indices, dtype=np.int64).ravel()
         block = np.asarray(block, dtype=np.float32)
         if block.shape[1] != len(indices):
@@ -325,6 +331,7 @@ class OracleReconstructor:
         indices, block = indices[keep], block[:, keep, :]
         if len(indices) == 0:
             return self.n
+        block = preprocess_apply(block, self.gaussian, self.tilt)        # recon.py:357
         if self.kind == "parallel":
             filt = np.transpose(filter_rows(block, self.pixel_u, self.window), (1, 0, 2))
         else:

@@ -214,10 +214,17 @@ def test_edge_cases_and_errors():
     # partial tiles
     orc = O.OracleReconstructor("cone", 9, 13, g.volume_box(), vectors=g.vectors, store_f32=True)
     orc.add_projections([3, 3, 5], proj[:, [3, 3, 5], :])
+    got, want = [], []
     for n in (1, 31, 33, 70):
         a, _ = rec.reconstruct_plane((0.1, 0.2, 0.3), (0.3, -1, 0.4), n=n)
         b, _ = orc.reconstruct_plane((0.1, 0.2, 0.3), (0.3, -1, 0.4), n=n)
-        assert rel_l2(a, b) < TOL if np.any(b) else not np.any(a)
+        np.testing.assert_array_equal(a == 0, b == 0)
+        got.append(a.ravel())
+        want.append(b.ravel())
+    # three views of filtered noise: single pixels cancel, so compare all sizes together
+    err = rel_l2(np.concatenate(got), np.concatenate(want))
+    print(f"partial tiles rel-L2 {err:.2e}")
+    assert err < TOL
 
 
 def test_module_level_bp_functions_vs_oracle():
@@ -282,3 +289,75 @@ def test_storage_precision_vs_float64_oracle(small_cone):
         qref = orc.q[:4]
         assert rel_l2(q, qref) < {"f32": TOL, "f16": 1e-3, "bf16": 5e-3}[storage]
     assert errs["f32"] < errs["f16"] < errs["bf16"]
+
+
+@pytest.mark.parametrize("variant", ["1", "2", "3"])
+def test_bp_kernel_variants_vs_oracle(small_cone, variant, monkeypatch):
+    """Both tile-kernel inner loops (MINIRECAST_BP_VARIANT) against the float64 oracle,
+    for every storage form, cone beam; and parallel beam on the reference fixture."""
+    from minirecast import recon
+    monkeypatch.setenv("MINIRECAST_BP_VARIANT", variant)
+    w, proj, _, orc = small_cone
+    p = dict(origin=(0.5, -0.3, 0.2), normal=(1.0, 0.5, 0.8), n=77)
+    ref, _ = orc.reconstruct_plane(p['origin'], p['normal'], n=p['n'])
+    for storage, tol in (("f32", TOL), ("f16", 1e-3), ("bf16", 5e-3)):
+        rec = recon.SliceReconstructor(proj, w.geom, window=w.window, verbose=False,
+                                       storage=storage)
+        img, _ = rec.reconstruct_plane(p['origin'], p['normal'], n=p['n'])
+        assert rel_l2(img, ref) < tol, (variant, storage, rel_l2(img, ref))
+
+
+@pytest.mark.parametrize("variant", ["1", "2", "3"])
+def test_bp_kernel_variants_parallel(golden, variant, monkeypatch):
+    from minirecast import recon
+    from minirecast.geometry import ParallelGeometry
+    monkeypatch.setenv("MINIRECAST_BP_VARIANT", variant)
+    R, C = (int(x) for x in golden['par_shape'])
+    pu, pv = golden['par_pixel']
+    g = ParallelGeometry(golden['par_angles'], R, C, pu, pv)
+    rec = recon.SliceReconstructor(golden['par_proj'], g, window="hann", verbose=False)
+    img, _ = rec.reconstruct_plane((1.0, 0, 0), (1, 0.5, 0.8), n=36, size=100.0)
+    assert rel_l2(img, golden['par_plane_oblique']) < TOL
+    img, _ = rec.reconstruct_plane((0, 0, 0), (0, 0, 1), n=32, cor_shift=-0.75)
+    assert rel_l2(img, golden['par_plane_cor']) < TOL
+
+
+# ---------------------------------------------------------------- preprocessing
+@pytest.mark.parametrize("sigma", [0.7, 2.0, 9.0])
+def test_gaussian_blur_vs_oracle(sigma):
+    from minirecast import preprocess
+    rng = np.random.default_rng(1)
+    block = rng.normal(1.0, 0.3, (40, 3, 50)).astype(np.float32)
+    got = preprocess.gaussian_blur(block, sigma)
+    assert got.dtype == np.float32 and got.shape == block.shape
+    assert rel_l2(got, O.gaussian_blur(block, sigma)) < 1e-6
+    if sigma == 2.0:        # selftest.py:104-111
+        assert abs(got.mean() - block.mean()) < 0.01 and got.std() < 0.5 * block.std()
+
+
+def test_rotate_projections_vs_oracle():
+    from minirecast import preprocess
+    rng = np.random.default_rng(2)
+    block = rng.normal(size=(31, 2, 45)).astype(np.float32)
+    for deg in (90.0, -7.5, 0.3):
+        got = preprocess.rotate_projections(block, deg)
+        assert rel_l2(got, O.rotate_projections(block, deg)) < 1e-6
+    img = np.zeros((41, 1, 41), np.float32)
+    img[20, 0, 30] = 1.0                                   # selftest.py:112-116
+    rot = preprocess.rotate_projections(img, 90.0)
+    r, c = np.unravel_index(np.argmax(rot[:, 0, :]), (41, 41))
+    assert (r, c) in ((10, 20), (30, 20))
+    np.testing.assert_array_equal(preprocess.apply(block), block)
+
+
+def test_reconstructor_with_preprocessing_vs_oracle(small_cone):
+    from minirecast import recon
+    w, proj, _, _ = small_cone
+    rec = recon.SliceReconstructor(proj, w.geom, window=w.window, verbose=False, gaussian=1.5,
+                                   tilt=0.7)
+    orc = O.OracleReconstructor("cone", w.geom.rows, w.geom.cols, w.geom.volume_box(),
+                                window=w.window, vectors=w.geom.vectors, gaussian=1.5, tilt=0.7)
+    orc.add_projections(np.arange(w.geom.n_angles), proj)
+    img, _ = rec.reconstruct_plane((0, 0, 0), (1.0, 0.5, 0.8), n=64)
+    ref, _ = orc.reconstruct_plane((0, 0, 0), (1.0, 0.5, 0.8), n=64)
+    assert rel_l2(img, ref) < TOL

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
 
 #include "mr_internal.h"
@@ -320,6 +321,10 @@ int mr_ctx_create(int device, mr_ctx** out) {
     MR_HIP(hipSetDevice(device));
     mr_ctx* c = new mr_ctx();
     c->device = device;
+    if (const char* v = std::getenv("MINIRECAST_BP_VARIANT")) {
+        const int k = std::atoi(v);
+        c->bp_variant = (k >= 1 && k <= 3) ? k : c->bp_variant;
+    }
     // default-priority, non-blocking streams
     if (hipStreamCreateWithFlags(&c->s_ingest, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->s_req, hipStreamNonBlocking) != hipSuccess ||
@@ -474,11 +479,23 @@ static int append_views(mr_ctx* c, const int64_t* indices, int k, const float* b
     MR_HIP(hipSetDevice(c->device));
     const size_t blk = sizeof(float) * static_cast<size_t>(c->R) * k * c->C;
     const float* d_block = block;
-    if (!on_device) {
+    const bool pre = filter && (c->gaussian > 0.0 || std::fabs(c->tilt) >= 1e-9);
+    if (!on_device || pre) {
         rc = ensure(reinterpret_cast<void**>(&c->d_stage), &c->stage_bytes, blk);
         if (rc) return rc;
-        MR_HIP(hipMemcpyAsync(c->d_stage, block, blk, hipMemcpyHostToDevice, c->s_ingest));
+        MR_HIP(hipMemcpyAsync(c->d_stage, block, blk,
+                              on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->s_ingest));
         d_block = c->d_stage;
+    }
+    if (pre) {
+        // preprocess.apply (recon.py:357): tilt, then Gaussian, on the staged block
+        rc = ensure(reinterpret_cast<void**>(&c->d_pre), &c->pre_bytes, blk);
+        if (rc) return rc;
+        float* res = nullptr;
+        rc = launch_preprocess(c, c->d_stage, c->d_pre, c->R, k, c->C, c->gaussian, c->tilt, &res,
+                               c->s_ingest);
+        if (rc) return rc;
+        d_block = res;
     }
     std::vector<FilterView> fv(k);
     std::vector<ViewGeom> gv(k);
@@ -516,6 +533,41 @@ int mr_push_views(mr_ctx* c, const int64_t* indices, int k, const float* block,
 int mr_store_views(mr_ctx* c, const int64_t* indices, int k, const float* q, int q_on_device,
                    int* n_out) {
     return append_views(c, indices, k, q, q_on_device, n_out, false);
+}
+
+int mr_set_preprocess(mr_ctx* c, double gaussian, double tilt) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    MR_CHECK(std::isfinite(gaussian) && std::isfinite(tilt), MR_EVALUE, "non-finite preprocessing");
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->gaussian = gaussian > 0.0 ? gaussian : 0.0;
+    c->tilt = tilt;
+    return MR_OK;
+}
+
+int mr_preprocess(mr_ctx* c, const float* in, int rows, int k, int cols, double gaussian,
+                  double tilt, float* out) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    MR_CHECK(rows >= 0 && k >= 0 && cols >= 0, MR_EVALUE, "negative block shape");
+    MR_CHECK(std::isfinite(gaussian) && std::isfinite(tilt), MR_EVALUE, "non-finite preprocessing");
+    const size_t blk = sizeof(float) * static_cast<size_t>(rows) * k * cols;
+    if (blk == 0) return MR_OK;
+    MR_CHECK(in && out, MR_EVALUE, "null block");
+    std::lock_guard<std::mutex> lk(c->mu);
+    MR_HIP(hipSetDevice(c->device));
+    rc = ensure(reinterpret_cast<void**>(&c->d_stage), &c->stage_bytes, blk);
+    if (rc) return rc;
+    rc = ensure(reinterpret_cast<void**>(&c->d_pre), &c->pre_bytes, blk);
+    if (rc) return rc;
+    MR_HIP(hipMemcpyAsync(c->d_stage, in, blk, hipMemcpyHostToDevice, c->s_ingest));
+    float* res = c->d_stage;
+    rc = launch_preprocess(c, c->d_stage, c->d_pre, rows, k, cols, gaussian > 0.0 ? gaussian : 0.0,
+                           tilt, &res, c->s_ingest);
+    if (rc) return rc;
+    MR_HIP(hipMemcpyAsync(out, res, blk, hipMemcpyDeviceToHost, c->s_ingest));
+    MR_HIP(hipStreamSynchronize(c->s_ingest));
+    return MR_OK;
 }
 
 int mr_num_views(mr_ctx* c, int* n) {

@@ -63,9 +63,14 @@ __device__ bool in_box(const mr_task& T, int p, int s) {
 // the in-tile offsets (ds along the fast axis b, dp along the slow axis a), shifted
 // so that uh/wh, vh/wh are columns/rows relative to the reference (cref, rref);
 // clamp bounds of the relative coordinates; per-view storage scale; base index.
+struct RecRef {
+    double cref, rref;                        // integer detector reference (col, row)
+    double U0, Ua, Ub, V0, Va, Vb, W0, Wa, Wb;  // float64 relative coefficients (unscaled)
+};
+
 __device__ void make_record(const ViewGeom* __restrict__ gp, float vs, const mr_task& T,
                             const double* xt, int R, int C, int pitch, bool cone,
-                            float* __restrict__ rec) {
+                            float* __restrict__ rec, RecRef* ref = nullptr) {
     const ViewGeom g = *gp;
     const double sh = T.cor_shift;
     const double W0 = dot3(g.P[2], xt) + g.P[2][3];
@@ -89,6 +94,7 @@ __device__ void make_record(const ViewGeom* __restrict__ gp, float vs, const mr_
     const double dc = cc - cref, dr = cr - rref;
     U0 += dc * W0; Ua += dc * Wa; Ub += dc * Wb;
     V0 += dr * W0; Va += dr * Wa; Vb += dr * Wb;
+    if (ref) *ref = RecRef{cref, rref, U0, Ua, Ub, V0, Va, Vb, W0, Wa, Wb};
     // cone: the weight 1/wh^2 absorbs the view's storage scale
     const double f = cone ? 1.0 / sqrt(static_cast<double>(vs)) : 1.0;
     float4* r4 = reinterpret_cast<float4*>(rec);
@@ -237,6 +243,422 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel(
     }
 }
 
+// ---------------------------------------------------------------- v2 inner loop
+// Same tiles, records and arithmetic as bp_tiles_kernel; the gathers go through a
+// buffer resource per view (base = the view's first element, range = the view), so
+// the per-point address is one 24-bit multiply-add and one shift-add of absolute
+// byte offsets, the second detector row comes from the SGPR offset, the +1 column
+// from the immediate offset, and an out-of-range offset can only read zero.
+__device__ inline int cvt_flr(float x) {
+    int r;
+    asm volatile("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+template <typename ST> struct Taps;
+template <> struct Taps<float> {
+    static constexpr int LOG_ESZ = 2;
+    __device__ static inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int soff1, float& a,
+                                       float& b, float& c, float& d) {
+        // the builtin returns the dword's bits
+        a = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, voff, 0, 0));
+        b = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, voff + 4, 0, 0));
+        c = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff1, 0));
+        d = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, voff + 4, soff1, 0));
+    }
+};
+template <> struct Taps<__half> {
+    static constexpr int LOG_ESZ = 1;
+    __device__ static inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int soff1, float& a,
+                                       float& b, float& c, float& d) {
+        const unsigned short ua = __builtin_amdgcn_raw_buffer_load_b16(rs, voff, 0, 0);
+        const unsigned short ub = __builtin_amdgcn_raw_buffer_load_b16(rs, voff + 2, 0, 0);
+        const unsigned short uc = __builtin_amdgcn_raw_buffer_load_b16(rs, voff, soff1, 0);
+        const unsigned short ud = __builtin_amdgcn_raw_buffer_load_b16(rs, voff + 2, soff1, 0);
+        a = __half2float(__ushort_as_half(ua));
+        b = __half2float(__ushort_as_half(ub));
+        c = __half2float(__ushort_as_half(uc));
+        d = __half2float(__ushort_as_half(ud));
+    }
+};
+template <> struct Taps<uint16_t> {
+    static constexpr int LOG_ESZ = 1;
+    __device__ static inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int soff1, float& a,
+                                       float& b, float& c, float& d) {
+        const unsigned short ua = __builtin_amdgcn_raw_buffer_load_b16(rs, voff, 0, 0);
+        const unsigned short ub = __builtin_amdgcn_raw_buffer_load_b16(rs, voff + 2, 0, 0);
+        const unsigned short uc = __builtin_amdgcn_raw_buffer_load_b16(rs, voff, soff1, 0);
+        const unsigned short ud = __builtin_amdgcn_raw_buffer_load_b16(rs, voff + 2, soff1, 0);
+        a = __uint_as_float(static_cast<uint32_t>(ua) << 16);
+        b = __uint_as_float(static_cast<uint32_t>(ub) << 16);
+        c = __uint_as_float(static_cast<uint32_t>(uc) << 16);
+        d = __uint_as_float(static_cast<uint32_t>(ud) << 16);
+    }
+};
+
+template <bool CONE, typename ST>
+__global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel2(
+    const ST* __restrict__ ring, int64_t view_stride, int pitch, int R, int C,
+    const ViewGeom* __restrict__ geom, const float* __restrict__ vscale,
+    const mr_task* __restrict__ tasks, const Tile* __restrict__ tiles, int n_views,
+    float scale, float* __restrict__ out) {
+    __shared__ float4 rec4[BP_VPASS * BP_REC / 4];
+    float* rec = reinterpret_cast<float*>(rec4);
+    constexpr int LE = Taps<ST>::LOG_ESZ;
+
+    const Tile tl = tiles[blockIdx.x];
+    const mr_task& T = tasks[tl.task];
+    const int t = threadIdx.x;
+    const int tx = t & (BP_TS - 1);
+    const int ty = t / BP_TS;
+    constexpr int PSTEP = BP_THREADS / BP_TS;
+    constexpr int PPT = BP_TP / PSTEP;
+
+    const int p0 = tl.tp * BP_TP, s0 = tl.ts * BP_TS;
+    double xt[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        xt[d] = T.o[d] + tl.q * T.c[d] + p0 * T.a[d] + s0 * T.b[d];
+
+    const int s = s0 + tx;
+    bool valid[PPT], inside[PPT];
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) {
+        const int p = p0 + ty + PSTEP * m;
+        valid[m] = (p < T.np) && (s < T.ns);
+        inside[m] = valid[m] && (!T.mask_box || in_box(T, p, s));
+    }
+
+    const float ds = static_cast<float>(tx);
+    const float dp0 = static_cast<float>(ty);
+    const int pitchb = pitch << LE;                        // row pitch in bytes
+    const int view_bytes = static_cast<int>(view_stride << LE);
+    float acc[PPT], accb[PPT], fr_par[PPT];
+    int roff_par[PPT];
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) { acc[m] = 0.0f; accb[m] = 0.0f; fr_par[m] = 0.0f; roff_par[m] = 0; }
+
+    for (int vb = 0; vb < n_views; vb += BP_VPASS) {
+        const int nv = min(BP_VPASS, n_views - vb);
+        __syncthreads();
+        if (t < nv)
+            make_record(geom + vb + t, vscale[vb + t], T, xt, R, C, pitch, CONE, rec + t * BP_REC);
+        __syncthreads();
+        if (!CONE && vb == 0) {
+            const float4 r0 = rec4[0], r1 = rec4[1], r2 = rec4[2], r3 = rec4[3];
+#pragma unroll
+            for (int m = 0; m < PPT; ++m) {
+                const float dp = dp0 + static_cast<float>(PSTEP * m);
+                float row = fmaf(dp, r1.y, fmaf(ds, r1.x, r0.w));
+                row = __builtin_amdgcn_fmed3f(row, r2.w, r3.x);
+                fr_par[m] = __builtin_amdgcn_fractf(row);
+                roff_par[m] = cvt_flr(row) * pitchb;
+            }
+        }
+#pragma unroll 2
+        for (int v = 0; v < nv; ++v) {
+            const float4 r0 = rec4[v * 4 + 0];
+            const float4 r1 = rec4[v * 4 + 1];
+            const float4 r2 = rec4[v * 4 + 2];
+            const float4 r3 = rec4[v * 4 + 3];
+            const int baseb = __float_as_int(r3.z) << LE;
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<ST*>(ring + static_cast<int64_t>(vb + v) * view_stride), 0, view_bytes,
+                0x00020000);
+            if (CONE) {
+                const float uh0 = fmaf(dp0, r0.z, fmaf(ds, r0.y, r0.x));
+                const float vh0 = fmaf(dp0, r1.y, fmaf(ds, r1.x, r0.w));
+                const float wh0 = fmaf(dp0, r2.x, fmaf(ds, r1.w, r1.z));
+#pragma unroll
+                for (int m = 0; m < PPT; ++m) {
+                    const float dm = static_cast<float>(PSTEP * m);
+                    const float uh = m ? fmaf(dm, r0.z, uh0) : uh0;
+                    const float vh = m ? fmaf(dm, r1.y, vh0) : vh0;
+                    const float wh = m ? fmaf(dm, r2.x, wh0) : wh0;
+                    const float rw = __builtin_amdgcn_rcpf(wh);
+                    const float col = __builtin_amdgcn_fmed3f(uh * rw, r2.y, r2.z);
+                    const float row = __builtin_amdgcn_fmed3f(vh * rw, r2.w, r3.x);
+                    const float wt = rw * rw;
+                    const float fc = __builtin_amdgcn_fractf(col);
+                    const float fr = __builtin_amdgcn_fractf(row);
+                    const int voff = __mul24(cvt_flr(row), pitchb) + ((cvt_flr(col) << LE) + baseb);
+                    float a, b, c, d;
+                    Taps<ST>::load(rs, voff, pitchb, a, b, c, d);
+                    const float top = fmaf(fc, b - a, a);
+                    const float bot = fmaf(fc, d - c, c);
+                    acc[m] = fmaf(wt, fmaf(fr, bot - top, top), acc[m]);
+                }
+            } else {
+                const float vs = r3.y;
+                const float uh0 = fmaf(dp0, r0.z, fmaf(ds, r0.y, r0.x));
+#pragma unroll
+                for (int m = 0; m < PPT; ++m) {
+                    const float dm = static_cast<float>(PSTEP * m);
+                    const float col = __builtin_amdgcn_fmed3f(m ? fmaf(dm, r0.z, uh0) : uh0, r2.y, r2.z);
+                    const float fc = __builtin_amdgcn_fractf(col);
+                    const int voff = roff_par[m] + ((cvt_flr(col) << LE) + baseb);
+                    float a, b, c, d;
+                    Taps<ST>::load(rs, voff, pitchb, a, b, c, d);
+                    acc[m] = fmaf(vs, fmaf(fc, b - a, a), acc[m]);
+                    accb[m] = fmaf(vs, fmaf(fc, d - c, c), accb[m]);
+                }
+            }
+        }
+    }
+
+    const int64_t obase = T.out_offset + static_cast<int64_t>(tl.q) * T.stride_q;
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) {
+        if (!valid[m]) continue;
+        const int p = p0 + ty + PSTEP * m;
+        float val = acc[m];
+        if (!CONE) val = fmaf(fr_par[m], accb[m] - acc[m], acc[m]);
+        out[obase + static_cast<int64_t>(p) * T.stride_p + static_cast<int64_t>(s) * T.stride_s] =
+            inside[m] ? val * scale : 0.0f;
+    }
+}
+
+// ---------------------------------------------------------------- v3: LDS-staged taps
+// Oblique slices make the 64 lanes of one gather instruction fall on ~30 detector
+// rows, so direct gathers fetch mostly unused cache lines.  v3 stages, per (tile,
+// view), the detector bounding box of the tile's taps into LDS with 16-byte loads
+// (fully coalesced rows, one buffer resource per view so that reads past the view
+// return zero), then gathers the four taps from LDS.  A view whose box does not fit
+// the stage buffer falls back to the direct gathers of v2 (uniform branch).
+constexpr int V3_REC = 24;            // floats per (tile, view) record
+constexpr int V3_VPASS = 32;          // views per record pass
+constexpr int V3_STAGE_BYTES = 32768; // LDS stage buffer
+
+// The box: float64 projections of the 4 tile corners (a projective map sends the
+// tile's convex hull into the hull of the corner images), clamped like the inner loop,
+// widened by one pixel each side (FP32 inner-loop coordinates differ from float64 by
+// ~1e-5 px), rows/cols of the two taps, aligned to 16 bytes.
+template <int LE>
+__device__ void make_record3(const ViewGeom* __restrict__ gp, float vs, const mr_task& T,
+                             const double* xt, int R, int C, int pitch, bool cone,
+                             float* __restrict__ rec) {
+    RecRef q;
+    make_record(gp, vs, T, xt, R, C, pitch, cone, rec, &q);
+    // corner images of the tile, relative to the reference, then absolute
+    double cmin = 1e300, cmax = -1e300, rmin = 1e300, rmax = -1e300;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double ds = (k & 1) ? BP_TS - 1 : 0, dp = (k & 2) ? BP_TP - 1 : 0;
+        const double w = q.W0 + ds * q.Wa + dp * q.Wb;
+        const double col = (q.U0 + ds * q.Ua + dp * q.Ub) / w + q.cref;
+        const double row = (q.V0 + ds * q.Va + dp * q.Vb) / w + q.rref;
+        cmin = fmin(cmin, col); cmax = fmax(cmax, col);
+        rmin = fmin(rmin, row); rmax = fmax(rmax, row);
+    }
+    // absolute range of the taps after the inner loop's clamp, one pixel margin
+    const double c_lo = fmax(floor(fmin(fmax(cmin, -1.0), (double)C)) - 1.0, -1.0);
+    const double c_hi = fmin(floor(fmin(fmax(cmax, -1.0), (double)C)) + 2.0, C + 1.0);
+    const double r_lo = fmax(floor(fmin(fmax(rmin, -1.0), (double)R)) - 1.0, -1.0);
+    const double r_hi = fmin(floor(fmin(fmax(rmax, -1.0), (double)R)) + 2.0, R + 1.0);
+    constexpr int ALIGN = 16 >> LE;                       // elements per 16 bytes
+    const int pc0 = (static_cast<int>(c_lo) + 1) & ~(ALIGN - 1);   // padded coordinates
+    const int pc1 = static_cast<int>(c_hi) + 1;
+    const int pr0 = static_cast<int>(r_lo) + 1;
+    const int pr1 = static_cast<int>(r_hi) + 1;
+    int nc = (pc1 - pc0 + ALIGN) & ~(ALIGN - 1);
+    int nr = pr1 - pr0 + 1;
+    const bool finite = cmin == cmin && cmax == cmax && rmin == rmin && rmax == rmax &&
+                        fabs(cmin) < 1e9 && fabs(cmax) < 1e9 && fabs(rmin) < 1e9 && fabs(rmax) < 1e9;
+    const int stride = nc + ALIGN;    // keeps 16-byte rows and shifts LDS banks per row
+    if (!finite || nr <= 0 || nc <= 0 || nr * stride * (1 << LE) > V3_STAGE_BYTES) nr = nc = 0;
+    int* ri = reinterpret_cast<int*>(rec);
+    ri[16] = pr0 - (static_cast<int>(q.rref) + 1);     // box row 0 relative to rref
+    ri[17] = pc0 - (static_cast<int>(q.cref) + 1);     // box col 0 relative to cref
+    ri[18] = nr;
+    ri[19] = nc;
+    ri[20] = pr0 * pitch + pc0;                        // element offset of the box origin
+    ri[21] = stride;
+    ri[22] = 0;
+    ri[23] = 0;
+}
+
+template <typename ST> struct Lds;
+template <> struct Lds<float> {
+    __device__ static inline void taps(const float* __restrict__ st, int i, int stride, float& a,
+                                       float& b, float& c, float& d) {
+        a = st[i]; b = st[i + 1]; c = st[i + stride]; d = st[i + stride + 1];
+    }
+};
+template <> struct Lds<__half> {
+    __device__ static inline void taps(const __half* __restrict__ st, int i, int stride, float& a,
+                                       float& b, float& c, float& d) {
+        a = __half2float(st[i]); b = __half2float(st[i + 1]);
+        c = __half2float(st[i + stride]); d = __half2float(st[i + stride + 1]);
+    }
+};
+template <> struct Lds<uint16_t> {
+    __device__ static inline void taps(const uint16_t* __restrict__ st, int i, int stride, float& a,
+                                       float& b, float& c, float& d) {
+        a = __uint_as_float(static_cast<uint32_t>(st[i]) << 16);
+        b = __uint_as_float(static_cast<uint32_t>(st[i + 1]) << 16);
+        c = __uint_as_float(static_cast<uint32_t>(st[i + stride]) << 16);
+        d = __uint_as_float(static_cast<uint32_t>(st[i + stride + 1]) << 16);
+    }
+};
+
+template <bool CONE, typename ST>
+__global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel3(
+    const ST* __restrict__ ring, int64_t view_stride, int pitch, int R, int C,
+    const ViewGeom* __restrict__ geom, const float* __restrict__ vscale,
+    const mr_task* __restrict__ tasks, const Tile* __restrict__ tiles, int n_views,
+    float scale, float* __restrict__ out) {
+    __shared__ float4 rec4[V3_VPASS * V3_REC / 4];
+    __shared__ float4 stage4[V3_STAGE_BYTES / 16];
+    float* rec = reinterpret_cast<float*>(rec4);
+    const ST* stage = reinterpret_cast<const ST*>(stage4);
+    constexpr int LE = Taps<ST>::LOG_ESZ;
+    constexpr int ALIGN = 16 >> LE;
+
+    const Tile tl = tiles[blockIdx.x];
+    const mr_task& T = tasks[tl.task];
+    const int t = threadIdx.x;
+    const int tx = t & (BP_TS - 1);
+    const int ty = t / BP_TS;
+    constexpr int PSTEP = BP_THREADS / BP_TS;
+    constexpr int PPT = BP_TP / PSTEP;
+
+    const int p0 = tl.tp * BP_TP, s0 = tl.ts * BP_TS;
+    double xt[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        xt[d] = T.o[d] + tl.q * T.c[d] + p0 * T.a[d] + s0 * T.b[d];
+
+    const int s = s0 + tx;
+    bool valid[PPT], inside[PPT];
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) {
+        const int p = p0 + ty + PSTEP * m;
+        valid[m] = (p < T.np) && (s < T.ns);
+        inside[m] = valid[m] && (!T.mask_box || in_box(T, p, s));
+    }
+
+    const float ds = static_cast<float>(tx);
+    const float dp0 = static_cast<float>(ty);
+    const int pitchb = pitch << LE;
+    const int view_bytes = static_cast<int>(view_stride << LE);
+    float acc[PPT], accb[PPT], fr_par[PPT];
+    int rrow_par[PPT];
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) { acc[m] = 0.0f; accb[m] = 0.0f; fr_par[m] = 0.0f; rrow_par[m] = 0; }
+
+    for (int vb = 0; vb < n_views; vb += V3_VPASS) {
+        const int nv = min(V3_VPASS, n_views - vb);
+        __syncthreads();
+        if (t < nv)
+            make_record3<LE>(geom + vb + t, vscale[vb + t], T, xt, R, C, pitch, CONE,
+                             rec + t * V3_REC);
+        __syncthreads();
+        if (!CONE && vb == 0) {
+            const float4 r0 = rec4[0], r1 = rec4[1], r2 = rec4[2], r3 = rec4[3];
+#pragma unroll
+            for (int m = 0; m < PPT; ++m) {
+                const float dp = dp0 + static_cast<float>(PSTEP * m);
+                float row = fmaf(dp, r1.y, fmaf(ds, r1.x, r0.w));
+                row = __builtin_amdgcn_fmed3f(row, r2.w, r3.x);
+                fr_par[m] = __builtin_amdgcn_fractf(row);
+                rrow_par[m] = cvt_flr(row);
+            }
+        }
+        for (int v = 0; v < nv; ++v) {
+            const float* rv = rec + v * V3_REC;
+            const float4 r0 = rec4[v * (V3_REC / 4) + 0];
+            const float4 r1 = rec4[v * (V3_REC / 4) + 1];
+            const float4 r2 = rec4[v * (V3_REC / 4) + 2];
+            const float4 r3 = rec4[v * (V3_REC / 4) + 3];
+            const int4 bx = *reinterpret_cast<const int4*>(rv + 16);   // sr0, sc0, nr, nc
+            const int4 by = *reinterpret_cast<const int4*>(rv + 20);   // gofs, stride
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<ST*>(ring + static_cast<int64_t>(vb + v) * view_stride), 0, view_bytes,
+                0x00020000);
+            const bool staged = bx.z > 0;
+            if (staged) {
+                // stage the box: chunks of 16 bytes, rows of nc elements
+                const int cpr = bx.w / ALIGN;                 // chunks per row
+                const int total = bx.z * cpr;
+                const float inv = 1.0f / static_cast<float>(cpr);
+                __syncthreads();
+                for (int e = t; e < total; e += BP_THREADS) {
+                    const int r = static_cast<int>((static_cast<float>(e) + 0.5f) * inv);
+                    const int cc = e - r * cpr;
+                    const int goff = (by.x + r * pitch + cc * ALIGN) << LE;
+                    const float4 val = __builtin_bit_cast(
+                        float4, __builtin_amdgcn_raw_buffer_load_b128(rs, goff, 0, 0));
+                    stage4[(r * by.y + cc * ALIGN) / ALIGN] = val;
+                }
+                __syncthreads();
+            }
+            const int baseb = __float_as_int(r3.z) << LE;
+            if (CONE) {
+                const float uh0 = fmaf(dp0, r0.z, fmaf(ds, r0.y, r0.x));
+                const float vh0 = fmaf(dp0, r1.y, fmaf(ds, r1.x, r0.w));
+                const float wh0 = fmaf(dp0, r2.x, fmaf(ds, r1.w, r1.z));
+#pragma unroll
+                for (int m = 0; m < PPT; ++m) {
+                    const float dm = static_cast<float>(PSTEP * m);
+                    const float uh = m ? fmaf(dm, r0.z, uh0) : uh0;
+                    const float vh = m ? fmaf(dm, r1.y, vh0) : vh0;
+                    const float wh = m ? fmaf(dm, r2.x, wh0) : wh0;
+                    const float rw = __builtin_amdgcn_rcpf(wh);
+                    const float col = __builtin_amdgcn_fmed3f(uh * rw, r2.y, r2.z);
+                    const float row = __builtin_amdgcn_fmed3f(vh * rw, r2.w, r3.x);
+                    const float wt = rw * rw;
+                    const float fc = __builtin_amdgcn_fractf(col);
+                    const float fr = __builtin_amdgcn_fractf(row);
+                    const int ci = cvt_flr(col), ri = cvt_flr(row);
+                    float a, b, c, d;
+                    if (staged) {
+                        const int li = __mul24(ri - bx.x, by.y) + (ci - bx.y);
+                        Lds<ST>::taps(stage, li, by.y, a, b, c, d);
+                    } else {
+                        const int voff = __mul24(ri, pitchb) + ((ci << LE) + baseb);
+                        Taps<ST>::load(rs, voff, pitchb, a, b, c, d);
+                    }
+                    const float top = fmaf(fc, b - a, a);
+                    const float bot = fmaf(fc, d - c, c);
+                    acc[m] = fmaf(wt, fmaf(fr, bot - top, top), acc[m]);
+                }
+            } else {
+                const float vs = r3.y;
+                const float uh0 = fmaf(dp0, r0.z, fmaf(ds, r0.y, r0.x));
+#pragma unroll
+                for (int m = 0; m < PPT; ++m) {
+                    const float dm = static_cast<float>(PSTEP * m);
+                    const float col =
+                        __builtin_amdgcn_fmed3f(m ? fmaf(dm, r0.z, uh0) : uh0, r2.y, r2.z);
+                    const float fc = __builtin_amdgcn_fractf(col);
+                    const int ci = cvt_flr(col);
+                    float a, b, c, d;
+                    if (staged) {
+                        const int li = __mul24(rrow_par[m] - bx.x, by.y) + (ci - bx.y);
+                        Lds<ST>::taps(stage, li, by.y, a, b, c, d);
+                    } else {
+                        const int voff = __mul24(rrow_par[m], pitchb) + ((ci << LE) + baseb);
+                        Taps<ST>::load(rs, voff, pitchb, a, b, c, d);
+                    }
+                    acc[m] = fmaf(vs, fmaf(fc, b - a, a), acc[m]);
+                    accb[m] = fmaf(vs, fmaf(fc, d - c, c), accb[m]);
+                }
+            }
+        }
+    }
+
+    const int64_t obase = T.out_offset + static_cast<int64_t>(tl.q) * T.stride_q;
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) {
+        if (!valid[m]) continue;
+        const int p = p0 + ty + PSTEP * m;
+        float val = acc[m];
+        if (!CONE) val = fmaf(fr_par[m], accb[m] - acc[m], acc[m]);
+        out[obase + static_cast<int64_t>(p) * T.stride_p + static_cast<int64_t>(s) * T.stride_s] =
+            inside[m] ? val * scale : 0.0f;
+    }
+}
+
 // Point-list back-projection with float64 coordinates (reconstruct_points): each
 // thread owns one point; per view the projective form is evaluated in FP64, the
 // fractional parts and FDK weight are rounded to FP32 for the interpolation.
@@ -298,6 +720,50 @@ template <bool CONE>
 int dispatch_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles, int n_views,
                    float scale, float* d_out, hipStream_t s) {
     dim3 grid(ntiles), block(BP_THREADS);
+    if (c->bp_variant == 3) {
+        switch (c->storage) {
+        case MR_STORE_F32:
+            hipLaunchKernelGGL((bp_tiles_kernel3<CONE, float>), grid, block, 0, s,
+                               static_cast<const float*>(c->ring), c->view_stride, c->pitch, c->R,
+                               c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out);
+            break;
+        case MR_STORE_F16:
+            hipLaunchKernelGGL((bp_tiles_kernel3<CONE, __half>), grid, block, 0, s,
+                               static_cast<const __half*>(c->ring), c->view_stride, c->pitch, c->R,
+                               c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out);
+            break;
+        default:
+            hipLaunchKernelGGL((bp_tiles_kernel3<CONE, uint16_t>), grid, block, 0, s,
+                               static_cast<const uint16_t*>(c->ring), c->view_stride, c->pitch,
+                               c->R, c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale,
+                               d_out);
+            break;
+        }
+        MR_HIP(hipGetLastError());
+        return MR_OK;
+    }
+    if (c->bp_variant == 2) {
+        switch (c->storage) {
+        case MR_STORE_F32:
+            hipLaunchKernelGGL((bp_tiles_kernel2<CONE, float>), grid, block, 0, s,
+                               static_cast<const float*>(c->ring), c->view_stride, c->pitch, c->R,
+                               c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out);
+            break;
+        case MR_STORE_F16:
+            hipLaunchKernelGGL((bp_tiles_kernel2<CONE, __half>), grid, block, 0, s,
+                               static_cast<const __half*>(c->ring), c->view_stride, c->pitch, c->R,
+                               c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out);
+            break;
+        default:
+            hipLaunchKernelGGL((bp_tiles_kernel2<CONE, uint16_t>), grid, block, 0, s,
+                               static_cast<const uint16_t*>(c->ring), c->view_stride, c->pitch,
+                               c->R, c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale,
+                               d_out);
+            break;
+        }
+        MR_HIP(hipGetLastError());
+        return MR_OK;
+    }
     switch (c->storage) {
     case MR_STORE_F32:
         hipLaunchKernelGGL((bp_tiles_kernel<CONE, float>), grid, block, 0, s,

@@ -76,6 +76,7 @@ struct Ctx {
     hipStream_t s_ingest = nullptr, s_req = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_kernel_ms = 0.0;
+    int bp_variant = 1;              // back-projection inner loop (MINIRECAST_BP_VARIANT)
 
     int kind = MR_KIND_NONE;
     int n_total = 0, R = 0, C = 0;
@@ -90,6 +91,7 @@ struct Ctx {
     std::vector<FilterView> filt_all; // per geometry index (slot filled at push)
     std::vector<uint8_t> received;
     double pixel_u = 1.0, pixel_v = 1.0;
+    double gaussian = 0.0, tilt = 0.0;  // preprocessing applied at ingest
 
     // device buffers
     void* ring = nullptr;            // [n_total][Rp][pitch] storage elements
@@ -99,6 +101,8 @@ struct Ctx {
     float2* d_tw = nullptr;          // [N] twiddles exp(-2 pi i k / N)
     float* d_stage = nullptr;        // raw block staging
     size_t stage_bytes = 0;
+    float* d_pre = nullptr;          // preprocessing ping-pong buffer
+    size_t pre_bytes = 0;
     float* d_fstage = nullptr;       // f32 filtered staging (f16 storage)
     size_t fstage_bytes = 0;
     unsigned* d_vmax = nullptr;      // per-batch per-view max |q| (float bits)
@@ -125,6 +129,8 @@ int ensure_host(void** p, size_t* have, size_t need);
 // ---------------------------------------------------------------- kernels (launchers)
 int launch_filter(Ctx* c, const float* d_block, int k, int n0, hipStream_t s);
 int launch_store(Ctx* c, const float* d_q, int k, hipStream_t s);
+int launch_preprocess(Ctx* c, float* d_a, float* d_b, int R, int k, int C, double gaussian,
+                      double tilt, float** d_res, hipStream_t s);
 int launch_filter_rows_plain(Ctx* c, const float* d_in, int rows, int n, float gain_tau,
                              const float* d_H, const float2* d_tw, int N, float* d_out,
                              hipStream_t s);

@@ -33,6 +33,9 @@ SIGNATURES = [
      [_P, _c.c_int, _c.c_int, _c.c_int, _P, _c.c_double, _c.c_double, _c.c_int, _c.c_int]),
     ("mr_push_views", _c.c_int, [_P, _P, _c.c_int, _P, _c.c_int, _c.POINTER(_c.c_int)]),
     ("mr_store_views", _c.c_int, [_P, _P, _c.c_int, _P, _c.c_int, _c.POINTER(_c.c_int)]),
+    ("mr_set_preprocess", _c.c_int, [_P, _c.c_double, _c.c_double]),
+    ("mr_preprocess", _c.c_int,
+     [_P, _P, _c.c_int, _c.c_int, _c.c_int, _c.c_double, _c.c_double, _P]),
     ("mr_num_views", _c.c_int, [_P, _c.POINTER(_c.c_int)]),
     ("mr_bp_points", _c.c_int, [_P, _P, _c.c_int64, _c.c_double, _c.c_int, _P]),
     ("mr_bp_tasks", _c.c_int, [_P, _P, _c.c_int, _c.c_int, _P, _c.c_int64, _c.c_int]),
@@ -190,6 +193,17 @@ class Context:
         check(self.lib.mr_store_views(self.h, ptr(idx), len(idx), ctypes.c_void_p(dptr), 1,
                                       ctypes.byref(n)))
         return n.value
+
+    def set_preprocess(self, gaussian, tilt):
+        check(self.lib.mr_set_preprocess(self.h, float(gaussian), float(tilt)))
+
+    def preprocess(self, block, gaussian, tilt):
+        b = np.ascontiguousarray(block, dtype=np.float32)
+        R, k, C = b.shape
+        out = np.empty_like(b)
+        check(self.lib.mr_preprocess(self.h, ptr(b), R, k, C, float(gaussian), float(tilt),
+                                     ptr(out)))
+        return out
 
     def num_views(self):
         n = ctypes.c_int(0)

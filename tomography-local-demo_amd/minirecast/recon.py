@@ -17,7 +17,6 @@ import time
 import numpy as np
 
 from . import _hip
-from . import preprocess
 from .geometry import ConeVecGeometry, ParallelGeometry
 
 # The reference's CPU JIT backends are not part of this build.
@@ -294,6 +293,7 @@ class SliceReconstructor:
         self._received = np.zeros(self.n_total, dtype=bool)
         self.n = 0
         self._ctx = _hip.Context(device)
+        self._ctx.set_preprocess(self.gaussian, self.tilt)
         if isinstance(geom, ParallelGeometry):
             self.kind = "parallel"
             self._ctx.set_parallel(geom.angles, geom.rows, geom.cols, geom.pixel_u,
@@ -330,7 +330,7 @@ class SliceReconstructor:
             indices, block = indices[keep], block[:, keep, :]
             if len(indices) == 0:
                 return self.n
-            block = preprocess.apply(block, gaussian=self.gaussian, tilt=self.tilt)
+            # preprocess.apply (tilt, Gaussian) runs on the device before filtering
             n1 = self._ctx.push(indices, block)
             with self._lock:
                 n0 = self.n
