@@ -76,6 +76,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--views", type=int, default=None, help="debug: fewer views")
     ap.add_argument("--forms", default="f32,f16,bf16", help="storage forms to time")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run a 1 GiB device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
@@ -84,6 +86,14 @@ def main():
         sys.exit(2)
 
     from minirecast import _hip, phantom, recon, workloads
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("calibration copy: 1 GiB read + 1 GiB written")
+        del a, b
     w = workloads.full_cone()
     geom = w.geom
     if args.views:

@@ -145,6 +145,32 @@ int mr_bp_tasks(mr_ctx* ctx, const mr_task* tasks, int ntasks, int n_views,
  * scores: host [ntasks] float64. */
 int mr_cor_scores(mr_ctx* ctx, const mr_task* tasks, int ntasks, int n_views, double* scores);
 
+/* ---- differentiation (no counterpart in the reference: the transposes of the maps above)
+ *
+ * Adjoint of mr_bp_tasks with respect to the stored views: scatters the task-output
+ * weights g (laid out as mr_bp_tasks' output, g_len elements) with the forward's
+ * bilinear and FDK distance weights and scale pi / n_views into gq [n_views][rows][cols]
+ * float32 (zeroed by the call).  Storage scales are not applied: the gradient is with
+ * respect to the dequantised views. */
+int mr_bp_tasks_adjoint(mr_ctx* ctx, const mr_task* tasks, int ntasks, int n_views,
+                        const float* g, int64_t g_len, int g_on_device, float* gq,
+                        int gq_on_device);
+
+/* Derivative of every mr_bp_tasks output element with respect to its task's
+ * cor_shift (recon.py:412-424: the detector moved to D - s*u), same layout as out. */
+int mr_bp_tasks_dshift(mr_ctx* ctx, const mr_task* tasks, int ntasks, int n_views, float* out,
+                       int64_t out_len, int out_on_device);
+
+/* Adjoint of the ingest filter (fdk_filter_block, recon.py:85-104) for views with
+ * geometry `indices`: graw [rows][k][cols] = w * F(gq) for gq [k][rows][cols]; the
+ * zero-padded ramp filter F is a real even circular convolution and its own transpose,
+ * w is the FDK cosine weight (1 for parallel beam). */
+int mr_filter_adjoint(mr_ctx* ctx, const int64_t* indices, int k, const float* gq,
+                      int gq_on_device, float* graw, int graw_on_device);
+
+/* Forgets the stored views (n = 0) so that the ring can be refilled. */
+int mr_reset_views(mr_ctx* ctx);
+
 /* Parity hook for filter_rows (recon.py:58-64): ramp-filters `rows` host rows of
  * length n (float32) with pitch tau on the context's device. */
 int mr_filter_rows(mr_ctx* ctx, const float* in, int rows, int n, double tau,

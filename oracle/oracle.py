@@ -33,6 +33,9 @@ How each function is pinned (tests/test_oracle.py):
   arrival_order        known answer [0,8,4,2,6,1,9,5,3,7].
   gaussian_blur / rotate_projections   selftest.py:104-117 invariants.
   line_integrals       closed-form chord of a sphere.
+  adjoints            dot-product tests <A x, y> = <x, A^T y> against the forward
+                       maps and central finite differences for the shift derivative
+                       (tests/test_oracle.py).
   OracleReconstructor  the reference's own outputs on its analytic phantom
                        (tests/golden/reference_composition.npz): bit-exact for
                        cone beam, 1e-6 for parallel beam (the reference filters
@@ -226,6 +229,83 @@ def bp_cone_scalar(q, S, D, UU, VV, N, SDD, SOD, pts, scale):
                         v += wr * wc * float(q[k, rr_, cc_])
             acc += w * w * v
         out[i] = acc * scale
+    return out
+
+
+# --------------------------------------------------------------------------
+# Adjoints (the transposes of the linear maps above), float64
+# --------------------------------------------------------------------------
+def bilinear_adjoint(g, r, c, R, C):
+    """Transpose of ``bilinear``: scatters g with the same per-tap weights into [R, C];
+    taps outside the image are dropped."""
+    r = np.asarray(r, np.float64)
+    c = np.asarray(c, np.float64)
+    r0 = np.floor(r).astype(np.int64)
+    c0 = np.floor(c).astype(np.int64)
+    fr, fc = r - r0, c - c0
+    out = np.zeros((R, C))
+    for dr, wr in ((0, 1.0 - fr), (1, fr)):
+        for dc, wc in ((0, 1.0 - fc), (1, fc)):
+            rr, cc = r0 + dr, c0 + dc
+            ok = (rr >= 0) & (rr < R) & (cc >= 0) & (cc < C)
+            np.add.at(out, (rr[ok], cc[ok]), (wr * wc * g)[ok])
+    return out
+
+
+def bp_cone_adjoint(g, S, D, UU, VV, N, SDD, SOD, pts, scale, R, C):
+    """Transpose of bp_cone with respect to q: [A, R, C] from point weights g [M]."""
+    pts = np.asarray(pts, np.float64).reshape(-1, 3)
+    A = len(S)
+    out = np.zeros((A, R, C))
+    for k in range(A):
+        row, col, w = cone_coords(S[k], D[k], UU[k], VV[k], N[k], SDD[k], SOD[k], pts, R, C)
+        out[k] = bilinear_adjoint(scale * w * g, row, col, R, C)
+    return out
+
+
+def bilinear_grad(img, r, c):
+    """(d/dr, d/dc) of ``bilinear`` inside a cell (piecewise-linear interpolant)."""
+    R, C = img.shape
+    r0 = np.floor(r).astype(np.int64)
+    c0 = np.floor(c).astype(np.int64)
+    fr, fc = r - r0, c - c0
+
+    def tap(dr, dc):
+        rr, cc = r0 + dr, c0 + dc
+        ok = (rr >= 0) & (rr < R) & (cc >= 0) & (cc < C)
+        v = np.zeros(r.shape)
+        v[ok] = img[rr[ok], cc[ok]]
+        return v
+    a, b, cq, d = tap(0, 0), tap(0, 1), tap(1, 0), tap(1, 1)
+    ddc = (1 - fr) * (b - a) + fr * (d - cq)
+    ddr = (1 - fc) * (cq - a) + fc * (d - b)
+    return ddr, ddc
+
+
+def bp_cone_dshift(q, S, D, U, UU, VV, N, SDD, SOD, pts, scale, shift):
+    """d/ds of bp_cone with D -> D - s*u at s = shift (recon.py:420), per point."""
+    q = np.asarray(q)
+    A, R, C = q.shape
+    pts = np.asarray(pts, np.float64).reshape(-1, 3)
+    acc = np.zeros(len(pts))
+    for k in range(A):
+        Dk = D[k] - shift * U[k]
+        row, col, w = cone_coords(S[k], Dk, UU[k], VV[k], N[k], SDD[k], SOD[k], pts, R, C)
+        ddr, ddc = bilinear_grad(q[k], row, col)
+        acc += w * (ddc * (U[k] @ UU[k]) + ddr * (U[k] @ VV[k]))
+    return acc * scale
+
+
+def fdk_filter_adjoint(gq, K, window="shepp-logan"):
+    """Transpose of fdk_filter_block: gq [k, R, C] -> [R, k, C].  The zero-padded
+    ramp filter is a real even circular convolution cropped to n samples, so it is
+    its own transpose; the cosine weight is diagonal."""
+    gq = np.asarray(gq, np.float64)
+    A, R, C = gq.shape
+    out = np.empty((R, A, C))
+    for k in range(A):
+        w, tau_iso = fdk_weights(K, k, R, C)
+        out[:, k, :] = w * filter_rows(gq[k], tau_iso, window)
     return out
 
 

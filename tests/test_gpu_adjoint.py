@@ -1,0 +1,141 @@
+"""GPU tests of the differentiable path: the hand-written adjoints against the forward
+maps (dot-product tests), against the float64 oracle's adjoints, and the shift
+derivative against central finite differences; then autograd end to end."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def setup():
+    from minirecast import _hip, recon
+    from minirecast.geometry import circular_cone_vectors
+    g = circular_cone_vectors(24, 400.0, 600.0, 20, 32, 0.6, det_shift_u=0.3)
+    box = g.volume_box()
+    planes = [dict(origin=(0.0, 0.0, 0.0), normal=(1.0, 0.5, 0.8), n=40),
+              dict(origin=(0.3, -0.2, 0.5), normal=(-0.3, 1.0, 0.4), n=33)]
+    tasks, off = [], 0
+    for p in planes:
+        tasks.append(recon.plane_task(box, p['origin'], p['normal'], p['n'], out_offset=off))
+        off += p['n'] ** 2
+    tasks = np.array(tasks, dtype=_hip.TASK_DTYPE)
+    ctx = _hip.Context(0)
+    ctx.set_cone(g.vectors, g.rows, g.cols, "shepp-logan")
+    return g, box, planes, tasks, off, ctx
+
+
+def _plane_points(box, planes):
+    from minirecast import recon
+    pts, masks = [], []
+    for p in planes:
+        pp, m = recon.slice_points(box, p['origin'], p['normal'], p['n'])
+        pts.append(pp.reshape(-1, 3))
+        masks.append(m.ravel())
+    return np.concatenate(pts), np.concatenate(masks)
+
+
+def test_bp_adjoint_dot_product_and_oracle(setup):
+    g, box, planes, tasks, out_len, ctx = setup
+    rng = np.random.default_rng(0)
+    A, R, C = g.n_angles, g.rows, g.cols
+    q = rng.normal(size=(A, R, C)).astype(np.float32)
+    ctx.reset_views()
+    ctx.store(np.arange(A), q)
+    y = ctx.bp_tasks(tasks, A, out_len)
+    w = rng.normal(size=out_len).astype(np.float32)
+    gq = ctx.bp_tasks_adjoint(tasks, A, w, R, C)
+    lhs = float(np.dot(y.astype(np.float64), w))
+    rhs = float(np.sum(q.astype(np.float64) * gq))
+    assert abs(lhs - rhs) <= 1e-5 * np.sqrt(np.sum(y.astype(float) ** 2) * np.sum(w.astype(float) ** 2))
+    # float64 oracle adjoint on the masked points
+    pts, mask = _plane_points(box, planes)
+    K = O.cone_constants(g.vectors)
+    want = O.bp_cone_adjoint(np.where(mask, w, 0.0).astype(np.float64)[mask], K['s'], K['d'],
+                             K['uu'], K['vv'], K['n'], K['sdd'], K['sod'], pts[mask], np.pi / A,
+                             R, C)
+    assert rel_l2(gq, want) < 1e-5
+
+
+def test_filter_adjoint_dot_product_and_oracle(setup):
+    g, box, planes, tasks, out_len, ctx = setup
+    rng = np.random.default_rng(1)
+    A, R, C = g.n_angles, g.rows, g.cols
+    x = rng.normal(size=(R, A, C)).astype(np.float32)
+    ctx.reset_views()
+    ctx.push(np.arange(A), x)
+    fx = ctx.read_views(0, A, R, C)                      # F(W x): [A, R, C]
+    z = rng.normal(size=(A, R, C)).astype(np.float32)
+    ftz = ctx.filter_adjoint(np.arange(A), z)            # W F(z): [R, A, C]
+    lhs = float(np.sum(fx.astype(np.float64) * z))
+    rhs = float(np.sum(x.astype(np.float64) * ftz))
+    assert abs(lhs - rhs) <= 1e-5 * np.linalg.norm(fx) * np.linalg.norm(z)
+    K = O.cone_constants(g.vectors)
+    assert rel_l2(ftz, O.fdk_filter_adjoint(z, K, "shepp-logan")) < 1e-5
+
+
+def test_dshift_vs_oracle_and_finite_difference(setup):
+    from minirecast import phantom
+    g, box, planes, tasks, out_len, ctx = setup
+    A, R, C = g.n_angles, g.rows, g.cols
+    ph = phantom.random_ellipsoids(3, 6, box, (0.2, 1.0))
+    proj = phantom.project_cone(ph, g)
+    ctx.reset_views()
+    ctx.push(np.arange(A), proj)
+    s0 = 0.37
+    T = tasks.copy()
+    T['cor_shift'] = s0
+    d = ctx.bp_tasks_dshift(T, A, out_len)
+    q = ctx.read_views(0, A, R, C).astype(np.float64)
+    pts, mask = _plane_points(box, planes)
+    K = O.cone_constants(g.vectors)
+    want = np.zeros(out_len)
+    want[mask] = O.bp_cone_dshift(q, K['s'], K['d'], K['u'], K['uu'], K['vv'], K['n'], K['sdd'],
+                                  K['sod'], pts[mask], np.pi / A, s0)
+    assert rel_l2(d, want) < 1e-4
+    # central finite difference of a weighted loss, h = 1e-3 px
+    wts = np.random.default_rng(2).normal(size=out_len)
+    h = 1e-3
+
+    def loss(sft):
+        Tx = tasks.copy()
+        Tx['cor_shift'] = sft
+        return float(ctx.bp_tasks(Tx, A, out_len).astype(np.float64) @ wts)
+    fd = (loss(s0 + h) - loss(s0 - h)) / (2 * h)
+    an = float(d.astype(np.float64) @ wts)
+    assert abs(fd - an) <= 2e-3 * abs(an) + 1e-6
+
+
+def test_autograd_end_to_end(setup):
+    import torch
+    from minirecast import autograd, phantom
+    g, box, planes, tasks, out_len, _ = setup
+    op = autograd.FDKSlices(g, planes, "shepp-logan")
+    ph = phantom.random_ellipsoids(4, 6, box, (0.2, 1.0))
+    proj_np = phantom.project_cone(ph, g)
+    proj = torch.tensor(proj_np, device="cuda", requires_grad=True)
+    shift = torch.tensor(0.25, device="cuda", requires_grad=True)
+    wts = torch.tensor(np.random.default_rng(3).normal(size=out_len), dtype=torch.float32,
+                       device="cuda")
+    y = autograd.fdk_slices(proj, shift, op)
+    loss = (y * wts).sum()
+    loss.backward()
+    # float64 reference gradient with respect to the raw projections
+    A, R, C = g.n_angles, g.rows, g.cols
+    pts, mask = _plane_points(box, planes)
+    K = O.cone_constants(g.vectors)
+    D = K['d'] - 0.25 * K['u']
+    gq = O.bp_cone_adjoint(wts.cpu().numpy().astype(np.float64)[mask], K['s'], D, K['uu'], K['vv'],
+                           K['n'], K['sdd'], K['sod'], pts[mask], np.pi / A, R, C)
+    gref = O.fdk_filter_adjoint(gq, K, "shepp-logan")
+    assert rel_l2(proj.grad.cpu().numpy(), gref) < 1e-5
+    # shift gradient against a central difference through the forward
+    with torch.no_grad():
+        h = 1e-3
+        lp = float((autograd.fdk_slices(proj, 0.25 + h, op) * wts).sum())
+        lm = float((autograd.fdk_slices(proj, 0.25 - h, op) * wts).sum())
+    fd = (lp - lm) / (2 * h)
+    assert abs(float(shift.grad) - fd) <= 2e-3 * abs(fd) + 1e-5

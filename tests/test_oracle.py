@@ -337,3 +337,38 @@ def test_oracle_stream_order_matches_reference(golden):
         live.add_projections(idx, golden['cone_proj'][:, idx, :])
     img, _ = live.reconstruct_plane((0, 0, 0), (0.2, 0.1, 1.0), n=32)
     assert rel_l2(img, golden['cone_stream_plane']) < 1e-12
+
+
+# ---------------------------------------------------------------- adjoints
+def test_bp_cone_adjoint_dot_product():
+    K, q, pts = _tiny_cone(7, A=4, R=7, C=11)
+    g = np.random.default_rng(8).normal(size=len(pts))
+    args = (K['s'], K['d'], K['uu'], K['vv'], K['n'], K['sdd'], K['sod'], pts, 0.7)
+    lhs = O.bp_cone(q, *args) @ g
+    rhs = np.sum(q * O.bp_cone_adjoint(g, *args, 7, 11))
+    assert lhs == pytest.approx(rhs, rel=1e-12)
+
+
+def test_fdk_filter_adjoint_dot_product():
+    rng = np.random.default_rng(9)
+    V = O.circular_cone_vectors(3, 400.0, 600.0, 0.3, det_shift_u=0.4)
+    K = O.cone_constants(V)
+    x = rng.normal(size=(6, 3, 13))
+    z = rng.normal(size=(3, 6, 13))
+    for window in O.WINDOWS:
+        lhs = np.sum(O.fdk_filter_block(x, K, window) * z)
+        rhs = np.sum(x * O.fdk_filter_adjoint(z, K, window))
+        assert lhs == pytest.approx(rhs, rel=1e-12)
+
+
+def test_bp_cone_dshift_matches_finite_difference():
+    K, q, pts = _tiny_cone(10, A=5, R=8, C=14)
+    pts = pts * 0.5
+    base = (K['s'], K['d'], K['u'], K['uu'], K['vv'], K['n'], K['sdd'], K['sod'], pts, 0.3)
+    s0, h = 0.31, 1e-6
+
+    def f(sft):
+        D = K['d'] - sft * K['u']
+        return O.bp_cone(q, K['s'], D, K['uu'], K['vv'], K['n'], K['sdd'], K['sod'], pts, 0.3)
+    fd = (f(s0 + h) - f(s0 - h)) / (2 * h)
+    np.testing.assert_allclose(O.bp_cone_dshift(q, *base, s0), fd, rtol=1e-5, atol=1e-7)

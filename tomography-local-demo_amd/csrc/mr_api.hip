@@ -184,8 +184,11 @@ void build_tiles(const mr_task* tasks, int ntasks, std::vector<Tile>& tiles) {
     }
 }
 
-// Uploads tasks + tiles and launches the tile kernel into d_out (device), timing it.
-int run_tasks(Ctx* c, const mr_task* tasks, int ntasks, int n_views, float* d_out) {
+enum TaskOp { OP_BP = 0, OP_ADJOINT = 1, OP_DSHIFT = 2 };
+
+// Uploads tasks + tiles and launches the tile kernel of `op` (device buffers), timing it.
+int run_tasks(Ctx* c, const mr_task* tasks, int ntasks, int n_views, float* d_out,
+              int op = OP_BP, const float* d_g = nullptr) {
     std::vector<Tile> tiles;
     build_tiles(tasks, ntasks, tiles);
     const size_t tb = sizeof(mr_task) * ntasks;
@@ -203,8 +206,13 @@ int run_tasks(Ctx* c, const mr_task* tasks, int ntasks, int n_views, float* d_ou
     const Tile* d_tiles = reinterpret_cast<const Tile*>(static_cast<char*>(c->d_req) + off_tiles);
     const float scale = n_views > 0 ? static_cast<float>(M_PI / n_views) : 0.0f;
     MR_HIP(hipEventRecord(c->ev0, c->s_req));
-    rc = launch_bp_tiles(c, d_tasks, d_tiles, static_cast<int>(tiles.size()), n_views, scale, d_out,
-                         c->s_req);
+    const int nt = static_cast<int>(tiles.size());
+    if (op == OP_BP)
+        rc = launch_bp_tiles(c, d_tasks, d_tiles, nt, n_views, scale, d_out, c->s_req);
+    else if (op == OP_ADJOINT)
+        rc = launch_bp_adjoint(c, d_tasks, d_tiles, nt, n_views, scale, d_g, d_out, c->s_req);
+    else
+        rc = launch_bp_dshift(c, d_tasks, d_tiles, nt, n_views, scale, d_out, c->s_req);
     if (rc) return rc;
     MR_HIP(hipEventRecord(c->ev1, c->s_req));
     return MR_OK;
@@ -639,6 +647,118 @@ int mr_bp_tasks(mr_ctx* c, const mr_task* tasks, int ntasks, int n_views, float*
     if (!out_on_device)
         MR_HIP(hipMemcpyAsync(out, d_out, sizeof(float) * out_len, hipMemcpyDeviceToHost, c->s_req));
     return finish_timing(c);
+}
+
+int mr_bp_tasks_dshift(mr_ctx* c, const mr_task* tasks, int ntasks, int n_views, float* out,
+                       int64_t out_len, int out_on_device) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    MR_CHECK(c->kind != MR_KIND_NONE, MR_ERUNTIME, "no geometry set");
+    MR_CHECK(ntasks >= 0 && out_len >= 0, MR_EVALUE, "negative task count or output length");
+    MR_CHECK(n_views >= 0 && n_views <= c->n, MR_EVALUE, "n_views outside the stored range");
+    if (ntasks == 0 || out_len == 0) return MR_OK;
+    MR_CHECK(tasks && out, MR_EVALUE, "null tasks or output");
+    rc = validate_tasks(tasks, ntasks, out_len);
+    if (rc) return rc;
+    MR_HIP(hipSetDevice(c->device));
+    float* d_out = out;
+    if (!out_on_device) {
+        rc = ensure(reinterpret_cast<void**>(&c->d_out), &c->out_bytes, sizeof(float) * out_len);
+        if (rc) return rc;
+        d_out = c->d_out;
+        MR_HIP(hipMemsetAsync(d_out, 0, sizeof(float) * out_len, c->s_req));
+    }
+    rc = run_tasks(c, tasks, ntasks, n_views, d_out, OP_DSHIFT);
+    if (rc) return rc;
+    if (!out_on_device)
+        MR_HIP(hipMemcpyAsync(out, d_out, sizeof(float) * out_len, hipMemcpyDeviceToHost, c->s_req));
+    return finish_timing(c);
+}
+
+int mr_bp_tasks_adjoint(mr_ctx* c, const mr_task* tasks, int ntasks, int n_views, const float* g,
+                        int64_t g_len, int g_on_device, float* gq, int gq_on_device) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    MR_CHECK(c->kind != MR_KIND_NONE, MR_ERUNTIME, "no geometry set");
+    MR_CHECK(ntasks >= 0 && g_len >= 0, MR_EVALUE, "negative task count or length");
+    MR_CHECK(n_views >= 0 && n_views <= c->n, MR_EVALUE, "n_views outside the stored range");
+    MR_CHECK(gq != nullptr, MR_EVALUE, "null gradient output");
+    const size_t gq_bytes = sizeof(float) * static_cast<size_t>(n_views) * c->R * c->C;
+    MR_HIP(hipSetDevice(c->device));
+    float* d_gq = gq;
+    if (!gq_on_device) {
+        rc = ensure(reinterpret_cast<void**>(&c->d_fstage), &c->fstage_bytes, gq_bytes);
+        if (rc) return rc;
+        d_gq = c->d_fstage;
+    }
+    if (gq_bytes) MR_HIP(hipMemsetAsync(d_gq, 0, gq_bytes, c->s_req));
+    if (ntasks > 0 && g_len > 0 && n_views > 0) {
+        MR_CHECK(tasks && g, MR_EVALUE, "null tasks or weights");
+        rc = validate_tasks(tasks, ntasks, g_len);
+        if (rc) return rc;
+        const float* d_g = g;
+        if (!g_on_device) {
+            rc = ensure(reinterpret_cast<void**>(&c->d_out), &c->out_bytes, sizeof(float) * g_len);
+            if (rc) return rc;
+            MR_HIP(hipMemcpyAsync(c->d_out, g, sizeof(float) * g_len, hipMemcpyHostToDevice, c->s_req));
+            d_g = c->d_out;
+        }
+        rc = run_tasks(c, tasks, ntasks, n_views, d_gq, OP_ADJOINT, d_g);
+        if (rc) return rc;
+    }
+    if (!gq_on_device && gq_bytes)
+        MR_HIP(hipMemcpyAsync(gq, d_gq, gq_bytes, hipMemcpyDeviceToHost, c->s_req));
+    MR_HIP(hipStreamSynchronize(c->s_req));
+    return MR_OK;
+}
+
+int mr_filter_adjoint(mr_ctx* c, const int64_t* indices, int k, const float* gq, int gq_on_device,
+                      float* graw, int graw_on_device) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    MR_CHECK(c->kind != MR_KIND_NONE, MR_ERUNTIME, "no geometry set");
+    MR_CHECK(k >= 0, MR_EVALUE, "negative view count");
+    if (k == 0) return MR_OK;
+    MR_CHECK(indices && gq && graw, MR_EVALUE, "null argument");
+    for (int j = 0; j < k; ++j)
+        MR_CHECK(indices[j] >= 0 && indices[j] < c->n_total, MR_EVALUE, "view index out of range");
+    MR_HIP(hipSetDevice(c->device));
+    const size_t bytes = sizeof(float) * static_cast<size_t>(c->R) * k * c->C;
+    const float* d_in = gq;
+    float* d_o = graw;
+    if (!gq_on_device) {
+        rc = ensure(reinterpret_cast<void**>(&c->d_stage), &c->stage_bytes, bytes);
+        if (rc) return rc;
+        MR_HIP(hipMemcpyAsync(c->d_stage, gq, bytes, hipMemcpyHostToDevice, c->s_ingest));
+        d_in = c->d_stage;
+    }
+    if (!graw_on_device) {
+        rc = ensure(reinterpret_cast<void**>(&c->d_pre), &c->pre_bytes, bytes);
+        if (rc) return rc;
+        d_o = c->d_pre;
+    }
+    std::vector<FilterView> fv(k);
+    for (int j = 0; j < k; ++j) fv[j] = c->filt_all[indices[j]];
+    rc = ensure(reinterpret_cast<void**>(&c->d_fv), &c->fv_len, sizeof(FilterView) * k);
+    if (rc) return rc;
+    MR_HIP(hipMemcpyAsync(c->d_fv, fv.data(), sizeof(FilterView) * k, hipMemcpyHostToDevice, c->s_ingest));
+    rc = launch_filter_adjoint(c, d_in, k, d_o, c->s_ingest);
+    if (rc) return rc;
+    if (!graw_on_device)
+        MR_HIP(hipMemcpyAsync(graw, d_o, bytes, hipMemcpyDeviceToHost, c->s_ingest));
+    MR_HIP(hipStreamSynchronize(c->s_ingest));
+    return MR_OK;
+}
+
+int mr_reset_views(mr_ctx* c) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->n = 0;
+    return MR_OK;
 }
 
 int mr_cor_scores(mr_ctx* c, const mr_task* tasks, int ntasks, int n_views, double* scores) {

@@ -105,7 +105,8 @@ __device__ void make_record(const ViewGeom* __restrict__ gp, float vs, const mr_
     const int base = (static_cast<int>(rref) + 1) * pitch + (static_cast<int>(cref) + 1);
     r4[2] = make_float4(static_cast<float>(Wb * f), static_cast<float>(-1.0 - cref),
                         static_cast<float>(C - cref), static_cast<float>(-1.0 - rref));
-    r4[3] = make_float4(static_cast<float>(R - rref), vs, __int_as_float(base), 0.0f);
+    r4[3] = make_float4(static_cast<float>(R - rref), vs, __int_as_float(base),
+                        static_cast<float>(g.uvv));
 }
 
 template <bool CONE, typename ST>
@@ -659,6 +660,177 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel3(
     }
 }
 
+// ---------------------------------------------------------------- adjoint and shift derivative
+// Transpose of the tile back-projection with respect to the stored (dequantised) views:
+// every point scatters scale * w_k * g_p with the forward's bilinear weights into
+// gq[view][R][C]; taps outside the detector are dropped, exactly the transpose of the
+// forward's zero taps (a clamped coordinate puts weight 1 on a zero tap and 0 on the
+// other).  Records are built with storage scale 1.
+template <bool CONE>
+__global__ __launch_bounds__(BP_THREADS) void bp_adjoint_kernel(
+    int pitch, int R, int C, const ViewGeom* __restrict__ geom, const mr_task* __restrict__ tasks,
+    const Tile* __restrict__ tiles, int n_views, float scale, const float* __restrict__ g,
+    float* __restrict__ gq) {
+    __shared__ float4 rec4[BP_VPASS * BP_REC / 4];
+    float* rec = reinterpret_cast<float*>(rec4);
+    const Tile tl = tiles[blockIdx.x];
+    const mr_task& T = tasks[tl.task];
+    const int t = threadIdx.x, tx = t & (BP_TS - 1), ty = t / BP_TS;
+    constexpr int PSTEP = BP_THREADS / BP_TS;
+    constexpr int PPT = BP_TP / PSTEP;
+    const int p0 = tl.tp * BP_TP, s0 = tl.ts * BP_TS;
+    double xt[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) xt[d] = T.o[d] + tl.q * T.c[d] + p0 * T.a[d] + s0 * T.b[d];
+    const int s = s0 + tx;
+    const int64_t obase = T.out_offset + static_cast<int64_t>(tl.q) * T.stride_q;
+    float gp[PPT];
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) {
+        const int p = p0 + ty + PSTEP * m;
+        const bool ok = (p < T.np) && (s < T.ns) && (!T.mask_box || in_box(T, p, s));
+        gp[m] = ok ? g[obase + static_cast<int64_t>(p) * T.stride_p +
+                       static_cast<int64_t>(s) * T.stride_s] * scale
+                   : 0.0f;
+    }
+    const float ds = static_cast<float>(tx), dp0 = static_cast<float>(ty);
+    const int64_t vstride = static_cast<int64_t>(R) * C;
+    for (int vb = 0; vb < n_views; vb += BP_VPASS) {
+        const int nv = min(BP_VPASS, n_views - vb);
+        __syncthreads();
+        if (t < nv) make_record(geom + vb + t, 1.0f, T, xt, R, C, pitch, CONE, rec + t * BP_REC);
+        __syncthreads();
+        for (int v = 0; v < nv; ++v) {
+            const float4 r0 = rec4[v * 4 + 0], r1 = rec4[v * 4 + 1];
+            const float4 r2 = rec4[v * 4 + 2], r3 = rec4[v * 4 + 3];
+            const int base = __float_as_int(r3.z);
+            // absolute detector coordinates of the reference: base = (rref+1)*pitch + cref+1
+            const int rref = static_cast<int>(-1.0f - r2.w);        // rlo = -1 - rref
+            const int cref = static_cast<int>(-1.0f - r2.y);        // clo = -1 - cref
+            (void)base;
+            float* gv = gq + static_cast<int64_t>(vb + v) * vstride;
+#pragma unroll
+            for (int m = 0; m < PPT; ++m) {
+                if (gp[m] == 0.0f) continue;
+                const float dp = dp0 + static_cast<float>(PSTEP * m);
+                const float uh = fmaf(dp, r0.z, fmaf(ds, r0.y, r0.x));
+                const float vh = fmaf(dp, r1.y, fmaf(ds, r1.x, r0.w));
+                float col, row, wt;
+                if (CONE) {
+                    const float wh = fmaf(dp, r2.x, fmaf(ds, r1.w, r1.z));
+                    const float rw = __builtin_amdgcn_rcpf(wh);
+                    col = uh * rw; row = vh * rw; wt = rw * rw;
+                } else {
+                    col = uh; row = vh; wt = 1.0f;
+                }
+                col = __builtin_amdgcn_fmed3f(col, r2.y, r2.z);
+                row = __builtin_amdgcn_fmed3f(row, r2.w, r3.x);
+                const float fc = __builtin_amdgcn_fractf(col), fr = __builtin_amdgcn_fractf(row);
+                const int c0 = cvt_flr(col) + cref, rr0 = cvt_flr(row) + rref;   // absolute
+                const float sv = wt * gp[m];
+                const float w00 = (1.0f - fr) * (1.0f - fc) * sv, w01 = (1.0f - fr) * fc * sv;
+                const float w10 = fr * (1.0f - fc) * sv, w11 = fr * fc * sv;
+                const bool c0ok = c0 >= 0 && c0 < C, c1ok = c0 + 1 >= 0 && c0 + 1 < C;
+                if (rr0 >= 0 && rr0 < R) {
+                    float* rowp = gv + static_cast<int64_t>(rr0) * C;
+                    if (c0ok) atomicAdd(rowp + c0, w00);
+                    if (c1ok) atomicAdd(rowp + c0 + 1, w01);
+                }
+                if (rr0 + 1 >= 0 && rr0 + 1 < R) {
+                    float* rowp = gv + static_cast<int64_t>(rr0 + 1) * C;
+                    if (c0ok) atomicAdd(rowp + c0, w10);
+                    if (c1ok) atomicAdd(rowp + c0 + 1, w11);
+                }
+            }
+        }
+    }
+}
+
+// d(out)/d(cor_shift) of the tile back-projection: the shift adds s to the column and
+// s * (u . vv) to the row of every tap (recon.py:420), so per view the derivative of
+// the interpolant is dbil/dcol + (u . vv) dbil/drow, zero where the coordinate is
+// clamped outside the detector.
+template <bool CONE, typename ST>
+__global__ __launch_bounds__(BP_THREADS) void bp_dshift_kernel(
+    const ST* __restrict__ ring, int64_t view_stride, int pitch, int R, int C,
+    const ViewGeom* __restrict__ geom, const float* __restrict__ vscale,
+    const mr_task* __restrict__ tasks, const Tile* __restrict__ tiles, int n_views, float scale,
+    float* __restrict__ out) {
+    __shared__ float4 rec4[BP_VPASS * BP_REC / 4];
+    float* rec = reinterpret_cast<float*>(rec4);
+    constexpr int LE = Taps<ST>::LOG_ESZ;
+    const Tile tl = tiles[blockIdx.x];
+    const mr_task& T = tasks[tl.task];
+    const int t = threadIdx.x, tx = t & (BP_TS - 1), ty = t / BP_TS;
+    constexpr int PSTEP = BP_THREADS / BP_TS;
+    constexpr int PPT = BP_TP / PSTEP;
+    const int p0 = tl.tp * BP_TP, s0 = tl.ts * BP_TS;
+    double xt[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) xt[d] = T.o[d] + tl.q * T.c[d] + p0 * T.a[d] + s0 * T.b[d];
+    const int s = s0 + tx;
+    bool valid[PPT], inside[PPT];
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) {
+        const int p = p0 + ty + PSTEP * m;
+        valid[m] = (p < T.np) && (s < T.ns);
+        inside[m] = valid[m] && (!T.mask_box || in_box(T, p, s));
+    }
+    const float ds = static_cast<float>(tx), dp0 = static_cast<float>(ty);
+    const int pitchb = pitch << LE;
+    const int view_bytes = static_cast<int>(view_stride << LE);
+    float acc[PPT];
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) acc[m] = 0.0f;
+    for (int vb = 0; vb < n_views; vb += BP_VPASS) {
+        const int nv = min(BP_VPASS, n_views - vb);
+        __syncthreads();
+        if (t < nv)
+            make_record(geom + vb + t, vscale[vb + t], T, xt, R, C, pitch, CONE, rec + t * BP_REC);
+        __syncthreads();
+        for (int v = 0; v < nv; ++v) {
+            const float4 r0 = rec4[v * 4 + 0], r1 = rec4[v * 4 + 1];
+            const float4 r2 = rec4[v * 4 + 2], r3 = rec4[v * 4 + 3];
+            const int baseb = __float_as_int(r3.z) << LE;
+            const float uvv = r3.w;
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<ST*>(ring + static_cast<int64_t>(vb + v) * view_stride), 0, view_bytes,
+                0x00020000);
+#pragma unroll
+            for (int m = 0; m < PPT; ++m) {
+                const float dp = dp0 + static_cast<float>(PSTEP * m);
+                const float uh = fmaf(dp, r0.z, fmaf(ds, r0.y, r0.x));
+                const float vh = fmaf(dp, r1.y, fmaf(ds, r1.x, r0.w));
+                float colr, rowr, wt;
+                if (CONE) {
+                    const float wh = fmaf(dp, r2.x, fmaf(ds, r1.w, r1.z));
+                    const float rw = __builtin_amdgcn_rcpf(wh);
+                    colr = uh * rw; rowr = vh * rw; wt = rw * rw;
+                } else {
+                    colr = uh; rowr = vh; wt = r3.y;
+                }
+                const float col = __builtin_amdgcn_fmed3f(colr, r2.y, r2.z);
+                const float row = __builtin_amdgcn_fmed3f(rowr, r2.w, r3.x);
+                const float fc = __builtin_amdgcn_fractf(col), fr = __builtin_amdgcn_fractf(row);
+                const int voff = __mul24(cvt_flr(row), pitchb) + ((cvt_flr(col) << LE) + baseb);
+                float a, b, c, d;
+                Taps<ST>::load(rs, voff, pitchb, a, b, c, d);
+                const float ddc = (col == colr) ? fmaf(fr, (d - c) - (b - a), b - a) : 0.0f;
+                const float ddr = (row == rowr) ? fmaf(fc, (d - b) - (c - a), c - a) : 0.0f;
+                acc[m] = fmaf(wt, fmaf(uvv, ddr, ddc), acc[m]);
+            }
+        }
+    }
+    const int64_t obase = T.out_offset + static_cast<int64_t>(tl.q) * T.stride_q;
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) {
+        if (!valid[m]) continue;
+        const int p = p0 + ty + PSTEP * m;
+        out[obase + static_cast<int64_t>(p) * T.stride_p + static_cast<int64_t>(s) * T.stride_s] =
+            inside[m] ? acc[m] * scale : 0.0f;
+    }
+}
+
 // Point-list back-projection with float64 coordinates (reconstruct_points): each
 // thread owns one point; per view the projective form is evaluated in FP64, the
 // fractional parts and FDK weight are rounded to FP32 for the interpolation.
@@ -826,6 +998,53 @@ int launch_bp_points(Ctx* c, const double* d_pts, int64_t M, double cor_shift, i
     return c->kind == MR_KIND_CONE
                ? dispatch_points<true>(c, d_pts, M, cor_shift, n_views, scale, d_out, s)
                : dispatch_points<false>(c, d_pts, M, cor_shift, n_views, scale, d_out, s);
+}
+
+int launch_bp_adjoint(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles,
+                      int n_views, float scale, const float* d_g, float* d_gq, hipStream_t s) {
+    if (ntiles <= 0) return MR_OK;
+    dim3 grid(ntiles), block(BP_THREADS);
+    if (c->kind == MR_KIND_CONE)
+        hipLaunchKernelGGL(bp_adjoint_kernel<true>, grid, block, 0, s, c->pitch, c->R, c->C,
+                           c->d_geom, d_tasks, d_tiles, n_views, scale, d_g, d_gq);
+    else
+        hipLaunchKernelGGL(bp_adjoint_kernel<false>, grid, block, 0, s, c->pitch, c->R, c->C,
+                           c->d_geom, d_tasks, d_tiles, n_views, scale, d_g, d_gq);
+    MR_HIP(hipGetLastError());
+    return MR_OK;
+}
+
+template <bool CONE>
+static int dshift_dispatch(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles,
+                           int n_views, float scale, float* d_out, hipStream_t s) {
+    dim3 grid(ntiles), block(BP_THREADS);
+    switch (c->storage) {
+    case MR_STORE_F32:
+        hipLaunchKernelGGL((bp_dshift_kernel<CONE, float>), grid, block, 0, s,
+                           static_cast<const float*>(c->ring), c->view_stride, c->pitch, c->R, c->C,
+                           c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out);
+        break;
+    case MR_STORE_F16:
+        hipLaunchKernelGGL((bp_dshift_kernel<CONE, __half>), grid, block, 0, s,
+                           static_cast<const __half*>(c->ring), c->view_stride, c->pitch, c->R,
+                           c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out);
+        break;
+    default:
+        hipLaunchKernelGGL((bp_dshift_kernel<CONE, uint16_t>), grid, block, 0, s,
+                           static_cast<const uint16_t*>(c->ring), c->view_stride, c->pitch, c->R,
+                           c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out);
+        break;
+    }
+    MR_HIP(hipGetLastError());
+    return MR_OK;
+}
+
+int launch_bp_dshift(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles, int n_views,
+                     float scale, float* d_out, hipStream_t s) {
+    if (ntiles <= 0) return MR_OK;
+    return c->kind == MR_KIND_CONE
+               ? dshift_dispatch<true>(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s)
+               : dshift_dispatch<false>(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s);
 }
 
 int launch_read_views(Ctx* c, int first, int count, float* d_out, hipStream_t s) {

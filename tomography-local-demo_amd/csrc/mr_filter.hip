@@ -99,15 +99,18 @@ __global__ __launch_bounds__(FT) void filter_kernel(
     const int v = blockIdx.y;
     const FilterView P = fv[v];
     const bool has1 = r1 < R;
-    const float* x0 = block + (static_cast<int64_t>(r0) * k + v) * C;
-    const float* x1 = block + (static_cast<int64_t>(has1 ? r1 : r0) * k + v) * C;
+    // MODE 4 (adjoint) reads views [k][R][C]; the others read blocks [R][k][C]
+    const float* x0 = MODE == 4 ? block + (static_cast<int64_t>(v) * R + r0) * C
+                                : block + (static_cast<int64_t>(r0) * k + v) * C;
+    const float* x1 = MODE == 4 ? block + (static_cast<int64_t>(v) * R + (has1 ? r1 : r0)) * C
+                                : block + (static_cast<int64_t>(has1 ? r1 : r0) * k + v) * C;
     const float b0 = P.b0 + r0 * P.db, b1 = P.b0 + r1 * P.db;
     for (int c = threadIdx.x; c < N; c += FT) {
         float y0 = 0.0f, y1 = 0.0f;
         if (c < C) {
             y0 = x0[c];
             y1 = has1 ? x1[c] : 0.0f;
-            if (P.cone) {
+            if (MODE != 4 && P.cone) {
                 const float a = P.a0 + c * P.da;
                 const float aa = fmaf(a, a, P.sdd2);
                 y0 *= P.sdd * __frsqrt_rn(fmaf(b0, b0, aa));
@@ -152,10 +155,22 @@ __global__ __launch_bounds__(FT) void filter_kernel(
                 q[static_cast<int64_t>(r1) * C + c] = y1;
                 lmax = fmaxf(lmax, fabsf(y1));
             }
-        } else {
+        } else if (MODE == 3) {
             float* q = static_cast<float*>(dst);
             q[static_cast<int64_t>(r0) * C + c] = y0;
             if (has1) q[static_cast<int64_t>(r1) * C + c] = y1;
+        } else {
+            // adjoint: the cosine weight after the (self-adjoint) filter, block layout
+            float w0 = 1.0f, w1 = 1.0f;
+            if (P.cone) {
+                const float a = P.a0 + c * P.da;
+                const float aa = fmaf(a, a, P.sdd2);
+                w0 = P.sdd * __frsqrt_rn(fmaf(b0, b0, aa));
+                w1 = P.sdd * __frsqrt_rn(fmaf(b1, b1, aa));
+            }
+            float* q = static_cast<float*>(dst);
+            q[(static_cast<int64_t>(r0) * k + v) * C + c] = y0 * w0;
+            if (has1) q[(static_cast<int64_t>(r1) * k + v) * C + c] = y1 * w1;
         }
     }
     if (MODE == 2) {
@@ -300,6 +315,12 @@ int launch_store(Ctx* c, const float* d_q, int k, hipStream_t s) {
     }
     MR_HIP(hipGetLastError());
     return MR_OK;
+}
+
+int launch_filter_adjoint(Ctx* c, const float* d_gq, int k, float* d_graw, hipStream_t s) {
+    dim3 grid((c->R + 1) / 2, k);
+    return launch_mode<4>(log2i(c->N), grid, s, d_gq, c->R, k, c->C, c->d_fv, c->d_H, c->d_tw,
+                          d_graw, 0, 0, nullptr);
 }
 
 int launch_filter_rows_plain(Ctx* c, const float* d_in, int rows, int n, float gain,

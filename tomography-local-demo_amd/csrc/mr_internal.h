@@ -139,6 +139,11 @@ int launch_bp_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int nti
 int launch_bp_points(Ctx* c, const double* d_pts, int64_t M, double cor_shift, int n_views,
                      float scale, float* d_out, hipStream_t s);
 int launch_read_views(Ctx* c, int first, int count, float* d_out, hipStream_t s);
+int launch_bp_adjoint(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles,
+                      int n_views, float scale, const float* d_g, float* d_gq, hipStream_t s);
+int launch_bp_dshift(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles, int n_views,
+                     float scale, float* d_out, hipStream_t s);
+int launch_filter_adjoint(Ctx* c, const float* d_gq, int k, float* d_graw, hipStream_t s);
 int launch_cor_scores(Ctx* c, const float* d_img, int nimg, int64_t npix, double* d_scores,
                       hipStream_t s);
 int launch_phantom(const double* d_ell, int K, int kind, const double* d_geo, int A, int R,

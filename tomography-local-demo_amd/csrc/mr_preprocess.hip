@@ -9,10 +9,9 @@
 // gaussian_blur (preprocess.py:8-28): per axis (rows, then columns), the reference
 // edge-pads each line by pad = ceil(3 sigma) to length N = n + 2 pad and multiplies
 // its rfft by H = exp(-2 (pi f sigma)^2).  That is a circular convolution of the
-// padded line with g = irfft(H, N).  The host computes g exactly in float64; |g(m)|
-// falls below 1e-20 g(0) beyond |m| = 10 sigma, so the device sums the taps
-// m in [-L, L], L = ceil(10 sigma) + 2 (all N taps when 2L+1 >= N), in float64,
-// and rounds each pass to float32 like the reference.
+// padded line with g = irfft(H, N).  The host computes g exactly in float64 and keeps
+// the taps m in [-L, L] outside which |g| < 1e-13 g(0) (all N taps for small sigma);
+// the device sums them in float64 and rounds each pass to float32 like the reference.
 #include <cmath>
 
 #include "mr_internal.h"
@@ -73,14 +72,15 @@ __global__ void gauss_kernel(const float* __restrict__ in, float* __restrict__ o
     out[(static_cast<int64_t>(r) * k + v) * C + c] = static_cast<float>(acc);
 }
 
-// irfft(H, N) at offsets m0..m0+M-1 (float64, exact cosine sum)
+// Taps of g = irfft(H, N), H = exp(-2 (pi f sigma)^2), in float64 (exact cosine sum).
+// For sigma >~ 2.4 the spectrum is negligible before Nyquist and g decays like a
+// Gaussian; for smaller sigma H(1/2) is not small and g keeps an oscillating tail of
+// amplitude ~H(1/2)/N, so all N taps are needed.  The window [-L, L] is the smallest
+// one outside which every |g| < 1e-13 g(0); otherwise the full circle is used.
 void gauss_taps(int n, double sigma, int* pad_out, int* N_out, int* m0_out,
                 std::vector<double>& g) {
     const int pad = static_cast<int>(std::ceil(3.0 * sigma));
     const int N = n + 2 * pad;
-    const int L = static_cast<int>(std::ceil(10.0 * sigma)) + 2;
-    int m0, M;
-    if (2 * L + 1 >= N) { m0 = 0; M = N; } else { m0 = -L; M = 2 * L + 1; }
     const int K = N / 2 + 1;                         // rfft bins
     std::vector<double> H(K);
     for (int q = 0; q < K; ++q) {
@@ -88,15 +88,28 @@ void gauss_taps(int n, double sigma, int* pad_out, int* N_out, int* m0_out,
         const double a = M_PI * f * sigma;
         H[q] = std::exp(-2.0 * a * a);
     }
-    g.assign(M, 0.0);
-    for (int t = 0; t < M; ++t) {
-        const int m = m0 + t;
+    std::vector<double> full(N);
+    for (int m = 0; m < N; ++m) {
         double s = H[0];
         for (int q = 1; q < K; ++q) {
             const double w = (N % 2 == 0 && q == N / 2) ? 1.0 : 2.0;
             s += w * H[q] * std::cos(2.0 * M_PI * static_cast<double>(q) * m / N);
         }
-        g[t] = s / N;
+        full[m] = s / N;
+    }
+    int L = N / 2;
+    const double thr = 1e-13 * std::fabs(full[0]);
+    for (int m = N / 2; m >= 1; --m) {
+        if (std::fabs(full[m]) >= thr || std::fabs(full[N - m]) >= thr) { L = m; break; }
+        L = m - 1;
+    }
+    int m0, M;
+    if (2 * L + 1 >= N) { m0 = 0; M = N; } else { m0 = -L; M = 2 * L + 1; }
+    g.resize(M);
+    for (int t = 0; t < M; ++t) {
+        int m = (m0 + t) % N;
+        if (m < 0) m += N;
+        g[t] = full[m];
     }
     *pad_out = pad;
     *N_out = N;

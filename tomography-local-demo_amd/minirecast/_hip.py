@@ -40,6 +40,11 @@ SIGNATURES = [
     ("mr_bp_points", _c.c_int, [_P, _P, _c.c_int64, _c.c_double, _c.c_int, _P]),
     ("mr_bp_tasks", _c.c_int, [_P, _P, _c.c_int, _c.c_int, _P, _c.c_int64, _c.c_int]),
     ("mr_cor_scores", _c.c_int, [_P, _P, _c.c_int, _c.c_int, _P]),
+    ("mr_bp_tasks_adjoint", _c.c_int,
+     [_P, _P, _c.c_int, _c.c_int, _P, _c.c_int64, _c.c_int, _P, _c.c_int]),
+    ("mr_bp_tasks_dshift", _c.c_int, [_P, _P, _c.c_int, _c.c_int, _P, _c.c_int64, _c.c_int]),
+    ("mr_filter_adjoint", _c.c_int, [_P, _P, _c.c_int, _P, _c.c_int, _P, _c.c_int]),
+    ("mr_reset_views", _c.c_int, [_P]),
     ("mr_filter_rows", _c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_double, _c.c_int, _P]),
     ("mr_read_views", _c.c_int, [_P, _c.c_int, _c.c_int, _P]),
     ("mr_phantom_cone", _c.c_int, [_P, _P, _c.c_int, _P, _c.c_int, _c.c_int, _c.c_int, _P]),
@@ -231,6 +236,51 @@ class Context:
         T = np.ascontiguousarray(tasks, dtype=TASK_DTYPE)
         check(self.lib.mr_bp_tasks(self.h, ptr(T), len(T), int(n_views), ctypes.c_void_p(dptr),
                                    int(out_len), 1))
+
+    def bp_tasks_adjoint(self, tasks, n_views, g, rows, cols):
+        """Host: g [len] float32 -> gq [n_views, rows, cols] float32."""
+        T = np.ascontiguousarray(tasks, dtype=TASK_DTYPE)
+        gg = np.ascontiguousarray(g, dtype=np.float32).ravel()
+        gq = np.empty((int(n_views), int(rows), int(cols)), dtype=np.float32)
+        check(self.lib.mr_bp_tasks_adjoint(self.h, ptr(T), len(T), int(n_views), ptr(gg), len(gg),
+                                           0, ptr(gq), 0))
+        return gq
+
+    def bp_tasks_adjoint_device(self, tasks, n_views, d_g, g_len, d_gq):
+        T = np.ascontiguousarray(tasks, dtype=TASK_DTYPE)
+        check(self.lib.mr_bp_tasks_adjoint(self.h, ptr(T), len(T), int(n_views),
+                                           ctypes.c_void_p(d_g), int(g_len), 1,
+                                           ctypes.c_void_p(d_gq), 1))
+
+    def bp_tasks_dshift(self, tasks, n_views, out_len):
+        T = np.ascontiguousarray(tasks, dtype=TASK_DTYPE)
+        out = np.zeros(int(out_len), dtype=np.float32)
+        if len(T) and out_len:
+            check(self.lib.mr_bp_tasks_dshift(self.h, ptr(T), len(T), int(n_views), ptr(out),
+                                              int(out_len), 0))
+        return out
+
+    def bp_tasks_dshift_device(self, tasks, n_views, d_out, out_len):
+        T = np.ascontiguousarray(tasks, dtype=TASK_DTYPE)
+        check(self.lib.mr_bp_tasks_dshift(self.h, ptr(T), len(T), int(n_views),
+                                          ctypes.c_void_p(d_out), int(out_len), 1))
+
+    def filter_adjoint(self, indices, gq):
+        """Host: gq [k, rows, cols] -> graw [rows, k, cols]."""
+        idx = np.ascontiguousarray(indices, dtype=np.int64).ravel()
+        g = np.ascontiguousarray(gq, dtype=np.float32)
+        k, R, C = g.shape
+        out = np.empty((R, k, C), dtype=np.float32)
+        check(self.lib.mr_filter_adjoint(self.h, ptr(idx), len(idx), ptr(g), 0, ptr(out), 0))
+        return out
+
+    def filter_adjoint_device(self, indices, d_gq, d_graw):
+        idx = np.ascontiguousarray(indices, dtype=np.int64).ravel()
+        check(self.lib.mr_filter_adjoint(self.h, ptr(idx), len(idx), ctypes.c_void_p(d_gq), 1,
+                                         ctypes.c_void_p(d_graw), 1))
+
+    def reset_views(self):
+        check(self.lib.mr_reset_views(self.h))
 
     def cor_scores(self, tasks, n_views):
         T = np.ascontiguousarray(tasks, dtype=TASK_DTYPE)
