@@ -171,6 +171,33 @@ int mr_filter_adjoint(mr_ctx* ctx, const int64_t* indices, int k, const float* g
 /* Forgets the stored views (n = 0) so that the ring can be refilled. */
 int mr_reset_views(mr_ctx* ctx);
 
+/* ---- plans: batched requests on a device work queue (recon.py:426-475 callers that
+ * re-request the same slices after every arriving batch; no counterpart in the reference)
+ *
+ * A plan compiles a fixed batch of tasks once: its tiles are enumerated on the device
+ * (in-box test included) into a work queue that a persistent back-projection launch
+ * drains.  out_len floats of output live in device memory owned by the plan. */
+typedef struct mr_plan mr_plan;
+int mr_plan_create(mr_ctx* ctx, const mr_task* tasks, int ntasks, int64_t out_len, mr_plan** out);
+int mr_plan_destroy(mr_plan* plan);
+
+/* Runs the plan over the first n_views stored views (n_views < 0: all stored views);
+ * copies the output to out_host when it is not NULL. */
+int mr_plan_run(mr_plan* plan, int n_views, float* out_host);
+
+/* Streaming update (recon.py:346-376 followed by the plan's requests): ingests k views
+ * (indices, host block [rows][k][cols]) and re-runs the plan over all stored views, as
+ * one HIP graph captured per batch size and replayed for every later batch of that
+ * size.  Copies the output to out_host when it is not NULL. */
+int mr_plan_stream(mr_plan* plan, const int64_t* indices, int k, const float* block,
+                   float* out_host);
+
+/* Device pointer and length of the plan's output. */
+int mr_plan_output(mr_plan* plan, float** out_device, int64_t* out_len);
+
+/* Duration of the last mr_plan_run / mr_plan_stream on the device (milliseconds). */
+int mr_plan_last_ms(mr_plan* plan, double* ms);
+
 /* Parity hook for filter_rows (recon.py:58-64): ramp-filters `rows` host rows of
  * length n (float32) with pitch tau on the context's device. */
 int mr_filter_rows(mr_ctx* ctx, const float* in, int rows, int n, double tau,

@@ -96,17 +96,30 @@ def test_dshift_vs_oracle_and_finite_difference(setup):
     want[mask] = O.bp_cone_dshift(q, K['s'], K['d'], K['u'], K['uu'], K['vv'], K['n'], K['sdd'],
                                   K['sod'], pts[mask], np.pi / A, s0)
     assert rel_l2(d, want) < 1e-4
-    # central finite difference of a weighted loss, h = 1e-3 px
+    # central finite difference (h = 1e-3 px) of the float64 forward on the same views
     wts = np.random.default_rng(2).normal(size=out_len)
     h = 1e-3
+    D = K['d']
 
-    def loss(sft):
+    def loss64(sft):
+        y = np.zeros(out_len)
+        y[mask] = O.bp_cone(q, K['s'], D - sft * K['u'], K['uu'], K['vv'], K['n'], K['sdd'],
+                            K['sod'], pts[mask], np.pi / A)
+        return float(y @ wts)
+    fd64 = (loss64(s0 + h) - loss64(s0 - h)) / (2 * h)
+    an = float(d.astype(np.float64) @ wts)
+    assert abs(fd64 - an) <= 1e-3 * abs(an), (fd64, an)
+
+    # the float32 forward's own difference quotients (diagnostic, loose bound)
+    def loss32(sft):
         Tx = tasks.copy()
         Tx['cor_shift'] = sft
         return float(ctx.bp_tasks(Tx, A, out_len).astype(np.float64) @ wts)
-    fd = (loss(s0 + h) - loss(s0 - h)) / (2 * h)
-    an = float(d.astype(np.float64) @ wts)
-    assert abs(fd - an) <= 2e-3 * abs(an) + 1e-6
+    fd_a = (loss32(s0 + 1e-3) - loss32(s0 - 1e-3)) / 2e-3
+    fd_b = (loss32(s0 + 1e-2) - loss32(s0 - 1e-2)) / 2e-2
+    print(f"dL/ds analytic {an:.6f}  fd64(1e-3) {fd64:.6f}  fd32(1e-3) {fd_a:.6f}  "
+          f"fd32(1e-2) {fd_b:.6f}")
+    assert abs(fd_b - an) <= 0.05 * abs(an)
 
 
 def test_autograd_end_to_end(setup):
@@ -132,10 +145,16 @@ def test_autograd_end_to_end(setup):
                            K['n'], K['sdd'], K['sod'], pts[mask], np.pi / A, R, C)
     gref = O.fdk_filter_adjoint(gq, K, "shepp-logan")
     assert rel_l2(proj.grad.cpu().numpy(), gref) < 1e-5
-    # shift gradient against a central difference through the forward
-    with torch.no_grad():
-        h = 1e-3
-        lp = float((autograd.fdk_slices(proj, 0.25 + h, op) * wts).sum())
-        lm = float((autograd.fdk_slices(proj, 0.25 - h, op) * wts).sum())
-    fd = (lp - lm) / (2 * h)
-    assert abs(float(shift.grad) - fd) <= 2e-3 * abs(fd) + 1e-5
+    # shift gradient against a central difference (h = 1e-3 px) of the float64 forward
+    # on the views the operator filtered
+    op.load(proj.detach())
+    q = op.ctx.read_views(0, A, R, C).astype(np.float64)
+    w64 = wts.cpu().numpy().astype(np.float64)[mask]
+
+    def loss64(sft):
+        return float(O.bp_cone(q, K['s'], K['d'] - sft * K['u'], K['uu'], K['vv'], K['n'],
+                               K['sdd'], K['sod'], pts[mask], np.pi / A) @ w64)
+    h = 1e-3
+    fd = (loss64(0.25 + h) - loss64(0.25 - h)) / (2 * h)
+    print(f"shift grad {float(shift.grad):.6f}  fd64 {fd:.6f}")
+    assert abs(float(shift.grad) - fd) <= 1e-3 * abs(fd) + 1e-6

@@ -1,0 +1,119 @@
+"""GPU tests of the batched forms: persistent launches over a device work queue
+(plans), graph-replayed streaming updates, and the back-projection variants."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def workload():
+    from minirecast import phantom, recon, workloads
+    w = workloads.small_cone()
+    proj = phantom.project_cone(w.phantom, w.geom)
+    box = w.geom.volume_box()
+    planes = [dict(origin=(0.0, 0.0, 0.0), normal=(1.0, 0.5, 0.8), n=96),
+              dict(origin=(0.5, 0.0, -1.0), normal=(-0.3, 1.0, 0.4), n=64, zoom=1.7,
+                   center=(1.0, 0.0, 0.0)),
+              dict(origin=(0.0, 0.0, 2.0), normal=(0.0, 0.0, 1.0), n=47)]
+    tasks, off = [], 0
+    for p in planes:
+        tasks.append(recon.plane_task(box, p['origin'], p['normal'], p['n'], p.get('size'),
+                                      p.get('zoom', 1.0), p.get('center'), 0.0, off))
+        off += p['n'] ** 2
+    return w, proj, planes, np.array(tasks), off
+
+
+def _ctx(w, storage="f32", variant="2", monkeypatch=None):
+    from minirecast import _hip
+    if monkeypatch is not None:
+        monkeypatch.setenv("MINIRECAST_BP_VARIANT", variant)
+    ctx = _hip.Context(0)
+    ctx.set_cone(w.geom.vectors, w.geom.rows, w.geom.cols, w.window, storage)
+    return ctx
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_plan_run_matches_tasks(workload, storage, monkeypatch):
+    from minirecast import _hip
+    w, proj, planes, tasks, out_len = workload
+    ctx = _ctx(w, storage, "2", monkeypatch)
+    A = w.geom.n_angles
+    ctx.push(np.arange(A), proj)
+    T = np.array(tasks, dtype=_hip.TASK_DTYPE)
+    ref = ctx.bp_tasks(T, A, out_len)
+    plan = _hip.Plan(ctx, T, out_len)
+    for _ in range(3):                                  # repeated runs reuse the queue
+        got = plan.run()
+        np.testing.assert_array_equal(got == 0, ref == 0)
+        assert rel_l2(got, ref) < 1e-6
+    half = plan.run(A // 2)
+    assert rel_l2(half, ctx.bp_tasks(T, A // 2, out_len)) < 1e-6
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_plan_stream_graph_replays(workload, storage, monkeypatch):
+    """Streaming updates replay one captured graph per batch size; every replay (with
+    eager work in between) matches the eager ingest + back-projection."""
+    from minirecast import _hip
+    from minirecast.stream import arrival_order
+    w, proj, planes, tasks, out_len = workload
+    A = w.geom.n_angles
+    T = np.array(tasks, dtype=_hip.TASK_DTYPE)
+    live = _ctx(w, storage, "2", monkeypatch)
+    eager = _ctx(w, storage, "2", monkeypatch)
+    plan = _hip.Plan(live, T, out_len)
+    order = arrival_order(A, "interleaved")
+    batches = [order[i:i + 10] for i in range(0, A, 10)]
+    for bi, idx in enumerate(batches):
+        blk = np.ascontiguousarray(proj[:, idx, :])
+        got = plan.stream(idx, blk)
+        eager.push(idx, blk)
+        want = eager.bp_tasks(T, eager.num_views(), out_len)
+        assert live.num_views() == eager.num_views()
+        assert rel_l2(got, want) < 1e-6, (bi, rel_l2(got, want))
+    # the final state equals the float64 oracle
+    orc = O.OracleReconstructor("cone", w.geom.rows, w.geom.cols, w.geom.volume_box(),
+                                window=w.window, vectors=w.geom.vectors)
+    for idx in batches:
+        orc.add_projections(idx, proj[:, idx, :])
+    off = 0
+    tol = 1e-5 if storage == "f32" else 1e-3
+    for p in planes:
+        img, _ = orc.reconstruct_plane(p['origin'], p['normal'], n=p['n'], zoom=p.get('zoom', 1.0),
+                                       center=p.get('center'))
+        assert rel_l2(got[off:off + p['n'] ** 2], img.ravel()) < tol
+        off += p['n'] ** 2
+
+
+@pytest.mark.parametrize("variant,ppt", [("2", "4"), ("4", "4"), ("4", "2")])
+def test_bp_variants_full_check(workload, variant, ppt, monkeypatch):
+    from minirecast import _hip
+    monkeypatch.setenv("MINIRECAST_BP_PPT", ppt)
+    w, proj, planes, tasks, out_len = workload
+    T = np.array(tasks, dtype=_hip.TASK_DTYPE)
+    ref_ctx = _ctx(w, "f32", "1", monkeypatch)
+    ref_ctx.push(np.arange(w.geom.n_angles), proj)
+    ref = ref_ctx.bp_tasks(T, w.geom.n_angles, out_len)
+    for storage, tol in (("f32", 1e-6), ("f16", 1e-3), ("bf16", 5e-3)):
+        ctx = _ctx(w, storage, variant, monkeypatch)
+        ctx.push(np.arange(w.geom.n_angles), proj)
+        got = ctx.bp_tasks(T, w.geom.n_angles, out_len)
+        np.testing.assert_array_equal(got == 0, ref == 0)
+        assert rel_l2(got, ref) < tol, (variant, ppt, storage, rel_l2(got, ref))
+    # parallel beam through the same variant
+    from minirecast import phantom, recon
+    from minirecast.geometry import ParallelGeometry
+    g = ParallelGeometry(np.arange(90) * np.pi / 90, 24, 80, 1.0, 1.0)
+    ph = phantom.random_ellipsoids(9, 8, g.volume_box(), (0.1, 1.0))
+    pp = phantom.project_parallel(ph, g)
+    rec = recon.SliceReconstructor(pp, g, window="hann", verbose=False)
+    img, _ = rec.reconstruct_plane((1.0, 2.0, 3.0), (1.0, 0.5, 0.8), n=70, cor_shift=0.6)
+    orc = O.OracleReconstructor("parallel", 24, 80, g.volume_box(), window="hann",
+                                angles=g.angles, pixel_u=1.0, pixel_v=1.0)
+    orc.add_projections(np.arange(90), pp)
+    ref2, _ = orc.reconstruct_plane((1.0, 2.0, 3.0), (1.0, 0.5, 0.8), n=70, cor_shift=0.6)
+    assert rel_l2(img, ref2) < 1e-5

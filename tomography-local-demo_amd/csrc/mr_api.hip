@@ -294,6 +294,7 @@ int ramp_response_host(int n, double tau, int window, std::vector<double>& H) {
 using namespace mr;
 
 struct mr_ctx : Ctx {};
+struct mr_plan : Plan {};
 
 extern "C" {
 
@@ -331,8 +332,9 @@ int mr_ctx_create(int device, mr_ctx** out) {
     c->device = device;
     if (const char* v = std::getenv("MINIRECAST_BP_VARIANT")) {
         const int k = std::atoi(v);
-        c->bp_variant = (k >= 1 && k <= 3) ? k : c->bp_variant;
+        c->bp_variant = (k >= 1 && k <= 4) ? k : c->bp_variant;
     }
+    if (const char* v = std::getenv("MINIRECAST_BP_PPT")) c->bp_ppt = std::atoi(v) == 2 ? 2 : 4;
     // default-priority, non-blocking streams
     if (hipStreamCreateWithFlags(&c->s_ingest, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->s_req, hipStreamNonBlocking) != hipSuccess ||
@@ -349,7 +351,8 @@ int mr_ctx_destroy(mr_ctx* c) {
     hipSetDevice(c->device);
     hipDeviceSynchronize();
     free_geometry(c);
-    void* bufs[] = {c->d_stage, c->d_fstage, c->d_vmax, c->d_fv, c->d_req, c->d_out, c->d_pts, c->d_misc};
+    void* bufs[] = {c->d_stage, c->d_fstage, c->d_vmax, c->d_fv, c->d_req, c->d_out, c->d_pts, c->d_misc,
+                    c->d_pre, c->d_recs};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (c->h_req) hipHostFree(c->h_req);
@@ -750,6 +753,220 @@ int mr_filter_adjoint(mr_ctx* c, const int64_t* indices, int k, const float* gq,
     if (!graw_on_device)
         MR_HIP(hipMemcpyAsync(graw, d_o, bytes, hipMemcpyDeviceToHost, c->s_ingest));
     MR_HIP(hipStreamSynchronize(c->s_ingest));
+    return MR_OK;
+}
+
+static int64_t task_candidates(const mr_task& T) {
+    if (T.nq <= 0 || T.np <= 0 || T.ns <= 0) return 0;
+    return static_cast<int64_t>(T.nq) * ((T.np + BP_TP - 1) / BP_TP) * ((T.ns + BP_TS - 1) / BP_TS);
+}
+
+static void plan_free(Plan* p) {
+    for (auto& g : p->graphs) hipGraphExecDestroy(g.second);
+    p->graphs.clear();
+    void* dev[] = {p->d_tasks, p->d_toff, p->d_queue, p->d_counters, p->d_out, p->d_block, p->d_meta};
+    for (void* d : dev)
+        if (d) hipFree(d);
+    if (p->h_block) hipHostFree(p->h_block);
+    if (p->h_meta) hipHostFree(p->h_meta);
+}
+
+int mr_plan_create(mr_ctx* c, const mr_task* tasks, int ntasks, int64_t out_len, mr_plan** out) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    MR_CHECK(out != nullptr, MR_EVALUE, "null output");
+    *out = nullptr;
+    std::lock_guard<std::mutex> lk(c->mu);
+    MR_CHECK(c->kind != MR_KIND_NONE, MR_ERUNTIME, "no geometry set");
+    MR_CHECK(ntasks >= 1 && tasks, MR_EVALUE, "a plan needs at least one task");
+    MR_CHECK(out_len >= 1, MR_EVALUE, "a plan needs an output");
+    MR_CHECK(c->bp_variant == 2 || c->bp_variant == 4, MR_ERUNTIME,
+             "plans run the persistent v2 back-projection kernel");
+    rc = validate_tasks(tasks, ntasks, out_len);
+    if (rc) return rc;
+    MR_HIP(hipSetDevice(c->device));
+    mr_plan* p = new mr_plan();
+    p->c = c;
+    p->ntasks = ntasks;
+    p->out_len = out_len;
+    std::vector<int64_t> toff(ntasks + 1, 0);
+    for (int i = 0; i < ntasks; ++i) toff[i + 1] = toff[i] + task_candidates(tasks[i]);
+    p->total_candidates = toff[ntasks];
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+    chk(hipMalloc(reinterpret_cast<void**>(&p->d_tasks), sizeof(mr_task) * ntasks));
+    chk(hipMalloc(reinterpret_cast<void**>(&p->d_toff), sizeof(int64_t) * (ntasks + 1)));
+    chk(hipMalloc(reinterpret_cast<void**>(&p->d_queue), sizeof(Tile) * std::max<int64_t>(p->total_candidates, 1)));
+    chk(hipMalloc(reinterpret_cast<void**>(&p->d_counters), sizeof(int) * 4));
+    chk(hipMalloc(reinterpret_cast<void**>(&p->d_out), sizeof(float) * out_len));
+    if (e == hipSuccess) {
+        chk(hipMemcpy(p->d_tasks, tasks, sizeof(mr_task) * ntasks, hipMemcpyHostToDevice));
+        chk(hipMemcpy(p->d_toff, toff.data(), sizeof(int64_t) * (ntasks + 1), hipMemcpyHostToDevice));
+        chk(hipMemset(p->d_out, 0, sizeof(float) * out_len));
+        const int nv[2] = {c->n, 0};
+        chk(hipMemcpy(p->d_counters + 2, nv, sizeof(int) * 2, hipMemcpyHostToDevice));
+    }
+    if (e != hipSuccess) {
+        plan_free(p);
+        delete p;
+        return fail(MR_ERUNTIME, std::string("plan allocation: ") + hipGetErrorString(e));
+    }
+    rc = launch_enumerate_tiles(p->d_tasks, p->d_toff, ntasks, p->total_candidates, p->d_queue,
+                                p->d_counters, c->s_req);
+    if (!rc) {
+        e = hipStreamSynchronize(c->s_req);
+        if (e != hipSuccess) rc = fail(MR_ERUNTIME, std::string("plan queue: ") + hipGetErrorString(e));
+    }
+    if (rc) {
+        plan_free(p);
+        delete p;
+        return rc;
+    }
+    p->grid = persistent_grid(c);
+    *out = p;
+    return MR_OK;
+}
+
+int mr_plan_destroy(mr_plan* p) {
+    if (!p) return MR_OK;
+    {
+        std::lock_guard<std::mutex> lk(p->c->mu);
+        hipSetDevice(p->c->device);
+        hipStreamSynchronize(p->c->s_req);
+        plan_free(p);
+    }
+    delete p;
+    return MR_OK;
+}
+
+static int plan_copy_out(mr_plan* p, float* out_host) {
+    if (out_host)
+        MR_HIP(hipMemcpyAsync(out_host, p->d_out, sizeof(float) * p->out_len, hipMemcpyDeviceToHost,
+                              p->c->s_req));
+    return MR_OK;
+}
+
+int mr_plan_run(mr_plan* p, int n_views, float* out_host) {
+    MR_CHECK(p != nullptr, MR_EVALUE, "null plan");
+    Ctx* c = p->c;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (n_views < 0) n_views = c->n;
+    MR_CHECK(n_views <= c->n, MR_EVALUE, "n_views beyond the stored views");
+    MR_HIP(hipSetDevice(c->device));
+    TileQueue q;
+    q.head = p->d_counters + 1;
+    q.count = p->d_counters;
+    q.grid = p->grid;
+    const float scale = n_views > 0 ? static_cast<float>(M_PI / n_views) : 0.0f;
+    MR_HIP(hipEventRecord(c->ev0, c->s_req));
+    int rc = launch_reset_head(p->d_counters, c->s_req);
+    if (!rc) rc = launch_bp_tiles(c, p->d_tasks, p->d_queue, 0, n_views, scale, p->d_out, c->s_req, &q);
+    if (rc) return rc;
+    MR_HIP(hipEventRecord(c->ev1, c->s_req));
+    rc = plan_copy_out(p, out_host);
+    if (rc) return rc;
+    rc = finish_timing(c);
+    p->last_ms = c->last_kernel_ms;
+    return rc;
+}
+
+int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block, float* out_host) {
+    MR_CHECK(p != nullptr, MR_EVALUE, "null plan");
+    Ctx* c = p->c;
+    std::lock_guard<std::mutex> lk(c->mu);
+    MR_CHECK(k >= 1 && indices && block, MR_EVALUE, "empty or null batch");
+    MR_CHECK(c->n + k <= c->n_total, MR_EVALUE, "ring full");
+    MR_CHECK(c->gaussian <= 0.0 && std::fabs(c->tilt) < 1e-9, MR_ERUNTIME,
+             "streaming plans do not preprocess; push the batch with mr_push_views");
+    for (int j = 0; j < k; ++j)
+        MR_CHECK(indices[j] >= 0 && indices[j] < c->n_total, MR_EVALUE, "view index out of range");
+    MR_HIP(hipSetDevice(c->device));
+    const size_t blk = sizeof(float) * static_cast<size_t>(c->R) * k * c->C;
+    const size_t meta = (sizeof(FilterView) + sizeof(ViewGeom)) * k;
+    if (k > p->kmax) {
+        // staging grows: existing graphs captured the old addresses
+        for (auto& g : p->graphs) hipGraphExecDestroy(g.second);
+        p->graphs.clear();
+        if (p->h_block) hipHostFree(p->h_block);
+        if (p->h_meta) hipHostFree(p->h_meta);
+        if (p->d_block) hipFree(p->d_block);
+        if (p->d_meta) hipFree(p->d_meta);
+        p->h_block = nullptr; p->h_meta = nullptr; p->d_block = nullptr; p->d_meta = nullptr;
+        p->kmax = 0;
+        MR_HIP(hipHostMalloc(reinterpret_cast<void**>(&p->h_block), blk, hipHostMallocDefault));
+        MR_HIP(hipHostMalloc(&p->h_meta, meta, hipHostMallocDefault));
+        MR_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_block), blk));
+        MR_HIP(hipMalloc(&p->d_meta, meta));
+        p->kmax = k;
+    }
+    int rc = 0;
+    if (c->storage == MR_STORE_F16) {
+        rc = ensure(reinterpret_cast<void**>(&c->d_fstage), &c->fstage_bytes, blk);
+        if (!rc) rc = ensure(reinterpret_cast<void**>(&c->d_vmax), &c->vmax_len, sizeof(unsigned) * k);
+        if (rc) return rc;
+    }
+    // stage the batch and its metadata in pinned memory (fixed addresses)
+    std::memcpy(p->h_block, block, blk);
+    FilterView* fv = static_cast<FilterView*>(p->h_meta);
+    ViewGeom* gv = reinterpret_cast<ViewGeom*>(static_cast<char*>(p->h_meta) + sizeof(FilterView) * k);
+    for (int j = 0; j < k; ++j) {
+        fv[j] = c->filt_all[indices[j]];
+        fv[j].slot = c->n + j;
+        gv[j] = c->geom_all[indices[j]];
+    }
+    // the device view count follows the host one (other ingest paths may have run)
+    MR_HIP(hipMemcpyAsync(p->d_counters + 2, &c->n, sizeof(int), hipMemcpyHostToDevice, c->s_req));
+    MR_HIP(hipStreamSynchronize(c->s_req));
+    hipGraphExec_t exec = nullptr;
+    for (auto& g : p->graphs)
+        if (g.first == k) exec = g.second;
+    if (!exec) {
+        TileQueue q;
+        q.head = p->d_counters + 1;
+        q.count = p->d_counters;
+        q.n_views = p->d_counters + 2;
+        q.grid = p->grid;
+        hipGraph_t graph = nullptr;
+        MR_HIP(hipStreamBeginCapture(c->s_req, hipStreamCaptureModeThreadLocal));
+        hipError_t e1 = hipMemcpyAsync(p->d_block, p->h_block, blk, hipMemcpyHostToDevice, c->s_req);
+        hipError_t e2 = hipMemcpyAsync(p->d_meta, p->h_meta, meta, hipMemcpyHostToDevice, c->s_req);
+        int r = (e1 != hipSuccess || e2 != hipSuccess) ? MR_ERUNTIME : MR_OK;
+        if (!r) r = launch_stream_prep(c, p->d_meta, k, p->d_counters + 2, c->s_req);
+        if (!r) r = launch_filter(c, p->d_block, k, 0, c->s_req, static_cast<const FilterView*>(p->d_meta));
+        if (!r) r = launch_reset_head(p->d_counters, c->s_req);
+        if (!r) r = launch_bp_tiles(c, p->d_tasks, p->d_queue, 0, 0, 0.0f, p->d_out, c->s_req, &q);
+        hipError_t ec = hipStreamEndCapture(c->s_req, &graph);
+        if (r || ec != hipSuccess) {
+            if (graph) hipGraphDestroy(graph);
+            return r ? r : fail(MR_ERUNTIME, std::string("graph capture: ") + hipGetErrorString(ec));
+        }
+        hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (ei != hipSuccess) return fail(MR_ERUNTIME, std::string("graph instantiate: ") + hipGetErrorString(ei));
+        p->graphs.emplace_back(k, exec);
+    }
+    MR_HIP(hipEventRecord(c->ev0, c->s_req));
+    MR_HIP(hipGraphLaunch(exec, c->s_req));
+    MR_HIP(hipEventRecord(c->ev1, c->s_req));
+    rc = plan_copy_out(p, out_host);
+    if (rc) return rc;
+    rc = finish_timing(c);
+    if (rc) return rc;
+    p->last_ms = c->last_kernel_ms;
+    c->n += k;
+    return MR_OK;
+}
+
+int mr_plan_output(mr_plan* p, float** out_device, int64_t* out_len) {
+    MR_CHECK(p != nullptr, MR_EVALUE, "null plan");
+    if (out_device) *out_device = p->d_out;
+    if (out_len) *out_len = p->out_len;
+    return MR_OK;
+}
+
+int mr_plan_last_ms(mr_plan* p, double* ms) {
+    MR_CHECK(p && ms, MR_EVALUE, "null argument");
+    *ms = p->last_ms;
     return MR_OK;
 }
 

@@ -301,13 +301,32 @@ template <bool CONE, typename ST>
 __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel2(
     const ST* __restrict__ ring, int64_t view_stride, int pitch, int R, int C,
     const ViewGeom* __restrict__ geom, const float* __restrict__ vscale,
-    const mr_task* __restrict__ tasks, const Tile* __restrict__ tiles, int n_views,
-    float scale, float* __restrict__ out) {
+    const mr_task* __restrict__ tasks, const Tile* __restrict__ tiles, int n_views_arg,
+    float scale_arg, float* __restrict__ out, int* __restrict__ qhead,
+    const int* __restrict__ qcount, const int* __restrict__ n_views_dev) {
     __shared__ float4 rec4[BP_VPASS * BP_REC / 4];
+    __shared__ int s_ti;
     float* rec = reinterpret_cast<float*>(rec4);
     constexpr int LE = Taps<ST>::LOG_ESZ;
-
-    const Tile tl = tiles[blockIdx.x];
+    // persistent mode: tiles come from a device queue (qhead != null) and the view
+    // count may come from device memory, so that a captured graph can be replayed
+    const int n_views = n_views_dev ? *n_views_dev : n_views_arg;
+    const float scale = n_views_dev ? (n_views > 0 ? static_cast<float>(3.141592653589793 / n_views) : 0.0f)
+                                    : scale_arg;
+    const int ntile = qhead ? *qcount : 0;
+    for (int it = 0;; ++it) {
+    int ti;
+    if (qhead) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_ti = atomicAdd(qhead, 1);
+        __syncthreads();
+        ti = s_ti;
+        if (ti >= ntile) break;
+    } else {
+        if (it > 0) break;
+        ti = blockIdx.x;
+    }
+    const Tile tl = tiles[ti];
     const mr_task& T = tasks[tl.task];
     const int t = threadIdx.x;
     const int tx = t & (BP_TS - 1);
@@ -417,6 +436,7 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel2(
         out[obase + static_cast<int64_t>(p) * T.stride_p + static_cast<int64_t>(s) * T.stride_s] =
             inside[m] ? val * scale : 0.0f;
     }
+    }   // tile loop
 }
 
 // ---------------------------------------------------------------- v3: LDS-staged taps
@@ -660,6 +680,162 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel3(
     }
 }
 
+// ---------------------------------------------------------------- v4: precomputed records
+// The float64 per-(tile, view) records are computed by their own kernel into global
+// memory ([tile][view][16] floats), so the back-projection kernel reads them with
+// scalar loads (uniform per wave): no LDS, no barriers, no float64 in the hot loop,
+// fewer VGPRs and more waves.  The view loop is software-pipelined: the four taps of
+// view v+1 are requested before view v is interpolated, doubling the gathers in flight
+// per wave.  PPT points per thread (rows ty + STEP*m of a 32 x 32 tile).
+__global__ __launch_bounds__(256) void records_kernel(const ViewGeom* __restrict__ geom,
+                                                      const float* __restrict__ vscale,
+                                                      const mr_task* __restrict__ tasks,
+                                                      const Tile* __restrict__ tiles, int ntiles,
+                                                      int n_views, int R, int C, int pitch,
+                                                      int cone, float* __restrict__ recs) {
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (gid >= static_cast<int64_t>(ntiles) * n_views) return;
+    const int ti = static_cast<int>(gid / n_views);
+    const int v = static_cast<int>(gid - static_cast<int64_t>(ti) * n_views);
+    const Tile tl = tiles[ti];
+    const mr_task& T = tasks[tl.task];
+    const int p0 = tl.tp * BP_TP, s0 = tl.ts * BP_TS;
+    double xt[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) xt[d] = T.o[d] + tl.q * T.c[d] + p0 * T.a[d] + s0 * T.b[d];
+    make_record(geom + v, vscale[v], T, xt, R, C, pitch, cone != 0, recs + gid * BP_REC);
+}
+
+template <int PPT>
+struct PointState {
+    float fc[PPT], fr[PPT], wt[PPT];
+    float a[PPT], b[PPT], c[PPT], d[PPT];
+};
+
+template <bool CONE, typename ST, int PPT>
+__global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel4(
+    const ST* __restrict__ ring, int64_t view_stride, int pitch,
+    const float* __restrict__ recs, const mr_task* __restrict__ tasks,
+    const Tile* __restrict__ tiles, int n_views, float scale, float* __restrict__ out) {
+    constexpr int LE = Taps<ST>::LOG_ESZ;
+    constexpr int PSTEP = BP_THREADS / BP_TS;        // 8 rows between a thread's points
+    constexpr int SUBS = BP_TP / (PSTEP * PPT);      // workgroups per 32-row tile
+    static_assert(SUBS * PSTEP * PPT == BP_TP, "tile shape");
+    const int tile = blockIdx.x / SUBS;
+    const int sub = blockIdx.x - tile * SUBS;
+    const Tile tl = tiles[tile];
+    const mr_task& T = tasks[tl.task];
+    const int t = threadIdx.x;
+    const int tx = t & (BP_TS - 1);
+    const int ty = t / BP_TS + sub * PSTEP * PPT;    // first row of this thread in the tile
+    const int p0 = tl.tp * BP_TP, s0 = tl.ts * BP_TS;
+    const int s = s0 + tx;
+    bool valid[PPT], inside[PPT];
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) {
+        const int p = p0 + ty + PSTEP * m;
+        valid[m] = (p < T.np) && (s < T.ns);
+        inside[m] = valid[m] && (!T.mask_box || in_box(T, p, s));
+    }
+    const float ds = static_cast<float>(tx), dp0 = static_cast<float>(ty);
+    const int pitchb = pitch << LE;
+    const int view_bytes = static_cast<int>(view_stride << LE);
+    const float* __restrict__ rt = recs + static_cast<int64_t>(tile) * n_views * BP_REC;
+
+    float acc[PPT], accb[PPT], frp[PPT];
+    int roffp[PPT];
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) { acc[m] = 0.0f; accb[m] = 0.0f; frp[m] = 0.0f; roffp[m] = 0; }
+    if (!CONE && n_views > 0) {
+        // parallel beam: the row of a point is the same for every view
+#pragma unroll
+        for (int m = 0; m < PPT; ++m) {
+            const float dp = dp0 + static_cast<float>(PSTEP * m);
+            float row = fmaf(dp, rt[5], fmaf(ds, rt[4], rt[3]));
+            row = __builtin_amdgcn_fmed3f(row, rt[11], rt[12]);
+            frp[m] = __builtin_amdgcn_fractf(row);
+            roffp[m] = cvt_flr(row) * pitchb;
+        }
+    }
+
+    // request the taps of view v into st
+    auto issue = [&](int v, PointState<PPT>& st) {
+        const float* r = rt + static_cast<int64_t>(v) * BP_REC;
+        const float U0 = r[0], Ua = r[1], Ub = r[2];
+        const int baseb = __float_as_int(r[14]) << LE;
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<ST*>(ring + static_cast<int64_t>(v) * view_stride), 0, view_bytes, 0x00020000);
+        const float uh0 = fmaf(dp0, Ub, fmaf(ds, Ua, U0));
+        if (CONE) {
+            const float V0 = r[3], Va = r[4], Vb = r[5], W0 = r[6], Wa = r[7], Wb = r[8];
+            const float vh0 = fmaf(dp0, Vb, fmaf(ds, Va, V0));
+            const float wh0 = fmaf(dp0, Wb, fmaf(ds, Wa, W0));
+#pragma unroll
+            for (int m = 0; m < PPT; ++m) {
+                const float dm = static_cast<float>(PSTEP * m);
+                const float uh = m ? fmaf(dm, Ub, uh0) : uh0;
+                const float vh = m ? fmaf(dm, Vb, vh0) : vh0;
+                const float wh = m ? fmaf(dm, Wb, wh0) : wh0;
+                const float rw = __builtin_amdgcn_rcpf(wh);
+                const float col = __builtin_amdgcn_fmed3f(uh * rw, r[9], r[10]);
+                const float row = __builtin_amdgcn_fmed3f(vh * rw, r[11], r[12]);
+                st.wt[m] = rw * rw;
+                st.fc[m] = __builtin_amdgcn_fractf(col);
+                st.fr[m] = __builtin_amdgcn_fractf(row);
+                const int voff = __mul24(cvt_flr(row), pitchb) + ((cvt_flr(col) << LE) + baseb);
+                Taps<ST>::load(rs, voff, pitchb, st.a[m], st.b[m], st.c[m], st.d[m]);
+            }
+        } else {
+            st.wt[0] = r[13];
+#pragma unroll
+            for (int m = 0; m < PPT; ++m) {
+                const float dm = static_cast<float>(PSTEP * m);
+                const float col = __builtin_amdgcn_fmed3f(m ? fmaf(dm, Ub, uh0) : uh0, r[9], r[10]);
+                st.fc[m] = __builtin_amdgcn_fractf(col);
+                const int voff = roffp[m] + ((cvt_flr(col) << LE) + baseb);
+                Taps<ST>::load(rs, voff, pitchb, st.a[m], st.b[m], st.c[m], st.d[m]);
+            }
+        }
+    };
+    auto consume = [&](const PointState<PPT>& st) {
+#pragma unroll
+        for (int m = 0; m < PPT; ++m) {
+            const float top = fmaf(st.fc[m], st.b[m] - st.a[m], st.a[m]);
+            const float bot = fmaf(st.fc[m], st.d[m] - st.c[m], st.c[m]);
+            if (CONE) {
+                acc[m] = fmaf(st.wt[m], fmaf(st.fr[m], bot - top, top), acc[m]);
+            } else {
+                acc[m] = fmaf(st.wt[0], top, acc[m]);
+                accb[m] = fmaf(st.wt[0], bot, accb[m]);
+            }
+        }
+    };
+
+    if (n_views > 0) {
+        PointState<PPT> s0st, s1st;
+        issue(0, s0st);
+        int v = 0;
+        for (; v + 2 <= n_views; v += 2) {
+            if (v + 1 < n_views) issue(v + 1, s1st);
+            consume(s0st);
+            if (v + 2 < n_views) issue(v + 2, s0st);
+            consume(s1st);
+        }
+        if (v < n_views) consume(s0st);
+    }
+
+    const int64_t obase = T.out_offset + static_cast<int64_t>(tl.q) * T.stride_q;
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) {
+        if (!valid[m]) continue;
+        const int p = p0 + ty + PSTEP * m;
+        float val = acc[m];
+        if (!CONE) val = fmaf(frp[m], accb[m] - acc[m], acc[m]);
+        out[obase + static_cast<int64_t>(p) * T.stride_p + static_cast<int64_t>(s) * T.stride_s] =
+            inside[m] ? val * scale : 0.0f;
+    }
+}
+
 // ---------------------------------------------------------------- adjoint and shift derivative
 // Transpose of the tile back-projection with respect to the stored (dequantised) views:
 // every point scatters scale * w_k * g_p with the forward's bilinear weights into
@@ -890,8 +1066,9 @@ __global__ void read_views_kernel(const ST* __restrict__ ring, int64_t view_stri
 
 template <bool CONE>
 int dispatch_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles, int n_views,
-                   float scale, float* d_out, hipStream_t s) {
-    dim3 grid(ntiles), block(BP_THREADS);
+                   float scale, float* d_out, hipStream_t s, const TileQueue& q) {
+    // persistent launches size the grid to the resident capacity and pull tiles from q
+    dim3 grid(q.head ? q.grid : ntiles), block(BP_THREADS);
     if (c->bp_variant == 3) {
         switch (c->storage) {
         case MR_STORE_F32:
@@ -919,18 +1096,20 @@ int dispatch_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntil
         case MR_STORE_F32:
             hipLaunchKernelGGL((bp_tiles_kernel2<CONE, float>), grid, block, 0, s,
                                static_cast<const float*>(c->ring), c->view_stride, c->pitch, c->R,
-                               c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out);
+                               c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out,
+                               q.head, q.count, q.n_views);
             break;
         case MR_STORE_F16:
             hipLaunchKernelGGL((bp_tiles_kernel2<CONE, __half>), grid, block, 0, s,
                                static_cast<const __half*>(c->ring), c->view_stride, c->pitch, c->R,
-                               c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out);
+                               c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out,
+                               q.head, q.count, q.n_views);
             break;
         default:
             hipLaunchKernelGGL((bp_tiles_kernel2<CONE, uint16_t>), grid, block, 0, s,
                                static_cast<const uint16_t*>(c->ring), c->view_stride, c->pitch,
                                c->R, c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale,
-                               d_out);
+                               d_out, q.head, q.count, q.n_views);
             break;
         }
         MR_HIP(hipGetLastError());
@@ -984,12 +1163,118 @@ int dispatch_points(Ctx* c, const double* d_pts, int64_t M, double shift, int n_
 
 }  // namespace
 
+int launch_bp_tiles_v4(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles, int n_views,
+                       float scale, float* d_out, hipStream_t s);
+
 int launch_bp_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles, int n_views,
-                    float scale, float* d_out, hipStream_t s) {
-    if (ntiles <= 0) return MR_OK;
+                    float scale, float* d_out, hipStream_t s, const TileQueue* q) {
+    const TileQueue none{};
+    if (!q && ntiles <= 0) return MR_OK;
+    if (!q && c->bp_variant == 4 && n_views > 0)
+        return launch_bp_tiles_v4(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s);
+    if (q && c->bp_variant != 2 && c->bp_variant != 4)
+        return fail(MR_ERUNTIME, "persistent launches need the v2 kernel");
+    if (q) {                               // persistent launches always run the v2 loop
+        const int saved = c->bp_variant;
+        c->bp_variant = 2;
+        const int rc = c->kind == MR_KIND_CONE
+                           ? dispatch_tiles<true>(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s, *q)
+                           : dispatch_tiles<false>(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s, *q);
+        c->bp_variant = saved;
+        return rc;
+    }
     return c->kind == MR_KIND_CONE
-               ? dispatch_tiles<true>(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s)
-               : dispatch_tiles<false>(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s);
+               ? dispatch_tiles<true>(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s, q ? *q : none)
+               : dispatch_tiles<false>(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s, q ? *q : none);
+}
+
+// Device work queue: one thread per candidate tile of every task (prefix offsets per
+// task), the conservative box test of the host path, block-ordered appends.
+__global__ __launch_bounds__(256) void enumerate_tiles_kernel(const mr_task* __restrict__ tasks,
+                                                              const int64_t* __restrict__ toff,
+                                                              int ntasks, Tile* __restrict__ queue,
+                                                              int* __restrict__ count) {
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    const int64_t total = toff[ntasks];
+    bool hit = false;
+    Tile tl{0, 0, 0, 0};
+    if (gid < total) {
+        int lo = 0, hi = ntasks - 1;                // last task with toff[task] <= gid
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (toff[mid] <= gid) lo = mid; else hi = mid - 1;
+        }
+        const mr_task& T = tasks[lo];
+        const int ntp = (T.np + BP_TP - 1) / BP_TP, nts = (T.ns + BP_TS - 1) / BP_TS;
+        const int64_t loc = gid - toff[lo];
+        const int q = static_cast<int>(loc / (static_cast<int64_t>(ntp) * nts));
+        const int rem = static_cast<int>(loc - static_cast<int64_t>(q) * ntp * nts);
+        const int tp = rem / nts, ts = rem - tp * nts;
+        tl = Tile{lo, q, tp, ts};
+        hit = true;
+        if (T.mask_box) {
+            const int p0 = tp * BP_TP, s0 = ts * BP_TS;
+            const int p1 = min(p0 + BP_TP, T.np) - 1, s1 = min(s0 + BP_TS, T.ns) - 1;
+            for (int d = 0; d < 3 && hit; ++d) {
+                const double lim = T.box[d] * (1.0 + 1e-9) + 1e-9;
+                bool all_hi = true, all_lo = true;
+                for (int cp = 0; cp < 2; ++cp)
+                    for (int cs = 0; cs < 2; ++cs) {
+                        const int p = cp ? p1 : p0, sx = cs ? s1 : s0;
+                        const double x = T.o[d] + q * T.c[d] + p * T.a[d] + sx * T.b[d];
+                        all_hi = all_hi && (x > lim);
+                        all_lo = all_lo && (x < -lim);
+                    }
+                if (all_hi || all_lo) hit = false;
+            }
+        }
+    }
+    // block-level prefix of the hits, one atomic per block
+    __shared__ int s_pos[256];
+    __shared__ int s_base;
+    s_pos[threadIdx.x] = hit ? 1 : 0;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int v = threadIdx.x >= off ? s_pos[threadIdx.x - off] : 0;
+        __syncthreads();
+        s_pos[threadIdx.x] += v;
+        __syncthreads();
+    }
+    if (threadIdx.x == 255) s_base = atomicAdd(count, s_pos[255]);
+    __syncthreads();
+    if (hit) queue[s_base + s_pos[threadIdx.x] - 1] = tl;
+}
+
+__global__ void fill_int_kernel(int* __restrict__ p, int n, int value) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+int launch_enumerate_tiles(const mr_task* d_tasks, const int64_t* d_toff, int ntasks,
+                           int64_t total, Tile* d_queue, int* d_counters, hipStream_t s) {
+    // counters: [0] queue length, [1] head; cleared with a kernel (graph-safe)
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, d_counters, 2, 0);
+    if (total > 0)
+        hipLaunchKernelGGL(enumerate_tiles_kernel, dim3(static_cast<unsigned>((total + 255) / 256)),
+                           dim3(256), 0, s, d_tasks, d_toff, ntasks, d_queue, d_counters);
+    MR_HIP(hipGetLastError());
+    return MR_OK;
+}
+
+int persistent_grid(Ctx* c) {
+    int cus = 0, per = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess)
+        cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, bp_tiles_kernel2<true, float>,
+                                                     BP_THREADS, 0) != hipSuccess || per < 1)
+        per = 4;
+    return cus * per;
+}
+
+int launch_reset_head(int* d_counters, hipStream_t s) {
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, d_counters + 1, 1, 0);
+    MR_HIP(hipGetLastError());
+    return MR_OK;
 }
 
 int launch_bp_points(Ctx* c, const double* d_pts, int64_t M, double cor_shift, int n_views,
@@ -998,6 +1283,57 @@ int launch_bp_points(Ctx* c, const double* d_pts, int64_t M, double cor_shift, i
     return c->kind == MR_KIND_CONE
                ? dispatch_points<true>(c, d_pts, M, cor_shift, n_views, scale, d_out, s)
                : dispatch_points<false>(c, d_pts, M, cor_shift, n_views, scale, d_out, s);
+}
+
+template <bool CONE, typename ST>
+static void launch_v4_kernel(Ctx* c, const float* recs, const mr_task* d_tasks, const Tile* d_tiles,
+                             int nt, int n_views, float scale, float* d_out, hipStream_t s) {
+    if (c->bp_ppt == 2)
+        hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, 2>), dim3(2 * nt), dim3(BP_THREADS), 0, s,
+                           static_cast<const ST*>(c->ring), c->view_stride, c->pitch, recs, d_tasks,
+                           d_tiles, n_views, scale, d_out);
+    else
+        hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, 4>), dim3(nt), dim3(BP_THREADS), 0, s,
+                           static_cast<const ST*>(c->ring), c->view_stride, c->pitch, recs, d_tasks,
+                           d_tiles, n_views, scale, d_out);
+}
+
+int launch_bp_tiles_v4(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles, int n_views,
+                       float scale, float* d_out, hipStream_t s) {
+    if (ntiles <= 0 || n_views <= 0) {
+        // no views: every point is zero; run with one view count of 0 through v2
+        return launch_bp_tiles(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s);
+    }
+    const size_t per_tile = sizeof(float) * BP_REC * static_cast<size_t>(n_views);
+    const size_t budget = static_cast<size_t>(512) << 20;          // record buffer <= 512 MB
+    int chunk = static_cast<int>(std::max<size_t>(1, budget / per_tile));
+    chunk = std::min(chunk, ntiles);
+    int rc = ensure(reinterpret_cast<void**>(&c->d_recs), &c->recs_bytes, per_tile * chunk);
+    if (rc) return rc;
+    for (int t0 = 0; t0 < ntiles; t0 += chunk) {
+        const int nt = std::min(chunk, ntiles - t0);
+        const int64_t nrec = static_cast<int64_t>(nt) * n_views;
+        hipLaunchKernelGGL(records_kernel, dim3(static_cast<unsigned>((nrec + 255) / 256)), dim3(256),
+                           0, s, c->d_geom, c->d_vscale, d_tasks, d_tiles + t0, nt, n_views, c->R,
+                           c->C, c->pitch, c->kind == MR_KIND_CONE ? 1 : 0, c->d_recs);
+        const bool cone = c->kind == MR_KIND_CONE;
+        switch (c->storage) {
+        case MR_STORE_F32:
+            cone ? launch_v4_kernel<true, float>(c, c->d_recs, d_tasks, d_tiles + t0, nt, n_views, scale, d_out, s)
+                 : launch_v4_kernel<false, float>(c, c->d_recs, d_tasks, d_tiles + t0, nt, n_views, scale, d_out, s);
+            break;
+        case MR_STORE_F16:
+            cone ? launch_v4_kernel<true, __half>(c, c->d_recs, d_tasks, d_tiles + t0, nt, n_views, scale, d_out, s)
+                 : launch_v4_kernel<false, __half>(c, c->d_recs, d_tasks, d_tiles + t0, nt, n_views, scale, d_out, s);
+            break;
+        default:
+            cone ? launch_v4_kernel<true, uint16_t>(c, c->d_recs, d_tasks, d_tiles + t0, nt, n_views, scale, d_out, s)
+                 : launch_v4_kernel<false, uint16_t>(c, c->d_recs, d_tasks, d_tiles + t0, nt, n_views, scale, d_out, s);
+            break;
+        }
+        MR_HIP(hipGetLastError());
+    }
+    return MR_OK;
 }
 
 int launch_bp_adjoint(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles,

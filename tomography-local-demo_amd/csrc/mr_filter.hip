@@ -230,6 +230,25 @@ __global__ void store_kernel(const float* __restrict__ q, int R, int C,
     }
 }
 
+__global__ void zero_u32_kernel(unsigned* __restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// Streaming update, before filtering: copy the staged per-view geometry of a batch into
+// its ring slots and advance the device view count by k (graph-replayable: every
+// address is fixed, the slots come from the staged metadata).
+__global__ void stream_prep_kernel(const FilterView* __restrict__ fv, const ViewGeom* __restrict__ gv,
+                                   int k, ViewGeom* __restrict__ geom, float* __restrict__ vscale,
+                                   int* __restrict__ nviews, int store_f16) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < k) {
+        geom[fv[j].slot] = gv[j];
+        if (!store_f16) vscale[fv[j].slot] = 1.0f;
+    }
+    if (j == 0) atomicAdd(nviews, k);
+}
+
 // Per-view max |q| of q [k][R][C] (F16 quantisation of stored views).
 __global__ void vmax_kernel(const float* __restrict__ q, int R, int C, unsigned* __restrict__ vmax) {
     const int v = blockIdx.y, r = blockIdx.x;
@@ -276,23 +295,26 @@ int log2i(int N) {
 
 }  // namespace
 
-int launch_filter(Ctx* c, const float* d_block, int k, int n0, hipStream_t s) {
+int launch_filter(Ctx* c, const float* d_block, int k, int n0, hipStream_t s,
+                  const FilterView* fv) {
     (void)n0;
+    if (!fv) fv = c->d_fv;
     const int logn = log2i(c->N);
     dim3 grid((c->R + 1) / 2, k);
     if (c->storage == MR_STORE_F32)
-        return launch_mode<0>(logn, grid, s, d_block, c->R, k, c->C, c->d_fv, c->d_H, c->d_tw,
+        return launch_mode<0>(logn, grid, s, d_block, c->R, k, c->C, fv, c->d_H, c->d_tw,
                               c->ring, c->pitch, c->view_stride, nullptr);
     if (c->storage == MR_STORE_BF16)
-        return launch_mode<1>(logn, grid, s, d_block, c->R, k, c->C, c->d_fv, c->d_H, c->d_tw,
+        return launch_mode<1>(logn, grid, s, d_block, c->R, k, c->C, fv, c->d_H, c->d_tw,
                               c->ring, c->pitch, c->view_stride, nullptr);
-    // F16: filter to float32 staging, then quantise with a per-view scale
-    MR_HIP(hipMemsetAsync(c->d_vmax, 0, sizeof(unsigned) * k, s));
-    int rc = launch_mode<2>(logn, grid, s, d_block, c->R, k, c->C, c->d_fv, c->d_H, c->d_tw,
+    // F16: filter to float32 staging, then quantise with a per-view scale (the max
+    // buffer is cleared with a kernel so that the sequence can be graph-captured)
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((k + 255) / 256), dim3(256), 0, s, c->d_vmax, k);
+    int rc = launch_mode<2>(logn, grid, s, d_block, c->R, k, c->C, fv, c->d_H, c->d_tw,
                             c->d_fstage, c->pitch, c->view_stride, c->d_vmax);
     if (rc) return rc;
     hipLaunchKernelGGL(quantize_f16_kernel, dim3(c->R, k), dim3(256), 0, s, c->d_fstage, c->R,
-                       c->C, c->d_fv, c->d_vmax, static_cast<__half*>(c->ring), c->pitch,
+                       c->C, fv, c->d_vmax, static_cast<__half*>(c->ring), c->pitch,
                        c->view_stride, c->d_vscale);
     MR_HIP(hipGetLastError());
     return MR_OK;
@@ -307,7 +329,7 @@ int launch_store(Ctx* c, const float* d_q, int k, hipStream_t s) {
         hipLaunchKernelGGL(store_kernel<1>, grid, block, 0, s, d_q, c->R, c->C, c->d_fv, c->ring,
                            c->pitch, c->view_stride);
     } else {
-        MR_HIP(hipMemsetAsync(c->d_vmax, 0, sizeof(unsigned) * k, s));
+        hipLaunchKernelGGL(zero_u32_kernel, dim3((k + 255) / 256), dim3(256), 0, s, c->d_vmax, k);
         hipLaunchKernelGGL(vmax_kernel, grid, block, 0, s, d_q, c->R, c->C, c->d_vmax);
         hipLaunchKernelGGL(quantize_f16_kernel, grid, block, 0, s, d_q, c->R, c->C, c->d_fv,
                            c->d_vmax, static_cast<__half*>(c->ring), c->pitch, c->view_stride,
@@ -321,6 +343,16 @@ int launch_filter_adjoint(Ctx* c, const float* d_gq, int k, float* d_graw, hipSt
     dim3 grid((c->R + 1) / 2, k);
     return launch_mode<4>(log2i(c->N), grid, s, d_gq, c->R, k, c->C, c->d_fv, c->d_H, c->d_tw,
                           d_graw, 0, 0, nullptr);
+}
+
+int launch_stream_prep(Ctx* c, const void* d_meta, int k, int* d_nviews, hipStream_t s) {
+    const FilterView* fv = static_cast<const FilterView*>(d_meta);
+    const ViewGeom* gv = reinterpret_cast<const ViewGeom*>(
+        static_cast<const char*>(d_meta) + sizeof(FilterView) * k);
+    hipLaunchKernelGGL(stream_prep_kernel, dim3((k + 63) / 64), dim3(64), 0, s, fv, gv, k, c->d_geom,
+                       c->d_vscale, d_nviews, c->storage == MR_STORE_F16 ? 1 : 0);
+    MR_HIP(hipGetLastError());
+    return MR_OK;
 }
 
 int launch_filter_rows_plain(Ctx* c, const float* d_in, int rows, int n, float gain,

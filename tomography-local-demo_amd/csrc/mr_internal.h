@@ -62,6 +62,14 @@ struct Tile {
     int32_t task, q, tp, ts;
 };
 
+// Device work queue of a persistent back-projection launch.
+struct TileQueue {
+    int* head = nullptr;             // atomic tile counter (null: one tile per workgroup)
+    const int* count = nullptr;      // number of queued tiles
+    const int* n_views = nullptr;    // view count read on the device (null: argument)
+    int grid = 0;                    // persistent grid size
+};
+
 // BP tile shape: TP x TS points per 256-thread workgroup, 4 points per thread.
 constexpr int BP_TS = 32;
 constexpr int BP_TP = 32;
@@ -77,6 +85,9 @@ struct Ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_kernel_ms = 0.0;
     int bp_variant = 1;              // back-projection inner loop (MINIRECAST_BP_VARIANT)
+    int bp_ppt = 4;                  // v4 points per thread (MINIRECAST_BP_PPT)
+    float* d_recs = nullptr;         // v4 per-(tile, view) records
+    size_t recs_bytes = 0;
 
     int kind = MR_KIND_NONE;
     int n_total = 0, R = 0, C = 0;
@@ -123,11 +134,36 @@ struct Ctx {
     size_t misc_bytes = 0;
 };
 
+// A compiled request: tasks and their device work queue, re-run on demand or after
+// every ingested batch as one captured HIP graph per batch size.
+struct Plan {
+    Ctx* c = nullptr;
+    int ntasks = 0;
+    int64_t out_len = 0;
+    int64_t total_candidates = 0;
+    mr_task* d_tasks = nullptr;
+    int64_t* d_toff = nullptr;
+    Tile* d_queue = nullptr;
+    int* d_counters = nullptr;       // [0] queue length, [1] head, [2] stored views
+    float* d_out = nullptr;          // plan output
+    int grid = 0;                    // persistent grid
+    // streaming update staging (pinned host, fixed device addresses for graph replay)
+    int kmax = 0;
+    float* h_block = nullptr;        // [R][kmax][C]
+    float* d_block = nullptr;
+    void* h_meta = nullptr;          // FilterView[kmax] + ViewGeom[kmax] + int slots[kmax]
+    void* d_meta = nullptr;
+    size_t meta_bytes = 0;
+    std::vector<std::pair<int, hipGraphExec_t>> graphs;   // per batch size
+    double last_ms = 0.0;
+};
+
 int ensure(void** p, size_t* have, size_t need);   // device buffer growth
 int ensure_host(void** p, size_t* have, size_t need);
 
 // ---------------------------------------------------------------- kernels (launchers)
-int launch_filter(Ctx* c, const float* d_block, int k, int n0, hipStream_t s);
+int launch_filter(Ctx* c, const float* d_block, int k, int n0, hipStream_t s,
+                  const FilterView* fv = nullptr);
 int launch_store(Ctx* c, const float* d_q, int k, hipStream_t s);
 int launch_preprocess(Ctx* c, float* d_a, float* d_b, int R, int k, int C, double gaussian,
                       double tilt, float** d_res, hipStream_t s);
@@ -135,7 +171,13 @@ int launch_filter_rows_plain(Ctx* c, const float* d_in, int rows, int n, float g
                              const float* d_H, const float2* d_tw, int N, float* d_out,
                              hipStream_t s);
 int launch_bp_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles,
-                    int n_views, float scale, float* d_out, hipStream_t s);
+                    int n_views, float scale, float* d_out, hipStream_t s,
+                    const TileQueue* q = nullptr);
+int launch_enumerate_tiles(const mr_task* d_tasks, const int64_t* d_toff, int ntasks,
+                           int64_t total, Tile* d_queue, int* d_counters, hipStream_t s);
+int launch_reset_head(int* d_counters, hipStream_t s);
+int launch_stream_prep(Ctx* c, const void* d_meta, int k, int* d_nviews, hipStream_t s);
+int persistent_grid(Ctx* c);
 int launch_bp_points(Ctx* c, const double* d_pts, int64_t M, double cor_shift, int n_views,
                      float scale, float* d_out, hipStream_t s);
 int launch_read_views(Ctx* c, int first, int count, float* d_out, hipStream_t s);

@@ -45,6 +45,12 @@ SIGNATURES = [
     ("mr_bp_tasks_dshift", _c.c_int, [_P, _P, _c.c_int, _c.c_int, _P, _c.c_int64, _c.c_int]),
     ("mr_filter_adjoint", _c.c_int, [_P, _P, _c.c_int, _P, _c.c_int, _P, _c.c_int]),
     ("mr_reset_views", _c.c_int, [_P]),
+    ("mr_plan_create", _c.c_int, [_P, _P, _c.c_int, _c.c_int64, _c.POINTER(_P)]),
+    ("mr_plan_destroy", _c.c_int, [_P]),
+    ("mr_plan_run", _c.c_int, [_P, _c.c_int, _P]),
+    ("mr_plan_stream", _c.c_int, [_P, _P, _c.c_int, _P, _P]),
+    ("mr_plan_output", _c.c_int, [_P, _c.POINTER(_P), _c.POINTER(_c.c_int64)]),
+    ("mr_plan_last_ms", _c.c_int, [_P, _c.POINTER(_c.c_double)]),
     ("mr_filter_rows", _c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_double, _c.c_int, _P]),
     ("mr_read_views", _c.c_int, [_P, _c.c_int, _c.c_int, _P]),
     ("mr_phantom_cone", _c.c_int, [_P, _P, _c.c_int, _P, _c.c_int, _c.c_int, _c.c_int, _P]),
@@ -76,6 +82,31 @@ class HipUnavailable(RuntimeError):
     """The HIP library or a GPU is not available."""
 
 
+def _share_torch_hip_runtime():
+    """Make this library and PyTorch use one HIP runtime.
+
+    The library needs libamdhip64.so.7.  PyTorch ships its own copy and its extension
+    modules load it by another file name; if the system copy were loaded first, the
+    process would hold two HIP/HSA runtimes and PyTorch's would not see the GPU.
+    Loading PyTorch's copy first (without importing torch) lets the library bind to it
+    through the shared soname.  MINIRECAST_SYSTEM_HIP=1 keeps the system runtime."""
+    if os.environ.get("MINIRECAST_SYSTEM_HIP") == "1":
+        return None
+    try:
+        import importlib.util
+        spec = importlib.util.find_spec("torch")
+    except Exception:
+        return None
+    if spec is None or not spec.submodule_search_locations:
+        return None
+    for root in spec.submodule_search_locations:
+        cand = os.path.join(root, "lib", "libamdhip64.so")
+        if os.path.exists(cand):
+            ctypes.CDLL(cand, mode=ctypes.RTLD_GLOBAL)
+            return cand
+    return None
+
+
 def load(path=None):
     """Load (once) and return the ctypes library; RuntimeError if it is missing."""
     global _lib
@@ -85,6 +116,7 @@ def load(path=None):
         p = path or os.environ.get("MINIRECAST_HIP_LIB", LIB_PATH)
         if not os.path.exists(p):
             raise HipUnavailable(f"{LIB_NAME} not built ({p}); run tomography-local-demo_amd/build.py")
+        _share_torch_hip_runtime()
         lib = ctypes.CDLL(p)
         for name, res, args in SIGNATURES:
             fn = getattr(lib, name)
@@ -339,3 +371,50 @@ class Context:
         check(self.lib.mr_phantom_parallel(self.h, ptr(E), len(E), ptr(a), len(a), int(rows),
                                            int(cols), float(pu), float(pv), float(far),
                                            ctypes.c_void_p(dptr)))
+
+
+class Plan:
+    """A compiled batch of tasks on a device work queue (mr_plan)."""
+
+    def __init__(self, ctx, tasks, out_len):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        T = np.ascontiguousarray(tasks, dtype=TASK_DTYPE)
+        h = ctypes.c_void_p()
+        check(self.lib.mr_plan_create(ctx.h, ptr(T), len(T), int(out_len), ctypes.byref(h)))
+        self.h = h
+        self.out_len = int(out_len)
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.mr_plan_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, n_views=-1, copy=True):
+        out = np.empty(self.out_len, dtype=np.float32) if copy else None
+        check(self.lib.mr_plan_run(self.h, int(n_views), ptr(out)))
+        return out
+
+    def stream(self, indices, block, copy=True):
+        idx = np.ascontiguousarray(indices, dtype=np.int64).ravel()
+        blk = np.ascontiguousarray(block, dtype=np.float32)
+        out = np.empty(self.out_len, dtype=np.float32) if copy else None
+        check(self.lib.mr_plan_stream(self.h, ptr(idx), len(idx), ptr(blk), ptr(out)))
+        return out
+
+    def output_ptr(self):
+        p = ctypes.c_void_p()
+        n = ctypes.c_int64()
+        check(self.lib.mr_plan_output(self.h, ctypes.byref(p), ctypes.byref(n)))
+        return p.value
+
+    def last_ms(self):
+        ms = ctypes.c_double()
+        check(self.lib.mr_plan_last_ms(self.h, ctypes.byref(ms)))
+        return ms.value
