@@ -268,32 +268,30 @@ template <> struct Taps<float> {
         d = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, voff + 4, soff1, 0));
     }
 };
+// 16-bit formats: one dword per detector row holds columns c0 (low half) and c0 + 1
+// (high half); it is read at 2-byte alignment (the queues run in unaligned mode).
 template <> struct Taps<__half> {
     static constexpr int LOG_ESZ = 1;
     __device__ static inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int soff1, float& a,
                                        float& b, float& c, float& d) {
-        const unsigned short ua = __builtin_amdgcn_raw_buffer_load_b16(rs, voff, 0, 0);
-        const unsigned short ub = __builtin_amdgcn_raw_buffer_load_b16(rs, voff + 2, 0, 0);
-        const unsigned short uc = __builtin_amdgcn_raw_buffer_load_b16(rs, voff, soff1, 0);
-        const unsigned short ud = __builtin_amdgcn_raw_buffer_load_b16(rs, voff + 2, soff1, 0);
-        a = __half2float(__ushort_as_half(ua));
-        b = __half2float(__ushort_as_half(ub));
-        c = __half2float(__ushort_as_half(uc));
-        d = __half2float(__ushort_as_half(ud));
+        const uint32_t w0 = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, 0, 0);
+        const uint32_t w1 = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff1, 0);
+        a = __half2float(__ushort_as_half(static_cast<unsigned short>(w0 & 0xffffu)));
+        b = __half2float(__ushort_as_half(static_cast<unsigned short>(w0 >> 16)));
+        c = __half2float(__ushort_as_half(static_cast<unsigned short>(w1 & 0xffffu)));
+        d = __half2float(__ushort_as_half(static_cast<unsigned short>(w1 >> 16)));
     }
 };
 template <> struct Taps<uint16_t> {
     static constexpr int LOG_ESZ = 1;
     __device__ static inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int soff1, float& a,
                                        float& b, float& c, float& d) {
-        const unsigned short ua = __builtin_amdgcn_raw_buffer_load_b16(rs, voff, 0, 0);
-        const unsigned short ub = __builtin_amdgcn_raw_buffer_load_b16(rs, voff + 2, 0, 0);
-        const unsigned short uc = __builtin_amdgcn_raw_buffer_load_b16(rs, voff, soff1, 0);
-        const unsigned short ud = __builtin_amdgcn_raw_buffer_load_b16(rs, voff + 2, soff1, 0);
-        a = __uint_as_float(static_cast<uint32_t>(ua) << 16);
-        b = __uint_as_float(static_cast<uint32_t>(ub) << 16);
-        c = __uint_as_float(static_cast<uint32_t>(uc) << 16);
-        d = __uint_as_float(static_cast<uint32_t>(ud) << 16);
+        const uint32_t w0 = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, 0, 0);
+        const uint32_t w1 = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff1, 0);
+        a = __uint_as_float(w0 << 16);
+        b = __uint_as_float(w0 & 0xffff0000u);
+        c = __uint_as_float(w1 << 16);
+        d = __uint_as_float(w1 & 0xffff0000u);
     }
 };
 
@@ -706,10 +704,55 @@ __global__ __launch_bounds__(256) void records_kernel(const ViewGeom* __restrict
     make_record(geom + v, vscale[v], T, xt, R, C, pitch, cone != 0, recs + gid * BP_REC);
 }
 
-template <int PPT>
+// Taps of one point as loaded by v4: f32 keeps four floats; the 16-bit formats keep
+// the two row dwords packed until the lerp, where f16 halves feed v_fma_mix_f32
+// directly (no separate conversions).
+template <typename ST> struct Pair;
+template <> struct Pair<float> {
+    float a, b, c, d;
+    __device__ inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int soff1) {
+        Taps<float>::load(rs, voff, soff1, a, b, c, d);
+    }
+    __device__ inline void lerp(float fc, float& top, float& bot) const {
+        top = fmaf(fc, b - a, a);
+        bot = fmaf(fc, d - c, c);
+    }
+};
+__device__ inline float half_lo(uint32_t w) {
+    return __half2float(__ushort_as_half(static_cast<unsigned short>(w & 0xffffu)));
+}
+__device__ inline float half_hi(uint32_t w) {
+    return __half2float(__ushort_as_half(static_cast<unsigned short>(w >> 16)));
+}
+template <> struct Pair<__half> {
+    uint32_t w0, w1;
+    __device__ inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int soff1) {
+        w0 = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, 0, 0);
+        w1 = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff1, 0);
+    }
+    __device__ inline void lerp(float fc, float& top, float& bot) const {
+        top = fmaf(fc, half_hi(w0), fmaf(-fc, half_lo(w0), half_lo(w0)));
+        bot = fmaf(fc, half_hi(w1), fmaf(-fc, half_lo(w1), half_lo(w1)));
+    }
+};
+template <> struct Pair<uint16_t> {
+    uint32_t w0, w1;
+    __device__ inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int soff1) {
+        w0 = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, 0, 0);
+        w1 = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff1, 0);
+    }
+    __device__ inline void lerp(float fc, float& top, float& bot) const {
+        const float a = __uint_as_float(w0 << 16), b = __uint_as_float(w0 & 0xffff0000u);
+        const float c = __uint_as_float(w1 << 16), d = __uint_as_float(w1 & 0xffff0000u);
+        top = fmaf(fc, b - a, a);
+        bot = fmaf(fc, d - c, c);
+    }
+};
+
+template <typename ST, int PPT>
 struct PointState {
     float fc[PPT], fr[PPT], wt[PPT];
-    float a[PPT], b[PPT], c[PPT], d[PPT];
+    Pair<ST> tap[PPT];
 };
 
 template <bool CONE, typename ST, int PPT>
@@ -759,7 +802,7 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel4(
     }
 
     // request the taps of view v into st
-    auto issue = [&](int v, PointState<PPT>& st) {
+    auto issue = [&](int v, PointState<ST, PPT>& st) {
         const float* r = rt + static_cast<int64_t>(v) * BP_REC;
         const float U0 = r[0], Ua = r[1], Ub = r[2];
         const int baseb = __float_as_int(r[14]) << LE;
@@ -783,7 +826,7 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel4(
                 st.fc[m] = __builtin_amdgcn_fractf(col);
                 st.fr[m] = __builtin_amdgcn_fractf(row);
                 const int voff = __mul24(cvt_flr(row), pitchb) + ((cvt_flr(col) << LE) + baseb);
-                Taps<ST>::load(rs, voff, pitchb, st.a[m], st.b[m], st.c[m], st.d[m]);
+                st.tap[m].load(rs, voff, pitchb);
             }
         } else {
             st.wt[0] = r[13];
@@ -793,15 +836,15 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel4(
                 const float col = __builtin_amdgcn_fmed3f(m ? fmaf(dm, Ub, uh0) : uh0, r[9], r[10]);
                 st.fc[m] = __builtin_amdgcn_fractf(col);
                 const int voff = roffp[m] + ((cvt_flr(col) << LE) + baseb);
-                Taps<ST>::load(rs, voff, pitchb, st.a[m], st.b[m], st.c[m], st.d[m]);
+                st.tap[m].load(rs, voff, pitchb);
             }
         }
     };
-    auto consume = [&](const PointState<PPT>& st) {
+    auto consume = [&](const PointState<ST, PPT>& st) {
 #pragma unroll
         for (int m = 0; m < PPT; ++m) {
-            const float top = fmaf(st.fc[m], st.b[m] - st.a[m], st.a[m]);
-            const float bot = fmaf(st.fc[m], st.d[m] - st.c[m], st.c[m]);
+            float top, bot;
+            st.tap[m].lerp(st.fc[m], top, bot);
             if (CONE) {
                 acc[m] = fmaf(st.wt[m], fmaf(st.fr[m], bot - top, top), acc[m]);
             } else {
@@ -812,7 +855,7 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel4(
     };
 
     if (n_views > 0) {
-        PointState<PPT> s0st, s1st;
+        PointState<ST, PPT> s0st, s1st;
         issue(0, s0st);
         int v = 0;
         for (; v + 2 <= n_views; v += 2) {

@@ -75,14 +75,14 @@ class FDKSlices:
                                  self.out_len)
         return out
 
-    def grad_projections(self, g):
-        """dL/dp for output weights g (float32 [out_len] on the GPU)."""
+    def grad_projections(self, g, shift=0.0):
+        """dL/dp for output weights g (float32 [out_len] on the GPU) at shift s."""
         A, R, C = self.geom.n_angles, self.geom.rows, self.geom.cols
         g = g.contiguous()
         gq = torch.empty((A, R, C), dtype=torch.float32, device=g.device)
         graw = torch.empty((R, A, C), dtype=torch.float32, device=g.device)
         torch.cuda.synchronize(g.device)
-        self.ctx.bp_tasks_adjoint_device(self.tasks(0.0), A, g.data_ptr(), self.out_len,
+        self.ctx.bp_tasks_adjoint_device(self.tasks(shift), A, g.data_ptr(), self.out_len,
                                          gq.data_ptr())
         self.ctx.filter_adjoint_device(np.arange(A), gq.data_ptr(), graw.data_ptr())
         return graw
@@ -119,7 +119,7 @@ class _FDKFunction(torch.autograd.Function):
         op = ctx.op
         gp = gs = None
         if ctx.needs_input_grad[0]:
-            gp = op.grad_projections(g.float())
+            gp = op.grad_projections(g.float(), float(shift))
         if ctx.needs_input_grad[1]:
             ds = op.dshift(proj, float(shift))
             gs = (g.float() * ds).sum().to(shift.dtype).reshape(shift.shape)
