@@ -134,7 +134,9 @@ typedef struct mr_task {
  * reconstruct_plane (one task per plane), recon.py:442-475 reconstruct_volume /
  * preview_volume (one task for the whole grid) and the trial slices of
  * recon.py:498-512 estimate_cor.  out: float32, host (out_on_device = 0) or device
- * (1), sized by the caller to hold every task's output range. */
+ * (1), sized by the caller to hold every task's output range.  Every output element
+ * of every task is written (points outside the volume box with 0); elements that no
+ * task addresses are left as they are. */
 int mr_bp_tasks(mr_ctx* ctx, const mr_task* tasks, int ntasks, int n_views,
                 float* out, int64_t out_len, int out_on_device);
 
@@ -157,7 +159,8 @@ int mr_bp_tasks_adjoint(mr_ctx* ctx, const mr_task* tasks, int ntasks, int n_vie
                         int gq_on_device);
 
 /* Derivative of every mr_bp_tasks output element with respect to its task's
- * cor_shift (recon.py:412-424: the detector moved to D - s*u), same layout as out. */
+ * cor_shift (recon.py:412-424: the detector moved to D - s*u), same layout and
+ * write rule as mr_bp_tasks. */
 int mr_bp_tasks_dshift(mr_ctx* ctx, const mr_task* tasks, int ntasks, int n_views, float* out,
                        int64_t out_len, int out_on_device);
 

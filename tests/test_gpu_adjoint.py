@@ -86,7 +86,10 @@ def test_dshift_vs_oracle_and_finite_difference(setup):
     move the quotient by several percent.  The checks are therefore (1) the float64
     analytic derivative against a float64 quotient at h = 1e-6 (no kink inside for
     almost every sample), and (2) the quotient at h = 1e-3 of the float64 and of the
-    float32 forward against the GPU derivative averaged over 64 midpoints of the step.
+    float32 forward against the GPU derivative averaged over 128 midpoints of the step.
+    The midpoint rule leaves an error of a few 1e-3 (each kink inside the step is
+    placed to within 1/128 of it); the quotient differs from dy/ds at s by ~8% here,
+    so a 1% bound still tells the two apart.
     """
     from minirecast import phantom
     g, box, planes, tasks, out_len, ctx = setup
@@ -117,7 +120,7 @@ def test_dshift_vs_oracle_and_finite_difference(setup):
     fd_fine = (loss64(s0 + 1e-6) - loss64(s0 - 1e-6)) / 2e-6
     assert abs(fd_fine - an64) <= 1e-6 * abs(an64), (fd_fine, an64)
 
-    h, M = 1e-3, 64
+    h, M = 1e-3, 128
     mean_d = np.zeros(out_len)
     for k in range(M):
         Tk = tasks.copy()
@@ -134,8 +137,9 @@ def test_dshift_vs_oracle_and_finite_difference(setup):
     print(f"dL/ds at s: {float(d.astype(np.float64) @ wts):.6f} (float64 {an64:.6f}, "
           f"fd64 h=1e-6 {fd_fine:.6f}); mean over step {an_mean:.6f}, fd64 h=1e-3 {fd64:.6f}, "
           f"fd32 h=1e-3 {fd32:.6f}")
-    assert abs(fd64 - an_mean) <= 2e-3 * abs(fd64), (fd64, an_mean)
+    assert abs(fd64 - an_mean) <= 1e-2 * abs(fd64), (fd64, an_mean)
     assert abs(fd32 - an_mean) <= 1e-2 * abs(fd64), (fd32, an_mean)
+    assert abs(fd64 - an64) > 3e-2 * abs(fd64)      # the kinks matter at this step
 
 
 def test_autograd_end_to_end(setup):
@@ -174,10 +178,10 @@ def test_autograd_end_to_end(setup):
     fd = (loss64(0.25 + 1e-6) - loss64(0.25 - 1e-6)) / 2e-6
     print(f"shift grad {float(shift.grad):.6f}  fd64(1e-6) {fd:.6f}")
     assert abs(float(shift.grad) - fd) <= 1e-4 * abs(fd) + 1e-6
-    h, M = 1e-3, 64
+    h, M = 1e-3, 128
     mean_g = 0.0
     for k in range(M):
         ds = op.dshift(proj.detach(), 0.25 - h + (k + 0.5) * (2 * h / M))
         mean_g += float((ds * wts).sum()) / M
     fd3 = (loss64(0.25 + h) - loss64(0.25 - h)) / (2 * h)
-    assert abs(fd3 - mean_g) <= 2e-3 * abs(fd3), (fd3, mean_g)
+    assert abs(fd3 - mean_g) <= 1e-2 * abs(fd3), (fd3, mean_g)

@@ -191,6 +191,36 @@ def test_batched_planes_equal_single_planes(small_cone):
         np.testing.assert_array_equal(img, one)
 
 
+def test_device_output_fully_written(small_cone):
+    """mr_bp_tasks and mr_bp_tasks_dshift write every element of every task into a
+    device buffer (tiles outside the volume box included) and nothing else."""
+    from minirecast import _hip, recon
+    w, _, rec, _ = small_cone
+    box = w.geom.volume_box()
+    # a zoomed-out plane: most tiles lie outside the box and are culled
+    tasks = np.array([recon.plane_task(box, (0, 0, 0), (1.0, 0.5, 0.8), 70, zoom=0.3,
+                                       out_offset=5),
+                      recon.plane_task(box, (0, 0, 2.0), (0, 0, 1), 40, out_offset=5 + 4900)],
+                     dtype=_hip.TASK_DTYPE)
+    out_len = 5 + 4900 + 1600 + 7
+    ctx = rec._ctx
+    want = ctx.bp_tasks(tasks, rec.n, out_len)
+    dwant = ctx.bp_tasks_dshift(tasks, rec.n, out_len)
+    sentinel = np.full(out_len, np.nan, np.float32)
+    d = ctx.alloc(4 * out_len)
+    try:
+        for call, ref in ((ctx.bp_tasks_device, want), (ctx.bp_tasks_dshift_device, dwant)):
+            ctx.h2d(d, sentinel)
+            call(tasks, rec.n, d, out_len)
+            got = np.empty(out_len, np.float32)
+            ctx.d2h(got, d)
+            assert np.isnan(got[:5]).all() and np.isnan(got[-7:]).all()
+            np.testing.assert_array_equal(got[5:-7], ref[5:-7])
+    finally:
+        ctx.free(d)
+    assert np.count_nonzero(want[5:5 + 4900]) < 4900 // 2
+
+
 # ---------------------------------------------------------------- edge cases
 def test_edge_cases_and_errors():
     from minirecast import recon
@@ -291,7 +321,7 @@ def test_storage_precision_vs_float64_oracle(small_cone):
     assert errs["f32"] < errs["f16"] < errs["bf16"]
 
 
-@pytest.mark.parametrize("variant", ["1", "2", "3"])
+@pytest.mark.parametrize("variant", ["1", "2", "3", "4"])
 def test_bp_kernel_variants_vs_oracle(small_cone, variant, monkeypatch):
     """Both tile-kernel inner loops (MINIRECAST_BP_VARIANT) against the float64 oracle,
     for every storage form, cone beam; and parallel beam on the reference fixture."""
@@ -307,7 +337,7 @@ def test_bp_kernel_variants_vs_oracle(small_cone, variant, monkeypatch):
         assert rel_l2(img, ref) < tol, (variant, storage, rel_l2(img, ref))
 
 
-@pytest.mark.parametrize("variant", ["1", "2", "3"])
+@pytest.mark.parametrize("variant", ["1", "2", "3", "4"])
 def test_bp_kernel_variants_parallel(golden, variant, monkeypatch):
     from minirecast import recon
     from minirecast.geometry import ParallelGeometry

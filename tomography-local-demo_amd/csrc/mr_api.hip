@@ -167,8 +167,12 @@ int validate_tasks(const mr_task* tasks, int ntasks, int64_t out_len) {
     return MR_OK;
 }
 
-void build_tiles(const mr_task* tasks, int ntasks, std::vector<Tile>& tiles) {
+// tiles: those that may hit the box; empty (optional): the culled rest, whose points
+// are all zero.
+void build_tiles(const mr_task* tasks, int ntasks, std::vector<Tile>& tiles,
+                 std::vector<Tile>* empty = nullptr) {
     tiles.clear();
+    if (empty) empty->clear();
     for (int i = 0; i < ntasks; ++i) {
         const mr_task& T = tasks[i];
         if (T.nq <= 0 || T.np <= 0 || T.ns <= 0) continue;
@@ -179,7 +183,10 @@ void build_tiles(const mr_task* tasks, int ntasks, std::vector<Tile>& tiles) {
                     const int p0 = tp * BP_TP, s0 = ts * BP_TS;
                     const int p1 = std::min(p0 + BP_TP, T.np) - 1;
                     const int s1 = std::min(s0 + BP_TS, T.ns) - 1;
-                    if (tile_may_hit_box(T, q, p0, p1, s0, s1)) tiles.push_back({i, q, tp, ts});
+                    if (tile_may_hit_box(T, q, p0, p1, s0, s1))
+                        tiles.push_back({i, q, tp, ts});
+                    else if (empty)
+                        empty->push_back({i, q, tp, ts});
                 }
     }
 }
@@ -187,10 +194,14 @@ void build_tiles(const mr_task* tasks, int ntasks, std::vector<Tile>& tiles) {
 enum TaskOp { OP_BP = 0, OP_ADJOINT = 1, OP_DSHIFT = 2 };
 
 // Uploads tasks + tiles and launches the tile kernel of `op` (device buffers), timing it.
+// For OP_BP and OP_DSHIFT every output element of every task is written: the tiles
+// culled by the box test are filled with zeros by their own small launch.
 int run_tasks(Ctx* c, const mr_task* tasks, int ntasks, int n_views, float* d_out,
               int op = OP_BP, const float* d_g = nullptr) {
-    std::vector<Tile> tiles;
-    build_tiles(tasks, ntasks, tiles);
+    std::vector<Tile> tiles, empty;
+    build_tiles(tasks, ntasks, tiles, op == OP_ADJOINT ? nullptr : &empty);
+    const size_t ne = empty.size();
+    tiles.insert(tiles.end(), empty.begin(), empty.end());
     const size_t tb = sizeof(mr_task) * ntasks;
     const size_t tl = sizeof(Tile) * tiles.size();
     const size_t off_tiles = (tb + 255) / 256 * 256;
@@ -205,8 +216,12 @@ int run_tasks(Ctx* c, const mr_task* tasks, int ntasks, int n_views, float* d_ou
     const mr_task* d_tasks = static_cast<const mr_task*>(c->d_req);
     const Tile* d_tiles = reinterpret_cast<const Tile*>(static_cast<char*>(c->d_req) + off_tiles);
     const float scale = n_views > 0 ? static_cast<float>(M_PI / n_views) : 0.0f;
+    const int nt = static_cast<int>(tiles.size() - ne);
+    if (ne) {
+        rc = launch_zero_tiles(d_tasks, d_tiles + nt, static_cast<int>(ne), d_out, c->s_req);
+        if (rc) return rc;
+    }
     MR_HIP(hipEventRecord(c->ev0, c->s_req));
-    const int nt = static_cast<int>(tiles.size());
     if (op == OP_BP)
         rc = launch_bp_tiles(c, d_tasks, d_tiles, nt, n_views, scale, d_out, c->s_req);
     else if (op == OP_ADJOINT)
@@ -335,6 +350,7 @@ int mr_ctx_create(int device, mr_ctx** out) {
         c->bp_variant = (k >= 1 && k <= 4) ? k : c->bp_variant;
     }
     if (const char* v = std::getenv("MINIRECAST_BP_PPT")) c->bp_ppt = std::atoi(v) == 2 ? 2 : 4;
+    if (const char* v = std::getenv("MINIRECAST_XCD_MAP")) c->xcd_map = std::atoi(v) ? 1 : 0;
     // default-priority, non-blocking streams
     if (hipStreamCreateWithFlags(&c->s_ingest, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->s_req, hipStreamNonBlocking) != hipSuccess ||

@@ -295,13 +295,23 @@ template <> struct Taps<uint16_t> {
     }
 };
 
+// Workgroup IDs are dealt round-robin to the 8 XCDs (ID mod 8), each with its own L2.
+// Remapping ID b to tile x * per + min(x, rem) + b / 8 (x = b mod 8) hands every XCD a
+// contiguous run of the tile list; neighbouring tiles on a slice then share the
+// detector cache lines at their borders inside one L2.  The map is a bijection on
+// [0, nb) for any nb.
+__device__ inline int xcd_block(int b, int nb) {
+    const int x = b & 7, k = b >> 3, per = nb >> 3, rem = nb & 7;
+    return x * per + (x < rem ? x : rem) + k;
+}
+
 template <bool CONE, typename ST>
 __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel2(
     const ST* __restrict__ ring, int64_t view_stride, int pitch, int R, int C,
     const ViewGeom* __restrict__ geom, const float* __restrict__ vscale,
     const mr_task* __restrict__ tasks, const Tile* __restrict__ tiles, int n_views_arg,
     float scale_arg, float* __restrict__ out, int* __restrict__ qhead,
-    const int* __restrict__ qcount, const int* __restrict__ n_views_dev) {
+    const int* __restrict__ qcount, const int* __restrict__ n_views_dev, int xcd) {
     __shared__ float4 rec4[BP_VPASS * BP_REC / 4];
     __shared__ int s_ti;
     float* rec = reinterpret_cast<float*>(rec4);
@@ -322,7 +332,7 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel2(
         if (ti >= ntile) break;
     } else {
         if (it > 0) break;
-        ti = blockIdx.x;
+        ti = xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     }
     const Tile tl = tiles[ti];
     const mr_task& T = tasks[tl.task];
@@ -759,13 +769,14 @@ template <bool CONE, typename ST, int PPT>
 __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel4(
     const ST* __restrict__ ring, int64_t view_stride, int pitch,
     const float* __restrict__ recs, const mr_task* __restrict__ tasks,
-    const Tile* __restrict__ tiles, int n_views, float scale, float* __restrict__ out) {
+    const Tile* __restrict__ tiles, int n_views, float scale, float* __restrict__ out, int xcd) {
     constexpr int LE = Taps<ST>::LOG_ESZ;
     constexpr int PSTEP = BP_THREADS / BP_TS;        // 8 rows between a thread's points
     constexpr int SUBS = BP_TP / (PSTEP * PPT);      // workgroups per 32-row tile
     static_assert(SUBS * PSTEP * PPT == BP_TP, "tile shape");
-    const int tile = blockIdx.x / SUBS;
-    const int sub = blockIdx.x - tile * SUBS;
+    const int lb = xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int tile = lb / SUBS;
+    const int sub = lb - tile * SUBS;
     const Tile tl = tiles[tile];
     const mr_task& T = tasks[tl.task];
     const int t = threadIdx.x;
@@ -1140,19 +1151,19 @@ int dispatch_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntil
             hipLaunchKernelGGL((bp_tiles_kernel2<CONE, float>), grid, block, 0, s,
                                static_cast<const float*>(c->ring), c->view_stride, c->pitch, c->R,
                                c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out,
-                               q.head, q.count, q.n_views);
+                               q.head, q.count, q.n_views, c->xcd_map);
             break;
         case MR_STORE_F16:
             hipLaunchKernelGGL((bp_tiles_kernel2<CONE, __half>), grid, block, 0, s,
                                static_cast<const __half*>(c->ring), c->view_stride, c->pitch, c->R,
                                c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out,
-                               q.head, q.count, q.n_views);
+                               q.head, q.count, q.n_views, c->xcd_map);
             break;
         default:
             hipLaunchKernelGGL((bp_tiles_kernel2<CONE, uint16_t>), grid, block, 0, s,
                                static_cast<const uint16_t*>(c->ring), c->view_stride, c->pitch,
                                c->R, c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale,
-                               d_out, q.head, q.count, q.n_views);
+                               d_out, q.head, q.count, q.n_views, c->xcd_map);
             break;
         }
         MR_HIP(hipGetLastError());
@@ -1229,6 +1240,29 @@ int launch_bp_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int nti
     return c->kind == MR_KIND_CONE
                ? dispatch_tiles<true>(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s, q ? *q : none)
                : dispatch_tiles<false>(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s, q ? *q : none);
+}
+
+__global__ __launch_bounds__(256) void zero_tiles_kernel(const mr_task* __restrict__ tasks,
+                                                        const Tile* __restrict__ tiles,
+                                                        float* __restrict__ out) {
+    const Tile tl = tiles[blockIdx.x];
+    const mr_task& T = tasks[tl.task];
+    const int s = tl.ts * BP_TS + (threadIdx.x & (BP_TS - 1));
+    const int64_t obase = T.out_offset + static_cast<int64_t>(tl.q) * T.stride_q;
+    for (int r = threadIdx.x / BP_TS; r < BP_TP; r += 256 / BP_TS) {
+        const int p = tl.tp * BP_TP + r;
+        if (p < T.np && s < T.ns)
+            out[obase + static_cast<int64_t>(p) * T.stride_p + static_cast<int64_t>(s) * T.stride_s] =
+                0.0f;
+    }
+}
+
+int launch_zero_tiles(const mr_task* d_tasks, const Tile* d_tiles, int ntiles, float* d_out,
+                      hipStream_t s) {
+    if (ntiles <= 0) return MR_OK;
+    hipLaunchKernelGGL(zero_tiles_kernel, dim3(ntiles), dim3(256), 0, s, d_tasks, d_tiles, d_out);
+    MR_HIP(hipGetLastError());
+    return MR_OK;
 }
 
 // Device work queue: one thread per candidate tile of every task (prefix offsets per
@@ -1334,11 +1368,11 @@ static void launch_v4_kernel(Ctx* c, const float* recs, const mr_task* d_tasks, 
     if (c->bp_ppt == 2)
         hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, 2>), dim3(2 * nt), dim3(BP_THREADS), 0, s,
                            static_cast<const ST*>(c->ring), c->view_stride, c->pitch, recs, d_tasks,
-                           d_tiles, n_views, scale, d_out);
+                           d_tiles, n_views, scale, d_out, c->xcd_map);
     else
         hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, 4>), dim3(nt), dim3(BP_THREADS), 0, s,
                            static_cast<const ST*>(c->ring), c->view_stride, c->pitch, recs, d_tasks,
-                           d_tiles, n_views, scale, d_out);
+                           d_tiles, n_views, scale, d_out, c->xcd_map);
 }
 
 int launch_bp_tiles_v4(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles, int n_views,

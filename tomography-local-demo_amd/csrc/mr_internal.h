@@ -86,6 +86,7 @@ struct Ctx {
     double last_kernel_ms = 0.0;
     int bp_variant = 1;              // back-projection inner loop (MINIRECAST_BP_VARIANT)
     int bp_ppt = 4;                  // v4 points per thread (MINIRECAST_BP_PPT)
+    int xcd_map = 1;                 // XCD-contiguous tile order (MINIRECAST_XCD_MAP)
     float* d_recs = nullptr;         // v4 per-(tile, view) records
     size_t recs_bytes = 0;
 
@@ -173,6 +174,9 @@ int launch_filter_rows_plain(Ctx* c, const float* d_in, int rows, int n, float g
 int launch_bp_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles,
                     int n_views, float scale, float* d_out, hipStream_t s,
                     const TileQueue* q = nullptr);
+// zeros every point of the given tiles (tiles culled by the box test)
+int launch_zero_tiles(const mr_task* d_tasks, const Tile* d_tiles, int ntiles, float* d_out,
+                      hipStream_t s);
 int launch_enumerate_tiles(const mr_task* d_tasks, const int64_t* d_toff, int ntasks,
                            int64_t total, Tile* d_queue, int* d_counters, hipStream_t s);
 int launch_reset_head(int* d_counters, hipStream_t s);
