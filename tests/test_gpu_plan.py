@@ -89,7 +89,7 @@ def test_plan_stream_graph_replays(workload, storage, monkeypatch):
         off += p['n'] ** 2
 
 
-@pytest.mark.parametrize("variant,ppt", [("2", "4"), ("4", "4"), ("4", "2")])
+@pytest.mark.parametrize("variant,ppt", [("2", "4"), ("4", "4"), ("4", "2"), ("5", "4"), ("5", "2")])
 def test_bp_variants_full_check(workload, variant, ppt, monkeypatch):
     from minirecast import _hip
     monkeypatch.setenv("MINIRECAST_BP_PPT", ppt)

@@ -347,7 +347,7 @@ int mr_ctx_create(int device, mr_ctx** out) {
     c->device = device;
     if (const char* v = std::getenv("MINIRECAST_BP_VARIANT")) {
         const int k = std::atoi(v);
-        c->bp_variant = (k >= 1 && k <= 4) ? k : c->bp_variant;
+        c->bp_variant = (k >= 1 && k <= 5) ? k : c->bp_variant;
     }
     if (const char* v = std::getenv("MINIRECAST_BP_PPT")) c->bp_ppt = std::atoi(v) == 2 ? 2 : 4;
     if (const char* v = std::getenv("MINIRECAST_XCD_MAP")) c->xcd_map = std::atoi(v) ? 1 : 0;
@@ -796,7 +796,7 @@ int mr_plan_create(mr_ctx* c, const mr_task* tasks, int ntasks, int64_t out_len,
     MR_CHECK(c->kind != MR_KIND_NONE, MR_ERUNTIME, "no geometry set");
     MR_CHECK(ntasks >= 1 && tasks, MR_EVALUE, "a plan needs at least one task");
     MR_CHECK(out_len >= 1, MR_EVALUE, "a plan needs an output");
-    MR_CHECK(c->bp_variant == 2 || c->bp_variant == 4, MR_ERUNTIME,
+    MR_CHECK(c->bp_variant == 2 || c->bp_variant == 4 || c->bp_variant == 5, MR_ERUNTIME,
              "plans run the persistent v2 back-projection kernel");
     rc = validate_tasks(tasks, ntasks, out_len);
     if (rc) return rc;

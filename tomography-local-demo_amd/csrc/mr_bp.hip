@@ -890,6 +890,299 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel4(
     }
 }
 
+// ---------------------------------------------------------------- v5: LDS-staged detector boxes
+// Direct gathers cost one cache-line lookup per distinct line a wave touches (~16 per
+// instruction for an oblique tile), and every line fetched from HBM is waited on
+// inside the view loop.  v5 instead stages, per (sub-tile, view), the bounding box of
+// the sub-tile's taps into LDS with coalesced 16-byte loads, issued two views ahead
+// into registers, and reads the four taps from LDS.
+//
+// Record: v4's 16 floats, then per sub-tile an 8-int box descriptor:
+//   [0] goff  element offset of the box origin in the view slot (padded coordinates)
+//   [1] nr    box rows (0: the box does not fit; the view uses direct gathers)
+//   [2] nch   16-byte chunks per box row
+//   [3] strb  LDS bytes per staged row (nch * 16 + 4: an odd dword stride)
+//   [4] koffb LDS byte offset of relative tap (0, 0): a = ri * strb + ci * esz - koffb
+//   [5] 1 / nch as float bits, [6] nr * nch
+// The box is derived in FP64 from the four corner images of the sub-tile (a
+// projective map with w > 0 sends the rectangle into the hull of its corner images),
+// clamped like the inner loop, one pixel of margin each side (FP32 coordinates differ
+// from FP64 by ~1e-5 px), columns aligned down to 16 bytes.
+constexpr int V5_REC = 32;            // floats per (tile, view) record
+constexpr int V5_BOX = 8;             // ints per sub-tile box descriptor
+
+template <int LE> struct V5Cfg {
+    static constexpr int K = LE == 2 ? 3 : 2;                // 16-byte chunks per thread
+    static constexpr int CH = 16 >> LE;                      // elements per chunk
+    static constexpr int BUF = K * BP_THREADS * 16 + 1024;   // LDS bytes per stage buffer
+};
+
+__device__ void make_box5(const RecRef& q, int p_lo, int p_hi, int R, int C, int pitch, int LE,
+                          int kcap, int bufbytes, int* __restrict__ d) {
+    double cmin = 1e300, cmax = -1e300, rmin = 1e300, rmax = -1e300;
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double ds = (k & 1) ? BP_TS - 1 : 0, dp = (k & 2) ? p_hi : p_lo;
+        const double w = q.W0 + ds * q.Wa + dp * q.Wb;
+        ok = ok && (w > 0.0);
+        const double col = (q.U0 + ds * q.Ua + dp * q.Ub) / w;   // relative to cref
+        const double row = (q.V0 + ds * q.Va + dp * q.Vb) / w;   // relative to rref
+        cmin = fmin(cmin, col); cmax = fmax(cmax, col);
+        rmin = fmin(rmin, row); rmax = fmax(rmax, row);
+    }
+    ok = ok && fabs(cmin) < 1e7 && fabs(cmax) < 1e7 && fabs(rmin) < 1e7 && fabs(rmax) < 1e7;
+    for (int i = 0; i < V5_BOX; ++i) d[i] = 0;
+    if (!ok) return;
+    const double cl = -1.0 - q.cref, ch = C - q.cref, rl = -1.0 - q.rref, rh = R - q.rref;
+    const double c_lo = floor(fmin(fmax(cmin, cl), ch)) - 1.0;
+    const double c_hi = floor(fmin(fmax(cmax, cl), ch)) + 2.0;
+    const double r_lo = floor(fmin(fmax(rmin, rl), rh)) - 1.0;
+    const double r_hi = floor(fmin(fmax(rmax, rl), rh)) + 2.0;
+    // padded ring coordinates: columns 0..C+2, rows 0..R+2
+    const int pc_lo = max(static_cast<int>(c_lo + q.cref) + 1, 0);
+    const int pc_hi = min(static_cast<int>(c_hi + q.cref) + 1, C + 2);
+    const int pr_lo = max(static_cast<int>(r_lo + q.rref) + 1, 0);
+    const int pr_hi = min(static_cast<int>(r_hi + q.rref) + 1, R + 2);
+    const int CH = 16 >> LE;
+    const int pc0 = pc_lo & ~(CH - 1);
+    const int nch = (pc_hi - pc0) / CH + 1;
+    const int nr = pr_hi - pr_lo + 1;
+    const int strb = nch * 16 + 4;
+    if (nr <= 0 || nch <= 0 || nr * nch > kcap || nr * strb > bufbytes) return;
+    const int brel_r = pr_lo - static_cast<int>(q.rref) - 1;
+    const int brel_c = pc0 - static_cast<int>(q.cref) - 1;
+    d[0] = pr_lo * pitch + pc0;
+    d[1] = nr;
+    d[2] = nch;
+    d[3] = strb;
+    d[4] = brel_r * strb + (brel_c << LE);
+    d[5] = __float_as_int(1.0f / static_cast<float>(nch));
+    d[6] = nr * nch;
+}
+
+__global__ __launch_bounds__(256) void records5_kernel(const ViewGeom* __restrict__ geom,
+                                                       const float* __restrict__ vscale,
+                                                       const mr_task* __restrict__ tasks,
+                                                       const Tile* __restrict__ tiles, int ntiles,
+                                                       int n_views, int R, int C, int pitch,
+                                                       int subs, int le, int kcap, int bufbytes,
+                                                       float* __restrict__ recs,
+                                                       int* __restrict__ fit) {
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (gid >= static_cast<int64_t>(ntiles) * n_views) return;
+    const int ti = static_cast<int>(gid / n_views);
+    const int v = static_cast<int>(gid - static_cast<int64_t>(ti) * n_views);
+    const Tile tl = tiles[ti];
+    const mr_task& T = tasks[tl.task];
+    const int p0 = tl.tp * BP_TP, s0 = tl.ts * BP_TS;
+    double xt[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) xt[d] = T.o[d] + tl.q * T.c[d] + p0 * T.a[d] + s0 * T.b[d];
+    float* rec = recs + gid * V5_REC;
+    RecRef q;
+    make_record(geom + v, vscale[v], T, xt, R, C, pitch, true, rec, &q);
+    const int rows = BP_TP / subs;
+    int* bx = reinterpret_cast<int*>(rec + BP_REC);
+    bool all = true;
+    for (int sb = 0; sb < 2; ++sb) {
+        if (sb < subs) {
+            make_box5(q, sb * rows, sb * rows + rows - 1, R, C, pitch, le, kcap, bufbytes,
+                      bx + V5_BOX * sb);
+            all = all && bx[V5_BOX * sb + 1] > 0;
+        } else {
+            for (int i = 0; i < V5_BOX; ++i) bx[V5_BOX * sb + i] = 0;
+        }
+    }
+    if (!all) fit[ti] = 0;     // some view of this tile does not fit: direct gathers
+}
+
+__global__ void set_int_kernel(int* __restrict__ p, int n, int value) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+template <typename ST, int K>
+struct BoxStage {
+    uint4 val[K];
+    int lds[K];          // LDS byte offset of chunk k, -1 when the chunk is outside the box
+};
+
+template <typename ST> struct LdsPair;
+template <> struct LdsPair<float> {
+    __device__ static inline Pair<float> read(const unsigned char* __restrict__ p, int strb) {
+        Pair<float> t;
+        t.a = *reinterpret_cast<const float*>(p);
+        t.b = *reinterpret_cast<const float*>(p + 4);
+        t.c = *reinterpret_cast<const float*>(p + strb);
+        t.d = *reinterpret_cast<const float*>(p + strb + 4);
+        return t;
+    }
+};
+template <typename ST> struct LdsPair {       // 16-bit formats: one dword per row
+    __device__ static inline Pair<ST> read(const unsigned char* __restrict__ p, int strb) {
+        Pair<ST> t;
+        __builtin_memcpy(&t.w0, p, 4);            // 2-byte aligned: unaligned LDS access
+        __builtin_memcpy(&t.w1, p + strb, 4);
+        return t;
+    }
+};
+
+template <typename ST, int PPT>
+__global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel5(
+    const ST* __restrict__ ring, int64_t view_stride, int pitch,
+    const float* __restrict__ recs, const mr_task* __restrict__ tasks,
+    const Tile* __restrict__ tiles, int n_views, float scale, float* __restrict__ out, int xcd,
+    const int* __restrict__ fit) {
+    constexpr int LE = Taps<ST>::LOG_ESZ;
+    using Cfg = V5Cfg<LE>;
+    constexpr int K = Cfg::K;
+    constexpr int PSTEP = BP_THREADS / BP_TS;
+    constexpr int SUBS = BP_TP / (PSTEP * PPT);
+    static_assert(SUBS * PSTEP * PPT == BP_TP && SUBS <= 2, "tile shape");
+    __shared__ __attribute__((aligned(16))) unsigned char stage[2][Cfg::BUF];
+    const int lb = xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int tile = lb / SUBS;
+    const int sub = lb - tile * SUBS;
+    const Tile tl = tiles[tile];
+    const mr_task& T = tasks[tl.task];
+    const int t = threadIdx.x;
+    const int tx = t & (BP_TS - 1);
+    const int ty = t / BP_TS + sub * PSTEP * PPT;
+    const int p0 = tl.tp * BP_TP, s0 = tl.ts * BP_TS;
+    const int s = s0 + tx;
+    bool valid[PPT], inside[PPT];
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) {
+        const int p = p0 + ty + PSTEP * m;
+        valid[m] = (p < T.np) && (s < T.ns);
+        inside[m] = valid[m] && (!T.mask_box || in_box(T, p, s));
+    }
+    const float ds = static_cast<float>(tx), dp0 = static_cast<float>(ty);
+    const int pitchb = pitch << LE;
+    const int view_bytes = static_cast<int>(view_stride << LE);
+    const float* __restrict__ rt = recs + static_cast<int64_t>(tile) * n_views * V5_REC;
+    const int boxo = BP_REC + V5_BOX * sub;
+    float acc[PPT];
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) acc[m] = 0.0f;
+
+    auto rsrc = [&](int v) {
+        return __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<ST*>(ring + static_cast<int64_t>(v) * view_stride), 0, view_bytes, 0x00020000);
+    };
+    // request the box of view v into registers
+    auto load_box = [&](int v, BoxStage<ST, K>& st) {
+        const int* bx = reinterpret_cast<const int*>(rt + static_cast<int64_t>(v) * V5_REC + boxo);
+        const int goff = bx[0], nch = bx[2], strb = bx[3], tot = bx[6];
+        const float inv = __int_as_float(bx[5]);
+        const __amdgpu_buffer_rsrc_t rs = rsrc(v);
+        // every lane loads (lanes past the box re-read the box origin): unconditional
+        // loads keep the vector-memory counter exact, so the wait before staging view v
+        // leaves view v+1's loads in flight
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int e = t + BP_THREADS * k;
+            const bool in = e < tot;
+            const int ee = in ? e : 0;
+            const int row = static_cast<int>((static_cast<float>(ee) + 0.5f) * inv);
+            const int cc = ee - row * nch;
+            const int off = ((goff + row * pitch) << LE) + cc * 16;
+            st.val[k] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+            st.lds[k] = in ? row * strb + cc * 16 : -1;
+        }
+    };
+    auto store_box = [&](const BoxStage<ST, K>& st, unsigned char* __restrict__ buf) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (st.lds[k] >= 0) __builtin_memcpy(buf + st.lds[k], &st.val[k], 16);   // 4-byte aligned
+    };
+    auto coords = [&](const float* r, int m, float& fc, float& fr, float& wt, int& ri, int& ci) {
+        const float dp = dp0 + static_cast<float>(PSTEP * m);
+        const float uh = fmaf(dp, r[2], fmaf(ds, r[1], r[0]));
+        const float vh = fmaf(dp, r[5], fmaf(ds, r[4], r[3]));
+        const float wh = fmaf(dp, r[8], fmaf(ds, r[7], r[6]));
+        const float rw = __builtin_amdgcn_rcpf(wh);
+        const float col = __builtin_amdgcn_fmed3f(uh * rw, r[9], r[10]);
+        const float row = __builtin_amdgcn_fmed3f(vh * rw, r[11], r[12]);
+        wt = rw * rw;
+        fc = __builtin_amdgcn_fractf(col);
+        fr = __builtin_amdgcn_fractf(row);
+        ci = cvt_flr(col);
+        ri = cvt_flr(row);
+    };
+    auto compute = [&](int v, const unsigned char* __restrict__ buf) {
+        const float* r = rt + static_cast<int64_t>(v) * V5_REC;
+        const int* bx = reinterpret_cast<const int*>(r + boxo);
+        const int strb = bx[3], koffb = bx[4];
+#pragma unroll
+        for (int m = 0; m < PPT; ++m) {
+            float fc, fr, wt;
+            int ri, ci;
+            coords(r, m, fc, fr, wt, ri, ci);
+            const int a = __mul24(ri, strb) + ci * (1 << LE) - koffb;
+            const Pair<ST> tp = LdsPair<ST>::read(buf + a, strb);
+            float top, bot;
+            tp.lerp(fc, top, bot);
+            acc[m] = fmaf(wt, fmaf(fr, bot - top, top), acc[m]);
+        }
+    };
+    // direct gathers from the ring (tiles with a view whose box does not fit)
+    auto compute_direct = [&](int v) {
+        const float* r = rt + static_cast<int64_t>(v) * V5_REC;
+        const __amdgpu_buffer_rsrc_t rs = rsrc(v);
+        const int baseb = __float_as_int(r[14]) << LE;
+#pragma unroll
+        for (int m = 0; m < PPT; ++m) {
+            float fc, fr, wt;
+            int ri, ci;
+            coords(r, m, fc, fr, wt, ri, ci);
+            Pair<ST> tp;
+            tp.load(rs, __mul24(ri, pitchb) + ((ci << LE) + baseb), pitchb);
+            float top, bot;
+            tp.lerp(fc, top, bot);
+            acc[m] = fmaf(wt, fmaf(fr, bot - top, top), acc[m]);
+        }
+    };
+
+    if (n_views > 0 && !fit[tile]) {
+        for (int v = 0; v < n_views; ++v) compute_direct(v);
+    } else if (n_views > 0) {
+        BoxStage<ST, K> st0, st1;
+        load_box(0, st0);
+        load_box(min(1, n_views - 1), st1);
+        // the prefetches run unconditionally (past the end they re-read the last view)
+        // so that the wait before each staging counts exactly the loads issued after it
+        // and the loop body has no conditional half (odd counts end in an epilogue)
+        const int last = n_views - 1;
+        int v = 0;
+        for (; v + 1 < n_views; v += 2) {
+            store_box(st0, stage[0]);
+            load_box(min(v + 2, last), st0);
+            __syncthreads();
+            compute(v, stage[0]);
+            store_box(st1, stage[1]);
+            load_box(min(v + 3, last), st1);
+            __syncthreads();
+            compute(v + 1, stage[1]);
+        }
+        if (v < n_views) {
+            store_box(st0, stage[0]);
+            __syncthreads();
+            compute(v, stage[0]);
+        }
+    }
+
+    const int64_t obase = T.out_offset + static_cast<int64_t>(tl.q) * T.stride_q;
+#pragma unroll
+    for (int m = 0; m < PPT; ++m) {
+        if (!valid[m]) continue;
+        const int p = p0 + ty + PSTEP * m;
+        out[obase + static_cast<int64_t>(p) * T.stride_p + static_cast<int64_t>(s) * T.stride_s] =
+            inside[m] ? acc[m] * scale : 0.0f;
+    }
+}
+
 // ---------------------------------------------------------------- adjoint and shift derivative
 // Transpose of the tile back-projection with respect to the stored (dequantised) views:
 // every point scatters scale * w_k * g_p with the forward's bilinear weights into
@@ -1224,9 +1517,9 @@ int launch_bp_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int nti
                     float scale, float* d_out, hipStream_t s, const TileQueue* q) {
     const TileQueue none{};
     if (!q && ntiles <= 0) return MR_OK;
-    if (!q && c->bp_variant == 4 && n_views > 0)
+    if (!q && (c->bp_variant == 4 || c->bp_variant == 5) && n_views > 0)
         return launch_bp_tiles_v4(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s);
-    if (q && c->bp_variant != 2 && c->bp_variant != 4)
+    if (q && c->bp_variant != 2 && c->bp_variant != 4 && c->bp_variant != 5)
         return fail(MR_ERUNTIME, "persistent launches need the v2 kernel");
     if (q) {                               // persistent launches always run the v2 loop
         const int saved = c->bp_variant;
@@ -1375,25 +1668,69 @@ static void launch_v4_kernel(Ctx* c, const float* recs, const mr_task* d_tasks, 
                            d_tiles, n_views, scale, d_out, c->xcd_map);
 }
 
+template <typename ST>
+static void launch_v5_kernel(Ctx* c, const float* recs, const int* fit, const mr_task* d_tasks,
+                             const Tile* d_tiles, int nt, int n_views, float scale, float* d_out,
+                             hipStream_t s) {
+    if (c->bp_ppt == 2)
+        hipLaunchKernelGGL((bp_tiles_kernel5<ST, 2>), dim3(2 * nt), dim3(BP_THREADS), 0, s,
+                           static_cast<const ST*>(c->ring), c->view_stride, c->pitch, recs, d_tasks,
+                           d_tiles, n_views, scale, d_out, c->xcd_map, fit);
+    else
+        hipLaunchKernelGGL((bp_tiles_kernel5<ST, 4>), dim3(nt), dim3(BP_THREADS), 0, s,
+                           static_cast<const ST*>(c->ring), c->view_stride, c->pitch, recs, d_tasks,
+                           d_tiles, n_views, scale, d_out, c->xcd_map, fit);
+}
+
+// v4 (and v5 for cone beam): per chunk of tiles, the record kernel then the tile kernel.
 int launch_bp_tiles_v4(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles, int n_views,
                        float scale, float* d_out, hipStream_t s) {
     if (ntiles <= 0 || n_views <= 0) {
         // no views: every point is zero; run with one view count of 0 through v2
         return launch_bp_tiles(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s);
     }
-    const size_t per_tile = sizeof(float) * BP_REC * static_cast<size_t>(n_views);
+    const bool cone = c->kind == MR_KIND_CONE;
+    const bool v5 = cone && c->bp_variant == 5;
+    const int rec_floats = v5 ? V5_REC : BP_REC;
+    const size_t per_tile = sizeof(float) * rec_floats * static_cast<size_t>(n_views);
     const size_t budget = static_cast<size_t>(512) << 20;          // record buffer <= 512 MB
     int chunk = static_cast<int>(std::max<size_t>(1, budget / per_tile));
     chunk = std::min(chunk, ntiles);
-    int rc = ensure(reinterpret_cast<void**>(&c->d_recs), &c->recs_bytes, per_tile * chunk);
+    // v5: one "every box fits" flag per tile after the records
+    const size_t flag_off = (per_tile * chunk + 255) / 256 * 256;
+    int rc = ensure(reinterpret_cast<void**>(&c->d_recs), &c->recs_bytes,
+                    flag_off + sizeof(int) * chunk);
     if (rc) return rc;
+    int* d_fit = reinterpret_cast<int*>(reinterpret_cast<char*>(c->d_recs) + flag_off);
+    const int le = c->storage == MR_STORE_F32 ? 2 : 1;
+    const int subs = c->bp_ppt == 2 ? 2 : 1;
+    const int kcap = (le == 2 ? V5Cfg<2>::K : V5Cfg<1>::K) * BP_THREADS;
+    const int bufb = le == 2 ? V5Cfg<2>::BUF : V5Cfg<1>::BUF;
     for (int t0 = 0; t0 < ntiles; t0 += chunk) {
         const int nt = std::min(chunk, ntiles - t0);
         const int64_t nrec = static_cast<int64_t>(nt) * n_views;
-        hipLaunchKernelGGL(records_kernel, dim3(static_cast<unsigned>((nrec + 255) / 256)), dim3(256),
-                           0, s, c->d_geom, c->d_vscale, d_tasks, d_tiles + t0, nt, n_views, c->R,
-                           c->C, c->pitch, c->kind == MR_KIND_CONE ? 1 : 0, c->d_recs);
-        const bool cone = c->kind == MR_KIND_CONE;
+        const dim3 rgrid(static_cast<unsigned>((nrec + 255) / 256));
+        if (v5) {
+            hipLaunchKernelGGL(set_int_kernel, dim3((nt + 255) / 256), dim3(256), 0, s, d_fit, nt, 1);
+            hipLaunchKernelGGL(records5_kernel, rgrid, dim3(256), 0, s, c->d_geom, c->d_vscale,
+                               d_tasks, d_tiles + t0, nt, n_views, c->R, c->C, c->pitch, subs, le,
+                               kcap, bufb, c->d_recs, d_fit);
+            switch (c->storage) {
+            case MR_STORE_F32:
+                launch_v5_kernel<float>(c, c->d_recs, d_fit, d_tasks, d_tiles + t0, nt, n_views, scale, d_out, s);
+                break;
+            case MR_STORE_F16:
+                launch_v5_kernel<__half>(c, c->d_recs, d_fit, d_tasks, d_tiles + t0, nt, n_views, scale, d_out, s);
+                break;
+            default:
+                launch_v5_kernel<uint16_t>(c, c->d_recs, d_fit, d_tasks, d_tiles + t0, nt, n_views, scale, d_out, s);
+                break;
+            }
+            MR_HIP(hipGetLastError());
+            continue;
+        }
+        hipLaunchKernelGGL(records_kernel, rgrid, dim3(256), 0, s, c->d_geom, c->d_vscale, d_tasks,
+                           d_tiles + t0, nt, n_views, c->R, c->C, c->pitch, cone ? 1 : 0, c->d_recs);
         switch (c->storage) {
         case MR_STORE_F32:
             cone ? launch_v4_kernel<true, float>(c, c->d_recs, d_tasks, d_tiles + t0, nt, n_views, scale, d_out, s)

@@ -86,7 +86,7 @@ struct Ctx {
     double last_kernel_ms = 0.0;
     int bp_variant = 1;              // back-projection inner loop (MINIRECAST_BP_VARIANT)
     int bp_ppt = 4;                  // v4 points per thread (MINIRECAST_BP_PPT)
-    int xcd_map = 1;                 // XCD-contiguous tile order (MINIRECAST_XCD_MAP)
+    int xcd_map = 0;                 // XCD-contiguous tile order (MINIRECAST_XCD_MAP=1)
     float* d_recs = nullptr;         // v4 per-(tile, view) records
     size_t recs_bytes = 0;
 
