@@ -89,21 +89,32 @@ def test_plan_stream_graph_replays(workload, storage, monkeypatch):
         off += p['n'] ** 2
 
 
-@pytest.mark.parametrize("variant,ppt", [("2", "4"), ("4", "4"), ("4", "2"), ("5", "4"), ("5", "2")])
-def test_bp_variants_full_check(workload, variant, ppt, monkeypatch):
+@pytest.mark.parametrize("variant,ppt,layout,depth", [
+    ("2", "4", "0", "2"), ("4", "4", "0", "2"), ("4", "2", "0", "2"), ("4", "2", "1", "3"),
+    ("4", "1", "1", "2"), ("4", "1", "0", "3"), ("4", "4", "1", "3"), ("5", "4", "0", "2"),
+    ("5", "2", "0", "2")])
+def test_bp_variants_full_check(workload, variant, ppt, layout, depth, monkeypatch):
+    """Every tile-kernel configuration (kernel, points per thread, lane layout, views in
+    flight) against the plain v1 kernel on all outputs, for view counts that leave
+    every pipeline tail length (n, n - 1, 2, 1)."""
     from minirecast import _hip
     monkeypatch.setenv("MINIRECAST_BP_PPT", ppt)
+    monkeypatch.setenv("MINIRECAST_BP_LAYOUT", layout)
+    monkeypatch.setenv("MINIRECAST_BP_DEPTH", depth)
     w, proj, planes, tasks, out_len = workload
     T = np.array(tasks, dtype=_hip.TASK_DTYPE)
+    A = w.geom.n_angles
     ref_ctx = _ctx(w, "f32", "1", monkeypatch)
-    ref_ctx.push(np.arange(w.geom.n_angles), proj)
-    ref = ref_ctx.bp_tasks(T, w.geom.n_angles, out_len)
+    ref_ctx.push(np.arange(A), proj)
+    refs = {nv: ref_ctx.bp_tasks(T, nv, out_len) for nv in (A, A - 1, 2, 1)}
     for storage, tol in (("f32", 1e-6), ("f16", 1e-3), ("bf16", 5e-3)):
         ctx = _ctx(w, storage, variant, monkeypatch)
-        ctx.push(np.arange(w.geom.n_angles), proj)
-        got = ctx.bp_tasks(T, w.geom.n_angles, out_len)
-        np.testing.assert_array_equal(got == 0, ref == 0)
-        assert rel_l2(got, ref) < tol, (variant, ppt, storage, rel_l2(got, ref))
+        ctx.push(np.arange(A), proj)
+        for nv, ref in refs.items():
+            got = ctx.bp_tasks(T, nv, out_len)
+            np.testing.assert_array_equal(got == 0, ref == 0)
+            assert rel_l2(got, ref) < tol, (variant, ppt, layout, depth, storage, nv,
+                                            rel_l2(got, ref))
     # parallel beam through the same variant
     from minirecast import phantom, recon
     from minirecast.geometry import ParallelGeometry

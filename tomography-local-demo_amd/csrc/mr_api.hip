@@ -349,7 +349,12 @@ int mr_ctx_create(int device, mr_ctx** out) {
         const int k = std::atoi(v);
         c->bp_variant = (k >= 1 && k <= 5) ? k : c->bp_variant;
     }
-    if (const char* v = std::getenv("MINIRECAST_BP_PPT")) c->bp_ppt = std::atoi(v) == 2 ? 2 : 4;
+    if (const char* v = std::getenv("MINIRECAST_BP_PPT")) {
+        const int k = std::atoi(v);
+        c->bp_ppt = (k == 1 || k == 2) ? k : 4;
+    }
+    if (const char* v = std::getenv("MINIRECAST_BP_LAYOUT")) c->bp_layout = std::atoi(v) ? 1 : 0;
+    if (const char* v = std::getenv("MINIRECAST_BP_DEPTH")) c->bp_depth = std::atoi(v) >= 3 ? 3 : 2;
     if (const char* v = std::getenv("MINIRECAST_XCD_MAP")) c->xcd_map = std::atoi(v) ? 1 : 0;
     // default-priority, non-blocking streams
     if (hipStreamCreateWithFlags(&c->s_ingest, hipStreamNonBlocking) != hipSuccess ||

@@ -765,33 +765,56 @@ struct PointState {
     Pair<ST> tap[PPT];
 };
 
-template <bool CONE, typename ST, int PPT>
+// Lane layouts of v4: which in-tile point (dp along a, ds along b) thread t handles
+// as its m-th point, in a workgroup that covers rows [sub * 8 * PPT, (sub + 1) * 8 * PPT)
+// of the 32 x 32 tile.  LAY 0: each wave covers 2 rows x 32 columns per point
+// (points 8 rows apart); LAY 1: each wave covers a 4 x 16 patch per point.  A gather
+// instruction costs one cache-line lookup per distinct line its 64 lanes touch, and the
+// better layout depends on how the slice axes project onto the detector.
+template <int PPT, int LAY> struct Map4 {
+    static constexpr int SUBS = BP_TP / (8 * PPT);
+    __device__ static inline void base(int t, int sub, int& p, int& s) {
+        if (LAY == 0) {
+            s = t & (BP_TS - 1);
+            p = (t >> 5) + sub * 8 * PPT;
+        } else {
+            const int l = t & 63, w = t >> 6;
+            s = l & 15;
+            if (PPT == 1) { p = sub * 8 + 4 * (w >> 1) + (l >> 4); s += 16 * (w & 1); }
+            else if (PPT == 2) p = sub * 16 + 4 * w + (l >> 4);
+            else p = 8 * w + (l >> 4);
+        }
+    }
+    static constexpr int dp(int m) { return LAY == 0 ? 8 * m : (PPT == 4 ? 4 * (m >> 1) : 0); }
+    static constexpr int ds(int m) { return LAY == 0 ? 0 : (PPT == 4 ? 16 * (m & 1) : 16 * m); }
+};
+
+template <bool CONE, typename ST, int PPT, int LAY, int DEPTH>
 __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel4(
     const ST* __restrict__ ring, int64_t view_stride, int pitch,
     const float* __restrict__ recs, const mr_task* __restrict__ tasks,
     const Tile* __restrict__ tiles, int n_views, float scale, float* __restrict__ out, int xcd) {
+    using M = Map4<PPT, LAY>;
     constexpr int LE = Taps<ST>::LOG_ESZ;
-    constexpr int PSTEP = BP_THREADS / BP_TS;        // 8 rows between a thread's points
-    constexpr int SUBS = BP_TP / (PSTEP * PPT);      // workgroups per 32-row tile
-    static_assert(SUBS * PSTEP * PPT == BP_TP, "tile shape");
+    constexpr int SUBS = M::SUBS;                    // workgroups per 32-row tile
+    static_assert(SUBS * 8 * PPT == BP_TP, "tile shape");
     const int lb = xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const int tile = lb / SUBS;
     const int sub = lb - tile * SUBS;
     const Tile tl = tiles[tile];
     const mr_task& T = tasks[tl.task];
     const int t = threadIdx.x;
-    const int tx = t & (BP_TS - 1);
-    const int ty = t / BP_TS + sub * PSTEP * PPT;    // first row of this thread in the tile
+    int pt, st0;
+    M::base(t, sub, pt, st0);
     const int p0 = tl.tp * BP_TP, s0 = tl.ts * BP_TS;
-    const int s = s0 + tx;
     bool valid[PPT], inside[PPT];
 #pragma unroll
     for (int m = 0; m < PPT; ++m) {
-        const int p = p0 + ty + PSTEP * m;
+        const int p = p0 + pt + M::dp(m), s = s0 + st0 + M::ds(m);
         valid[m] = (p < T.np) && (s < T.ns);
         inside[m] = valid[m] && (!T.mask_box || in_box(T, p, s));
     }
-    const float ds = static_cast<float>(tx), dp0 = static_cast<float>(ty);
+    const float ds = static_cast<float>(st0), dp0 = static_cast<float>(pt);
     const int pitchb = pitch << LE;
     const int view_bytes = static_cast<int>(view_stride << LE);
     const float* __restrict__ rt = recs + static_cast<int64_t>(tile) * n_views * BP_REC;
@@ -804,14 +827,22 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel4(
         // parallel beam: the row of a point is the same for every view
 #pragma unroll
         for (int m = 0; m < PPT; ++m) {
-            const float dp = dp0 + static_cast<float>(PSTEP * m);
-            float row = fmaf(dp, rt[5], fmaf(ds, rt[4], rt[3]));
+            const float dp = dp0 + static_cast<float>(M::dp(m));
+            const float dsm = ds + static_cast<float>(M::ds(m));
+            float row = fmaf(dp, rt[5], fmaf(dsm, rt[4], rt[3]));
             row = __builtin_amdgcn_fmed3f(row, rt[11], rt[12]);
             frp[m] = __builtin_amdgcn_fractf(row);
             roffp[m] = cvt_flr(row) * pitchb;
         }
     }
 
+    // coefficient c(m) = c0 + dp(m) * cp + ds(m) * cs with compile-time offsets
+    auto step = [](float c0, int m, float cp, float cs) {
+        float c = c0;
+        if (M::dp(m)) c = fmaf(static_cast<float>(M::dp(m)), cp, c);
+        if (M::ds(m)) c = fmaf(static_cast<float>(M::ds(m)), cs, c);
+        return c;
+    };
     // request the taps of view v into st
     auto issue = [&](int v, PointState<ST, PPT>& st) {
         const float* r = rt + static_cast<int64_t>(v) * BP_REC;
@@ -826,10 +857,9 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel4(
             const float wh0 = fmaf(dp0, Wb, fmaf(ds, Wa, W0));
 #pragma unroll
             for (int m = 0; m < PPT; ++m) {
-                const float dm = static_cast<float>(PSTEP * m);
-                const float uh = m ? fmaf(dm, Ub, uh0) : uh0;
-                const float vh = m ? fmaf(dm, Vb, vh0) : vh0;
-                const float wh = m ? fmaf(dm, Wb, wh0) : wh0;
+                const float uh = step(uh0, m, Ub, Ua);
+                const float vh = step(vh0, m, Vb, Va);
+                const float wh = step(wh0, m, Wb, Wa);
                 const float rw = __builtin_amdgcn_rcpf(wh);
                 const float col = __builtin_amdgcn_fmed3f(uh * rw, r[9], r[10]);
                 const float row = __builtin_amdgcn_fmed3f(vh * rw, r[11], r[12]);
@@ -843,8 +873,7 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel4(
             st.wt[0] = r[13];
 #pragma unroll
             for (int m = 0; m < PPT; ++m) {
-                const float dm = static_cast<float>(PSTEP * m);
-                const float col = __builtin_amdgcn_fmed3f(m ? fmaf(dm, Ub, uh0) : uh0, r[9], r[10]);
+                const float col = __builtin_amdgcn_fmed3f(step(uh0, m, Ub, Ua), r[9], r[10]);
                 st.fc[m] = __builtin_amdgcn_fractf(col);
                 const int voff = roffp[m] + ((cvt_flr(col) << LE) + baseb);
                 st.tap[m].load(rs, voff, pitchb);
@@ -865,24 +894,35 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel4(
         }
     };
 
+    // DEPTH views in flight: the taps of view v + DEPTH - 1 are requested before view v
+    // is interpolated.  Requests past the last view re-read it (never consumed), so the
+    // loop body is unconditional and every wait counts exactly the loads after it.
     if (n_views > 0) {
-        PointState<ST, PPT> s0st, s1st;
-        issue(0, s0st);
+        PointState<ST, PPT> S[DEPTH];
+        const int last = n_views - 1;
+#pragma unroll
+        for (int j = 0; j + 1 < DEPTH; ++j) issue(min(j, last), S[j]);
         int v = 0;
-        for (; v + 2 <= n_views; v += 2) {
-            if (v + 1 < n_views) issue(v + 1, s1st);
-            consume(s0st);
-            if (v + 2 < n_views) issue(v + 2, s0st);
-            consume(s1st);
+        for (; v + DEPTH <= n_views; v += DEPTH) {
+#pragma unroll
+            for (int j = 0; j < DEPTH; ++j) {
+                issue(min(v + j + DEPTH - 1, last), S[(j + DEPTH - 1) % DEPTH]);
+                // keep the requests ahead of the interpolation of older views (the
+                // scheduler would otherwise sink them below the waits)
+                __builtin_amdgcn_sched_barrier(0);
+                consume(S[j]);
+            }
         }
-        if (v < n_views) consume(s0st);
+#pragma unroll
+        for (int j = 0; j + 1 < DEPTH; ++j)
+            if (v + j < n_views) consume(S[j]);
     }
 
     const int64_t obase = T.out_offset + static_cast<int64_t>(tl.q) * T.stride_q;
 #pragma unroll
     for (int m = 0; m < PPT; ++m) {
         if (!valid[m]) continue;
-        const int p = p0 + ty + PSTEP * m;
+        const int p = p0 + pt + M::dp(m), s = s0 + st0 + M::ds(m);
         float val = acc[m];
         if (!CONE) val = fmaf(frp[m], accb[m] - acc[m], acc[m]);
         out[obase + static_cast<int64_t>(p) * T.stride_p + static_cast<int64_t>(s) * T.stride_s] =
@@ -1655,17 +1695,33 @@ int launch_bp_points(Ctx* c, const double* d_pts, int64_t M, double cor_shift, i
                : dispatch_points<false>(c, d_pts, M, cor_shift, n_views, scale, d_out, s);
 }
 
-template <bool CONE, typename ST>
-static void launch_v4_kernel(Ctx* c, const float* recs, const mr_task* d_tasks, const Tile* d_tiles,
-                             int nt, int n_views, float scale, float* d_out, hipStream_t s) {
-    if (c->bp_ppt == 2)
-        hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, 2>), dim3(2 * nt), dim3(BP_THREADS), 0, s,
+template <bool CONE, typename ST, int PPT, int LAY>
+static void launch_v4_one(Ctx* c, const float* recs, const mr_task* d_tasks, const Tile* d_tiles,
+                          int nt, int n_views, float scale, float* d_out, hipStream_t s) {
+    const dim3 grid(Map4<PPT, LAY>::SUBS * nt), block(BP_THREADS);
+    if (c->bp_depth >= 3)
+        hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, 3>), grid, block, 0, s,
                            static_cast<const ST*>(c->ring), c->view_stride, c->pitch, recs, d_tasks,
                            d_tiles, n_views, scale, d_out, c->xcd_map);
     else
-        hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, 4>), dim3(nt), dim3(BP_THREADS), 0, s,
+        hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, 2>), grid, block, 0, s,
                            static_cast<const ST*>(c->ring), c->view_stride, c->pitch, recs, d_tasks,
                            d_tiles, n_views, scale, d_out, c->xcd_map);
+}
+
+template <bool CONE, typename ST>
+static void launch_v4_kernel(Ctx* c, const float* recs, const mr_task* d_tasks, const Tile* d_tiles,
+                             int nt, int n_views, float scale, float* d_out, hipStream_t s) {
+    const int lay = c->bp_layout;
+    if (c->bp_ppt == 1)
+        lay ? launch_v4_one<CONE, ST, 1, 1>(c, recs, d_tasks, d_tiles, nt, n_views, scale, d_out, s)
+            : launch_v4_one<CONE, ST, 1, 0>(c, recs, d_tasks, d_tiles, nt, n_views, scale, d_out, s);
+    else if (c->bp_ppt == 2)
+        lay ? launch_v4_one<CONE, ST, 2, 1>(c, recs, d_tasks, d_tiles, nt, n_views, scale, d_out, s)
+            : launch_v4_one<CONE, ST, 2, 0>(c, recs, d_tasks, d_tiles, nt, n_views, scale, d_out, s);
+    else
+        lay ? launch_v4_one<CONE, ST, 4, 1>(c, recs, d_tasks, d_tiles, nt, n_views, scale, d_out, s)
+            : launch_v4_one<CONE, ST, 4, 0>(c, recs, d_tasks, d_tiles, nt, n_views, scale, d_out, s);
 }
 
 template <typename ST>

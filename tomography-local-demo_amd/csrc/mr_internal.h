@@ -87,6 +87,8 @@ struct Ctx {
     int bp_variant = 1;              // back-projection inner loop (MINIRECAST_BP_VARIANT)
     int bp_ppt = 4;                  // v4 points per thread (MINIRECAST_BP_PPT)
     int xcd_map = 0;                 // XCD-contiguous tile order (MINIRECAST_XCD_MAP=1)
+    int bp_layout = 0;               // v4 lane layout: 0 = 2 x 32, 1 = 4 x 16 (MINIRECAST_BP_LAYOUT)
+    int bp_depth = 2;                // v4 views in flight, 2 or 3 (MINIRECAST_BP_DEPTH)
     float* d_recs = nullptr;         // v4 per-(tile, view) records
     size_t recs_bytes = 0;
 
