@@ -13,9 +13,13 @@ Reported (one JSON line on stdout):
   launches, f16, bf16); rel-L2 and max-abs against the float64 oracle at 3 x 2048
   sampled in-box pixels (float64 filtering and back-projection of the same float32
   projections, oracle/liboracle_exact.so).
-  roofline   for the back-projection kernel: compulsory HBM bytes (union of the 2x2 tap
-             footprints over all requested points, per view, x bytes per sample, plus
-             the output) / average kernel time (HIP events), against 8 TB/s.
+  roofline   for the back-projection tile kernel: compulsory HBM bytes (union of the 2x2
+             tap footprints over all requested points, per view, x bytes per sample,
+             plus the output) / its average duration (HIP events around the tile kernel
+             on its stream), against 8 TB/s; traffic = measured HBM bytes per launch
+             from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) of a short run of
+             the same f16 update, each scaled by the counter's reading for a 1 GiB
+             device copy (null when rocprofv3 is unavailable or --no-traffic).
   cpu_baseline  the Numba-equivalent CPU restatement (oracle/liboracle_fast.so:
              fast-math, OpenMP over points, views serial, float64 accumulation),
              timed on this host on a bounded sample of the same update.
@@ -23,9 +27,14 @@ Reported (one JSON line on stdout):
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-check] [--no-cpu]
 """
 import argparse
+import csv
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import threading
 import time
 
@@ -67,6 +76,60 @@ def footprint_pixels(geom, pts, views):
     return float(np.mean(counts))
 
 
+def _pmc_pass(counter):
+    """One rocprofv3 --pmc pass of a short f16 run of this benchmark (child process):
+    mean counter value per dispatch of the f16 cone tile kernel, and the value for the
+    1 GiB copy run first by the child."""
+    out = tempfile.mkdtemp(prefix="mr_pmc_")
+    cmd = ["rocprofv3", "--pmc", counter, "-d", out, "-o", "run", "--output-format", "csv",
+           "--", sys.executable, "-u", os.path.abspath(__file__), "--pmc-child", "--steps", "2",
+           "--warmup", "1", "--no-check", "--no-cpu", "--forms", "f16", "--calibrate-copy"]
+    try:
+        with open(os.path.join(out, "child.log"), "w") as fh:
+            subprocess.run(cmd, stdout=fh, stderr=subprocess.STDOUT, timeout=240, check=False)
+        files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            return None
+        bp, cp = [], []
+        with open(files[0]) as fh:
+            for row in csv.DictReader(fh):
+                if row.get("Counter_Name") != counter:
+                    continue
+                name, val = row.get("Kernel_Name", ""), float(row["Counter_Value"])
+                if ("bp_tiles_kernel4<true, __half" in name or
+                        "bp_tiles_kernel4ILb1E6__half" in name):
+                    bp.append(val)
+                elif "copyBuffer" in name or "copy" in name.lower():
+                    cp.append(val)
+        if not bp or not cp:
+            return None
+        return float(np.mean(bp)), float(max(cp))
+    except (OSError, subprocess.SubprocessError, ValueError, KeyError):
+        return None
+    finally:
+        shutil.rmtree(out, ignore_errors=True)
+
+
+def pmc_traffic():
+    """HBM bytes per launch of the f16 tile kernel: FETCH_SIZE and WRITE_SIZE (two
+    passes, one counter group each), each scaled by its reading for the 1 GiB copy."""
+    if shutil.which("rocprofv3") is None:
+        log("traffic: rocprofv3 not found")
+        return None
+    res = {}
+    for counter, key in (("FETCH_SIZE", "fetch"), ("WRITE_SIZE", "write")):
+        r = _pmc_pass(counter)
+        if r is None:
+            log(f"traffic: {counter} pass gave no reading")
+            return None
+        bp, copy = r
+        scale = float(1 << 30) / copy            # bytes per counter unit
+        res[key] = bp * scale
+        res[key + "_bytes_per_unit"] = round(scale, 3)
+    log(f"traffic: {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +139,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--views", type=int, default=None, help="debug: fewer views")
     ap.add_argument("--forms", default="f32,f16,bf16", help="storage forms to time")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE passes")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a 1 GiB device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -174,14 +240,17 @@ def main():
         c.synchronize()
         kms = []
         t1 = time.perf_counter()
+        tms = []
         for _ in range(args.steps):
             c.bp_tasks_device(tasks, A, dout, out_len)
             kms.append(c.last_kernel_ms())
+            tms.append(c.last_tile_ms())
         c.synchronize()
         wall = (time.perf_counter() - t1) * 1e3 / args.steps
         img = np.empty(out_len, np.float32)
         c.d2h(img, dout)
-        res = dict(ms=wall, kernel_ms=float(np.mean(kms)), gups=n_in * A / (wall * 1e-3) / 1e9)
+        res = dict(ms=wall, kernel_ms=float(np.mean(kms)), tile_kernel_ms=float(np.mean(tms)),
+                   gups=n_in * A / (wall * 1e-3) / 1e9, config=c.bp_config())
         if check:
             got = np.concatenate([img[o:o + p['n'] ** 2][si] for o, p, si in
                                   zip(np.cumsum([0] + [p['n'] ** 2 for p in w.planes[:-1]]),
@@ -220,14 +289,21 @@ def main():
     fviews = np.linspace(0, A - 1, 8).astype(int)
     F = footprint_pixels(geom, all_pts, fviews)
     head = "f16" if "f16" in forms else "f32"
-    hbm_bytes = F * A * 2 + out_len * 4
-    kern_s = results[head]["kernel_ms"] * 1e-3
+    esz = 2 if head in ("f16", "bf16") else 4
+    hbm_bytes = F * A * esz + out_len * 4
+    kern_s = results[head]["tile_kernel_ms"] * 1e-3
     achieved = hbm_bytes / kern_s / 1e9
+    traffic = None
+    if not args.no_traffic and not args.pmc_child and head == "f16":
+        traffic = pmc_traffic()
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel=f"bp_tiles_kernel (cone, {head})",
+                    frac=round(achieved / HBM_PEAK_GBS, 4),
+                    traffic=(int(traffic["fetch"] + traffic["write"]) if traffic else None),
+                    kernel=f"bp_tiles_kernel4 (cone, {head})",
                     algorithmic_bytes=int(hbm_bytes), footprint_px_per_view=round(F),
-                    gather_bytes_l2=int(4 * 2 * n_in * A))
+                    gather_bytes_l2=int(4 * esz * n_in * A))
+    if traffic:
+        roofline["traffic_detail"] = traffic
     log(f"roofline: {roofline}")
 
     # ---------------------------------------------------------------- CPU baseline

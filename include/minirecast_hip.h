@@ -91,6 +91,28 @@ int mr_push_views(mr_ctx* ctx, const int64_t* indices, int k, const float* block
 int mr_store_views(mr_ctx* ctx, const int64_t* indices, int k, const float* q, int q_on_device,
                    int* n_out);
 
+/* Raw-count correction for mr_push_raw, replacing the host-side correction of the
+ * loaders (flexray.py:174-184 corrected(), with _read_binned flexray.py:111-116;
+ * hdf5.py:77-78, with _bin hdf5.py:24-31).  Raw frames are [frame_rows][frame_cols]
+ * uint16 detector images.  Each is binned by sample x sample block means, optionally
+ * flipped vertically (flip = 1: flexray's np.flipud), and cropped to the geometry's
+ * rows x cols at binned (row0, col0).  dark and scale are [rows][cols] float32 in that
+ * cropped frame; the line integral is
+ *     -log(max(y, 1e-3)),  y = (x - dark) * scale (op 0, flexray: scale = 1/max(flat-dark, 1))
+ *                          y = (x - dark) / scale (op 1, hdf5:    scale = max(flat-dark, 1e-6)).
+ * Valid until the geometry is set again. */
+int mr_set_raw_correction(mr_ctx* ctx, const float* dark, const float* scale, int op,
+                          int frame_rows, int frame_cols, int sample, int flip, int row0,
+                          int col0);
+
+/* mr_push_views for raw frames [k][frame_rows][frame_cols] uint16 (host or device
+ * pointer): the correction above runs on the device, then preprocessing, filtering
+ * and storage as in mr_push_views.  Uploads 2 bytes per raw pixel instead of 4 per
+ * corrected pixel (no counterpart in the reference: its loaders correct on the host
+ * before recon.py:346 add_projections). */
+int mr_push_raw(mr_ctx* ctx, const int64_t* indices, int k, const uint16_t* frames,
+                int frames_on_device, int* n_out);
+
 /* Preprocessing applied by mr_push_views to every later block (preprocess.py:59-64
  * apply, called at recon.py:357): rotation by -tilt degrees about the detector centre,
  * then a Gaussian low-pass of sigma = gaussian detector pixels; 0 disables each. */
@@ -139,6 +161,24 @@ typedef struct mr_task {
  * task addresses are left as they are. */
 int mr_bp_tasks(mr_ctx* ctx, const mr_task* tasks, int ntasks, int n_views,
                 float* out, int64_t out_len, int out_on_device);
+
+/* ---- back-projection tuning (no counterpart in the reference: its Numba loop has
+ * no shape parameters)
+ *
+ * cfg[4] = {kernel, points per thread, lane layout, views in flight}: kernel 2 (device
+ * work queue, records staged in LDS), 4 (records precomputed in global memory, the
+ * default) or 5 (cone only: per-view detector boxes staged in LDS); points per thread
+ * 1, 2 or 4; lane layout 0 (waves of 2 x 32 points) or 1 (4 x 16); 2 or 3 views in
+ * flight (kernel 4). */
+int mr_set_bp_config(mr_ctx* ctx, int kernel, int ppt, int layout, int depth);
+int mr_get_bp_config(mr_ctx* ctx, int* cfg);
+
+/* Autotuner: times every configuration on an evenly spaced sample of at most
+ * max_tiles 32 x 32 tiles of `tasks` (reps timed runs each, after one warm-up, against
+ * the first n_views stored views), keeps the fastest for this context and returns it
+ * in cfg[4] with its time per run in *best_ms.  out_len sizes a scratch output. */
+int mr_autotune(mr_ctx* ctx, const mr_task* tasks, int ntasks, int n_views, int64_t out_len,
+                int max_tiles, int reps, int* cfg, float* best_ms);
 
 /* Rotation-centre search scores: recon.py:498-513.  Each task is one trial image
  * (nq = 1, all np x ns equal; out_offset/strides are ignored): it is back-projected
@@ -230,6 +270,11 @@ int mr_synchronize(mr_ctx* ctx);
 /* Timing of the last mr_bp_tasks / mr_bp_points kernel on the request stream,
  * measured with HIP events around the back-projection kernel (milliseconds). */
 int mr_last_kernel_ms(mr_ctx* ctx, double* ms);
+
+/* The tile kernel alone within the last mr_bp_tasks (kernels 4 and 5 first build their
+ * per-view records with a separate kernel; for the others this equals
+ * mr_last_kernel_ms). */
+int mr_last_tile_kernel_ms(mr_ctx* ctx, double* ms);
 
 #ifdef __cplusplus
 }

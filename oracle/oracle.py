@@ -544,6 +544,48 @@ def preprocess_apply(block, gaussian=0.0, tilt=0.0):
 
 
 # --------------------------------------------------------------------------
+# Raw-count correction of the loaders (flexray.py:111-116, 174-184; hdf5.py:24-31, 77-78)
+# --------------------------------------------------------------------------
+def bin_frames(im, sample):
+    """float32 means of sample x sample blocks of the last two axes, trailing rows and
+    columns dropped (flexray.py:111-115 _read_binned, hdf5.py:24-31 _bin)."""
+    im = np.asarray(im).astype(np.float32)
+    if sample <= 1:
+        return im
+    *lead, h, w = im.shape
+    h2, w2 = (h // sample) * sample, (w // sample) * sample
+    im = im[..., :h2, :w2].reshape(*lead, h2 // sample, sample, w2 // sample, sample)
+    return im.mean(axis=(-1, -3))
+
+
+def raw_correct(frames, dark, flat, style, sample, flip, crop):
+    """uint16 frames [k, H, W] -> line integrals [R, k, C] float32 in the loaders'
+    float32 arithmetic: flexray.py:174-184 corrected() (gain = 1/max(flat-dark, 1),
+    multiply, clip, log, negate) or hdf5.py:77-78 (divide by max(flat-dark, 1e-6));
+    flexray's np.flipud (flexray.py:116) when flip; crop (r0, r1, c0, c1) as
+    flexray.py:201."""
+    dark = np.asarray(dark, np.float32)
+    flat = np.asarray(flat, np.float32)
+    r0, r1, c0, c1 = crop
+    out = []
+    for f in frames:
+        im = bin_frames(f, sample)
+        if flip:
+            im = np.flipud(im)
+        if style == "flexray":
+            gain = 1.0 / np.maximum(flat - dark, 1.0)
+            im = (im - dark) * gain
+            np.clip(im, 1e-3, None, out=im)
+            np.log(im, out=im)
+            np.negative(im, out=im)
+        else:
+            trans = (im - dark) / np.maximum(flat - dark, 1e-6)
+            im = -np.log(np.clip(trans, 1e-3, None)).astype(np.float32)
+        out.append(im[r0:r1, c0:c1])
+    return np.ascontiguousarray(np.stack(out, axis=1))
+
+
+# --------------------------------------------------------------------------
 # Analytic ellipsoid phantom (phantom.py:25-98), generalised to 3-D rotations
 # --------------------------------------------------------------------------
 def line_integrals(centers, mats, vals, starts, dirs):

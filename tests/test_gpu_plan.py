@@ -128,3 +128,27 @@ def test_bp_variants_full_check(workload, variant, ppt, layout, depth, monkeypat
     orc.add_projections(np.arange(90), pp)
     ref2, _ = orc.reconstruct_plane((1.0, 2.0, 3.0), (1.0, 0.5, 0.8), n=70, cor_shift=0.6)
     assert rel_l2(img, ref2) < 1e-5
+
+
+def test_autotune_picks_a_configuration_and_keeps_parity(workload):
+    """The autotuner times the configurations, keeps one, and the tuned context still
+    matches the plain v1 kernel; configurations are validated."""
+    from minirecast import _hip, recon
+    w, proj, planes, tasks, out_len = workload
+    T = np.array(tasks, dtype=_hip.TASK_DTYPE)
+    A = w.geom.n_angles
+    ref_ctx = _hip.Context(0)
+    ref_ctx.set_cone(w.geom.vectors, w.geom.rows, w.geom.cols, w.window)
+    ref_ctx.set_bp_config(kernel=1, ppt=4, layout=0, depth=2)
+    ref_ctx.push(np.arange(A), proj)
+    ref = ref_ctx.bp_tasks(T, A, out_len)
+    rec = recon.SliceReconstructor(proj, w.geom, window=w.window, verbose=False)
+    cfg, ms = rec.autotune(planes, max_tiles=64, reps=1)
+    assert ms > 0 and cfg == rec._ctx.bp_config()
+    assert cfg["kernel"] in (2, 4, 5) and cfg["ppt"] in (1, 2, 4)
+    got = rec._ctx.bp_tasks(T, A, out_len)
+    assert rel_l2(got, ref) < 1e-6, (cfg, rel_l2(got, ref))
+    with pytest.raises(ValueError):
+        rec._ctx.set_bp_config(kernel=7)
+    with pytest.raises(ValueError):
+        rec._ctx.set_bp_config(kernel=4, ppt=3)

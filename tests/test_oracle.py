@@ -372,3 +372,56 @@ def test_bp_cone_dshift_matches_finite_difference():
         return O.bp_cone(q, K['s'], D, K['uu'], K['vv'], K['n'], K['sdd'], K['sod'], pts, 0.3)
     fd = (f(s0 + h) - f(s0 - h)) / (2 * h)
     np.testing.assert_allclose(O.bp_cone_dshift(q, *base, s0), fd, rtol=1e-5, atol=1e-7)
+
+
+# ---------------------------------------------------------------- raw-count correction
+@pytest.mark.parametrize("style", ["flexray", "hdf5"])
+def test_raw_correct_known_answers(style):
+    """The loaders' correction: transmission 1 gives 0, transmission <= 1e-3 is clipped
+    to -log(1e-3), a constant frame gives -log((I - D) / (F - D)), and binning, flip
+    and crop pick the pixels a brute-force loop picks."""
+    rng = np.random.default_rng(5)
+    s, H, W = 2, 13, 17                       # trailing odd row/column dropped by binning
+    hb, wb = H // s, W // s
+    dark = rng.uniform(90, 110, (hb, wb)).astype(np.float32)
+    flat = rng.uniform(9e3, 1.1e4, (hb, wb)).astype(np.float32)
+    crop = (1, 5, 2, 7)
+    flip = style == "flexray"
+    # frames whose binned values equal flat and dark exactly (constant blocks)
+    def frame_of(img):
+        f = np.repeat(np.repeat(img, s, 0), s, 1)
+        big = np.zeros((H, W), np.float32)
+        big[:hb * s, :wb * s] = f
+        return big
+    for img, want in ((flat, 0.0), (dark, -np.log(np.float32(1e-3)))):
+        src = np.flipud(img) if flip else img
+        fr = frame_of(np.round(src)).astype(np.uint16)[None]
+        d, fl = np.round(dark).astype(np.float32), np.round(flat).astype(np.float32)
+        got = O.raw_correct(fr, d, fl, style, s, flip, crop)
+        ref = np.full(got.shape, want, np.float32)
+        if want == 0.0:
+            # the binned frame equals the flat field exactly
+            tgt = np.round(img)
+            np.testing.assert_array_equal(fr[0][:hb * s:s, :wb * s:s][::-1 if flip else 1], tgt)
+        np.testing.assert_allclose(got, ref, atol=2e-6)
+    # constant raw frame: closed form
+    I = 5000
+    fr = np.full((1, H, W), I, np.uint16)
+    got = O.raw_correct(fr, dark, flat, style, s, flip, crop)[:, 0, :]
+    r0, r1, c0, c1 = crop
+    want = -np.log((I - dark.astype(float)) / (flat.astype(float) - dark))[r0:r1, c0:c1]
+    np.testing.assert_allclose(got, want, rtol=1e-5)
+    # brute force binning, flip and crop on random counts
+    fr = rng.integers(0, 65535, (2, H, W)).astype(np.uint16)
+    got = O.raw_correct(fr, dark, flat, style, s, flip, crop)
+    for k in range(2):
+        for r in range(r1 - r0):
+            for c in range(c1 - c0):
+                br = r0 + r
+                if flip:
+                    br = hb - 1 - br
+                bc = c0 + c
+                x = float(fr[k, br * s:br * s + s, bc * s:bc * s + s].astype(float).mean())
+                dd, ff = float(dark[r0 + r, c0 + c]), float(flat[r0 + r, c0 + c])
+                y = (x - dd) / (max(ff - dd, 1.0) if style == "flexray" else max(ff - dd, 1e-6))
+                assert abs(got[r, k, c] - (-np.log(max(y, 1e-3)))) < 1e-5

@@ -84,6 +84,10 @@ void free_geometry(Ctx* c) {
     if (c->d_H) hipFree(c->d_H);
     if (c->d_tw) hipFree(c->d_tw);
     c->ring = nullptr; c->d_geom = nullptr; c->d_vscale = nullptr; c->d_H = nullptr; c->d_tw = nullptr;
+    // the raw-count correction is per detector frame: a new geometry drops it
+    if (c->raw.d_dark) hipFree(c->raw.d_dark);
+    if (c->raw.d_scale) hipFree(c->raw.d_scale);
+    c->raw = RawCorr{};
     c->kind = MR_KIND_NONE;
     c->n = 0;
 }
@@ -221,6 +225,7 @@ int run_tasks(Ctx* c, const mr_task* tasks, int ntasks, int n_views, float* d_ou
         rc = launch_zero_tiles(d_tasks, d_tiles + nt, static_cast<int>(ne), d_out, c->s_req);
         if (rc) return rc;
     }
+    c->tile_timed = false;
     MR_HIP(hipEventRecord(c->ev0, c->s_req));
     if (op == OP_BP)
         rc = launch_bp_tiles(c, d_tasks, d_tiles, nt, n_views, scale, d_out, c->s_req);
@@ -238,6 +243,11 @@ int finish_timing(Ctx* c) {
     float ms = 0.0f;
     MR_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_kernel_ms = ms;
+    c->last_tile_ms = ms;
+    if (c->tile_timed) {
+        MR_HIP(hipEventElapsedTime(&ms, c->evk0, c->evk1));
+        c->last_tile_ms = ms;
+    }
     return MR_OK;
 }
 
@@ -359,7 +369,8 @@ int mr_ctx_create(int device, mr_ctx** out) {
     // default-priority, non-blocking streams
     if (hipStreamCreateWithFlags(&c->s_ingest, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->s_req, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->evk0) != hipSuccess || hipEventCreate(&c->evk1) != hipSuccess) {
         delete c;
         return fail(MR_ERUNTIME, "stream/event creation failed");
     }
@@ -373,12 +384,14 @@ int mr_ctx_destroy(mr_ctx* c) {
     hipDeviceSynchronize();
     free_geometry(c);
     void* bufs[] = {c->d_stage, c->d_fstage, c->d_vmax, c->d_fv, c->d_req, c->d_out, c->d_pts, c->d_misc,
-                    c->d_pre, c->d_recs};
+                    c->d_pre, c->d_recs, c->d_raw};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (c->h_req) hipHostFree(c->h_req);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->evk0) hipEventDestroy(c->evk0);
+    if (c->evk1) hipEventDestroy(c->evk1);
     if (c->s_ingest) hipStreamDestroy(c->s_ingest);
     if (c->s_req) hipStreamDestroy(c->s_req);
     delete c;
@@ -490,11 +503,10 @@ int mr_set_geometry_parallel(mr_ctx* c, int n_total, int rows, int cols, const d
     return MR_OK;
 }
 
-static int append_views(mr_ctx* c, const int64_t* indices, int k, const float* block,
-                        int on_device, int* n_out, bool filter) {
-    int rc = check_ctx(c);
-    if (rc) return rc;
-    std::lock_guard<std::mutex> lk(c->mu);
+// Caller holds c->mu.  in_stage: the block is already in c->d_stage (device).
+static int append_views_locked(mr_ctx* c, const int64_t* indices, int k, const float* block,
+                               int on_device, int* n_out, bool filter, bool in_stage) {
+    int rc = MR_OK;
     MR_CHECK(c->kind != MR_KIND_NONE, MR_ERUNTIME, "no geometry set");
     MR_CHECK(k >= 0, MR_EVALUE, "negative view count");
     if (k == 0) {
@@ -512,7 +524,9 @@ static int append_views(mr_ctx* c, const int64_t* indices, int k, const float* b
     const size_t blk = sizeof(float) * static_cast<size_t>(c->R) * k * c->C;
     const float* d_block = block;
     const bool pre = filter && (c->gaussian > 0.0 || std::fabs(c->tilt) >= 1e-9);
-    if (!on_device || pre) {
+    if (in_stage) {
+        d_block = c->d_stage;
+    } else if (!on_device || pre) {
         rc = ensure(reinterpret_cast<void**>(&c->d_stage), &c->stage_bytes, blk);
         if (rc) return rc;
         MR_HIP(hipMemcpyAsync(c->d_stage, block, blk,
@@ -557,9 +571,81 @@ static int append_views(mr_ctx* c, const int64_t* indices, int k, const float* b
     return MR_OK;
 }
 
+static int append_views(mr_ctx* c, const int64_t* indices, int k, const float* block,
+                        int on_device, int* n_out, bool filter) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    return append_views_locked(c, indices, k, block, on_device, n_out, filter, false);
+}
+
 int mr_push_views(mr_ctx* c, const int64_t* indices, int k, const float* block,
                   int block_on_device, int* n_out) {
     return append_views(c, indices, k, block, block_on_device, n_out, true);
+}
+
+int mr_set_raw_correction(mr_ctx* c, const float* dark, const float* scale, int op, int frame_rows,
+                          int frame_cols, int sample, int flip, int row0, int col0) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    MR_CHECK(c->kind != MR_KIND_NONE, MR_ERUNTIME, "no geometry set");
+    MR_CHECK(dark && scale, MR_EVALUE, "null dark or scale");
+    MR_CHECK(op == 0 || op == 1, MR_EVALUE, "op must be 0 (multiply) or 1 (divide)");
+    MR_CHECK(sample >= 1 && sample <= 16, MR_EVALUE, "sample must be in 1..16");
+    MR_CHECK(frame_rows > 0 && frame_cols > 0, MR_EVALUE, "empty raw frame");
+    const int hb = frame_rows / sample, wb = frame_cols / sample;
+    MR_CHECK(row0 >= 0 && col0 >= 0 && row0 + c->R <= hb && col0 + c->C <= wb, MR_EVALUE,
+             "crop " + std::to_string(c->R) + "x" + std::to_string(c->C) + " at (" +
+                 std::to_string(row0) + ", " + std::to_string(col0) + ") exceeds the binned frame " +
+                 std::to_string(hb) + "x" + std::to_string(wb));
+    MR_HIP(hipSetDevice(c->device));
+    const size_t bytes = sizeof(float) * static_cast<size_t>(c->R) * c->C;
+    if (!c->raw.d_dark) MR_HIP(hipMalloc(reinterpret_cast<void**>(&c->raw.d_dark), bytes));
+    if (!c->raw.d_scale) MR_HIP(hipMalloc(reinterpret_cast<void**>(&c->raw.d_scale), bytes));
+    MR_HIP(hipMemcpy(c->raw.d_dark, dark, bytes, hipMemcpyHostToDevice));
+    MR_HIP(hipMemcpy(c->raw.d_scale, scale, bytes, hipMemcpyHostToDevice));
+    c->raw.H = frame_rows;
+    c->raw.W = frame_cols;
+    c->raw.sample = sample;
+    c->raw.flip = flip ? 1 : 0;
+    c->raw.row0 = row0;
+    c->raw.col0 = col0;
+    c->raw.op = op;
+    c->raw.set = 1;
+    return MR_OK;
+}
+
+int mr_push_raw(mr_ctx* c, const int64_t* indices, int k, const uint16_t* frames,
+                int frames_on_device, int* n_out) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    MR_CHECK(c->raw.set, MR_ERUNTIME, "no raw correction set (mr_set_raw_correction)");
+    MR_CHECK(k >= 0, MR_EVALUE, "negative view count");
+    if (k == 0) {
+        if (n_out) *n_out = c->n;
+        return MR_OK;
+    }
+    MR_CHECK(indices && frames, MR_EVALUE, "null indices or frames");
+    MR_CHECK(c->n + k <= c->n_total, MR_EVALUE,
+             "ring full: " + std::to_string(c->n) + " + " + std::to_string(k) + " views > " +
+                 std::to_string(c->n_total));
+    MR_HIP(hipSetDevice(c->device));
+    const size_t fbytes = sizeof(uint16_t) * static_cast<size_t>(c->raw.H) * c->raw.W * k;
+    const uint16_t* d_frames = frames;
+    if (!frames_on_device) {
+        rc = ensure(reinterpret_cast<void**>(&c->d_raw), &c->raw_bytes, fbytes);
+        if (rc) return rc;
+        MR_HIP(hipMemcpyAsync(c->d_raw, frames, fbytes, hipMemcpyHostToDevice, c->s_ingest));
+        d_frames = c->d_raw;
+    }
+    const size_t blk = sizeof(float) * static_cast<size_t>(c->R) * k * c->C;
+    rc = ensure(reinterpret_cast<void**>(&c->d_stage), &c->stage_bytes, blk);
+    if (rc) return rc;
+    rc = launch_raw_correct(c, d_frames, k, c->d_stage, c->s_ingest);
+    if (rc) return rc;
+    return append_views_locked(c, indices, k, c->d_stage, 1, n_out, true, true);
 }
 
 int mr_store_views(mr_ctx* c, const int64_t* indices, int k, const float* q, int q_on_device,
@@ -671,6 +757,118 @@ int mr_bp_tasks(mr_ctx* c, const mr_task* tasks, int ntasks, int n_views, float*
     if (!out_on_device)
         MR_HIP(hipMemcpyAsync(out, d_out, sizeof(float) * out_len, hipMemcpyDeviceToHost, c->s_req));
     return finish_timing(c);
+}
+
+namespace {
+// Back-projection configurations the autotuner times: kernel (2: LDS records and
+// device queue, 4: precomputed records, 5: LDS-staged boxes, cone only), points per
+// thread, lane layout (2 x 32 or 4 x 16 waves) and views in flight.
+struct BpConfig { int variant, ppt, layout, depth; };
+const BpConfig kBpCandidates[] = {
+    {4, 2, 0, 2}, {4, 2, 0, 3}, {4, 2, 1, 2}, {4, 2, 1, 3}, {4, 1, 0, 2}, {4, 1, 0, 3},
+    {4, 1, 1, 2}, {4, 1, 1, 3}, {4, 4, 0, 2}, {4, 4, 1, 2}, {5, 2, 0, 2}, {5, 4, 0, 2},
+    {2, 4, 0, 2}};
+
+bool valid_config(int variant, int ppt, int layout, int depth) {
+    return (variant >= 1 && variant <= 5) && (ppt == 1 || ppt == 2 || ppt == 4) &&
+           (layout == 0 || layout == 1) && (depth == 2 || depth == 3);
+}
+}  // namespace
+
+int mr_set_bp_config(mr_ctx* c, int variant, int ppt, int layout, int depth) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    MR_CHECK(valid_config(variant, ppt, layout, depth), MR_EVALUE,
+             "invalid back-projection configuration");
+    c->bp_variant = variant;
+    c->bp_ppt = ppt;
+    c->bp_layout = layout;
+    c->bp_depth = depth;
+    return MR_OK;
+}
+
+int mr_get_bp_config(mr_ctx* c, int* cfg) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    MR_CHECK(cfg, MR_EVALUE, "null configuration");
+    std::lock_guard<std::mutex> lk(c->mu);
+    cfg[0] = c->bp_variant;
+    cfg[1] = c->bp_ppt;
+    cfg[2] = c->bp_layout;
+    cfg[3] = c->bp_depth;
+    return MR_OK;
+}
+
+int mr_autotune(mr_ctx* c, const mr_task* tasks, int ntasks, int n_views, int64_t out_len,
+                int max_tiles, int reps, int* cfg, float* best_ms) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    MR_CHECK(c->kind != MR_KIND_NONE, MR_ERUNTIME, "no geometry set");
+    MR_CHECK(ntasks >= 1 && tasks && out_len >= 1, MR_EVALUE, "autotuning needs tasks and an output");
+    MR_CHECK(n_views >= 1 && n_views <= c->n, MR_EVALUE,
+             "n_views " + std::to_string(n_views) + " outside [1, " + std::to_string(c->n) + "]");
+    MR_CHECK(max_tiles >= 1 && reps >= 1 && reps <= 100, MR_EVALUE, "max_tiles >= 1, reps in 1..100");
+    rc = validate_tasks(tasks, ntasks, out_len);
+    if (rc) return rc;
+    MR_HIP(hipSetDevice(c->device));
+    std::vector<Tile> all, tiles;
+    build_tiles(tasks, ntasks, all);
+    // an evenly spaced sample of at most max_tiles tiles keeps tuning cheap
+    if (static_cast<int64_t>(all.size()) > max_tiles) {
+        for (int i = 0; i < max_tiles; ++i)
+            tiles.push_back(all[static_cast<size_t>(i) * all.size() / max_tiles]);
+    } else {
+        tiles = all;
+    }
+    if (cfg) { cfg[0] = c->bp_variant; cfg[1] = c->bp_ppt; cfg[2] = c->bp_layout; cfg[3] = c->bp_depth; }
+    if (best_ms) *best_ms = 0.0f;
+    if (tiles.empty()) return MR_OK;
+    const size_t tb = sizeof(mr_task) * ntasks, tl = sizeof(Tile) * tiles.size();
+    const size_t off_tiles = (tb + 255) / 256 * 256;
+    rc = ensure(&c->d_req, &c->req_bytes, off_tiles + tl + 256);
+    if (rc) return rc;
+    rc = ensure_host(&c->h_req, &c->h_req_bytes, off_tiles + tl + 256);
+    if (rc) return rc;
+    std::memcpy(c->h_req, tasks, tb);
+    std::memcpy(static_cast<char*>(c->h_req) + off_tiles, tiles.data(), tl);
+    MR_HIP(hipMemcpyAsync(c->d_req, c->h_req, off_tiles + tl, hipMemcpyHostToDevice, c->s_req));
+    rc = ensure(reinterpret_cast<void**>(&c->d_out), &c->out_bytes, sizeof(float) * out_len);
+    if (rc) return rc;
+    const mr_task* d_tasks = static_cast<const mr_task*>(c->d_req);
+    const Tile* d_tiles = reinterpret_cast<const Tile*>(static_cast<char*>(c->d_req) + off_tiles);
+    const float scale = static_cast<float>(M_PI / n_views);
+    const int nt = static_cast<int>(tiles.size());
+    BpConfig best{c->bp_variant, c->bp_ppt, c->bp_layout, c->bp_depth};
+    float best_t = 1e30f;
+    for (const BpConfig& k : kBpCandidates) {
+        if (k.variant == 5 && c->kind != MR_KIND_CONE) continue;
+        c->bp_variant = k.variant;
+        c->bp_ppt = k.ppt;
+        c->bp_layout = k.layout;
+        c->bp_depth = k.depth;
+        rc = launch_bp_tiles(c, d_tasks, d_tiles, nt, n_views, scale, c->d_out, c->s_req);  // warm-up
+        if (rc) break;
+        MR_HIP(hipEventRecord(c->ev0, c->s_req));
+        for (int r = 0; r < reps && !rc; ++r)
+            rc = launch_bp_tiles(c, d_tasks, d_tiles, nt, n_views, scale, c->d_out, c->s_req);
+        if (rc) break;
+        MR_HIP(hipEventRecord(c->ev1, c->s_req));
+        MR_HIP(hipEventSynchronize(c->ev1));
+        float ms = 0.0f;
+        MR_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        ms /= reps;
+        if (ms < best_t) { best_t = ms; best = k; }
+    }
+    c->bp_variant = best.variant;
+    c->bp_ppt = best.ppt;
+    c->bp_layout = best.layout;
+    c->bp_depth = best.depth;
+    if (rc) return rc;
+    if (cfg) { cfg[0] = best.variant; cfg[1] = best.ppt; cfg[2] = best.layout; cfg[3] = best.depth; }
+    if (best_ms) *best_ms = best_t;
+    return MR_OK;
 }
 
 int mr_bp_tasks_dshift(mr_ctx* c, const mr_task* tasks, int ntasks, int n_views, float* out,
@@ -1165,6 +1363,14 @@ int mr_last_kernel_ms(mr_ctx* c, double* ms) {
     if (rc) return rc;
     MR_CHECK(ms != nullptr, MR_EVALUE, "null output");
     *ms = c->last_kernel_ms;
+    return MR_OK;
+}
+
+int mr_last_tile_kernel_ms(mr_ctx* c, double* ms) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    MR_CHECK(ms != nullptr, MR_EVALUE, "null output");
+    *ms = c->last_tile_ms;
     return MR_OK;
 }
 

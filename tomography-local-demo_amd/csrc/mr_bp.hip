@@ -1766,11 +1766,13 @@ int launch_bp_tiles_v4(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int 
         const int nt = std::min(chunk, ntiles - t0);
         const int64_t nrec = static_cast<int64_t>(nt) * n_views;
         const dim3 rgrid(static_cast<unsigned>((nrec + 255) / 256));
+        const bool last_chunk = t0 + chunk >= ntiles;
         if (v5) {
             hipLaunchKernelGGL(set_int_kernel, dim3((nt + 255) / 256), dim3(256), 0, s, d_fit, nt, 1);
             hipLaunchKernelGGL(records5_kernel, rgrid, dim3(256), 0, s, c->d_geom, c->d_vscale,
                                d_tasks, d_tiles + t0, nt, n_views, c->R, c->C, c->pitch, subs, le,
                                kcap, bufb, c->d_recs, d_fit);
+            if (t0 == 0) MR_HIP(hipEventRecord(c->evk0, s));
             switch (c->storage) {
             case MR_STORE_F32:
                 launch_v5_kernel<float>(c, c->d_recs, d_fit, d_tasks, d_tiles + t0, nt, n_views, scale, d_out, s);
@@ -1783,10 +1785,15 @@ int launch_bp_tiles_v4(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int 
                 break;
             }
             MR_HIP(hipGetLastError());
+            if (last_chunk) {
+                MR_HIP(hipEventRecord(c->evk1, s));
+                c->tile_timed = true;
+            }
             continue;
         }
         hipLaunchKernelGGL(records_kernel, rgrid, dim3(256), 0, s, c->d_geom, c->d_vscale, d_tasks,
                            d_tiles + t0, nt, n_views, c->R, c->C, c->pitch, cone ? 1 : 0, c->d_recs);
+        if (t0 == 0) MR_HIP(hipEventRecord(c->evk0, s));
         switch (c->storage) {
         case MR_STORE_F32:
             cone ? launch_v4_kernel<true, float>(c, c->d_recs, d_tasks, d_tiles + t0, nt, n_views, scale, d_out, s)
@@ -1802,6 +1809,10 @@ int launch_bp_tiles_v4(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int 
             break;
         }
         MR_HIP(hipGetLastError());
+        if (last_chunk) {
+            MR_HIP(hipEventRecord(c->evk1, s));
+            c->tile_timed = true;
+        }
     }
     return MR_OK;
 }

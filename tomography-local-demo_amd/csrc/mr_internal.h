@@ -78,17 +78,31 @@ constexpr int BP_VPASS = 64;     // views per LDS record pass
 constexpr int BP_REC = 16;       // floats per (tile, view) record
 
 // ---------------------------------------------------------------- context
+// Raw-count correction set by mr_set_raw_correction.
+struct RawCorr {
+    int set = 0;
+    int H = 0, W = 0, sample = 1, flip = 0, row0 = 0, col0 = 0, op = 0;
+    float* d_dark = nullptr;         // [R][C]
+    float* d_scale = nullptr;        // [R][C]
+};
+
 struct Ctx {
     int device = 0;
     std::mutex mu;
     hipStream_t s_ingest = nullptr, s_req = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_kernel_ms = 0.0;
-    int bp_variant = 1;              // back-projection inner loop (MINIRECAST_BP_VARIANT)
-    int bp_ppt = 4;                  // v4 points per thread (MINIRECAST_BP_PPT)
+    hipEvent_t evk0 = nullptr, evk1 = nullptr;   // around the tile kernel alone (v4 / v5)
+    bool tile_timed = false;
+    double last_tile_ms = 0.0;
+    int bp_variant = 4;              // back-projection inner loop (MINIRECAST_BP_VARIANT)
+    int bp_ppt = 2;                  // v4/v5 points per thread (MINIRECAST_BP_PPT)
     int xcd_map = 0;                 // XCD-contiguous tile order (MINIRECAST_XCD_MAP=1)
     int bp_layout = 0;               // v4 lane layout: 0 = 2 x 32, 1 = 4 x 16 (MINIRECAST_BP_LAYOUT)
     int bp_depth = 2;                // v4 views in flight, 2 or 3 (MINIRECAST_BP_DEPTH)
+    RawCorr raw;                     // raw-count correction of mr_push_raw
+    uint16_t* d_raw = nullptr;       // staged raw frames
+    size_t raw_bytes = 0;
     float* d_recs = nullptr;         // v4 per-(tile, view) records
     size_t recs_bytes = 0;
 
@@ -176,6 +190,8 @@ int launch_filter_rows_plain(Ctx* c, const float* d_in, int rows, int n, float g
 int launch_bp_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles,
                     int n_views, float scale, float* d_out, hipStream_t s,
                     const TileQueue* q = nullptr);
+// raw frames [k][H][W] uint16 -> corrected [R][k][C] float32 (c->raw)
+int launch_raw_correct(Ctx* c, const uint16_t* d_frames, int k, float* d_out, hipStream_t s);
 // zeros every point of the given tiles (tiles culled by the box test)
 int launch_zero_tiles(const mr_task* d_tasks, const Tile* d_tiles, int ntiles, float* d_out,
                       hipStream_t s);

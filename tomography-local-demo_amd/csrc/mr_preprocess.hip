@@ -116,6 +116,40 @@ void gauss_taps(int n, double sigma, int* pad_out, int* N_out, int* m0_out,
     *m0_out = m0;
 }
 
+// Raw detector counts -> line integrals (flexray.py:111-116 + 174-184, hdf5.py:24-31 +
+// 77-78).  Frame v of `frames` is a [H][W] uint16 detector image.  Output pixel (r, c)
+// of the [R][k][C] block is binned frame pixel (row0 + r, col0 + c), counted after the
+// optional vertical flip (flexray stores detector rows bottom-up):
+//   x = mean of the sample x sample raw counts (an exact integer sum, one rounding)
+//   y = (x - dark) * scale   (op 0, flexray: scale = 1 / max(flat - dark, 1))
+//   y = (x - dark) / scale   (op 1, hdf5:    scale = max(flat - dark, 1e-6))
+//   out = -log(max(y, 1e-3))  (NaN stays NaN, like np.clip)
+template <int FLIP>
+__global__ void raw_correct_kernel(const uint16_t* __restrict__ frames, int H, int W, int k,
+                                   int R, int C, int sample, int row0, int col0,
+                                   const float* __restrict__ dark, const float* __restrict__ scale,
+                                   int op, float* __restrict__ out) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    const int r = blockIdx.y;
+    const int v = blockIdx.z;
+    if (c >= C) return;
+    int br = row0 + r;
+    if (FLIP) br = H / sample - 1 - br;
+    const int bc = col0 + c;
+    const uint16_t* f = frames + static_cast<int64_t>(v) * H * W;
+    uint32_t sum = 0;
+    for (int i = 0; i < sample; ++i) {
+        const uint16_t* row = f + static_cast<int64_t>(br * sample + i) * W + bc * sample;
+        for (int j = 0; j < sample; ++j) sum += row[j];
+    }
+    float x = static_cast<float>(sum);
+    if (sample > 1) x = x / static_cast<float>(sample * sample);
+    const float d = dark[r * C + c], g = scale[r * C + c];
+    float y = op == 0 ? (x - d) * g : (x - d) / g;
+    y = y < 1e-3f ? 1e-3f : y;
+    out[(static_cast<int64_t>(r) * k + v) * C + c] = -logf(y);
+}
+
 }  // namespace
 
 // d_a holds the block; the result ends in *d_res (d_a or d_b).
@@ -157,6 +191,19 @@ int launch_preprocess(Ctx* c, float* d_a, float* d_b, int R, int k, int C, doubl
         }
     }
     *d_res = cur;
+    return MR_OK;
+}
+
+int launch_raw_correct(Ctx* c, const uint16_t* d_frames, int k, float* d_out, hipStream_t s) {
+    const RawCorr& q = c->raw;
+    dim3 grid((c->C + 255) / 256, c->R, k), block(256);
+    if (q.flip)
+        hipLaunchKernelGGL(raw_correct_kernel<1>, grid, block, 0, s, d_frames, q.H, q.W, k, c->R,
+                           c->C, q.sample, q.row0, q.col0, q.d_dark, q.d_scale, q.op, d_out);
+    else
+        hipLaunchKernelGGL(raw_correct_kernel<0>, grid, block, 0, s, d_frames, q.H, q.W, k, c->R,
+                           c->C, q.sample, q.row0, q.col0, q.d_dark, q.d_scale, q.op, d_out);
+    MR_HIP(hipGetLastError());
     return MR_OK;
 }
 

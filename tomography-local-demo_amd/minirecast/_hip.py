@@ -34,6 +34,14 @@ SIGNATURES = [
     ("mr_push_views", _c.c_int, [_P, _P, _c.c_int, _P, _c.c_int, _c.POINTER(_c.c_int)]),
     ("mr_store_views", _c.c_int, [_P, _P, _c.c_int, _P, _c.c_int, _c.POINTER(_c.c_int)]),
     ("mr_set_preprocess", _c.c_int, [_P, _c.c_double, _c.c_double]),
+    ("mr_set_raw_correction", _c.c_int, [_P, _P, _P, _c.c_int, _c.c_int, _c.c_int, _c.c_int,
+                                         _c.c_int, _c.c_int, _c.c_int]),
+    ("mr_push_raw", _c.c_int, [_P, _P, _c.c_int, _P, _c.c_int, _c.POINTER(_c.c_int)]),
+    ("mr_last_tile_kernel_ms", _c.c_int, [_P, _c.POINTER(_c.c_double)]),
+    ("mr_set_bp_config", _c.c_int, [_P, _c.c_int, _c.c_int, _c.c_int, _c.c_int]),
+    ("mr_get_bp_config", _c.c_int, [_P, _P]),
+    ("mr_autotune", _c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_int64, _c.c_int, _c.c_int, _P,
+                               _c.POINTER(_c.c_float)]),
     ("mr_preprocess", _c.c_int,
      [_P, _P, _c.c_int, _c.c_int, _c.c_int, _c.c_double, _c.c_double, _P]),
     ("mr_num_views", _c.c_int, [_P, _c.POINTER(_c.c_int)]),
@@ -216,6 +224,49 @@ class Context:
                                      ctypes.byref(n)))
         return n.value
 
+    def set_raw_correction(self, dark, scale, op, frame_shape, sample=1, flip=False, row0=0,
+                           col0=0):
+        d = np.ascontiguousarray(dark, dtype=np.float32)
+        g = np.ascontiguousarray(scale, dtype=np.float32)
+        check(self.lib.mr_set_raw_correction(self.h, ptr(d), ptr(g), int(op), int(frame_shape[0]),
+                                             int(frame_shape[1]), int(sample), int(bool(flip)),
+                                             int(row0), int(col0)))
+
+    def push_raw(self, indices, frames):
+        """Raw uint16 frames [k, H, W] -> corrected, filtered and stored views."""
+        idx = np.ascontiguousarray(indices, dtype=np.int64).ravel()
+        f = np.ascontiguousarray(frames, dtype=np.uint16)
+        n = ctypes.c_int(0)
+        check(self.lib.mr_push_raw(self.h, ptr(idx), len(idx), ptr(f), 0, ctypes.byref(n)))
+        return n.value
+
+    def push_raw_device(self, indices, dptr):
+        idx = np.ascontiguousarray(indices, dtype=np.int64).ravel()
+        n = ctypes.c_int(0)
+        check(self.lib.mr_push_raw(self.h, ptr(idx), len(idx), ctypes.c_void_p(dptr), 1,
+                                   ctypes.byref(n)))
+        return n.value
+
+    BP_CONFIG_KEYS = ("kernel", "ppt", "layout", "depth")
+
+    def set_bp_config(self, kernel=4, ppt=2, layout=0, depth=2):
+        check(self.lib.mr_set_bp_config(self.h, int(kernel), int(ppt), int(layout), int(depth)))
+
+    def bp_config(self):
+        cfg = np.zeros(4, np.int32)
+        check(self.lib.mr_get_bp_config(self.h, ptr(cfg)))
+        return dict(zip(self.BP_CONFIG_KEYS, (int(x) for x in cfg)))
+
+    def autotune(self, tasks, n_views, out_len, max_tiles=512, reps=2):
+        """Time every back-projection configuration on (a sample of) tasks, keep the
+        fastest; returns (configuration dict, ms per run)."""
+        T = np.ascontiguousarray(tasks, dtype=TASK_DTYPE)
+        cfg = np.zeros(4, np.int32)
+        ms = ctypes.c_float(0.0)
+        check(self.lib.mr_autotune(self.h, ptr(T), len(T), int(n_views), int(out_len),
+                                   int(max_tiles), int(reps), ptr(cfg), ctypes.byref(ms)))
+        return dict(zip(self.BP_CONFIG_KEYS, (int(x) for x in cfg))), float(ms.value)
+
     def store(self, indices, q):
         """Append already-filtered views q [k, rows, cols]."""
         idx = np.ascontiguousarray(indices, dtype=np.int64).ravel()
@@ -337,6 +388,12 @@ class Context:
     def last_kernel_ms(self):
         ms = ctypes.c_double(0.0)
         check(self.lib.mr_last_kernel_ms(self.h, ctypes.byref(ms)))
+        return ms.value
+
+    def last_tile_ms(self):
+        """The tile kernel alone within the last bp_tasks call (HIP events)."""
+        ms = ctypes.c_double(0.0)
+        check(self.lib.mr_last_tile_kernel_ms(self.h, ctypes.byref(ms)))
         return ms.value
 
     # ---------------------------------------------------------------- device memory

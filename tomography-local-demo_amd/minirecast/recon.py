@@ -292,6 +292,7 @@ class SliceReconstructor:
         self._order = np.zeros(self.n_total, dtype=np.int64)
         self._received = np.zeros(self.n_total, dtype=bool)
         self.n = 0
+        self._raw_key = None
         self._ctx = _hip.Context(device)
         self._ctx.set_preprocess(self.gaussian, self.tilt)
         if isinstance(geom, ParallelGeometry):
@@ -332,6 +333,44 @@ class SliceReconstructor:
                 return self.n
             # preprocess.apply (tilt, Gaussian) runs on the device before filtering
             n1 = self._ctx.push(indices, block)
+            with self._lock:
+                n0 = self.n
+                self._order[n0:n1] = indices
+                self._received[indices] = True
+                self.n = n1
+                self.version += 1
+            return self.n
+
+    def add_raw_projections(self, indices, frames, correction):
+        """Add raw uint16 detector frames [k, H, W] (geometry indices ``indices``),
+        corrected on the GPU with ``correction`` (a rawcorr.RawCorrection): the
+        loaders' flat/dark correction (flexray.py:174-184, hdf5.py:77-78) fused into
+        add_projections.  Already received views are ignored."""
+        indices = np.asarray(indices, dtype=np.int64).ravel()
+        frames = np.asarray(frames)
+        if frames.dtype != np.uint16 or frames.ndim != 3:
+            raise ValueError("frames must be uint16 [k, H, W]")
+        if frames.shape[0] != len(indices):
+            raise ValueError("frames has %d projections but %d indices"
+                             % (frames.shape[0], len(indices)))
+        if tuple(frames.shape[1:]) != correction.frame_shape:
+            raise ValueError(f"frames are {frames.shape[1:]}, the correction expects "
+                             f"{correction.frame_shape}")
+        if correction.shape != (self.geom.rows, self.geom.cols):
+            raise ValueError(f"the correction crops to {correction.shape}, the geometry is "
+                             f"{(self.geom.rows, self.geom.cols)}")
+        with self._ingest_lock:
+            keep = ~self._received[indices]
+            indices, frames = indices[keep], frames[keep]
+            if len(indices) == 0:
+                return self.n
+            if self._raw_key is not correction:
+                r0, _, c0, _ = correction.crop
+                self._ctx.set_raw_correction(correction.dark, correction.scale, correction.op,
+                                             correction.frame_shape, correction.sample,
+                                             correction.flip, r0, c0)
+                self._raw_key = correction
+            n1 = self._ctx.push_raw(indices, frames)
             with self._lock:
                 n0 = self.n
                 self._order[n0:n1] = indices
@@ -461,6 +500,26 @@ class SliceReconstructor:
         if self.n == 0:
             return
         self.reconstruct_points(np.zeros((4, 3)))
+
+    def autotune(self, planes=None, n=256, max_tiles=512, reps=2):
+        """Time every back-projection configuration of the library on ``planes``
+        (reconstruct_plane keyword dicts; default: the three orthogonal planes through
+        the centre at n x n) with the views received so far, and keep the fastest for
+        later requests.  Returns (configuration dict, ms per run of the sample)."""
+        if self.n == 0:
+            raise RuntimeError("autotune needs received projections")
+        if planes is None:
+            planes = [dict(origin=(0.0, 0.0, 0.0), normal=nrm, n=n)
+                      for nrm in ((0, 0, 1), (1, 0, 0), (0, 1, 0))]
+        tasks, off = [], 0
+        for p in planes:
+            m = int(p.get("n", n))
+            tasks.append(plane_task(self.box, p.get("origin", (0.0, 0.0, 0.0)), p["normal"], m,
+                                    p.get("size"), p.get("zoom", 1.0), p.get("center"),
+                                    self.cor_shift, off))
+            off += m * m
+        with _KERNEL_LOCK:
+            return self._ctx.autotune(np.array(tasks), self.n, off, max_tiles, reps)
 
     # ----------------------------------------------------------- rotation centre
     def set_cor_shift(self, px):
