@@ -195,6 +195,27 @@ void build_tiles(const mr_task* tasks, int ntasks, std::vector<Tile>& tiles,
     }
 }
 
+// Orders tiles by the height z of their centre.  With the XCD-contiguous block map each
+// XCD then takes a band of heights, i.e. (for a circular orbit about z) a band of
+// detector rows in every view, so a 128-byte detector line is fetched into few of the
+// eight L2 caches instead of all of them.
+void order_tiles_by_height(const mr_task* tasks, std::vector<Tile>& tiles) {
+    std::vector<std::pair<double, int>> key(tiles.size());
+    for (size_t i = 0; i < tiles.size(); ++i) {
+        const Tile& t = tiles[i];
+        const mr_task& T = tasks[t.task];
+        const double p = t.tp * BP_TP + 0.5 * (BP_TP - 1), s = t.ts * BP_TS + 0.5 * (BP_TS - 1);
+        key[i] = {T.o[2] + t.q * T.c[2] + p * T.a[2] + s * T.b[2], static_cast<int>(i)};
+    }
+    std::stable_sort(key.begin(), key.end(),
+                     [](const std::pair<double, int>& a, const std::pair<double, int>& b) {
+                         return a.first < b.first;
+                     });
+    std::vector<Tile> out(tiles.size());
+    for (size_t i = 0; i < tiles.size(); ++i) out[i] = tiles[key[i].second];
+    tiles.swap(out);
+}
+
 enum TaskOp { OP_BP = 0, OP_ADJOINT = 1, OP_DSHIFT = 2 };
 
 // Uploads tasks + tiles and launches the tile kernel of `op` (device buffers), timing it.
@@ -204,6 +225,7 @@ int run_tasks(Ctx* c, const mr_task* tasks, int ntasks, int n_views, float* d_ou
               int op = OP_BP, const float* d_g = nullptr) {
     std::vector<Tile> tiles, empty;
     build_tiles(tasks, ntasks, tiles, op == OP_ADJOINT ? nullptr : &empty);
+    if (c->xcd_map == 2) order_tiles_by_height(tasks, tiles);
     const size_t ne = empty.size();
     tiles.insert(tiles.end(), empty.begin(), empty.end());
     const size_t tb = sizeof(mr_task) * ntasks;
@@ -365,7 +387,10 @@ int mr_ctx_create(int device, mr_ctx** out) {
     }
     if (const char* v = std::getenv("MINIRECAST_BP_LAYOUT")) c->bp_layout = std::atoi(v) ? 1 : 0;
     if (const char* v = std::getenv("MINIRECAST_BP_DEPTH")) c->bp_depth = std::atoi(v) >= 3 ? 3 : 2;
-    if (const char* v = std::getenv("MINIRECAST_XCD_MAP")) c->xcd_map = std::atoi(v) ? 1 : 0;
+    if (const char* v = std::getenv("MINIRECAST_XCD_MAP")) {
+        const int k = std::atoi(v);
+        c->xcd_map = (k == 1 || k == 2) ? k : 0;
+    }
     // default-priority, non-blocking streams
     if (hipStreamCreateWithFlags(&c->s_ingest, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->s_req, hipStreamNonBlocking) != hipSuccess ||

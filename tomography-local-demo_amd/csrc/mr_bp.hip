@@ -1484,19 +1484,19 @@ int dispatch_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntil
             hipLaunchKernelGGL((bp_tiles_kernel2<CONE, float>), grid, block, 0, s,
                                static_cast<const float*>(c->ring), c->view_stride, c->pitch, c->R,
                                c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out,
-                               q.head, q.count, q.n_views, c->xcd_map);
+                               q.head, q.count, q.n_views, c->xcd_map != 0);
             break;
         case MR_STORE_F16:
             hipLaunchKernelGGL((bp_tiles_kernel2<CONE, __half>), grid, block, 0, s,
                                static_cast<const __half*>(c->ring), c->view_stride, c->pitch, c->R,
                                c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out,
-                               q.head, q.count, q.n_views, c->xcd_map);
+                               q.head, q.count, q.n_views, c->xcd_map != 0);
             break;
         default:
             hipLaunchKernelGGL((bp_tiles_kernel2<CONE, uint16_t>), grid, block, 0, s,
                                static_cast<const uint16_t*>(c->ring), c->view_stride, c->pitch,
                                c->R, c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale,
-                               d_out, q.head, q.count, q.n_views, c->xcd_map);
+                               d_out, q.head, q.count, q.n_views, c->xcd_map != 0);
             break;
         }
         MR_HIP(hipGetLastError());
@@ -1702,11 +1702,11 @@ static void launch_v4_one(Ctx* c, const float* recs, const mr_task* d_tasks, con
     if (c->bp_depth >= 3)
         hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, 3>), grid, block, 0, s,
                            static_cast<const ST*>(c->ring), c->view_stride, c->pitch, recs, d_tasks,
-                           d_tiles, n_views, scale, d_out, c->xcd_map);
+                           d_tiles, n_views, scale, d_out, c->xcd_map != 0);
     else
         hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, 2>), grid, block, 0, s,
                            static_cast<const ST*>(c->ring), c->view_stride, c->pitch, recs, d_tasks,
-                           d_tiles, n_views, scale, d_out, c->xcd_map);
+                           d_tiles, n_views, scale, d_out, c->xcd_map != 0);
 }
 
 template <bool CONE, typename ST>
@@ -1731,11 +1731,11 @@ static void launch_v5_kernel(Ctx* c, const float* recs, const int* fit, const mr
     if (c->bp_ppt == 2)
         hipLaunchKernelGGL((bp_tiles_kernel5<ST, 2>), dim3(2 * nt), dim3(BP_THREADS), 0, s,
                            static_cast<const ST*>(c->ring), c->view_stride, c->pitch, recs, d_tasks,
-                           d_tiles, n_views, scale, d_out, c->xcd_map, fit);
+                           d_tiles, n_views, scale, d_out, c->xcd_map != 0, fit);
     else
         hipLaunchKernelGGL((bp_tiles_kernel5<ST, 4>), dim3(nt), dim3(BP_THREADS), 0, s,
                            static_cast<const ST*>(c->ring), c->view_stride, c->pitch, recs, d_tasks,
-                           d_tiles, n_views, scale, d_out, c->xcd_map, fit);
+                           d_tiles, n_views, scale, d_out, c->xcd_map != 0, fit);
 }
 
 // v4 (and v5 for cone beam): per chunk of tiles, the record kernel then the tile kernel.

@@ -97,7 +97,8 @@ struct Ctx {
     double last_tile_ms = 0.0;
     int bp_variant = 4;              // back-projection inner loop (MINIRECAST_BP_VARIANT)
     int bp_ppt = 2;                  // v4/v5 points per thread (MINIRECAST_BP_PPT)
-    int xcd_map = 0;                 // XCD-contiguous tile order (MINIRECAST_XCD_MAP=1)
+    int xcd_map = 2;                 // 1: XCD-contiguous tiles, 2: also sorted by height
+                                     // (MINIRECAST_XCD_MAP)
     int bp_layout = 0;               // v4 lane layout: 0 = 2 x 32, 1 = 4 x 16 (MINIRECAST_BP_LAYOUT)
     int bp_depth = 2;                // v4 views in flight, 2 or 3 (MINIRECAST_BP_DEPTH)
     RawCorr raw;                     // raw-count correction of mr_push_raw
