@@ -130,9 +130,207 @@ def pmc_traffic():
     return res
 
 
+def _line(metric, value, unit, ms, steps, warmup, cfg, extra, higher=True, dtype="f32"):
+    d = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
+         "warmup": warmup, "ms_per_step": ms, "higher_is_better": higher, "scaling": "weak",
+         "vs_baseline": None, "dtype": dtype,
+         "data": "synthetic (seeded ellipsoid phantom, exact line integrals)", "config": cfg}
+    d.update(extra)
+    print(json.dumps(d), flush=True)
+
+
+def bench_stream_cone(args):
+    """Streaming ingest: 600 views in interleaved batches of 10 (Poisson noise, 384 x 486
+    detector); after every batch the three oblique 512^2 slices and the 64^3 preview
+    are brought up to date.  Graph path: one mr_plan whose per-batch update (upload,
+    filter, persistent back-projection of all tasks from the device work queue) is a
+    captured hipGraph replayed per batch; eager path: SliceReconstructor calls."""
+    from minirecast import _hip, phantom, recon, workloads
+    from minirecast.stream import arrival_order
+    w = workloads.stream_cone()
+    g = w.geom
+    A, R, C = g.n_angles, g.rows, g.cols
+    proj = phantom.project_cone(w.phantom, g)
+    proj = workloads.add_poisson_noise(proj, w.extra["photons"], w.seed + 100)
+    box = g.volume_box()
+    dims, origin, h = recon.volume_grid(box, w.extra["preview"])
+    tasks, off, n_in = [], 0, 0
+    for p in w.planes:
+        tasks.append(recon.plane_task(box, p['origin'], p['normal'], p['n'], out_offset=off))
+        _, m = recon.slice_points(box, p['origin'], p['normal'], p['n'])
+        n_in += int(m.sum())
+        off += p['n'] ** 2
+    vt = recon.volume_task(origin, h, dims, 0, int(dims[2]))
+    vt['out_offset'] = off
+    tasks.append(vt)
+    nvox = int(np.prod(dims))
+    out_len = off + nvox
+    order = arrival_order(A, w.extra["order"])
+    batch = w.extra["batch"]
+    blocks = [(order[i:i + batch], np.ascontiguousarray(proj[:, order[i:i + batch], :]))
+              for i in range(0, A, batch)]
+    # graph path
+    ctx = _hip.Context(0)
+    ctx.set_cone(g.vectors, R, C, w.window)
+    plan = _hip.Plan(ctx, np.array(tasks, dtype=_hip.TASK_DTYPE), out_len)
+    gpu_ms, wall_ms = [], []
+    for idx, blk in blocks:
+        t0 = time.perf_counter()
+        plan.stream(idx, blk, copy=False)
+        wall_ms.append((time.perf_counter() - t0) * 1e3)
+        gpu_ms.append(plan.last_ms())
+    out_graph = plan.run(-1, copy=True)
+    # eager path
+    rec = recon.SliceReconstructor(None, g, window=w.window, verbose=False)
+    eager_ms = []
+    for idx, blk in blocks:
+        t0 = time.perf_counter()
+        rec.add_projections(idx, blk)
+        imgs = rec.reconstruct_planes(w.planes)
+        vol, _, _ = rec.preview_volume(w.extra["preview"])
+        eager_ms.append((time.perf_counter() - t0) * 1e3)
+    ref = np.concatenate([im.ravel() for im in imgs] + [vol.ravel()])
+    err = float(np.linalg.norm(out_graph - ref) / np.linalg.norm(ref))
+    updates = sum(min((b + 1) * batch, A) * (n_in + nvox) for b in range(len(blocks)))
+    total = sum(wall_ms) * 1e-3
+    res = dict(graph_ms_per_batch_median=float(np.median(wall_ms)),
+               graph_gpu_ms_per_batch_median=float(np.median(gpu_ms)),
+               graph_ms_last_batch=float(wall_ms[-1]), eager_ms_per_batch_median=float(np.median(eager_ms)),
+               eager_ms_last_batch=float(eager_ms[-1]), graph_vs_eager_rel_l2=err,
+               batches=len(blocks), first_batch_ms=float(wall_ms[0]))
+    log(f"stream_cone: {res}")
+    _line("ms per streamed batch (10 views ingested + 3 oblique 512^2 slices + 64^3 preview)",
+          round(updates / total / 1e9, 3), "GUPS", round(float(np.median(wall_ms)), 4),
+          len(blocks), 0,
+          {"workload": "stream_cone: 600 views in 60 interleaved batches of 10, 384x486 "
+                       "detector, 3 oblique 512^2 slices + 64^3 preview per batch",
+           "path": "mr_plan hipGraph per batch (upload + filter + persistent BP)"},
+          {"stream": res})
+    plan.close()
+
+
+def bench_parallel_cor(args):
+    """Rotation-centre search, parallel beam: 1800 views, 64 x 2048 detector, 25 shifts
+    x axial slices 1024^2 (z = 0), 768^2 (z = +16), 512^2 (z = -16); every trial image of
+    a slice size back-projected in one launch and scored (entropy) on the device."""
+    from minirecast import _hip, phantom, recon, workloads
+    w = workloads.parallel_cor()
+    g = w.geom
+    proj = phantom.project_parallel(w.phantom, g)
+    rec = recon.SliceReconstructor(proj, g, window=w.window, verbose=False)
+    del proj
+    box = g.volume_box()
+    shifts = w.extra["shifts"]
+    groups, upd = [], 0
+    for p in w.planes:
+        T = np.array([recon.plane_task(box, p['origin'], p['normal'], p['n'], cor_shift=float(sh))
+                      for sh in shifts], dtype=_hip.TASK_DTYPE)
+        _, m = recon.slice_points(box, p['origin'], p['normal'], p['n'])
+        upd += int(m.sum()) * len(shifts) * g.n_angles
+        groups.append(T)
+    ctx = rec._ctx
+    for T in groups:
+        ctx.cor_scores(T, rec.n)                         # warm-up
+    times, best = [], []
+    for _ in range(max(1, args.steps)):
+        t0 = time.perf_counter()
+        sc = [ctx.cor_scores(T, rec.n) for T in groups]
+        times.append((time.perf_counter() - t0) * 1e3)
+    best = [float(shifts[int(np.argmax(x))]) for x in sc]
+    ms = float(np.median(times))
+    res = dict(ms_per_search=ms, best_shift_per_slice=best, trial_images=25 * 3,
+               updates=upd)
+    log(f"parallel_cor: {res}")
+    _line("ms per COR search (25 shifts x 3 axial slices, 1800 views) + GUPS",
+          round(upd / (ms * 1e-3) / 1e9, 2), "GUPS", round(ms, 3), len(times), 1,
+          {"workload": "parallel_cor: 1800 views / 180 deg, 64x2048 detector, hann, "
+                       "25 shifts x axial 1024^2, 768^2, 512^2"}, {"cor": res})
+
+
+def bench_diff_cone(args):
+    """Differentiable back-projection (256^3-box phantom, 360 views, 256 x 256 detector,
+    3 oblique 256^2 slices): forward + backward through the hand-written adjoint,
+    gradient-based COR refinement (true shift 0.75 px) and 20 SIRT-style steps
+    (gradient steps on the projections toward target slice values)."""
+    import torch
+    from minirecast import autograd, phantom, workloads
+    w = workloads.diff_cone()
+    g = w.geom
+    op = autograd.FDKSlices(g, w.planes, w.window)
+    p_true = torch.tensor(phantom.project_cone(w.phantom, g), device="cuda")
+    wts = torch.tensor(np.random.default_rng(3).normal(size=op.out_len), dtype=torch.float32,
+                       device="cuda")
+    # forward + backward timing
+    def fb():
+        proj = p_true.clone().requires_grad_(True)
+        s = torch.tensor(0.25, device="cuda", requires_grad=True)
+        y = autograd.fdk_slices(proj, s, op)
+        (y * wts).sum().backward()
+        torch.cuda.synchronize()
+    for _ in range(2):
+        fb()
+    t = []
+    for _ in range(max(1, args.steps)):
+        t0 = time.perf_counter()
+        fb()
+        t.append((time.perf_counter() - t0) * 1e3)
+    fb_ms = float(np.median(t))
+    # COR refinement: match slices reconstructed at the true shift
+    with torch.no_grad():
+        y_star = autograd.fdk_slices(p_true, torch.tensor(0.75, device="cuda"), op)
+    s = torch.tensor(0.0, device="cuda", requires_grad=True)
+    hist = []
+    t0 = time.perf_counter()
+    for _ in range(15):
+        y = autograd.fdk_slices(p_true, s, op)
+        loss = 0.5 * ((y - y_star) ** 2).sum()
+        g_s, = torch.autograd.grad(loss, s)
+        # Gauss-Newton on the shift: dy/ds from the library
+        ds = op.dshift(p_true, float(s))
+        step = float(g_s) / max(float((ds * ds).sum()), 1e-30)
+        with torch.no_grad():
+            s -= step
+        hist.append(float(s))
+    cor_ms = (time.perf_counter() - t0) * 1e3
+    # 20 SIRT-style steps on the projections
+    p = torch.zeros_like(p_true).requires_grad_(True)
+    with torch.no_grad():
+        y_t = autograd.fdk_slices(p_true, torch.tensor(0.0, device="cuda"), op)
+    losses = []
+    lr = None
+    t0 = time.perf_counter()
+    for k in range(20):
+        y = autograd.fdk_slices(p, torch.tensor(0.0, device="cuda"), op)
+        loss = 0.5 * ((y - y_t) ** 2).sum()
+        gp, = torch.autograd.grad(loss, p)
+        if lr is None:
+            # exact line search on the quadratic for the first direction
+            with torch.no_grad():
+                Ag = autograd.fdk_slices(gp, torch.tensor(0.0, device="cuda"), op)
+            lr = float((gp * gp).sum()) / max(float((Ag * Ag).sum()), 1e-30)
+        with torch.no_grad():
+            p -= lr * gp
+        losses.append(float(loss))
+    torch.cuda.synchronize()
+    sirt_ms = (time.perf_counter() - t0) * 1e3 / 20
+    res = dict(forward_backward_ms=fb_ms, cor_refinement=dict(true=0.75, final=hist[-1],
+                                                                 iterations=len(hist),
+                                                                 ms_total=cor_ms),
+               sirt=dict(steps=20, ms_per_step=sirt_ms, loss_first=losses[0],
+                         loss_last=losses[-1]))
+    log(f"diff_cone: {res}")
+    n_upd = op.out_len * g.n_angles
+    _line("ms per forward+backward (slices -> grad wrt projections and COR shift)",
+          round(fb_ms, 4), "ms", round(fb_ms, 4), len(t), 2,
+          {"workload": "diff_cone: 360 views, 256x256 detector, 3 oblique 256^2 slices",
+           "updates_per_forward": n_upd}, {"diff": res}, higher=False)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="full_cone",
+                    choices=["full_cone", "stream_cone", "parallel_cor", "diff_cone"])
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-check", action="store_true", help="skip the float64 oracle error")
@@ -151,6 +349,10 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.workload != "full_cone":
+        {"stream_cone": bench_stream_cone, "parallel_cor": bench_parallel_cor,
+         "diff_cone": bench_diff_cone}[args.workload](args)
+        return
     from minirecast import _hip, phantom, recon, workloads
     if args.calibrate_copy:
         import torch
@@ -276,6 +478,14 @@ def main():
             wall = (time.perf_counter() - t1) * 1e3 / args.steps
             results["f32_separate"] = dict(ms=wall, kernel_ms=float(np.mean(kms)),
                                            gups=n_in * A / (wall * 1e-3) / 1e9)
+        if s == "f16":
+            # PCIe-inclusive: the same update with the slices copied back to host memory
+            # (what reconstruct_planes returns through the Python API)
+            c.bp_tasks(tasks, A, out_len)
+            t1 = time.perf_counter()
+            for _ in range(args.steps):
+                c.bp_tasks(tasks, A, out_len)
+            res["host_output_ms"] = (time.perf_counter() - t1) * 1e3 / args.steps
         c.free(dout)
         log(f"{s}: {results[s]}")
     log(f"f32 three launches: {results.get('f32_separate')}")

@@ -92,7 +92,7 @@ def test_plan_stream_graph_replays(workload, storage, monkeypatch):
 @pytest.mark.parametrize("variant,ppt,layout,depth", [
     ("2", "4", "0", "2"), ("4", "4", "0", "2"), ("4", "2", "0", "2"), ("4", "2", "1", "3"),
     ("4", "1", "1", "2"), ("4", "1", "0", "3"), ("4", "4", "1", "3"), ("5", "4", "0", "2"),
-    ("5", "2", "0", "2")])
+    ("5", "2", "0", "2"), ("6", "2", "0", "2"), ("6", "1", "1", "3"), ("6", "4", "0", "2")])
 def test_bp_variants_full_check(workload, variant, ppt, layout, depth, monkeypatch):
     """Every tile-kernel configuration (kernel, points per thread, lane layout, views in
     flight) against the plain v1 kernel on all outputs, for view counts that leave
@@ -152,3 +152,26 @@ def test_autotune_picks_a_configuration_and_keeps_parity(workload):
         rec._ctx.set_bp_config(kernel=7)
     with pytest.raises(ValueError):
         rec._ctx.set_bp_config(kernel=4, ppt=3)
+
+
+def test_pair_ring_follows_appends_and_resets(workload, monkeypatch):
+    """Kernel 6 reads a row-pair copy of the ring built on first use: it must take in views
+    appended after that use, and rebuild after the views are reset and replaced."""
+    from minirecast import _hip
+    w, proj, planes, tasks, out_len = workload
+    T = np.array(tasks, dtype=_hip.TASK_DTYPE)
+    A = w.geom.n_angles
+    ref_ctx = _ctx(w, "f16", "1", monkeypatch)
+    ctx = _ctx(w, "f16", "6", monkeypatch)
+    h = A // 2
+    for c in (ref_ctx, ctx):
+        c.push(np.arange(h), proj[:, :h])
+    assert rel_l2(ctx.bp_tasks(T, h, out_len), ref_ctx.bp_tasks(T, h, out_len)) < 1e-6
+    for c in (ref_ctx, ctx):
+        c.push(np.arange(h, A), proj[:, h:])
+    assert rel_l2(ctx.bp_tasks(T, A, out_len), ref_ctx.bp_tasks(T, A, out_len)) < 1e-6
+    other = proj[:, ::-1].copy()
+    for c in (ref_ctx, ctx):
+        c.reset_views()
+        c.push(np.arange(A), other)
+    assert rel_l2(ctx.bp_tasks(T, A, out_len), ref_ctx.bp_tasks(T, A, out_len)) < 1e-6

@@ -84,6 +84,10 @@ void free_geometry(Ctx* c) {
     if (c->d_H) hipFree(c->d_H);
     if (c->d_tw) hipFree(c->d_tw);
     c->ring = nullptr; c->d_geom = nullptr; c->d_vscale = nullptr; c->d_H = nullptr; c->d_tw = nullptr;
+    if (c->pring) hipFree(c->pring);
+    c->pring = nullptr;
+    c->pring_bytes = 0;
+    c->pring_n = 0;
     // the raw-count correction is per detector frame: a new geometry drops it
     if (c->raw.d_dark) hipFree(c->raw.d_dark);
     if (c->raw.d_scale) hipFree(c->raw.d_scale);
@@ -379,7 +383,7 @@ int mr_ctx_create(int device, mr_ctx** out) {
     c->device = device;
     if (const char* v = std::getenv("MINIRECAST_BP_VARIANT")) {
         const int k = std::atoi(v);
-        c->bp_variant = (k >= 1 && k <= 5) ? k : c->bp_variant;
+        c->bp_variant = (k >= 1 && k <= 6) ? k : c->bp_variant;
     }
     if (const char* v = std::getenv("MINIRECAST_BP_PPT")) {
         const int k = std::atoi(v);
@@ -792,10 +796,10 @@ struct BpConfig { int variant, ppt, layout, depth; };
 const BpConfig kBpCandidates[] = {
     {4, 2, 0, 2}, {4, 2, 0, 3}, {4, 2, 1, 2}, {4, 2, 1, 3}, {4, 1, 0, 2}, {4, 1, 0, 3},
     {4, 1, 1, 2}, {4, 1, 1, 3}, {4, 4, 0, 2}, {4, 4, 1, 2}, {5, 2, 0, 2}, {5, 4, 0, 2},
-    {2, 4, 0, 2}};
+    {2, 4, 0, 2}, {6, 2, 0, 2}, {6, 2, 0, 3}, {6, 1, 0, 3}, {6, 4, 0, 2}};
 
 bool valid_config(int variant, int ppt, int layout, int depth) {
-    return (variant >= 1 && variant <= 5) && (ppt == 1 || ppt == 2 || ppt == 4) &&
+    return (variant >= 1 && variant <= 6) && (ppt == 1 || ppt == 2 || ppt == 4) &&
            (layout == 0 || layout == 1) && (depth == 2 || depth == 3);
 }
 }  // namespace
@@ -1024,7 +1028,7 @@ int mr_plan_create(mr_ctx* c, const mr_task* tasks, int ntasks, int64_t out_len,
     MR_CHECK(c->kind != MR_KIND_NONE, MR_ERUNTIME, "no geometry set");
     MR_CHECK(ntasks >= 1 && tasks, MR_EVALUE, "a plan needs at least one task");
     MR_CHECK(out_len >= 1, MR_EVALUE, "a plan needs an output");
-    MR_CHECK(c->bp_variant == 2 || c->bp_variant == 4 || c->bp_variant == 5, MR_ERUNTIME,
+    MR_CHECK(c->bp_variant == 2 || c->bp_variant >= 4, MR_ERUNTIME,
              "plans run the persistent v2 back-projection kernel");
     rc = validate_tasks(tasks, ntasks, out_len);
     if (rc) return rc;
@@ -1219,6 +1223,7 @@ int mr_reset_views(mr_ctx* c) {
     if (rc) return rc;
     std::lock_guard<std::mutex> lk(c->mu);
     c->n = 0;
+    c->pring_n = 0;
     return MR_OK;
 }
 

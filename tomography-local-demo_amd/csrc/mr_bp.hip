@@ -16,6 +16,8 @@
 // near the tile centre*, so the FP32 inner loop works with small relative
 // coordinates (|col - ref| < ~100 px, ulp < 1e-5 px) instead of absolute columns
 // near 1000 px (ulp 6e-5 px), which would dominate the error budget.
+#include <type_traits>
+
 #include "mr_internal.h"
 
 namespace mr {
@@ -759,11 +761,74 @@ template <> struct Pair<uint16_t> {
     }
 };
 
-template <typename ST, int PPT>
+// Taps from the row-pair ring: one element of pair-row r holds (q(r, c), q(r+1, c)) side
+// by side, so the four taps of a point are contiguous — 8 bytes (16-bit forms, one
+// buffer_load_dwordx2) or 16 bytes (f32, one buffer_load_dwordx4) — and a gather costs
+// one instruction instead of two.
+template <typename ST> struct PairP;
+template <> struct PairP<float> {
+    float a, b, c, d;
+    __device__ inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int) {
+        const uint4 x = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, 0, 0));
+        a = __uint_as_float(x.x);      // q(r, c)
+        c = __uint_as_float(x.y);      // q(r+1, c)
+        b = __uint_as_float(x.z);      // q(r, c+1)
+        d = __uint_as_float(x.w);      // q(r+1, c+1)
+    }
+    __device__ inline void lerp(float fc, float& top, float& bot) const {
+        top = fmaf(fc, b - a, a);
+        bot = fmaf(fc, d - c, c);
+    }
+};
+template <> struct PairP<__half> {
+    uint32_t w0, w1;                   // (q(r, c) | q(r+1, c) << 16), same for c+1
+    __device__ inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int) {
+        const uint2 x = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, 0, 0));
+        w0 = x.x;
+        w1 = x.y;
+    }
+    __device__ inline void lerp(float fc, float& top, float& bot) const {
+        top = fmaf(fc, half_lo(w1), fmaf(-fc, half_lo(w0), half_lo(w0)));
+        bot = fmaf(fc, half_hi(w1), fmaf(-fc, half_hi(w0), half_hi(w0)));
+    }
+};
+template <> struct PairP<uint16_t> {
+    uint32_t w0, w1;
+    __device__ inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int) {
+        const uint2 x = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, 0, 0));
+        w0 = x.x;
+        w1 = x.y;
+    }
+    __device__ inline void lerp(float fc, float& top, float& bot) const {
+        const float a = __uint_as_float(w0 << 16), b = __uint_as_float(w1 << 16);
+        const float c = __uint_as_float(w0 & 0xffff0000u), d = __uint_as_float(w1 & 0xffff0000u);
+        top = fmaf(fc, b - a, a);
+        bot = fmaf(fc, d - c, c);
+    }
+};
+
+template <typename ST, int PPT, bool PAIRED = false>
 struct PointState {
     float fc[PPT], fr[PPT], wt[PPT];
-    Pair<ST> tap[PPT];
+    typename std::conditional<PAIRED, PairP<ST>, Pair<ST>>::type tap[PPT];
 };
+
+// Row-pair ring from the ring: pair-row p (padded rows p and p+1, p = 0..R+1) of view v,
+// element (col) = (ring[v][p][col], ring[v][p+1][col]).
+template <typename ST>
+__global__ __launch_bounds__(256) void pair_convert_kernel(const ST* __restrict__ ring,
+                                                          ST* __restrict__ pring,
+                                                          int64_t view_stride, int64_t pview_stride,
+                                                          int pitch, int v0) {
+    const int col = blockIdx.x * 256 + threadIdx.x;
+    const int pr = blockIdx.y;
+    const int64_t v = v0 + static_cast<int64_t>(blockIdx.z);
+    if (col >= pitch) return;
+    const ST* src = ring + v * view_stride + static_cast<int64_t>(pr) * pitch + col;
+    ST* dst = pring + v * pview_stride + (static_cast<int64_t>(pr) * pitch + col) * 2;
+    dst[0] = src[0];
+    dst[1] = src[pitch];
+}
 
 // Lane layouts of v4: which in-tile point (dp along a, ds along b) thread t handles
 // as its m-th point, in a workgroup that covers rows [sub * 8 * PPT, (sub + 1) * 8 * PPT)
@@ -789,7 +854,7 @@ template <int PPT, int LAY> struct Map4 {
     static constexpr int ds(int m) { return LAY == 0 ? 0 : (PPT == 4 ? 16 * (m & 1) : 16 * m); }
 };
 
-template <bool CONE, typename ST, int PPT, int LAY, int DEPTH>
+template <bool CONE, typename ST, int PPT, int LAY, int DEPTH, bool PAIRED>
 __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel4(
     const ST* __restrict__ ring, int64_t view_stride, int pitch,
     const float* __restrict__ recs, const mr_task* __restrict__ tasks,
@@ -815,7 +880,9 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel4(
         inside[m] = valid[m] && (!T.mask_box || in_box(T, p, s));
     }
     const float ds = static_cast<float>(st0), dp0 = static_cast<float>(pt);
-    const int pitchb = pitch << LE;
+    // PAIRED: `ring` is the row-pair ring (two elements per padded position)
+    constexpr int SH = LE + (PAIRED ? 1 : 0);
+    const int pitchb = pitch << SH;
     const int view_bytes = static_cast<int>(view_stride << LE);
     const float* __restrict__ rt = recs + static_cast<int64_t>(tile) * n_views * BP_REC;
 
@@ -844,10 +911,10 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel4(
         return c;
     };
     // request the taps of view v into st
-    auto issue = [&](int v, PointState<ST, PPT>& st) {
+    auto issue = [&](int v, PointState<ST, PPT, PAIRED>& st) {
         const float* r = rt + static_cast<int64_t>(v) * BP_REC;
         const float U0 = r[0], Ua = r[1], Ub = r[2];
-        const int baseb = __float_as_int(r[14]) << LE;
+        const int baseb = __float_as_int(r[14]) << SH;
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<ST*>(ring + static_cast<int64_t>(v) * view_stride), 0, view_bytes, 0x00020000);
         const float uh0 = fmaf(dp0, Ub, fmaf(ds, Ua, U0));
@@ -866,7 +933,7 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel4(
                 st.wt[m] = rw * rw;
                 st.fc[m] = __builtin_amdgcn_fractf(col);
                 st.fr[m] = __builtin_amdgcn_fractf(row);
-                const int voff = __mul24(cvt_flr(row), pitchb) + ((cvt_flr(col) << LE) + baseb);
+                const int voff = __mul24(cvt_flr(row), pitchb) + ((cvt_flr(col) << SH) + baseb);
                 st.tap[m].load(rs, voff, pitchb);
             }
         } else {
@@ -875,12 +942,12 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel4(
             for (int m = 0; m < PPT; ++m) {
                 const float col = __builtin_amdgcn_fmed3f(step(uh0, m, Ub, Ua), r[9], r[10]);
                 st.fc[m] = __builtin_amdgcn_fractf(col);
-                const int voff = roffp[m] + ((cvt_flr(col) << LE) + baseb);
+                const int voff = roffp[m] + ((cvt_flr(col) << SH) + baseb);
                 st.tap[m].load(rs, voff, pitchb);
             }
         }
     };
-    auto consume = [&](const PointState<ST, PPT>& st) {
+    auto consume = [&](const PointState<ST, PPT, PAIRED>& st) {
 #pragma unroll
         for (int m = 0; m < PPT; ++m) {
             float top, bot;
@@ -898,7 +965,7 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel4(
     // is interpolated.  Requests past the last view re-read it (never consumed), so the
     // loop body is unconditional and every wait counts exactly the loads after it.
     if (n_views > 0) {
-        PointState<ST, PPT> S[DEPTH];
+        PointState<ST, PPT, PAIRED> S[DEPTH];
         const int last = n_views - 1;
 #pragma unroll
         for (int j = 0; j + 1 < DEPTH; ++j) issue(min(j, last), S[j]);
@@ -1557,9 +1624,9 @@ int launch_bp_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int nti
                     float scale, float* d_out, hipStream_t s, const TileQueue* q) {
     const TileQueue none{};
     if (!q && ntiles <= 0) return MR_OK;
-    if (!q && (c->bp_variant == 4 || c->bp_variant == 5) && n_views > 0)
+    if (!q && c->bp_variant >= 4 && n_views > 0)
         return launch_bp_tiles_v4(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s);
-    if (q && c->bp_variant != 2 && c->bp_variant != 4 && c->bp_variant != 5)
+    if (q && c->bp_variant != 2 && c->bp_variant < 4)
         return fail(MR_ERUNTIME, "persistent launches need the v2 kernel");
     if (q) {                               // persistent launches always run the v2 loop
         const int saved = c->bp_variant;
@@ -1699,14 +1766,24 @@ template <bool CONE, typename ST, int PPT, int LAY>
 static void launch_v4_one(Ctx* c, const float* recs, const mr_task* d_tasks, const Tile* d_tiles,
                           int nt, int n_views, float scale, float* d_out, hipStream_t s) {
     const dim3 grid(Map4<PPT, LAY>::SUBS * nt), block(BP_THREADS);
-    if (c->bp_depth >= 3)
-        hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, 3>), grid, block, 0, s,
-                           static_cast<const ST*>(c->ring), c->view_stride, c->pitch, recs, d_tasks,
-                           d_tiles, n_views, scale, d_out, c->xcd_map != 0);
-    else
-        hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, 2>), grid, block, 0, s,
-                           static_cast<const ST*>(c->ring), c->view_stride, c->pitch, recs, d_tasks,
-                           d_tiles, n_views, scale, d_out, c->xcd_map != 0);
+    const bool paired = c->bp_variant == 6;
+    const ST* ring = static_cast<const ST*>(paired ? c->pring : c->ring);
+    const int64_t vs = paired ? 2 * static_cast<int64_t>(c->R + 2) * c->pitch : c->view_stride;
+    const int xcd = c->xcd_map != 0;
+    if (paired) {
+        if (c->bp_depth >= 3)
+            hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, 3, true>), grid, block, 0, s,
+                               ring, vs, c->pitch, recs, d_tasks, d_tiles, n_views, scale, d_out, xcd);
+        else
+            hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, 2, true>), grid, block, 0, s,
+                               ring, vs, c->pitch, recs, d_tasks, d_tiles, n_views, scale, d_out, xcd);
+    } else if (c->bp_depth >= 3) {
+        hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, 3, false>), grid, block, 0, s,
+                           ring, vs, c->pitch, recs, d_tasks, d_tiles, n_views, scale, d_out, xcd);
+    } else {
+        hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, 2, false>), grid, block, 0, s,
+                           ring, vs, c->pitch, recs, d_tasks, d_tiles, n_views, scale, d_out, xcd);
+    }
 }
 
 template <bool CONE, typename ST>
@@ -1738,6 +1815,35 @@ static void launch_v5_kernel(Ctx* c, const float* recs, const int* fit, const mr
                            d_tiles, n_views, scale, d_out, c->xcd_map != 0, fit);
 }
 
+// Kernel 6 reads a row-pair copy of the ring (twice its size), built on first use and
+// extended to the views stored since (slots below pring_n never change until the views
+// are reset).
+int ensure_pair_ring(Ctx* c, int n_views, hipStream_t s) {
+    const size_t esz = c->storage == MR_STORE_F32 ? 4 : 2;
+    const int64_t pvs = 2 * static_cast<int64_t>(c->R + 2) * c->pitch;
+    if (!c->pring) {
+        c->pring_bytes = esz * static_cast<size_t>(pvs) * c->n_total;
+        MR_HIP(hipMalloc(&c->pring, c->pring_bytes));
+        c->pring_n = 0;
+    }
+    const int v1 = n_views;
+    for (int v0 = c->pring_n; v0 < v1; v0 += 4096) {
+        const int nv = std::min(4096, v1 - v0);
+        const dim3 grid((c->pitch + 255) / 256, c->R + 2, nv);
+        if (esz == 4)
+            hipLaunchKernelGGL(pair_convert_kernel<float>, grid, dim3(256), 0, s,
+                               static_cast<const float*>(c->ring), static_cast<float*>(c->pring),
+                               c->view_stride, pvs, c->pitch, v0);
+        else
+            hipLaunchKernelGGL(pair_convert_kernel<uint16_t>, grid, dim3(256), 0, s,
+                               static_cast<const uint16_t*>(c->ring), static_cast<uint16_t*>(c->pring),
+                               c->view_stride, pvs, c->pitch, v0);
+        MR_HIP(hipGetLastError());
+    }
+    c->pring_n = std::max(c->pring_n, v1);
+    return MR_OK;
+}
+
 // v4 (and v5 for cone beam): per chunk of tiles, the record kernel then the tile kernel.
 int launch_bp_tiles_v4(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles, int n_views,
                        float scale, float* d_out, hipStream_t s) {
@@ -1747,6 +1853,10 @@ int launch_bp_tiles_v4(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int 
     }
     const bool cone = c->kind == MR_KIND_CONE;
     const bool v5 = cone && c->bp_variant == 5;
+    if (c->bp_variant == 6) {
+        int rc = ensure_pair_ring(c, n_views, s);
+        if (rc) return rc;
+    }
     const int rec_floats = v5 ? V5_REC : BP_REC;
     const size_t per_tile = sizeof(float) * rec_floats * static_cast<size_t>(n_views);
     const size_t budget = static_cast<size_t>(512) << 20;          // record buffer <= 512 MB

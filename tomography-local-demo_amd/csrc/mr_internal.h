@@ -102,6 +102,9 @@ struct Ctx {
     int bp_layout = 0;               // v4 lane layout: 0 = 2 x 32, 1 = 4 x 16 (MINIRECAST_BP_LAYOUT)
     int bp_depth = 2;                // v4 views in flight, 2 or 3 (MINIRECAST_BP_DEPTH)
     RawCorr raw;                     // raw-count correction of mr_push_raw
+    void* pring = nullptr;           // row-pair copy of the ring (kernel 6)
+    size_t pring_bytes = 0;
+    int pring_n = 0;                 // views converted into pring
     uint16_t* d_raw = nullptr;       // staged raw frames
     size_t raw_bytes = 0;
     float* d_recs = nullptr;         // v4 per-(tile, view) records
