@@ -286,38 +286,44 @@ def bench_diff_cone(args):
         loss = 0.5 * ((y - y_star) ** 2).sum()
         g_s, = torch.autograd.grad(loss, s)
         # Gauss-Newton on the shift: dy/ds from the library
-        ds = op.dshift(p_true, float(s))
-        step = float(g_s) / max(float((ds * ds).sum()), 1e-30)
+        ds = op.dshift(p_true, float(s.detach()))
+        step = float(g_s.detach()) / max(float((ds * ds).sum()), 1e-30)
         with torch.no_grad():
             s -= step
-        hist.append(float(s))
+        hist.append(float(s.detach()))
     cor_ms = (time.perf_counter() - t0) * 1e3
-    # 20 SIRT-style steps on the projections
-    p = torch.zeros_like(p_true).requires_grad_(True)
+    # 20 SIRT-style (Landweber) steps on the projections toward target slice values:
+    # p <- p - g / L, g the autograd gradient of 0.5 |A p - y|^2 (A: projections ->
+    # slices, linear), L the largest eigenvalue of A^T A from 8 power iterations
+    zero = torch.tensor(0.0, device="cuda")
     with torch.no_grad():
-        y_t = autograd.fdk_slices(p_true, torch.tensor(0.0, device="cuda"), op)
+        y_t = autograd.fdk_slices(p_true, zero, op)
+    v = torch.randn_like(p_true)
+    v /= v.norm()
+    for _ in range(8):
+        vv = v.clone().requires_grad_(True)
+        av = autograd.fdk_slices(vv, zero, op)
+        atav, = torch.autograd.grad(0.5 * (av * av).sum(), vv)
+        L = float(atav.norm())
+        v = atav / L
+    p = torch.zeros_like(p_true).requires_grad_(True)
     losses = []
-    lr = None
     t0 = time.perf_counter()
     for k in range(20):
-        y = autograd.fdk_slices(p, torch.tensor(0.0, device="cuda"), op)
+        y = autograd.fdk_slices(p, zero, op)
         loss = 0.5 * ((y - y_t) ** 2).sum()
         gp, = torch.autograd.grad(loss, p)
-        if lr is None:
-            # exact line search on the quadratic for the first direction
-            with torch.no_grad():
-                Ag = autograd.fdk_slices(gp, torch.tensor(0.0, device="cuda"), op)
-            lr = float((gp * gp).sum()) / max(float((Ag * Ag).sum()), 1e-30)
         with torch.no_grad():
-            p -= lr * gp
+            p -= gp / L
         losses.append(float(loss))
     torch.cuda.synchronize()
     sirt_ms = (time.perf_counter() - t0) * 1e3 / 20
     res = dict(forward_backward_ms=fb_ms, cor_refinement=dict(true=0.75, final=hist[-1],
                                                                  iterations=len(hist),
                                                                  ms_total=cor_ms),
-               sirt=dict(steps=20, ms_per_step=sirt_ms, loss_first=losses[0],
-                         loss_last=losses[-1]))
+               sirt=dict(steps=20, ms_per_step=sirt_ms, lipschitz=L, loss_first=losses[0],
+                         loss_last=losses[-1], monotone=bool(all(b <= a for a, b in
+                                                                  zip(losses, losses[1:])))))
     log(f"diff_cone: {res}")
     n_upd = op.out_len * g.n_angles
     _line("ms per forward+backward (slices -> grad wrt projections and COR shift)",

@@ -122,6 +122,15 @@ int alloc_ring(Ctx* c, int n_total, int rows, int cols, int window, int storage)
     c->window = window;
     c->storage = storage;
     c->N = fft_length(cols);
+    if (!c->bp_explicit) {
+        // measured defaults: 16-bit forms read the row-pair ring (kernel 6, 4 points per
+        // thread); f32 reads the ring directly (kernel 4, 2 points per thread)
+        const bool f32 = storage == MR_STORE_F32;
+        c->bp_variant = f32 ? 4 : 6;
+        c->bp_ppt = f32 ? 2 : 4;
+        c->bp_layout = 0;
+        c->bp_depth = 2;
+    }
     c->pitch = (cols + 3 + 63) / 64 * 64;
     c->Rp = rows + 3;
     c->view_stride = static_cast<int64_t>(c->Rp) * c->pitch;
@@ -381,6 +390,8 @@ int mr_ctx_create(int device, mr_ctx** out) {
     MR_HIP(hipSetDevice(device));
     mr_ctx* c = new mr_ctx();
     c->device = device;
+    c->bp_explicit = std::getenv("MINIRECAST_BP_VARIANT") || std::getenv("MINIRECAST_BP_PPT") ||
+                     std::getenv("MINIRECAST_BP_LAYOUT") || std::getenv("MINIRECAST_BP_DEPTH");
     if (const char* v = std::getenv("MINIRECAST_BP_VARIANT")) {
         const int k = std::atoi(v);
         c->bp_variant = (k >= 1 && k <= 6) ? k : c->bp_variant;
@@ -814,6 +825,7 @@ int mr_set_bp_config(mr_ctx* c, int variant, int ppt, int layout, int depth) {
     c->bp_ppt = ppt;
     c->bp_layout = layout;
     c->bp_depth = depth;
+    c->bp_explicit = true;
     return MR_OK;
 }
 
