@@ -1,0 +1,111 @@
+"""GPU tests at the benchmark's full size: 1200 views of a 1536 x 1944 detector, three
+oblique 1024^2 slices (1,985,962 in-box points), f32 and f16 rings in their default
+back-projection configurations.
+
+Checked: the float64 oracle on 3 x 256 sampled in-box pixels (f32 <= 1e-5, f16 <= 1e-3
+relative L2), stream == batch (interleaved arrival in batches of 40), exact linearity
+of the f32 path (projections x 2 give outputs x 2 bit for bit: scaling by 2 is exact in
+binary floating point), and the default f16 kernel against the plain v1 kernel on every
+output.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def full():
+    import torch
+    from minirecast import _hip, phantom, recon, workloads
+    from minirecast.stream import arrival_order
+    from oracle import oracle_c as OC
+    w = workloads.full_cone()
+    g = w.geom
+    A, R, C = g.n_angles, g.rows, g.cols
+    box = g.volume_box()
+    names = ("f32", "f16", "f32_stream", "f32_x2")
+    ctx = {}
+    for name in names:
+        ctx[name] = _hip.Context(0)
+        ctx[name].set_cone(g.vectors, R, C, w.window, name.split("_")[0])
+    batch = 40
+    raw = torch.empty(R * batch * C, dtype=torch.float32, device="cuda")
+    rng = np.random.default_rng(2024)
+    samp_idx, samp_pts = [], []
+    tasks, off = [], 0
+    for p in w.planes:
+        pts, mask = recon.slice_points(box, p['origin'], p['normal'], p['n'])
+        sel = np.sort(rng.choice(np.flatnonzero(mask.ravel()), size=256, replace=False))
+        samp_idx.append(off + sel)
+        samp_pts.append(pts.reshape(-1, 3)[sel])
+        tasks.append(recon.plane_task(box, p['origin'], p['normal'], p['n'], out_offset=off))
+        off += p['n'] ** 2
+    samp_idx = np.concatenate(samp_idx)
+    samp_pts = np.concatenate(samp_pts)
+    acc64 = np.zeros(len(samp_pts))
+    host = np.empty((R, batch, C), np.float32)
+    OC.set_threads(16)
+
+    def ingest(names_, order):
+        for b0 in range(0, A, batch):
+            idx = order[b0:b0 + batch]
+            phantom.project_cone_device(w.phantom, type(g)(g.vectors[idx], R, C), ctx["f32"],
+                                        raw.data_ptr())
+            torch.cuda.synchronize()
+            for name in names_:
+                if name == "f32_x2":
+                    raw.mul_(2.0)
+                    torch.cuda.synchronize()
+                ctx[name].push_device(idx, raw.data_ptr())
+                if name == "f32_x2":
+                    raw.mul_(0.5)
+                    torch.cuda.synchronize()
+            if "f32" in names_:
+                ctx["f32"].d2h(host, raw.data_ptr())
+                for j, v in enumerate(idx):
+                    q64 = OC.fdk_filter_view(host, j, g.vectors[v], w.window)
+                    OC.bp_cone_acc_view(q64, g.vectors[v], 0.0, samp_pts, acc64)
+
+    ingest(("f32", "f16", "f32_x2"), np.arange(A))
+    ingest(("f32_stream",), arrival_order(A, "interleaved"))
+    del raw
+    torch.cuda.empty_cache()
+    T = np.array(tasks, dtype=_hip.TASK_DTYPE)
+    out = {name: ctx[name].bp_tasks(T, A, off) for name in names}
+    return dict(w=w, ctx=ctx, T=T, out_len=off, out=out, samp_idx=samp_idx,
+                ref=acc64 * (np.pi / A))
+
+
+def test_full_size_vs_float64_oracle(full):
+    for name, tol in (("f32", 1e-5), ("f16", 1e-3)):
+        got = full["out"][name][full["samp_idx"]].astype(np.float64)
+        err = rel_l2(got, full["ref"])
+        print(f"{name}: rel-L2 {err:.2e} vs float64 at {len(got)} sampled pixels")
+        assert err < tol, (name, err)
+
+
+def test_full_size_stream_equals_batch(full):
+    a, b = full["out"]["f32"], full["out"]["f32_stream"]
+    np.testing.assert_array_equal(a == 0, b == 0)
+    assert np.max(np.abs(a - b)) <= 1e-6 * np.max(np.abs(a))
+
+
+def test_full_size_linearity_is_exact(full):
+    np.testing.assert_array_equal(full["out"]["f32_x2"], 2.0 * full["out"]["f32"])
+
+
+def test_full_size_default_f16_kernel_vs_plain_kernel(full):
+    ctx = full["ctx"]["f16"]
+    cfg = ctx.bp_config()
+    assert cfg["kernel"] == 6                 # the default for 16-bit storage
+    ctx.set_bp_config(kernel=1, ppt=4, layout=0, depth=2)
+    try:
+        plain = ctx.bp_tasks(full["T"], full["w"].geom.n_angles, full["out_len"])
+    finally:
+        ctx.set_bp_config(**cfg)
+    got = full["out"]["f16"]
+    np.testing.assert_array_equal(got == 0, plain == 0)
+    assert rel_l2(got, plain) < 1e-6
