@@ -88,9 +88,13 @@ def test_full_size_vs_float64_oracle(full):
 
 
 def test_full_size_stream_equals_batch(full):
+    """The ring holds views in arrival order, so interleaved arrival only changes the
+    FP32 summation order over 1200 views (~2e-6 of the maximum here); the reference's
+    own stream == batch check allows 1e-4."""
     a, b = full["out"]["f32"], full["out"]["f32_stream"]
     np.testing.assert_array_equal(a == 0, b == 0)
-    assert np.max(np.abs(a - b)) <= 1e-6 * np.max(np.abs(a))
+    assert np.max(np.abs(a - b)) <= 1e-5 * np.max(np.abs(a))
+    assert rel_l2(b, a) < 1e-6
 
 
 def test_full_size_linearity_is_exact(full):

@@ -402,6 +402,7 @@ int mr_ctx_create(int device, mr_ctx** out) {
     }
     if (const char* v = std::getenv("MINIRECAST_BP_LAYOUT")) c->bp_layout = std::atoi(v) ? 1 : 0;
     if (const char* v = std::getenv("MINIRECAST_BP_DEPTH")) c->bp_depth = std::atoi(v) >= 3 ? 3 : 2;
+    if (const char* v = std::getenv("MINIRECAST_BP_SYNC")) c->bp_sync = std::atoi(v) ? 1 : 0;
     if (const char* v = std::getenv("MINIRECAST_XCD_MAP")) {
         const int k = std::atoi(v);
         c->xcd_map = (k == 1 || k == 2) ? k : 0;

@@ -854,7 +854,7 @@ template <int PPT, int LAY> struct Map4 {
     static constexpr int ds(int m) { return LAY == 0 ? 0 : (PPT == 4 ? 16 * (m & 1) : 16 * m); }
 };
 
-template <bool CONE, typename ST, int PPT, int LAY, int DEPTH, bool PAIRED>
+template <bool CONE, typename ST, int PPT, int LAY, int DEPTH, bool PAIRED, bool SYNC = false>
 __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel4(
     const ST* __restrict__ ring, int64_t view_stride, int pitch,
     const float* __restrict__ recs, const mr_task* __restrict__ tasks,
@@ -977,6 +977,10 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel4(
                 // keep the requests ahead of the interpolation of older views (the
                 // scheduler would otherwise sink them below the waits)
                 __builtin_amdgcn_sched_barrier(0);
+                // SYNC: the four waves (adjacent rows of one tile, so mostly the same
+                // detector lines) request each view together and share its L1 fills;
+                // an execution barrier only, loads stay in flight across it
+                if (SYNC) __builtin_amdgcn_s_barrier();
                 consume(S[j]);
             }
         }
@@ -1770,12 +1774,26 @@ static void launch_v4_one(Ctx* c, const float* recs, const mr_task* d_tasks, con
     const ST* ring = static_cast<const ST*>(paired ? c->pring : c->ring);
     const int64_t vs = paired ? 2 * static_cast<int64_t>(c->R + 2) * c->pitch : c->view_stride;
     const int xcd = c->xcd_map != 0;
-    if (paired) {
+    if (paired && c->bp_sync) {
+        if (c->bp_depth >= 3)
+            hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, 3, true, true>), grid, block, 0, s,
+                               ring, vs, c->pitch, recs, d_tasks, d_tiles, n_views, scale, d_out, xcd);
+        else
+            hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, 2, true, true>), grid, block, 0, s,
+                               ring, vs, c->pitch, recs, d_tasks, d_tiles, n_views, scale, d_out, xcd);
+    } else if (paired) {
         if (c->bp_depth >= 3)
             hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, 3, true>), grid, block, 0, s,
                                ring, vs, c->pitch, recs, d_tasks, d_tiles, n_views, scale, d_out, xcd);
         else
             hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, 2, true>), grid, block, 0, s,
+                               ring, vs, c->pitch, recs, d_tasks, d_tiles, n_views, scale, d_out, xcd);
+    } else if (c->bp_sync) {
+        if (c->bp_depth >= 3)
+            hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, 3, false, true>), grid, block, 0, s,
+                               ring, vs, c->pitch, recs, d_tasks, d_tiles, n_views, scale, d_out, xcd);
+        else
+            hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, 2, false, true>), grid, block, 0, s,
                                ring, vs, c->pitch, recs, d_tasks, d_tiles, n_views, scale, d_out, xcd);
     } else if (c->bp_depth >= 3) {
         hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, 3, false>), grid, block, 0, s,

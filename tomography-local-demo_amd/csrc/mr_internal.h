@@ -102,6 +102,8 @@ struct Ctx {
     int bp_layout = 0;               // v4 lane layout: 0 = 2 x 32, 1 = 4 x 16 (MINIRECAST_BP_LAYOUT)
     int bp_depth = 2;                // v4 views in flight, 2 or 3 (MINIRECAST_BP_DEPTH)
     bool bp_explicit = false;        // configuration chosen by the caller or the environment
+    int bp_sync = 0;                 // kernels 4/6: waves of a workgroup in lockstep per view
+                                     // (MINIRECAST_BP_SYNC)
     RawCorr raw;                     // raw-count correction of mr_push_raw
     void* pring = nullptr;           // row-pair copy of the ring (kernel 6)
     size_t pring_bytes = 0;
