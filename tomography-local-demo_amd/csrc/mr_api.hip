@@ -130,6 +130,7 @@ int alloc_ring(Ctx* c, int n_total, int rows, int cols, int window, int storage)
         c->bp_ppt = f32 ? 2 : 4;
         c->bp_layout = 0;
         c->bp_depth = 2;
+        c->bp_sync = f32 ? 0 : 1;
     }
     c->pitch = (cols + 3 + 63) / 64 * 64;
     c->Rp = rows + 3;
@@ -391,7 +392,8 @@ int mr_ctx_create(int device, mr_ctx** out) {
     mr_ctx* c = new mr_ctx();
     c->device = device;
     c->bp_explicit = std::getenv("MINIRECAST_BP_VARIANT") || std::getenv("MINIRECAST_BP_PPT") ||
-                     std::getenv("MINIRECAST_BP_LAYOUT") || std::getenv("MINIRECAST_BP_DEPTH");
+                     std::getenv("MINIRECAST_BP_LAYOUT") || std::getenv("MINIRECAST_BP_DEPTH") ||
+                     std::getenv("MINIRECAST_BP_SYNC");
     if (const char* v = std::getenv("MINIRECAST_BP_VARIANT")) {
         const int k = std::atoi(v);
         c->bp_variant = (k >= 1 && k <= 6) ? k : c->bp_variant;
