@@ -162,6 +162,22 @@ typedef struct mr_task {
 int mr_bp_tasks(mr_ctx* ctx, const mr_task* tasks, int ntasks, int n_views,
                 float* out, int64_t out_len, int out_on_device);
 
+/* ---- volume statistics for the threshold after a volume reconstruction (model.py:18-29
+ * otsu_threshold, which the reference computes with numpy on the host)
+ *
+ * Percentiles of the non-zero voxels: out[i] = np.percentile(vol[vol != 0], 100 q[i])
+ * (method "linear") for nq <= 4 fractions q[i] in [0, 1] (pass them as numpy computes
+ * them: np.true_divide(percent, np.float32(100))).  Exact order statistics by radix
+ * select on the device; *nonzero receives the count (MR_EVALUE when it is 0).  vol is
+ * float32 [n], a host (on_device = 0) or device (1) pointer. */
+int mr_nonzero_percentiles(mr_ctx* ctx, const float* vol, int64_t n, int on_device,
+                           const double* q, int nq, double* out, int64_t* nonzero);
+
+/* hist[bins] = np.histogram(np.clip(vol[vol != 0], lo, hi), bins, range=(lo, hi))[0],
+ * with edges[bins + 1] = np.linspace(lo, hi, bins + 1) supplied by the caller. */
+int mr_clip_histogram(mr_ctx* ctx, const float* vol, int64_t n, int on_device, double lo,
+                      double hi, int bins, const double* edges, int64_t* hist);
+
 /* ---- back-projection tuning (no counterpart in the reference: its Numba loop has
  * no shape parameters)
  *

@@ -586,6 +586,40 @@ def raw_correct(frames, dark, flat, style, sample, flip, crop):
 
 
 # --------------------------------------------------------------------------
+# Volume threshold (model.py:18-29)
+# --------------------------------------------------------------------------
+def otsu_threshold(vol, bins=512):
+    """Otsu's threshold on the non-zero voxels, restating model.py:18-29 with numpy."""
+    v = vol[vol != 0]
+    lo, hi = np.percentile(v, [0.1, 99.95])
+    hist, edges = np.histogram(np.clip(v, lo, hi), bins=bins, range=(lo, hi))
+    centers = 0.5 * (edges[1:] + edges[:-1])
+    w0 = np.cumsum(hist)
+    w1 = w0[-1] - w0
+    m0 = np.cumsum(hist * centers) / np.maximum(w0, 1)
+    m1 = (np.sum(hist * centers) - np.cumsum(hist * centers)) / np.maximum(w1, 1)
+    between = w0 * w1 * (m0 - m1) ** 2
+    return float(centers[int(np.argmax(between))])
+
+
+def percentile_from_sorted(s, q):
+    """np.percentile(x, 100 q) (method "linear") of float32 data from its sorted copy s:
+    virtual index (n - 1) q, gamma its fractional part, the neighbours' difference in
+    float32, the interpolation in float64 from b when gamma >= 0.5."""
+    n = len(s)
+    vi = (n - 1) * float(q)
+    if vi >= n - 1:
+        i = j = n - 1
+    else:
+        i = int(np.floor(vi))
+        j = i + 1
+    g = vi - np.floor(vi)
+    a, b = s[i], s[j]
+    d = np.float32(b - a)
+    return (float(b) - float(d) * (1.0 - g)) if g >= 0.5 else (float(a) + float(d) * g)
+
+
+# --------------------------------------------------------------------------
 # Analytic ellipsoid phantom (phantom.py:25-98), generalised to 3-D rotations
 # --------------------------------------------------------------------------
 def line_integrals(centers, mats, vals, starts, dirs):

@@ -443,3 +443,42 @@ def test_raw_ingest_vs_host_correction(style, sample):
     assert a.add_raw_projections(order[:3], frames[order[:3]], corr) == A and a.version == v0
     with pytest.raises(ValueError):
         a.add_raw_projections([0], frames[:1, :-1], corr)
+
+
+# ---------------------------------------------------------------- volume threshold
+def test_volume_percentiles_histogram_and_otsu_vs_numpy():
+    """Device percentiles of the non-zero voxels, the clipped histogram and the Otsu
+    threshold equal the reference's numpy computation (model.py:18-29) exactly, on
+    volumes with zeros, negative values and ties, from host arrays and device memory."""
+    from minirecast import _hip, model, recon
+    ctx = _hip.Context(0)
+    rng = np.random.default_rng(21)
+    cases = [rng.normal(0.2, 0.5, (40, 50, 30)).astype(np.float32),
+             np.round(rng.normal(0, 3, 120000)).astype(np.float32),          # ties, zeros
+             np.concatenate([rng.uniform(0.1, 0.3, 30000), rng.uniform(0.8, 1.0, 9000),
+                             np.zeros(20000), [-1e-30, 1e30]]).astype(np.float32)]
+    for vol in cases:
+        v = vol[vol != 0]
+        q = np.true_divide([0.1, 99.95], np.float32(100))
+        want = np.percentile(v, [0.1, 99.95])
+        got, m = ctx.nonzero_percentiles(vol, q)
+        assert m == v.size
+        np.testing.assert_array_equal(got, want)
+        lo, hi = want
+        want_h, edges = np.histogram(np.clip(v, lo, hi), bins=512, range=(lo, hi))
+        np.testing.assert_array_equal(ctx.clip_histogram(vol, lo, hi, edges), want_h)
+        assert model.otsu_threshold(vol, ctx=ctx) == O.otsu_threshold(vol)
+    # device-resident volume (a reconstruction left in HBM)
+    vol = cases[0]
+    d = ctx.alloc(vol.nbytes)
+    try:
+        ctx.h2d(d, vol)
+        got, m = ctx.nonzero_percentiles(None, q, dptr=d, n=vol.size)
+        np.testing.assert_array_equal(got, np.percentile(vol[vol != 0], [0.1, 99.95]))
+    finally:
+        ctx.free(d)
+    with pytest.raises(ValueError):
+        ctx.nonzero_percentiles(np.zeros(10, np.float32), q)
+    # a constant volume: numpy's widened range
+    c = np.full((5, 6, 7), 0.75, np.float32)
+    assert model.otsu_threshold(c, ctx=ctx) == O.otsu_threshold(c)

@@ -425,3 +425,39 @@ def test_raw_correct_known_answers(style):
                 dd, ff = float(dark[r0 + r, c0 + c]), float(flat[r0 + r, c0 + c])
                 y = (x - dd) / (max(ff - dd, 1.0) if style == "flexray" else max(ff - dd, 1e-6))
                 assert abs(got[r, k, c] - (-np.log(max(y, 1e-3)))) < 1e-5
+
+
+# ---------------------------------------------------------------- volume threshold
+def test_percentile_restatement_matches_numpy():
+    """The order-statistic form of np.percentile that the device implements (float32
+    neighbour difference, float64 interpolation) equals numpy bit for bit."""
+    rng = np.random.default_rng(9)
+    for _ in range(500):
+        n = int(rng.integers(1, 3000))
+        v = (rng.normal(size=n) * rng.uniform(0.1, 100)).astype(np.float32)
+        if rng.random() < 0.3:
+            v = np.round(v)                      # ties
+        s = np.sort(v)
+        q = np.true_divide([0.1, 99.95], np.float32(100))
+        want = np.percentile(v, [0.1, 99.95])
+        for qq, w in zip(q, want):
+            assert O.percentile_from_sorted(s, qq) == float(w)
+
+
+def test_otsu_known_answers():
+    """Two well separated uniform populations: the threshold lies between them (zeros
+    ignored); a constant volume gives the first bin centre of numpy's widened range."""
+    rng = np.random.default_rng(4)
+    a = rng.uniform(0.1, 0.3, 20000)
+    b = rng.uniform(0.8, 1.0, 5000)
+    vol = np.concatenate([a, b, np.zeros(7000)]).astype(np.float32)
+    rng.shuffle(vol)
+    t = O.otsu_threshold(vol.reshape(-1, 8))
+    # the between-class variance is flat across the gap and argmax takes its first
+    # maximum: the centre of the bin holding the largest value of the lower population
+    width = (np.percentile(vol[vol != 0], 99.95) - np.percentile(vol[vol != 0], 0.1)) / 512
+    assert a.max() - width <= t <= b.min()
+    # a constant volume: numpy widens the range to value -+ 0.5, the between-class
+    # variance is zero in every bin and argmax returns the first bin centre
+    c = np.full(100, 0.25, np.float32)
+    assert O.otsu_threshold(c) == (0.25 - 0.5) + 0.5 / 512

@@ -427,7 +427,7 @@ int mr_ctx_destroy(mr_ctx* c) {
     hipDeviceSynchronize();
     free_geometry(c);
     void* bufs[] = {c->d_stage, c->d_fstage, c->d_vmax, c->d_fv, c->d_req, c->d_out, c->d_pts, c->d_misc,
-                    c->d_pre, c->d_recs, c->d_raw};
+                    c->d_pre, c->d_recs, c->d_raw, c->d_vol, c->d_stat};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (c->h_req) hipHostFree(c->h_req);
@@ -817,6 +817,160 @@ bool valid_config(int variant, int ppt, int layout, int depth) {
            (layout == 0 || layout == 1) && (depth == 2 || depth == 3);
 }
 }  // namespace
+
+namespace {
+// The volume on the device: a caller pointer, or a staged copy of a host array.
+int stage_volume(Ctx* c, const float* vol, int64_t n, int on_device, const float** d_v) {
+    if (on_device) {
+        *d_v = vol;
+        return MR_OK;
+    }
+    int rc = ensure(&c->d_vol, &c->vol_bytes, sizeof(float) * static_cast<size_t>(n));
+    if (rc) return rc;
+    MR_HIP(hipMemcpyAsync(c->d_vol, vol, sizeof(float) * n, hipMemcpyHostToDevice, c->s_req));
+    *d_v = static_cast<const float*>(c->d_vol);
+    return MR_OK;
+}
+
+float key_float(uint32_t k) {
+    const uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    float x;
+    std::memcpy(&x, &u, 4);
+    return x;
+}
+
+// np.percentile(x, 100 q), method "linear", from the order statistics a = x_(i), b =
+// x_(i+1) of float32 data: virtual index (n - 1) q, gamma its fractional part, the
+// difference b - a in float32 (numpy subtracts the two float32 neighbours), the
+// interpolation in float64, from b when gamma >= 0.5.
+double numpy_lerp(float a, float b, double gamma) {
+#pragma clang fp contract(off)
+    const float d = b - a;
+    if (gamma >= 0.5) return static_cast<double>(b) - static_cast<double>(d) * (1.0 - gamma);
+    return static_cast<double>(a) + static_cast<double>(d) * gamma;
+}
+}  // namespace
+
+int mr_nonzero_percentiles(mr_ctx* c, const float* vol, int64_t n, int on_device,
+                           const double* q, int nq, double* out, int64_t* nonzero) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    MR_CHECK(n >= 0 && (n == 0 || vol), MR_EVALUE, "null volume");
+    MR_CHECK(nq >= 1 && nq <= 4 && q && out, MR_EVALUE, "1..4 quantiles with an output");
+    for (int i = 0; i < nq; ++i)
+        MR_CHECK(q[i] >= 0.0 && q[i] <= 1.0, MR_EVALUE, "quantiles must be in [0, 1]");
+    MR_HIP(hipSetDevice(c->device));
+    const float* d_v = nullptr;
+    if (n > 0) {
+        rc = stage_volume(c, vol, n, on_device, &d_v);
+        if (rc) return rc;
+    }
+    constexpr int T = 8;
+    const size_t hist_bytes = sizeof(unsigned long long) * T * 256;
+    rc = ensure(&c->d_stat, &c->stat_bytes, hist_bytes + 256);
+    if (rc) return rc;
+    auto* d_hist = static_cast<unsigned long long*>(c->d_stat);
+    auto* d_pre = reinterpret_cast<uint32_t*>(static_cast<char*>(c->d_stat) + hist_bytes);
+    std::vector<unsigned long long> h(T * 256);
+    // pass 0: the top byte of every non-zero key (also the count)
+    int64_t m = 0;
+    if (n > 0) {
+        MR_HIP(hipMemsetAsync(d_hist, 0, hist_bytes, c->s_req));
+        rc = launch_radix_hist(d_v, n, d_pre, 1, 24, d_hist, c->s_req);
+        if (rc) return rc;
+        MR_HIP(hipMemcpyAsync(h.data(), d_hist, sizeof(unsigned long long) * 256,
+                              hipMemcpyDeviceToHost, c->s_req));
+        MR_HIP(hipStreamSynchronize(c->s_req));
+        for (int b = 0; b < 256; ++b) m += static_cast<int64_t>(h[b]);
+    }
+    if (nonzero) *nonzero = m;
+    MR_CHECK(m > 0, MR_EVALUE, "no non-zero values");
+    // target ranks (numpy _get_indexes): floor and floor + 1 of (m - 1) q, both m - 1
+    // at or above the last index
+    std::vector<int64_t> lo(nq), hi(nq);
+    std::vector<double> gam(nq);
+    std::vector<int64_t> ranks;
+    for (int i = 0; i < nq; ++i) {
+        const double vi = static_cast<double>(m - 1) * q[i];
+        if (vi >= static_cast<double>(m - 1)) {
+            lo[i] = hi[i] = m - 1;
+        } else {
+            lo[i] = static_cast<int64_t>(std::floor(vi));
+            hi[i] = lo[i] + 1;
+        }
+        gam[i] = vi - std::floor(vi);
+        ranks.push_back(lo[i]);
+        ranks.push_back(hi[i]);
+    }
+    std::sort(ranks.begin(), ranks.end());
+    ranks.erase(std::unique(ranks.begin(), ranks.end()), ranks.end());
+    const int nt = static_cast<int>(ranks.size());
+    std::vector<uint32_t> prefix(nt, 0u);
+    std::vector<int64_t> rem(ranks);
+    auto walk = [&](const unsigned long long* hist, int t, int shift) {
+        int64_t cum = 0;
+        for (int b = 0; b < 256; ++b) {
+            const int64_t cnt = static_cast<int64_t>(hist[b]);
+            if (rem[t] < cum + cnt) {
+                prefix[t] |= static_cast<uint32_t>(b) << shift;
+                rem[t] -= cum;
+                return;
+            }
+            cum += cnt;
+        }
+    };
+    for (int t = 0; t < nt; ++t) walk(h.data(), t, 24);
+    for (int shift = 16; shift >= 0; shift -= 8) {
+        MR_HIP(hipMemsetAsync(d_hist, 0, hist_bytes, c->s_req));
+        MR_HIP(hipMemcpyAsync(d_pre, prefix.data(), sizeof(uint32_t) * nt, hipMemcpyHostToDevice,
+                              c->s_req));
+        rc = launch_radix_hist(d_v, n, d_pre, nt, shift, d_hist, c->s_req);
+        if (rc) return rc;
+        MR_HIP(hipMemcpyAsync(h.data(), d_hist, sizeof(unsigned long long) * 256 * nt,
+                              hipMemcpyDeviceToHost, c->s_req));
+        MR_HIP(hipStreamSynchronize(c->s_req));
+        for (int t = 0; t < nt; ++t) walk(h.data() + 256 * t, t, shift);
+    }
+    auto value_of = [&](int64_t r) {
+        const int t = static_cast<int>(std::lower_bound(ranks.begin(), ranks.end(), r) - ranks.begin());
+        return key_float(prefix[t]);
+    };
+    for (int i = 0; i < nq; ++i) out[i] = numpy_lerp(value_of(lo[i]), value_of(hi[i]), gam[i]);
+    return MR_OK;
+}
+
+int mr_clip_histogram(mr_ctx* c, const float* vol, int64_t n, int on_device, double lo, double hi,
+                      int bins, const double* edges, int64_t* hist) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    MR_CHECK(n >= 0 && (n == 0 || vol), MR_EVALUE, "null volume");
+    MR_CHECK(bins >= 1 && bins <= 8192 && edges && hist, MR_EVALUE, "1..8192 bins with edges");
+    MR_CHECK(std::isfinite(lo) && std::isfinite(hi) && lo < hi, MR_EVALUE,
+             "range must be finite with lo < hi");
+    MR_HIP(hipSetDevice(c->device));
+    const size_t hb = sizeof(unsigned long long) * bins, eb = sizeof(double) * (bins + 1);
+    rc = ensure(&c->d_stat, &c->stat_bytes, hb + eb + 256);
+    if (rc) return rc;
+    auto* d_hist = static_cast<unsigned long long*>(c->d_stat);
+    auto* d_edges = reinterpret_cast<double*>(static_cast<char*>(c->d_stat) + (hb + 255) / 256 * 256);
+    MR_CHECK((hb + 255) / 256 * 256 + eb <= c->stat_bytes, MR_ERUNTIME, "statistics buffer");
+    MR_HIP(hipMemsetAsync(d_hist, 0, hb, c->s_req));
+    MR_HIP(hipMemcpyAsync(d_edges, edges, eb, hipMemcpyHostToDevice, c->s_req));
+    if (n > 0) {
+        const float* d_v = nullptr;
+        rc = stage_volume(c, vol, n, on_device, &d_v);
+        if (rc) return rc;
+        rc = launch_clip_hist(d_v, n, lo, hi, bins, d_edges, d_hist, c->s_req);
+        if (rc) return rc;
+    }
+    std::vector<unsigned long long> h(bins);
+    MR_HIP(hipMemcpyAsync(h.data(), d_hist, hb, hipMemcpyDeviceToHost, c->s_req));
+    MR_HIP(hipStreamSynchronize(c->s_req));
+    for (int b = 0; b < bins; ++b) hist[b] = static_cast<int64_t>(h[b]);
+    return MR_OK;
+}
 
 int mr_set_bp_config(mr_ctx* c, int variant, int ppt, int layout, int depth) {
     int rc = check_ctx(c);

@@ -110,6 +110,10 @@ struct Ctx {
     int pring_n = 0;                 // views converted into pring
     uint16_t* d_raw = nullptr;       // staged raw frames
     size_t raw_bytes = 0;
+    void* d_vol = nullptr;           // staged volume for the statistics entry points
+    size_t vol_bytes = 0;
+    void* d_stat = nullptr;          // histograms, prefixes, edges
+    size_t stat_bytes = 0;
     float* d_recs = nullptr;         // v4 per-(tile, view) records
     size_t recs_bytes = 0;
 
@@ -199,6 +203,12 @@ int launch_bp_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int nti
                     const TileQueue* q = nullptr);
 // raw frames [k][H][W] uint16 -> corrected [R][k][C] float32 (c->raw)
 int launch_raw_correct(Ctx* c, const uint16_t* d_frames, int k, float* d_out, hipStream_t s);
+// volume statistics (mr_volume.hip): per-pass radix histograms of up to 8 target ranks
+// and the clipped uniform-bin histogram
+int launch_radix_hist(const float* d_v, int64_t n, const uint32_t* d_prefix, int ntarget,
+                      int shift, unsigned long long* d_hist, hipStream_t s);
+int launch_clip_hist(const float* d_v, int64_t n, double lo, double hi, int bins,
+                     const double* d_edges, unsigned long long* d_hist, hipStream_t s);
 // zeros every point of the given tiles (tiles culled by the box test)
 int launch_zero_tiles(const mr_task* d_tasks, const Tile* d_tiles, int ntiles, float* d_out,
                       hipStream_t s);

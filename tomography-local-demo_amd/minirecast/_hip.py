@@ -38,6 +38,10 @@ SIGNATURES = [
                                          _c.c_int, _c.c_int, _c.c_int]),
     ("mr_push_raw", _c.c_int, [_P, _P, _c.c_int, _P, _c.c_int, _c.POINTER(_c.c_int)]),
     ("mr_last_tile_kernel_ms", _c.c_int, [_P, _c.POINTER(_c.c_double)]),
+    ("mr_nonzero_percentiles", _c.c_int, [_P, _P, _c.c_int64, _c.c_int, _P, _c.c_int, _P,
+                                          _c.POINTER(_c.c_int64)]),
+    ("mr_clip_histogram", _c.c_int, [_P, _P, _c.c_int64, _c.c_int, _c.c_double, _c.c_double,
+                                     _c.c_int, _P, _P]),
     ("mr_set_bp_config", _c.c_int, [_P, _c.c_int, _c.c_int, _c.c_int, _c.c_int]),
     ("mr_get_bp_config", _c.c_int, [_P, _P]),
     ("mr_autotune", _c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_int64, _c.c_int, _c.c_int, _P,
@@ -246,6 +250,33 @@ class Context:
         check(self.lib.mr_push_raw(self.h, ptr(idx), len(idx), ctypes.c_void_p(dptr), 1,
                                    ctypes.byref(n)))
         return n.value
+
+    def nonzero_percentiles(self, vol, q, dptr=None, n=None):
+        """np.percentile(vol[vol != 0], 100 q) for fractions q (host array or device
+        pointer + count); returns (values float64, non-zero count)."""
+        qq = np.ascontiguousarray(q, dtype=np.float64).ravel()
+        out = np.zeros(len(qq))
+        m = ctypes.c_int64(0)
+        if dptr is None:
+            v = np.ascontiguousarray(vol, dtype=np.float32).ravel()
+            check(self.lib.mr_nonzero_percentiles(self.h, ptr(v), v.size, 0, ptr(qq), len(qq),
+                                                  ptr(out), ctypes.byref(m)))
+        else:
+            check(self.lib.mr_nonzero_percentiles(self.h, ctypes.c_void_p(dptr), int(n), 1,
+                                                  ptr(qq), len(qq), ptr(out), ctypes.byref(m)))
+        return out, int(m.value)
+
+    def clip_histogram(self, vol, lo, hi, edges, dptr=None, n=None):
+        e = np.ascontiguousarray(edges, dtype=np.float64)
+        hist = np.zeros(len(e) - 1, dtype=np.int64)
+        if dptr is None:
+            v = np.ascontiguousarray(vol, dtype=np.float32).ravel()
+            check(self.lib.mr_clip_histogram(self.h, ptr(v), v.size, 0, float(lo), float(hi),
+                                             len(hist), ptr(e), ptr(hist)))
+        else:
+            check(self.lib.mr_clip_histogram(self.h, ctypes.c_void_p(dptr), int(n), 1, float(lo),
+                                             float(hi), len(hist), ptr(e), ptr(hist)))
+        return hist
 
     BP_CONFIG_KEYS = ("kernel", "ppt", "layout", "depth")
 
