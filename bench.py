@@ -139,6 +139,62 @@ def _line(metric, value, unit, ms, steps, warmup, cfg, extra, higher=True, dtype
     print(json.dumps(d), flush=True)
 
 
+def bench_full_volume(args):
+    """The 768^3 volume of the full_cone scan (5.44e11 updates), reconstructed into HBM
+    from the f16 ring in one request (z-planes of 32 x 32 tiles), then the Otsu
+    threshold of model.py computed on the device-resident volume."""
+    from minirecast import _hip, model, phantom, recon, workloads
+    w = workloads.full_cone()
+    g = w.geom
+    A, R, C = g.n_angles, g.rows, g.cols
+    ctx = _hip.Context(0)
+    ctx.set_cone(g.vectors, R, C, w.window, "f16")
+    batch = 40
+    d_raw = ctx.alloc(R * batch * C * 4)
+    for b0 in range(0, A, batch):
+        idx = np.arange(b0, min(A, b0 + batch))
+        phantom.project_cone_device(w.phantom, type(g)(g.vectors[idx], R, C), ctx, d_raw)
+        ctx.push_device(idx, d_raw)
+    ctx.free(d_raw)
+    box = g.volume_box()
+    dims, origin, h = recon.volume_grid(box, w.extra["volume"])
+    nvox = int(np.prod(dims))
+    T = recon.volume_task(origin, h, dims, 0, int(dims[2]))[None]
+    dvol = ctx.alloc(nvox * 4)
+    ctx.bp_tasks_device(T, A, dvol, nvox)                  # warm-up (row-pair ring, records)
+    ctx.synchronize()
+    times, tile_ms = [], []
+    for _ in range(max(1, args.steps)):
+        t0 = time.perf_counter()
+        ctx.bp_tasks_device(T, A, dvol, nvox)
+        ctx.synchronize()
+        times.append((time.perf_counter() - t0) * 1e3)
+        tile_ms.append(ctx.last_tile_ms())
+    ms = float(np.median(times))
+    t0 = time.perf_counter()
+    q = np.true_divide(np.asarray([0.1, 99.95]), np.float32(100))
+    (lo, hi), m = ctx.nonzero_percentiles(None, q, dptr=dvol, n=nvox)
+    edges = np.linspace(lo, hi, 513)
+    hist = ctx.clip_histogram(None, lo, hi, edges, dptr=dvol, n=nvox)
+    centers = 0.5 * (edges[1:] + edges[:-1])
+    w0 = np.cumsum(hist)
+    w1 = w0[-1] - w0
+    m0 = np.cumsum(hist * centers) / np.maximum(w0, 1)
+    m1 = (np.sum(hist * centers) - np.cumsum(hist * centers)) / np.maximum(w1, 1)
+    thr = float(centers[int(np.argmax(w0 * w1 * (m0 - m1) ** 2))])
+    otsu_ms = (time.perf_counter() - t0) * 1e3
+    ctx.free(dvol)
+    upd = nvox * A
+    res = dict(dims=[int(d) for d in dims], voxel_mm=float(h), ms=ms,
+               tile_kernel_ms=float(np.median(tile_ms)), gups=upd / (ms * 1e-3) / 1e9,
+               otsu_ms=otsu_ms, otsu_threshold=thr, nonzero_voxels=m, config=ctx.bp_config())
+    log(f"full_volume: {res}")
+    _line("ms per 768^3 volume from 1200 views (f16 ring) + GUPS", round(res["gups"], 2),
+          "GUPS", round(ms, 3), len(times), 1,
+          {"workload": "full_cone volume: 768^3 grid over the box from 1200 views, 1536x1944 "
+                       "detector", "storage": "f16"}, {"volume": res})
+
+
 def bench_stream_cone(args):
     """Streaming ingest: 600 views in interleaved batches of 10 (Poisson noise, 384 x 486
     detector); after every batch the three oblique 512^2 slices and the 64^3 preview
@@ -336,7 +392,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="full_cone",
-                    choices=["full_cone", "stream_cone", "parallel_cor", "diff_cone"])
+                    choices=["full_cone", "full_volume", "stream_cone", "parallel_cor",
+                             "diff_cone"])
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-check", action="store_true", help="skip the float64 oracle error")
@@ -357,7 +414,7 @@ def main():
 
     if args.workload != "full_cone":
         {"stream_cone": bench_stream_cone, "parallel_cor": bench_parallel_cor,
-         "diff_cone": bench_diff_cone}[args.workload](args)
+         "diff_cone": bench_diff_cone, "full_volume": bench_full_volume}[args.workload](args)
         return
     from minirecast import _hip, phantom, recon, workloads
     if args.calibrate_copy:
