@@ -854,8 +854,11 @@ template <int PPT, int LAY> struct Map4 {
     static constexpr int ds(int m) { return LAY == 0 ? 0 : (PPT == 4 ? 16 * (m & 1) : 16 * m); }
 };
 
+// 16-bit forms, 2 views in flight: at least 8 waves per SIMD (the 4-point kernel then fits
+// 59 VGPRs without spills instead of 72, i.e. 7 waves); the others keep the compiler's
+// choice (they would spill).
 template <bool CONE, typename ST, int PPT, int LAY, int DEPTH, bool PAIRED, bool SYNC = false>
-__global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel4(
+__global__ __launch_bounds__(BP_THREADS, (sizeof(ST) == 2 && DEPTH == 2) ? 8 : 1) void bp_tiles_kernel4(
     const ST* __restrict__ ring, int64_t view_stride, int pitch,
     const float* __restrict__ recs, const mr_task* __restrict__ tasks,
     const Tile* __restrict__ tiles, int n_views, float scale, float* __restrict__ out, int xcd) {
