@@ -482,3 +482,62 @@ def test_volume_percentiles_histogram_and_otsu_vs_numpy():
     # a constant volume: numpy's widened range
     c = np.full((5, 6, 7), 0.75, np.float32)
     assert model.otsu_threshold(c, ctx=ctx) == O.otsu_threshold(c)
+
+
+# ---------------------------------------------------------------- serving
+def test_slice_service_replies_and_coalescing(small_cone):
+    """The server's requests answered on the GPU path equal direct calls; concurrent
+    slice requests are served by fewer back-projection launches than requests."""
+    import threading
+    from multiprocessing import Pipe
+    from minirecast import recon, serve
+    w, proj, _, _ = small_cone
+    rec = recon.SliceReconstructor(proj, w.geom, window=w.window, verbose=False)
+    svc = serve.SliceService(window=w.window, coalesce_ms=20.0)
+    h, arr = serve.geometry_message(w.geom)
+    assert svc.handle(h, arr)[0] == {"ok": True}
+    A = w.geom.n_angles
+    for i in range(0, A, 60):
+        rep, _ = svc.handle({"cmd": "projections", "indices": list(range(i, i + 60))},
+                            proj[:, i:i + 60, :])
+        assert rep == {"ok": True, "n": i + 60}
+    planes = [dict(origin=(0.1 * k, 0.0, 0.2), normal=(1.0, 0.3 * k, 0.8), n=40 + k)
+              for k in range(6)]
+    direct = [rec.reconstruct_plane(p['origin'], p['normal'], n=p['n'])[0] for p in planes]
+    got = [None] * len(planes)
+
+    def worker(k):
+        rep, img = svc.handle({"cmd": "slice", **planes[k]})
+        got[k] = img
+
+    threads = [threading.Thread(target=worker, args=(k,)) for k in range(len(planes))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    for k in range(len(planes)):
+        assert got[k].dtype == np.float32
+        np.testing.assert_allclose(got[k], direct[k], rtol=0, atol=1e-6 * np.abs(direct[k]).max())
+    assert svc.slices_served == len(planes) and svc.launches < len(planes)
+    # the other requests
+    rep, flat = svc.handle({"cmd": "slices", "planes": planes[:2]})
+    assert rep["sizes"] == [40, 41] and flat.size == 40 * 40 + 41 * 41
+    assert svc.handle({"cmd": "info"})[0]["n"] == A
+    rep, vol = svc.handle({"cmd": "preview", "maxdim": 16})
+    ref_vol, ref_origin, ref_h = rec.preview_volume(16)
+    np.testing.assert_allclose(vol, ref_vol, atol=1e-6 * np.abs(ref_vol).max())
+    assert rep["spacing"] == ref_h
+    assert "best" in svc.handle({"cmd": "auto_cor"})[0]
+    assert svc.handle({"cmd": "set_cor", "px": 0.5})[0] == {"ok": True}
+    assert "error" in svc.handle({"cmd": "nope"})[0]
+    # the request loop over a connection
+    a, b = Pipe()
+    t = threading.Thread(target=svc.serve, args=(b,), daemon=True)
+    t.start()
+    serve.send(a, {"cmd": "slice", **planes[0]})
+    rep, img = serve.recv(a)
+    assert img.shape == (40, 40) and rep["version"] == svc.recon.version
+    serve.send(a, {"cmd": "projections", "indices": [0]}, proj[:, :2, :])   # count mismatch
+    assert "error" in serve.recv(a)[0]
+    a.close()
+    t.join(timeout=10)
