@@ -15,11 +15,13 @@ Reported (one JSON line on stdout):
   projections, oracle/liboracle_exact.so).
   roofline   for the back-projection tile kernel: compulsory HBM bytes (union of the 2x2
              tap footprints over all requested points, per view, x bytes per sample,
-             plus the output) / its average duration (HIP events around the tile kernel
-             on its stream), against 8 TB/s; traffic = measured HBM bytes per launch
-             from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) of a short run of
-             the same f16 update, each scaled by the counter's reading for a 1 GiB
-             device copy (null when rocprofv3 is unavailable or --no-traffic).
+             plus the output) / the duration of one update's tile-kernel launches (HIP
+             events around them on their stream; an update that fills the device runs
+             as one launch per block of 64 views), against 8 TB/s; traffic = measured
+             HBM bytes per update (all its tile-kernel dispatches) from two rocprofv3
+             --pmc passes (FETCH_SIZE, WRITE_SIZE) of a short run of the same f16
+             update, each scaled by the counter's reading for a 1 GiB device copy (null
+             when rocprofv3 is unavailable or --no-traffic).
   cpu_baseline  the Numba-equivalent CPU restatement (oracle/liboracle_fast.so:
              fast-math, OpenMP over points, views serial, float64 accumulation),
              timed on this host on a bounded sample of the same update.
@@ -78,8 +80,9 @@ def footprint_pixels(geom, pts, views):
 
 def _pmc_pass(counter):
     """One rocprofv3 --pmc pass of a short f16 run of this benchmark (child process):
-    mean counter value per dispatch of the f16 cone tile kernel, and the value for the
-    1 GiB copy run first by the child."""
+    counter value per update of the f16 cone tile kernel (mean per dispatch x the
+    dispatches of one update, which the child reports), and the value for the 1 GiB copy
+    run first by the child."""
     out = tempfile.mkdtemp(prefix="mr_pmc_")
     cmd = ["rocprofv3", "--pmc", counter, "-d", out, "-o", "run", "--output-format", "csv",
            "--", sys.executable, "-u", os.path.abspath(__file__), "--pmc-child", "--steps", "2",
@@ -103,7 +106,13 @@ def _pmc_pass(counter):
                     cp.append(val)
         if not bp or not cp:
             return None
-        return float(np.mean(bp)), float(max(cp))
+        # one update = `launches` dispatches of the tile kernel (blocks of views)
+        launches = 1
+        with open(os.path.join(out, "child.log")) as fh:
+            for ln in fh:
+                if "tile launches per update:" in ln:
+                    launches = int(ln.rsplit(":", 1)[1])
+        return float(np.mean(bp)) * launches, float(max(cp))
     except (OSError, subprocess.SubprocessError, ValueError, KeyError):
         return None
     finally:
@@ -515,7 +524,10 @@ def main():
         img = np.empty(out_len, np.float32)
         c.d2h(img, dout)
         res = dict(ms=wall, kernel_ms=float(np.mean(kms)), tile_kernel_ms=float(np.mean(tms)),
+                   tile_launches=c.last_tile_launches(),
                    gups=n_in * A / (wall * 1e-3) / 1e9, config=c.bp_config())
+        if args.pmc_child and s == "f16":
+            log(f"tile launches per update: {res['tile_launches']}")
         if check:
             got = np.concatenate([img[o:o + p['n'] ** 2][si] for o, p, si in
                                   zip(np.cumsum([0] + [p['n'] ** 2 for p in w.planes[:-1]]),

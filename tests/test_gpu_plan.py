@@ -145,13 +145,37 @@ def test_autotune_picks_a_configuration_and_keeps_parity(workload):
     rec = recon.SliceReconstructor(proj, w.geom, window=w.window, verbose=False)
     cfg, ms = rec.autotune(planes, max_tiles=64, reps=1)
     assert ms > 0 and cfg == rec._ctx.bp_config()
-    assert cfg["kernel"] in (2, 4, 5) and cfg["ppt"] in (1, 2, 4)
+    assert cfg["kernel"] in (2, 4, 5, 6) and cfg["ppt"] in (1, 2, 4)
     got = rec._ctx.bp_tasks(T, A, out_len)
     assert rel_l2(got, ref) < 1e-6, (cfg, rel_l2(got, ref))
     with pytest.raises(ValueError):
         rec._ctx.set_bp_config(kernel=7)
     with pytest.raises(ValueError):
         rec._ctx.set_bp_config(kernel=4, ppt=3)
+
+
+@pytest.mark.parametrize("variant,vchunk", [("6", "1"), ("6", "7"), ("4", "64"), ("4", "5")])
+def test_view_chunked_launches_are_exact(workload, variant, vchunk, monkeypatch):
+    """Cone-beam updates split into launches of a few views each (the FP32 sums carried
+    through the output between launches) equal the single launch bit for bit.  The
+    split applies to grids of at least 8 workgroups per CU: the slices here are small,
+    so MINIRECAST_BP_VCHUNK_ANY lifts that condition."""
+    from minirecast import _hip
+    w, proj, planes, tasks, out_len = workload
+    T = np.array(tasks, dtype=_hip.TASK_DTYPE)
+    A = w.geom.n_angles
+    monkeypatch.setenv("MINIRECAST_BP_PPT", "4")
+    for storage in ("f32", "f16"):
+        monkeypatch.setenv("MINIRECAST_BP_VCHUNK", "0")
+        one = _ctx(w, storage, variant, monkeypatch)
+        monkeypatch.setenv("MINIRECAST_BP_VCHUNK", vchunk)
+        monkeypatch.setenv("MINIRECAST_BP_VCHUNK_ANY", "1")
+        split = _ctx(w, storage, variant, monkeypatch)
+        monkeypatch.delenv("MINIRECAST_BP_VCHUNK_ANY")
+        for c in (one, split):
+            c.push(np.arange(A), proj)
+        for nv in (A, A - 1, 2, 1):
+            np.testing.assert_array_equal(split.bp_tasks(T, nv, out_len), one.bp_tasks(T, nv, out_len))
 
 
 def test_pair_ring_follows_appends_and_resets(workload, monkeypatch):

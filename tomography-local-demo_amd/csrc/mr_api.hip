@@ -132,6 +132,9 @@ int alloc_ring(Ctx* c, int n_total, int rows, int cols, int window, int storage)
         c->bp_depth = 2;
         c->bp_sync = f32 ? 0 : 1;
     }
+    // measured: cone updates that fill the device run fastest in launches of 32 (f32) or
+    // 64 (16-bit) views
+    if (!c->vchunk_explicit) c->bp_vchunk = storage == MR_STORE_F32 ? 32 : 64;
     c->pitch = (cols + 3 + 63) / 64 * 64;
     c->Rp = rows + 3;
     c->view_stride = static_cast<int64_t>(c->Rp) * c->pitch;
@@ -262,6 +265,7 @@ int run_tasks(Ctx* c, const mr_task* tasks, int ntasks, int n_views, float* d_ou
         if (rc) return rc;
     }
     c->tile_timed = false;
+    c->tile_launches = 0;
     MR_HIP(hipEventRecord(c->ev0, c->s_req));
     if (op == OP_BP)
         rc = launch_bp_tiles(c, d_tasks, d_tiles, nt, n_views, scale, d_out, c->s_req);
@@ -280,6 +284,7 @@ int finish_timing(Ctx* c) {
     MR_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_kernel_ms = ms;
     c->last_tile_ms = ms;
+    c->last_tile_launches = c->tile_launches;
     if (c->tile_timed) {
         MR_HIP(hipEventElapsedTime(&ms, c->evk0, c->evk1));
         c->last_tile_ms = ms;
@@ -391,6 +396,9 @@ int mr_ctx_create(int device, mr_ctx** out) {
     MR_HIP(hipSetDevice(device));
     mr_ctx* c = new mr_ctx();
     c->device = device;
+    if (hipDeviceGetAttribute(&c->n_cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        c->n_cus < 1)
+        c->n_cus = 256;
     c->bp_explicit = std::getenv("MINIRECAST_BP_VARIANT") || std::getenv("MINIRECAST_BP_PPT") ||
                      std::getenv("MINIRECAST_BP_LAYOUT") || std::getenv("MINIRECAST_BP_DEPTH") ||
                      std::getenv("MINIRECAST_BP_SYNC");
@@ -405,6 +413,11 @@ int mr_ctx_create(int device, mr_ctx** out) {
     if (const char* v = std::getenv("MINIRECAST_BP_LAYOUT")) c->bp_layout = std::atoi(v) ? 1 : 0;
     if (const char* v = std::getenv("MINIRECAST_BP_DEPTH")) c->bp_depth = std::atoi(v) >= 3 ? 3 : 2;
     if (const char* v = std::getenv("MINIRECAST_BP_SYNC")) c->bp_sync = std::atoi(v) ? 1 : 0;
+    if (const char* v = std::getenv("MINIRECAST_BP_VCHUNK")) {
+        c->bp_vchunk = std::max(0, std::atoi(v));
+        c->vchunk_explicit = true;
+    }
+    if (const char* v = std::getenv("MINIRECAST_BP_VCHUNK_ANY")) c->vchunk_any = std::atoi(v) != 0;
     if (const char* v = std::getenv("MINIRECAST_XCD_MAP")) {
         const int k = std::atoi(v);
         c->xcd_map = (k == 1 || k == 2) ? k : 0;
@@ -1570,6 +1583,14 @@ int mr_last_tile_kernel_ms(mr_ctx* c, double* ms) {
     if (rc) return rc;
     MR_CHECK(ms != nullptr, MR_EVALUE, "null output");
     *ms = c->last_tile_ms;
+    return MR_OK;
+}
+
+int mr_last_tile_kernel_launches(mr_ctx* c, int* n) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    MR_CHECK(n != nullptr, MR_EVALUE, "null output");
+    *n = c->last_tile_launches;
     return MR_OK;
 }
 

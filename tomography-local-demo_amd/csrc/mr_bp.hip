@@ -857,11 +857,16 @@ template <int PPT, int LAY> struct Map4 {
 // 16-bit forms, 2 views in flight: at least 8 waves per SIMD (the 4-point kernel then fits
 // 59 VGPRs without spills instead of 72, i.e. 7 waves); the others keep the compiler's
 // choice (they would spill).
+//
+// Cone beam, views [vb, ve) of n_views: a launch with vb > 0 resumes from the unscaled
+// FP32 sums the previous launch left in `out`, and one with ve < n_views leaves them
+// there, so an update split into several launches equals the single launch bit for bit.
 template <bool CONE, typename ST, int PPT, int LAY, int DEPTH, bool PAIRED, bool SYNC = false>
 __global__ __launch_bounds__(BP_THREADS, (sizeof(ST) == 2 && DEPTH == 2) ? 8 : 1) void bp_tiles_kernel4(
     const ST* __restrict__ ring, int64_t view_stride, int pitch,
     const float* __restrict__ recs, const mr_task* __restrict__ tasks,
-    const Tile* __restrict__ tiles, int n_views, float scale, float* __restrict__ out, int xcd) {
+    const Tile* __restrict__ tiles, int n_views, float scale, float* __restrict__ out, int xcd,
+    int vb, int ve) {
     using M = Map4<PPT, LAY>;
     constexpr int LE = Taps<ST>::LOG_ESZ;
     constexpr int SUBS = M::SUBS;                    // workgroups per 32-row tile
@@ -893,6 +898,16 @@ __global__ __launch_bounds__(BP_THREADS, (sizeof(ST) == 2 && DEPTH == 2) ? 8 : 1
     int roffp[PPT];
 #pragma unroll
     for (int m = 0; m < PPT; ++m) { acc[m] = 0.0f; accb[m] = 0.0f; frp[m] = 0.0f; roffp[m] = 0; }
+    const int64_t obase = T.out_offset + static_cast<int64_t>(tl.q) * T.stride_q;
+    auto oidx = [&](int m) {
+        return obase + static_cast<int64_t>(p0 + pt + M::dp(m)) * T.stride_p +
+               static_cast<int64_t>(s0 + st0 + M::ds(m)) * T.stride_s;
+    };
+    if (CONE && vb > 0) {
+#pragma unroll
+        for (int m = 0; m < PPT; ++m)
+            if (valid[m]) acc[m] = out[oidx(m)];
+    }
     if (!CONE && n_views > 0) {
         // parallel beam: the row of a point is the same for every view
 #pragma unroll
@@ -967,13 +982,13 @@ __global__ __launch_bounds__(BP_THREADS, (sizeof(ST) == 2 && DEPTH == 2) ? 8 : 1
     // DEPTH views in flight: the taps of view v + DEPTH - 1 are requested before view v
     // is interpolated.  Requests past the last view re-read it (never consumed), so the
     // loop body is unconditional and every wait counts exactly the loads after it.
-    if (n_views > 0) {
+    if (ve > vb) {
         PointState<ST, PPT, PAIRED> S[DEPTH];
-        const int last = n_views - 1;
+        const int last = ve - 1;
 #pragma unroll
-        for (int j = 0; j + 1 < DEPTH; ++j) issue(min(j, last), S[j]);
-        int v = 0;
-        for (; v + DEPTH <= n_views; v += DEPTH) {
+        for (int j = 0; j + 1 < DEPTH; ++j) issue(min(vb + j, last), S[j]);
+        int v = vb;
+        for (; v + DEPTH <= ve; v += DEPTH) {
 #pragma unroll
             for (int j = 0; j < DEPTH; ++j) {
                 issue(min(v + j + DEPTH - 1, last), S[(j + DEPTH - 1) % DEPTH]);
@@ -989,18 +1004,16 @@ __global__ __launch_bounds__(BP_THREADS, (sizeof(ST) == 2 && DEPTH == 2) ? 8 : 1
         }
 #pragma unroll
         for (int j = 0; j + 1 < DEPTH; ++j)
-            if (v + j < n_views) consume(S[j]);
+            if (v + j < ve) consume(S[j]);
     }
 
-    const int64_t obase = T.out_offset + static_cast<int64_t>(tl.q) * T.stride_q;
+    const bool final_views = !CONE || ve >= n_views;
 #pragma unroll
     for (int m = 0; m < PPT; ++m) {
         if (!valid[m]) continue;
-        const int p = p0 + pt + M::dp(m), s = s0 + st0 + M::ds(m);
         float val = acc[m];
         if (!CONE) val = fmaf(frp[m], accb[m] - acc[m], acc[m]);
-        out[obase + static_cast<int64_t>(p) * T.stride_p + static_cast<int64_t>(s) * T.stride_s] =
-            inside[m] ? val * scale : 0.0f;
+        out[oidx(m)] = final_views ? (inside[m] ? val * scale : 0.0f) : val;
     }
 }
 
@@ -1769,42 +1782,43 @@ int launch_bp_points(Ctx* c, const double* d_pts, int64_t M, double cor_shift, i
                : dispatch_points<false>(c, d_pts, M, cor_shift, n_views, scale, d_out, s);
 }
 
+template <bool CONE, typename ST, int PPT, int LAY, int DEPTH, bool PAIRED, bool SYNC>
+static void launch_k4(Ctx* c, const ST* ring, int64_t vs, const float* recs, const mr_task* d_tasks,
+                      const Tile* d_tiles, int nt, int n_views, float scale, float* d_out,
+                      hipStream_t s) {
+    const int nb = Map4<PPT, LAY>::SUBS * nt;
+    // cone beam, a grid that fills the device: one launch per block of bp_vchunk views, so
+    // that all tiles read the same few views at a time (the launches then share detector
+    // lines in the L2: -30% L2-miss traffic at 32 views) at the cost of a sum round trip
+    // through `out` per launch
+    const bool split = CONE && c->bp_vchunk > 0 && (c->vchunk_any || nb >= 8 * c->n_cus);
+    const int k = split ? c->bp_vchunk : std::max(n_views, 1);
+    int vb = 0;
+    do {
+        const int ve = std::min(n_views, vb + k);
+        hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, DEPTH, PAIRED, SYNC>), dim3(nb),
+                           dim3(BP_THREADS), 0, s, ring, vs, c->pitch, recs, d_tasks, d_tiles,
+                           n_views, scale, d_out, c->xcd_map != 0 ? 1 : 0, vb, ve);
+        ++c->tile_launches;
+        vb = ve;
+    } while (vb < n_views);
+}
+
+// Kernels 4 and 6: the storage-independent switches (row-pair ring, lockstep waves, views
+// in flight) as template arguments.
 template <bool CONE, typename ST, int PPT, int LAY>
 static void launch_v4_one(Ctx* c, const float* recs, const mr_task* d_tasks, const Tile* d_tiles,
                           int nt, int n_views, float scale, float* d_out, hipStream_t s) {
-    const dim3 grid(Map4<PPT, LAY>::SUBS * nt), block(BP_THREADS);
     const bool paired = c->bp_variant == 6;
     const ST* ring = static_cast<const ST*>(paired ? c->pring : c->ring);
     const int64_t vs = paired ? 2 * static_cast<int64_t>(c->R + 2) * c->pitch : c->view_stride;
-    const int xcd = c->xcd_map != 0;
-    if (paired && c->bp_sync) {
-        if (c->bp_depth >= 3)
-            hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, 3, true, true>), grid, block, 0, s,
-                               ring, vs, c->pitch, recs, d_tasks, d_tiles, n_views, scale, d_out, xcd);
-        else
-            hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, 2, true, true>), grid, block, 0, s,
-                               ring, vs, c->pitch, recs, d_tasks, d_tiles, n_views, scale, d_out, xcd);
-    } else if (paired) {
-        if (c->bp_depth >= 3)
-            hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, 3, true>), grid, block, 0, s,
-                               ring, vs, c->pitch, recs, d_tasks, d_tiles, n_views, scale, d_out, xcd);
-        else
-            hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, 2, true>), grid, block, 0, s,
-                               ring, vs, c->pitch, recs, d_tasks, d_tiles, n_views, scale, d_out, xcd);
-    } else if (c->bp_sync) {
-        if (c->bp_depth >= 3)
-            hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, 3, false, true>), grid, block, 0, s,
-                               ring, vs, c->pitch, recs, d_tasks, d_tiles, n_views, scale, d_out, xcd);
-        else
-            hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, 2, false, true>), grid, block, 0, s,
-                               ring, vs, c->pitch, recs, d_tasks, d_tiles, n_views, scale, d_out, xcd);
-    } else if (c->bp_depth >= 3) {
-        hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, 3, false>), grid, block, 0, s,
-                           ring, vs, c->pitch, recs, d_tasks, d_tiles, n_views, scale, d_out, xcd);
-    } else {
-        hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, 2, false>), grid, block, 0, s,
-                           ring, vs, c->pitch, recs, d_tasks, d_tiles, n_views, scale, d_out, xcd);
-    }
+    const bool d3 = c->bp_depth >= 3;
+#define MR_K4(D, P, S) launch_k4<CONE, ST, PPT, LAY, D, P, S>(c, ring, vs, recs, d_tasks, d_tiles, nt, n_views, scale, d_out, s)
+    if (paired && c->bp_sync) d3 ? MR_K4(3, true, true) : MR_K4(2, true, true);
+    else if (paired) d3 ? MR_K4(3, true, false) : MR_K4(2, true, false);
+    else if (c->bp_sync) d3 ? MR_K4(3, false, true) : MR_K4(2, false, true);
+    else d3 ? MR_K4(3, false, false) : MR_K4(2, false, false);
+#undef MR_K4
 }
 
 template <bool CONE, typename ST>

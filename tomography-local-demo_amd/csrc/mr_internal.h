@@ -94,6 +94,8 @@ struct Ctx {
     double last_kernel_ms = 0.0;
     hipEvent_t evk0 = nullptr, evk1 = nullptr;   // around the tile kernel alone (v4 / v5)
     bool tile_timed = false;
+    int tile_launches = 0;           // tile-kernel dispatches of the request in flight
+    int last_tile_launches = 0;      // ... of the last finished request
     double last_tile_ms = 0.0;
     int bp_variant = 4;              // back-projection inner loop (MINIRECAST_BP_VARIANT)
     int bp_ppt = 2;                  // v4/v5 points per thread (MINIRECAST_BP_PPT)
@@ -102,6 +104,10 @@ struct Ctx {
     int bp_layout = 0;               // v4 lane layout: 0 = 2 x 32, 1 = 4 x 16 (MINIRECAST_BP_LAYOUT)
     int bp_depth = 2;                // v4 views in flight, 2 or 3 (MINIRECAST_BP_DEPTH)
     bool bp_explicit = false;        // configuration chosen by the caller or the environment
+    int bp_vchunk = 0;               // kernels 4/6 (cone): views per launch, 0 = all (MINIRECAST_BP_VCHUNK)
+    bool vchunk_explicit = false;    // views per launch set by the environment
+    bool vchunk_any = false;         // split grids of any size (MINIRECAST_BP_VCHUNK_ANY, tests)
+    int n_cus = 256;                 // compute units of the device
     int bp_sync = 0;                 // kernels 4/6: waves of a workgroup in lockstep per view
                                      // (MINIRECAST_BP_SYNC)
     RawCorr raw;                     // raw-count correction of mr_push_raw

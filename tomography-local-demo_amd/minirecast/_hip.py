@@ -38,6 +38,7 @@ SIGNATURES = [
                                          _c.c_int, _c.c_int, _c.c_int]),
     ("mr_push_raw", _c.c_int, [_P, _P, _c.c_int, _P, _c.c_int, _c.POINTER(_c.c_int)]),
     ("mr_last_tile_kernel_ms", _c.c_int, [_P, _c.POINTER(_c.c_double)]),
+    ("mr_last_tile_kernel_launches", _c.c_int, [_P, _c.POINTER(_c.c_int)]),
     ("mr_nonzero_percentiles", _c.c_int, [_P, _P, _c.c_int64, _c.c_int, _P, _c.c_int, _P,
                                           _c.POINTER(_c.c_int64)]),
     ("mr_clip_histogram", _c.c_int, [_P, _P, _c.c_int64, _c.c_int, _c.c_double, _c.c_double,
@@ -426,6 +427,12 @@ class Context:
         ms = ctypes.c_double(0.0)
         check(self.lib.mr_last_tile_kernel_ms(self.h, ctypes.byref(ms)))
         return ms.value
+
+    def last_tile_launches(self):
+        """Tile-kernel dispatches within the last bp_tasks call (kernels 4 and 6)."""
+        n = ctypes.c_int(0)
+        check(self.lib.mr_last_tile_kernel_launches(self.h, ctypes.byref(n)))
+        return n.value
 
     # ---------------------------------------------------------------- device memory
     def alloc(self, nbytes):
