@@ -294,10 +294,11 @@ int mr_last_kernel_ms(mr_ctx* ctx, double* ms);
  * mr_last_kernel_ms). */
 int mr_last_tile_kernel_ms(mr_ctx* ctx, double* ms);
 
-/* Dispatches of kernels 4/6 within the last mr_bp_tasks: a cone-beam update whose grid
- * fills the device (>= 8 workgroups per CU) runs as one launch per block of 32 (f32) or
- * 64 (16-bit) views, environment MINIRECAST_BP_VCHUNK (0: one launch).  0 for the other
- * kernels.  No counterpart in the reference (measurement only). */
+/* Dispatches of kernels 4/6 within the last mr_bp_tasks: a cone-beam request of
+ * single-plane tasks whose grid fills the device (>= 8 workgroups per CU) runs as one
+ * launch per block of 32 (f32) or 64 (16-bit) views, environment MINIRECAST_BP_VCHUNK (0:
+ * one launch).  0 for the other kernels.  No counterpart in the reference (measurement
+ * only). */
 int mr_last_tile_kernel_launches(mr_ctx* ctx, int* n);
 
 #ifdef __cplusplus

@@ -266,6 +266,8 @@ int run_tasks(Ctx* c, const mr_task* tasks, int ntasks, int n_views, float* d_ou
     }
     c->tile_timed = false;
     c->tile_launches = 0;
+    c->req_slices = true;
+    for (int i = 0; i < ntasks; ++i) c->req_slices = c->req_slices && tasks[i].nq <= 1;
     MR_HIP(hipEventRecord(c->ev0, c->s_req));
     if (op == OP_BP)
         rc = launch_bp_tiles(c, d_tasks, d_tiles, nt, n_views, scale, d_out, c->s_req);
@@ -1051,6 +1053,8 @@ int mr_autotune(mr_ctx* c, const mr_task* tasks, int ntasks, int n_views, int64_
     const Tile* d_tiles = reinterpret_cast<const Tile*>(static_cast<char*>(c->d_req) + off_tiles);
     const float scale = static_cast<float>(M_PI / n_views);
     const int nt = static_cast<int>(tiles.size());
+    c->req_slices = true;
+    for (int i = 0; i < ntasks; ++i) c->req_slices = c->req_slices && tasks[i].nq <= 1;
     BpConfig best{c->bp_variant, c->bp_ppt, c->bp_layout, c->bp_depth};
     float best_t = 1e30f;
     for (const BpConfig& k : kBpCandidates) {

@@ -1787,11 +1787,14 @@ static void launch_k4(Ctx* c, const ST* ring, int64_t vs, const float* recs, con
                       const Tile* d_tiles, int nt, int n_views, float scale, float* d_out,
                       hipStream_t s) {
     const int nb = Map4<PPT, LAY>::SUBS * nt;
-    // cone beam, a grid that fills the device: one launch per block of bp_vchunk views, so
-    // that all tiles read the same few views at a time (the launches then share detector
-    // lines in the L2: -30% L2-miss traffic at 32 views) at the cost of a sum round trip
-    // through `out` per launch
-    const bool split = CONE && c->bp_vchunk > 0 && (c->vchunk_any || nb >= 8 * c->n_cus);
+    // cone beam, slice requests whose grid fills the device: one launch per block of
+    // bp_vchunk views, so that all tiles read the same few views at a time and share
+    // detector lines in the L2 (-22% L2-miss traffic at 64 views) at the cost of a round
+    // trip of the sums through `out` per launch.  Smaller grids lose more to the launches
+    // than they gain; volume requests (stacks of planes, whose tiles share lines plane to
+    // plane already) would only pay the round trips.
+    const bool split = CONE && c->bp_vchunk > 0 &&
+                       (c->vchunk_any || (c->req_slices && nb >= 8 * c->n_cus));
     const int k = split ? c->bp_vchunk : std::max(n_views, 1);
     int vb = 0;
     do {
