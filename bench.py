@@ -139,6 +139,18 @@ def pmc_traffic():
     return res
 
 
+def cpu_model():
+    """The host CPU's model string (/proc/cpuinfo), or "unknown CPU"."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
 def _line(metric, value, unit, ms, steps, warmup, cfg, extra, higher=True, dtype="f32"):
     d = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
          "warmup": warmup, "ms_per_step": ms, "higher_is_better": higher, "scaling": "weak",
@@ -615,7 +627,8 @@ def main():
         cpu = dict(value=round(gups, 4), unit="GUPS", cores=threads, kind="port",
                    sample=f"the full 3-slice update: all {len(all_pts)} in-box points x all {A} "
                           f"views, median of 3 runs ({dt * 1e3:.0f} ms each; "
-                          f"{sum(times):.1f} s of CPU work)")
+                          f"{sum(times):.1f} s of CPU work); Numba-equivalent C/OpenMP path "
+                          f"(numba unavailable) on {cpu_model()}")
         log(f"cpu baseline: {cpu}")
 
     h = results[head]
