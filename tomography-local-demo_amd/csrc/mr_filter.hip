@@ -8,92 +8,156 @@
 //   ifft(Z H) = conj(fft(conj(Z) H)) / N,
 // whose real part is row r filtered and imaginary part row r+1 filtered.
 //
-// The FFT is an in-place LDS Stockham transform with radix-4 passes (one radix-2
-// pass first when log2 N is odd).  Each pass reads its butterflies' inputs into
-// registers, synchronises, and writes the outputs, so one N-point buffer suffices.
-// Twiddles exp(-2 pi i k / N) come from a table computed in float64 on the host.
+// The FFT is a Stockham transform in radix-16 passes (one radix-2/4/8 pass first when
+// log2 N is not a multiple of 4): each thread holds the 16 inputs of a butterfly in
+// registers, twiddles them, runs a 16-point DFT in registers and writes the outputs, so
+// N = 4096 takes 3 passes and 5 barriers for both transforms instead of 12 radix-4 passes.
+// The first pass reads the weighted, zero-padded rows straight from global memory, the
+// multiplication by H is fused into the first pass of the second transform, and its last
+// pass writes the filtered rows out (in natural order, coalesced).  Between passes the
+// N-point buffer lives in LDS with the low 4 index bits XORed with the next 4, so that
+// the stride-16 writes of the first pass do not conflict in the banks.  Twiddles
+// exp(-2 pi i k / N) come from a table computed in float64 on the host.
 #include "mr_internal.h"
 
 namespace mr {
 
 namespace {
 
-constexpr int FT = 256;   // threads per workgroup
+// Threads per workgroup: 256 (one radix-16 butterfly per thread per pass at N = 4096),
+// 512 for N = 8192.
+constexpr int ft_of(int logn) { return logn >= 13 ? 512 : 256; }
 
 __device__ inline float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-template <int N, int R>
-__device__ void stockham_pass(float2* __restrict__ buf, const float2* __restrict__ tw, int Ns) {
-    constexpr int NB = N / R;
-    constexpr int PER = (NB + FT - 1) / FT;
-    float2 v[PER][R];
-    const int tstep_num = N / R;   // twiddle index = r * k * N / (Ns * R)
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+// a * exp(-2 pi i e / 16), e = 0..7 (a constant once the DFT loops are unrolled):
+// (x + iy)(cos t - i sin t) = (x cos t + y sin t) + i (y cos t - x sin t)
+__device__ __forceinline__ float2 w16(float2 a, int e) {
+    constexpr float C1 = 0.92387953251128674f;   // cos(pi/8)
+    constexpr float S1 = 0.38268343236508977f;   // sin(pi/8)
+    constexpr float H2 = 0.70710678118654752f;   // cos(pi/4)
+    switch (e) {
+    case 0: return a;
+    case 1: return make_float2(a.x * C1 + a.y * S1, a.y * C1 - a.x * S1);
+    case 2: return make_float2((a.x + a.y) * H2, (a.y - a.x) * H2);
+    case 3: return make_float2(a.x * S1 + a.y * C1, a.y * S1 - a.x * C1);
+    case 4: return make_float2(a.y, -a.x);
+    case 5: return make_float2(a.y * C1 - a.x * S1, -(a.y * S1 + a.x * C1));
+    case 6: return make_float2((a.y - a.x) * H2, -(a.x + a.y) * H2);
+    default: return make_float2(a.y * S1 - a.x * C1, -(a.y * C1 + a.x * S1));
+    }
+}
+
+// In-register DFT of R = 2..16 points, decimation in frequency: x[bitrev(q)] = X[q].
+template <int R>
+__device__ __forceinline__ void dft_dif(float2 (&x)[R]) {
 #pragma unroll
-    for (int e = 0; e < PER; ++e) {
-        const int j = threadIdx.x + e * FT;
-        if (j < NB) {
-            const int k = j & (Ns - 1);
-            const int tstep = tstep_num / Ns;
+    for (int s = R / 2; s >= 1; s >>= 1) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[e][r] = buf[j + r * NB];
+        for (int b = 0; b < R; b += 2 * s) {
 #pragma unroll
-            for (int r = 1; r < R; ++r) v[e][r] = cmul(v[e][r], tw[r * k * tstep]);
-            if constexpr (R == 2) {
-                const float2 a = v[e][0], b = v[e][1];
-                v[e][0] = make_float2(a.x + b.x, a.y + b.y);
-                v[e][1] = make_float2(a.x - b.x, a.y - b.y);
-            } else {
-                const float2 a0 = v[e][0], a1 = v[e][1], a2 = v[e][2], a3 = v[e][3];
-                const float2 t0 = make_float2(a0.x + a2.x, a0.y + a2.y);
-                const float2 t1 = make_float2(a0.x - a2.x, a0.y - a2.y);
-                const float2 t2 = make_float2(a1.x + a3.x, a1.y + a3.y);
-                const float2 d = make_float2(a1.x - a3.x, a1.y - a3.y);
-                const float2 t3 = make_float2(d.y, -d.x);   // (a1 - a3) * (-i)
-                v[e][0] = make_float2(t0.x + t2.x, t0.y + t2.y);
-                v[e][1] = make_float2(t1.x + t3.x, t1.y + t3.y);
-                v[e][2] = make_float2(t0.x - t2.x, t0.y - t2.y);
-                v[e][3] = make_float2(t1.x - t3.x, t1.y - t3.y);
+            for (int i = 0; i < s; ++i) {
+                const float2 a = x[b + i], c = x[b + i + s];
+                x[b + i] = cadd(a, c);
+                x[b + i + s] = w16(csub(a, c), i * (8 / s));   // W_{2s}^i = W_16^{8i/s}
             }
         }
     }
-    __syncthreads();
+}
+
+constexpr int bitrev(int q, int R) {
+    int r = 0;
+    for (int b = 1; b < R; b <<= 1) {
+        r = (r << 1) | (q & 1);
+        q >>= 1;
+    }
+    return r;
+}
+
+// LDS index of buffer element i: the low 4 bits XORed with the next 4 (a bijection).
+__device__ __forceinline__ int sw(int i) { return i ^ ((i >> 4) & 15); }
+
+// Passes of an N = 2^LOGN transform: radix 2^(LOGN mod 4) first (if not 1), then 16s.
+template <int LOGN> struct FftPlan {
+    static constexpr int REM = LOGN % 4;
+    static constexpr int NPASS = LOGN / 4 + (REM ? 1 : 0);
+    static constexpr int radix(int p) { return (REM && p == 0) ? (1 << REM) : 16; }
+    static constexpr int span(int p) {          // product of the radices before pass p
+        int n = 1;
+        for (int i = 0; i < p; ++i) n *= radix(i);
+        return n;
+    }
+};
+
+// One Stockham pass of radix R over span NS (the product of the earlier radices):
+// butterfly j (k = j mod NS) takes in(j + r N/R), r < R, twiddles input r by
+// exp(-2 pi i r k / (NS R)), and writes DFT output q to out((j - k) R + k + q NS).
+// SYNC: a barrier between reading and writing (in-place passes through LDS).
+template <int N, int R, int NS, bool SYNC, int FT, typename In, typename Out>
+__device__ __forceinline__ void radix_pass(const float2* __restrict__ tw, In in, Out out) {
+    constexpr int NB = N / R;
+    constexpr int PER = (NB + FT - 1) / FT;
+    float2 v[PER][R];
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
         const int j = threadIdx.x + e * FT;
-        if (j < NB) {
-            const int k = j & (Ns - 1);
-            const int dst = (j - k) * R + k;   // (j / Ns) * Ns * R + j % Ns
+        if (NB % FT == 0 || j < NB) {
+            const int k = j & (NS - 1);
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[dst + r * Ns] = v[e][r];
+            for (int r = 0; r < R; ++r) v[e][r] = in(j + r * NB);
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[e][r] = cmul(v[e][r], tw[r * k * (N / (NS * R))]);
+            }
+            dft_dif<R>(v[e]);
         }
     }
-    __syncthreads();
+    if constexpr (SYNC) __syncthreads();
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int j = threadIdx.x + e * FT;
+        if (NB % FT == 0 || j < NB) {
+            const int k = j & (NS - 1);
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) out(base + q * NS, v[e][bitrev(q, R)]);
+        }
+    }
 }
 
-template <int LOGN>
-__device__ void fft_forward(float2* __restrict__ buf, const float2* __restrict__ tw) {
-    constexpr int N = 1 << LOGN;
-    int Ns = 1;
-    if constexpr ((LOGN & 1) != 0) {
-        stockham_pass<N, 2>(buf, tw, Ns);
-        Ns = 2;
-    }
-    if constexpr (N >= 4) {
-#pragma unroll 1
-        for (; Ns < N; Ns *= 4) stockham_pass<N, 4>(buf, tw, Ns);
+// The transform from pass P on: pass P reads `in` (IN_LDS: from the LDS buffer), the
+// passes in between go through LDS, the last pass writes `out` (OUT_LDS: into the LDS
+// buffer, followed by a barrier).
+template <int LOGN, int P, bool IN_LDS, bool OUT_LDS, typename In, typename Out>
+__device__ __forceinline__ void fft_from(const float2* __restrict__ tw, float2* buf, In in, Out out) {
+    using PL = FftPlan<LOGN>;
+    constexpr int N = 1 << LOGN, R = PL::radix(P), NS = PL::span(P), FT = ft_of(LOGN);
+    if constexpr (P + 1 == PL::NPASS) {
+        radix_pass<N, R, NS, IN_LDS && OUT_LDS, FT>(tw, in, out);
+        if constexpr (OUT_LDS) __syncthreads();
+    } else {
+        radix_pass<N, R, NS, IN_LDS, FT>(tw, in, [buf](int i, float2 z) { buf[sw(i)] = z; });
+        __syncthreads();
+        fft_from<LOGN, P + 1, true, OUT_LDS>(tw, buf, [buf](int i) { return buf[sw(i)]; }, out);
     }
 }
 
 // MODE 0: ring float32; 1: ring bfloat16; 2: float32 staging [k][R][C] plus per-view
 // max |q| (for F16 quantisation); 3: plain float32 rows [R][C] (filter_rows hook).
+// At least 3 waves per SIMD at N = 4096 (the radix-16 passes fit 168 VGPRs without
+// spilling; 4 waves would spill), 4 below.
 template <int LOGN, int MODE>
-__global__ __launch_bounds__(FT) void filter_kernel(
+__global__ __launch_bounds__(ft_of(LOGN))
+__attribute__((amdgpu_waves_per_eu(LOGN >= 13 ? 1 : LOGN == 12 ? 3 : 4))) void filter_kernel(
     const float* __restrict__ block, int R, int k, int C, const FilterView* __restrict__ fv,
     const float* __restrict__ H, const float2* __restrict__ tw, void* __restrict__ dst, int pitch,
     int64_t view_stride, unsigned* __restrict__ vmax) {
-    constexpr int N = 1 << LOGN;
+    constexpr int N = 1 << LOGN, FT = ft_of(LOGN);
     __shared__ float2 buf[N];
     const int r0 = 2 * blockIdx.x, r1 = r0 + 1;
     const int v = blockIdx.y;
@@ -105,7 +169,8 @@ __global__ __launch_bounds__(FT) void filter_kernel(
     const float* x1 = MODE == 4 ? block + (static_cast<int64_t>(v) * R + (has1 ? r1 : r0)) * C
                                 : block + (static_cast<int64_t>(has1 ? r1 : r0) * k + v) * C;
     const float b0 = P.b0 + r0 * P.db, b1 = P.b0 + r1 * P.db;
-    for (int c = threadIdx.x; c < N; c += FT) {
+    // transform 1 reads the (cosine-weighted, zero-padded) row pair from global memory
+    auto in_rows = [&](int c) -> float2 {
         float y0 = 0.0f, y1 = 0.0f;
         if (c < C) {
             y0 = x0[c];
@@ -117,22 +182,19 @@ __global__ __launch_bounds__(FT) void filter_kernel(
                 y1 *= P.sdd * __frsqrt_rn(fmaf(b1, b1, aa));
             }
         }
-        buf[c] = make_float2(y0, y1);
-    }
-    __syncthreads();
-    fft_forward<LOGN>(buf, tw);
-    for (int c = threadIdx.x; c < N; c += FT) {
+        return make_float2(y0, y1);
+    };
+    fft_from<LOGN, 0, false, true>(tw, buf, in_rows, [&](int i, float2 z) { buf[sw(i)] = z; });
+    // transform 2 starts from conj(Z) H and writes the filtered rows
+    auto in_h = [&](int c) -> float2 {
+        const float2 z = buf[sw(c)];
         const float h = H[c];
-        const float2 z = buf[c];
-        buf[c] = make_float2(z.x * h, -z.y * h);
-    }
-    __syncthreads();
-    fft_forward<LOGN>(buf, tw);
-
+        return make_float2(z.x * h, -z.y * h);
+    };
     const float g = P.gain;
     float lmax = 0.0f;
-    for (int c = threadIdx.x; c < C; c += FT) {
-        const float2 z = buf[c];
+    auto out_rows = [&](int c, float2 z) {
+        if (c >= C) return;
         const float y0 = z.x * g, y1 = -z.y * g;
         if (MODE == 0) {
             float* q = static_cast<float*>(dst) + static_cast<int64_t>(P.slot) * view_stride;
@@ -172,7 +234,8 @@ __global__ __launch_bounds__(FT) void filter_kernel(
             q[(static_cast<int64_t>(r0) * k + v) * C + c] = y0 * w0;
             if (has1) q[(static_cast<int64_t>(r1) * k + v) * C + c] = y1 * w1;
         }
-    }
+    };
+    fft_from<LOGN, 0, true, false>(tw, buf, in_h, out_rows);
     if (MODE == 2) {
         // workgroup max through LDS, then one atomic per workgroup
         __shared__ float red[FT];
@@ -271,7 +334,7 @@ int launch_mode(int logn, dim3 grid, hipStream_t s, const float* block, int R, i
                 int64_t view_stride, unsigned* vmax) {
 #define MR_FILTER_CASE(L)                                                                  \
     case L:                                                                                \
-        hipLaunchKernelGGL((filter_kernel<L, MODE>), grid, dim3(FT), 0, s, block, R, k, C, \
+        hipLaunchKernelGGL((filter_kernel<L, MODE>), grid, dim3(ft_of(L)), 0, s, block, R, k, C, \
                            fv, H, tw, dst, pitch, view_stride, vmax);                      \
         break;
     switch (logn) {
