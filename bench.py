@@ -182,7 +182,7 @@ def bench_full_volume(args):
     nvox = int(np.prod(dims))
     T = recon.volume_task(origin, h, dims, 0, int(dims[2]))[None]
     dvol = ctx.alloc(nvox * 4)
-    ctx.bp_tasks_device(T, A, dvol, nvox)                  # warm-up (row-pair ring, records)
+    ctx.bp_tasks_device(T, A, dvol, nvox)                  # warm-up (band ring, records)
     ctx.synchronize()
     times, tile_ms = [], []
     for _ in range(max(1, args.steps)):

@@ -92,7 +92,8 @@ def test_plan_stream_graph_replays(workload, storage, monkeypatch):
 @pytest.mark.parametrize("variant,ppt,layout,depth", [
     ("2", "4", "0", "2"), ("4", "4", "0", "2"), ("4", "2", "0", "2"), ("4", "2", "1", "3"),
     ("4", "1", "1", "2"), ("4", "1", "0", "3"), ("4", "4", "1", "3"), ("5", "4", "0", "2"),
-    ("5", "2", "0", "2"), ("6", "2", "0", "2"), ("6", "1", "1", "3"), ("6", "4", "0", "2")])
+    ("5", "2", "0", "2"), ("6", "2", "0", "2"), ("6", "1", "1", "3"), ("6", "4", "0", "2"),
+    ("7", "2", "0", "2"), ("7", "1", "1", "3"), ("7", "4", "0", "2")])
 def test_bp_variants_full_check(workload, variant, ppt, layout, depth, monkeypatch):
     """Every tile-kernel configuration (kernel, points per thread, lane layout, views in
     flight) against the plain v1 kernel on all outputs, for view counts that leave
@@ -145,16 +146,16 @@ def test_autotune_picks_a_configuration_and_keeps_parity(workload):
     rec = recon.SliceReconstructor(proj, w.geom, window=w.window, verbose=False)
     cfg, ms = rec.autotune(planes, max_tiles=64, reps=1)
     assert ms > 0 and cfg == rec._ctx.bp_config()
-    assert cfg["kernel"] in (2, 4, 5, 6) and cfg["ppt"] in (1, 2, 4)
+    assert cfg["kernel"] in (2, 4, 5, 6, 7) and cfg["ppt"] in (1, 2, 4)
     got = rec._ctx.bp_tasks(T, A, out_len)
     assert rel_l2(got, ref) < 1e-6, (cfg, rel_l2(got, ref))
     with pytest.raises(ValueError):
-        rec._ctx.set_bp_config(kernel=7)
+        rec._ctx.set_bp_config(kernel=8)
     with pytest.raises(ValueError):
         rec._ctx.set_bp_config(kernel=4, ppt=3)
 
 
-@pytest.mark.parametrize("variant,vchunk", [("6", "1"), ("6", "7"), ("4", "64"), ("4", "5")])
+@pytest.mark.parametrize("variant,vchunk", [("6", "1"), ("6", "7"), ("4", "64"), ("4", "5"), ("7", "3")])
 def test_view_chunked_launches_are_exact(workload, variant, vchunk, monkeypatch):
     """Cone-beam updates split into launches of a few views each (the FP32 sums carried
     through the output between launches) equal the single launch bit for bit.  The
@@ -178,15 +179,17 @@ def test_view_chunked_launches_are_exact(workload, variant, vchunk, monkeypatch)
             np.testing.assert_array_equal(split.bp_tasks(T, nv, out_len), one.bp_tasks(T, nv, out_len))
 
 
-def test_pair_ring_follows_appends_and_resets(workload, monkeypatch):
-    """Kernel 6 reads a row-pair copy of the ring built on first use: it must take in views
-    appended after that use, and rebuild after the views are reset and replaced."""
+@pytest.mark.parametrize("variant", ["6", "7"])
+def test_pair_ring_follows_appends_and_resets(workload, variant, monkeypatch):
+    """Kernels 6 and 7 read a row-pair or band copy of the ring built on first use: it must
+    take in views appended after that use, and rebuild after the views are reset and
+    replaced."""
     from minirecast import _hip
     w, proj, planes, tasks, out_len = workload
     T = np.array(tasks, dtype=_hip.TASK_DTYPE)
     A = w.geom.n_angles
     ref_ctx = _ctx(w, "f16", "1", monkeypatch)
-    ctx = _ctx(w, "f16", "6", monkeypatch)
+    ctx = _ctx(w, "f16", variant, monkeypatch)
     h = A // 2
     for c in (ref_ctx, ctx):
         c.push(np.arange(h), proj[:, :h])

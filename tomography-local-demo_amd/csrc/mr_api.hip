@@ -88,6 +88,10 @@ void free_geometry(Ctx* c) {
     c->pring = nullptr;
     c->pring_bytes = 0;
     c->pring_n = 0;
+    if (c->bring) hipFree(c->bring);
+    c->bring = nullptr;
+    c->bring_bytes = 0;
+    c->bring_n = 0;
     // the raw-count correction is per detector frame: a new geometry drops it
     if (c->raw.d_dark) hipFree(c->raw.d_dark);
     if (c->raw.d_scale) hipFree(c->raw.d_scale);
@@ -123,11 +127,11 @@ int alloc_ring(Ctx* c, int n_total, int rows, int cols, int window, int storage)
     c->storage = storage;
     c->N = fft_length(cols);
     if (!c->bp_explicit) {
-        // measured defaults: 16-bit forms read the row-pair ring (kernel 6, 4 points per
+        // measured defaults: 16-bit forms read the band ring (kernel 7, 2 points per
         // thread); f32 reads the ring directly (kernel 4, 2 points per thread)
         const bool f32 = storage == MR_STORE_F32;
-        c->bp_variant = f32 ? 4 : 6;
-        c->bp_ppt = f32 ? 2 : 4;
+        c->bp_variant = f32 ? 4 : 7;
+        c->bp_ppt = 2;
         c->bp_layout = 0;
         c->bp_depth = 2;
         c->bp_sync = f32 ? 0 : 1;
@@ -406,7 +410,7 @@ int mr_ctx_create(int device, mr_ctx** out) {
                      std::getenv("MINIRECAST_BP_SYNC");
     if (const char* v = std::getenv("MINIRECAST_BP_VARIANT")) {
         const int k = std::atoi(v);
-        c->bp_variant = (k >= 1 && k <= 6) ? k : c->bp_variant;
+        c->bp_variant = (k >= 1 && k <= 7) ? k : c->bp_variant;
     }
     if (const char* v = std::getenv("MINIRECAST_BP_PPT")) {
         const int k = std::atoi(v);
@@ -825,10 +829,11 @@ struct BpConfig { int variant, ppt, layout, depth; };
 const BpConfig kBpCandidates[] = {
     {4, 2, 0, 2}, {4, 2, 0, 3}, {4, 2, 1, 2}, {4, 2, 1, 3}, {4, 1, 0, 2}, {4, 1, 0, 3},
     {4, 1, 1, 2}, {4, 1, 1, 3}, {4, 4, 0, 2}, {4, 4, 1, 2}, {5, 2, 0, 2}, {5, 4, 0, 2},
-    {2, 4, 0, 2}, {6, 2, 0, 2}, {6, 2, 0, 3}, {6, 1, 0, 3}, {6, 4, 0, 2}};
+    {2, 4, 0, 2}, {6, 2, 0, 2}, {6, 2, 0, 3}, {6, 1, 0, 3}, {6, 4, 0, 2}, {7, 2, 0, 2},
+    {7, 1, 0, 2}, {7, 2, 0, 3}};
 
 bool valid_config(int variant, int ppt, int layout, int depth) {
-    return (variant >= 1 && variant <= 6) && (ppt == 1 || ppt == 2 || ppt == 4) &&
+    return (variant >= 1 && variant <= 7) && (ppt == 1 || ppt == 2 || ppt == 4) &&
            (layout == 0 || layout == 1) && (depth == 2 || depth == 3);
 }
 }  // namespace
@@ -1410,6 +1415,7 @@ int mr_reset_views(mr_ctx* c) {
     std::lock_guard<std::mutex> lk(c->mu);
     c->n = 0;
     c->pring_n = 0;
+    c->bring_n = 0;
     return MR_OK;
 }
 

@@ -72,7 +72,7 @@ struct RecRef {
 
 __device__ void make_record(const ViewGeom* __restrict__ gp, float vs, const mr_task& T,
                             const double* xt, int R, int C, int pitch, bool cone,
-                            float* __restrict__ rec, RecRef* ref = nullptr) {
+                            float* __restrict__ rec, RecRef* ref = nullptr, bool band = false) {
     const ViewGeom g = *gp;
     const double sh = T.cor_shift;
     const double W0 = dot3(g.P[2], xt) + g.P[2][3];
@@ -104,7 +104,11 @@ __device__ void make_record(const ViewGeom* __restrict__ gp, float vs, const mr_
                         static_cast<float>(Ub * f), static_cast<float>(V0 * f));
     r4[1] = make_float4(static_cast<float>(Va * f), static_cast<float>(Vb * f),
                         static_cast<float>(W0 * f), static_cast<float>(Wa * f));
-    const int base = (static_cast<int>(rref) + 1) * pitch + (static_cast<int>(cref) + 1);
+    // [14]: element offset of the padded reference position; for the band ring instead
+    // the padded reference row (low 16 bits) and column (high 16 bits), both signed
+    const int base = band ? static_cast<int>((static_cast<uint32_t>(static_cast<int>(rref) + 1) & 0xffffu) |
+                                             (static_cast<uint32_t>(static_cast<int>(cref) + 1) << 16))
+                          : (static_cast<int>(rref) + 1) * pitch + (static_cast<int>(cref) + 1);
     r4[2] = make_float4(static_cast<float>(Wb * f), static_cast<float>(-1.0 - cref),
                         static_cast<float>(C - cref), static_cast<float>(-1.0 - rref));
     r4[3] = make_float4(static_cast<float>(R - rref), vs, __int_as_float(base),
@@ -702,7 +706,7 @@ __global__ __launch_bounds__(256) void records_kernel(const ViewGeom* __restrict
                                                       const mr_task* __restrict__ tasks,
                                                       const Tile* __restrict__ tiles, int ntiles,
                                                       int n_views, int R, int C, int pitch,
-                                                      int cone, float* __restrict__ recs) {
+                                                      int cone, float* __restrict__ recs, int band) {
     const int64_t gid = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
     if (gid >= static_cast<int64_t>(ntiles) * n_views) return;
     const int ti = static_cast<int>(gid / n_views);
@@ -713,7 +717,8 @@ __global__ __launch_bounds__(256) void records_kernel(const ViewGeom* __restrict
     double xt[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) xt[d] = T.o[d] + tl.q * T.c[d] + p0 * T.a[d] + s0 * T.b[d];
-    make_record(geom + v, vscale[v], T, xt, R, C, pitch, cone != 0, recs + gid * BP_REC);
+    make_record(geom + v, vscale[v], T, xt, R, C, pitch, cone != 0, recs + gid * BP_REC, nullptr,
+                band != 0);
 }
 
 // Taps of one point as loaded by v4: f32 keeps four floats; the 16-bit formats keep
@@ -807,10 +812,35 @@ template <> struct PairP<uint16_t> {
     }
 };
 
-template <typename ST, int PPT, bool PAIRED = false>
+// Taps from the band ring (16-bit forms): an element of band k holds the padded rows
+// 3k..3k+3 of one column (8 bytes), so the 2x2 stencil of every point lies in one band
+// and one buffer_load_dwordx4 fetches columns c and c+1; the two rows the point needs,
+// 3k+j and 3k+j+1, are picked at the lerp by a 16j-bit funnel shift.
+template <typename ST> struct BandP {
+    uint32_t d0, d1, d2, d3;           // (rows 0,1 | rows 2,3) of column c, then of c+1
+    uint32_t sh;                       // 16 j
+    __device__ inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int shift) {
+        const uint4 x = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, 0, 0));
+        d0 = x.x;
+        d1 = x.y;
+        d2 = x.z;
+        d3 = x.w;
+        sh = static_cast<uint32_t>(shift);
+    }
+    __device__ inline void lerp(float fc, float& top, float& bot) const {
+        PairP<ST> p;
+        p.w0 = static_cast<uint32_t>(((static_cast<uint64_t>(d1) << 32) | d0) >> sh);
+        p.w1 = static_cast<uint32_t>(((static_cast<uint64_t>(d3) << 32) | d2) >> sh);
+        p.lerp(fc, top, bot);
+    }
+};
+
+// RING: 0 the ring itself, 1 the row-pair ring, 2 the band ring (16-bit forms).
+template <typename ST, int PPT, int RING = 0>
 struct PointState {
     float fc[PPT], fr[PPT], wt[PPT];
-    typename std::conditional<PAIRED, PairP<ST>, Pair<ST>>::type tap[PPT];
+    typename std::conditional<RING == 2, BandP<ST>,
+                              typename std::conditional<RING == 1, PairP<ST>, Pair<ST>>::type>::type tap[PPT];
 };
 
 // Row-pair ring from the ring: pair-row p (padded rows p and p+1, p = 0..R+1) of view v,
@@ -828,6 +858,27 @@ __global__ __launch_bounds__(256) void pair_convert_kernel(const ST* __restrict_
     ST* dst = pring + v * pview_stride + (static_cast<int64_t>(pr) * pitch + col) * 2;
     dst[0] = src[0];
     dst[1] = src[pitch];
+}
+
+// Band ring from the ring (16-bit forms): element (band b, col) of view v = padded rows
+// 3b..3b+3 of that column (rows past the ring's Rp rows read as zero), 8 bytes.
+__global__ __launch_bounds__(256) void band_convert_kernel(const uint16_t* __restrict__ ring,
+                                                           uint16_t* __restrict__ bring,
+                                                           int64_t view_stride, int64_t bview_stride,
+                                                           int pitch, int Rp, int v0) {
+    const int col = blockIdx.x * 256 + threadIdx.x;
+    const int b = blockIdx.y;
+    const int64_t v = v0 + static_cast<int64_t>(blockIdx.z);
+    if (col >= pitch) return;
+    const uint16_t* src = ring + v * view_stride + col;
+    uint32_t q[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int r = 3 * b + i;
+        q[i] = r < Rp ? src[static_cast<int64_t>(r) * pitch] : 0u;
+    }
+    uint2* dst = reinterpret_cast<uint2*>(bring + v * bview_stride + (static_cast<int64_t>(b) * pitch + col) * 4);
+    *dst = make_uint2(q[0] | (q[1] << 16), q[2] | (q[3] << 16));
 }
 
 // Lane layouts of v4: which in-tile point (dp along a, ds along b) thread t handles
@@ -861,8 +912,9 @@ template <int PPT, int LAY> struct Map4 {
 // Cone beam, views [vb, ve) of n_views: a launch with vb > 0 resumes from the unscaled
 // FP32 sums the previous launch left in `out`, and one with ve < n_views leaves them
 // there, so an update split into several launches equals the single launch bit for bit.
-template <bool CONE, typename ST, int PPT, int LAY, int DEPTH, bool PAIRED, bool SYNC = false>
-__global__ __launch_bounds__(BP_THREADS, (sizeof(ST) == 2 && DEPTH == 2) ? 8 : 1) void bp_tiles_kernel4(
+template <bool CONE, typename ST, int PPT, int LAY, int DEPTH, int RING, bool SYNC = false>
+__global__ __launch_bounds__(BP_THREADS, (sizeof(ST) == 2 && DEPTH == 2 && (RING < 2 || PPT < 4)) ? 8 : 1)
+void bp_tiles_kernel4(
     const ST* __restrict__ ring, int64_t view_stride, int pitch,
     const float* __restrict__ recs, const mr_task* __restrict__ tasks,
     const Tile* __restrict__ tiles, int n_views, float scale, float* __restrict__ out, int xcd,
@@ -888,16 +940,19 @@ __global__ __launch_bounds__(BP_THREADS, (sizeof(ST) == 2 && DEPTH == 2) ? 8 : 1
         inside[m] = valid[m] && (!T.mask_box || in_box(T, p, s));
     }
     const float ds = static_cast<float>(st0), dp0 = static_cast<float>(pt);
-    // PAIRED: `ring` is the row-pair ring (two elements per padded position)
+    // RING 1: `ring` is the row-pair ring (two elements per padded position); RING 2: the
+    // band ring (four elements per band and column)
+    static_assert(RING < 2 || sizeof(ST) == 2, "band ring of 16-bit storage only");
+    constexpr bool PAIRED = RING == 1;
     constexpr int SH = LE + (PAIRED ? 1 : 0);
-    const int pitchb = pitch << SH;
+    const int pitchb = RING == 2 ? pitch << 3 : pitch << SH;
     const int view_bytes = static_cast<int>(view_stride << LE);
     const float* __restrict__ rt = recs + static_cast<int64_t>(tile) * n_views * BP_REC;
 
     float acc[PPT], accb[PPT], frp[PPT];
-    int roffp[PPT];
+    int roffp[PPT], jsh[PPT];
 #pragma unroll
-    for (int m = 0; m < PPT; ++m) { acc[m] = 0.0f; accb[m] = 0.0f; frp[m] = 0.0f; roffp[m] = 0; }
+    for (int m = 0; m < PPT; ++m) { acc[m] = 0.0f; accb[m] = 0.0f; frp[m] = 0.0f; roffp[m] = 0; jsh[m] = 0; }
     const int64_t obase = T.out_offset + static_cast<int64_t>(tl.q) * T.stride_q;
     auto oidx = [&](int m) {
         return obase + static_cast<int64_t>(p0 + pt + M::dp(m)) * T.stride_p +
@@ -917,7 +972,15 @@ __global__ __launch_bounds__(BP_THREADS, (sizeof(ST) == 2 && DEPTH == 2) ? 8 : 1
             float row = fmaf(dp, rt[5], fmaf(dsm, rt[4], rt[3]));
             row = __builtin_amdgcn_fmed3f(row, rt[11], rt[12]);
             frp[m] = __builtin_amdgcn_fractf(row);
-            roffp[m] = cvt_flr(row) * pitchb;
+            if (RING == 2) {
+                // band offset and 16 j of the padded row (the same for every view)
+                const int rp = cvt_flr(row) + static_cast<int>(static_cast<int16_t>(__float_as_int(rt[14]) & 0xffff));
+                const uint32_t pr = __umul24(static_cast<uint32_t>(rp), 43691u);
+                roffp[m] = __mul24(static_cast<int>(pr >> 17), pitchb);
+                jsh[m] = static_cast<int>((pr >> 11) & 0x30u);
+            } else {
+                roffp[m] = cvt_flr(row) * pitchb;
+            }
         }
     }
 
@@ -929,10 +992,13 @@ __global__ __launch_bounds__(BP_THREADS, (sizeof(ST) == 2 && DEPTH == 2) ? 8 : 1
         return c;
     };
     // request the taps of view v into st
-    auto issue = [&](int v, PointState<ST, PPT, PAIRED>& st) {
+    auto issue = [&](int v, PointState<ST, PPT, RING>& st) {
         const float* r = rt + static_cast<int64_t>(v) * BP_REC;
         const float U0 = r[0], Ua = r[1], Ub = r[2];
-        const int baseb = __float_as_int(r[14]) << SH;
+        // band ring: [14] packs the padded reference row and column (signed 16-bit halves)
+        const int ref14 = __float_as_int(r[14]);
+        const int rpref = static_cast<int>(static_cast<int16_t>(ref14 & 0xffff));
+        const int baseb = RING == 2 ? (ref14 >> 16) << 3 : ref14 << SH;
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<ST*>(ring + static_cast<int64_t>(v) * view_stride), 0, view_bytes, 0x00020000);
         const float uh0 = fmaf(dp0, Ub, fmaf(ds, Ua, U0));
@@ -951,8 +1017,17 @@ __global__ __launch_bounds__(BP_THREADS, (sizeof(ST) == 2 && DEPTH == 2) ? 8 : 1
                 st.wt[m] = rw * rw;
                 st.fc[m] = __builtin_amdgcn_fractf(col);
                 st.fr[m] = __builtin_amdgcn_fractf(row);
-                const int voff = __mul24(cvt_flr(row), pitchb) + ((cvt_flr(col) << SH) + baseb);
-                st.tap[m].load(rs, voff, pitchb);
+                if (RING == 2) {
+                    // padded row rp = 3 b + j (rp < 98304, so rp / 3 is the 17-bit shift)
+                    // padded row rp = 3 b + j: with p = rp * 43691, b = p >> 17 and
+                    // j = (p >> 15) & 3, exact for rp < 32768
+                    const uint32_t pr = __umul24(static_cast<uint32_t>(cvt_flr(row) + rpref), 43691u);
+                    const int voff = __mul24(static_cast<int>(pr >> 17), pitchb) + ((cvt_flr(col) << 3) + baseb);
+                    st.tap[m].load(rs, voff, static_cast<int>((pr >> 11) & 0x30u));
+                } else {
+                    const int voff = __mul24(cvt_flr(row), pitchb) + ((cvt_flr(col) << SH) + baseb);
+                    st.tap[m].load(rs, voff, pitchb);
+                }
             }
         } else {
             st.wt[0] = r[13];
@@ -960,12 +1035,16 @@ __global__ __launch_bounds__(BP_THREADS, (sizeof(ST) == 2 && DEPTH == 2) ? 8 : 1
             for (int m = 0; m < PPT; ++m) {
                 const float col = __builtin_amdgcn_fmed3f(step(uh0, m, Ub, Ua), r[9], r[10]);
                 st.fc[m] = __builtin_amdgcn_fractf(col);
-                const int voff = roffp[m] + ((cvt_flr(col) << SH) + baseb);
-                st.tap[m].load(rs, voff, pitchb);
+                if (RING == 2) {
+                    st.tap[m].load(rs, roffp[m] + ((cvt_flr(col) << 3) + baseb), jsh[m]);
+                } else {
+                    const int voff = roffp[m] + ((cvt_flr(col) << SH) + baseb);
+                    st.tap[m].load(rs, voff, pitchb);
+                }
             }
         }
     };
-    auto consume = [&](const PointState<ST, PPT, PAIRED>& st) {
+    auto consume = [&](const PointState<ST, PPT, RING>& st) {
 #pragma unroll
         for (int m = 0; m < PPT; ++m) {
             float top, bot;
@@ -983,7 +1062,7 @@ __global__ __launch_bounds__(BP_THREADS, (sizeof(ST) == 2 && DEPTH == 2) ? 8 : 1
     // is interpolated.  Requests past the last view re-read it (never consumed), so the
     // loop body is unconditional and every wait counts exactly the loads after it.
     if (ve > vb) {
-        PointState<ST, PPT, PAIRED> S[DEPTH];
+        PointState<ST, PPT, RING> S[DEPTH];
         const int last = ve - 1;
 #pragma unroll
         for (int j = 0; j + 1 < DEPTH; ++j) issue(min(vb + j, last), S[j]);
@@ -1639,13 +1718,28 @@ int dispatch_points(Ctx* c, const double* d_pts, int64_t M, double shift, int n_
 
 int launch_bp_tiles_v4(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles, int n_views,
                        float scale, float* d_out, hipStream_t s);
+int64_t band_view_stride(const Ctx* c);
+int ensure_band_ring(Ctx* c, int n_views, hipStream_t s);
 
 int launch_bp_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles, int n_views,
                     float scale, float* d_out, hipStream_t s, const TileQueue* q) {
     const TileQueue none{};
     if (!q && ntiles <= 0) return MR_OK;
-    if (!q && c->bp_variant >= 4 && n_views > 0)
+    if (!q && c->bp_variant >= 4 && n_views > 0) {
+        if (!c->bp_explicit && c->bp_variant == 7 && !c->req_slices) {
+            // default configuration, volume request (stacks of z-planes): a z-plane tile
+            // touches only a few detector rows per view, where the row-pair ring (32
+            // columns per 128-byte line) beats the band ring (16): kernel 6, 4 points
+            const int sv = c->bp_variant, sp = c->bp_ppt;
+            c->bp_variant = 6;
+            c->bp_ppt = 4;
+            const int rc = launch_bp_tiles_v4(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s);
+            c->bp_variant = sv;
+            c->bp_ppt = sp;
+            return rc;
+        }
         return launch_bp_tiles_v4(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s);
+    }
     if (q && c->bp_variant != 2 && c->bp_variant < 4)
         return fail(MR_ERUNTIME, "persistent launches need the v2 kernel");
     if (q) {                               // persistent launches always run the v2 loop
@@ -1782,7 +1876,7 @@ int launch_bp_points(Ctx* c, const double* d_pts, int64_t M, double cor_shift, i
                : dispatch_points<false>(c, d_pts, M, cor_shift, n_views, scale, d_out, s);
 }
 
-template <bool CONE, typename ST, int PPT, int LAY, int DEPTH, bool PAIRED, bool SYNC>
+template <bool CONE, typename ST, int PPT, int LAY, int DEPTH, int RING, bool SYNC>
 static void launch_k4(Ctx* c, const ST* ring, int64_t vs, const float* recs, const mr_task* d_tasks,
                       const Tile* d_tiles, int nt, int n_views, float scale, float* d_out,
                       hipStream_t s) {
@@ -1799,7 +1893,7 @@ static void launch_k4(Ctx* c, const ST* ring, int64_t vs, const float* recs, con
     int vb = 0;
     do {
         const int ve = std::min(n_views, vb + k);
-        hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, DEPTH, PAIRED, SYNC>), dim3(nb),
+        hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, DEPTH, RING, SYNC>), dim3(nb),
                            dim3(BP_THREADS), 0, s, ring, vs, c->pitch, recs, d_tasks, d_tiles,
                            n_views, scale, d_out, c->xcd_map != 0 ? 1 : 0, vb, ve);
         ++c->tile_launches;
@@ -1807,20 +1901,28 @@ static void launch_k4(Ctx* c, const ST* ring, int64_t vs, const float* recs, con
     } while (vb < n_views);
 }
 
-// Kernels 4 and 6: the storage-independent switches (row-pair ring, lockstep waves, views
-// in flight) as template arguments.
+// Kernels 4, 6 and 7: the storage-independent switches (ring layout, lockstep waves, views
+// in flight) as template arguments.  Kernel 7 reads the band ring for 16-bit storage and
+// the row-pair ring for f32.
 template <bool CONE, typename ST, int PPT, int LAY>
 static void launch_v4_one(Ctx* c, const float* recs, const mr_task* d_tasks, const Tile* d_tiles,
                           int nt, int n_views, float scale, float* d_out, hipStream_t s) {
-    const bool paired = c->bp_variant == 6;
-    const ST* ring = static_cast<const ST*>(paired ? c->pring : c->ring);
-    const int64_t vs = paired ? 2 * static_cast<int64_t>(c->R + 2) * c->pitch : c->view_stride;
+    constexpr bool B16 = sizeof(ST) == 2;
+    const int ring_kind = (B16 && c->bp_variant == 7) ? 2 : c->bp_variant >= 6 ? 1 : 0;
+    const ST* ring = static_cast<const ST*>(ring_kind == 2 ? c->bring : ring_kind == 1 ? c->pring : c->ring);
+    const int64_t vs = ring_kind == 2 ? band_view_stride(c)
+                     : ring_kind == 1 ? 2 * static_cast<int64_t>(c->R + 2) * c->pitch : c->view_stride;
     const bool d3 = c->bp_depth >= 3;
-#define MR_K4(D, P, S) launch_k4<CONE, ST, PPT, LAY, D, P, S>(c, ring, vs, recs, d_tasks, d_tiles, nt, n_views, scale, d_out, s)
-    if (paired && c->bp_sync) d3 ? MR_K4(3, true, true) : MR_K4(2, true, true);
-    else if (paired) d3 ? MR_K4(3, true, false) : MR_K4(2, true, false);
-    else if (c->bp_sync) d3 ? MR_K4(3, false, true) : MR_K4(2, false, true);
-    else d3 ? MR_K4(3, false, false) : MR_K4(2, false, false);
+#define MR_K4(D, G, S) launch_k4<CONE, ST, PPT, LAY, D, G, S>(c, ring, vs, recs, d_tasks, d_tiles, nt, n_views, scale, d_out, s)
+    if (ring_kind == 2) {
+        if constexpr (B16) {
+            if (c->bp_sync) d3 ? MR_K4(3, 2, true) : MR_K4(2, 2, true);
+            else d3 ? MR_K4(3, 2, false) : MR_K4(2, 2, false);
+        }
+    } else if (ring_kind == 1 && c->bp_sync) d3 ? MR_K4(3, 1, true) : MR_K4(2, 1, true);
+    else if (ring_kind == 1) d3 ? MR_K4(3, 1, false) : MR_K4(2, 1, false);
+    else if (c->bp_sync) d3 ? MR_K4(3, 0, true) : MR_K4(2, 0, true);
+    else d3 ? MR_K4(3, 0, false) : MR_K4(2, 0, false);
 #undef MR_K4
 }
 
@@ -1882,6 +1984,31 @@ int ensure_pair_ring(Ctx* c, int n_views, hipStream_t s) {
     return MR_OK;
 }
 
+// Kernel 7 (16-bit storage) reads the band ring: bands of 4 padded rows stepping 3 rows
+// (4/3 of the ring), built and extended like the row-pair ring.
+int64_t band_view_stride(const Ctx* c) {
+    return static_cast<int64_t>(band_count(c->R)) * c->pitch * 4;
+}
+
+int ensure_band_ring(Ctx* c, int n_views, hipStream_t s) {
+    const int64_t bvs = band_view_stride(c);
+    if (!c->bring) {
+        c->bring_bytes = sizeof(uint16_t) * static_cast<size_t>(bvs) * c->n_total;
+        MR_HIP(hipMalloc(&c->bring, c->bring_bytes));
+        c->bring_n = 0;
+    }
+    for (int v0 = c->bring_n; v0 < n_views; v0 += 4096) {
+        const int nv = std::min(4096, n_views - v0);
+        const dim3 grid((c->pitch + 255) / 256, band_count(c->R), nv);
+        hipLaunchKernelGGL(band_convert_kernel, grid, dim3(256), 0, s,
+                           static_cast<const uint16_t*>(c->ring), static_cast<uint16_t*>(c->bring),
+                           c->view_stride, bvs, c->pitch, c->Rp, v0);
+        MR_HIP(hipGetLastError());
+    }
+    c->bring_n = std::max(c->bring_n, n_views);
+    return MR_OK;
+}
+
 // v4 (and v5 for cone beam): per chunk of tiles, the record kernel then the tile kernel.
 int launch_bp_tiles_v4(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles, int n_views,
                        float scale, float* d_out, hipStream_t s) {
@@ -1891,10 +2018,22 @@ int launch_bp_tiles_v4(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int 
     }
     const bool cone = c->kind == MR_KIND_CONE;
     const bool v5 = cone && c->bp_variant == 5;
-    if (c->bp_variant == 6) {
+    if (c->bp_variant == 6 || (c->bp_variant == 7 && c->storage == MR_STORE_F32)) {
         int rc = ensure_pair_ring(c, n_views, s);
         if (rc) return rc;
+    } else if (c->bp_variant == 7) {
+        if (c->R + 1 >= 32768) {
+            // the band index arithmetic of kernel 7 is exact for padded rows below 32768
+            const int sv = c->bp_variant;
+            c->bp_variant = 6;
+            const int rc = launch_bp_tiles_v4(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s);
+            c->bp_variant = sv;
+            return rc;
+        }
+        int rc = ensure_band_ring(c, n_views, s);
+        if (rc) return rc;
     }
+    const bool band = c->bp_variant == 7 && c->storage != MR_STORE_F32;   // band-ring records
     const int rec_floats = v5 ? V5_REC : BP_REC;
     const size_t per_tile = sizeof(float) * rec_floats * static_cast<size_t>(n_views);
     const size_t budget = static_cast<size_t>(512) << 20;          // record buffer <= 512 MB
@@ -1940,7 +2079,8 @@ int launch_bp_tiles_v4(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int 
             continue;
         }
         hipLaunchKernelGGL(records_kernel, rgrid, dim3(256), 0, s, c->d_geom, c->d_vscale, d_tasks,
-                           d_tiles + t0, nt, n_views, c->R, c->C, c->pitch, cone ? 1 : 0, c->d_recs);
+                           d_tiles + t0, nt, n_views, c->R, c->C, c->pitch, cone ? 1 : 0, c->d_recs,
+                           band ? 1 : 0);
         if (t0 == 0) MR_HIP(hipEventRecord(c->evk0, s));
         switch (c->storage) {
         case MR_STORE_F32:

@@ -70,6 +70,10 @@ struct TileQueue {
     int grid = 0;                    // persistent grid size
 };
 
+// Bands of the band ring (kernel 7): padded rows 0..R+1 can hold a point's upper tap,
+// band b covers padded rows 3b..3b+3.
+constexpr int band_count(int R) { return (R + 1) / 3 + 1; }
+
 // BP tile shape: TP x TS points per 256-thread workgroup, 4 points per thread.
 constexpr int BP_TS = 32;
 constexpr int BP_TP = 32;
@@ -115,6 +119,9 @@ struct Ctx {
     void* pring = nullptr;           // row-pair copy of the ring (kernel 6)
     size_t pring_bytes = 0;
     int pring_n = 0;                 // views converted into pring
+    void* bring = nullptr;           // band copy of a 16-bit ring (kernel 7)
+    size_t bring_bytes = 0;
+    int bring_n = 0;                 // views converted into bring
     uint16_t* d_raw = nullptr;       // staged raw frames
     size_t raw_bytes = 0;
     void* d_vol = nullptr;           // staged volume for the statistics entry points
