@@ -202,3 +202,49 @@ def test_pair_ring_follows_appends_and_resets(workload, variant, monkeypatch):
         c.reset_views()
         c.push(np.arange(A), other)
     assert rel_l2(ctx.bp_tasks(T, A, out_len), ref_ctx.bp_tasks(T, A, out_len)) < 1e-6
+
+
+@pytest.mark.parametrize("rows", [37, 38, 39])
+def test_band_ring_rows_and_borders(rows, monkeypatch):
+    """Kernel 7 (band ring) against kernel 1 on the same 16-bit ring, for detector heights
+    of every residue mod 3, cone and parallel beam: oblique slices, and a volume grid 1.6x
+    the box whose outer points project off the detector (the zero border and clamps)."""
+    from minirecast import _hip, phantom, recon
+    from minirecast.geometry import ParallelGeometry, circular_cone_vectors
+    C = 50
+    cone = circular_cone_vectors(90, 400.0, 600.0, rows, C, 0.3)
+    par = ParallelGeometry(np.arange(90) * np.pi / 90, rows, C, 1.0, 1.0)
+    for g in (cone, par):
+        box = g.volume_box()
+        ph = phantom.random_ellipsoids(3, 6, box, (0.1, 1.0))
+        if g is cone:
+            proj = phantom.project_cone(ph, g)
+        else:
+            proj = phantom.project_parallel(ph, g)
+        tasks, off = [], 0
+        for normal, n in (((1.0, 0.5, 0.8), 61), ((0.2, -0.1, 1.0), 45)):
+            tasks.append(recon.plane_task(box, (0.0, 0.0, 0.3), normal, n, out_offset=off))
+            off += n * n
+        dims, origin, h = recon.volume_grid(1.6 * np.asarray(box), 24)
+        vt = recon.volume_task(origin, h, dims, 0, int(dims[2]))
+        vt['out_offset'] = off
+        tasks.append(vt)
+        out_len = off + int(np.prod(dims))
+        T = np.array(tasks, dtype=_hip.TASK_DTYPE)
+        for storage in ("f16", "bf16"):
+            outs = []
+            for variant in ("1", "7"):
+                monkeypatch.setenv("MINIRECAST_BP_VARIANT", variant)
+                monkeypatch.setenv("MINIRECAST_BP_PPT", "2")
+                ctx = _hip.Context(0)
+                if g is cone:
+                    ctx.set_cone(g.vectors, rows, C, "ram-lak", storage)
+                else:
+                    ctx.set_parallel(g.angles, rows, C, 1.0, 1.0, "ram-lak", storage)
+                assert ctx.bp_config()["kernel"] == int(variant)
+                ctx.push(np.arange(g.n_angles), proj)
+                outs.append(ctx.bp_tasks(T, g.n_angles, out_len))
+            ref, got = outs
+            np.testing.assert_array_equal(got == 0, ref == 0)
+            assert rel_l2(got, ref) < 1e-6, (rows, type(g).__name__, storage, rel_l2(got, ref))
+            assert np.count_nonzero(ref[off:]) < ref[off:].size   # border points present
