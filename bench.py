@@ -289,13 +289,12 @@ def bench_stream_cone(args):
 def bench_parallel_cor(args):
     """Rotation-centre search, parallel beam: 1800 views, 64 x 2048 detector, 25 shifts
     x axial slices 1024^2 (z = 0), 768^2 (z = +16), 512^2 (z = -16); every trial image of
-    a slice size back-projected in one launch and scored (entropy) on the device."""
+    a slice size back-projected in one launch and scored (entropy) on the device.  Timed
+    for the float32 ring (the line's value) and the float16 ring (per-view scale)."""
     from minirecast import _hip, phantom, recon, workloads
     w = workloads.parallel_cor()
     g = w.geom
     proj = phantom.project_parallel(w.phantom, g)
-    rec = recon.SliceReconstructor(proj, g, window=w.window, verbose=False)
-    del proj
     box = g.volume_box()
     shifts = w.extra["shifts"]
     groups, upd = [], 0
@@ -305,21 +304,27 @@ def bench_parallel_cor(args):
         _, m = recon.slice_points(box, p['origin'], p['normal'], p['n'])
         upd += int(m.sum()) * len(shifts) * g.n_angles
         groups.append(T)
-    ctx = rec._ctx
-    for T in groups:
-        ctx.cor_scores(T, rec.n)                         # warm-up
-    times, best = [], []
-    for _ in range(max(1, args.steps)):
-        t0 = time.perf_counter()
-        sc = [ctx.cor_scores(T, rec.n) for T in groups]
-        times.append((time.perf_counter() - t0) * 1e3)
-    best = [float(shifts[int(np.argmax(x))]) for x in sc]
-    ms = float(np.median(times))
-    res = dict(ms_per_search=ms, best_shift_per_slice=best, trial_images=25 * 3,
-               updates=upd)
+    forms = {}
+    for storage in ("f32", "f16"):
+        rec = recon.SliceReconstructor(proj, g, window=w.window, verbose=False, storage=storage)
+        ctx = rec._ctx
+        for T in groups:
+            ctx.cor_scores(T, rec.n)                     # warm-up
+        times = []
+        for _ in range(max(1, args.steps)):
+            t0 = time.perf_counter()
+            sc = [ctx.cor_scores(T, rec.n) for T in groups]
+            times.append((time.perf_counter() - t0) * 1e3)
+        ms = float(np.median(times))
+        forms[storage] = dict(ms_per_search=ms, gups=upd / (ms * 1e-3) / 1e9,
+                              best_shift_per_slice=[float(shifts[int(np.argmax(x))]) for x in sc],
+                              config=ctx.bp_config())
+        del rec, ctx                                     # frees the context's rings
+    res = dict(forms, trial_images=25 * 3, updates=upd)
     log(f"parallel_cor: {res}")
+    ms = forms["f32"]["ms_per_search"]
     _line("ms per COR search (25 shifts x 3 axial slices, 1800 views) + GUPS",
-          round(upd / (ms * 1e-3) / 1e9, 2), "GUPS", round(ms, 3), len(times), 1,
+          round(upd / (ms * 1e-3) / 1e9, 2), "GUPS", round(ms, 3), args.steps, 1,
           {"workload": "parallel_cor: 1800 views / 180 deg, 64x2048 detector, hann, "
                        "25 shifts x axial 1024^2, 768^2, 512^2"}, {"cor": res})
 
