@@ -246,44 +246,54 @@ def bench_stream_cone(args):
     batch = w.extra["batch"]
     blocks = [(order[i:i + batch], np.ascontiguousarray(proj[:, order[i:i + batch], :]))
               for i in range(0, A, batch)]
-    # graph path
-    ctx = _hip.Context(0)
-    ctx.set_cone(g.vectors, R, C, w.window)
-    plan = _hip.Plan(ctx, np.array(tasks, dtype=_hip.TASK_DTYPE), out_len)
-    gpu_ms, wall_ms = [], []
-    for idx, blk in blocks:
-        t0 = time.perf_counter()
-        plan.stream(idx, blk, copy=False)
-        wall_ms.append((time.perf_counter() - t0) * 1e3)
-        gpu_ms.append(plan.last_ms())
-    out_graph = plan.run(-1, copy=True)
-    # eager path
-    rec = recon.SliceReconstructor(None, g, window=w.window, verbose=False)
-    eager_ms = []
-    for idx, blk in blocks:
-        t0 = time.perf_counter()
-        rec.add_projections(idx, blk)
-        imgs = rec.reconstruct_planes(w.planes)
-        vol, _, _ = rec.preview_volume(w.extra["preview"])
-        eager_ms.append((time.perf_counter() - t0) * 1e3)
-    ref = np.concatenate([im.ravel() for im in imgs] + [vol.ravel()])
-    err = float(np.linalg.norm(out_graph - ref) / np.linalg.norm(ref))
     updates = sum(min((b + 1) * batch, A) * (n_in + nvox) for b in range(len(blocks)))
-    total = sum(wall_ms) * 1e-3
-    res = dict(graph_ms_per_batch_median=float(np.median(wall_ms)),
-               graph_gpu_ms_per_batch_median=float(np.median(gpu_ms)),
-               graph_ms_last_batch=float(wall_ms[-1]), eager_ms_per_batch_median=float(np.median(eager_ms)),
-               eager_ms_last_batch=float(eager_ms[-1]), graph_vs_eager_rel_l2=err,
-               batches=len(blocks), first_batch_ms=float(wall_ms[0]))
+
+    def run(storage):
+        # graph path
+        ctx = _hip.Context(0)
+        ctx.set_cone(g.vectors, R, C, w.window, storage)
+        plan = _hip.Plan(ctx, np.array(tasks, dtype=_hip.TASK_DTYPE), out_len)
+        gpu_ms, wall_ms = [], []
+        for idx, blk in blocks:
+            t0 = time.perf_counter()
+            plan.stream(idx, blk, copy=False)
+            wall_ms.append((time.perf_counter() - t0) * 1e3)
+            gpu_ms.append(plan.last_ms())
+        out_graph = plan.run(-1, copy=True)
+        plan.close()
+        # eager path
+        rec = recon.SliceReconstructor(None, g, window=w.window, verbose=False, storage=storage)
+        eager_ms = []
+        for idx, blk in blocks:
+            t0 = time.perf_counter()
+            rec.add_projections(idx, blk)
+            imgs = rec.reconstruct_planes(w.planes)
+            vol, _, _ = rec.preview_volume(w.extra["preview"])
+            eager_ms.append((time.perf_counter() - t0) * 1e3)
+        ref = np.concatenate([im.ravel() for im in imgs] + [vol.ravel()])
+        err = float(np.linalg.norm(out_graph - ref) / np.linalg.norm(ref))
+        res = dict(graph_ms_per_batch_median=float(np.median(wall_ms)),
+                   graph_gpu_ms_per_batch_median=float(np.median(gpu_ms)),
+                   graph_ms_last_batch=float(wall_ms[-1]),
+                   eager_ms_per_batch_median=float(np.median(eager_ms)),
+                   eager_ms_last_batch=float(eager_ms[-1]), graph_vs_eager_rel_l2=err,
+                   batches=len(blocks), first_batch_ms=float(wall_ms[0]),
+                   gups=updates / (sum(wall_ms) * 1e-3) / 1e9)
+        return res, ref
+
+    res, ref32 = run("f32")
+    res16, ref16 = run("f16")
+    res16["f16_vs_f32_rel_l2"] = float(np.linalg.norm(ref16 - ref32) / np.linalg.norm(ref32))
+    res["f16"] = res16
     log(f"stream_cone: {res}")
     _line("ms per streamed batch (10 views ingested + 3 oblique 512^2 slices + 64^3 preview)",
-          round(updates / total / 1e9, 3), "GUPS", round(float(np.median(wall_ms)), 4),
+          round(res["gups"], 3), "GUPS", round(res["graph_ms_per_batch_median"], 4),
           len(blocks), 0,
           {"workload": "stream_cone: 600 views in 60 interleaved batches of 10, 384x486 "
                        "detector, 3 oblique 512^2 slices + 64^3 preview per batch",
-           "path": "mr_plan hipGraph per batch (upload + filter + persistent BP)"},
+           "path": "mr_plan hipGraph per batch (upload + filter + persistent BP), f32 ring; "
+                   "f16 ring under stream.f16"},
           {"stream": res})
-    plan.close()
 
 
 def bench_parallel_cor(args):
