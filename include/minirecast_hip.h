@@ -196,9 +196,19 @@ int mr_get_bp_config(mr_ctx* ctx, int* cfg);
 /* Autotuner: times every configuration on an evenly spaced sample of at most
  * max_tiles 32 x 32 tiles of `tasks` (reps timed runs each, after one warm-up, against
  * the first n_views stored views), keeps the fastest for this context and returns it
- * in cfg[4] with its time per run in *best_ms.  out_len sizes a scratch output. */
+ * in cfg[4] with its time per run in *best_ms.  out_len sizes a scratch output.  When
+ * the tiles timed are all of the request's, the request is cone beam of single-plane
+ * tasks and its grid fills the device, the views per launch of the chosen configuration
+ * (mr_set_bp_views_per_pass) are then tuned too, among 16, 32, 64, 128 and all. */
 int mr_autotune(mr_ctx* ctx, const mr_task* tasks, int ntasks, int n_views, int64_t out_len,
                 int max_tiles, int reps, int* cfg, float* best_ms);
+
+/* Views per launch of kernels 4, 6 and 7 for cone-beam requests of single-plane tasks
+ * whose grid fills the device (>= 8 workgroups per CU): 0 = all views in one launch.
+ * Until set (here, by MINIRECAST_BP_VCHUNK or by mr_autotune) it follows the storage
+ * form: 32 for F32, 64 for F16 and BF16.  No counterpart in the reference. */
+int mr_set_bp_views_per_pass(mr_ctx* ctx, int views);
+int mr_get_bp_views_per_pass(mr_ctx* ctx, int* views);
 
 /* Rotation-centre search scores: recon.py:498-513.  Each task is one trial image
  * (nq = 1, all np x ns equal; out_offset/strides are ignored): it is back-projected

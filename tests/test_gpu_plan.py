@@ -248,3 +248,38 @@ def test_band_ring_rows_and_borders(rows, monkeypatch):
             np.testing.assert_array_equal(got == 0, ref == 0)
             assert rel_l2(got, ref) < 1e-6, (rows, type(g).__name__, storage, rel_l2(got, ref))
             assert np.count_nonzero(ref[off:]) < ref[off:].size   # border points present
+
+
+def test_autotune_views_per_pass(workload):
+    """With every tile timed and a grid that fills the device, the autotuner also picks
+    the views per launch; the tuned context still matches the plain v1 kernel, and the
+    setting is validated and reported."""
+    from minirecast import _hip, recon
+    w, proj, planes, tasks, out_len = workload
+    A = w.geom.n_angles
+    box = w.geom.volume_box()
+    big = [dict(origin=(0.0, 0.0, 0.0), normal=nrm, n=1024)
+           for nrm in ((1.0, 0.5, 0.8), (-0.3, 1.0, 0.4), (0.2, -0.1, 1.0))]
+    T, off = [], 0
+    for p in big:
+        T.append(recon.plane_task(box, p['origin'], p['normal'], p['n'], out_offset=off))
+        off += p['n'] ** 2
+    T = np.array(T, dtype=_hip.TASK_DTYPE)
+    ref_ctx = _hip.Context(0)
+    ref_ctx.set_cone(w.geom.vectors, w.geom.rows, w.geom.cols, w.window, "f16")
+    ref_ctx.set_bp_config(kernel=1, ppt=4, layout=0, depth=2)
+    ref_ctx.push(np.arange(A), proj)
+    ref = ref_ctx.bp_tasks(T, A, off)
+    ctx = _hip.Context(0)
+    ctx.set_cone(w.geom.vectors, w.geom.rows, w.geom.cols, w.window, "f16")
+    ctx.push(np.arange(A), proj)
+    assert ctx.views_per_pass() == 64                    # the 16-bit default
+    cfg, ms = ctx.autotune(T, A, off, max_tiles=10 ** 6, reps=1)
+    assert ms > 0 and ctx.views_per_pass() in (0, 16, 32, 64, 128)
+    got = ctx.bp_tasks(T, A, off)
+    np.testing.assert_array_equal(got == 0, ref == 0)
+    assert rel_l2(got, ref) < 1e-6, (cfg, ctx.views_per_pass(), rel_l2(got, ref))
+    ctx.set_views_per_pass(0)
+    assert ctx.views_per_pass() == 0
+    with pytest.raises(ValueError):
+        ctx.set_views_per_pass(-1)

@@ -1086,8 +1086,52 @@ int mr_autotune(mr_ctx* c, const mr_task* tasks, int ntasks, int n_views, int64_
     c->bp_layout = best.layout;
     c->bp_depth = best.depth;
     if (rc) return rc;
+    // views per launch, when the whole request was timed and its launches can be split
+    const int subs = (best.variant >= 4 && best.variant != 5) ? BP_TP / (8 * best.ppt) : 0;
+    if (subs > 0 && tiles.size() == all.size() && c->kind == MR_KIND_CONE && c->req_slices &&
+        nt * subs >= 8 * c->n_cus) {
+        const int saved = c->bp_vchunk;
+        int best_vc = saved;
+        for (const int vc : {16, 32, 64, 128, 0}) {
+            c->bp_vchunk = vc;
+            rc = launch_bp_tiles(c, d_tasks, d_tiles, nt, n_views, scale, c->d_out, c->s_req);
+            if (rc) break;
+            MR_HIP(hipEventRecord(c->ev0, c->s_req));
+            for (int r = 0; r < reps && !rc; ++r)
+                rc = launch_bp_tiles(c, d_tasks, d_tiles, nt, n_views, scale, c->d_out, c->s_req);
+            if (rc) break;
+            MR_HIP(hipEventRecord(c->ev1, c->s_req));
+            MR_HIP(hipEventSynchronize(c->ev1));
+            float ms = 0.0f;
+            MR_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            ms /= reps;
+            if (ms < best_t) { best_t = ms; best_vc = vc; }
+        }
+        c->bp_vchunk = best_vc;
+        c->vchunk_explicit = true;
+        if (rc) return rc;
+    }
     if (cfg) { cfg[0] = best.variant; cfg[1] = best.ppt; cfg[2] = best.layout; cfg[3] = best.depth; }
     if (best_ms) *best_ms = best_t;
+    return MR_OK;
+}
+
+int mr_set_bp_views_per_pass(mr_ctx* c, int views) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    MR_CHECK(views >= 0, MR_EVALUE, "views per pass must be >= 0 (0: all views in one launch)");
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->bp_vchunk = views;
+    c->vchunk_explicit = true;
+    return MR_OK;
+}
+
+int mr_get_bp_views_per_pass(mr_ctx* c, int* views) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    MR_CHECK(views != nullptr, MR_EVALUE, "null output");
+    std::lock_guard<std::mutex> lk(c->mu);
+    *views = c->bp_vchunk;
     return MR_OK;
 }
 

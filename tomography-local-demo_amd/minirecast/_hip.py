@@ -44,6 +44,8 @@ SIGNATURES = [
     ("mr_clip_histogram", _c.c_int, [_P, _P, _c.c_int64, _c.c_int, _c.c_double, _c.c_double,
                                      _c.c_int, _P, _P]),
     ("mr_set_bp_config", _c.c_int, [_P, _c.c_int, _c.c_int, _c.c_int, _c.c_int]),
+    ("mr_set_bp_views_per_pass", _c.c_int, [_P, _c.c_int]),
+    ("mr_get_bp_views_per_pass", _c.c_int, [_P, _c.POINTER(_c.c_int)]),
     ("mr_get_bp_config", _c.c_int, [_P, _P]),
     ("mr_autotune", _c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_int64, _c.c_int, _c.c_int, _P,
                                _c.POINTER(_c.c_float)]),
@@ -288,6 +290,15 @@ class Context:
         cfg = np.zeros(4, np.int32)
         check(self.lib.mr_get_bp_config(self.h, ptr(cfg)))
         return dict(zip(self.BP_CONFIG_KEYS, (int(x) for x in cfg)))
+
+    def set_views_per_pass(self, views):
+        """Views per launch of cone slice requests that fill the device (0: all)."""
+        check(self.lib.mr_set_bp_views_per_pass(self.h, int(views)))
+
+    def views_per_pass(self):
+        v = ctypes.c_int(0)
+        check(self.lib.mr_get_bp_views_per_pass(self.h, ctypes.byref(v)))
+        return v.value
 
     def autotune(self, tasks, n_views, out_len, max_tiles=512, reps=2):
         """Time every back-projection configuration on (a sample of) tasks, keep the
