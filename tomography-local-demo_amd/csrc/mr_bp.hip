@@ -311,6 +311,51 @@ __device__ inline int xcd_block(int b, int nb) {
     return x * per + (x < rem ? x : rem) + k;
 }
 
+// Taps of one point as loaded by kernels 2 and 4: f32 keeps four floats; the 16-bit formats keep
+// the two row dwords packed until the lerp, where f16 halves feed v_fma_mix_f32
+// directly (no separate conversions).
+template <typename ST> struct Pair;
+template <> struct Pair<float> {
+    float a, b, c, d;
+    __device__ inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int soff1) {
+        Taps<float>::load(rs, voff, soff1, a, b, c, d);
+    }
+    __device__ inline void lerp(float fc, float& top, float& bot) const {
+        top = fmaf(fc, b - a, a);
+        bot = fmaf(fc, d - c, c);
+    }
+};
+__device__ inline float half_lo(uint32_t w) {
+    return __half2float(__ushort_as_half(static_cast<unsigned short>(w & 0xffffu)));
+}
+__device__ inline float half_hi(uint32_t w) {
+    return __half2float(__ushort_as_half(static_cast<unsigned short>(w >> 16)));
+}
+template <> struct Pair<__half> {
+    uint32_t w0, w1;
+    __device__ inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int soff1) {
+        w0 = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, 0, 0);
+        w1 = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff1, 0);
+    }
+    __device__ inline void lerp(float fc, float& top, float& bot) const {
+        top = fmaf(fc, half_hi(w0), fmaf(-fc, half_lo(w0), half_lo(w0)));
+        bot = fmaf(fc, half_hi(w1), fmaf(-fc, half_lo(w1), half_lo(w1)));
+    }
+};
+template <> struct Pair<uint16_t> {
+    uint32_t w0, w1;
+    __device__ inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int soff1) {
+        w0 = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, 0, 0);
+        w1 = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff1, 0);
+    }
+    __device__ inline void lerp(float fc, float& top, float& bot) const {
+        const float a = __uint_as_float(w0 << 16), b = __uint_as_float(w0 & 0xffff0000u);
+        const float c = __uint_as_float(w1 << 16), d = __uint_as_float(w1 & 0xffff0000u);
+        top = fmaf(fc, b - a, a);
+        bot = fmaf(fc, d - c, c);
+    }
+};
+
 template <bool CONE, typename ST>
 __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel2(
     const ST* __restrict__ ring, int64_t view_stride, int pitch, int R, int C,
@@ -416,10 +461,10 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel2(
                     const float fc = __builtin_amdgcn_fractf(col);
                     const float fr = __builtin_amdgcn_fractf(row);
                     const int voff = __mul24(cvt_flr(row), pitchb) + ((cvt_flr(col) << LE) + baseb);
-                    float a, b, c, d;
-                    Taps<ST>::load(rs, voff, pitchb, a, b, c, d);
-                    const float top = fmaf(fc, b - a, a);
-                    const float bot = fmaf(fc, d - c, c);
+                    Pair<ST> tap;                  // 16-bit taps: lerps on the packed halves
+                    tap.load(rs, voff, pitchb);
+                    float top, bot;
+                    tap.lerp(fc, top, bot);
                     acc[m] = fmaf(wt, fmaf(fr, bot - top, top), acc[m]);
                 }
             } else {
@@ -431,10 +476,12 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel2(
                     const float col = __builtin_amdgcn_fmed3f(m ? fmaf(dm, r0.z, uh0) : uh0, r2.y, r2.z);
                     const float fc = __builtin_amdgcn_fractf(col);
                     const int voff = roff_par[m] + ((cvt_flr(col) << LE) + baseb);
-                    float a, b, c, d;
-                    Taps<ST>::load(rs, voff, pitchb, a, b, c, d);
-                    acc[m] = fmaf(vs, fmaf(fc, b - a, a), acc[m]);
-                    accb[m] = fmaf(vs, fmaf(fc, d - c, c), accb[m]);
+                    Pair<ST> tap;
+                    tap.load(rs, voff, pitchb);
+                    float top, bot;
+                    tap.lerp(fc, top, bot);
+                    acc[m] = fmaf(vs, top, acc[m]);
+                    accb[m] = fmaf(vs, bot, accb[m]);
                 }
             }
         }
@@ -720,51 +767,6 @@ __global__ __launch_bounds__(256) void records_kernel(const ViewGeom* __restrict
     make_record(geom + v, vscale[v], T, xt, R, C, pitch, cone != 0, recs + gid * BP_REC, nullptr,
                 band != 0);
 }
-
-// Taps of one point as loaded by v4: f32 keeps four floats; the 16-bit formats keep
-// the two row dwords packed until the lerp, where f16 halves feed v_fma_mix_f32
-// directly (no separate conversions).
-template <typename ST> struct Pair;
-template <> struct Pair<float> {
-    float a, b, c, d;
-    __device__ inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int soff1) {
-        Taps<float>::load(rs, voff, soff1, a, b, c, d);
-    }
-    __device__ inline void lerp(float fc, float& top, float& bot) const {
-        top = fmaf(fc, b - a, a);
-        bot = fmaf(fc, d - c, c);
-    }
-};
-__device__ inline float half_lo(uint32_t w) {
-    return __half2float(__ushort_as_half(static_cast<unsigned short>(w & 0xffffu)));
-}
-__device__ inline float half_hi(uint32_t w) {
-    return __half2float(__ushort_as_half(static_cast<unsigned short>(w >> 16)));
-}
-template <> struct Pair<__half> {
-    uint32_t w0, w1;
-    __device__ inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int soff1) {
-        w0 = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, 0, 0);
-        w1 = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff1, 0);
-    }
-    __device__ inline void lerp(float fc, float& top, float& bot) const {
-        top = fmaf(fc, half_hi(w0), fmaf(-fc, half_lo(w0), half_lo(w0)));
-        bot = fmaf(fc, half_hi(w1), fmaf(-fc, half_lo(w1), half_lo(w1)));
-    }
-};
-template <> struct Pair<uint16_t> {
-    uint32_t w0, w1;
-    __device__ inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int soff1) {
-        w0 = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, 0, 0);
-        w1 = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff1, 0);
-    }
-    __device__ inline void lerp(float fc, float& top, float& bot) const {
-        const float a = __uint_as_float(w0 << 16), b = __uint_as_float(w0 & 0xffff0000u);
-        const float c = __uint_as_float(w1 << 16), d = __uint_as_float(w1 & 0xffff0000u);
-        top = fmaf(fc, b - a, a);
-        bot = fmaf(fc, d - c, c);
-    }
-};
 
 // Taps from the row-pair ring: one element of pair-row r holds (q(r, c), q(r+1, c)) side
 // by side, so the four taps of a point are contiguous — 8 bytes (16-bit forms, one
