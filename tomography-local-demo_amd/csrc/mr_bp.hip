@@ -331,11 +331,21 @@ __device__ inline float half_lo(uint32_t w) {
 __device__ inline float half_hi(uint32_t w) {
     return __half2float(__ushort_as_half(static_cast<unsigned short>(w >> 16)));
 }
+// 16-bit taps at odd columns are not dword-aligned: both rows are read as 8 aligned
+// bytes and the dword holding columns c, c+1 is taken by a funnel shift.
+__device__ inline void load_pair16(__amdgpu_buffer_rsrc_t rs, int voff, int soff1, uint32_t& w0,
+                                   uint32_t& w1) {
+    const int a = voff & ~3;
+    const uint32_t sh = static_cast<uint32_t>(voff & 2) << 3;
+    const uint2 x0 = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(rs, a, 0, 0));
+    const uint2 x1 = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(rs, a, soff1, 0));
+    w0 = __builtin_amdgcn_alignbit(x0.y, x0.x, sh);
+    w1 = __builtin_amdgcn_alignbit(x1.y, x1.x, sh);
+}
 template <> struct Pair<__half> {
     uint32_t w0, w1;
     __device__ inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int soff1) {
-        w0 = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, 0, 0);
-        w1 = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff1, 0);
+        load_pair16(rs, voff, soff1, w0, w1);
     }
     __device__ inline void lerp(float fc, float& top, float& bot) const {
         top = fmaf(fc, half_hi(w0), fmaf(-fc, half_lo(w0), half_lo(w0)));
@@ -345,8 +355,7 @@ template <> struct Pair<__half> {
 template <> struct Pair<uint16_t> {
     uint32_t w0, w1;
     __device__ inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int soff1) {
-        w0 = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, 0, 0);
-        w1 = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff1, 0);
+        load_pair16(rs, voff, soff1, w0, w1);
     }
     __device__ inline void lerp(float fc, float& top, float& bot) const {
         const float a = __uint_as_float(w0 << 16), b = __uint_as_float(w0 & 0xffff0000u);
