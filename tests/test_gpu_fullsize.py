@@ -2,8 +2,8 @@
 oblique 1024^2 slices (1,985,962 in-box points), f32 and f16 rings in their default
 back-projection configurations.
 
-Checked: the float64 oracle on 3 x 256 sampled in-box pixels (f32 <= 1e-5, f16 <= 1e-3
-relative L2), stream == batch (interleaved arrival in batches of 40), exact linearity
+Checked: the float64 oracle on 3 x 256 sampled in-box pixels (f32 <= 1e-5, f16 <= 1e-3,
+bf16 <= 5e-3 relative L2), stream == batch (interleaved arrival in batches of 40), exact linearity
 of the f32 path (projections x 2 give outputs x 2 bit for bit: scaling by 2 is exact in
 binary floating point), and the default f16 kernel (band ring) against the plain v1
 kernel on every output.
@@ -26,7 +26,7 @@ def full():
     g = w.geom
     A, R, C = g.n_angles, g.rows, g.cols
     box = g.volume_box()
-    names = ("f32", "f16", "f32_stream", "f32_x2")
+    names = ("f32", "f16", "bf16", "f32_stream", "f32_x2")
     ctx = {}
     for name in names:
         ctx[name] = _hip.Context(0)
@@ -69,7 +69,7 @@ def full():
                     q64 = OC.fdk_filter_view(host, j, g.vectors[v], w.window)
                     OC.bp_cone_acc_view(q64, g.vectors[v], 0.0, samp_pts, acc64)
 
-    ingest(("f32", "f16", "f32_x2"), np.arange(A))
+    ingest(("f32", "f16", "bf16", "f32_x2"), np.arange(A))
     ingest(("f32_stream",), arrival_order(A, "interleaved"))
     del raw
     torch.cuda.empty_cache()
@@ -80,7 +80,7 @@ def full():
 
 
 def test_full_size_vs_float64_oracle(full):
-    for name, tol in (("f32", 1e-5), ("f16", 1e-3)):
+    for name, tol in (("f32", 1e-5), ("f16", 1e-3), ("bf16", 5e-3)):
         got = full["out"][name][full["samp_idx"]].astype(np.float64)
         err = rel_l2(got, full["ref"])
         print(f"{name}: rel-L2 {err:.2e} vs float64 at {len(got)} sampled pixels")
