@@ -300,7 +300,8 @@ def bench_parallel_cor(args):
     """Rotation-centre search, parallel beam: 1800 views, 64 x 2048 detector, 25 shifts
     x axial slices 1024^2 (z = 0), 768^2 (z = +16), 512^2 (z = -16); every trial image of
     a slice size back-projected in one launch and scored (entropy) on the device.  Timed
-    for the float32 ring (the line's value) and the float16 ring (per-view scale)."""
+    for the float32 ring (the line's value) and the float16 ring (per-view scale), and as
+    one replayed graph per slice size (persistent launch + scoring) beside it."""
     from minirecast import _hip, phantom, recon, workloads
     w = workloads.parallel_cor()
     g = w.geom
@@ -326,10 +327,30 @@ def bench_parallel_cor(args):
             sc = [ctx.cor_scores(T, rec.n) for T in groups]
             times.append((time.perf_counter() - t0) * 1e3)
         ms = float(np.median(times))
+        # the same search as one replayed graph per slice size (persistent launch over a
+        # device queue of every trial image's tiles + device scoring)
+        plans = []
+        for T in groups:
+            P = T.copy()
+            npix = int(P[0]['np']) * int(P[0]['ns'])
+            P['out_offset'] = np.arange(len(P)) * npix
+            plans.append(_hip.Plan(ctx, P, len(P) * npix))
+            plans[-1].cor_scores(rec.n)                  # capture + warm-up
+        ptimes = []
+        for _ in range(max(1, args.steps)):
+            t0 = time.perf_counter()
+            psc = [pl.cor_scores(rec.n) for pl in plans]
+            ptimes.append((time.perf_counter() - t0) * 1e3)
+        pms = float(np.median(ptimes))
         forms[storage] = dict(ms_per_search=ms, gups=upd / (ms * 1e-3) / 1e9,
                               best_shift_per_slice=[float(shifts[int(np.argmax(x))]) for x in sc],
-                              config=ctx.bp_config())
-        del rec, ctx                                     # frees the context's rings
+                              config=ctx.bp_config(),
+                              plan_graph_ms_per_search=pms,
+                              plan_graph_best_shift_per_slice=[float(shifts[int(np.argmax(x))])
+                                                               for x in psc])
+        for pl in plans:
+            pl.close()
+        del rec, ctx, plans                              # frees the context's rings
     res = dict(forms, trial_images=25 * 3, updates=upd)
     log(f"parallel_cor: {res}")
     ms = forms["f32"]["ms_per_search"]

@@ -265,6 +265,14 @@ int mr_plan_run(mr_plan* plan, int n_views, float* out_host);
 int mr_plan_stream(mr_plan* plan, const int64_t* indices, int k, const float* block,
                    float* out_host);
 
+/* Rotation-centre scores (recon.py:498-513) of a plan whose tasks are the trial images
+ * of one search: equal-size single planes, contiguous in task order (out_offset =
+ * i * np * ns).  Back-projects them from the first n_views stored views (< 0: all) in
+ * the plan's persistent launch and scores each image (histogram entropy of its non-zero
+ * values, as mr_cor_scores) on the device, both in one HIP graph captured on the first
+ * call and replayed after; scores[ntasks]. */
+int mr_plan_cor_scores(mr_plan* plan, int n_views, double* scores);
+
 /* Device pointer and length of the plan's output. */
 int mr_plan_output(mr_plan* plan, float** out_device, int64_t* out_len);
 

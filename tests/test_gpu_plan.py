@@ -283,3 +283,38 @@ def test_autotune_views_per_pass(workload):
     assert ctx.views_per_pass() == 0
     with pytest.raises(ValueError):
         ctx.set_views_per_pass(-1)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_plan_cor_scores_graph(workload, storage, monkeypatch):
+    """The rotation-centre search as one graph (persistent back-projection of every trial
+    slice + device scoring): every replay, with eager work in between, matches the
+    one-launch search and picks the same shift; plans that are not a search raise."""
+    from minirecast import _hip, recon
+    w, proj, planes, tasks, out_len = workload
+    A = w.geom.n_angles
+    box = w.geom.volume_box()
+    shifts = np.arange(-2.0, 2.01, 0.5)
+    n = 48
+    T = np.array([recon.plane_task(box, (0.0, 0.0, 0.3), (0.0, 0.0, 1.0), n, cor_shift=s,
+                                   out_offset=i * n * n) for i, s in enumerate(shifts)],
+                 dtype=_hip.TASK_DTYPE)
+    ctx = _ctx(w, storage, "2", monkeypatch)
+    ctx.push(np.arange(A), proj)
+    plan = _hip.Plan(ctx, T, len(shifts) * n * n)
+    first = None
+    for nv in (A, A // 2, A):
+        want = ctx.cor_scores(T, nv)
+        got = plan.cor_scores(nv)
+        np.testing.assert_allclose(got, want, rtol=2e-3, atol=2e-3)
+        assert np.argmax(got) == np.argmax(want)
+        if nv == A:
+            if first is None:
+                first = got
+            else:
+                np.testing.assert_array_equal(got, first)   # replay after eager work
+        imgs = plan.run(nv)                                 # the images stay readable
+        assert rel_l2(imgs, ctx.bp_tasks(T, nv, len(shifts) * n * n)) < 1e-6
+    bad = _hip.Plan(ctx, np.array(tasks, dtype=_hip.TASK_DTYPE), out_len)
+    with pytest.raises(ValueError):
+        bad.cor_scores()

@@ -1247,7 +1247,10 @@ static int64_t task_candidates(const mr_task& T) {
 static void plan_free(Plan* p) {
     for (auto& g : p->graphs) hipGraphExecDestroy(g.second);
     p->graphs.clear();
-    void* dev[] = {p->d_tasks, p->d_toff, p->d_queue, p->d_counters, p->d_out, p->d_block, p->d_meta};
+    if (p->cor_exec) hipGraphExecDestroy(p->cor_exec);
+    p->cor_exec = nullptr;
+    void* dev[] = {p->d_tasks, p->d_toff, p->d_queue, p->d_counters, p->d_out, p->d_block, p->d_meta,
+                   p->d_scores};
     for (void* d : dev)
         if (d) hipFree(d);
     if (p->h_block) hipHostFree(p->h_block);
@@ -1272,6 +1275,7 @@ int mr_plan_create(mr_ctx* c, const mr_task* tasks, int ntasks, int64_t out_len,
     p->c = c;
     p->ntasks = ntasks;
     p->out_len = out_len;
+    p->h_tasks.assign(tasks, tasks + ntasks);
     std::vector<int64_t> toff(ntasks + 1, 0);
     for (int i = 0; i < ntasks; ++i) toff[i + 1] = toff[i] + task_candidates(tasks[i]);
     p->total_candidates = toff[ntasks];
@@ -1349,6 +1353,60 @@ int mr_plan_run(mr_plan* p, int n_views, float* out_host) {
     rc = plan_copy_out(p, out_host);
     if (rc) return rc;
     rc = finish_timing(c);
+    p->last_ms = c->last_kernel_ms;
+    return rc;
+}
+
+int mr_plan_cor_scores(mr_plan* p, int n_views, double* scores) {
+    MR_CHECK(p != nullptr, MR_EVALUE, "null plan");
+    MR_CHECK(scores != nullptr, MR_EVALUE, "null scores");
+    Ctx* c = p->c;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (n_views < 0) n_views = c->n;
+    MR_CHECK(n_views <= c->n, MR_EVALUE, "n_views beyond the stored views");
+    // the images must be the contiguous equal-size single-plane outputs the scoring
+    // kernel reads: task i at out_offset i * npix, rows of ns pixels
+    const mr_task& T0 = p->h_tasks[0];
+    const int64_t npix = static_cast<int64_t>(T0.np) * T0.ns;
+    for (int i = 0; i < p->ntasks; ++i) {
+        const mr_task& T = p->h_tasks[i];
+        MR_CHECK(T.nq == 1 && static_cast<int64_t>(T.np) * T.ns == npix && T.out_offset == i * npix &&
+                     T.stride_p == T.ns && T.stride_s == 1,
+                 MR_EVALUE, "score plans need single images of equal size, contiguous in task order");
+    }
+    MR_HIP(hipSetDevice(c->device));
+    if (!p->d_scores) MR_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_scores), sizeof(double) * p->ntasks));
+    // the device view count of the persistent launch
+    MR_HIP(hipMemcpyAsync(p->d_counters + 2, &n_views, sizeof(int), hipMemcpyHostToDevice, c->s_req));
+    MR_HIP(hipStreamSynchronize(c->s_req));
+    if (!p->cor_exec) {
+        TileQueue q;
+        q.head = p->d_counters + 1;
+        q.count = p->d_counters;
+        q.n_views = p->d_counters + 2;
+        q.grid = p->grid;
+        hipGraph_t graph = nullptr;
+        MR_HIP(hipStreamBeginCapture(c->s_req, hipStreamCaptureModeThreadLocal));
+        int r = launch_reset_head(p->d_counters, c->s_req);
+        if (!r) r = launch_bp_tiles(c, p->d_tasks, p->d_queue, 0, 0, 0.0f, p->d_out, c->s_req, &q);
+        if (!r) r = launch_cor_scores(c, p->d_out, p->ntasks, npix, p->d_scores, c->s_req);
+        hipError_t ec = hipStreamEndCapture(c->s_req, &graph);
+        if (r || ec != hipSuccess) {
+            if (graph) hipGraphDestroy(graph);
+            return r ? r : fail(MR_ERUNTIME, std::string("graph capture: ") + hipGetErrorString(ec));
+        }
+        hipError_t ei = hipGraphInstantiate(&p->cor_exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            p->cor_exec = nullptr;
+            return fail(MR_ERUNTIME, std::string("graph instantiate: ") + hipGetErrorString(ei));
+        }
+    }
+    MR_HIP(hipEventRecord(c->ev0, c->s_req));
+    MR_HIP(hipGraphLaunch(p->cor_exec, c->s_req));
+    MR_HIP(hipEventRecord(c->ev1, c->s_req));
+    MR_HIP(hipMemcpyAsync(scores, p->d_scores, sizeof(double) * p->ntasks, hipMemcpyDeviceToHost, c->s_req));
+    int rc = finish_timing(c);
     p->last_ms = c->last_kernel_ms;
     return rc;
 }

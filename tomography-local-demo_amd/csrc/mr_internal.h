@@ -198,6 +198,10 @@ struct Plan {
     size_t meta_bytes = 0;
     std::vector<std::pair<int, hipGraphExec_t>> graphs;   // per batch size
     double last_ms = 0.0;
+    // rotation-centre scoring of the plan's images (one graph: persistent BP + scores)
+    std::vector<mr_task> h_tasks;    // host copy of the tasks
+    double* d_scores = nullptr;
+    hipGraphExec_t cor_exec = nullptr;
 };
 
 int ensure(void** p, size_t* have, size_t need);   // device buffer growth

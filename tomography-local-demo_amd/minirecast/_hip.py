@@ -64,6 +64,7 @@ SIGNATURES = [
     ("mr_plan_destroy", _c.c_int, [_P]),
     ("mr_plan_run", _c.c_int, [_P, _c.c_int, _P]),
     ("mr_plan_stream", _c.c_int, [_P, _P, _c.c_int, _P, _P]),
+    ("mr_plan_cor_scores", _c.c_int, [_P, _c.c_int, _P]),
     ("mr_plan_output", _c.c_int, [_P, _c.POINTER(_P), _c.POINTER(_c.c_int64)]),
     ("mr_plan_last_ms", _c.c_int, [_P, _c.POINTER(_c.c_double)]),
     ("mr_filter_rows", _c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_double, _c.c_int, _P]),
@@ -490,6 +491,7 @@ class Plan:
         check(self.lib.mr_plan_create(ctx.h, ptr(T), len(T), int(out_len), ctypes.byref(h)))
         self.h = h
         self.out_len = int(out_len)
+        self.ntasks = len(T)
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
@@ -513,6 +515,13 @@ class Plan:
         out = np.empty(self.out_len, dtype=np.float32) if copy else None
         check(self.lib.mr_plan_stream(self.h, ptr(idx), len(idx), ptr(blk), ptr(out)))
         return out
+
+    def cor_scores(self, n_views=-1):
+        """Entropy scores of the plan's trial images (one graph: persistent launch +
+        device scoring), from the first n_views stored views (-1: all)."""
+        sc = np.empty(self.ntasks, np.float64)
+        check(self.lib.mr_plan_cor_scores(self.h, int(n_views), ptr(sc)))
+        return sc
 
     def output_ptr(self):
         p = ctypes.c_void_p()
