@@ -92,26 +92,36 @@ __device__ void make_record(const ViewGeom* __restrict__ gp, float vs, const mr_
     double rowc = (V0 + hs * Va + hp * Vb) / wc + cr;
     colc = fmin(fmax(colc, -static_cast<double>(C)), 2.0 * C);   // NaN -> -C
     rowc = fmin(fmax(rowc, -static_cast<double>(R)), 2.0 * R);
-    const double cref = floor(colc), rref = floor(rowc);
+    const double cref = floor(colc);
+    // band ring: the reference row is the last row before a band start (padded row
+    // rref + 1 = 3 k), and rows are counted in bands, so that the band of a point is
+    // floor(row3) + k and its row within the band floor(3 fract(row3))
+    const double kb = floor((rowc + 1.0) / 3.0);
+    const double rref = band ? 3.0 * kb - 1.0 : floor(rowc);
     const double dc = cc - cref, dr = cr - rref;
     U0 += dc * W0; Ua += dc * Wa; Ub += dc * Wb;
     V0 += dr * W0; Va += dr * Wa; Vb += dr * Wb;
     if (ref) *ref = RecRef{cref, rref, U0, Ua, Ub, V0, Va, Vb, W0, Wa, Wb};
     // cone: the weight 1/wh^2 absorbs the view's storage scale
     const double f = cone ? 1.0 / sqrt(static_cast<double>(vs)) : 1.0;
+    const double rs = band ? 1.0 / 3.0 : 1.0;
     float4* r4 = reinterpret_cast<float4*>(rec);
     r4[0] = make_float4(static_cast<float>(U0 * f), static_cast<float>(Ua * f),
-                        static_cast<float>(Ub * f), static_cast<float>(V0 * f));
-    r4[1] = make_float4(static_cast<float>(Va * f), static_cast<float>(Vb * f),
+                        static_cast<float>(Ub * f), static_cast<float>(V0 * f * rs));
+    r4[1] = make_float4(static_cast<float>(Va * f * rs), static_cast<float>(Vb * f * rs),
                         static_cast<float>(W0 * f), static_cast<float>(Wa * f));
-    // [14]: element offset of the padded reference position; for the band ring instead
-    // the padded reference row (low 16 bits) and column (high 16 bits), both signed
-    const int base = band ? static_cast<int>((static_cast<uint32_t>(static_cast<int>(rref) + 1) & 0xffffu) |
-                                             (static_cast<uint32_t>(static_cast<int>(cref) + 1) << 16))
+    // [14]: element offset of the padded reference position; for the band ring the byte
+    // offset of (band k, padded reference column)
+    const int base = band ? static_cast<int>(kb) * (pitch << 3) + ((static_cast<int>(cref) + 1) << 3)
                           : (static_cast<int>(rref) + 1) * pitch + (static_cast<int>(cref) + 1);
+    // row clamp [-1, R]; in bands [-k, (R + 0.5 - rref) / 3]: the lower bound is exact,
+    // and the upper one lands half a row inside the two zero rows R + 1, R + 2 (padded),
+    // so a clamped point reads only zeros although the bound is rounded
+    const double rlo = band ? -kb : -1.0 - rref;
+    const double rhi = band ? (R + 0.5 - rref) * rs : R - rref;
     r4[2] = make_float4(static_cast<float>(Wb * f), static_cast<float>(-1.0 - cref),
-                        static_cast<float>(C - cref), static_cast<float>(-1.0 - rref));
-    r4[3] = make_float4(static_cast<float>(R - rref), vs, __int_as_float(base),
+                        static_cast<float>(C - cref), static_cast<float>(rlo));
+    r4[3] = make_float4(static_cast<float>(rhi), vs, __int_as_float(base),
                         static_cast<float>(g.uvv));
 }
 
@@ -259,6 +269,16 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel(
 __device__ inline int cvt_flr(float x) {
     int r;
     asm volatile("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+// Byte offset b * pitchb + c * 8 + base of band-ring element (b, c) as two signed 24-bit
+// multiply-adds (b and c are relative to the record's reference and may be negative,
+// |b|, |c| < 2^23); the compiler would otherwise form a multiply, a shift and a
+// three-input add.
+__device__ inline int band_offset(int b, int c, int pitchb, int base) {
+    int x, r;
+    asm("v_mad_i32_i24 %0, %1, 8, %2" : "=v"(x) : "v"(c), "s"(base));
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(b), "s"(pitchb), "v"(x));
     return r;
 }
 
@@ -982,14 +1002,14 @@ void bp_tiles_kernel4(
             const float dsm = ds + static_cast<float>(M::ds(m));
             float row = fmaf(dp, rt[5], fmaf(dsm, rt[4], rt[3]));
             row = __builtin_amdgcn_fmed3f(row, rt[11], rt[12]);
-            frp[m] = __builtin_amdgcn_fractf(row);
             if (RING == 2) {
-                // band offset and 16 j of the padded row (the same for every view)
-                const int rp = cvt_flr(row) + static_cast<int>(static_cast<int16_t>(__float_as_int(rt[14]) & 0xffff));
-                const uint32_t pr = __umul24(static_cast<uint32_t>(rp), 43691u);
-                roffp[m] = __mul24(static_cast<int>(pr >> 17), pitchb);
-                jsh[m] = static_cast<int>((pr >> 11) & 0x30u);
+                // band ring: row in bands; band offset and 16 j (the same for every view)
+                const float t3 = 3.0f * __builtin_amdgcn_fractf(row);
+                frp[m] = __builtin_amdgcn_fractf(t3);
+                roffp[m] = __mul24(cvt_flr(row), pitchb);
+                jsh[m] = cvt_flr(t3) << 4;
             } else {
+                frp[m] = __builtin_amdgcn_fractf(row);
                 roffp[m] = cvt_flr(row) * pitchb;
             }
         }
@@ -1006,10 +1026,9 @@ void bp_tiles_kernel4(
     auto issue = [&](int v, PointState<ST, PPT, RING>& st) {
         const float* r = rt + static_cast<int64_t>(v) * BP_REC;
         const float U0 = r[0], Ua = r[1], Ub = r[2];
-        // band ring: [14] packs the padded reference row and column (signed 16-bit halves)
+        // band ring: [14] is the byte offset of the reference band and column
         const int ref14 = __float_as_int(r[14]);
-        const int rpref = static_cast<int>(static_cast<int16_t>(ref14 & 0xffff));
-        const int baseb = RING == 2 ? (ref14 >> 16) << 3 : ref14 << SH;
+        const int baseb = RING == 2 ? ref14 : ref14 << SH;
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<ST*>(ring + static_cast<int64_t>(v) * view_stride), 0, view_bytes, 0x00020000);
         const float uh0 = fmaf(dp0, Ub, fmaf(ds, Ua, U0));
@@ -1027,15 +1046,15 @@ void bp_tiles_kernel4(
                 const float row = __builtin_amdgcn_fmed3f(vh * rw, r[11], r[12]);
                 st.wt[m] = rw * rw;
                 st.fc[m] = __builtin_amdgcn_fractf(col);
-                st.fr[m] = __builtin_amdgcn_fractf(row);
                 if (RING == 2) {
-                    // padded row rp = 3 b + j (rp < 98304, so rp / 3 is the 17-bit shift)
-                    // padded row rp = 3 b + j: with p = rp * 43691, b = p >> 17 and
-                    // j = (p >> 15) & 3, exact for rp < 32768
-                    const uint32_t pr = __umul24(static_cast<uint32_t>(cvt_flr(row) + rpref), 43691u);
-                    const int voff = __mul24(static_cast<int>(pr >> 17), pitchb) + ((cvt_flr(col) << 3) + baseb);
-                    st.tap[m].load(rs, voff, static_cast<int>((pr >> 11) & 0x30u));
+                    // row in bands (3 rows) from the reference band: band floor(row),
+                    // row in the band j = floor(t3), fraction fract(t3), t3 = 3 fract(row)
+                    const float t3 = 3.0f * __builtin_amdgcn_fractf(row);
+                    st.fr[m] = __builtin_amdgcn_fractf(t3);
+                    st.tap[m].load(rs, band_offset(cvt_flr(row), cvt_flr(col), pitchb, baseb),
+                                   cvt_flr(t3) << 4);
                 } else {
+                    st.fr[m] = __builtin_amdgcn_fractf(row);
                     const int voff = __mul24(cvt_flr(row), pitchb) + ((cvt_flr(col) << SH) + baseb);
                     st.tap[m].load(rs, voff, pitchb);
                 }
@@ -2033,8 +2052,11 @@ int launch_bp_tiles_v4(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int 
         int rc = ensure_pair_ring(c, n_views, s);
         if (rc) return rc;
     } else if (c->bp_variant == 7) {
-        if (c->R + 1 >= 32768) {
-            // the band index arithmetic of kernel 7 is exact for padded rows below 32768
+        if (band_view_stride(c) * static_cast<int64_t>(sizeof(uint16_t)) >= (int64_t{1} << 30) ||
+            (static_cast<int64_t>(c->pitch) << 3) >= (int64_t{1} << 23)) {
+            // kernel 7 addresses a view's bands with 32-bit byte offsets (a record's
+            // reference band can lie up to a view beyond the last) and multiplies by the
+            // band pitch in 24 bits
             const int sv = c->bp_variant;
             c->bp_variant = 6;
             const int rc = launch_bp_tiles_v4(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s);
