@@ -93,7 +93,7 @@ def test_plan_stream_graph_replays(workload, storage, monkeypatch):
     ("2", "4", "0", "2"), ("4", "4", "0", "2"), ("4", "2", "0", "2"), ("4", "2", "1", "3"),
     ("4", "1", "1", "2"), ("4", "1", "0", "3"), ("4", "4", "1", "3"), ("5", "4", "0", "2"),
     ("5", "2", "0", "2"), ("6", "2", "0", "2"), ("6", "1", "1", "3"), ("6", "4", "0", "2"),
-    ("7", "2", "0", "2"), ("7", "1", "1", "3"), ("7", "4", "0", "2")])
+    ("7", "2", "0", "2"), ("7", "2", "0", "3"), ("7", "1", "1", "3"), ("7", "4", "0", "2")])
 def test_bp_variants_full_check(workload, variant, ppt, layout, depth, monkeypatch):
     """Every tile-kernel configuration (kernel, points per thread, lane layout, views in
     flight) against the plain v1 kernel on all outputs, for view counts that leave

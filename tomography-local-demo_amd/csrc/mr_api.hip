@@ -128,12 +128,13 @@ int alloc_ring(Ctx* c, int n_total, int rows, int cols, int window, int storage)
     c->N = fft_length(cols);
     if (!c->bp_explicit) {
         // measured defaults: 16-bit forms read the band ring (kernel 7, 2 points per
-        // thread); f32 reads the ring directly (kernel 4, 2 points per thread)
+        // thread, 3 views in flight); f32 reads the ring directly (kernel 4, 2 points per
+        // thread, 2 views in flight)
         const bool f32 = storage == MR_STORE_F32;
         c->bp_variant = f32 ? 4 : 7;
         c->bp_ppt = 2;
         c->bp_layout = 0;
-        c->bp_depth = 2;
+        c->bp_depth = f32 ? 2 : 3;
         c->bp_sync = f32 ? 0 : 1;
     }
     // measured: cone updates that fill the device run fastest in launches of 32 (f32) or

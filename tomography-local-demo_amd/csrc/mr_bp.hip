@@ -936,15 +936,16 @@ template <int PPT, int LAY> struct Map4 {
     static constexpr int ds(int m) { return LAY == 0 ? 0 : (PPT == 4 ? 16 * (m & 1) : 16 * m); }
 };
 
-// 16-bit forms, 2 views in flight: at least 8 waves per SIMD (the 4-point kernel then fits
-// 59 VGPRs without spills instead of 72, i.e. 7 waves); the others keep the compiler's
-// choice (they would spill).
+// 16-bit forms with 2 views in flight, and the 2-point band kernel with 3: at least 8
+// waves per SIMD (the row-pair 4-point kernel then fits 59 VGPRs without spills instead of
+// 72, the 2-point band kernel with 3 views 61 instead of 69, i.e. 7 waves); the others
+// keep the compiler's choice (they would spill).
 //
 // Cone beam, views [vb, ve) of n_views: a launch with vb > 0 resumes from the unscaled
 // FP32 sums the previous launch left in `out`, and one with ve < n_views leaves them
 // there, so an update split into several launches equals the single launch bit for bit.
 template <bool CONE, typename ST, int PPT, int LAY, int DEPTH, int RING, bool SYNC = false>
-__global__ __launch_bounds__(BP_THREADS, (sizeof(ST) == 2 && DEPTH == 2 && (RING < 2 || PPT < 4)) ? 8 : 1)
+__global__ __launch_bounds__(BP_THREADS, (sizeof(ST) == 2 && ((DEPTH == 2 && (RING < 2 || PPT < 4)) || (RING == 2 && PPT == 2))) ? 8 : 1)
 void bp_tiles_kernel4(
     const ST* __restrict__ ring, int64_t view_stride, int pitch,
     const float* __restrict__ recs, const mr_task* __restrict__ tasks,
@@ -1759,13 +1760,16 @@ int launch_bp_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int nti
         if (!c->bp_explicit && c->bp_variant == 7 && !c->req_slices) {
             // default configuration, volume request (stacks of z-planes): a z-plane tile
             // touches only a few detector rows per view, where the row-pair ring (32
-            // columns per 128-byte line) beats the band ring (16): kernel 6, 4 points
-            const int sv = c->bp_variant, sp = c->bp_ppt;
+            // columns per 128-byte line) beats the band ring (16): kernel 6, 4 points,
+            // 2 views in flight
+            const int sv = c->bp_variant, sp = c->bp_ppt, sd = c->bp_depth;
             c->bp_variant = 6;
             c->bp_ppt = 4;
+            c->bp_depth = 2;
             const int rc = launch_bp_tiles_v4(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s);
             c->bp_variant = sv;
             c->bp_ppt = sp;
+            c->bp_depth = sd;
             return rc;
         }
         return launch_bp_tiles_v4(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s);
