@@ -17,11 +17,12 @@ Reported (one JSON line on stdout):
              tap footprints over all requested points, per view, x bytes per sample,
              plus the output) / the duration of one update's tile-kernel launches (HIP
              events around them on their stream; an update that fills the device runs
-             as one launch per block of 64 views), against 8 TB/s; traffic = measured
-             HBM bytes per update (all its tile-kernel dispatches) from two rocprofv3
-             --pmc passes (FETCH_SIZE, WRITE_SIZE) of a short run of the same f16
-             update, each scaled by the counter's reading for a 1 GiB device copy (null
-             when rocprofv3 is unavailable or --no-traffic).
+             as one launch per block of 64 views, so per launch this is the same rate),
+             against 8 TB/s; traffic = measured HBM bytes per launch (mean per tile-kernel
+             dispatch; traffic_per_update = x the launches of one update) from two
+             rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) of a short run of the same
+             f16 update, each scaled by the counter's reading for a 1 GiB device copy
+             (null when rocprofv3 is unavailable or --no-traffic).
   cpu_baseline  the Numba-equivalent CPU restatement (oracle/liboracle_fast.so:
              fast-math, OpenMP over points, views serial, float64 accumulation),
              timed on this host on a bounded sample of the same update.
@@ -80,9 +81,9 @@ def footprint_pixels(geom, pts, views):
 
 def _pmc_pass(counter):
     """One rocprofv3 --pmc pass of a short f16 run of this benchmark (child process):
-    counter value per update of the f16 cone tile kernel (mean per dispatch x the
-    dispatches of one update, which the child reports), and the value for the 1 GiB copy
-    run first by the child."""
+    the counter's mean per dispatch of the f16 cone tile kernel, the dispatches of one
+    update (which the child reports), and the value for the 1 GiB copy run first by the
+    child."""
     out = tempfile.mkdtemp(prefix="mr_pmc_")
     cmd = ["rocprofv3", "--pmc", counter, "-d", out, "-o", "run", "--output-format", "csv",
            "--", sys.executable, "-u", os.path.abspath(__file__), "--pmc-child", "--steps", "2",
@@ -112,7 +113,7 @@ def _pmc_pass(counter):
             for ln in fh:
                 if "tile launches per update:" in ln:
                     launches = int(ln.rsplit(":", 1)[1])
-        return float(np.mean(bp)) * launches, float(max(cp))
+        return float(np.mean(bp)), launches, float(max(cp))
     except (OSError, subprocess.SubprocessError, ValueError, KeyError):
         return None
     finally:
@@ -120,8 +121,9 @@ def _pmc_pass(counter):
 
 
 def pmc_traffic():
-    """HBM bytes per launch of the f16 tile kernel: FETCH_SIZE and WRITE_SIZE (two
-    passes, one counter group each), each scaled by its reading for the 1 GiB copy."""
+    """HBM bytes per launch (mean per dispatch) of the f16 tile kernel, and the launches
+    of one update: FETCH_SIZE and WRITE_SIZE (two passes, one counter group each), each
+    scaled by its reading for the 1 GiB copy."""
     if shutil.which("rocprofv3") is None:
         log("traffic: rocprofv3 not found")
         return None
@@ -131,10 +133,11 @@ def pmc_traffic():
         if r is None:
             log(f"traffic: {counter} pass gave no reading")
             return None
-        bp, copy = r
+        bp, launches, copy = r
         scale = float(1 << 30) / copy            # bytes per counter unit
         res[key] = bp * scale
         res[key + "_bytes_per_unit"] = round(scale, 3)
+        res["launches_per_update"] = launches
     log(f"traffic: {res}")
     return res
 
@@ -625,15 +628,22 @@ def main():
     esz = 2 if head in ("f16", "bf16") else 4
     hbm_bytes = F * A * esz + out_len * 4
     kern_s = results[head]["tile_kernel_ms"] * 1e-3
-    achieved = hbm_bytes / kern_s / 1e9
+    launches = max(1, int(results[head].get("tile_launches", 1)))
+    # per launch: the update's algorithmic bytes and tile-kernel time over its launches
+    # (blocks of views), i.e. the same rate as for the whole update
+    achieved = (hbm_bytes / launches) / (kern_s / launches) / 1e9
     traffic = None
     if not args.no_traffic and not args.pmc_child and head == "f16":
         traffic = pmc_traffic()
+    tl = int(traffic["fetch"] + traffic["write"]) if traffic else None
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4),
-                    traffic=(int(traffic["fetch"] + traffic["write"]) if traffic else None),
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=tl,
                     kernel=f"bp_tiles_kernel4 (cone, {head})",
-                    algorithmic_bytes=int(hbm_bytes), footprint_px_per_view=round(F),
+                    launches_per_update=launches,
+                    algorithmic_bytes_per_launch=int(hbm_bytes / launches),
+                    algorithmic_bytes=int(hbm_bytes),
+                    traffic_per_update=(int(tl * launches) if tl else None),
+                    footprint_px_per_view=round(F),
                     gather_bytes_l2=int(4 * esz * n_in * A))
     if traffic:
         roofline["traffic_detail"] = traffic
