@@ -204,11 +204,12 @@ def test_pair_ring_follows_appends_and_resets(workload, variant, monkeypatch):
     assert rel_l2(ctx.bp_tasks(T, A, out_len), ref_ctx.bp_tasks(T, A, out_len)) < 1e-6
 
 
-@pytest.mark.parametrize("rows", [37, 38, 39])
+@pytest.mark.parametrize("rows", [1, 2, 4, 37, 38, 39])
 def test_band_ring_rows_and_borders(rows, monkeypatch):
     """Kernel 7 (band ring) against kernel 1 on the same 16-bit ring, for detector heights
-    of every residue mod 3, cone and parallel beam: oblique slices, and a volume grid 1.6x
-    the box whose outer points project off the detector (the zero border and clamps)."""
+    of every residue mod 3 (down to single-row detectors), cone and parallel beam: oblique
+    slices, and a volume grid over 1.6x the box across and 4x along z whose outer points
+    project off the detector (the zero border and clamps)."""
     from minirecast import _hip, phantom, recon
     from minirecast.geometry import ParallelGeometry, circular_cone_vectors
     C = 50
@@ -225,7 +226,8 @@ def test_band_ring_rows_and_borders(rows, monkeypatch):
         for normal, n in (((1.0, 0.5, 0.8), 61), ((0.2, -0.1, 1.0), 45)):
             tasks.append(recon.plane_task(box, (0.0, 0.0, 0.3), normal, n, out_offset=off))
             off += n * n
-        dims, origin, h = recon.volume_grid(1.6 * np.asarray(box), 24)
+        # 1.6x the box across, 4x along z: the outer z-planes project off the detector
+        dims, origin, h = recon.volume_grid(np.asarray(box) * (1.6, 1.6, 4.0), 24)
         vt = recon.volume_task(origin, h, dims, 0, int(dims[2]))
         vt['out_offset'] = off
         tasks.append(vt)

@@ -94,8 +94,8 @@ __device__ void make_record(const ViewGeom* __restrict__ gp, float vs, const mr_
     rowc = fmin(fmax(rowc, -static_cast<double>(R)), 2.0 * R);
     const double cref = floor(colc);
     // band ring: the reference row is the last row before a band start (padded row
-    // rref + 1 = 3 k), and rows are counted in bands, so that the band of a point is
-    // floor(row3) + k and its row within the band floor(3 fract(row3))
+    // rref + 1 = 3 k), so a point whose row is r rows past it (r = floor(row)) lies in
+    // band k + floor(r / 3) at row r mod 3 of the band
     const double kb = floor((rowc + 1.0) / 3.0);
     const double rref = band ? 3.0 * kb - 1.0 : floor(rowc);
     const double dc = cc - cref, dr = cr - rref;
@@ -104,24 +104,18 @@ __device__ void make_record(const ViewGeom* __restrict__ gp, float vs, const mr_
     if (ref) *ref = RecRef{cref, rref, U0, Ua, Ub, V0, Va, Vb, W0, Wa, Wb};
     // cone: the weight 1/wh^2 absorbs the view's storage scale
     const double f = cone ? 1.0 / sqrt(static_cast<double>(vs)) : 1.0;
-    const double rs = band ? 1.0 / 3.0 : 1.0;
     float4* r4 = reinterpret_cast<float4*>(rec);
     r4[0] = make_float4(static_cast<float>(U0 * f), static_cast<float>(Ua * f),
-                        static_cast<float>(Ub * f), static_cast<float>(V0 * f * rs));
-    r4[1] = make_float4(static_cast<float>(Va * f * rs), static_cast<float>(Vb * f * rs),
+                        static_cast<float>(Ub * f), static_cast<float>(V0 * f));
+    r4[1] = make_float4(static_cast<float>(Va * f), static_cast<float>(Vb * f),
                         static_cast<float>(W0 * f), static_cast<float>(Wa * f));
     // [14]: element offset of the padded reference position; for the band ring the byte
     // offset of (band k, padded reference column)
     const int base = band ? static_cast<int>(kb) * (pitch << 3) + ((static_cast<int>(cref) + 1) << 3)
                           : (static_cast<int>(rref) + 1) * pitch + (static_cast<int>(cref) + 1);
-    // row clamp [-1, R]; in bands [-k, (R + 0.5 - rref) / 3]: the lower bound is exact,
-    // and the upper one lands half a row inside the two zero rows R + 1, R + 2 (padded),
-    // so a clamped point reads only zeros although the bound is rounded
-    const double rlo = band ? -kb : -1.0 - rref;
-    const double rhi = band ? (R + 0.5 - rref) * rs : R - rref;
     r4[2] = make_float4(static_cast<float>(Wb * f), static_cast<float>(-1.0 - cref),
-                        static_cast<float>(C - cref), static_cast<float>(rlo));
-    r4[3] = make_float4(static_cast<float>(rhi), vs, __int_as_float(base),
+                        static_cast<float>(C - cref), static_cast<float>(-1.0 - rref));
+    r4[3] = make_float4(static_cast<float>(R - rref), vs, __int_as_float(base),
                         static_cast<float>(g.uvv));
 }
 
@@ -270,6 +264,13 @@ __device__ inline int cvt_flr(float x) {
     int r;
     asm volatile("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
     return r;
+}
+// Row r (relative to a band start, |r| < 2^22) -> band b = floor(r / 3) and 16 (r - 3 b),
+// the shift that selects rows r, r + 1 of the band's four.  (r + 0.5) / 3 lies at least
+// 1/6 from an integer, so its float floor is exact.
+__device__ inline void band_split(int r, int& b, int& jsh16) {
+    b = cvt_flr(fmaf(static_cast<float>(r), 1.0f / 3.0f, 1.0f / 6.0f));
+    jsh16 = __mul24(b, -48) + (r << 4);
 }
 // Byte offset b * pitchb + c * 8 + base of band-ring element (b, c) as two signed 24-bit
 // multiply-adds (b and c are relative to the record's reference and may be negative,
@@ -1003,14 +1004,14 @@ void bp_tiles_kernel4(
             const float dsm = ds + static_cast<float>(M::ds(m));
             float row = fmaf(dp, rt[5], fmaf(dsm, rt[4], rt[3]));
             row = __builtin_amdgcn_fmed3f(row, rt[11], rt[12]);
+            frp[m] = __builtin_amdgcn_fractf(row);
             if (RING == 2) {
-                // band ring: row in bands; band offset and 16 j (the same for every view)
-                const float t3 = 3.0f * __builtin_amdgcn_fractf(row);
-                frp[m] = __builtin_amdgcn_fractf(t3);
-                roffp[m] = __mul24(cvt_flr(row), pitchb);
-                jsh[m] = cvt_flr(t3) << 4;
+                // band ring: band offset and 16 j of the row (the same for every view)
+                int b, jsh16;
+                band_split(cvt_flr(row), b, jsh16);
+                roffp[m] = __mul24(b, pitchb);
+                jsh[m] = jsh16;
             } else {
-                frp[m] = __builtin_amdgcn_fractf(row);
                 roffp[m] = cvt_flr(row) * pitchb;
             }
         }
@@ -1047,15 +1048,12 @@ void bp_tiles_kernel4(
                 const float row = __builtin_amdgcn_fmed3f(vh * rw, r[11], r[12]);
                 st.wt[m] = rw * rw;
                 st.fc[m] = __builtin_amdgcn_fractf(col);
+                st.fr[m] = __builtin_amdgcn_fractf(row);
                 if (RING == 2) {
-                    // row in bands (3 rows) from the reference band: band floor(row),
-                    // row in the band j = floor(t3), fraction fract(t3), t3 = 3 fract(row)
-                    const float t3 = 3.0f * __builtin_amdgcn_fractf(row);
-                    st.fr[m] = __builtin_amdgcn_fractf(t3);
-                    st.tap[m].load(rs, band_offset(cvt_flr(row), cvt_flr(col), pitchb, baseb),
-                                   cvt_flr(t3) << 4);
+                    int b, jsh16;
+                    band_split(cvt_flr(row), b, jsh16);
+                    st.tap[m].load(rs, band_offset(b, cvt_flr(col), pitchb, baseb), jsh16);
                 } else {
-                    st.fr[m] = __builtin_amdgcn_fractf(row);
                     const int voff = __mul24(cvt_flr(row), pitchb) + ((cvt_flr(col) << SH) + baseb);
                     st.tap[m].load(rs, voff, pitchb);
                 }
