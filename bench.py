@@ -223,8 +223,11 @@ def bench_stream_cone(args):
     """Streaming ingest: 600 views in interleaved batches of 10 (Poisson noise, 384 x 486
     detector); after every batch the three oblique 512^2 slices and the 64^3 preview
     are brought up to date.  Graph path: one mr_plan whose per-batch update (upload,
-    filter, persistent back-projection of all tasks from the device work queue) is a
-    captured hipGraph replayed per batch; eager path: SliceReconstructor calls."""
+    filter, persistent back-projection of the batch's new views into the running FP32
+    sums of all tasks, from the device work queue) is a captured hipGraph replayed per
+    batch; eager path: SliceReconstructor calls (every request back-projects all views
+    received so far, as the reference does).  The line's value is the median ms per
+    batch of the graph path."""
     from minirecast import _hip, phantom, recon, workloads
     from minirecast.stream import arrival_order
     w = workloads.stream_cone()
@@ -249,7 +252,9 @@ def bench_stream_cone(args):
     batch = w.extra["batch"]
     blocks = [(order[i:i + batch], np.ascontiguousarray(proj[:, order[i:i + batch], :]))
               for i in range(0, A, batch)]
+    # a full pass per batch (the reference's requests) and each view once (running sums)
     updates = sum(min((b + 1) * batch, A) * (n_in + nvox) for b in range(len(blocks)))
+    updates_once = A * (n_in + nvox)
 
     def run(storage):
         # graph path
@@ -281,7 +286,9 @@ def bench_stream_cone(args):
                    eager_ms_per_batch_median=float(np.median(eager_ms)),
                    eager_ms_last_batch=float(eager_ms[-1]), graph_vs_eager_rel_l2=err,
                    batches=len(blocks), first_batch_ms=float(wall_ms[0]),
-                   gups=updates / (sum(wall_ms) * 1e-3) / 1e9)
+                   gups_full_pass_equivalent=updates / (sum(wall_ms) * 1e-3) / 1e9,
+                   gups_computed=updates_once / (sum(wall_ms) * 1e-3) / 1e9,
+                   eager_gups=updates / (sum(eager_ms) * 1e-3) / 1e9)
         return res, ref
 
     res, ref32 = run("f32")
@@ -290,13 +297,13 @@ def bench_stream_cone(args):
     res["f16"] = res16
     log(f"stream_cone: {res}")
     _line("ms per streamed batch (10 views ingested + 3 oblique 512^2 slices + 64^3 preview)",
-          round(res["gups"], 3), "GUPS", round(res["graph_ms_per_batch_median"], 4),
+          round(res["graph_ms_per_batch_median"], 4), "ms", round(res["graph_ms_per_batch_median"], 4),
           len(blocks), 0,
           {"workload": "stream_cone: 600 views in 60 interleaved batches of 10, 384x486 "
                        "detector, 3 oblique 512^2 slices + 64^3 preview per batch",
-           "path": "mr_plan hipGraph per batch (upload + filter + persistent BP), f32 ring; "
-                   "f16 ring under stream.f16"},
-          {"stream": res})
+           "path": "mr_plan hipGraph per batch (upload + filter + persistent BP of the new "
+                   "views into running sums), f32 ring; f16 ring under stream.f16"},
+          {"stream": res}, higher=False)
 
 
 def bench_parallel_cor(args):

@@ -320,3 +320,65 @@ def test_plan_cor_scores_graph(workload, storage, monkeypatch):
     bad = _hip.Plan(ctx, np.array(tasks, dtype=_hip.TASK_DTYPE), out_len)
     with pytest.raises(ValueError):
         bad.cor_scores()
+
+
+@pytest.mark.parametrize("beam", ["cone", "parallel"])
+def test_plan_stream_running_sums_are_exact(workload, beam):
+    """A streaming plan keeps the unscaled FP32 sums and adds only each batch's new views:
+    every output equals a full pass over all stored views bit for bit, also after views
+    pushed by another ingest path (the sums catch up), after the views are reset and
+    refilled (the sums restart), and for both storage forms.  A plan refuses to run once
+    the context's geometry has been set again."""
+    from minirecast import _hip, phantom, recon
+    from minirecast.geometry import ParallelGeometry
+    from minirecast.stream import arrival_order
+    if beam == "cone":
+        w, proj, planes, tasks, out_len = workload
+        g = w.geom
+        T = np.array(tasks, dtype=_hip.TASK_DTYPE)
+    else:
+        g = ParallelGeometry(np.arange(60) * np.pi / 60, 24, 80, 1.0, 1.0)
+        box = g.volume_box()
+        ph = phantom.random_ellipsoids(5, 8, box, (0.1, 1.0))
+        proj = phantom.project_parallel(ph, g)
+        T, off = [], 0
+        for nrm, n in (((1.0, 0.5, 0.8), 70), ((0.0, 0.0, 1.0), 40)):
+            T.append(recon.plane_task(box, (1.0, 2.0, 3.0), nrm, n, cor_shift=0.6, out_offset=off))
+            off += n * n
+        T, out_len = np.array(T, dtype=_hip.TASK_DTYPE), off
+    A = g.n_angles
+    order = arrival_order(A, "interleaved")
+    batches = [order[i:i + 7] for i in range(0, A, 7)]
+    for storage in ("f32", "f16"):
+        ctx = _hip.Context(0)
+        if beam == "cone":
+            ctx.set_cone(g.vectors, g.rows, g.cols, "ram-lak", storage)
+        else:
+            ctx.set_parallel(g.angles, g.rows, g.cols, 1.0, 1.0, "ram-lak", storage)
+        plan = _hip.Plan(ctx, T, out_len)
+        h = len(batches) // 2
+        for bi, idx in enumerate(batches[:h]):
+            got = plan.stream(idx, np.ascontiguousarray(proj[:, idx, :]))
+            np.testing.assert_array_equal(got, plan.run(), err_msg=f"{storage} batch {bi}")
+        # views from another ingest path, then streaming again
+        idx = batches[h]
+        ctx.push(idx, np.ascontiguousarray(proj[:, idx, :]))
+        for idx in batches[h + 1:]:
+            got = plan.stream(idx, np.ascontiguousarray(proj[:, idx, :]))
+            np.testing.assert_array_equal(got, plan.run())
+        assert ctx.num_views() == A
+        # reset and refill with other data: the sums restart from zero
+        ctx.reset_views()
+        other = proj[:, ::-1].copy()
+        for idx in batches[:3]:
+            got = plan.stream(idx, np.ascontiguousarray(other[:, idx, :]))
+            np.testing.assert_array_equal(got, plan.run())
+        if beam == "cone":
+            ctx.set_cone(g.vectors, g.rows, g.cols, "ram-lak", storage)
+        else:
+            ctx.set_parallel(g.angles, g.rows, g.cols, 1.0, 1.0, "ram-lak", storage)
+        with pytest.raises(RuntimeError):
+            plan.stream(batches[0], np.ascontiguousarray(proj[:, batches[0], :]))
+        with pytest.raises(RuntimeError):
+            plan.run()
+        plan.close()

@@ -98,6 +98,8 @@ void free_geometry(Ctx* c) {
     c->raw = RawCorr{};
     c->kind = MR_KIND_NONE;
     c->n = 0;
+    ++c->geom_epoch;
+    ++c->views_epoch;
 }
 
 // float32 device tables of the ramp response (tau = 1) and FFT twiddles for length N
@@ -1251,7 +1253,7 @@ static void plan_free(Plan* p) {
     if (p->cor_exec) hipGraphExecDestroy(p->cor_exec);
     p->cor_exec = nullptr;
     void* dev[] = {p->d_tasks, p->d_toff, p->d_queue, p->d_counters, p->d_out, p->d_block, p->d_meta,
-                   p->d_scores};
+                   p->d_scores, p->d_acc};
     for (void* d : dev)
         if (d) hipFree(d);
     if (p->h_block) hipHostFree(p->h_block);
@@ -1277,6 +1279,7 @@ int mr_plan_create(mr_ctx* c, const mr_task* tasks, int ntasks, int64_t out_len,
     p->ntasks = ntasks;
     p->out_len = out_len;
     p->h_tasks.assign(tasks, tasks + ntasks);
+    p->geom_epoch = c->geom_epoch;
     std::vector<int64_t> toff(ntasks + 1, 0);
     for (int i = 0; i < ntasks; ++i) toff[i + 1] = toff[i] + task_candidates(tasks[i]);
     p->total_candidates = toff[ntasks];
@@ -1338,6 +1341,7 @@ int mr_plan_run(mr_plan* p, int n_views, float* out_host) {
     MR_CHECK(p != nullptr, MR_EVALUE, "null plan");
     Ctx* c = p->c;
     std::lock_guard<std::mutex> lk(c->mu);
+    MR_CHECK(p->geom_epoch == c->geom_epoch, MR_ERUNTIME, "the plan was made for an earlier geometry");
     if (n_views < 0) n_views = c->n;
     MR_CHECK(n_views <= c->n, MR_EVALUE, "n_views beyond the stored views");
     MR_HIP(hipSetDevice(c->device));
@@ -1363,6 +1367,7 @@ int mr_plan_cor_scores(mr_plan* p, int n_views, double* scores) {
     MR_CHECK(scores != nullptr, MR_EVALUE, "null scores");
     Ctx* c = p->c;
     std::lock_guard<std::mutex> lk(c->mu);
+    MR_CHECK(p->geom_epoch == c->geom_epoch, MR_ERUNTIME, "the plan was made for an earlier geometry");
     if (n_views < 0) n_views = c->n;
     MR_CHECK(n_views <= c->n, MR_EVALUE, "n_views beyond the stored views");
     // the images must be the contiguous equal-size single-plane outputs the scoring
@@ -1416,6 +1421,7 @@ int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block
     MR_CHECK(p != nullptr, MR_EVALUE, "null plan");
     Ctx* c = p->c;
     std::lock_guard<std::mutex> lk(c->mu);
+    MR_CHECK(p->geom_epoch == c->geom_epoch, MR_ERUNTIME, "the plan was made for an earlier geometry");
     MR_CHECK(k >= 1 && indices && block, MR_EVALUE, "empty or null batch");
     MR_CHECK(c->n + k <= c->n_total, MR_EVALUE, "ring full");
     MR_CHECK(c->gaussian <= 0.0 && std::fabs(c->tilt) < 1e-9, MR_ERUNTIME,
@@ -1456,8 +1462,18 @@ int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block
         fv[j].slot = c->n + j;
         gv[j] = c->geom_all[indices[j]];
     }
-    // the device view count follows the host one (other ingest paths may have run)
-    MR_HIP(hipMemcpyAsync(p->d_counters + 2, &c->n, sizeof(int), hipMemcpyHostToDevice, c->s_req));
+    // running sums: the batch adds views [acc_n, n + k) to them; they restart from zero
+    // when there are none yet or the stored views were replaced since
+    const int64_t acc_len = p->out_len * (c->kind == MR_KIND_PARALLEL ? 2 : 1);
+    if (!p->d_acc) MR_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_acc), sizeof(float) * acc_len));
+    if (p->acc_n < 0 || p->acc_epoch != c->views_epoch || p->acc_n > c->n) {
+        MR_HIP(hipMemsetAsync(p->d_acc, 0, sizeof(float) * acc_len, c->s_req));
+        p->acc_n = 0;
+    }
+    // the device view count follows the host one (other ingest paths may have run), and
+    // the first view of the running sums
+    const int nv_begin[2] = {c->n, p->acc_n};
+    MR_HIP(hipMemcpyAsync(p->d_counters + 2, nv_begin, sizeof(int) * 2, hipMemcpyHostToDevice, c->s_req));
     MR_HIP(hipStreamSynchronize(c->s_req));
     hipGraphExec_t exec = nullptr;
     for (auto& g : p->graphs)
@@ -1468,6 +1484,9 @@ int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block
         q.count = p->d_counters;
         q.n_views = p->d_counters + 2;
         q.grid = p->grid;
+        q.v_begin = p->d_counters + 3;
+        q.accum = p->d_acc;
+        q.accum_half = p->out_len;
         hipGraph_t graph = nullptr;
         MR_HIP(hipStreamBeginCapture(c->s_req, hipStreamCaptureModeThreadLocal));
         hipError_t e1 = hipMemcpyAsync(p->d_block, p->h_block, blk, hipMemcpyHostToDevice, c->s_req);
@@ -1496,6 +1515,8 @@ int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block
     if (rc) return rc;
     p->last_ms = c->last_kernel_ms;
     c->n += k;
+    p->acc_n = c->n;
+    p->acc_epoch = c->views_epoch;
     return MR_OK;
 }
 
@@ -1519,6 +1540,7 @@ int mr_reset_views(mr_ctx* c) {
     c->n = 0;
     c->pring_n = 0;
     c->bring_n = 0;
+    ++c->views_epoch;
     return MR_OK;
 }
 

@@ -392,7 +392,8 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel2(
     const ViewGeom* __restrict__ geom, const float* __restrict__ vscale,
     const mr_task* __restrict__ tasks, const Tile* __restrict__ tiles, int n_views_arg,
     float scale_arg, float* __restrict__ out, int* __restrict__ qhead,
-    const int* __restrict__ qcount, const int* __restrict__ n_views_dev, int xcd) {
+    const int* __restrict__ qcount, const int* __restrict__ n_views_dev, int xcd,
+    const int* __restrict__ v_begin_dev, float* __restrict__ accum, int64_t accum_half) {
     __shared__ float4 rec4[BP_VPASS * BP_REC / 4];
     __shared__ int s_ti;
     float* rec = reinterpret_cast<float*>(rec4);
@@ -402,6 +403,10 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel2(
     const int n_views = n_views_dev ? *n_views_dev : n_views_arg;
     const float scale = n_views_dev ? (n_views > 0 ? static_cast<float>(3.141592653589793 / n_views) : 0.0f)
                                     : scale_arg;
+    // running sums: resume from accum and add views [v_first, n_views) only (the host
+    // guarantees v_first < n_views); the FP32 sums then equal one pass over all views
+    // bit for bit, as each point adds its views in the same order either way
+    const int v_first = accum ? *v_begin_dev : 0;
     const int ntile = qhead ? *qcount : 0;
     for (int it = 0;; ++it) {
     int ti;
@@ -446,14 +451,27 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel2(
     int roff_par[PPT];
 #pragma unroll
     for (int m = 0; m < PPT; ++m) { acc[m] = 0.0f; accb[m] = 0.0f; fr_par[m] = 0.0f; roff_par[m] = 0; }
+    const int64_t obase = T.out_offset + static_cast<int64_t>(tl.q) * T.stride_q;
+    auto oidx = [&](int m) {
+        return obase + static_cast<int64_t>(p0 + ty + PSTEP * m) * T.stride_p +
+               static_cast<int64_t>(s) * T.stride_s;
+    };
+    if (accum) {
+#pragma unroll
+        for (int m = 0; m < PPT; ++m) {
+            if (!valid[m]) continue;
+            acc[m] = accum[oidx(m)];
+            if (!CONE) accb[m] = accum[accum_half + oidx(m)];
+        }
+    }
 
-    for (int vb = 0; vb < n_views; vb += BP_VPASS) {
+    for (int vb = v_first; vb < n_views; vb += BP_VPASS) {
         const int nv = min(BP_VPASS, n_views - vb);
         __syncthreads();
         if (t < nv)
             make_record(geom + vb + t, vscale[vb + t], T, xt, R, C, pitch, CONE, rec + t * BP_REC);
         __syncthreads();
-        if (!CONE && vb == 0) {
+        if (!CONE && vb == v_first) {
             const float4 r0 = rec4[0], r1 = rec4[1], r2 = rec4[2], r3 = rec4[3];
 #pragma unroll
             for (int m = 0; m < PPT; ++m) {
@@ -517,15 +535,16 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel2(
         }
     }
 
-    const int64_t obase = T.out_offset + static_cast<int64_t>(tl.q) * T.stride_q;
 #pragma unroll
     for (int m = 0; m < PPT; ++m) {
         if (!valid[m]) continue;
-        const int p = p0 + ty + PSTEP * m;
+        if (accum) {
+            accum[oidx(m)] = acc[m];
+            if (!CONE) accum[accum_half + oidx(m)] = accb[m];
+        }
         float val = acc[m];
         if (!CONE) val = fmaf(fr_par[m], accb[m] - acc[m], acc[m]);
-        out[obase + static_cast<int64_t>(p) * T.stride_p + static_cast<int64_t>(s) * T.stride_s] =
-            inside[m] ? val * scale : 0.0f;
+        out[oidx(m)] = inside[m] ? val * scale : 0.0f;
     }
     }   // tile loop
 }
@@ -1679,19 +1698,22 @@ int dispatch_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntil
             hipLaunchKernelGGL((bp_tiles_kernel2<CONE, float>), grid, block, 0, s,
                                static_cast<const float*>(c->ring), c->view_stride, c->pitch, c->R,
                                c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out,
-                               q.head, q.count, q.n_views, c->xcd_map != 0);
+                               q.head, q.count, q.n_views, c->xcd_map != 0, q.v_begin, q.accum,
+                               q.accum_half);
             break;
         case MR_STORE_F16:
             hipLaunchKernelGGL((bp_tiles_kernel2<CONE, __half>), grid, block, 0, s,
                                static_cast<const __half*>(c->ring), c->view_stride, c->pitch, c->R,
                                c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out,
-                               q.head, q.count, q.n_views, c->xcd_map != 0);
+                               q.head, q.count, q.n_views, c->xcd_map != 0, q.v_begin, q.accum,
+                               q.accum_half);
             break;
         default:
             hipLaunchKernelGGL((bp_tiles_kernel2<CONE, uint16_t>), grid, block, 0, s,
                                static_cast<const uint16_t*>(c->ring), c->view_stride, c->pitch,
                                c->R, c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale,
-                               d_out, q.head, q.count, q.n_views, c->xcd_map != 0);
+                               d_out, q.head, q.count, q.n_views, c->xcd_map != 0, q.v_begin, q.accum,
+                               q.accum_half);
             break;
         }
         MR_HIP(hipGetLastError());

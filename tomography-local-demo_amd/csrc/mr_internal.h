@@ -68,6 +68,12 @@ struct TileQueue {
     const int* count = nullptr;      // number of queued tiles
     const int* n_views = nullptr;    // view count read on the device (null: argument)
     int grid = 0;                    // persistent grid size
+    // running sums (persistent v2 launches): views [*v_begin, n) are added to the
+    // unscaled FP32 sums in accum (cone: [out_len]; parallel: two rows' sums,
+    // [2][out_len] at accum_half), which are written back; out gets the scaled result
+    const int* v_begin = nullptr;
+    float* accum = nullptr;
+    int64_t accum_half = 0;
 };
 
 // Bands of the band ring (kernel 7): padded rows 0..R+1 can hold a point's upper tap,
@@ -138,6 +144,9 @@ struct Ctx {
     int pitch = 0, Rp = 0;           // padded row pitch (elements) and padded rows
     int64_t view_stride = 0;         // elements per stored view
     int n = 0;                       // views stored
+    // bumped when the geometry is set (geom_epoch, views_epoch) or the stored views are
+    // forgotten (views_epoch): plans check them before reusing graphs or running sums
+    uint64_t geom_epoch = 0, views_epoch = 0;
 
     // host copies
     std::vector<ViewGeom> geom_all;   // per geometry index
@@ -198,6 +207,12 @@ struct Plan {
     size_t meta_bytes = 0;
     std::vector<std::pair<int, hipGraphExec_t>> graphs;   // per batch size
     double last_ms = 0.0;
+    uint64_t geom_epoch = 0;         // context geometry the plan was made for
+    // streaming: unscaled FP32 sums over views [0, acc_n) of the context's ring epoch
+    // acc_epoch (acc_n < 0: none yet), so that a batch adds only its new views
+    float* d_acc = nullptr;
+    int acc_n = -1;
+    uint64_t acc_epoch = 0;
     // rotation-centre scoring of the plan's images (one graph: persistent BP + scores)
     std::vector<mr_task> h_tasks;    // host copy of the tasks
     double* d_scores = nullptr;
