@@ -269,6 +269,14 @@ int mr_plan_run(mr_plan* plan, int n_views, float* out_host);
 int mr_plan_stream(mr_plan* plan, const int64_t* indices, int k, const float* block,
                    float* out_host);
 
+/* Brings the plan's output up to all stored views, however they were ingested
+ * (mr_push_views, mr_push_raw, mr_store_views, mr_plan_stream): the views not yet in the
+ * plan's running FP32 sums are added by one persistent launch (the same bits as
+ * mr_plan_run over all views; they restart after mr_reset_views).  For callers that
+ * re-request the same slices as views arrive (the viewer and server pattern of
+ * recon.py:426-475).  Copies the output to out_host when it is not NULL. */
+int mr_plan_update(mr_plan* plan, float* out_host);
+
 /* Rotation-centre scores (recon.py:498-513) of a plan whose tasks are the trial images
  * of one search: equal-size single planes, contiguous in task order (out_offset =
  * i * np * ns).  Back-projects them from the first n_views stored views (< 0: all) in

@@ -382,3 +382,31 @@ def test_plan_stream_running_sums_are_exact(workload, beam):
         with pytest.raises(RuntimeError):
             plan.run()
         plan.close()
+
+
+def test_live_planes_follow_add_projections(workload):
+    """SliceReconstructor.live_planes: after every batch ingested by add_projections,
+    refresh() equals a full pass of the same plan bit for bit and reconstruct_planes
+    (another kernel) to FP32 rounding; a refresh with no new views returns the same
+    images, and the views of a reset are summed from zero again."""
+    from minirecast import recon
+    from minirecast.stream import arrival_order
+    w, proj, planes, tasks, out_len = workload
+    A = w.geom.n_angles
+    rec = recon.SliceReconstructor(None, w.geom, window=w.window, verbose=False)
+    live = rec.live_planes(planes)
+    order = arrival_order(A, "interleaved")
+    for i in range(0, A, 9):
+        idx = order[i:i + 9]
+        rec.add_projections(idx, np.ascontiguousarray(proj[:, idx, :]))
+        imgs = live.refresh()
+        full = live._plan.run()
+        np.testing.assert_array_equal(np.concatenate([im.ravel() for im in imgs]),
+                                      full.astype(np.float64))
+        ref = rec.reconstruct_planes(planes)
+        for a, b in zip(imgs, ref):
+            assert rel_l2(a, b) < 1e-6
+    again = live.refresh()
+    for a, b in zip(again, imgs):
+        np.testing.assert_array_equal(a, b)
+    live.close()

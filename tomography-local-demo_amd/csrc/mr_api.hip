@@ -1417,6 +1417,55 @@ int mr_plan_cor_scores(mr_plan* p, int n_views, double* scores) {
     return rc;
 }
 
+// Running sums of a plan: allocated on first use; restarted from zero when there are
+// none yet or the stored views were reset or replaced since.  Writes the device view
+// count (it follows the host one: other ingest paths may have run) and the first view
+// not yet summed into the plan's counters [2] and [3].
+static int plan_prepare_sums(Plan* p) {
+    Ctx* c = p->c;
+    const int64_t acc_len = p->out_len * (c->kind == MR_KIND_PARALLEL ? 2 : 1);
+    if (!p->d_acc) MR_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_acc), sizeof(float) * acc_len));
+    if (p->acc_n < 0 || p->acc_epoch != c->views_epoch || p->acc_n > c->n) {
+        MR_HIP(hipMemsetAsync(p->d_acc, 0, sizeof(float) * acc_len, c->s_req));
+        p->acc_n = 0;
+        p->acc_epoch = c->views_epoch;
+    }
+    const int nv_begin[2] = {c->n, p->acc_n};
+    MR_HIP(hipMemcpyAsync(p->d_counters + 2, nv_begin, sizeof(int) * 2, hipMemcpyHostToDevice, c->s_req));
+    MR_HIP(hipStreamSynchronize(c->s_req));
+    return MR_OK;
+}
+
+int mr_plan_update(mr_plan* p, float* out_host) {
+    MR_CHECK(p != nullptr, MR_EVALUE, "null plan");
+    Ctx* c = p->c;
+    std::lock_guard<std::mutex> lk(c->mu);
+    MR_CHECK(p->geom_epoch == c->geom_epoch, MR_ERUNTIME, "the plan was made for an earlier geometry");
+    MR_HIP(hipSetDevice(c->device));
+    int rc = plan_prepare_sums(p);
+    if (rc) return rc;
+    TileQueue q;
+    q.head = p->d_counters + 1;
+    q.count = p->d_counters;
+    q.n_views = p->d_counters + 2;
+    q.grid = p->grid;
+    q.v_begin = p->d_counters + 3;
+    q.accum = p->d_acc;
+    q.accum_half = p->out_len;
+    MR_HIP(hipEventRecord(c->ev0, c->s_req));
+    rc = launch_reset_head(p->d_counters, c->s_req);
+    if (!rc) rc = launch_bp_tiles(c, p->d_tasks, p->d_queue, 0, 0, 0.0f, p->d_out, c->s_req, &q);
+    if (rc) return rc;
+    MR_HIP(hipEventRecord(c->ev1, c->s_req));
+    rc = plan_copy_out(p, out_host);
+    if (rc) return rc;
+    rc = finish_timing(c);
+    if (rc) return rc;
+    p->last_ms = c->last_kernel_ms;
+    p->acc_n = c->n;
+    return MR_OK;
+}
+
 int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block, float* out_host) {
     MR_CHECK(p != nullptr, MR_EVALUE, "null plan");
     Ctx* c = p->c;
@@ -1462,19 +1511,9 @@ int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block
         fv[j].slot = c->n + j;
         gv[j] = c->geom_all[indices[j]];
     }
-    // running sums: the batch adds views [acc_n, n + k) to them; they restart from zero
-    // when there are none yet or the stored views were replaced since
-    const int64_t acc_len = p->out_len * (c->kind == MR_KIND_PARALLEL ? 2 : 1);
-    if (!p->d_acc) MR_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_acc), sizeof(float) * acc_len));
-    if (p->acc_n < 0 || p->acc_epoch != c->views_epoch || p->acc_n > c->n) {
-        MR_HIP(hipMemsetAsync(p->d_acc, 0, sizeof(float) * acc_len, c->s_req));
-        p->acc_n = 0;
-    }
-    // the device view count follows the host one (other ingest paths may have run), and
-    // the first view of the running sums
-    const int nv_begin[2] = {c->n, p->acc_n};
-    MR_HIP(hipMemcpyAsync(p->d_counters + 2, nv_begin, sizeof(int) * 2, hipMemcpyHostToDevice, c->s_req));
-    MR_HIP(hipStreamSynchronize(c->s_req));
+    // running sums: the batch adds views [acc_n, n + k) to them
+    rc = plan_prepare_sums(p);
+    if (rc) return rc;
     hipGraphExec_t exec = nullptr;
     for (auto& g : p->graphs)
         if (g.first == k) exec = g.second;
@@ -1516,7 +1555,6 @@ int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block
     p->last_ms = c->last_kernel_ms;
     c->n += k;
     p->acc_n = c->n;
-    p->acc_epoch = c->views_epoch;
     return MR_OK;
 }
 

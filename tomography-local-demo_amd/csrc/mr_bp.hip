@@ -403,9 +403,9 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel2(
     const int n_views = n_views_dev ? *n_views_dev : n_views_arg;
     const float scale = n_views_dev ? (n_views > 0 ? static_cast<float>(3.141592653589793 / n_views) : 0.0f)
                                     : scale_arg;
-    // running sums: resume from accum and add views [v_first, n_views) only (the host
-    // guarantees v_first < n_views); the FP32 sums then equal one pass over all views
-    // bit for bit, as each point adds its views in the same order either way
+    // running sums: resume from accum and add views [v_first, n_views) only (none when
+    // v_first >= n_views); the FP32 sums then equal one pass over all views bit for bit,
+    // as each point adds its views in the same order either way
     const int v_first = accum ? *v_begin_dev : 0;
     const int ntile = qhead ? *qcount : 0;
     for (int it = 0;; ++it) {
@@ -464,6 +464,23 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel2(
             if (!CONE) accb[m] = accum[accum_half + oidx(m)];
         }
     }
+    if (!CONE && n_views > 0) {
+        // parallel beam: the detector row of a point is the same for every view; take it
+        // from the record of the first view summed here (view 0 when none is)
+        const int v0 = v_first < n_views ? v_first : 0;
+        __syncthreads();
+        if (t == 0) make_record(geom + v0, vscale[v0], T, xt, R, C, pitch, CONE, rec);
+        __syncthreads();
+        const float4 r0 = rec4[0], r1 = rec4[1], r2 = rec4[2], r3 = rec4[3];
+#pragma unroll
+        for (int m = 0; m < PPT; ++m) {
+            const float dp = dp0 + static_cast<float>(PSTEP * m);
+            float row = fmaf(dp, r1.y, fmaf(ds, r1.x, r0.w));
+            row = __builtin_amdgcn_fmed3f(row, r2.w, r3.x);
+            fr_par[m] = __builtin_amdgcn_fractf(row);
+            roff_par[m] = cvt_flr(row) * pitchb;
+        }
+    }
 
     for (int vb = v_first; vb < n_views; vb += BP_VPASS) {
         const int nv = min(BP_VPASS, n_views - vb);
@@ -471,17 +488,6 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel2(
         if (t < nv)
             make_record(geom + vb + t, vscale[vb + t], T, xt, R, C, pitch, CONE, rec + t * BP_REC);
         __syncthreads();
-        if (!CONE && vb == v_first) {
-            const float4 r0 = rec4[0], r1 = rec4[1], r2 = rec4[2], r3 = rec4[3];
-#pragma unroll
-            for (int m = 0; m < PPT; ++m) {
-                const float dp = dp0 + static_cast<float>(PSTEP * m);
-                float row = fmaf(dp, r1.y, fmaf(ds, r1.x, r0.w));
-                row = __builtin_amdgcn_fmed3f(row, r2.w, r3.x);
-                fr_par[m] = __builtin_amdgcn_fractf(row);
-                roff_par[m] = cvt_flr(row) * pitchb;
-            }
-        }
 #pragma unroll 2
         for (int v = 0; v < nv; ++v) {
             const float4 r0 = rec4[v * 4 + 0];

@@ -64,6 +64,7 @@ SIGNATURES = [
     ("mr_plan_destroy", _c.c_int, [_P]),
     ("mr_plan_run", _c.c_int, [_P, _c.c_int, _P]),
     ("mr_plan_stream", _c.c_int, [_P, _P, _c.c_int, _P, _P]),
+    ("mr_plan_update", _c.c_int, [_P, _P]),
     ("mr_plan_cor_scores", _c.c_int, [_P, _c.c_int, _P]),
     ("mr_plan_output", _c.c_int, [_P, _c.POINTER(_P), _c.POINTER(_c.c_int64)]),
     ("mr_plan_last_ms", _c.c_int, [_P, _c.POINTER(_c.c_double)]),
@@ -514,6 +515,13 @@ class Plan:
         blk = np.ascontiguousarray(block, dtype=np.float32)
         out = np.empty(self.out_len, dtype=np.float32) if copy else None
         check(self.lib.mr_plan_stream(self.h, ptr(idx), len(idx), ptr(blk), ptr(out)))
+        return out
+
+    def update(self, copy=True):
+        """Output over all stored views, adding only the views not yet in the plan's
+        running sums (however they were ingested)."""
+        out = np.empty(self.out_len, dtype=np.float32) if copy else None
+        check(self.lib.mr_plan_update(self.h, ptr(out)))
         return out
 
     def cor_scores(self, n_views=-1):

@@ -457,6 +457,15 @@ class SliceReconstructor:
             o += n * n
         return out
 
+    def live_planes(self, planes, cor_shift=None):
+        """Slices kept up to date as views arrive (no counterpart in the reference, whose
+        viewer and server re-request the same planes after every batch, recon.py:426-440):
+        ``planes`` as for reconstruct_planes.  ``refresh()`` on the returned object gives
+        the images over all views received so far, back-projecting only the views added
+        since its previous call into running FP32 sums on the device."""
+        shift = self.cor_shift if cor_shift is None else float(cor_shift)
+        return LivePlanes(self, planes, shift)
+
     def reconstruct_volume(self, voxel=None, maxdim=None, chunk=262144, progress=None):
         """Full 3-D reconstruction of the box on a regular grid (recon.py:442-470),
         in z-slabs of max(1, chunk // (nx*ny)) planes so that slice requests can
@@ -558,3 +567,35 @@ class SliceReconstructor:
         """Stored filtered views [count, rows, cols] float32 in arrival order."""
         count = self.n - first if count is None else count
         return self._ctx.read_views(first, count, self.geom.rows, self.geom.cols)
+
+
+class LivePlanes:
+    """Planes of one SliceReconstructor brought up to date incrementally (a device plan
+    with running sums; see SliceReconstructor.live_planes)."""
+
+    def __init__(self, rec, planes, shift):
+        self._rec = rec
+        tasks, self._sizes, off = [], [], 0
+        for p in planes:
+            n = int(p.get("n", 256))
+            tasks.append(plane_task(rec.box, p["origin"], p["normal"], n, p.get("size"),
+                                    p.get("zoom", 1.0), p.get("center"), shift, off))
+            self._sizes.append(n)
+            off += n * n
+        if not tasks:
+            raise ValueError("no planes")
+        with _KERNEL_LOCK:
+            self._plan = _hip.Plan(rec._ctx, np.array(tasks, dtype=_hip.TASK_DTYPE), off)
+
+    def refresh(self):
+        """Images (float64 [n, n] each) over all views received so far."""
+        with _KERNEL_LOCK:
+            flat = self._plan.update()
+        out, o = [], 0
+        for n in self._sizes:
+            out.append(flat[o:o + n * n].reshape(n, n).astype(np.float64))
+            o += n * n
+        return out
+
+    def close(self):
+        self._plan.close()
