@@ -410,3 +410,44 @@ def test_live_planes_follow_add_projections(workload):
     for a, b in zip(again, imgs):
         np.testing.assert_array_equal(a, b)
     live.close()
+
+
+def test_repeated_requests_reuse_tiles_exactly(workload):
+    """A context reuses the tile list of a request that repeats the previous one; any
+    change in the tasks (one plane moved, a culled tile appearing) or in the operation
+    rebuilds it: every result equals a fresh context's bit for bit."""
+    from minirecast import _hip, recon
+    w, proj, planes, tasks, out_len = workload
+    A = w.geom.n_angles
+    box = w.geom.volume_box()
+    T1 = np.array(tasks, dtype=_hip.TASK_DTYPE)
+    T2 = T1.copy()
+    T2[1] = recon.plane_task(box, (0.5, 0.0, 9.0), (-0.3, 1.0, 0.4), 64, None, 1.7,
+                             (1.0, 0.0, 0.0), 0.0, int(T1[1]['out_offset']))
+    ctx = _ctx(w, "f16")
+    ctx.push(np.arange(A), proj)
+    seq = [ctx.bp_tasks(T, A, out_len) for T in (T1, T1, T2, T1, T2, T2)]
+    fresh = {}
+    for name, T in (("1", T1), ("2", T2)):
+        c = _ctx(w, "f16")
+        c.push(np.arange(A), proj)
+        fresh[name] = c.bp_tasks(T, A, out_len)
+    for got, name in zip(seq, "112122"):
+        np.testing.assert_array_equal(got, fresh[name])
+    assert not np.array_equal(fresh["1"], fresh["2"])
+    # more distinct requests than the cache holds, cycled twice
+    sets = []
+    for k in range(6):
+        Tk = T1.copy()
+        Tk['cor_shift'] = 0.25 * k
+        sets.append(Tk)
+    ref = []
+    for Tk in sets:
+        c = _ctx(w, "f16")
+        c.push(np.arange(A), proj)
+        ref.append(c.bp_tasks(Tk, A, out_len))
+    for _ in range(2):
+        for Tk, r in zip(sets, ref):
+            np.testing.assert_array_equal(ctx.bp_tasks(Tk, A, out_len), r)
+    for Tk, r in zip(sets[::-1], ref[::-1]):
+        np.testing.assert_array_equal(ctx.bp_tasks(Tk, A, out_len), r)

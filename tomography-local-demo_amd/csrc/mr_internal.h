@@ -76,6 +76,16 @@ struct TileQueue {
     int64_t accum_half = 0;
 };
 
+// Tile list of a recent request of a context (keyed by the tasks' bytes and the
+// operation / block-order flags), reused when a request repeats it.
+constexpr size_t kTileCacheEntries = 4;
+struct TileCache {
+    int flags = 0;
+    std::vector<uint8_t> tasks;
+    std::vector<Tile> tiles;         // ordered tiles, then the culled ones
+    size_t ne = 0;                   // culled tiles at the end
+};
+
 // Bands of the band ring (kernel 7): padded rows 0..R+1 can hold a point's upper tap,
 // band b covers padded rows 3b..3b+3.
 constexpr int band_count(int R) { return (R + 1) / 3 + 1; }
@@ -147,6 +157,7 @@ struct Ctx {
     // bumped when the geometry is set (geom_epoch, views_epoch) or the stored views are
     // forgotten (views_epoch): plans check them before reusing graphs or running sums
     uint64_t geom_epoch = 0, views_epoch = 0;
+    std::vector<TileCache> tile_cache;   // most recent first, at most kTileCacheEntries
 
     // host copies
     std::vector<ViewGeom> geom_all;   // per geometry index
