@@ -1475,6 +1475,7 @@ int mr_plan_update(mr_plan* p, float* out_host) {
     q.v_begin = p->d_counters + 3;
     q.accum = p->d_acc;
     q.accum_half = p->out_len;
+    p->acc_n = -1;                     // until the launch is known to have completed
     MR_HIP(hipEventRecord(c->ev0, c->s_req));
     rc = launch_reset_head(p->d_counters, c->s_req);
     if (!rc) rc = launch_bp_tiles(c, p->d_tasks, p->d_queue, 0, 0, 0.0f, p->d_out, c->s_req, &q);
@@ -1568,6 +1569,7 @@ int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block
         if (ei != hipSuccess) return fail(MR_ERUNTIME, std::string("graph instantiate: ") + hipGetErrorString(ei));
         p->graphs.emplace_back(k, exec);
     }
+    p->acc_n = -1;                     // until the replay is known to have completed
     MR_HIP(hipEventRecord(c->ev0, c->s_req));
     MR_HIP(hipGraphLaunch(exec, c->s_req));
     MR_HIP(hipEventRecord(c->ev1, c->s_req));
