@@ -475,8 +475,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
-              f"(this build has no multi-GPU path; see DESIGN.md for how it would shard)",
-              file=sys.stderr)
+              f"(the path runs on one GPU; there is no multi-GPU path)", file=sys.stderr)
         sys.exit(2)
 
     if args.workload != "full_cone":

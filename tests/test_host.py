@@ -197,3 +197,15 @@ def test_no_cpu_fallback_in_product_code():
         if f.endswith(".py"):
             text = open(os.path.join(pkg, f)).read()
             assert "oracle" not in text.replace("the oracle", ""), f
+
+
+def test_bench_refuses_several_gpus():
+    """bench.py measures one GPU: --gpus above 1 exits with a message, before any GPU
+    work (runs on the CPU)."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
