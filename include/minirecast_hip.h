@@ -250,7 +250,8 @@ int mr_reset_views(mr_ctx* ctx);
  *
  * A plan compiles a fixed batch of tasks once: its tiles are enumerated on the device
  * (in-box test included) into a work queue that a persistent back-projection launch
- * drains.  out_len floats of output live in device memory owned by the plan. */
+ * drains (in a context configured for kernel 7, reading the band copy of a 16-bit ring).
+ * out_len floats of output live in device memory owned by the plan. */
 typedef struct mr_plan mr_plan;
 int mr_plan_create(mr_ctx* ctx, const mr_task* tasks, int ntasks, int64_t out_len, mr_plan** out);
 int mr_plan_destroy(mr_plan* plan);

@@ -451,3 +451,57 @@ def test_repeated_requests_reuse_tiles_exactly(workload):
             np.testing.assert_array_equal(ctx.bp_tasks(Tk, A, out_len), r)
     for Tk, r in zip(sets[::-1], ref[::-1]):
         np.testing.assert_array_equal(ctx.bp_tasks(Tk, A, out_len), r)
+
+
+@pytest.mark.parametrize("beam", ["cone", "parallel"])
+def test_plan_band_copy_matches_ring(workload, beam, monkeypatch):
+    """In a kernel-7 (default 16-bit) context the persistent plan launches read the band
+    copy of the ring: plan runs, streamed batches (whose graph converts each batch it
+    stores), updates after other ingest paths and COR scores equal the same plans on the
+    ring itself (kernel 2) to FP32 rounding, for f16 and bf16."""
+    from minirecast import _hip, phantom, recon
+    from minirecast.geometry import ParallelGeometry
+    from minirecast.stream import arrival_order
+    if beam == "cone":
+        w, proj, planes, tasks, out_len = workload
+        g = w.geom
+        T = np.array(tasks, dtype=_hip.TASK_DTYPE)
+    else:
+        g = ParallelGeometry(np.arange(60) * np.pi / 60, 24, 80, 1.0, 1.0)
+        box = g.volume_box()
+        ph = phantom.random_ellipsoids(5, 8, box, (0.1, 1.0))
+        proj = phantom.project_parallel(ph, g)
+        T, off = [], 0
+        for nrm, n in (((1.0, 0.5, 0.8), 70), ((0.0, 0.0, 1.0), 40)):
+            T.append(recon.plane_task(box, (1.0, 2.0, 3.0), nrm, n, cor_shift=0.6, out_offset=off))
+            off += n * n
+        T, out_len = np.array(T, dtype=_hip.TASK_DTYPE), off
+    A = g.n_angles
+    order = arrival_order(A, "interleaved")
+    batches = [order[i:i + 8] for i in range(0, A, 8)]
+    for storage in ("f16", "bf16"):
+        outs = {}
+        for variant in ("7", "2"):
+            monkeypatch.setenv("MINIRECAST_BP_VARIANT", variant)
+            ctx = _hip.Context(0)
+            if beam == "cone":
+                ctx.set_cone(g.vectors, g.rows, g.cols, "ram-lak", storage)
+            else:
+                ctx.set_parallel(g.angles, g.rows, g.cols, 1.0, 1.0, "ram-lak", storage)
+            plan = _hip.Plan(ctx, T, out_len)
+            res = []
+            h = len(batches) // 2
+            for idx in batches[:h]:
+                res.append(plan.stream(idx, np.ascontiguousarray(proj[:, idx, :])))
+            for idx in batches[h:]:
+                ctx.push(idx, np.ascontiguousarray(proj[:, idx, :]))
+            res.append(plan.update())
+            res.append(plan.run())
+            res.append(plan.run(A // 3))
+            outs[variant] = res
+            plan.close()
+        for i, (a, b) in enumerate(zip(outs["7"], outs["2"])):
+            np.testing.assert_array_equal(a == 0, b == 0)
+            assert rel_l2(a, b) < 1e-6, (storage, beam, i, rel_l2(a, b))
+        np.testing.assert_array_equal(outs["7"][-3], outs["7"][-2])   # update == run, exactly
+    monkeypatch.delenv("MINIRECAST_BP_VARIANT")

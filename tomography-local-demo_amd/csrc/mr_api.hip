@@ -1373,8 +1373,10 @@ int mr_plan_run(mr_plan* p, int n_views, float* out_host) {
     q.count = p->d_counters;
     q.grid = p->grid;
     const float scale = n_views > 0 ? static_cast<float>(M_PI / n_views) : 0.0f;
+    int rc = prepare_persistent_bp(c, n_views, c->s_req);
+    if (rc) return rc;
     MR_HIP(hipEventRecord(c->ev0, c->s_req));
-    int rc = launch_reset_head(p->d_counters, c->s_req);
+    rc = launch_reset_head(p->d_counters, c->s_req);
     if (!rc) rc = launch_bp_tiles(c, p->d_tasks, p->d_queue, 0, n_views, scale, p->d_out, c->s_req, &q);
     if (rc) return rc;
     MR_HIP(hipEventRecord(c->ev1, c->s_req));
@@ -1405,10 +1407,18 @@ int mr_plan_cor_scores(mr_plan* p, int n_views, double* scores) {
     }
     MR_HIP(hipSetDevice(c->device));
     if (!p->d_scores) MR_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_scores), sizeof(double) * p->ntasks));
+    int rc0 = prepare_persistent_bp(c, n_views, c->s_req);
+    if (rc0) return rc0;
+    const bool band = persistent_uses_band(c);
+    if (p->cor_exec && p->cor_band != band) {    // configuration changed since the capture
+        hipGraphExecDestroy(p->cor_exec);
+        p->cor_exec = nullptr;
+    }
     // the device view count of the persistent launch
     MR_HIP(hipMemcpyAsync(p->d_counters + 2, &n_views, sizeof(int), hipMemcpyHostToDevice, c->s_req));
     MR_HIP(hipStreamSynchronize(c->s_req));
     if (!p->cor_exec) {
+        p->cor_band = band;
         TileQueue q;
         q.head = p->d_counters + 1;
         q.count = p->d_counters;
@@ -1465,7 +1475,8 @@ int mr_plan_update(mr_plan* p, float* out_host) {
     std::lock_guard<std::mutex> lk(c->mu);
     MR_CHECK(p->geom_epoch == c->geom_epoch, MR_ERUNTIME, "the plan was made for an earlier geometry");
     MR_HIP(hipSetDevice(c->device));
-    int rc = plan_prepare_sums(p);
+    int rc = prepare_persistent_bp(c, c->n, c->s_req);
+    if (!rc) rc = plan_prepare_sums(p);
     if (rc) return rc;
     TileQueue q;
     q.head = p->d_counters + 1;
@@ -1535,6 +1546,15 @@ int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block
         fv[j].slot = c->n + j;
         gv[j] = c->geom_all[indices[j]];
     }
+    // the band copy (kernel-7 configuration) holds the views stored so far; the graph
+    // converts the batch it stores
+    rc = prepare_persistent_bp(c, c->n, c->s_req);
+    if (rc) return rc;
+    const bool band = persistent_uses_band(c);
+    if (!p->graphs.empty() && p->graphs_band != band) {   // configuration changed
+        for (auto& g : p->graphs) hipGraphExecDestroy(g.second);
+        p->graphs.clear();
+    }
     // running sums: the batch adds views [acc_n, n + k) to them
     rc = plan_prepare_sums(p);
     if (rc) return rc;
@@ -1557,6 +1577,7 @@ int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block
         int r = (e1 != hipSuccess || e2 != hipSuccess) ? MR_ERUNTIME : MR_OK;
         if (!r) r = launch_stream_prep(c, p->d_meta, k, p->d_counters + 2, c->s_req);
         if (!r) r = launch_filter(c, p->d_block, k, 0, c->s_req, static_cast<const FilterView*>(p->d_meta));
+        if (!r && band) r = launch_band_convert_tail(c, p->d_counters + 2, k, c->s_req);
         if (!r) r = launch_reset_head(p->d_counters, c->s_req);
         if (!r) r = launch_bp_tiles(c, p->d_tasks, p->d_queue, 0, 0, 0.0f, p->d_out, c->s_req, &q);
         hipError_t ec = hipStreamEndCapture(c->s_req, &graph);
@@ -1568,6 +1589,7 @@ int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block
         hipGraphDestroy(graph);
         if (ei != hipSuccess) return fail(MR_ERUNTIME, std::string("graph instantiate: ") + hipGetErrorString(ei));
         p->graphs.emplace_back(k, exec);
+        p->graphs_band = band;
     }
     p->acc_n = -1;                     // until the replay is known to have completed
     MR_HIP(hipEventRecord(c->ev0, c->s_req));
@@ -1579,6 +1601,7 @@ int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block
     if (rc) return rc;
     p->last_ms = c->last_kernel_ms;
     c->n += k;
+    if (band) c->bring_n = std::max(c->bring_n, c->n);
     p->acc_n = c->n;
     return MR_OK;
 }

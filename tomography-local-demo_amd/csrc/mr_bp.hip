@@ -386,7 +386,78 @@ template <> struct Pair<uint16_t> {
     }
 };
 
-template <bool CONE, typename ST>
+// Taps from the row-pair ring: one element of pair-row r holds (q(r, c), q(r+1, c)) side
+// by side, so the four taps of a point are contiguous — 8 bytes (16-bit forms, one
+// buffer_load_dwordx2) or 16 bytes (f32, one buffer_load_dwordx4) — and a gather costs
+// one instruction instead of two.
+template <typename ST> struct PairP;
+template <> struct PairP<float> {
+    float a, b, c, d;
+    __device__ inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int) {
+        const uint4 x = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, 0, 0));
+        a = __uint_as_float(x.x);      // q(r, c)
+        c = __uint_as_float(x.y);      // q(r+1, c)
+        b = __uint_as_float(x.z);      // q(r, c+1)
+        d = __uint_as_float(x.w);      // q(r+1, c+1)
+    }
+    __device__ inline void lerp(float fc, float& top, float& bot) const {
+        top = fmaf(fc, b - a, a);
+        bot = fmaf(fc, d - c, c);
+    }
+};
+template <> struct PairP<__half> {
+    uint32_t w0, w1;                   // (q(r, c) | q(r+1, c) << 16), same for c+1
+    __device__ inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int) {
+        const uint2 x = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, 0, 0));
+        w0 = x.x;
+        w1 = x.y;
+    }
+    __device__ inline void lerp(float fc, float& top, float& bot) const {
+        top = fmaf(fc, half_lo(w1), fmaf(-fc, half_lo(w0), half_lo(w0)));
+        bot = fmaf(fc, half_hi(w1), fmaf(-fc, half_hi(w0), half_hi(w0)));
+    }
+};
+template <> struct PairP<uint16_t> {
+    uint32_t w0, w1;
+    __device__ inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int) {
+        const uint2 x = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, 0, 0));
+        w0 = x.x;
+        w1 = x.y;
+    }
+    __device__ inline void lerp(float fc, float& top, float& bot) const {
+        const float a = __uint_as_float(w0 << 16), b = __uint_as_float(w1 << 16);
+        const float c = __uint_as_float(w0 & 0xffff0000u), d = __uint_as_float(w1 & 0xffff0000u);
+        top = fmaf(fc, b - a, a);
+        bot = fmaf(fc, d - c, c);
+    }
+};
+
+// Taps from the band ring (16-bit forms): an element of band k holds the padded rows
+// 3k..3k+3 of one column (8 bytes), so the 2x2 stencil of every point lies in one band
+// and one buffer_load_dwordx4 fetches columns c and c+1; the two rows the point needs,
+// 3k+j and 3k+j+1, are picked at the lerp by a 16j-bit funnel shift.
+template <typename ST> struct BandP {
+    uint32_t d0, d1, d2, d3;           // (rows 0,1 | rows 2,3) of column c, then of c+1
+    uint32_t sh;                       // 16 j
+    __device__ inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int shift) {
+        const uint4 x = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, 0, 0));
+        d0 = x.x;
+        d1 = x.y;
+        d2 = x.z;
+        d3 = x.w;
+        sh = static_cast<uint32_t>(shift);
+    }
+    __device__ inline void lerp(float fc, float& top, float& bot) const {
+        PairP<ST> p;
+        p.w0 = static_cast<uint32_t>(((static_cast<uint64_t>(d1) << 32) | d0) >> sh);
+        p.w1 = static_cast<uint32_t>(((static_cast<uint64_t>(d3) << 32) | d2) >> sh);
+        p.lerp(fc, top, bot);
+    }
+};
+
+// BAND (16-bit forms): `ring` is the band copy (view_stride its elements per view), the
+// records are band records, and each point-view is one 16-byte load (BandP).
+template <bool CONE, typename ST, bool BAND = false>
 __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel2(
     const ST* __restrict__ ring, int64_t view_stride, int pitch, int R, int C,
     const ViewGeom* __restrict__ geom, const float* __restrict__ vscale,
@@ -445,12 +516,15 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel2(
 
     const float ds = static_cast<float>(tx);
     const float dp0 = static_cast<float>(ty);
-    const int pitchb = pitch << LE;                        // row pitch in bytes
+    static_assert(!BAND || sizeof(ST) == 2, "band copy of 16-bit storage only");
+    const int pitchb = BAND ? pitch << 3 : pitch << LE;     // row (band) pitch in bytes
     const int view_bytes = static_cast<int>(view_stride << LE);
     float acc[PPT], accb[PPT], fr_par[PPT];
-    int roff_par[PPT];
+    int roff_par[PPT], jsh_par[PPT];
 #pragma unroll
-    for (int m = 0; m < PPT; ++m) { acc[m] = 0.0f; accb[m] = 0.0f; fr_par[m] = 0.0f; roff_par[m] = 0; }
+    for (int m = 0; m < PPT; ++m) {
+        acc[m] = 0.0f; accb[m] = 0.0f; fr_par[m] = 0.0f; roff_par[m] = 0; jsh_par[m] = 0;
+    }
     const int64_t obase = T.out_offset + static_cast<int64_t>(tl.q) * T.stride_q;
     auto oidx = [&](int m) {
         return obase + static_cast<int64_t>(p0 + ty + PSTEP * m) * T.stride_p +
@@ -469,7 +543,7 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel2(
         // from the record of the first view summed here (view 0 when none is)
         const int v0 = v_first < n_views ? v_first : 0;
         __syncthreads();
-        if (t == 0) make_record(geom + v0, vscale[v0], T, xt, R, C, pitch, CONE, rec);
+        if (t == 0) make_record(geom + v0, vscale[v0], T, xt, R, C, pitch, CONE, rec, nullptr, BAND);
         __syncthreads();
         const float4 r0 = rec4[0], r1 = rec4[1], r2 = rec4[2], r3 = rec4[3];
 #pragma unroll
@@ -478,7 +552,14 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel2(
             float row = fmaf(dp, r1.y, fmaf(ds, r1.x, r0.w));
             row = __builtin_amdgcn_fmed3f(row, r2.w, r3.x);
             fr_par[m] = __builtin_amdgcn_fractf(row);
-            roff_par[m] = cvt_flr(row) * pitchb;
+            if (BAND) {
+                int b, jsh16;
+                band_split(cvt_flr(row), b, jsh16);
+                roff_par[m] = __mul24(b, pitchb);
+                jsh_par[m] = jsh16;
+            } else {
+                roff_par[m] = cvt_flr(row) * pitchb;
+            }
         }
     }
 
@@ -486,7 +567,8 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel2(
         const int nv = min(BP_VPASS, n_views - vb);
         __syncthreads();
         if (t < nv)
-            make_record(geom + vb + t, vscale[vb + t], T, xt, R, C, pitch, CONE, rec + t * BP_REC);
+            make_record(geom + vb + t, vscale[vb + t], T, xt, R, C, pitch, CONE, rec + t * BP_REC,
+                        nullptr, BAND);
         __syncthreads();
 #pragma unroll 2
         for (int v = 0; v < nv; ++v) {
@@ -494,7 +576,7 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel2(
             const float4 r1 = rec4[v * 4 + 1];
             const float4 r2 = rec4[v * 4 + 2];
             const float4 r3 = rec4[v * 4 + 3];
-            const int baseb = __float_as_int(r3.z) << LE;
+            const int baseb = BAND ? __float_as_int(r3.z) : __float_as_int(r3.z) << LE;
             const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
                 const_cast<ST*>(ring + static_cast<int64_t>(vb + v) * view_stride), 0, view_bytes,
                 0x00020000);
@@ -514,11 +596,19 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel2(
                     const float wt = rw * rw;
                     const float fc = __builtin_amdgcn_fractf(col);
                     const float fr = __builtin_amdgcn_fractf(row);
-                    const int voff = __mul24(cvt_flr(row), pitchb) + ((cvt_flr(col) << LE) + baseb);
-                    Pair<ST> tap;                  // 16-bit taps: lerps on the packed halves
-                    tap.load(rs, voff, pitchb);
                     float top, bot;
-                    tap.lerp(fc, top, bot);
+                    if constexpr (BAND) {
+                        int b, jsh16;
+                        band_split(cvt_flr(row), b, jsh16);
+                        BandP<ST> tap;
+                        tap.load(rs, band_offset(b, cvt_flr(col), pitchb, baseb), jsh16);
+                        tap.lerp(fc, top, bot);
+                    } else {
+                        const int voff = __mul24(cvt_flr(row), pitchb) + ((cvt_flr(col) << LE) + baseb);
+                        Pair<ST> tap;              // 16-bit taps: lerps on the packed halves
+                        tap.load(rs, voff, pitchb);
+                        tap.lerp(fc, top, bot);
+                    }
                     acc[m] = fmaf(wt, fmaf(fr, bot - top, top), acc[m]);
                 }
             } else {
@@ -529,11 +619,17 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel2(
                     const float dm = static_cast<float>(PSTEP * m);
                     const float col = __builtin_amdgcn_fmed3f(m ? fmaf(dm, r0.z, uh0) : uh0, r2.y, r2.z);
                     const float fc = __builtin_amdgcn_fractf(col);
-                    const int voff = roff_par[m] + ((cvt_flr(col) << LE) + baseb);
-                    Pair<ST> tap;
-                    tap.load(rs, voff, pitchb);
                     float top, bot;
-                    tap.lerp(fc, top, bot);
+                    if constexpr (BAND) {
+                        BandP<ST> tap;
+                        tap.load(rs, roff_par[m] + ((cvt_flr(col) << 3) + baseb), jsh_par[m]);
+                        tap.lerp(fc, top, bot);
+                    } else {
+                        const int voff = roff_par[m] + ((cvt_flr(col) << LE) + baseb);
+                        Pair<ST> tap;
+                        tap.load(rs, voff, pitchb);
+                        tap.lerp(fc, top, bot);
+                    }
                     acc[m] = fmaf(vs, top, acc[m]);
                     accb[m] = fmaf(vs, bot, accb[m]);
                 }
@@ -823,75 +919,6 @@ __global__ __launch_bounds__(256) void records_kernel(const ViewGeom* __restrict
                 band != 0);
 }
 
-// Taps from the row-pair ring: one element of pair-row r holds (q(r, c), q(r+1, c)) side
-// by side, so the four taps of a point are contiguous — 8 bytes (16-bit forms, one
-// buffer_load_dwordx2) or 16 bytes (f32, one buffer_load_dwordx4) — and a gather costs
-// one instruction instead of two.
-template <typename ST> struct PairP;
-template <> struct PairP<float> {
-    float a, b, c, d;
-    __device__ inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int) {
-        const uint4 x = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, 0, 0));
-        a = __uint_as_float(x.x);      // q(r, c)
-        c = __uint_as_float(x.y);      // q(r+1, c)
-        b = __uint_as_float(x.z);      // q(r, c+1)
-        d = __uint_as_float(x.w);      // q(r+1, c+1)
-    }
-    __device__ inline void lerp(float fc, float& top, float& bot) const {
-        top = fmaf(fc, b - a, a);
-        bot = fmaf(fc, d - c, c);
-    }
-};
-template <> struct PairP<__half> {
-    uint32_t w0, w1;                   // (q(r, c) | q(r+1, c) << 16), same for c+1
-    __device__ inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int) {
-        const uint2 x = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, 0, 0));
-        w0 = x.x;
-        w1 = x.y;
-    }
-    __device__ inline void lerp(float fc, float& top, float& bot) const {
-        top = fmaf(fc, half_lo(w1), fmaf(-fc, half_lo(w0), half_lo(w0)));
-        bot = fmaf(fc, half_hi(w1), fmaf(-fc, half_hi(w0), half_hi(w0)));
-    }
-};
-template <> struct PairP<uint16_t> {
-    uint32_t w0, w1;
-    __device__ inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int) {
-        const uint2 x = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, 0, 0));
-        w0 = x.x;
-        w1 = x.y;
-    }
-    __device__ inline void lerp(float fc, float& top, float& bot) const {
-        const float a = __uint_as_float(w0 << 16), b = __uint_as_float(w1 << 16);
-        const float c = __uint_as_float(w0 & 0xffff0000u), d = __uint_as_float(w1 & 0xffff0000u);
-        top = fmaf(fc, b - a, a);
-        bot = fmaf(fc, d - c, c);
-    }
-};
-
-// Taps from the band ring (16-bit forms): an element of band k holds the padded rows
-// 3k..3k+3 of one column (8 bytes), so the 2x2 stencil of every point lies in one band
-// and one buffer_load_dwordx4 fetches columns c and c+1; the two rows the point needs,
-// 3k+j and 3k+j+1, are picked at the lerp by a 16j-bit funnel shift.
-template <typename ST> struct BandP {
-    uint32_t d0, d1, d2, d3;           // (rows 0,1 | rows 2,3) of column c, then of c+1
-    uint32_t sh;                       // 16 j
-    __device__ inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int shift) {
-        const uint4 x = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, 0, 0));
-        d0 = x.x;
-        d1 = x.y;
-        d2 = x.z;
-        d3 = x.w;
-        sh = static_cast<uint32_t>(shift);
-    }
-    __device__ inline void lerp(float fc, float& top, float& bot) const {
-        PairP<ST> p;
-        p.w0 = static_cast<uint32_t>(((static_cast<uint64_t>(d1) << 32) | d0) >> sh);
-        p.w1 = static_cast<uint32_t>(((static_cast<uint64_t>(d3) << 32) | d2) >> sh);
-        p.lerp(fc, top, bot);
-    }
-};
-
 // RING: 0 the ring itself, 1 the row-pair ring, 2 the band ring (16-bit forms).
 template <typename ST, int PPT, int RING = 0>
 struct PointState {
@@ -919,13 +946,17 @@ __global__ __launch_bounds__(256) void pair_convert_kernel(const ST* __restrict_
 
 // Band ring from the ring (16-bit forms): element (band b, col) of view v = padded rows
 // 3b..3b+3 of that column (rows past the ring's Rp rows read as zero), 8 bytes.
+// Views v0 + z, or with v_end_dev the last gridDim.z views before the device view count
+// (a captured streaming graph converts the batch it has just stored).
 __global__ __launch_bounds__(256) void band_convert_kernel(const uint16_t* __restrict__ ring,
                                                            uint16_t* __restrict__ bring,
                                                            int64_t view_stride, int64_t bview_stride,
-                                                           int pitch, int Rp, int v0) {
+                                                           int pitch, int Rp, int v0,
+                                                           const int* __restrict__ v_end_dev) {
     const int col = blockIdx.x * 256 + threadIdx.x;
     const int b = blockIdx.y;
-    const int64_t v = v0 + static_cast<int64_t>(blockIdx.z);
+    const int64_t v = (v_end_dev ? *v_end_dev - static_cast<int>(gridDim.z) : v0) +
+                      static_cast<int64_t>(blockIdx.z);
     if (col >= pitch) return;
     const uint16_t* src = ring + v * view_stride + col;
     uint32_t q[4];
@@ -1698,6 +1729,23 @@ int dispatch_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntil
         MR_HIP(hipGetLastError());
         return MR_OK;
     }
+    if (c->bp_variant == 2 && c->v2_band) {
+        const int64_t bvs = band_view_stride(c);
+        if (c->storage == MR_STORE_F16)
+            hipLaunchKernelGGL((bp_tiles_kernel2<CONE, __half, true>), grid, block, 0, s,
+                               static_cast<const __half*>(c->bring), bvs, c->pitch, c->R,
+                               c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out,
+                               q.head, q.count, q.n_views, c->xcd_map != 0, q.v_begin, q.accum,
+                               q.accum_half);
+        else
+            hipLaunchKernelGGL((bp_tiles_kernel2<CONE, uint16_t, true>), grid, block, 0, s,
+                               static_cast<const uint16_t*>(c->bring), bvs, c->pitch, c->R,
+                               c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out,
+                               q.head, q.count, q.n_views, c->xcd_map != 0, q.v_begin, q.accum,
+                               q.accum_half);
+        MR_HIP(hipGetLastError());
+        return MR_OK;
+    }
     if (c->bp_variant == 2) {
         switch (c->storage) {
         case MR_STORE_F32:
@@ -1804,11 +1852,13 @@ int launch_bp_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int nti
         return fail(MR_ERUNTIME, "persistent launches need the v2 kernel");
     if (q) {                               // persistent launches always run the v2 loop
         const int saved = c->bp_variant;
+        c->v2_band = persistent_uses_band(c);
         c->bp_variant = 2;
         const int rc = c->kind == MR_KIND_CONE
                            ? dispatch_tiles<true>(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s, *q)
                            : dispatch_tiles<false>(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s, *q);
         c->bp_variant = saved;
+        c->v2_band = false;
         return rc;
     }
     return c->kind == MR_KIND_CONE
@@ -2062,10 +2112,39 @@ int ensure_band_ring(Ctx* c, int n_views, hipStream_t s) {
         const dim3 grid((c->pitch + 255) / 256, band_count(c->R), nv);
         hipLaunchKernelGGL(band_convert_kernel, grid, dim3(256), 0, s,
                            static_cast<const uint16_t*>(c->ring), static_cast<uint16_t*>(c->bring),
-                           c->view_stride, bvs, c->pitch, c->Rp, v0);
+                           c->view_stride, bvs, c->pitch, c->Rp, v0, nullptr);
         MR_HIP(hipGetLastError());
     }
     c->bring_n = std::max(c->bring_n, n_views);
+    return MR_OK;
+}
+
+// The band copy of 16-bit storage fits kernel 7's 32-bit byte offsets and 24-bit pitch
+// products (a record's reference band can lie up to a view beyond the last).
+static bool band_fits(const Ctx* c) {
+    return c->storage != MR_STORE_F32 &&
+           band_view_stride(c) * static_cast<int64_t>(sizeof(uint16_t)) < (int64_t{1} << 30) &&
+           (static_cast<int64_t>(c->pitch) << 3) < (int64_t{1} << 23);
+}
+
+// Persistent launches of a context configured for kernel 7 run the v2 loop on the band
+// copy (16-bit storage).
+bool persistent_uses_band(const Ctx* c) { return c->bp_variant == 7 && band_fits(c); }
+
+int prepare_persistent_bp(Ctx* c, int n_views, hipStream_t s) {
+    return persistent_uses_band(c) ? ensure_band_ring(c, n_views, s) : MR_OK;
+}
+
+// In a captured streaming graph: band copy of the k views just stored, ending at the
+// device view count.
+int launch_band_convert_tail(Ctx* c, const int* d_n, int k, hipStream_t s) {
+    const int64_t bvs = band_view_stride(c);
+    if (!c->bring || k <= 0) return fail(MR_ERUNTIME, "band copy not allocated");
+    const dim3 grid((c->pitch + 255) / 256, band_count(c->R), k);
+    hipLaunchKernelGGL(band_convert_kernel, grid, dim3(256), 0, s,
+                       static_cast<const uint16_t*>(c->ring), static_cast<uint16_t*>(c->bring),
+                       c->view_stride, bvs, c->pitch, c->Rp, 0, d_n);
+    MR_HIP(hipGetLastError());
     return MR_OK;
 }
 
@@ -2082,11 +2161,7 @@ int launch_bp_tiles_v4(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int 
         int rc = ensure_pair_ring(c, n_views, s);
         if (rc) return rc;
     } else if (c->bp_variant == 7) {
-        if (band_view_stride(c) * static_cast<int64_t>(sizeof(uint16_t)) >= (int64_t{1} << 30) ||
-            (static_cast<int64_t>(c->pitch) << 3) >= (int64_t{1} << 23)) {
-            // kernel 7 addresses a view's bands with 32-bit byte offsets (a record's
-            // reference band can lie up to a view beyond the last) and multiplies by the
-            // band pitch in 24 bits
+        if (!band_fits(c)) {
             const int sv = c->bp_variant;
             c->bp_variant = 6;
             const int rc = launch_bp_tiles_v4(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s);

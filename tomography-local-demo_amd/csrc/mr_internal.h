@@ -158,6 +158,7 @@ struct Ctx {
     // forgotten (views_epoch): plans check them before reusing graphs or running sums
     uint64_t geom_epoch = 0, views_epoch = 0;
     std::vector<TileCache> tile_cache;   // most recent first, at most kTileCacheEntries
+    bool v2_band = false;            // the v2 launch in progress reads the band copy
 
     // host copies
     std::vector<ViewGeom> geom_all;   // per geometry index
@@ -224,6 +225,8 @@ struct Plan {
     float* d_acc = nullptr;
     int acc_n = -1;
     uint64_t acc_epoch = 0;
+    // whether the captured graphs read the band copy (recaptured when that changes)
+    bool graphs_band = false, cor_band = false;
     // rotation-centre scoring of the plan's images (one graph: persistent BP + scores)
     std::vector<mr_task> h_tasks;    // host copy of the tasks
     double* d_scores = nullptr;
@@ -259,6 +262,13 @@ int launch_zero_tiles(const mr_task* d_tasks, const Tile* d_tiles, int ntiles, f
 int launch_enumerate_tiles(const mr_task* d_tasks, const int64_t* d_toff, int ntasks,
                            int64_t total, Tile* d_queue, int* d_counters, hipStream_t s);
 int launch_reset_head(int* d_counters, hipStream_t s);
+// persistent (v2) launches of a kernel-7 context read the band copy: it must hold the
+// views a launch reads (prepare before launching or capturing; in a streaming graph the
+// tail conversion brings in the batch just stored)
+int64_t band_view_stride(const Ctx* c);
+bool persistent_uses_band(const Ctx* c);
+int prepare_persistent_bp(Ctx* c, int n_views, hipStream_t s);
+int launch_band_convert_tail(Ctx* c, const int* d_n, int k, hipStream_t s);
 int launch_stream_prep(Ctx* c, const void* d_meta, int k, int* d_nviews, hipStream_t s);
 int persistent_grid(Ctx* c);
 int launch_bp_points(Ctx* c, const double* d_pts, int64_t M, double cor_shift, int n_views,
