@@ -505,3 +505,33 @@ def test_plan_band_copy_matches_ring(workload, beam, monkeypatch):
             assert rel_l2(a, b) < 1e-6, (storage, beam, i, rel_l2(a, b))
         np.testing.assert_array_equal(outs["7"][-3], outs["7"][-2])   # update == run, exactly
     monkeypatch.delenv("MINIRECAST_BP_VARIANT")
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+def test_oversized_slice_requests(workload, storage, monkeypatch):
+    """Slice requests whose records exceed the records buffer (here shrunk to 1 MB) run
+    the v2 loop in the default configuration (on the band copy for 16-bit rings), and the
+    chunked records path in an explicit kernel-4/7 configuration; both equal kernel 1.
+    The buffer is shrunk to 64 KB so that these small slices need several chunks."""
+    from minirecast import _hip
+    w, proj, planes, tasks, out_len = workload
+    T = np.array(tasks, dtype=_hip.TASK_DTYPE)
+    A = w.geom.n_angles
+    monkeypatch.setenv("MINIRECAST_BP_RECORD_KB", "64")
+    outs, launches = {}, {}
+    for name, variant in (("default", None), ("explicit", "7" if storage != "f32" else "4"),
+                          ("ref", "1")):
+        if variant is None:
+            monkeypatch.delenv("MINIRECAST_BP_VARIANT", raising=False)
+        else:
+            monkeypatch.setenv("MINIRECAST_BP_VARIANT", variant)
+        ctx = _hip.Context(0)
+        ctx.set_cone(w.geom.vectors, w.geom.rows, w.geom.cols, w.window, storage)
+        ctx.push(np.arange(A), proj)
+        outs[name] = ctx.bp_tasks(T, A, out_len)
+        launches[name] = ctx.last_tile_launches()
+    assert launches["default"] == 0               # the v2 loop (no kernel-4 launches)
+    assert launches["explicit"] > 1                # records in several chunks
+    for name in ("default", "explicit"):
+        np.testing.assert_array_equal(outs[name] == 0, outs["ref"] == 0)
+        assert rel_l2(outs[name], outs["ref"]) < 1e-6, (storage, name, rel_l2(outs[name], outs["ref"]))

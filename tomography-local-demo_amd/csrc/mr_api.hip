@@ -450,6 +450,10 @@ int mr_ctx_create(int device, mr_ctx** out) {
         c->vchunk_explicit = true;
     }
     if (const char* v = std::getenv("MINIRECAST_BP_VCHUNK_ANY")) c->vchunk_any = std::atoi(v) != 0;
+    if (const char* v = std::getenv("MINIRECAST_BP_RECORD_KB")) {
+        const long kb = std::atol(v);
+        if (kb >= 1) c->rec_budget = static_cast<size_t>(kb) << 10;
+    }
     if (const char* v = std::getenv("MINIRECAST_XCD_MAP")) {
         const int k = std::atoi(v);
         c->xcd_map = (k == 1 || k == 2) ? k : 0;

@@ -1846,6 +1846,31 @@ int launch_bp_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int nti
             c->bp_depth = sd;
             return rc;
         }
+        if (!c->bp_explicit && (c->bp_variant == 7 || c->bp_variant == 4) && c->req_slices &&
+            static_cast<size_t>(ntiles) * n_views * BP_REC * sizeof(float) > c->rec_budget) {
+            // default configuration, slice requests whose records exceed one pass of the
+            // records buffer (a COR search of 25 trial images of 1024^2 over 1800 views
+            // needs 2.9 GB): the v2 loop, which builds each tile's records in LDS and keeps
+            // every view's sums in registers, on the band copy of a 16-bit ring or on the
+            // f32 ring, instead of records in chunks and view blocks whose sums make a
+            // round trip through the 105 MB output per launch (measured on that search:
+            // f16 42 against 54 ms, f32 75 against 82 ms)
+            const bool band = persistent_uses_band(c);
+            if (band) {
+                const int rc = ensure_band_ring(c, n_views, s);
+                if (rc) return rc;
+            }
+            const int sv = c->bp_variant;
+            c->bp_variant = 2;
+            c->v2_band = band;
+            const TileQueue none{};
+            const int rc = c->kind == MR_KIND_CONE
+                               ? dispatch_tiles<true>(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s, none)
+                               : dispatch_tiles<false>(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s, none);
+            c->bp_variant = sv;
+            c->v2_band = false;
+            return rc;
+        }
         return launch_bp_tiles_v4(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s);
     }
     if (q && c->bp_variant != 2 && c->bp_variant < 4)
@@ -2174,8 +2199,7 @@ int launch_bp_tiles_v4(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int 
     const bool band = c->bp_variant == 7 && c->storage != MR_STORE_F32;   // band-ring records
     const int rec_floats = v5 ? V5_REC : BP_REC;
     const size_t per_tile = sizeof(float) * rec_floats * static_cast<size_t>(n_views);
-    const size_t budget = static_cast<size_t>(512) << 20;          // record buffer <= 512 MB
-    int chunk = static_cast<int>(std::max<size_t>(1, budget / per_tile));
+    int chunk = static_cast<int>(std::max<size_t>(1, c->rec_budget / per_tile));
     chunk = std::min(chunk, ntiles);
     // v5: one "every box fits" flag per tile after the records
     const size_t flag_off = (per_tile * chunk + 255) / 256 * 256;

@@ -127,6 +127,7 @@ struct Ctx {
     int bp_vchunk = 0;               // kernels 4/6 (cone): views per launch, 0 = all (MINIRECAST_BP_VCHUNK)
     bool vchunk_explicit = false;    // views per launch set by the environment
     bool vchunk_any = false;         // split grids of any size (MINIRECAST_BP_VCHUNK_ANY, tests)
+    size_t rec_budget = size_t{512} << 20;   // kernel-4 records buffer (MINIRECAST_BP_RECORD_KB)
     bool req_slices = false;         // the request in flight has single-plane tasks only
     int n_cus = 256;                 // compute units of the device
     int bp_sync = 0;                 // kernels 4/6: waves of a workgroup in lockstep per view
