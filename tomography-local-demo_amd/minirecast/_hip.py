@@ -482,7 +482,13 @@ class Context:
 
 
 class Plan:
-    """A compiled batch of tasks on a device work queue (mr_plan)."""
+    """A compiled batch of tasks on a device work queue (mr_plan), back-projected by a
+    persistent launch.  ``run(n)``: a full pass over the first n stored views;
+    ``stream(indices, block)``: ingest a batch and bring the output up to all views from
+    one replayed HIP graph; ``update()``: bring it up to all views however they were
+    ingested; both add only the views not yet in the plan's running FP32 sums (the same
+    bits as ``run``).  ``cor_scores(n)``: entropy scores of a plan made of COR trial
+    images, one replayed graph."""
 
     def __init__(self, ctx, tasks, out_len):
         self.ctx = ctx
@@ -506,11 +512,14 @@ class Plan:
             pass
 
     def run(self, n_views=-1, copy=True):
+        """Output over the first n_views stored views (-1: all), a full pass."""
         out = np.empty(self.out_len, dtype=np.float32) if copy else None
         check(self.lib.mr_plan_run(self.h, int(n_views), ptr(out)))
         return out
 
     def stream(self, indices, block, copy=True):
+        """Ingest views `indices` (block [rows, k, cols] float32) and return the output
+        over all stored views."""
         idx = np.ascontiguousarray(indices, dtype=np.int64).ravel()
         blk = np.ascontiguousarray(block, dtype=np.float32)
         out = np.empty(self.out_len, dtype=np.float32) if copy else None
