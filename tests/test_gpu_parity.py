@@ -522,6 +522,19 @@ def test_slice_service_replies_and_coalescing(small_cone):
     # the other requests
     rep, flat = svc.handle({"cmd": "slices", "planes": planes[:2]})
     assert rep["sizes"] == [40, 41] and flat.size == 40 * 40 + 41 * 41
+    # the same set again, with views arriving in between: served from running sums
+    rec2 = recon.SliceReconstructor(None, w.geom, window=w.window, verbose=False)
+    h, arr = serve.geometry_message(w.geom)
+    svc.handle(h, arr)
+    for i in range(0, A, 45):
+        idx = list(range(i, min(A, i + 45)))
+        svc.handle({"cmd": "projections", "indices": idx}, proj[:, idx, :])
+        rec2.add_projections(idx, proj[:, idx, :])
+        rep, flat = svc.handle({"cmd": "slices", "planes": planes[:3]})
+        want = np.concatenate([im.ravel() for im in rec2.reconstruct_planes(planes[:3])])
+        assert rep["version"] == rec2.version
+        np.testing.assert_allclose(flat, want, rtol=0, atol=1e-6 * np.abs(want).max())
+    assert svc.live_hits >= 2
     assert svc.handle({"cmd": "info"})[0]["n"] == A
     rep, vol = svc.handle({"cmd": "preview", "maxdim": 16})
     ref_vol, ref_origin, ref_h = rec.preview_volume(16)

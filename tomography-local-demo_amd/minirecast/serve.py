@@ -17,11 +17,15 @@ same reply headers and float32 arrays, plus:
 Concurrent ``slice`` requests (several viewers) are coalesced: the first request of a
 burst waits ``coalesce_ms`` for others and all pending slices are back-projected in one
 launch of the tile kernel over one pass of the views, each caller receiving its own
-image.  ``serve(conn)`` runs the request loop of one connection.
+image.  A set of slices requested again (a viewer refreshing its planes as projections
+arrive) is answered from a device plan with running sums
+(``SliceReconstructor.live_planes``): only the views received since its previous request
+are back-projected.  ``serve(conn)`` runs the request loop of one connection.
 """
 import json
 import threading
 import time
+from collections import OrderedDict
 
 import numpy as np
 
@@ -104,6 +108,53 @@ class SliceService:
         self.raw_correction = None
         self.launches = 0                     # slice launches (coalescing statistics)
         self.slices_served = 0
+        # slice sets seen before: None after the first request, then a LivePlanes
+        self._live = OrderedDict()
+        self._live_lock = threading.Lock()
+        self.live_hits = 0
+
+    LIVE_SETS = 8                             # slice sets remembered
+
+    def _planes(self, r, planes):
+        """Images of `planes` over all views received so far: a set requested before
+        is refreshed from its running sums, a new one back-projected directly."""
+        key = (id(r), float(r.cor_shift), json.dumps(planes, sort_keys=True))
+        with self._live_lock:
+            seen = key in self._live
+            live = self._live.get(key)
+            if seen:
+                self._live.move_to_end(key)
+        if not seen:
+            imgs = r.reconstruct_planes(planes)
+            self._remember(key, None)
+            return imgs
+        if live is None:
+            live = r.live_planes(planes)
+            self._remember(key, live)
+        self.live_hits += 1
+        return live.refresh()
+
+    def _remember(self, key, live):
+        drop = []
+        with self._live_lock:
+            old = self._live.get(key)
+            if old is not None and old is not live:
+                drop.append(old)
+            self._live[key] = live
+            self._live.move_to_end(key)
+            while len(self._live) > self.LIVE_SETS:
+                _, v = self._live.popitem(last=False)
+                if v is not None:
+                    drop.append(v)
+        for v in drop:
+            v.close()
+
+    def _forget_live(self):
+        with self._live_lock:
+            items, self._live = list(self._live.values()), OrderedDict()
+        for v in items:
+            if v is not None:
+                v.close()
 
     def _recon(self):
         if self.recon is None:
@@ -134,7 +185,7 @@ class SliceService:
     def _run(self, batch):
         try:
             r = self._recon()
-            images = r.reconstruct_planes([q.plane for q in batch])
+            images = self._planes(r, [q.plane for q in batch])
             version = r.version
             self.launches += 1
             self.slices_served += len(batch)
@@ -156,6 +207,7 @@ class SliceService:
         if cmd == "geometry":
             geom = geometry_from_message(h, arr)
             with self.lock:
+                self._forget_live()
                 self.recon = SliceReconstructor(None, geom, verbose=False, **self.opts)
                 self.raw_correction = None
             return {"ok": True}, None
@@ -189,7 +241,7 @@ class SliceService:
                            zoom=float(p.get("zoom", 1.0)), center=p.get("center"))
                       for p in h["planes"]]
             r = self._recon()
-            imgs = r.reconstruct_planes(planes)
+            imgs = self._planes(r, planes)
             self.launches += 1
             self.slices_served += len(planes)
             flat = np.concatenate([im.astype(np.float32).ravel() for im in imgs])
