@@ -457,8 +457,9 @@ template <typename ST> struct BandP {
 
 // BAND (16-bit forms): `ring` is the band copy (view_stride its elements per view), the
 // records are band records, and each point-view is one 16-byte load (BandP).
+// At least 5 waves per SIMD (89-95 VGPRs instead of 93-111, no spills; 6 would spill).
 template <bool CONE, typename ST, bool BAND = false>
-__global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel2(
+__global__ __launch_bounds__(BP_THREADS, 5) void bp_tiles_kernel2(
     const ST* __restrict__ ring, int64_t view_stride, int pitch, int R, int C,
     const ViewGeom* __restrict__ geom, const float* __restrict__ vscale,
     const mr_task* __restrict__ tasks, const Tile* __restrict__ tiles, int n_views_arg,
