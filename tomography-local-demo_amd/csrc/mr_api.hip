@@ -476,7 +476,7 @@ int mr_ctx_destroy(mr_ctx* c) {
     hipDeviceSynchronize();
     free_geometry(c);
     void* bufs[] = {c->d_stage, c->d_fstage, c->d_vmax, c->d_fv, c->d_req, c->d_out, c->d_pts, c->d_misc,
-                    c->d_pre, c->d_recs, c->d_raw, c->d_vol, c->d_stat};
+                    c->d_pre, c->d_recs, c->d_raw, c->d_vol, c->d_stat, c->d_corws};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (c->h_req) hipHostFree(c->h_req);
@@ -1280,7 +1280,7 @@ static void plan_free(Plan* p) {
     if (p->cor_exec) hipGraphExecDestroy(p->cor_exec);
     p->cor_exec = nullptr;
     void* dev[] = {p->d_tasks, p->d_toff, p->d_queue, p->d_counters, p->d_out, p->d_block, p->d_meta,
-                   p->d_scores, p->d_acc};
+                   p->d_scores, p->d_acc, p->d_corws};
     for (void* d : dev)
         if (d) hipFree(d);
     if (p->h_block) hipHostFree(p->h_block);
@@ -1411,6 +1411,7 @@ int mr_plan_cor_scores(mr_plan* p, int n_views, double* scores) {
     }
     MR_HIP(hipSetDevice(c->device));
     if (!p->d_scores) MR_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_scores), sizeof(double) * p->ntasks));
+    if (!p->d_corws) MR_HIP(hipMalloc(&p->d_corws, cor_workspace_bytes(p->ntasks)));
     int rc0 = prepare_persistent_bp(c, n_views, c->s_req);
     if (rc0) return rc0;
     const bool band = persistent_uses_band(c);
@@ -1432,7 +1433,7 @@ int mr_plan_cor_scores(mr_plan* p, int n_views, double* scores) {
         MR_HIP(hipStreamBeginCapture(c->s_req, hipStreamCaptureModeThreadLocal));
         int r = launch_reset_head(p->d_counters, c->s_req);
         if (!r) r = launch_bp_tiles(c, p->d_tasks, p->d_queue, 0, 0, 0.0f, p->d_out, c->s_req, &q);
-        if (!r) r = launch_cor_scores(c, p->d_out, p->ntasks, npix, p->d_scores, c->s_req);
+        if (!r) r = launch_cor_scores(p->d_out, p->ntasks, npix, p->d_scores, p->d_corws, c->s_req);
         hipError_t ec = hipStreamEndCapture(c->s_req, &graph);
         if (r || ec != hipSuccess) {
             if (graph) hipGraphDestroy(graph);
@@ -1661,7 +1662,9 @@ int mr_cor_scores(mr_ctx* c, const mr_task* tasks, int ntasks, int n_views, doub
     MR_HIP(hipMemsetAsync(c->d_out, 0, sizeof(float) * total, c->s_req));
     rc = run_tasks(c, T.data(), ntasks, n_views, c->d_out);
     if (rc) return rc;
-    rc = launch_cor_scores(c, c->d_out, ntasks, npix, c->d_misc, c->s_req);
+    rc = ensure(&c->d_corws, &c->corws_bytes, cor_workspace_bytes(ntasks));
+    if (rc) return rc;
+    rc = launch_cor_scores(c->d_out, ntasks, npix, c->d_misc, c->d_corws, c->s_req);
     if (rc) return rc;
     MR_HIP(hipMemcpyAsync(scores, c->d_misc, sizeof(double) * ntasks, hipMemcpyDeviceToHost, c->s_req));
     return finish_timing(c);

@@ -6,83 +6,152 @@
 //   f = ((x - first) / (last - first)) * 128, i = int(f), i == 128 -> 127,
 //   edges[i] = i * ((last - first) / 128) + first (edges[128] = last),
 //   x < edges[i] -> i - 1;  x >= edges[i + 1] and i != 127 -> i + 1.
-// One 1024-thread workgroup per image: min/max reduction, LDS histogram, score.
+//
+// Each image is split into chunks, one 256-thread workgroup per (chunk, image), so that
+// a search of 25 images fills the device: (1) partial min / max of the non-zero values
+// per chunk; (2) the image's range from its partials, the chunk's histogram in LDS (one
+// copy per wave, then summed) added to the image's histogram with 128 global atomics;
+// (3) one thread per image forms the score in bin order.  The counts are integers and
+// min / max are order-free, so the scores do not depend on the chunking.
 #include "mr_internal.h"
 
 namespace mr {
 
 namespace {
 
-constexpr int CT = 1024;
+constexpr int CT = 256;
 constexpr int NBINS = 128;
+constexpr int MAX_CHUNKS = 64;
+constexpr int64_t PIX_PER_CHUNK = 16384;
 
 __device__ inline double edge(int i, double first, double last, double step) {
 #pragma clang fp contract(off)
     return (i == NBINS) ? last : static_cast<double>(i) * step + first;
 }
 
-__global__ __launch_bounds__(CT) void cor_score_kernel(const float* __restrict__ img,
-                                                       int64_t npix, double* __restrict__ scores) {
-#pragma clang fp contract(off)
+// Workspace: [nimg][MAX_CHUNKS][2] float partial (min, max), then [nimg][NBINS] counts.
+__device__ inline float* ws_minmax(void* ws, int img) {
+    return static_cast<float*>(ws) + static_cast<int64_t>(img) * MAX_CHUNKS * 2;
+}
+__device__ inline unsigned* ws_hist(void* ws, int nimg, int img) {
+    return reinterpret_cast<unsigned*>(static_cast<float*>(ws) + static_cast<int64_t>(nimg) * MAX_CHUNKS * 2) +
+           static_cast<int64_t>(img) * NBINS;
+}
+
+__device__ inline void chunk_range(int64_t npix, int nchunk, int chunk, int64_t& b, int64_t& e) {
+    const int64_t per = (npix + nchunk - 1) / nchunk;
+    b = static_cast<int64_t>(chunk) * per;
+    e = b + per < npix ? b + per : npix;
+}
+
+__global__ __launch_bounds__(CT) void cor_minmax_kernel(const float* __restrict__ img, int64_t npix,
+                                                        int nimg, void* __restrict__ ws) {
     __shared__ float smin[CT], smax[CT];
-    __shared__ unsigned hist[NBINS];
-    const float* im = img + static_cast<int64_t>(blockIdx.x) * npix;
+    const int im = blockIdx.y, chunk = blockIdx.x, t = threadIdx.x;
+    if (chunk == 0 && t < NBINS) ws_hist(ws, nimg, im)[t] = 0u;    // cleared for pass 2
+    int64_t b, e;
+    chunk_range(npix, gridDim.x, chunk, b, e);
+    const float* p = img + static_cast<int64_t>(im) * npix;
     float lo = INFINITY, hi = -INFINITY;
-    for (int64_t i = threadIdx.x; i < npix; i += CT) {
-        const float x = im[i];
+    for (int64_t i = b + t; i < e; i += CT) {
+        const float x = p[i];
         if (x != 0.0f) { lo = fminf(lo, x); hi = fmaxf(hi, x); }
     }
-    smin[threadIdx.x] = lo;
-    smax[threadIdx.x] = hi;
-    if (threadIdx.x < NBINS) hist[threadIdx.x] = 0;
+    smin[t] = lo;
+    smax[t] = hi;
     __syncthreads();
     for (int w = CT / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
-            smin[threadIdx.x] = fminf(smin[threadIdx.x], smin[threadIdx.x + w]);
-            smax[threadIdx.x] = fmaxf(smax[threadIdx.x], smax[threadIdx.x + w]);
+        if (t < w) {
+            smin[t] = fminf(smin[t], smin[t + w]);
+            smax[t] = fmaxf(smax[t], smax[t + w]);
         }
         __syncthreads();
     }
-    if (!(smin[0] <= smax[0])) {           // no non-zero value: empty histogram
-        if (threadIdx.x == 0) scores[blockIdx.x] = 0.0;
-        return;
+    if (t == 0) {
+        float* mm = ws_minmax(ws, im) + 2 * chunk;
+        mm[0] = smin[0];
+        mm[1] = smax[0];
     }
-    double first = smin[0], last = smax[0];
+}
+
+__global__ __launch_bounds__(CT) void cor_hist_kernel(const float* __restrict__ img, int64_t npix,
+                                                      int nimg, void* __restrict__ ws) {
+#pragma clang fp contract(off)
+    constexpr int NW = CT / 64;
+    __shared__ float smin[MAX_CHUNKS], smax[MAX_CHUNKS];
+    __shared__ unsigned hist[NW][NBINS];
+    const int im = blockIdx.y, chunk = blockIdx.x, t = threadIdx.x, nchunk = gridDim.x;
+    if (t < nchunk) {
+        const float* mm = ws_minmax(ws, im) + 2 * t;
+        smin[t] = mm[0];
+        smax[t] = mm[1];
+    }
+    for (int i = t; i < NW * NBINS; i += CT) hist[i / NBINS][i % NBINS] = 0u;
+    __syncthreads();
+    float lo = INFINITY, hi = -INFINITY;
+    for (int i = 0; i < nchunk; ++i) { lo = fminf(lo, smin[i]); hi = fmaxf(hi, smax[i]); }
+    if (!(lo <= hi)) return;                 // no non-zero value: empty histogram
+    double first = lo, last = hi;
     if (first == last) { first -= 0.5; last += 0.5; }
     const double denom = last - first;
     const double step = denom / NBINS;
-    for (int64_t i = threadIdx.x; i < npix; i += CT) {
-        const float xf = im[i];
+    int64_t b0, e0;
+    chunk_range(npix, nchunk, chunk, b0, e0);
+    const float* p = img + static_cast<int64_t>(im) * npix;
+    unsigned* h = hist[t >> 6];
+    for (int64_t i = b0 + t; i < e0; i += CT) {
+        const float xf = p[i];
         if (xf == 0.0f) continue;
         const double x = xf;
         int b = static_cast<int>(((x - first) / denom) * NBINS);
         if (b == NBINS) b -= 1;
         if (x < edge(b, first, last, step)) b -= 1;
         if (b != NBINS - 1 && x >= edge(b + 1, first, last, step)) b += 1;
-        atomicAdd(&hist[b], 1u);
+        atomicAdd(&h[b], 1u);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double total = 0.0;
-        for (int b = 0; b < NBINS; ++b) total += hist[b];
-        double s = 0.0;
-        const double denom_t = total > 1.0 ? total : 1.0;
-        for (int b = 0; b < NBINS; ++b) {
-            if (hist[b] == 0) continue;
-            const double p = hist[b] / denom_t;
-            s += p * log(p);
-        }
-        scores[blockIdx.x] = s;
+    if (t < NBINS) {
+        unsigned n = 0;
+        for (int w = 0; w < NW; ++w) n += hist[w][t];
+        if (n) atomicAdd(&ws_hist(ws, nimg, im)[t], n);
     }
+}
+
+__global__ __launch_bounds__(64) void cor_entropy_kernel(int nimg, void* __restrict__ ws,
+                                                         double* __restrict__ scores) {
+    const int im = blockIdx.x;
+    if (threadIdx.x != 0) return;
+    const unsigned* hist = ws_hist(ws, nimg, im);
+    double total = 0.0;
+    for (int b = 0; b < NBINS; ++b) total += hist[b];
+    double s = 0.0;
+    const double denom_t = total > 1.0 ? total : 1.0;
+    for (int b = 0; b < NBINS; ++b) {
+        if (hist[b] == 0) continue;
+        const double p = hist[b] / denom_t;
+        s += p * log(p);
+    }
+    scores[im] = s;
+}
+
+int chunks_for(int64_t npix) {
+    const int64_t n = (npix + PIX_PER_CHUNK - 1) / PIX_PER_CHUNK;
+    return static_cast<int>(n < 1 ? 1 : (n > MAX_CHUNKS ? MAX_CHUNKS : n));
 }
 
 }  // namespace
 
-int launch_cor_scores(Ctx* c, const float* d_img, int nimg, int64_t npix, double* d_scores,
+size_t cor_workspace_bytes(int nimg) {
+    return static_cast<size_t>(nimg) * (MAX_CHUNKS * 2 * sizeof(float) + NBINS * sizeof(unsigned));
+}
+
+int launch_cor_scores(const float* d_img, int nimg, int64_t npix, double* d_scores, void* d_ws,
                       hipStream_t s) {
-    (void)c;
     if (nimg <= 0) return MR_OK;
-    hipLaunchKernelGGL(cor_score_kernel, dim3(nimg), dim3(CT), 0, s, d_img, npix, d_scores);
+    const dim3 grid(chunks_for(npix), nimg);
+    hipLaunchKernelGGL(cor_minmax_kernel, grid, dim3(CT), 0, s, d_img, npix, nimg, d_ws);
+    hipLaunchKernelGGL(cor_hist_kernel, grid, dim3(CT), 0, s, d_img, npix, nimg, d_ws);
+    hipLaunchKernelGGL(cor_entropy_kernel, dim3(nimg), dim3(64), 0, s, nimg, d_ws, d_scores);
     MR_HIP(hipGetLastError());
     return MR_OK;
 }

@@ -196,6 +196,8 @@ struct Ctx {
     size_t pts_bytes = 0;
     double* d_misc = nullptr;
     size_t misc_bytes = 0;
+    void* d_corws = nullptr;         // COR scoring workspace
+    size_t corws_bytes = 0;
 };
 
 // A compiled request: tasks and their device work queue, re-run on demand or after
@@ -231,6 +233,7 @@ struct Plan {
     // rotation-centre scoring of the plan's images (one graph: persistent BP + scores)
     std::vector<mr_task> h_tasks;    // host copy of the tasks
     double* d_scores = nullptr;
+    void* d_corws = nullptr;         // scoring workspace of the captured graph
     hipGraphExec_t cor_exec = nullptr;
 };
 
@@ -280,7 +283,10 @@ int launch_bp_adjoint(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int n
 int launch_bp_dshift(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles, int n_views,
                      float scale, float* d_out, hipStream_t s);
 int launch_filter_adjoint(Ctx* c, const float* d_gq, int k, float* d_graw, hipStream_t s);
-int launch_cor_scores(Ctx* c, const float* d_img, int nimg, int64_t npix, double* d_scores,
+// COR entropy scores of nimg contiguous images of npix floats; d_ws: a device workspace
+// of cor_workspace_bytes(nimg) bytes (owned by the caller: a captured graph keeps it)
+size_t cor_workspace_bytes(int nimg);
+int launch_cor_scores(const float* d_img, int nimg, int64_t npix, double* d_scores, void* d_ws,
                       hipStream_t s);
 int launch_phantom(const double* d_ell, int K, int kind, const double* d_geo, int A, int R,
                    int C, double pu, double pv, double far, float* out, hipStream_t s);
