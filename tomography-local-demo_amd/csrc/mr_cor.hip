@@ -11,8 +11,9 @@
 // a search of 25 images fills the device: (1) partial min / max of the non-zero values
 // per chunk; (2) the image's range from its partials, the chunk's histogram in LDS (one
 // copy per wave, then summed) added to the image's histogram with 128 global atomics;
-// (3) one thread per image forms the score in bin order.  The counts are integers and
-// min / max are order-free, so the scores do not depend on the chunking.
+// (3) one workgroup per image forms the score (a term per bin, summed in bin order).
+// The counts are integers and min / max are order-free, so the scores do not depend on
+// the chunking.
 #include "mr_internal.h"
 
 namespace mr {
@@ -117,21 +118,34 @@ __global__ __launch_bounds__(CT) void cor_hist_kernel(const float* __restrict__ 
     }
 }
 
-__global__ __launch_bounds__(64) void cor_entropy_kernel(int nimg, void* __restrict__ ws,
-                                                         double* __restrict__ scores) {
-    const int im = blockIdx.x;
-    if (threadIdx.x != 0) return;
+// One thread per bin forms p ln p; thread 0 sums the terms in bin order.
+__global__ __launch_bounds__(NBINS) void cor_entropy_kernel(int nimg, void* __restrict__ ws,
+                                                            double* __restrict__ scores) {
+    __shared__ double term[NBINS];
+    __shared__ double s_total;
+    const int im = blockIdx.x, b = threadIdx.x;
     const unsigned* hist = ws_hist(ws, nimg, im);
-    double total = 0.0;
-    for (int b = 0; b < NBINS; ++b) total += hist[b];
-    double s = 0.0;
-    const double denom_t = total > 1.0 ? total : 1.0;
-    for (int b = 0; b < NBINS; ++b) {
-        if (hist[b] == 0) continue;
-        const double p = hist[b] / denom_t;
-        s += p * log(p);
+    if (b == 0) {
+        double total = 0.0;
+        for (int i = 0; i < NBINS; ++i) total += hist[i];
+        s_total = total;
     }
-    scores[im] = s;
+    __syncthreads();
+    const double denom_t = s_total > 1.0 ? s_total : 1.0;
+    const unsigned h = hist[b];
+    double t = 0.0;
+    if (h != 0) {
+        const double p = h / denom_t;
+        t = p * log(p);
+    }
+    term[b] = t;
+    __syncthreads();
+    if (b == 0) {
+        double s = 0.0;
+        for (int i = 0; i < NBINS; ++i)
+            if (hist[i] != 0) s += term[i];
+        scores[im] = s;
+    }
 }
 
 int chunks_for(int64_t npix) {
@@ -151,7 +165,7 @@ int launch_cor_scores(const float* d_img, int nimg, int64_t npix, double* d_scor
     const dim3 grid(chunks_for(npix), nimg);
     hipLaunchKernelGGL(cor_minmax_kernel, grid, dim3(CT), 0, s, d_img, npix, nimg, d_ws);
     hipLaunchKernelGGL(cor_hist_kernel, grid, dim3(CT), 0, s, d_img, npix, nimg, d_ws);
-    hipLaunchKernelGGL(cor_entropy_kernel, dim3(nimg), dim3(64), 0, s, nimg, d_ws, d_scores);
+    hipLaunchKernelGGL(cor_entropy_kernel, dim3(nimg), dim3(NBINS), 0, s, nimg, d_ws, d_scores);
     MR_HIP(hipGetLastError());
     return MR_OK;
 }
