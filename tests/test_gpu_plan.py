@@ -455,10 +455,11 @@ def test_repeated_requests_reuse_tiles_exactly(workload):
 
 @pytest.mark.parametrize("beam", ["cone", "parallel"])
 def test_plan_band_copy_matches_ring(workload, beam, monkeypatch):
-    """In a kernel-7 (default 16-bit) context the persistent plan launches read the band
-    copy of the ring: plan runs, streamed batches (whose graph converts each batch it
-    stores), updates after other ingest paths and COR scores equal the same plans on the
-    ring itself (kernel 2) to FP32 rounding, for f16 and bf16."""
+    """Persistent plan launches read a copy of the ring: the band copy in a kernel-7
+    (default 16-bit) context, the row-pair copy for f32 parallel beam.  Plan runs,
+    streamed batches (whose graph converts each batch it stores) and updates after other
+    ingest paths equal the same plans on the ring itself (kernel 2) to FP32 rounding, for
+    f16, bf16 and f32."""
     from minirecast import _hip, phantom, recon
     from minirecast.geometry import ParallelGeometry
     from minirecast.stream import arrival_order
@@ -479,7 +480,7 @@ def test_plan_band_copy_matches_ring(workload, beam, monkeypatch):
     A = g.n_angles
     order = arrival_order(A, "interleaved")
     batches = [order[i:i + 8] for i in range(0, A, 8)]
-    for storage in ("f16", "bf16"):
+    for storage in ("f16", "bf16", "f32"):
         outs = {}
         for variant in ("7", "2"):
             monkeypatch.setenv("MINIRECAST_BP_VARIANT", variant)
@@ -507,16 +508,32 @@ def test_plan_band_copy_matches_ring(workload, beam, monkeypatch):
     monkeypatch.delenv("MINIRECAST_BP_VARIANT")
 
 
+@pytest.mark.parametrize("beam", ["cone", "parallel"])
 @pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
-def test_oversized_slice_requests(workload, storage, monkeypatch):
-    """Slice requests whose records exceed the records buffer (here shrunk to 1 MB) run
-    the v2 loop in the default configuration (on the band copy for 16-bit rings), and the
-    chunked records path in an explicit kernel-4/7 configuration; both equal kernel 1.
-    The buffer is shrunk to 64 KB so that these small slices need several chunks."""
-    from minirecast import _hip
-    w, proj, planes, tasks, out_len = workload
-    T = np.array(tasks, dtype=_hip.TASK_DTYPE)
-    A = w.geom.n_angles
+def test_oversized_slice_requests(workload, storage, beam, monkeypatch):
+    """Slice requests whose records exceed the records buffer run the v2 loop in the
+    default configuration (on the band copy for 16-bit rings, the row-pair copy for f32
+    parallel beam), and the chunked records path in an explicit kernel-4/7 configuration;
+    both equal kernel 1.  The buffer is shrunk to 64 KB so that these small slices need
+    several chunks."""
+    from minirecast import _hip, phantom, recon
+    from minirecast.geometry import ParallelGeometry
+    if beam == "cone":
+        w, proj, planes, tasks, out_len = workload
+        g = w.geom
+        T = np.array(tasks, dtype=_hip.TASK_DTYPE)
+    else:
+        g = ParallelGeometry(np.arange(120) * np.pi / 120, 24, 80, 1.0, 1.0)
+        box = g.volume_box()
+        ph = phantom.random_ellipsoids(5, 8, box, (0.1, 1.0))
+        proj = phantom.project_parallel(ph, g)
+        T, out_len = [], 0
+        for k, sh in enumerate((-1.0, 0.0, 0.75)):
+            T.append(recon.plane_task(box, (0.0, 0.0, 2.0), (0.0, 0.0, 1.0), 70, cor_shift=sh,
+                                      out_offset=out_len))
+            out_len += 70 * 70
+        T = np.array(T, dtype=_hip.TASK_DTYPE)
+    A = g.n_angles
     monkeypatch.setenv("MINIRECAST_BP_RECORD_KB", "64")
     outs, launches = {}, {}
     for name, variant in (("default", None), ("explicit", "7" if storage != "f32" else "4"),
@@ -526,7 +543,10 @@ def test_oversized_slice_requests(workload, storage, monkeypatch):
         else:
             monkeypatch.setenv("MINIRECAST_BP_VARIANT", variant)
         ctx = _hip.Context(0)
-        ctx.set_cone(w.geom.vectors, w.geom.rows, w.geom.cols, w.window, storage)
+        if beam == "cone":
+            ctx.set_cone(g.vectors, g.rows, g.cols, "ram-lak", storage)
+        else:
+            ctx.set_parallel(g.angles, g.rows, g.cols, 1.0, 1.0, "ram-lak", storage)
         ctx.push(np.arange(A), proj)
         outs[name] = ctx.bp_tasks(T, A, out_len)
         launches[name] = ctx.last_tile_launches()

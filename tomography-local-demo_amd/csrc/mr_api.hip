@@ -1414,8 +1414,8 @@ int mr_plan_cor_scores(mr_plan* p, int n_views, double* scores) {
     if (!p->d_corws) MR_HIP(hipMalloc(&p->d_corws, cor_workspace_bytes(p->ntasks)));
     int rc0 = prepare_persistent_bp(c, n_views, c->s_req);
     if (rc0) return rc0;
-    const bool band = persistent_uses_band(c);
-    if (p->cor_exec && p->cor_band != band) {    // configuration changed since the capture
+    const int rk = v2_ring_kind(c);
+    if (p->cor_exec && p->cor_ring != rk) {      // configuration changed since the capture
         hipGraphExecDestroy(p->cor_exec);
         p->cor_exec = nullptr;
     }
@@ -1423,7 +1423,7 @@ int mr_plan_cor_scores(mr_plan* p, int n_views, double* scores) {
     MR_HIP(hipMemcpyAsync(p->d_counters + 2, &n_views, sizeof(int), hipMemcpyHostToDevice, c->s_req));
     MR_HIP(hipStreamSynchronize(c->s_req));
     if (!p->cor_exec) {
-        p->cor_band = band;
+        p->cor_ring = rk;
         TileQueue q;
         q.head = p->d_counters + 1;
         q.count = p->d_counters;
@@ -1551,12 +1551,12 @@ int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block
         fv[j].slot = c->n + j;
         gv[j] = c->geom_all[indices[j]];
     }
-    // the band copy (kernel-7 configuration) holds the views stored so far; the graph
-    // converts the batch it stores
+    // the ring copy the v2 loop reads here (band or row pair) holds the views stored so
+    // far; the graph converts the batch it stores
     rc = prepare_persistent_bp(c, c->n, c->s_req);
     if (rc) return rc;
-    const bool band = persistent_uses_band(c);
-    if (!p->graphs.empty() && p->graphs_band != band) {   // configuration changed
+    const int rk = v2_ring_kind(c);
+    if (!p->graphs.empty() && p->graphs_ring != rk) {     // configuration changed
         for (auto& g : p->graphs) hipGraphExecDestroy(g.second);
         p->graphs.clear();
     }
@@ -1582,7 +1582,8 @@ int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block
         int r = (e1 != hipSuccess || e2 != hipSuccess) ? MR_ERUNTIME : MR_OK;
         if (!r) r = launch_stream_prep(c, p->d_meta, k, p->d_counters + 2, c->s_req);
         if (!r) r = launch_filter(c, p->d_block, k, 0, c->s_req, static_cast<const FilterView*>(p->d_meta));
-        if (!r && band) r = launch_band_convert_tail(c, p->d_counters + 2, k, c->s_req);
+        if (!r && rk == 2) r = launch_band_convert_tail(c, p->d_counters + 2, k, c->s_req);
+        if (!r && rk == 1) r = launch_pair_convert_tail(c, p->d_counters + 2, k, c->s_req);
         if (!r) r = launch_reset_head(p->d_counters, c->s_req);
         if (!r) r = launch_bp_tiles(c, p->d_tasks, p->d_queue, 0, 0, 0.0f, p->d_out, c->s_req, &q);
         hipError_t ec = hipStreamEndCapture(c->s_req, &graph);
@@ -1594,7 +1595,7 @@ int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block
         hipGraphDestroy(graph);
         if (ei != hipSuccess) return fail(MR_ERUNTIME, std::string("graph instantiate: ") + hipGetErrorString(ei));
         p->graphs.emplace_back(k, exec);
-        p->graphs_band = band;
+        p->graphs_ring = rk;
     }
     p->acc_n = -1;                     // until the replay is known to have completed
     MR_HIP(hipEventRecord(c->ev0, c->s_req));
@@ -1606,7 +1607,8 @@ int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block
     if (rc) return rc;
     p->last_ms = c->last_kernel_ms;
     c->n += k;
-    if (band) c->bring_n = std::max(c->bring_n, c->n);
+    if (rk == 2) c->bring_n = std::max(c->bring_n, c->n);
+    if (rk == 1) c->pring_n = std::max(c->pring_n, c->n);
     p->acc_n = c->n;
     return MR_OK;
 }

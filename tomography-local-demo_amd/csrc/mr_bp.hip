@@ -455,10 +455,11 @@ template <typename ST> struct BandP {
     }
 };
 
-// BAND (16-bit forms): `ring` is the band copy (view_stride its elements per view), the
-// records are band records, and each point-view is one 16-byte load (BandP).
+// RK: the ring copy read, with view_stride its elements per view.  0: the ring itself
+// (two loads per point-view); 1: the row-pair copy (one load of the four taps, PairP);
+// 2 (16-bit forms): the band copy, with band records (one 16-byte load, BandP).
 // At least 5 waves per SIMD (89-95 VGPRs instead of 93-111, no spills; 6 would spill).
-template <bool CONE, typename ST, bool BAND = false>
+template <bool CONE, typename ST, int RK = 0>
 __global__ __launch_bounds__(BP_THREADS, 5) void bp_tiles_kernel2(
     const ST* __restrict__ ring, int64_t view_stride, int pitch, int R, int C,
     const ViewGeom* __restrict__ geom, const float* __restrict__ vscale,
@@ -517,8 +518,10 @@ __global__ __launch_bounds__(BP_THREADS, 5) void bp_tiles_kernel2(
 
     const float ds = static_cast<float>(tx);
     const float dp0 = static_cast<float>(ty);
-    static_assert(!BAND || sizeof(ST) == 2, "band copy of 16-bit storage only");
-    const int pitchb = BAND ? pitch << 3 : pitch << LE;     // row (band) pitch in bytes
+    static_assert(RK != 2 || sizeof(ST) == 2, "band copy of 16-bit storage only");
+    constexpr bool BAND = RK == 2;
+    constexpr int ESH = RK == 1 ? LE + 1 : LE;              // log2 bytes per ring element
+    const int pitchb = BAND ? pitch << 3 : pitch << ESH;    // row (band) pitch in bytes
     const int view_bytes = static_cast<int>(view_stride << LE);
     float acc[PPT], accb[PPT], fr_par[PPT];
     int roff_par[PPT], jsh_par[PPT];
@@ -577,7 +580,7 @@ __global__ __launch_bounds__(BP_THREADS, 5) void bp_tiles_kernel2(
             const float4 r1 = rec4[v * 4 + 1];
             const float4 r2 = rec4[v * 4 + 2];
             const float4 r3 = rec4[v * 4 + 3];
-            const int baseb = BAND ? __float_as_int(r3.z) : __float_as_int(r3.z) << LE;
+            const int baseb = BAND ? __float_as_int(r3.z) : __float_as_int(r3.z) << ESH;
             const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
                 const_cast<ST*>(ring + static_cast<int64_t>(vb + v) * view_stride), 0, view_bytes,
                 0x00020000);
@@ -604,6 +607,10 @@ __global__ __launch_bounds__(BP_THREADS, 5) void bp_tiles_kernel2(
                         BandP<ST> tap;
                         tap.load(rs, band_offset(b, cvt_flr(col), pitchb, baseb), jsh16);
                         tap.lerp(fc, top, bot);
+                    } else if constexpr (RK == 1) {
+                        PairP<ST> tap;
+                        tap.load(rs, __mul24(cvt_flr(row), pitchb) + ((cvt_flr(col) << ESH) + baseb), 0);
+                        tap.lerp(fc, top, bot);
                     } else {
                         const int voff = __mul24(cvt_flr(row), pitchb) + ((cvt_flr(col) << LE) + baseb);
                         Pair<ST> tap;              // 16-bit taps: lerps on the packed halves
@@ -624,6 +631,10 @@ __global__ __launch_bounds__(BP_THREADS, 5) void bp_tiles_kernel2(
                     if constexpr (BAND) {
                         BandP<ST> tap;
                         tap.load(rs, roff_par[m] + ((cvt_flr(col) << 3) + baseb), jsh_par[m]);
+                        tap.lerp(fc, top, bot);
+                    } else if constexpr (RK == 1) {
+                        PairP<ST> tap;
+                        tap.load(rs, roff_par[m] + ((cvt_flr(col) << ESH) + baseb), 0);
                         tap.lerp(fc, top, bot);
                     } else {
                         const int voff = roff_par[m] + ((cvt_flr(col) << LE) + baseb);
@@ -931,13 +942,16 @@ struct PointState {
 // Row-pair ring from the ring: pair-row p (padded rows p and p+1, p = 0..R+1) of view v,
 // element (col) = (ring[v][p][col], ring[v][p+1][col]).
 template <typename ST>
+// Views v0 + z, or with v_end_dev the last gridDim.z views before the device view count.
 __global__ __launch_bounds__(256) void pair_convert_kernel(const ST* __restrict__ ring,
                                                           ST* __restrict__ pring,
                                                           int64_t view_stride, int64_t pview_stride,
-                                                          int pitch, int v0) {
+                                                          int pitch, int v0,
+                                                          const int* __restrict__ v_end_dev) {
     const int col = blockIdx.x * 256 + threadIdx.x;
     const int pr = blockIdx.y;
-    const int64_t v = v0 + static_cast<int64_t>(blockIdx.z);
+    const int64_t v = (v_end_dev ? *v_end_dev - static_cast<int>(gridDim.z) : v0) +
+                      static_cast<int64_t>(blockIdx.z);
     if (col >= pitch) return;
     const ST* src = ring + v * view_stride + static_cast<int64_t>(pr) * pitch + col;
     ST* dst = pring + v * pview_stride + (static_cast<int64_t>(pr) * pitch + col) * 2;
@@ -1730,16 +1744,25 @@ int dispatch_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntil
         MR_HIP(hipGetLastError());
         return MR_OK;
     }
-    if (c->bp_variant == 2 && c->v2_band) {
+    if (c->bp_variant == 2 && c->v2_ring == 1 && c->storage == MR_STORE_F32) {
+        hipLaunchKernelGGL((bp_tiles_kernel2<CONE, float, 1>), grid, block, 0, s,
+                           static_cast<const float*>(c->pring), pair_view_stride(c), c->pitch, c->R,
+                           c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out,
+                           q.head, q.count, q.n_views, c->xcd_map != 0, q.v_begin, q.accum,
+                           q.accum_half);
+        MR_HIP(hipGetLastError());
+        return MR_OK;
+    }
+    if (c->bp_variant == 2 && c->v2_ring == 2) {
         const int64_t bvs = band_view_stride(c);
         if (c->storage == MR_STORE_F16)
-            hipLaunchKernelGGL((bp_tiles_kernel2<CONE, __half, true>), grid, block, 0, s,
+            hipLaunchKernelGGL((bp_tiles_kernel2<CONE, __half, 2>), grid, block, 0, s,
                                static_cast<const __half*>(c->bring), bvs, c->pitch, c->R,
                                c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out,
                                q.head, q.count, q.n_views, c->xcd_map != 0, q.v_begin, q.accum,
                                q.accum_half);
         else
-            hipLaunchKernelGGL((bp_tiles_kernel2<CONE, uint16_t, true>), grid, block, 0, s,
+            hipLaunchKernelGGL((bp_tiles_kernel2<CONE, uint16_t, 2>), grid, block, 0, s,
                                static_cast<const uint16_t*>(c->bring), bvs, c->pitch, c->R,
                                c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out,
                                q.head, q.count, q.n_views, c->xcd_map != 0, q.v_begin, q.accum,
@@ -1856,20 +1879,18 @@ int launch_bp_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int nti
             // f32 ring, instead of records in chunks and view blocks whose sums make a
             // round trip through the 105 MB output per launch (measured on that search:
             // f16 42 against 54 ms, f32 75 against 82 ms)
-            const bool band = persistent_uses_band(c);
-            if (band) {
-                const int rc = ensure_band_ring(c, n_views, s);
-                if (rc) return rc;
-            }
+            const int rk = v2_ring_kind(c);
+            const int rc0 = prepare_persistent_bp(c, n_views, s);
+            if (rc0) return rc0;
             const int sv = c->bp_variant;
             c->bp_variant = 2;
-            c->v2_band = band;
+            c->v2_ring = rk;
             const TileQueue none{};
             const int rc = c->kind == MR_KIND_CONE
                                ? dispatch_tiles<true>(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s, none)
                                : dispatch_tiles<false>(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s, none);
             c->bp_variant = sv;
-            c->v2_band = false;
+            c->v2_ring = 0;
             return rc;
         }
         return launch_bp_tiles_v4(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s);
@@ -1878,13 +1899,13 @@ int launch_bp_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int nti
         return fail(MR_ERUNTIME, "persistent launches need the v2 kernel");
     if (q) {                               // persistent launches always run the v2 loop
         const int saved = c->bp_variant;
-        c->v2_band = persistent_uses_band(c);
+        c->v2_ring = v2_ring_kind(c);
         c->bp_variant = 2;
         const int rc = c->kind == MR_KIND_CONE
                            ? dispatch_tiles<true>(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s, *q)
                            : dispatch_tiles<false>(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s, *q);
         c->bp_variant = saved;
-        c->v2_band = false;
+        c->v2_ring = 0;
         return rc;
     }
     return c->kind == MR_KIND_CONE
@@ -2109,11 +2130,11 @@ int ensure_pair_ring(Ctx* c, int n_views, hipStream_t s) {
         if (esz == 4)
             hipLaunchKernelGGL(pair_convert_kernel<float>, grid, dim3(256), 0, s,
                                static_cast<const float*>(c->ring), static_cast<float*>(c->pring),
-                               c->view_stride, pvs, c->pitch, v0);
+                               c->view_stride, pvs, c->pitch, v0, nullptr);
         else
             hipLaunchKernelGGL(pair_convert_kernel<uint16_t>, grid, dim3(256), 0, s,
                                static_cast<const uint16_t*>(c->ring), static_cast<uint16_t*>(c->pring),
-                               c->view_stride, pvs, c->pitch, v0);
+                               c->view_stride, pvs, c->pitch, v0, nullptr);
         MR_HIP(hipGetLastError());
     }
     c->pring_n = std::max(c->pring_n, v1);
@@ -2153,12 +2174,35 @@ static bool band_fits(const Ctx* c) {
            (static_cast<int64_t>(c->pitch) << 3) < (int64_t{1} << 23);
 }
 
-// Persistent launches of a context configured for kernel 7 run the v2 loop on the band
-// copy (16-bit storage).
-bool persistent_uses_band(const Ctx* c) { return c->bp_variant == 7 && band_fits(c); }
+int64_t pair_view_stride(const Ctx* c) { return 2 * static_cast<int64_t>(c->R + 2) * c->pitch; }
+
+// The ring copy the v2 loop reads in this context: the band copy for a kernel-7 context
+// of 16-bit storage; the row-pair copy for f32 parallel beam (its points keep one detector
+// row, so one 16-byte load brings both rows; measured on the parallel COR search: 38
+// against 73 ms; oblique cone slices lose, 7.4 against 6.7 ms); else the ring itself.
+int v2_ring_kind(const Ctx* c) {
+    if (c->bp_variant == 7 && band_fits(c)) return 2;
+    if (c->storage == MR_STORE_F32 && c->kind == MR_KIND_PARALLEL &&
+        (c->bp_variant == 4 || c->bp_variant >= 6))
+        return 1;
+    return 0;
+}
 
 int prepare_persistent_bp(Ctx* c, int n_views, hipStream_t s) {
-    return persistent_uses_band(c) ? ensure_band_ring(c, n_views, s) : MR_OK;
+    const int rk = v2_ring_kind(c);
+    return rk == 2 ? ensure_band_ring(c, n_views, s) : rk == 1 ? ensure_pair_ring(c, n_views, s) : MR_OK;
+}
+
+// In a captured streaming graph: row-pair copy of the k views just stored, ending at the
+// device view count.
+int launch_pair_convert_tail(Ctx* c, const int* d_n, int k, hipStream_t s) {
+    if (!c->pring || k <= 0 || c->storage != MR_STORE_F32) return fail(MR_ERUNTIME, "pair copy not allocated");
+    const dim3 grid((c->pitch + 255) / 256, c->R + 2, k);
+    hipLaunchKernelGGL(pair_convert_kernel<float>, grid, dim3(256), 0, s,
+                       static_cast<const float*>(c->ring), static_cast<float*>(c->pring),
+                       c->view_stride, pair_view_stride(c), c->pitch, 0, d_n);
+    MR_HIP(hipGetLastError());
+    return MR_OK;
 }
 
 // In a captured streaming graph: band copy of the k views just stored, ending at the

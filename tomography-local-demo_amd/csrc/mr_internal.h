@@ -159,7 +159,7 @@ struct Ctx {
     // forgotten (views_epoch): plans check them before reusing graphs or running sums
     uint64_t geom_epoch = 0, views_epoch = 0;
     std::vector<TileCache> tile_cache;   // most recent first, at most kTileCacheEntries
-    bool v2_band = false;            // the v2 launch in progress reads the band copy
+    int v2_ring = 0;                 // ring copy of the v2 launch in progress (v2_ring_kind)
 
     // host copies
     std::vector<ViewGeom> geom_all;   // per geometry index
@@ -228,8 +228,8 @@ struct Plan {
     float* d_acc = nullptr;
     int acc_n = -1;
     uint64_t acc_epoch = 0;
-    // whether the captured graphs read the band copy (recaptured when that changes)
-    bool graphs_band = false, cor_band = false;
+    // ring copy the captured graphs read (v2_ring_kind; recaptured when that changes)
+    int graphs_ring = 0, cor_ring = 0;
     // rotation-centre scoring of the plan's images (one graph: persistent BP + scores)
     std::vector<mr_task> h_tasks;    // host copy of the tasks
     double* d_scores = nullptr;
@@ -270,9 +270,12 @@ int launch_reset_head(int* d_counters, hipStream_t s);
 // views a launch reads (prepare before launching or capturing; in a streaming graph the
 // tail conversion brings in the batch just stored)
 int64_t band_view_stride(const Ctx* c);
-bool persistent_uses_band(const Ctx* c);
+int64_t pair_view_stride(const Ctx* c);
+int v2_ring_kind(const Ctx* c);      // 0 the ring, 1 the row-pair copy, 2 the band copy
 int prepare_persistent_bp(Ctx* c, int n_views, hipStream_t s);
 int launch_band_convert_tail(Ctx* c, const int* d_n, int k, hipStream_t s);
+int launch_pair_convert_tail(Ctx* c, const int* d_n, int k, hipStream_t s);
+int ensure_pair_ring(Ctx* c, int n_views, hipStream_t s);
 int launch_stream_prep(Ctx* c, const void* d_meta, int k, int* d_nviews, hipStream_t s);
 int persistent_grid(Ctx* c);
 int launch_bp_points(Ctx* c, const double* d_pts, int64_t M, double cor_shift, int n_views,
