@@ -158,7 +158,10 @@ typedef struct mr_task {
  * recon.py:498-512 estimate_cor.  out: float32, host (out_on_device = 0) or device
  * (1), sized by the caller to hold every task's output range.  Every output element
  * of every task is written (points outside the volume box with 0); elements that no
- * task addresses are left as they are. */
+ * task addresses are left as they are.  A slice batch whose per-(tile, view) records
+ * would exceed the 512 MB records buffer runs on the work-queue loop of kernel 2 (records
+ * built per tile in LDS) under the default configuration; an explicit kernel choice
+ * processes it in chunks of tiles instead. */
 int mr_bp_tasks(mr_ctx* ctx, const mr_task* tasks, int ntasks, int n_views,
                 float* out, int64_t out_len, int out_on_device);
 
@@ -250,7 +253,8 @@ int mr_reset_views(mr_ctx* ctx);
  *
  * A plan compiles a fixed batch of tasks once: its tiles are enumerated on the device
  * (in-box test included) into a work queue that a persistent back-projection launch
- * drains (in a context configured for kernel 7, reading the band copy of a 16-bit ring).
+ * drains (in a context configured for kernel 7, reading the band copy of a 16-bit ring;
+ * for F32 parallel beam with kernel 4 or 6 and later, reading the row-pair copy).
  * out_len floats of output live in device memory owned by the plan. */
 typedef struct mr_plan mr_plan;
 int mr_plan_create(mr_ctx* ctx, const mr_task* tasks, int ntasks, int64_t out_len, mr_plan** out);
