@@ -215,10 +215,10 @@ int mr_set_bp_views_per_pass(mr_ctx* ctx, int views);
 int mr_get_bp_views_per_pass(mr_ctx* ctx, int* views);
 
 /* Rotation-centre search scores: recon.py:498-513.  Each task is one trial image
- * (nq = 1, all np x ns equal; out_offset/strides are ignored): it is back-projected
- * into a device buffer and scored on the device as sum p ln p of a 128-bin
- * histogram of its non-zero values (np.histogram range min..max).
- * scores: host [ntasks] float64. */
+ * (nq = 1; out_offset/strides are ignored; sizes may differ, so the trial images of
+ * several slices can be scored in one call): it is back-projected into a device buffer
+ * and scored on the device as sum p ln p of a 128-bin histogram of its non-zero values
+ * (np.histogram range min..max).  scores: host [ntasks] float64. */
 int mr_cor_scores(mr_ctx* ctx, const mr_task* tasks, int ntasks, int n_views, double* scores);
 
 /* ---- differentiation (no counterpart in the reference: the transposes of the maps above)
@@ -282,12 +282,12 @@ int mr_plan_stream(mr_plan* plan, const int64_t* indices, int k, const float* bl
  * recon.py:426-475).  Copies the output to out_host when it is not NULL. */
 int mr_plan_update(mr_plan* plan, float* out_host);
 
-/* Rotation-centre scores (recon.py:498-513) of a plan whose tasks are the trial images
- * of one search: equal-size single planes, contiguous in task order (out_offset =
- * i * np * ns).  Back-projects them from the first n_views stored views (< 0: all) in
- * the plan's persistent launch and scores each image (histogram entropy of its non-zero
- * values, as mr_cor_scores) on the device, both in one HIP graph captured on the first
- * call and replayed after; scores[ntasks]. */
+/* Rotation-centre scores (recon.py:498-513) of a plan whose tasks are trial images:
+ * single planes of contiguous rows (nq = 1, stride_p = ns, stride_s = 1), any sizes (one
+ * plan can hold the searches of several slices).  Back-projects them from the first
+ * n_views stored views (< 0: all) in the plan's persistent launch and scores each image
+ * (histogram entropy of its non-zero values, as mr_cor_scores) on the device, both in one
+ * HIP graph captured on the first call and replayed after; scores[ntasks]. */
 int mr_plan_cor_scores(mr_plan* plan, int n_views, double* scores);
 
 /* Device pointer and length of the plan's output. */

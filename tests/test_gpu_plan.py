@@ -291,7 +291,8 @@ def test_autotune_views_per_pass(workload):
 def test_plan_cor_scores_graph(workload, storage, monkeypatch):
     """The rotation-centre search as one graph (persistent back-projection of every trial
     slice + device scoring): every replay, with eager work in between, matches the
-    one-launch search and picks the same shift; plans that are not a search raise."""
+    one-launch search and picks the same shift; a plan whose tasks are not single images
+    of contiguous rows (here a stack of z-planes) raises."""
     from minirecast import _hip, recon
     w, proj, planes, tasks, out_len = workload
     A = w.geom.n_angles
@@ -317,9 +318,52 @@ def test_plan_cor_scores_graph(workload, storage, monkeypatch):
                 np.testing.assert_array_equal(got, first)   # replay after eager work
         imgs = plan.run(nv)                                 # the images stay readable
         assert rel_l2(imgs, ctx.bp_tasks(T, nv, len(shifts) * n * n)) < 1e-6
-    bad = _hip.Plan(ctx, np.array(tasks, dtype=_hip.TASK_DTYPE), out_len)
+    vt = recon.volume_task(-box / 2, 0.1, (8, 8, 4), 0, 4)
+    bad = _hip.Plan(ctx, np.array([vt], dtype=_hip.TASK_DTYPE), 8 * 8 * 4)
     with pytest.raises(ValueError):
         bad.cor_scores()
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_cor_scores_of_uneven_images(workload, storage):
+    """The searches of several slice sizes scored together, as one request and as one
+    plan (one persistent launch over every trial image + scoring, one graph), equal the
+    per-size calls exactly: each image's sums and histogram do not depend on the other
+    images.  The plan holds the images in any order and with gaps between them."""
+    from minirecast import _hip, recon
+    w, proj, planes, tasks, out_len = workload
+    A = w.geom.n_angles
+    box = w.geom.volume_box()
+    shifts = np.arange(-1.0, 1.01, 0.5)
+    sizes = ((40, 0.3), (24, -0.2), (33, 0.0))
+    groups = [np.array([recon.plane_task(box, (0.0, 0.0, z), (0.0, 0.0, 1.0), n, cor_shift=s)
+                        for s in shifts], dtype=_hip.TASK_DTYPE) for n, z in sizes]
+    ctx = _hip.Context(0)
+    ctx.set_cone(w.geom.vectors, w.geom.rows, w.geom.cols, "ram-lak", storage)
+    ctx.push(np.arange(A), proj)
+    sep = [ctx.cor_scores(G, A) for G in groups]
+    allT = np.concatenate(groups)
+    np.testing.assert_array_equal(ctx.cor_scores(allT, A), np.concatenate(sep))
+    # plans: per size (contiguous images) and one plan of all sizes, reversed, 7-pixel gaps
+    per = []
+    for G in groups:
+        P = G.copy()
+        npx = int(P[0]['np']) * int(P[0]['ns'])
+        P['out_offset'] = np.arange(len(P)) * npx
+        pl = _hip.Plan(ctx, P, len(P) * npx)
+        per.append(pl.cor_scores(A))
+        pl.close()
+    P = allT[::-1].copy()
+    off = 0
+    for i in range(len(P)):
+        P[i]['out_offset'] = off
+        off += int(P[i]['np']) * int(P[i]['ns']) + 7
+    pl = _hip.Plan(ctx, P, off)
+    for _ in range(2):                                   # capture, then a replay
+        got = pl.cor_scores(A)[::-1]
+        np.testing.assert_array_equal(got, np.concatenate(per))
+    pl.close()
+    np.testing.assert_allclose(np.concatenate(per), np.concatenate(sep), rtol=2e-3, atol=2e-3)
 
 
 @pytest.mark.parametrize("beam", ["cone", "parallel"])

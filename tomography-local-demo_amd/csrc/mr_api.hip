@@ -1399,19 +1399,25 @@ int mr_plan_cor_scores(mr_plan* p, int n_views, double* scores) {
     MR_CHECK(p->geom_epoch == c->geom_epoch, MR_ERUNTIME, "the plan was made for an earlier geometry");
     if (n_views < 0) n_views = c->n;
     MR_CHECK(n_views <= c->n, MR_EVALUE, "n_views beyond the stored views");
-    // the images must be the contiguous equal-size single-plane outputs the scoring
-    // kernel reads: task i at out_offset i * npix, rows of ns pixels
-    const mr_task& T0 = p->h_tasks[0];
-    const int64_t npix = static_cast<int64_t>(T0.np) * T0.ns;
+    // each task is one image the scoring kernels read as np * ns contiguous pixels from
+    // its out_offset (sizes may differ: the trial images of several slices)
+    std::vector<int64_t> offs(p->ntasks), npix(p->ntasks);
     for (int i = 0; i < p->ntasks; ++i) {
         const mr_task& T = p->h_tasks[i];
-        MR_CHECK(T.nq == 1 && static_cast<int64_t>(T.np) * T.ns == npix && T.out_offset == i * npix &&
-                     T.stride_p == T.ns && T.stride_s == 1,
-                 MR_EVALUE, "score plans need single images of equal size, contiguous in task order");
+        MR_CHECK(T.nq == 1 && T.stride_p == T.ns && T.stride_s == 1, MR_EVALUE,
+                 "score plans need single images of contiguous rows (nq = 1, stride_p = ns, stride_s = 1)");
+        offs[i] = T.out_offset;
+        npix[i] = static_cast<int64_t>(T.np) * T.ns;
     }
     MR_HIP(hipSetDevice(c->device));
     if (!p->d_scores) MR_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_scores), sizeof(double) * p->ntasks));
-    if (!p->d_corws) MR_HIP(hipMalloc(&p->d_corws, cor_workspace_bytes(p->ntasks)));
+    if (!p->d_corws) {
+        MR_HIP(hipMalloc(&p->d_corws, cor_workspace_bytes(p->ntasks)));
+        std::vector<CorImage> imgs;
+        p->cor_chunks = cor_images(offs.data(), npix.data(), p->ntasks, imgs);
+        const int rc = write_cor_images(imgs, p->d_corws, c->s_req);
+        if (rc) return rc;
+    }
     int rc0 = prepare_persistent_bp(c, n_views, c->s_req);
     if (rc0) return rc0;
     const int rk = v2_ring_kind(c);
@@ -1433,7 +1439,7 @@ int mr_plan_cor_scores(mr_plan* p, int n_views, double* scores) {
         MR_HIP(hipStreamBeginCapture(c->s_req, hipStreamCaptureModeThreadLocal));
         int r = launch_reset_head(p->d_counters, c->s_req);
         if (!r) r = launch_bp_tiles(c, p->d_tasks, p->d_queue, 0, 0, 0.0f, p->d_out, c->s_req, &q);
-        if (!r) r = launch_cor_scores(p->d_out, p->ntasks, npix, p->d_scores, p->d_corws, c->s_req);
+        if (!r) r = launch_cor_scores(p->d_out, p->ntasks, p->cor_chunks, p->d_scores, p->d_corws, c->s_req);
         hipError_t ec = hipStreamEndCapture(c->s_req, &graph);
         if (r || ec != hipSuccess) {
             if (graph) hipGraphDestroy(graph);
@@ -1645,28 +1651,36 @@ int mr_cor_scores(mr_ctx* c, const mr_task* tasks, int ntasks, int n_views, doub
     MR_CHECK(n_views >= 0 && n_views <= c->n, MR_EVALUE, "n_views outside the stored range");
     if (ntasks <= 0) return MR_OK;
     MR_CHECK(tasks && scores, MR_EVALUE, "null tasks or scores");
-    const int64_t npix = static_cast<int64_t>(tasks[0].np) * tasks[0].ns;
+    // the images go one after another into a device buffer, whatever their sizes
     std::vector<mr_task> T(tasks, tasks + ntasks);
+    std::vector<int64_t> offs(ntasks), npix(ntasks);
+    int64_t total = 0;
     for (int i = 0; i < ntasks; ++i) {
-        MR_CHECK(T[i].nq == 1 && static_cast<int64_t>(T[i].np) * T[i].ns == npix, MR_EVALUE,
-                 "score tasks must be single images of equal size");
-        T[i].out_offset = i * npix;
-        T[i].stride_q = npix;
+        MR_CHECK(T[i].nq == 1 && T[i].np >= 0 && T[i].ns >= 0, MR_EVALUE, "score tasks must be single images");
+        npix[i] = static_cast<int64_t>(T[i].np) * T[i].ns;
+        offs[i] = total;
+        T[i].out_offset = total;
+        T[i].stride_q = npix[i];
         T[i].stride_p = T[i].ns;
         T[i].stride_s = 1;
+        total += npix[i];
     }
     MR_HIP(hipSetDevice(c->device));
-    const int64_t total = npix * ntasks;
     rc = ensure(reinterpret_cast<void**>(&c->d_out), &c->out_bytes, sizeof(float) * total);
     if (rc) return rc;
     rc = ensure(reinterpret_cast<void**>(&c->d_misc), &c->misc_bytes, sizeof(double) * ntasks);
     if (rc) return rc;
+    // image descriptors first: their copy waits for the stream, which is idle here
+    rc = ensure(&c->d_corws, &c->corws_bytes, cor_workspace_bytes(ntasks));
+    if (rc) return rc;
+    std::vector<CorImage> imgs;
+    const int chunks = cor_images(offs.data(), npix.data(), ntasks, imgs);
+    rc = write_cor_images(imgs, c->d_corws, c->s_req);
+    if (rc) return rc;
     MR_HIP(hipMemsetAsync(c->d_out, 0, sizeof(float) * total, c->s_req));
     rc = run_tasks(c, T.data(), ntasks, n_views, c->d_out);
     if (rc) return rc;
-    rc = ensure(&c->d_corws, &c->corws_bytes, cor_workspace_bytes(ntasks));
-    if (rc) return rc;
-    rc = launch_cor_scores(c->d_out, ntasks, npix, c->d_misc, c->d_corws, c->s_req);
+    rc = launch_cor_scores(c->d_out, ntasks, chunks, c->d_misc, c->d_corws, c->s_req);
     if (rc) return rc;
     MR_HIP(hipMemcpyAsync(scores, c->d_misc, sizeof(double) * ntasks, hipMemcpyDeviceToHost, c->s_req));
     return finish_timing(c);

@@ -13,7 +13,9 @@
 // copy per wave, then summed) added to the image's histogram with 128 global atomics;
 // (3) one workgroup per image forms the score (a term per bin, summed in bin order).
 // The counts are integers and min / max are order-free, so the scores do not depend on
-// the chunking.
+// the chunking.  Images may differ in size (the trial images of several slices scored
+// together): each has its own offset, pixel count and chunk count in a descriptor, and
+// the grid spans the largest chunk count (surplus workgroups of a smaller image return).
 #include "mr_internal.h"
 
 namespace mr {
@@ -30,7 +32,8 @@ __device__ inline double edge(int i, double first, double last, double step) {
     return (i == NBINS) ? last : static_cast<double>(i) * step + first;
 }
 
-// Workspace: [nimg][MAX_CHUNKS][2] float partial (min, max), then [nimg][NBINS] counts.
+// Workspace: [nimg][MAX_CHUNKS][2] float partial (min, max), then [nimg][NBINS] counts,
+// then [nimg] image descriptors (written by the host before the launches).
 __device__ inline float* ws_minmax(void* ws, int img) {
     return static_cast<float*>(ws) + static_cast<int64_t>(img) * MAX_CHUNKS * 2;
 }
@@ -39,20 +42,27 @@ __device__ inline unsigned* ws_hist(void* ws, int nimg, int img) {
            static_cast<int64_t>(img) * NBINS;
 }
 
+__host__ __device__ inline CorImage* ws_images(void* ws, int nimg) {
+    return reinterpret_cast<CorImage*>(static_cast<char*>(ws) +
+                                       static_cast<size_t>(nimg) * (MAX_CHUNKS * 2 * sizeof(float) + NBINS * sizeof(unsigned)));
+}
+
 __device__ inline void chunk_range(int64_t npix, int nchunk, int chunk, int64_t& b, int64_t& e) {
     const int64_t per = (npix + nchunk - 1) / nchunk;
     b = static_cast<int64_t>(chunk) * per;
     e = b + per < npix ? b + per : npix;
 }
 
-__global__ __launch_bounds__(CT) void cor_minmax_kernel(const float* __restrict__ img, int64_t npix,
-                                                        int nimg, void* __restrict__ ws) {
+__global__ __launch_bounds__(CT) void cor_minmax_kernel(const float* __restrict__ img, int nimg,
+                                                        void* __restrict__ ws) {
     __shared__ float smin[CT], smax[CT];
     const int im = blockIdx.y, chunk = blockIdx.x, t = threadIdx.x;
+    const CorImage d = ws_images(ws, nimg)[im];
+    if (chunk >= d.nchunk) return;                                // the whole workgroup
     if (chunk == 0 && t < NBINS) ws_hist(ws, nimg, im)[t] = 0u;    // cleared for pass 2
     int64_t b, e;
-    chunk_range(npix, gridDim.x, chunk, b, e);
-    const float* p = img + static_cast<int64_t>(im) * npix;
+    chunk_range(d.npix, d.nchunk, chunk, b, e);
+    const float* p = img + d.offset;
     float lo = INFINITY, hi = -INFINITY;
     for (int64_t i = b + t; i < e; i += CT) {
         const float x = p[i];
@@ -75,13 +85,16 @@ __global__ __launch_bounds__(CT) void cor_minmax_kernel(const float* __restrict_
     }
 }
 
-__global__ __launch_bounds__(CT) void cor_hist_kernel(const float* __restrict__ img, int64_t npix,
-                                                      int nimg, void* __restrict__ ws) {
+__global__ __launch_bounds__(CT) void cor_hist_kernel(const float* __restrict__ img, int nimg,
+                                                      void* __restrict__ ws) {
 #pragma clang fp contract(off)
     constexpr int NW = CT / 64;
     __shared__ float smin[MAX_CHUNKS], smax[MAX_CHUNKS];
     __shared__ unsigned hist[NW][NBINS];
-    const int im = blockIdx.y, chunk = blockIdx.x, t = threadIdx.x, nchunk = gridDim.x;
+    const int im = blockIdx.y, chunk = blockIdx.x, t = threadIdx.x;
+    const CorImage d = ws_images(ws, nimg)[im];
+    const int nchunk = d.nchunk;
+    if (chunk >= nchunk) return;                                  // the whole workgroup
     if (t < nchunk) {
         const float* mm = ws_minmax(ws, im) + 2 * t;
         smin[t] = mm[0];
@@ -97,8 +110,8 @@ __global__ __launch_bounds__(CT) void cor_hist_kernel(const float* __restrict__ 
     const double denom = last - first;
     const double step = denom / NBINS;
     int64_t b0, e0;
-    chunk_range(npix, nchunk, chunk, b0, e0);
-    const float* p = img + static_cast<int64_t>(im) * npix;
+    chunk_range(d.npix, nchunk, chunk, b0, e0);
+    const float* p = img + d.offset;
     unsigned* h = hist[t >> 6];
     for (int64_t i = b0 + t; i < e0; i += CT) {
         const float xf = p[i];
@@ -156,15 +169,39 @@ int chunks_for(int64_t npix) {
 }  // namespace
 
 size_t cor_workspace_bytes(int nimg) {
-    return static_cast<size_t>(nimg) * (MAX_CHUNKS * 2 * sizeof(float) + NBINS * sizeof(unsigned));
+    return static_cast<size_t>(nimg) *
+           (MAX_CHUNKS * 2 * sizeof(float) + NBINS * sizeof(unsigned) + sizeof(CorImage));
 }
 
-int launch_cor_scores(const float* d_img, int nimg, int64_t npix, double* d_scores, void* d_ws,
+int cor_images(const int64_t* offsets, const int64_t* npix, int nimg, std::vector<CorImage>& out) {
+    out.resize(nimg);
+    int most = 1;
+    for (int i = 0; i < nimg; ++i) {
+        out[i].offset = offsets[i];
+        out[i].npix = npix[i];
+        out[i].nchunk = chunks_for(npix[i]);
+        out[i].pad = 0;
+        most = std::max(most, out[i].nchunk);
+    }
+    return most;
+}
+
+int write_cor_images(const std::vector<CorImage>& imgs, void* d_ws, hipStream_t s) {
+    const int nimg = static_cast<int>(imgs.size());
+    if (nimg == 0) return MR_OK;
+    MR_HIP(hipMemcpyAsync(ws_images(d_ws, nimg), imgs.data(), sizeof(CorImage) * imgs.size(),
+                          hipMemcpyHostToDevice, s));
+    // the host vector may go out of scope when the caller returns
+    MR_HIP(hipStreamSynchronize(s));
+    return MR_OK;
+}
+
+int launch_cor_scores(const float* d_img, int nimg, int max_chunks, double* d_scores, void* d_ws,
                       hipStream_t s) {
     if (nimg <= 0) return MR_OK;
-    const dim3 grid(chunks_for(npix), nimg);
-    hipLaunchKernelGGL(cor_minmax_kernel, grid, dim3(CT), 0, s, d_img, npix, nimg, d_ws);
-    hipLaunchKernelGGL(cor_hist_kernel, grid, dim3(CT), 0, s, d_img, npix, nimg, d_ws);
+    const dim3 grid(std::max(1, std::min(max_chunks, MAX_CHUNKS)), nimg);
+    hipLaunchKernelGGL(cor_minmax_kernel, grid, dim3(CT), 0, s, d_img, nimg, d_ws);
+    hipLaunchKernelGGL(cor_hist_kernel, grid, dim3(CT), 0, s, d_img, nimg, d_ws);
     hipLaunchKernelGGL(cor_entropy_kernel, dim3(nimg), dim3(NBINS), 0, s, nimg, d_ws, d_scores);
     MR_HIP(hipGetLastError());
     return MR_OK;

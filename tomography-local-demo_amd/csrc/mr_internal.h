@@ -230,6 +230,7 @@ struct Plan {
     uint64_t acc_epoch = 0;
     // ring copy the captured graphs read (v2_ring_kind; recaptured when that changes)
     int graphs_ring = 0, cor_ring = 0;
+    int cor_chunks = 0;           // grid width of the scoring kernels (largest chunk count)
     // rotation-centre scoring of the plan's images (one graph: persistent BP + scores)
     std::vector<mr_task> h_tasks;    // host copy of the tasks
     double* d_scores = nullptr;
@@ -286,10 +287,19 @@ int launch_bp_adjoint(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int n
 int launch_bp_dshift(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles, int n_views,
                      float scale, float* d_out, hipStream_t s);
 int launch_filter_adjoint(Ctx* c, const float* d_gq, int k, float* d_graw, hipStream_t s);
-// COR entropy scores of nimg contiguous images of npix floats; d_ws: a device workspace
-// of cor_workspace_bytes(nimg) bytes (owned by the caller: a captured graph keeps it)
+// COR entropy scores of nimg images, image i = npix[i] floats from d_img + offset[i].
+// d_ws: a device workspace of cor_workspace_bytes(nimg) bytes (owned by the caller: a
+// captured graph keeps it) that also holds the image descriptors: cor_images forms them
+// (and returns the largest chunk count, the launch's grid width), write_cor_images copies
+// them in (outside any capture; they stay valid until the workspace is rewritten).
+struct CorImage {
+    int64_t offset, npix;
+    int nchunk, pad;
+};
 size_t cor_workspace_bytes(int nimg);
-int launch_cor_scores(const float* d_img, int nimg, int64_t npix, double* d_scores, void* d_ws,
+int cor_images(const int64_t* offsets, const int64_t* npix, int nimg, std::vector<CorImage>& out);
+int write_cor_images(const std::vector<CorImage>& imgs, void* d_ws, hipStream_t s);
+int launch_cor_scores(const float* d_img, int nimg, int max_chunks, double* d_scores, void* d_ws,
                       hipStream_t s);
 int launch_phantom(const double* d_ell, int K, int kind, const double* d_geo, int A, int R,
                    int C, double pu, double pv, double far, float* out, hipStream_t s);
