@@ -1412,11 +1412,18 @@ int mr_plan_cor_scores(mr_plan* p, int n_views, double* scores) {
     MR_HIP(hipSetDevice(c->device));
     if (!p->d_scores) MR_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_scores), sizeof(double) * p->ntasks));
     if (!p->d_corws) {
-        MR_HIP(hipMalloc(&p->d_corws, cor_workspace_bytes(p->ntasks)));
+        // kept only once its image descriptors are in place
+        void* ws = nullptr;
+        MR_HIP(hipMalloc(&ws, cor_workspace_bytes(p->ntasks)));
         std::vector<CorImage> imgs;
-        p->cor_chunks = cor_images(offs.data(), npix.data(), p->ntasks, imgs);
-        const int rc = write_cor_images(imgs, p->d_corws, c->s_req);
-        if (rc) return rc;
+        const int chunks = cor_images(offs.data(), npix.data(), p->ntasks, imgs);
+        const int rc = write_cor_images(imgs, ws, c->s_req);
+        if (rc) {
+            hipFree(ws);
+            return rc;
+        }
+        p->d_corws = ws;
+        p->cor_chunks = chunks;
     }
     int rc0 = prepare_persistent_bp(c, n_views, c->s_req);
     if (rc0) return rc0;
