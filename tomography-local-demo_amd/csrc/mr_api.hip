@@ -592,6 +592,10 @@ int mr_set_geometry_parallel(mr_ctx* c, int n_total, int rows, int cols, const d
         return rc;
     }
     c->kind = MR_KIND_PARALLEL;
+    // measured default for f32 parallel beam: kernel 6, the row-pair copy (a point keeps
+    // one detector row pair in every view, one 8-byte load per tap pair; a COR search of
+    // 75 images of up to 1024^2 over 1800 views: 44 ms against 74.5 ms on the ring)
+    if (!c->bp_explicit && storage == MR_STORE_F32) c->bp_variant = 6;
     return MR_OK;
 }
 

@@ -1870,15 +1870,16 @@ int launch_bp_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int nti
             c->bp_depth = sd;
             return rc;
         }
-        if (!c->bp_explicit && (c->bp_variant == 7 || c->bp_variant == 4) && c->req_slices &&
+        if (!c->bp_explicit && (c->bp_variant == 7 || c->bp_variant == 6 || c->bp_variant == 4) && c->req_slices &&
             static_cast<size_t>(ntiles) * n_views * BP_REC * sizeof(float) > c->rec_budget) {
             // default configuration, slice requests whose records exceed one pass of the
             // records buffer (a COR search of 25 trial images of 1024^2 over 1800 views
             // needs 2.9 GB): the v2 loop, which builds each tile's records in LDS and keeps
-            // every view's sums in registers, on the band copy of a 16-bit ring or on the
-            // f32 ring, instead of records in chunks and view blocks whose sums make a
-            // round trip through the 105 MB output per launch (measured on that search:
-            // f16 42 against 54 ms, f32 75 against 82 ms)
+            // every view's sums in registers, on the band copy of a 16-bit ring, the
+            // row-pair copy of an f32 parallel-beam ring or the f32 ring itself, instead
+            // of records in chunks and view blocks whose sums make a round trip through
+            // the 105 MB output per launch (measured on that search: f16 38 against 54
+            // ms, f32 parallel 38 against 82 ms)
             const int rk = v2_ring_kind(c);
             const int rc0 = prepare_persistent_bp(c, n_views, s);
             if (rc0) return rc0;
