@@ -188,9 +188,11 @@ int mr_clip_histogram(mr_ctx* ctx, const float* vol, int64_t n, int on_device, d
  * work queue, records staged in LDS), 4 (records precomputed in global memory; default
  * for F32 cone beam, 2 points, 2 views in flight), 5 (cone only: per-view detector boxes
  * staged in LDS), 6 (kernel 4 reading a row-pair copy of the ring, one wide load per tap
- * quad; default for F32 parallel beam, 2 points, 2 views in flight) or 7 (kernel 4 reading a copy of a 16-bit ring in bands of 4 rows stepping 3, one
- * 16-byte load per tap quad; F32 storage runs it as 6; default for F16 and BF16 storage,
- * 2 points, 3 views in flight, with volume requests run as kernel 6, 4 points, 2 views);
+ * quad; default for parallel beam, 2 points, 2 views in flight) or 7 (kernel 4 reading
+ * a copy of a 16-bit ring in bands of 4 rows stepping 3, one 16-byte load per tap quad;
+ * F32 storage runs it as 6; default for F16 and BF16 storage
+ * in cone beam, 2 points, 3 views in flight, with volume requests run as kernel 6, 4
+ * points, 2 views);
  * points per thread 1, 2 or 4; lane layout 0 (waves of 2 x 32 points) or 1 (4 x 16);
  * 2 or 3 views in flight (kernels 4, 6 and 7).
  * Until this is called the configuration follows the storage form of the geometry. */
@@ -253,8 +255,9 @@ int mr_reset_views(mr_ctx* ctx);
  *
  * A plan compiles a fixed batch of tasks once: its tiles are enumerated on the device
  * (in-box test included) into a work queue that a persistent back-projection launch
- * drains (in a context configured for kernel 7, reading the band copy of a 16-bit ring;
- * for F32 parallel beam with kernel 4 or 6 and later, reading the row-pair copy).
+ * drains (reading the band copy of a 16-bit ring in a context configured for kernel 7,
+ * or for kernel 6 with parallel beam; for F32 parallel beam with kernel 4 or 6 and later,
+ * the row-pair copy).
  * out_len floats of output live in device memory owned by the plan. */
 typedef struct mr_plan mr_plan;
 int mr_plan_create(mr_ctx* ctx, const mr_task* tasks, int ntasks, int64_t out_len, mr_plan** out);

@@ -592,10 +592,15 @@ int mr_set_geometry_parallel(mr_ctx* c, int n_total, int rows, int cols, const d
         return rc;
     }
     c->kind = MR_KIND_PARALLEL;
-    // measured default for f32 parallel beam: kernel 6, the row-pair copy (a point keeps
-    // one detector row pair in every view, one 8-byte load per tap pair; a COR search of
-    // 75 images of up to 1024^2 over 1800 views: 44 ms against 74.5 ms on the ring)
-    if (!c->bp_explicit && storage == MR_STORE_F32) c->bp_variant = 6;
+    // measured default for parallel beam: kernel 6, the row-pair copy (a point keeps one
+    // detector row pair in every view; a COR search of 75 images of up to 1024^2 over
+    // 1800 views with its records in memory: f32 44 ms against 74.5 ms on the ring, f16
+    // 44 ms against 47 ms on the band copy with kernel 7's 3 views in flight)
+    if (!c->bp_explicit) {
+        c->bp_variant = 6;
+        c->bp_depth = 2;
+        c->bp_sync = 0;
+    }
     return MR_OK;
 }
 

@@ -2177,15 +2177,15 @@ static bool band_fits(const Ctx* c) {
 
 int64_t pair_view_stride(const Ctx* c) { return 2 * static_cast<int64_t>(c->R + 2) * c->pitch; }
 
-// The ring copy the v2 loop reads in this context: the band copy for a kernel-7 context
-// of 16-bit storage; the row-pair copy for f32 parallel beam (its points keep one detector
-// row, so one 16-byte load brings both rows; measured on the parallel COR search: 38
-// against 73 ms; oblique cone slices lose, 7.4 against 6.7 ms); else the ring itself.
+// The ring copy the v2 loop reads in this context: the band copy for 16-bit storage in a
+// kernel-7 context, or a kernel-6 one of parallel beam (the 16-bit default there); the
+// row-pair copy for f32 parallel beam (its points keep one detector row, so one 16-byte
+// load brings both rows; measured on the parallel COR search: 38 against 73 ms; oblique
+// cone slices lose, 7.4 against 6.7 ms); else the ring itself.
 int v2_ring_kind(const Ctx* c) {
-    if (c->bp_variant == 7 && band_fits(c)) return 2;
-    if (c->storage == MR_STORE_F32 && c->kind == MR_KIND_PARALLEL &&
-        (c->bp_variant == 4 || c->bp_variant >= 6))
-        return 1;
+    const bool par = c->kind == MR_KIND_PARALLEL;
+    if ((c->bp_variant == 7 || (par && c->bp_variant == 6)) && band_fits(c)) return 2;
+    if (c->storage == MR_STORE_F32 && par && (c->bp_variant == 4 || c->bp_variant >= 6)) return 1;
     return 0;
 }
 
