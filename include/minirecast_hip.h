@@ -188,7 +188,7 @@ int mr_clip_histogram(mr_ctx* ctx, const float* vol, int64_t n, int on_device, d
  * work queue, records staged in LDS), 4 (records precomputed in global memory; default
  * for F32 cone beam, 2 points, 2 views in flight), 5 (cone only: per-view detector boxes
  * staged in LDS), 6 (kernel 4 reading a row-pair copy of the ring, one wide load per tap
- * quad; default for parallel beam, 2 points, 2 views in flight) or 7 (kernel 4 reading
+ * quad; default for parallel beam, 4 points, 2 views in flight) or 7 (kernel 4 reading
  * a copy of a 16-bit ring in bands of 4 rows stepping 3, one 16-byte load per tap quad;
  * F32 storage runs it as 6; default for F16 and BF16 storage
  * in cone beam, 2 points, 3 views in flight, with volume requests run as kernel 6, 4

@@ -593,11 +593,13 @@ int mr_set_geometry_parallel(mr_ctx* c, int n_total, int rows, int cols, const d
     }
     c->kind = MR_KIND_PARALLEL;
     // measured default for parallel beam: kernel 6, the row-pair copy (a point keeps one
-    // detector row pair in every view; a COR search of 75 images of up to 1024^2 over
-    // 1800 views with its records in memory: f32 44 ms against 74.5 ms on the ring, f16
-    // 44 ms against 47 ms on the band copy with kernel 7's 3 views in flight)
+    // detector row pair in every view), 4 points per thread, 2 views in flight.  A COR
+    // search of 75 images of up to 1024^2 over 1800 views with its records in memory:
+    // 38.7 ms (f32 and f16) against 42.4 ms with 2 points, 74.5 ms for f32 on the ring
+    // (kernel 4) and 47 ms for f16 on the band copy (kernel 7)
     if (!c->bp_explicit) {
         c->bp_variant = 6;
+        c->bp_ppt = 4;
         c->bp_depth = 2;
         c->bp_sync = 0;
     }
