@@ -2181,7 +2181,7 @@ int64_t pair_view_stride(const Ctx* c) { return 2 * static_cast<int64_t>(c->R + 
 // kernel-7 context, or a kernel-6 one of parallel beam (the 16-bit default there); the
 // row-pair copy for f32 parallel beam (its points keep one detector row, so one 16-byte
 // load brings both rows; measured on the parallel COR search: 38 against 73 ms; oblique
-// cone slices lose, 7.4 against 6.7 ms); else the ring itself.
+// cone slices lose, 7.4 against 5.8 ms on kernel 4); else the ring itself.
 int v2_ring_kind(const Ctx* c) {
     const bool par = c->kind == MR_KIND_PARALLEL;
     if ((c->bp_variant == 7 || (par && c->bp_variant == 6)) && band_fits(c)) return 2;
