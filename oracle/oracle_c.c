@@ -292,6 +292,42 @@ void orc_bp_cone_acc_view(const double* q, int R, int C, const double* V, double
     }
 }
 
+/* bp_parallel_numba (recon.py:175-191) on float64 filtered views q, element (view k, row
+ * r, col c) at q[k * s_view + r * s_row + c]; acc[i] += the sum over the A views (no
+ * scale), with the column shift col_shift (recon.py:181). */
+void orc_bp_parallel_acc(const double* q, int A, int R, int C, int64_t s_view, int64_t s_row,
+                         const double* cos_t, const double* sin_t, double pu, double pv,
+                         double col_shift, const double* pts, int64_t M, double* acc) {
+    const double cr = (R - 1) / 2.0, cc = (C - 1) / 2.0 + col_shift;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < M; ++i) {
+        const double x = pts[3 * i], y = pts[3 * i + 1];
+        const double row = pts[3 * i + 2] / pv + cr;
+        const int r0 = (int)floor(row);
+        const double fr = row - r0;
+        double sum = 0.0;
+        for (int k = 0; k < A; ++k) {
+            const double col = (-x * sin_t[k] + y * cos_t[k]) / pu + cc;
+            const int c0 = (int)floor(col);
+            const double fc = col - c0;
+            const double* qk = q + (int64_t)k * s_view;
+            double v = 0.0;
+            for (int dr = 0; dr < 2; ++dr) {
+                const int ri = r0 + dr;
+                if (ri < 0 || ri >= R) continue;
+                const double wr = dr ? fr : 1.0 - fr;
+                for (int dc = 0; dc < 2; ++dc) {
+                    const int ci = c0 + dc;
+                    if (ci < 0 || ci >= C) continue;
+                    v += wr * (dc ? fc : 1.0 - fc) * qk[(int64_t)ri * s_row + ci];
+                }
+            }
+            sum += v;
+        }
+        acc[i] += sum;
+    }
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

@@ -31,6 +31,8 @@ def lib(kind="exact"):
     L.orc_bp_cone.argtypes = [_p, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _d, _p]
     L.orc_bp_parallel.argtypes = [_p, _i, _i, _i, _p, _p, _d, _d, _p, _i64, _d, _d, _p]
     L.orc_bp_cone_acc_view.argtypes = [_p, _i, _i, _p, _d, _p, _i64, _p]
+    L.orc_bp_parallel_acc.argtypes = [_p, _i, _i, _i, _i64, _i64, _p, _p, _d, _d, _d, _p, _i64,
+                                      _p]
     L.orc_set_num_threads.argtypes = [_i]
     _libs[kind] = L
     return L
@@ -102,4 +104,20 @@ def bp_cone_acc_view(q64, vector, shift, pts, acc, kind="exact"):
     assert acc.dtype == np.float64 and acc.flags.c_contiguous and len(acc) == len(P)
     lib(kind).orc_bp_cone_acc_view(_ptr(q64), R, C, _ptr(V), float(shift), _ptr(P), len(P),
                                    _ptr(acc))
+    return acc
+
+
+def bp_parallel_acc(q64, cos_t, sin_t, pu, pv, col_shift, pts, acc, kind="exact"):
+    """acc += parallel-beam back-projection (unscaled) of float64 views q64, a
+    [views, rows, cols] array of any strides with unit column stride."""
+    assert q64.dtype == np.float64 and q64.ndim == 3 and q64.strides[2] == 8
+    A, R, C = q64.shape
+    c = np.ascontiguousarray(cos_t, np.float64)
+    s = np.ascontiguousarray(sin_t, np.float64)
+    assert len(c) == A and len(s) == A
+    P = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+    assert acc.dtype == np.float64 and acc.flags.c_contiguous and len(acc) == len(P)
+    lib(kind).orc_bp_parallel_acc(ctypes.c_void_p(q64.ctypes.data), A, R, C, q64.strides[0] // 8,
+                                  q64.strides[1] // 8, _ptr(c), _ptr(s), float(pu), float(pv),
+                                  float(col_shift), _ptr(P), len(P), _ptr(acc))
     return acc

@@ -3,7 +3,8 @@ oblique 1024^2 slices (1,985,962 in-box points), f32 and f16 rings in their defa
 back-projection configurations.
 
 Checked: the float64 oracle on 3 x 256 sampled in-box pixels (f32 <= 1e-5, f16 <= 1e-3,
-bf16 <= 5e-3 relative L2), stream == batch (interleaved arrival in batches of 40), exact linearity
+bf16 <= 1e-3 relative L2) and on 512 sampled voxels of the 768 x 768 x 607 volume over
+the box (reconstruct_volume(maxdim=768), recon.py:442-470), stream == batch (interleaved arrival in batches of 40), exact linearity
 of the f32 path (projections x 2 give outputs x 2 bit for bit: scaling by 2 is exact in
 binary floating point), and the default f16 kernel (band ring) against the plain v1
 kernel on every output.
@@ -44,6 +45,13 @@ def full():
         tasks.append(recon.plane_task(box, p['origin'], p['normal'], p['n'], out_offset=off))
         off += p['n'] ** 2
     samp_idx = np.concatenate(samp_idx)
+    n_slice = len(samp_idx)
+    # the 768^3 volume: its grid over the box (volume_grid, recon.py:278-283)
+    dims, vorigin, vh = recon.volume_grid(box, w.extra["volume"])
+    nvox = int(np.prod(dims))
+    vsel = np.sort(rng.choice(nvox, size=512, replace=False))
+    ijk = np.stack(np.unravel_index(vsel, tuple(int(d) for d in dims)), axis=1)
+    samp_pts.append(vorigin + vh * ijk)
     samp_pts = np.concatenate(samp_pts)
     acc64 = np.zeros(len(samp_pts))
     host = np.empty((R, batch, C), np.float32)
@@ -75,12 +83,14 @@ def full():
     torch.cuda.empty_cache()
     T = np.array(tasks, dtype=_hip.TASK_DTYPE)
     out = {name: ctx[name].bp_tasks(T, A, off) for name in names}
+    ref = acc64 * (np.pi / A)
     return dict(w=w, ctx=ctx, T=T, out_len=off, out=out, samp_idx=samp_idx,
-                ref=acc64 * (np.pi / A))
+                ref=ref[:n_slice], vol_ref=ref[n_slice:], vsel=vsel, dims=dims,
+                vorigin=vorigin, vh=vh)
 
 
 def test_full_size_vs_float64_oracle(full):
-    for name, tol in (("f32", 1e-5), ("f16", 1e-3), ("bf16", 5e-3)):
+    for name, tol in (("f32", 1e-5), ("f16", 1e-3), ("bf16", 1e-3)):
         got = full["out"][name][full["samp_idx"]].astype(np.float64)
         err = rel_l2(got, full["ref"])
         print(f"{name}: rel-L2 {err:.2e} vs float64 at {len(got)} sampled pixels")
@@ -113,3 +123,26 @@ def test_full_size_default_f16_kernel_vs_plain_kernel(full):
     got = full["out"]["f16"]
     np.testing.assert_array_equal(got == 0, plain == 0)
     assert rel_l2(got, plain) < 1e-6
+
+
+@pytest.mark.parametrize("name,tol", [("f32", 1e-5), ("f16", 1e-3)])
+def test_full_size_volume_vs_float64_oracle(full, name, tol):
+    """The 768 x 768 x 607 volume (4.3e11 updates) in one device-resident request, each
+    z-plane a tile stack of the volume task (the default volume kernel of each storage),
+    at 512 sampled voxels against the float64 oracle."""
+    from minirecast import recon
+    ctx = full["ctx"][name]
+    dims = full["dims"]
+    nvox = int(np.prod(dims))
+    A = full["w"].geom.n_angles
+    T = recon.volume_task(full["vorigin"], full["vh"], dims, 0, int(dims[2]))
+    dptr = ctx.alloc(4 * nvox)
+    try:
+        ctx.bp_tasks_device(T[None], A, dptr, nvox)
+        vol = ctx.d2h(np.empty(nvox, np.float32), dptr)
+    finally:
+        ctx.free(dptr)
+    got = vol[full["vsel"]].astype(np.float64)
+    err = rel_l2(got, full["vol_ref"])
+    print(f"{name}: volume {tuple(int(d) for d in dims)} rel-L2 {err:.2e} at {len(got)} voxels")
+    assert np.isfinite(vol).all() and err < tol, (name, err)

@@ -2,13 +2,13 @@
 composition fixtures, on the same inputs.
 
 Tolerance (floating point): relative L2 <= 1e-5 for the plain FP32 path against the
-reference-equivalent float64 result; FP16 (per-view scale) <= 1e-3 and BF16 <= 5e-3
+reference-equivalent float64 result; FP16 (per-view scale) and BF16 <= 1e-3
 against the float64 oracle.
 """
 import numpy as np
 import pytest
 
-from conftest import rel_l2
+from conftest import cor_trial_images, entropy_score, rel_l2
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -81,6 +81,11 @@ def test_reference_preview_and_cor_cone(golden, golden_cone):
     assert rel_l2(vol, golden['cone_preview']) < TOL
     shifts = golden['cone_cor_shifts']
     best, scores = rec.estimate_cor(shifts, n=32, verbose=False)
+    # the device histogram entropy equals numpy's on the very same trial images
+    host = [entropy_score(im) for im in cor_trial_images(rec, shifts, 32)]
+    np.testing.assert_allclose(scores, host, rtol=1e-12, atol=0)
+    # against the reference's images (float64 back-projection): ~820 in-box pixels, so
+    # one pixel changing bin moves a score by ~(ln p + 1) / 820, i.e. a few 1e-3
     np.testing.assert_allclose(scores, golden['cone_cor_scores'], rtol=2e-3, atol=2e-3)
     assert best == golden['cone_cor_best']
     rec.set_cor_shift(0.0)
@@ -113,6 +118,8 @@ def test_reference_planes_parallel(golden):
     img, _ = rec.reconstruct_plane((0, 0, 0), (0, 0, 1), n=32, cor_shift=-0.75)
     assert rel_l2(img, golden['par_plane_cor']) < TOL
     best, scores = rec.estimate_cor(np.arange(-1.0, 1.01, 0.5), n=32, verbose=False)
+    host = [entropy_score(im) for im in cor_trial_images(rec, np.arange(-1.0, 1.01, 0.5), 32)]
+    np.testing.assert_allclose(scores, host, rtol=1e-12, atol=0)
     np.testing.assert_allclose(scores, golden['par_cor_scores'], rtol=2e-3, atol=2e-3)
     assert best == golden['par_cor_best']
 
@@ -309,7 +316,7 @@ def test_storage_precision_vs_float64_oracle(small_cone):
     p = w.planes[0]
     ref, _ = orc.reconstruct_plane(p['origin'], (1.0, 0.5, 0.8), n=p['n'])
     errs = {}
-    for storage, tol in (("f32", TOL), ("f16", 1e-3), ("bf16", 5e-3)):
+    for storage, tol in (("f32", TOL), ("f16", 1e-3), ("bf16", 1e-3)):
         rec = recon.SliceReconstructor(proj, w.geom, window=w.window, verbose=False,
                                        storage=storage)
         img, _ = rec.reconstruct_plane(p['origin'], (1.0, 0.5, 0.8), n=p['n'])
@@ -317,7 +324,8 @@ def test_storage_precision_vs_float64_oracle(small_cone):
         assert errs[storage] < tol, (storage, errs[storage])
         q = rec.filtered_views(0, 4)
         qref = orc.q[:4]
-        assert rel_l2(q, qref) < {"f32": TOL, "f16": 1e-3, "bf16": 5e-3}[storage]
+        # one stored value: unit roundoff 2^-24 (f32), 2^-11 (f16), 2^-8 = 3.9e-3 (bf16)
+        assert rel_l2(q, qref) < {"f32": TOL, "f16": 1e-3, "bf16": 4e-3}[storage]
     assert errs["f32"] < errs["f16"] < errs["bf16"]
 
 
@@ -330,7 +338,7 @@ def test_bp_kernel_variants_vs_oracle(small_cone, variant, monkeypatch):
     w, proj, _, orc = small_cone
     p = dict(origin=(0.5, -0.3, 0.2), normal=(1.0, 0.5, 0.8), n=77)
     ref, _ = orc.reconstruct_plane(p['origin'], p['normal'], n=p['n'])
-    for storage, tol in (("f32", TOL), ("f16", 1e-3), ("bf16", 5e-3)):
+    for storage, tol in (("f32", TOL), ("f16", 1e-3), ("bf16", 1e-3)):
         rec = recon.SliceReconstructor(proj, w.geom, window=w.window, verbose=False,
                                        storage=storage)
         img, _ = rec.reconstruct_plane(p['origin'], p['normal'], n=p['n'])

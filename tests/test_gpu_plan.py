@@ -599,3 +599,27 @@ def test_oversized_slice_requests(workload, storage, beam, monkeypatch):
     for name in ("default", "explicit"):
         np.testing.assert_array_equal(outs[name] == 0, outs["ref"] == 0)
         assert rel_l2(outs[name], outs["ref"]) < 1e-6, (storage, name, rel_l2(outs[name], outs["ref"]))
+
+
+def test_stream_graphs_survive_context_buffer_regrowth(workload):
+    """A streaming plan's captured graphs keep their own f16 staging: an eager push of a
+    larger batch and an adjoint with a host gradient regrow (free and reallocate) the
+    context's staging buffers between two replays of the same batch-size graph, and the
+    replay still equals a full pass bit for bit."""
+    from minirecast import _hip
+    w, proj, planes, tasks, out_len = workload
+    T = np.array(tasks, dtype=_hip.TASK_DTYPE)
+    ctx = _hip.Context(0)
+    ctx.set_cone(w.geom.vectors, w.geom.rows, w.geom.cols, w.window, "f16")
+    plan = _hip.Plan(ctx, T, out_len)
+    got = plan.stream(np.arange(0, 7), np.ascontiguousarray(proj[:, 0:7, :]))
+    np.testing.assert_array_equal(got, plan.run())
+    ctx.push(np.arange(7, 71), np.ascontiguousarray(proj[:, 7:71, :]))
+    gq = ctx.bp_tasks_adjoint(T, ctx.num_views(), np.ones(out_len, np.float32),
+                              w.geom.rows, w.geom.cols)
+    assert gq.shape == (71, w.geom.rows, w.geom.cols) and np.isfinite(gq).all()
+    for i0 in (71, 78):
+        got = plan.stream(np.arange(i0, i0 + 7), np.ascontiguousarray(proj[:, i0:i0 + 7, :]))
+        np.testing.assert_array_equal(got, plan.run())
+    assert ctx.num_views() == 85
+    plan.close()

@@ -181,8 +181,6 @@ def test_reconstructor_argument_errors():
     g = circular_cone_vectors(8, 300.0, 600.0, 8, 8, 1.0)
     with pytest.raises(ValueError):
         recon.SliceReconstructor(None, None)
-    with pytest.raises(RuntimeError):
-        recon.SliceReconstructor(None, g, backend="numba")
     with pytest.raises(ValueError):
         recon.SliceReconstructor(None, g, window="blackman")
     with pytest.raises(ValueError):
@@ -209,3 +207,152 @@ def test_bench_refuses_several_gpus():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_reference_backend_names_map_to_hip(monkeypatch):
+    """backend="numba" / "numpy" (the reference's names, recon.py:302-306) warn and
+    select the GPU path instead of failing; with the HIP back-end forced off the
+    constructor fails loudly (no CPU fallback)."""
+    from minirecast import _hip, recon
+    from minirecast.geometry import circular_cone_vectors
+    g = circular_cone_vectors(8, 300.0, 600.0, 8, 8, 1.0)
+    monkeypatch.setenv(_hip.FORCE_FAILURE_ENV, "1")
+    for name in ("numba", "numpy"):
+        with pytest.warns(UserWarning, match="hip"):
+            with pytest.raises(RuntimeError, match=_hip.FORCE_FAILURE_ENV):
+                recon.SliceReconstructor(None, g, backend=name)
+    with pytest.raises(RuntimeError, match="no CPU"):
+        recon.SliceReconstructor(None, g)
+    with pytest.raises(_hip.HipUnavailable):
+        _hip.load()
+
+
+def test_live_planes_close_waits_for_refresh():
+    """LivePlanes.close() from another thread waits for a refresh in flight (both hold
+    the kernel lock), and a refresh after close raises LivePlanesClosed."""
+    import threading
+    import time
+    from minirecast import recon
+
+    class FakePlan:
+        def __init__(self):
+            self.closed = self.busy = self.bad = False
+
+        def update(self):
+            self.busy = True
+            time.sleep(0.05)
+            self.bad |= self.closed
+            self.busy = False
+            return np.arange(4, dtype=np.float32)
+
+        def close(self):
+            self.bad |= self.busy
+            self.closed = True
+
+    fake = FakePlan()
+    lp = object.__new__(recon.LivePlanes)
+    lp._plan, lp._sizes = fake, [2]
+    out = []
+    t = threading.Thread(target=lambda: out.append(lp.refresh()))
+    t.start()
+    time.sleep(0.01)
+    lp.close()
+    t.join()
+    assert fake.closed and not fake.bad
+    np.testing.assert_array_equal(out[0][0], [[0.0, 1.0], [2.0, 3.0]])
+    with pytest.raises(recon.LivePlanesClosed):
+        lp.refresh()
+    lp.close()                                   # closing twice is harmless
+
+
+def test_slice_service_eviction_during_refresh():
+    """Slice sets evicted (LRU) or forgotten by one request thread while another thread
+    refreshes them: no request fails and no closed plan is refreshed; a set whose plan
+    was closed under it is answered by a direct back-projection."""
+    import threading
+    import time
+    from minirecast import recon, serve
+
+    class FakeLive:
+        def __init__(self):
+            self.plan_open = True
+            self.lock = threading.Lock()
+
+        def refresh(self):
+            with recon._KERNEL_LOCK:
+                if not self.plan_open:
+                    raise recon.LivePlanesClosed("closed")
+                time.sleep(0.002)
+                return ["live"]
+
+        def close(self):
+            with recon._KERNEL_LOCK:
+                self.plan_open = False
+
+    class FakeRec:
+        cor_shift = 0.0
+        version = 1
+
+        def reconstruct_planes(self, planes):
+            return ["direct"]
+
+        def live_planes(self, planes):
+            return FakeLive()
+
+    svc = serve.SliceService()
+    svc.LIVE_SETS = 2
+    rec = FakeRec()
+    errors, results = [], []
+
+    def worker(k):
+        try:
+            for i in range(60):
+                plane = [dict(n=(k + i // 4) % 3)]
+                results.append(svc._planes(rec, plane)[0])
+                if i % 17 == 0:
+                    svc._forget_live()
+        except Exception as e:          # noqa: BLE001 - the test records every failure
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(k,)) for k in range(6)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors
+    assert set(results) <= {"live", "direct"} and "live" in results
+
+
+def test_feeder_pacing_and_stop():
+    """The feeder holds each shot until the detector rate has produced it, and stop()
+    ends the run before the next shot."""
+    import time
+    from minirecast.stream import Feeder, arrival_order
+
+    class Sink:
+        def __init__(self):
+            self.got, self.t = [], []
+
+        def add_projections(self, idx, block):
+            np.testing.assert_array_equal(block[0, :, 0], idx)
+            self.got.extend(int(i) for i in idx)
+            self.t.append(time.monotonic())
+
+    proj = np.zeros((1, 23, 2), np.float32)
+    proj[0, :, 0] = np.arange(23)
+    sink = Sink()
+    f = Feeder(sink, proj, order="interleaved", group=5, rate=200.0, verbose=False)
+    t0 = time.monotonic()
+    f.run()
+    assert sink.got == list(arrival_order(23, "interleaved"))
+    assert f.sent == 23 and f.shots_sent == 5
+    # shot k (views up to 5(k+1), the last 23) is not handed over before it is acquired
+    for k, t in enumerate(sink.t):
+        assert t - t0 >= min(5 * (k + 1), 23) / 200.0 - 1e-3
+    slow = Sink()
+    f = Feeder(slow, proj, group=4, rate=20.0, verbose=False)
+    f.start()
+    time.sleep(0.3)
+    f.stop()
+    f.join(timeout=5)
+    assert not f.is_alive() and 0 < f.shots_sent < 6 and f.sent == len(slow.got)

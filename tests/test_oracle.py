@@ -280,6 +280,82 @@ def test_preprocess_selftest_invariants():
     assert (r, c) in ((10, 20), (30, 20))
 
 
+def test_rotation_direction_closed_forms():
+    """Bilinear interpolation reproduces linear images exactly, so rotating f(r, c) = r
+    and g(r, c) = c about the centre has a closed form at every pixel whose source lies
+    inside the detector.  The reference's inverse map (preprocess.py:42-46) takes output
+    pixel (rr, cc) (centred) from source (cos t rr + sin t cc, -sin t rr + cos t cc): a
+    +90 degree rotation moves a point right of the centre to above it (rows grow
+    downwards), i.e. the selftest's delta at (20, 30) lands at (10, 20), not (30, 20).
+    preprocess.apply corrects a tilt by rotating by -tilt (preprocess.py:59-64)."""
+    R, C = 33, 48
+    rows = np.broadcast_to(np.arange(R, dtype=float)[:, None, None], (R, 2, C)).copy()
+    cols = np.broadcast_to(np.arange(C, dtype=float)[None, None, :], (R, 2, C)).copy()
+    rr, cc = np.meshgrid(np.arange(R) - (R - 1) / 2.0, np.arange(C) - (C - 1) / 2.0,
+                         indexing="ij")
+    for deg in (7.5, -7.5, 31.0):
+        t = np.deg2rad(deg)
+        src_r = np.cos(t) * rr + np.sin(t) * cc + (R - 1) / 2.0
+        src_c = -np.sin(t) * rr + np.cos(t) * cc + (C - 1) / 2.0
+        inside = (src_r >= 0) & (src_r <= R - 1) & (src_c >= 0) & (src_c <= C - 1)
+        assert inside.mean() > 0.5
+        got_r = O.rotate_projections(rows, deg)[:, 0, :]
+        got_c = O.rotate_projections(cols, deg)[:, 0, :]
+        np.testing.assert_allclose(got_r[inside], src_r[inside], atol=1e-12)
+        np.testing.assert_allclose(got_c[inside], src_c[inside], atol=1e-12)
+        # far outside the detector every tap is zero
+        far = (src_r < -1) | (src_r > R) | (src_c < -1) | (src_c > C)
+        assert np.all(got_r[far] == 0)
+        # the tilt correction of apply is the rotation by -tilt
+        np.testing.assert_array_equal(O.preprocess_apply(rows, tilt=deg),
+                                      O.rotate_projections(rows, -deg))
+    img = np.zeros((41, 1, 41))
+    img[20, 0, 30] = 1.0
+    rot = O.rotate_projections(img, 90.0)[:, 0, :]
+    assert rot[10, 20] == pytest.approx(1.0, abs=1e-12) and rot[30, 20] == pytest.approx(0.0)
+    np.testing.assert_allclose(O.rotate_projections(img, 0.0), img)
+
+
+@pytest.mark.parametrize("sigma", [2.0, 2.5, 3.7])
+def test_gaussian_blur_of_a_delta_is_the_sampled_gaussian(sigma):
+    """H(f) = exp(-2 (pi f sigma)^2) is the Fourier transform of the unit-area Gaussian
+    G(x) = exp(-x^2 / (2 sigma^2)) / (sigma sqrt(2 pi)) (preprocess.py:23).  For sigma >= 2
+    H is below 1e-8 at the Nyquist frequency, so the FFT blur of an interior delta equals
+    the sampled Gaussian, separably in rows and columns; a wrong sigma convention (a
+    factor 2 in the exponent, sigma in other units) or a shift of the crop fails."""
+    R, C = 61, 75
+    r0, c0 = 29, 40
+    img = np.zeros((R, 1, C))
+    img[r0, 0, c0] = 1.0
+    got = O.gaussian_blur(img, sigma)[:, 0, :]
+
+    def G(x):
+        return np.exp(-x ** 2 / (2 * sigma ** 2)) / (sigma * np.sqrt(2 * np.pi))
+
+    want = G(np.arange(R) - r0)[:, None] * G(np.arange(C) - c0)[None, :]
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-9 * want.max())
+    assert abs(got.sum() - 1.0) < 1e-9
+
+
+def test_gaussian_blur_constant_and_ramp():
+    """Edge padding leaves a constant block unchanged; the symmetric kernel keeps a
+    linear ramp unchanged away from the edges (no shift); sigma <= 0 is the identity."""
+    block = np.full((30, 2, 40), 0.375)
+    np.testing.assert_allclose(O.gaussian_blur(block, 1.7), block, atol=1e-13)
+    ramp = np.broadcast_to(np.arange(60, dtype=float)[None, None, :] * 0.5
+                           + np.arange(44, dtype=float)[:, None, None] * -0.25,
+                           (44, 2, 60)).copy()
+    out = O.gaussian_blur(ramp, 2.0)
+    m = 15                                       # edge effects decay as exp(-m^2 / 8)
+    np.testing.assert_allclose(out[m:-m, :, m:-m], ramp[m:-m, :, m:-m], atol=1e-9)
+    np.testing.assert_array_equal(O.gaussian_blur(ramp, 0.0), ramp)
+    # apply: tilt first, then the blur; the two do not commute near the edges
+    both = O.preprocess_apply(ramp, gaussian=2.0, tilt=5.0)
+    np.testing.assert_allclose(both, O.gaussian_blur(O.rotate_projections(ramp, -5.0), 2.0),
+                               atol=1e-12)
+    assert np.abs(both - O.rotate_projections(O.gaussian_blur(ramp, 2.0), -5.0)).max() > 1e-3
+
+
 # ---------------------------------------------------------------- reference composition
 def _golden_cone(golden):
     R, C = golden['cone_shape']

@@ -50,3 +50,24 @@ def test_c_bp_matches_numpy(kind):
     want = O.bp_cone(q.astype(np.float64), K['s'], D, K['uu'], K['vv'], K['n'], K['sdd'],
                      K['sod'], pts, 1.0)
     np.testing.assert_allclose(acc, want, rtol=1e-12, atol=1e-12)
+
+
+def test_c_bp_parallel_acc_matches_numpy():
+    """The strided float64 parallel-beam accumulation equals the NumPy oracle's
+    bp_parallel (without its scale), on a [rows, views, cols] array read as
+    [views, rows, cols] through its strides, with a column shift and points whose
+    taps leave the detector on every side."""
+    rng = np.random.default_rng(5)
+    R, A, C = 6, 11, 13
+    q = rng.normal(size=(R, A, C))
+    th = rng.uniform(0, np.pi, A)
+    pts = rng.uniform(-9, 9, size=(400, 3))
+    pts[:, 2] = rng.uniform(-4.5, 4.5, 400)
+    view = np.transpose(q, (1, 0, 2))                 # [A, R, C], not contiguous
+    assert not view.flags.c_contiguous
+    for shift in (0.0, -1.75, 2.5):
+        acc = np.full(len(pts), 0.5)
+        OC.bp_parallel_acc(view, np.cos(th), np.sin(th), 0.9, 1.2, shift, pts, acc)
+        want = O.bp_parallel(np.ascontiguousarray(view), np.cos(th), np.sin(th), 0.9, 1.2, pts,
+                             1.0, shift)
+        np.testing.assert_allclose(acc - 0.5, want, rtol=1e-12, atol=1e-12)

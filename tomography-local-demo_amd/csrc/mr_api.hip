@@ -1291,7 +1291,7 @@ static void plan_free(Plan* p) {
     if (p->cor_exec) hipGraphExecDestroy(p->cor_exec);
     p->cor_exec = nullptr;
     void* dev[] = {p->d_tasks, p->d_toff, p->d_queue, p->d_counters, p->d_out, p->d_block, p->d_meta,
-                   p->d_scores, p->d_acc, p->d_corws};
+                   p->d_fstage, p->d_vmax, p->d_scores, p->d_acc, p->d_corws};
     for (void* d : dev)
         if (d) hipFree(d);
     if (p->h_block) hipHostFree(p->h_block);
@@ -1552,20 +1552,24 @@ int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block
         if (p->h_meta) hipHostFree(p->h_meta);
         if (p->d_block) hipFree(p->d_block);
         if (p->d_meta) hipFree(p->d_meta);
+        if (p->d_fstage) hipFree(p->d_fstage);
+        if (p->d_vmax) hipFree(p->d_vmax);
         p->h_block = nullptr; p->h_meta = nullptr; p->d_block = nullptr; p->d_meta = nullptr;
+        p->d_fstage = nullptr; p->d_vmax = nullptr;
         p->kmax = 0;
         MR_HIP(hipHostMalloc(reinterpret_cast<void**>(&p->h_block), blk, hipHostMallocDefault));
         MR_HIP(hipHostMalloc(&p->h_meta, meta, hipHostMallocDefault));
         MR_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_block), blk));
         MR_HIP(hipMalloc(&p->d_meta, meta));
+        if (c->storage == MR_STORE_F16) {
+            // the graphs' own f16 staging: the context's d_fstage / d_vmax are regrown
+            // (freed) by other entry points, which would leave a graph on freed memory
+            MR_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_fstage), blk));
+            MR_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_vmax), sizeof(unsigned) * k));
+        }
         p->kmax = k;
     }
     int rc = 0;
-    if (c->storage == MR_STORE_F16) {
-        rc = ensure(reinterpret_cast<void**>(&c->d_fstage), &c->fstage_bytes, blk);
-        if (!rc) rc = ensure(reinterpret_cast<void**>(&c->d_vmax), &c->vmax_len, sizeof(unsigned) * k);
-        if (rc) return rc;
-    }
     // stage the batch and its metadata in pinned memory (fixed addresses)
     std::memcpy(p->h_block, block, blk);
     FilterView* fv = static_cast<FilterView*>(p->h_meta);
@@ -1605,7 +1609,8 @@ int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block
         hipError_t e2 = hipMemcpyAsync(p->d_meta, p->h_meta, meta, hipMemcpyHostToDevice, c->s_req);
         int r = (e1 != hipSuccess || e2 != hipSuccess) ? MR_ERUNTIME : MR_OK;
         if (!r) r = launch_stream_prep(c, p->d_meta, k, p->d_counters + 2, c->s_req);
-        if (!r) r = launch_filter(c, p->d_block, k, 0, c->s_req, static_cast<const FilterView*>(p->d_meta));
+        if (!r) r = launch_filter(c, p->d_block, k, 0, c->s_req, static_cast<const FilterView*>(p->d_meta),
+                                  p->d_fstage, p->d_vmax);
         if (!r && rk == 2) r = launch_band_convert_tail(c, p->d_counters + 2, k, c->s_req);
         if (!r && rk == 1) r = launch_pair_convert_tail(c, p->d_counters + 2, k, c->s_req);
         if (!r) r = launch_reset_head(p->d_counters, c->s_req);

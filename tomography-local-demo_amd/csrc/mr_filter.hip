@@ -359,9 +359,11 @@ int log2i(int N) {
 }  // namespace
 
 int launch_filter(Ctx* c, const float* d_block, int k, int n0, hipStream_t s,
-                  const FilterView* fv) {
+                  const FilterView* fv, float* fstage, unsigned* vmax) {
     (void)n0;
     if (!fv) fv = c->d_fv;
+    if (!fstage) fstage = c->d_fstage;
+    if (!vmax) vmax = c->d_vmax;
     const int logn = log2i(c->N);
     dim3 grid((c->R + 1) / 2, k);
     if (c->storage == MR_STORE_F32)
@@ -372,12 +374,12 @@ int launch_filter(Ctx* c, const float* d_block, int k, int n0, hipStream_t s,
                               c->ring, c->pitch, c->view_stride, nullptr);
     // F16: filter to float32 staging, then quantise with a per-view scale (the max
     // buffer is cleared with a kernel so that the sequence can be graph-captured)
-    hipLaunchKernelGGL(zero_u32_kernel, dim3((k + 255) / 256), dim3(256), 0, s, c->d_vmax, k);
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((k + 255) / 256), dim3(256), 0, s, vmax, k);
     int rc = launch_mode<2>(logn, grid, s, d_block, c->R, k, c->C, fv, c->d_H, c->d_tw,
-                            c->d_fstage, c->pitch, c->view_stride, c->d_vmax);
+                            fstage, c->pitch, c->view_stride, vmax);
     if (rc) return rc;
-    hipLaunchKernelGGL(quantize_f16_kernel, dim3(c->R, k), dim3(256), 0, s, c->d_fstage, c->R,
-                       c->C, fv, c->d_vmax, static_cast<__half*>(c->ring), c->pitch,
+    hipLaunchKernelGGL(quantize_f16_kernel, dim3(c->R, k), dim3(256), 0, s, fstage, c->R,
+                       c->C, fv, vmax, static_cast<__half*>(c->ring), c->pitch,
                        c->view_stride, c->d_vscale);
     MR_HIP(hipGetLastError());
     return MR_OK;

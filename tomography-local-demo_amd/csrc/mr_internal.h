@@ -220,6 +220,8 @@ struct Plan {
     void* h_meta = nullptr;          // FilterView[kmax] + ViewGeom[kmax] + int slots[kmax]
     void* d_meta = nullptr;
     size_t meta_bytes = 0;
+    float* d_fstage = nullptr;       // f16 storage: f32 filtered staging [R][kmax][C]
+    unsigned* d_vmax = nullptr;      // f16 storage: per-view max |q| [kmax]
     std::vector<std::pair<int, hipGraphExec_t>> graphs;   // per batch size
     double last_ms = 0.0;
     uint64_t geom_epoch = 0;         // context geometry the plan was made for
@@ -242,8 +244,12 @@ int ensure(void** p, size_t* have, size_t need);   // device buffer growth
 int ensure_host(void** p, size_t* have, size_t need);
 
 // ---------------------------------------------------------------- kernels (launchers)
+// fv, fstage, vmax: the batch's filter views, f32 staging [R][k][C] and per-view max
+// buffer [k] (f16 storage); null selects the context's own (d_fv, d_fstage, d_vmax).
+// A captured graph passes buffers it owns, since the context's may be reallocated.
 int launch_filter(Ctx* c, const float* d_block, int k, int n0, hipStream_t s,
-                  const FilterView* fv = nullptr);
+                  const FilterView* fv = nullptr, float* fstage = nullptr,
+                  unsigned* vmax = nullptr);
 int launch_store(Ctx* c, const float* d_q, int k, hipStream_t s);
 int launch_preprocess(Ctx* c, float* d_a, float* d_b, int R, int k, int C, double gaussian,
                       double tilt, float** d_res, hipStream_t s);

@@ -124,9 +124,19 @@ def _share_torch_hip_runtime():
     return None
 
 
+FORCE_FAILURE_ENV = "MINIRECAST_FORCE_HIP_FAILURE"
+
+
 def load(path=None):
-    """Load (once) and return the ctypes library; RuntimeError if it is missing."""
+    """Load (once) and return the ctypes library; RuntimeError if it is missing.
+
+    MINIRECAST_FORCE_HIP_FAILURE=1 makes every load fail as if the library or the GPU
+    were missing (the failure path of the reference's warmup, recon.py:477-487, forced
+    from the host).  There is no CPU path to fall back to: callers get HipUnavailable."""
     global _lib
+    if os.environ.get(FORCE_FAILURE_ENV) == "1":
+        raise HipUnavailable(f"HIP back-end disabled by {FORCE_FAILURE_ENV}=1; this build has "
+                             f"no CPU back-projection to fall back to")
     with _lib_lock:
         if _lib is not None:
             return _lib

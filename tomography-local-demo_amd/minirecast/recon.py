@@ -13,6 +13,7 @@ GPU, constructing a reconstructor raises RuntimeError.
 """
 import threading
 import time
+import warnings
 
 import numpy as np
 
@@ -262,6 +263,10 @@ class SliceReconstructor:
     storage : "f32" (default), "f16" (per-view scale) or "bf16" storage of the
               filtered views in HBM; accumulation is float32 in every case.
     device : HIP device ordinal.
+    backend : "auto" or "hip".  The reference's CPU backend names ("numba", "numpy",
+              recon.py:302-306, passed on by its CLI and server options) select the
+              same GPU path with a warning, so that callers started with them keep
+              working; there is no CPU back-projection in this build.
     """
 
     def __init__(self, proj=None, geom=None, window="shepp-logan", backend="auto",
@@ -271,8 +276,9 @@ class SliceReconstructor:
         if backend == "auto":
             backend = "hip"
         if backend != "hip":
-            raise RuntimeError(f"{backend} backend requested but this build only has the "
-                               f"'hip' backend (AMD GPU)")
+            warnings.warn(f"backend {backend!r} requested: this build runs every backend "
+                          f"name on the 'hip' backend (AMD GPU)", stacklevel=2)
+            backend = "hip"
         _hip.window_code(window)
         if storage not in _hip.STORAGE:
             raise ValueError(f"unknown storage {storage!r}")
@@ -569,9 +575,17 @@ class SliceReconstructor:
         return self._ctx.read_views(first, count, self.geom.rows, self.geom.cols)
 
 
+class LivePlanesClosed(RuntimeError):
+    """refresh() of a LivePlanes object that has been closed."""
+
+
 class LivePlanes:
     """Planes of one SliceReconstructor brought up to date incrementally (a device plan
-    with running sums; see SliceReconstructor.live_planes)."""
+    with running sums; see SliceReconstructor.live_planes).
+
+    refresh() and close() both run under _KERNEL_LOCK, so a close from another thread
+    waits for a refresh in flight and the device plan is never used after it is freed;
+    refresh() after close() raises LivePlanesClosed."""
 
     def __init__(self, rec, planes, shift):
         self._rec = rec
@@ -590,6 +604,8 @@ class LivePlanes:
     def refresh(self):
         """Images (float64 [n, n] each) over all views received so far."""
         with _KERNEL_LOCK:
+            if self._plan is None:
+                raise LivePlanesClosed("these live planes have been closed")
             flat = self._plan.update()
         out, o = [], 0
         for n in self._sizes:
@@ -598,4 +614,7 @@ class LivePlanes:
         return out
 
     def close(self):
-        self._plan.close()
+        with _KERNEL_LOCK:
+            plan, self._plan = self._plan, None
+            if plan is not None:
+                plan.close()

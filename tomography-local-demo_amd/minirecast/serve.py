@@ -31,7 +31,7 @@ import numpy as np
 
 from .geometry import ConeVecGeometry, ParallelGeometry
 from .rawcorr import RawCorrection
-from .recon import SliceReconstructor
+from .recon import LivePlanesClosed, SliceReconstructor
 
 
 # ------------------------------------------------------------------ framing
@@ -131,8 +131,13 @@ class SliceService:
         if live is None:
             live = r.live_planes(planes)
             self._remember(key, live)
+        try:
+            imgs = live.refresh()
+        except LivePlanesClosed:
+            # evicted or forgotten by another request thread since the lookup above
+            return r.reconstruct_planes(planes)
         self.live_hits += 1
-        return live.refresh()
+        return imgs
 
     def _remember(self, key, live):
         drop = []
