@@ -181,6 +181,22 @@ int mr_nonzero_percentiles(mr_ctx* ctx, const float* vol, int64_t n, int on_devi
 int mr_clip_histogram(mr_ctx* ctx, const float* vol, int64_t n, int on_device, double lo,
                       double hi, int bins, const double* edges, int64_t* hist);
 
+/* ---- the entry point that allocates no device memory */
+#define MR_STATIC_SCRATCH_BYTES  (256 * 1024)   /* per GPU, held until the process ends */
+#define MR_STATIC_HIST_MAX_BINS  8192
+
+/* mr_clip_histogram of a volume already in device memory (vol_device, float32 [n]),
+ * the same counts, without allocating: no hipMalloc, hipFree or allocating library
+ * routine.  No counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid it).  The
+ * counters and edges live in the library's static per-GPU scratch of
+ * MR_STATIC_SCRATCH_BYTES, cleared by a kernel at every call: bins above
+ * MR_STATIC_HIST_MAX_BINS are refused (MR_EVALUE).  Calls on one GPU share that scratch:
+ * the library serialises its own calls, but work that overlaps them on that GPU (for
+ * example a captured graph replayed during a call) races with them. */
+int mr_clip_histogram_static(mr_ctx* ctx, const float* vol_device, int64_t n, double lo,
+                             double hi, int bins, const double* edges, int64_t* hist);
+
 /* ---- back-projection tuning (no counterpart in the reference: its Numba loop has
  * no shape parameters)
  *

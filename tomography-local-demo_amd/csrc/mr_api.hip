@@ -466,6 +466,9 @@ int mr_ctx_create(int device, mr_ctx** out) {
         delete c;
         return fail(MR_ERUNTIME, "stream/event creation failed");
     }
+    // the address of this GPU's static scratch (mr_clip_histogram_static), once per
+    // device with the device current; a failure is reported by that entry point
+    static_scratch_init(device);
     *out = c;
     return MR_OK;
 }

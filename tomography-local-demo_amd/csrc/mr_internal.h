@@ -310,6 +310,10 @@ int launch_cor_scores(const float* d_img, int nimg, int max_chunks, double* d_sc
 int launch_phantom(const double* d_ell, int K, int kind, const double* d_geo, int A, int R,
                    int C, double pu, double pv, double far, float* out, hipStream_t s);
 
+// the static scratch of the entry points that allocate nothing (mr_noalloc.hip): takes
+// the device address of this GPU's copy, once per device, with that device current
+int static_scratch_init(int device);
+
 // host float64 helpers
 int ramp_response_host(int n, double tau, int window, std::vector<double>& H);
 

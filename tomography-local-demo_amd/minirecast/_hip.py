@@ -43,6 +43,8 @@ SIGNATURES = [
                                           _c.POINTER(_c.c_int64)]),
     ("mr_clip_histogram", _c.c_int, [_P, _P, _c.c_int64, _c.c_int, _c.c_double, _c.c_double,
                                      _c.c_int, _P, _P]),
+    ("mr_clip_histogram_static", _c.c_int, [_P, _P, _c.c_int64, _c.c_double, _c.c_double,
+                                            _c.c_int, _P, _P]),
     ("mr_set_bp_config", _c.c_int, [_P, _c.c_int, _c.c_int, _c.c_int, _c.c_int]),
     ("mr_set_bp_views_per_pass", _c.c_int, [_P, _c.c_int]),
     ("mr_get_bp_views_per_pass", _c.c_int, [_P, _c.POINTER(_c.c_int)]),
