@@ -142,6 +142,24 @@ def pmc_traffic():
     return res
 
 
+def cpu_threads():
+    """(threads, detail) for the CPU baseline: every CPU of the affinity mask, unless the
+    environment sets this process's share (OMP_NUM_THREADS, which the GPU hosts set to the
+    CPUs a job may use), and the cgroup quota when one is set."""
+    mask = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(mask, int(env)) if env.isdigit() and int(env) > 0 else mask
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            quota = round(int(q) / int(period), 2)
+    except (OSError, ValueError):
+        pass
+    return threads, {"affinity_cpus": mask, "omp_num_threads": env or None,
+                     "cgroup_cpu_quota": quota}
+
+
 def cpu_model():
     """The host CPU's model string (/proc/cpuinfo), or "unknown CPU"."""
     try:
@@ -559,7 +577,7 @@ def main():
     n_in = int(sum(m.sum() for m in masks))
     if check:
         from oracle import oracle_c as OC
-        OC.set_threads(min(16, len(os.sched_getaffinity(0))))
+        OC.set_threads(cpu_threads()[0])
     host_raw = np.empty((R, batch, C), np.float32)
     oracle_s = 0.0
     filt_ms = {s: 0.0 for s in forms}
@@ -690,7 +708,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         from oracle import oracle_c as OC
-        threads = min(16, len(os.sched_getaffinity(0)))
+        threads, cpu_detail = cpu_threads()
         OC.set_threads(threads, "fast")
         # the whole update (all in-box points x all views), median of 3 runs
         sub = np.arange(A)
@@ -711,7 +729,8 @@ def main():
                    sample=f"the full 3-slice update: all {len(all_pts)} in-box points x all {A} "
                           f"views, median of 3 runs ({dt * 1e3:.0f} ms each; "
                           f"{sum(times):.1f} s of CPU work); Numba-equivalent C/OpenMP path "
-                          f"(numba unavailable) on {cpu_model()}")
+                          f"(numba unavailable) on {cpu_model()}, {threads} OpenMP threads",
+                   **cpu_detail)
         log(f"cpu baseline: {cpu}")
 
     h = results[head]

@@ -562,3 +562,54 @@ def test_slice_service_replies_and_coalescing(small_cone):
     assert "error" in serve.recv(a)[0]
     a.close()
     t.join(timeout=10)
+
+
+def test_served_replies_match_reference_fixtures_and_oracle(golden):
+    """The server's replies (net.py:140-154) from the GPU path against the reference's
+    own results: served slices (the three fixture planes without a size, one with zoom
+    and centre), the slice after set_cor and the preview against the fixtures the
+    reference's numpy backend wrote; auto_cor (estimate_cor's defaults: 25 shifts,
+    n = 192) against the float64 oracle's search on the same projections."""
+    from minirecast import serve
+    from minirecast.geometry import ConeVecGeometry
+    R, C = (int(x) for x in golden['cone_shape'])
+    V = golden['cone_vectors']
+    g = ConeVecGeometry(V, R, C)
+    proj = golden['cone_proj']
+    svc = serve.SliceService(window="shepp-logan", coalesce_ms=0.0)
+    h, arr = serve.geometry_message(g)
+    assert svc.handle(h, arr)[0] == {"ok": True}
+    order = golden['cone_stream_order']
+    for i in range(0, len(order), 7):
+        idx = [int(k) for k in order[i:i + 7]]
+        svc.handle({"cmd": "projections", "indices": idx}, proj[:, idx, :])
+    for i in (1, 2, 3):                        # fixture planes whose size is the default
+        a = golden[f'cone_plane{i}_args']
+        assert a[7] == 0
+        req = {"cmd": "slice", "origin": list(a[0:3]), "normal": list(a[3:6]), "n": int(a[6]),
+               "zoom": float(a[8])}
+        if not np.isnan(a[9]):
+            req["center"] = list(a[9:12])
+        rep, img = svc.handle(req)
+        ref = golden[f'cone_plane{i}']
+        assert img.dtype == np.float32 and img.shape == ref.shape
+        np.testing.assert_array_equal(img == 0, ref == 0)
+        assert rel_l2(img, ref) < TOL, (i, rel_l2(img, ref))
+    assert svc.handle({"cmd": "set_cor", "px": 1.25})[0] == {"ok": True}
+    rep, img = svc.handle({"cmd": "slice", "origin": [0, 0, 0], "normal": [1, 0.5, 0.8], "n": 32})
+    assert rel_l2(img, golden['cone_plane_cor']) < TOL
+    assert svc.handle({"cmd": "set_cor", "px": 0.0})[0] == {"ok": True}
+    rep, vol = svc.handle({"cmd": "preview", "maxdim": 12})
+    assert vol.shape == golden['cone_preview'].shape and rel_l2(vol, golden['cone_preview']) < TOL
+    np.testing.assert_array_equal(rep["origin"], golden['cone_preview_origin'])
+    assert rep["spacing"] == golden['cone_preview_h']
+    rep, _ = svc.handle({"cmd": "auto_cor"})
+    s_, d_, u_, v_ = np.split(V[0], 4)
+    mag = np.linalg.norm(d_ - s_) / np.linalg.norm(s_)
+    box = np.array([C * np.linalg.norm(u_) / mag / 2] * 2 + [R * np.linalg.norm(v_) / mag / 2])
+    orc = O.OracleReconstructor("cone", R, C, box, vectors=V, store_f32=True)
+    orc.add_projections(np.arange(len(V)), proj)
+    shifts = np.arange(-3.0, 3.01, 0.25)
+    want = np.asarray(orc.cor_scores(shifts, n=192))
+    assert rep["best"] == float(shifts[int(np.argmax(want))])
+    assert svc.recon.cor_shift == rep["best"]

@@ -2,8 +2,9 @@
 
     python tomography-local-demo_amd/build.py [--jobs N]
 
-Each .hip source under csrc/ is compiled to an object (in parallel), then linked
-into minirecast/libminirecast_hip.so next to the Python package that loads it.
+Each .hip source under csrc/ is compiled to an object with hipcc and each host-only
+.cpp source with g++ (in parallel), then linked into minirecast/libminirecast_hip.so
+next to the Python package that loads it.
 """
 import argparse
 import os
@@ -32,13 +33,20 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
+# host-only C++ units (no device code), built with the host compiler
+HOST_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-Wall"]
+
+
 def sources():
-    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
 
 
 def compile_one(src):
     obj = os.path.join(OBJDIR, os.path.basename(src) + ".o")
-    cmd = [hipcc()] + COMMON + ["-c", src, "-o", obj]
+    if src.endswith(".cpp"):
+        cmd = [shutil.which("g++") or "g++"] + HOST_FLAGS + ["-c", src, "-o", obj]
+    else:
+        cmd = [hipcc()] + COMMON + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -160,84 +160,9 @@ int alloc_ring(Ctx* c, int n_total, int rows, int cols, int window, int storage)
     return MR_OK;
 }
 
-// conservative test: can any point of the tile lie inside the box?
-bool tile_may_hit_box(const mr_task& T, int q, int p0, int p1, int s0, int s1) {
-    if (!T.mask_box) return true;
-    for (int d = 0; d < 3; ++d) {
-        const double lim = T.box[d] * (1.0 + 1e-9) + 1e-9;
-        bool all_hi = true, all_lo = true;
-        for (int cp = 0; cp < 2; ++cp)
-            for (int cs = 0; cs < 2; ++cs) {
-                const int p = cp ? p1 : p0, s = cs ? s1 : s0;
-                const double x = T.o[d] + q * T.c[d] + p * T.a[d] + s * T.b[d];
-                all_hi = all_hi && (x > lim);
-                all_lo = all_lo && (x < -lim);
-            }
-        if (all_hi || all_lo) return false;
-    }
-    return true;
-}
-
 int validate_tasks(const mr_task* tasks, int ntasks, int64_t out_len) {
-    for (int i = 0; i < ntasks; ++i) {
-        const mr_task& T = tasks[i];
-        if (T.nq < 0 || T.np < 0 || T.ns < 0)
-            return fail(MR_EVALUE, "task " + std::to_string(i) + ": negative grid size");
-        if (T.nq == 0 || T.np == 0 || T.ns == 0) continue;
-        if (T.out_offset < 0 || T.stride_q < 0 || T.stride_p < 0 || T.stride_s < 0)
-            return fail(MR_EVALUE, "task " + std::to_string(i) + ": negative offset or stride");
-        const int64_t last = T.out_offset + (T.nq - 1) * T.stride_q + (T.np - 1) * T.stride_p +
-                             (T.ns - 1) * T.stride_s;
-        if (last >= out_len)
-            return fail(MR_EVALUE, "task " + std::to_string(i) + " writes beyond the output (" +
-                                       std::to_string(last) + " >= " + std::to_string(out_len) + ")");
-    }
-    return MR_OK;
-}
-
-// tiles: those that may hit the box; empty (optional): the culled rest, whose points
-// are all zero.
-void build_tiles(const mr_task* tasks, int ntasks, std::vector<Tile>& tiles,
-                 std::vector<Tile>* empty = nullptr) {
-    tiles.clear();
-    if (empty) empty->clear();
-    for (int i = 0; i < ntasks; ++i) {
-        const mr_task& T = tasks[i];
-        if (T.nq <= 0 || T.np <= 0 || T.ns <= 0) continue;
-        const int ntp = (T.np + BP_TP - 1) / BP_TP, nts = (T.ns + BP_TS - 1) / BP_TS;
-        for (int q = 0; q < T.nq; ++q)
-            for (int tp = 0; tp < ntp; ++tp)
-                for (int ts = 0; ts < nts; ++ts) {
-                    const int p0 = tp * BP_TP, s0 = ts * BP_TS;
-                    const int p1 = std::min(p0 + BP_TP, T.np) - 1;
-                    const int s1 = std::min(s0 + BP_TS, T.ns) - 1;
-                    if (tile_may_hit_box(T, q, p0, p1, s0, s1))
-                        tiles.push_back({i, q, tp, ts});
-                    else if (empty)
-                        empty->push_back({i, q, tp, ts});
-                }
-    }
-}
-
-// Orders tiles by the height z of their centre.  With the XCD-contiguous block map each
-// XCD then takes a band of heights, i.e. (for a circular orbit about z) a band of
-// detector rows in every view, so a 128-byte detector line is fetched into few of the
-// eight L2 caches instead of all of them.
-void order_tiles_by_height(const mr_task* tasks, std::vector<Tile>& tiles) {
-    std::vector<std::pair<double, int>> key(tiles.size());
-    for (size_t i = 0; i < tiles.size(); ++i) {
-        const Tile& t = tiles[i];
-        const mr_task& T = tasks[t.task];
-        const double p = t.tp * BP_TP + 0.5 * (BP_TP - 1), s = t.ts * BP_TS + 0.5 * (BP_TS - 1);
-        key[i] = {T.o[2] + t.q * T.c[2] + p * T.a[2] + s * T.b[2], static_cast<int>(i)};
-    }
-    std::stable_sort(key.begin(), key.end(),
-                     [](const std::pair<double, int>& a, const std::pair<double, int>& b) {
-                         return a.first < b.first;
-                     });
-    std::vector<Tile> out(tiles.size());
-    for (size_t i = 0; i < tiles.size(); ++i) out[i] = tiles[key[i].second];
-    tiles.swap(out);
+    const std::string m = validate_tasks_msg(tasks, ntasks, out_len);
+    return m.empty() ? MR_OK : fail(MR_EVALUE, m);
 }
 
 enum TaskOp { OP_BP = 0, OP_ADJOINT = 1, OP_DSHIFT = 2 };
@@ -253,29 +178,10 @@ int run_tasks(Ctx* c, const mr_task* tasks, int ntasks, int n_views, float* d_ou
     // instead of enumerating and sorting again (~0.1 ms on the host)
     const size_t tb = sizeof(mr_task) * ntasks;
     const int key_flags = (op == OP_ADJOINT ? 1 : 0) | (c->xcd_map << 1);
-    auto& cache = c->tile_cache;
-    const uint8_t* tk = reinterpret_cast<const uint8_t*>(tasks);
-    int hit = -1;
-    for (int i = 0; i < static_cast<int>(cache.size()) && hit < 0; ++i)
-        if (cache[i].flags == key_flags && cache[i].tasks.size() == tb &&
-            std::memcmp(cache[i].tasks.data(), tk, tb) == 0)
-            hit = i;
-    if (hit < 0) {
-        TileCache e;
-        std::vector<Tile> empty;
-        build_tiles(tasks, ntasks, e.tiles, op == OP_ADJOINT ? nullptr : &empty);
-        if (c->xcd_map == 2) order_tiles_by_height(tasks, e.tiles);
-        e.ne = empty.size();
-        e.tiles.insert(e.tiles.end(), empty.begin(), empty.end());
-        e.tasks.assign(tk, tk + tb);
-        e.flags = key_flags;
-        if (cache.size() >= kTileCacheEntries) cache.pop_back();
-        cache.insert(cache.begin(), std::move(e));
-    } else if (hit > 0) {
-        std::rotate(cache.begin(), cache.begin() + hit, cache.begin() + hit + 1);   // most recent first
-    }
-    const std::vector<Tile>& tiles = cache.front().tiles;
-    const size_t ne = cache.front().ne;
+    const TileCache& entry = cached_tiles(c->tile_cache, tasks, ntasks, key_flags,
+                                          op != OP_ADJOINT, c->xcd_map == 2);
+    const std::vector<Tile>& tiles = entry.tiles;
+    const size_t ne = entry.ne;
     const size_t tl = sizeof(Tile) * tiles.size();
     const size_t off_tiles = (tb + 255) / 256 * 256;
     const size_t need = off_tiles + tl + 256;

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/minirecast_hip.h"
+#include "mr_tiles.h"
 
 namespace mr {
 
@@ -57,11 +58,6 @@ struct FilterView {
 };
 static_assert(sizeof(FilterView) == 48, "FilterView layout");
 
-// Tile work item of a back-projection launch.
-struct Tile {
-    int32_t task, q, tp, ts;
-};
-
 // Device work queue of a persistent back-projection launch.
 struct TileQueue {
     int* head = nullptr;             // atomic tile counter (null: one tile per workgroup)
@@ -76,23 +72,11 @@ struct TileQueue {
     int64_t accum_half = 0;
 };
 
-// Tile list of a recent request of a context (keyed by the tasks' bytes and the
-// operation / block-order flags), reused when a request repeats it.
-constexpr size_t kTileCacheEntries = 4;
-struct TileCache {
-    int flags = 0;
-    std::vector<uint8_t> tasks;
-    std::vector<Tile> tiles;         // ordered tiles, then the culled ones
-    size_t ne = 0;                   // culled tiles at the end
-};
-
 // Bands of the band ring (kernel 7): padded rows 0..R+1 can hold a point's upper tap,
 // band b covers padded rows 3b..3b+3.
 constexpr int band_count(int R) { return (R + 1) / 3 + 1; }
 
-// BP tile shape: TP x TS points per 256-thread workgroup, 4 points per thread.
-constexpr int BP_TS = 32;
-constexpr int BP_TP = 32;
+// BP workgroup size (the tile shape BP_TP x BP_TS is in mr_tiles.h).
 constexpr int BP_THREADS = 256;
 constexpr int BP_VPASS = 64;     // views per LDS record pass
 constexpr int BP_REC = 16;       // floats per (tile, view) record

@@ -1,0 +1,58 @@
+// Host-only request preparation of the back-projection entry points: task validation,
+// tile enumeration with the conservative box test, the height order of the tiles, and
+// the per-context cache of recent tile lists.  Plain C++ (no HIP), so that it also
+// builds with g++ and the sanitizers on its own.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/minirecast_hip.h"
+
+namespace mr {
+
+// BP tile shape: TP x TS points per 256-thread workgroup.
+constexpr int BP_TS = 32;
+constexpr int BP_TP = 32;
+
+// Tile work item of a back-projection launch: task, z-plane q, tile row tp, tile column ts.
+struct Tile {
+    int32_t task, q, tp, ts;
+};
+
+// Tile list of a recent request of a context (keyed by the tasks' bytes and the
+// operation / block-order flags), reused when a request repeats it.
+constexpr size_t kTileCacheEntries = 4;
+struct TileCache {
+    int flags = 0;
+    std::vector<uint8_t> tasks;
+    std::vector<Tile> tiles;         // ordered tiles, then the culled ones
+    size_t ne = 0;                   // culled tiles at the end
+};
+
+// Empty string when every task's writes stay inside [0, out_len), else the reason.
+std::string validate_tasks_msg(const mr_task* tasks, int ntasks, int64_t out_len);
+
+// Conservative test: can any point of the tile lie inside the box?
+bool tile_may_hit_box(const mr_task& T, int q, int p0, int p1, int s0, int s1);
+
+// tiles: those that may hit the box; empty (optional): the culled rest, whose points
+// are all zero.
+void build_tiles(const mr_task* tasks, int ntasks, std::vector<Tile>& tiles,
+                 std::vector<Tile>* empty = nullptr);
+
+// Orders tiles by the height z of their centre (stable).
+void order_tiles_by_height(const mr_task* tasks, std::vector<Tile>& tiles);
+
+// The tile list of a request from the cache (most recent first, at most
+// kTileCacheEntries): a hit moves the entry to the front; a miss enumerates the tiles
+// (keeping the culled ones after them when keep_empty), orders them by height when
+// by_height, and puts the new entry in front, dropping the oldest.  flags is part of
+// the key.  The returned entry stays valid until the next call on the same cache.
+const TileCache& cached_tiles(std::vector<TileCache>& cache, const mr_task* tasks, int ntasks,
+                              int flags, bool keep_empty, bool by_height);
+
+}  // namespace mr
