@@ -419,6 +419,56 @@ def bench_parallel_cor(args):
                        "trial images, float32 ring"}, {"cor": res})
 
 
+def bench_filter(args):
+    """Ingest filtering of the full-resolution scan's views: FDK cosine weight + ramp filter
+    (N = 4096 FFTs of 1536 rows x 1944 columns per view) of device-resident raw batches of
+    40 views into the ring, for each storage form.  Per view the algorithmic traffic is
+    the raw view read (R C 4 B) and the stored view written (R C s B, s = 4 or 2); the FFT
+    work is 2 complex N-point transforms per row pair at 5 N log2 N flops each."""
+    from minirecast import _hip, phantom, workloads
+    w = workloads.full_cone()
+    g = w.geom
+    R, C = g.rows, g.cols
+    batch = 40
+    N = 1 << int(np.ceil(np.log2(2 * C)))
+    raw_ctx = _hip.Context(0)
+    d_raw = raw_ctx.alloc(4 * R * batch * C)
+    sub = type(g)(g.vectors[:batch], R, C)
+    phantom.project_cone_device(w.phantom, sub, raw_ctx, d_raw)
+    raw_ctx.synchronize()
+    flops = 2 * ((R + 1) // 2) * 5 * N * np.log2(N)
+    res = {}
+    for storage in ("f32", "f16", "bf16"):
+        esz = 4 if storage == "f32" else 2
+        ctx = _hip.Context(0)
+        ctx.set_cone(g.vectors, R, C, w.window, storage)
+        idx = np.arange(batch)
+        for _ in range(max(1, args.warmup)):
+            ctx.reset_views()
+            ctx.push_device(idx, d_raw)
+        times = []
+        for _ in range(max(1, args.steps)):
+            ctx.reset_views()
+            t0 = time.perf_counter()
+            ctx.push_device(idx, d_raw)
+            times.append((time.perf_counter() - t0) * 1e3)
+        ms = float(np.median(times))
+        byts = R * C * (4 + esz) * batch
+        res[storage] = dict(ms_per_batch=ms, ms_per_view=ms / batch,
+                            algorithmic_gbs=byts / (ms * 1e-3) / 1e9,
+                            hbm_frac=byts / (ms * 1e-3) / 8e12,
+                            fft_tflops=flops * batch / (ms * 1e-3) / 1e12)
+        log(f"filter {storage}: {res[storage]}")
+        ctx.close()
+    raw_ctx.free(d_raw)
+    h = res["f32"]
+    _line("ms per filtered view (FDK weight + ramp filter, 1536x1944 detector, N = 4096)",
+          round(h["ms_per_view"], 5), "ms", round(h["ms_per_batch"], 4), args.steps, args.warmup,
+          {"workload": "filter: full_cone ingest, device-resident raw batches of 40 views, "
+                       "f32 / f16 / bf16 ring", "batch": batch, "fft_length": N},
+          {"filter": res}, higher=False)
+
+
 def bench_diff_cone(args):
     """Differentiable back-projection (256^3-box phantom, 360 views, 256 x 256 detector,
     3 oblique 256^2 slices): forward + backward through the hand-written adjoint,
@@ -509,7 +559,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="full_cone",
                     choices=["full_cone", "full_volume", "stream_cone", "parallel_cor",
-                             "diff_cone"])
+                             "diff_cone", "filter"])
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-check", action="store_true", help="skip the float64 oracle error")
@@ -529,7 +579,8 @@ def main():
 
     if args.workload != "full_cone":
         {"stream_cone": bench_stream_cone, "parallel_cor": bench_parallel_cor,
-         "diff_cone": bench_diff_cone, "full_volume": bench_full_volume}[args.workload](args)
+         "diff_cone": bench_diff_cone, "full_volume": bench_full_volume,
+         "filter": bench_filter}[args.workload](args)
         return
     from minirecast import _hip, phantom, recon, workloads
     if args.calibrate_copy:

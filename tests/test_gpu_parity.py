@@ -23,7 +23,7 @@ def _box_cone(V, R, C):
 
 
 # ---------------------------------------------------------------- filtering
-@pytest.mark.parametrize("n", [2, 5, 96, 486, 1944, 2048])
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 9, 17, 96, 486, 1025, 1944, 2048, 3001, 4096])
 def test_filter_rows_vs_oracle(n):
     from minirecast import recon
     rng = np.random.default_rng(n)
@@ -33,6 +33,37 @@ def test_filter_rows_vs_oracle(n):
         want = O.filter_rows(x, 0.3, window)
         assert got.dtype == np.float32 and got.shape == x.shape
         assert rel_l2(got, want) < TOL, (n, window, rel_l2(got, want))
+
+
+@pytest.mark.parametrize("C", [48, 486, 1944, 4096])
+def test_f16_ingest_vs_oracle_and_f32(C):
+    """f16 rings store each view with a power-of-two scale chosen before filtering from
+    the bound max|q| <= max|x| gain sum|h|: the stored values against the float64 oracle
+    (<= 1e-3, one f16 rounding each), the scale an exact power of two with every stored
+    value in range, and the same views as the f32 ring to f16 rounding; the f16 ring's
+    slices against the f32 ring's."""
+    from minirecast import _hip, recon
+    from minirecast.geometry import circular_cone_vectors
+    R, A = 9, 6
+    g = circular_cone_vectors(A, 400.0, 600.0, R, C, 0.15)
+    rng = np.random.default_rng(C)
+    block = rng.uniform(0.0, 3.0, size=(R, A, C)).astype(np.float32)
+    block[:, 2, :] *= 1e-3                               # views of very different scales
+    block[:, 4, :] *= 50.0
+    r32 = recon.SliceReconstructor(block, g, window="hann", verbose=False)
+    r16 = recon.SliceReconstructor(block, g, window="hann", verbose=False, storage="f16")
+    q32 = r32.filtered_views()
+    q16 = r16.filtered_views()
+    K = O.cone_constants(g.vectors)
+    q64 = O.fdk_filter_block(block, K, "hann")
+    assert rel_l2(q32, q64) < TOL
+    for v in range(A):
+        assert rel_l2(q16[v], q64[v]) < 1e-3, (v, rel_l2(q16[v], q64[v]))
+        assert rel_l2(q16[v], q32[v]) < 1e-3
+    assert np.isfinite(q16).all()
+    img32, _ = r32.reconstruct_plane((0, 0, 0), (1, 0.5, 0.8), n=40)
+    img16, _ = r16.reconstruct_plane((0, 0, 0), (1, 0.5, 0.8), n=40)
+    assert rel_l2(img16, img32) < 1e-3
 
 
 def test_fdk_filter_block_vs_oracle():

@@ -102,11 +102,20 @@ void free_geometry(Ctx* c) {
     ++c->views_epoch;
 }
 
-// float32 device tables of the ramp response (tau = 1) and FFT twiddles for length N
-int make_tables(int N, int window, float** d_H, float2** d_tw) {
+// float32 device tables of the ramp response (tau = 1) and FFT twiddles for length N;
+// l1 (optional) = sum_m |h(m)|, h = the unnormalised inverse DFT of the response (real and
+// even, so a forward DFT gives it), in float64
+int make_tables(int N, int window, float** d_H, float2** d_tw, double* l1 = nullptr) {
     std::vector<double> H;
     int rc = ramp_response_host(N / 2, 1.0, window, H);
     if (rc) return rc;
+    if (l1) {
+        std::vector<std::complex<double>> h(H.begin(), H.end());
+        fft64(h);
+        double sum = 0.0;
+        for (const auto& z : h) sum += std::abs(z.real());
+        *l1 = sum;
+    }
     std::vector<float> Hf(N);
     std::vector<float2> tw(N);
     for (int k = 0; k < N; ++k) {
@@ -152,7 +161,7 @@ int alloc_ring(Ctx* c, int n_total, int rows, int cols, int window, int storage)
     MR_HIP(hipMalloc(reinterpret_cast<void**>(&c->d_vscale), sizeof(float) * n_total));
     std::vector<float> ones(n_total, 1.0f);
     MR_HIP(hipMemcpy(c->d_vscale, ones.data(), sizeof(float) * n_total, hipMemcpyHostToDevice));
-    int rc = make_tables(c->N, window, &c->d_H, &c->d_tw);
+    int rc = make_tables(c->N, window, &c->d_H, &c->d_tw, &c->h_l1);
     if (rc) return rc;
     MR_HIP(hipStreamSynchronize(c->s_ingest));
     c->received.assign(n_total, 0);
@@ -568,10 +577,6 @@ static int append_views_locked(mr_ctx* c, const int64_t* indices, int k, const f
     MR_HIP(hipMemcpyAsync(c->d_geom + c->n, gv.data(), sizeof(ViewGeom) * k, hipMemcpyHostToDevice,
                           c->s_ingest));
     if (c->storage == MR_STORE_F16) {
-        if (filter) {
-            rc = ensure(reinterpret_cast<void**>(&c->d_fstage), &c->fstage_bytes, blk);
-            if (rc) return rc;
-        }
         rc = ensure(reinterpret_cast<void**>(&c->d_vmax), &c->vmax_len, sizeof(unsigned) * k);
         if (rc) return rc;
     }
@@ -1200,7 +1205,7 @@ static void plan_free(Plan* p) {
     if (p->cor_exec) hipGraphExecDestroy(p->cor_exec);
     p->cor_exec = nullptr;
     void* dev[] = {p->d_tasks, p->d_toff, p->d_queue, p->d_counters, p->d_out, p->d_block, p->d_meta,
-                   p->d_fstage, p->d_vmax, p->d_scores, p->d_acc, p->d_corws};
+                   p->d_vmax, p->d_scores, p->d_acc, p->d_corws};
     for (void* d : dev)
         if (d) hipFree(d);
     if (p->h_block) hipHostFree(p->h_block);
@@ -1461,19 +1466,17 @@ int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block
         if (p->h_meta) hipHostFree(p->h_meta);
         if (p->d_block) hipFree(p->d_block);
         if (p->d_meta) hipFree(p->d_meta);
-        if (p->d_fstage) hipFree(p->d_fstage);
         if (p->d_vmax) hipFree(p->d_vmax);
         p->h_block = nullptr; p->h_meta = nullptr; p->d_block = nullptr; p->d_meta = nullptr;
-        p->d_fstage = nullptr; p->d_vmax = nullptr;
+        p->d_vmax = nullptr;
         p->kmax = 0;
         MR_HIP(hipHostMalloc(reinterpret_cast<void**>(&p->h_block), blk, hipHostMallocDefault));
         MR_HIP(hipHostMalloc(&p->h_meta, meta, hipHostMallocDefault));
         MR_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_block), blk));
         MR_HIP(hipMalloc(&p->d_meta, meta));
         if (c->storage == MR_STORE_F16) {
-            // the graphs' own f16 staging: the context's d_fstage / d_vmax are regrown
-            // (freed) by other entry points, which would leave a graph on freed memory
-            MR_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_fstage), blk));
+            // the graphs' own per-view max buffer: the context's d_vmax is regrown (freed)
+            // by other entry points, which would leave a graph on freed memory
             MR_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_vmax), sizeof(unsigned) * k));
         }
         p->kmax = k;
@@ -1519,7 +1522,7 @@ int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block
         int r = (e1 != hipSuccess || e2 != hipSuccess) ? MR_ERUNTIME : MR_OK;
         if (!r) r = launch_stream_prep(c, p->d_meta, k, p->d_counters + 2, c->s_req);
         if (!r) r = launch_filter(c, p->d_block, k, 0, c->s_req, static_cast<const FilterView*>(p->d_meta),
-                                  p->d_fstage, p->d_vmax);
+                                  p->d_vmax);
         if (!r && rk == 2) r = launch_band_convert_tail(c, p->d_counters + 2, k, c->s_req);
         if (!r && rk == 1) r = launch_pair_convert_tail(c, p->d_counters + 2, k, c->s_req);
         if (!r) r = launch_reset_head(p->d_counters, c->s_req);

@@ -82,11 +82,13 @@ constexpr int bitrev(int q, int R) {
 // LDS index of buffer element i: the low 4 bits XORed with the next 4 (a bijection).
 __device__ __forceinline__ int sw(int i) { return i ^ ((i >> 4) & 15); }
 
-// Passes of an N = 2^LOGN transform: radix 2^(LOGN mod 4) first (if not 1), then 16s.
-template <int LOGN> struct FftPlan {
+// Passes of an N = 2^LOGN transform: radix 2^(LOGN mod 4) first (if not 1), then 16s;
+// REV: the same radices in the reverse order (the small one last).
+template <int LOGN, bool REV = false> struct FftPlan {
     static constexpr int REM = LOGN % 4;
     static constexpr int NPASS = LOGN / 4 + (REM ? 1 : 0);
-    static constexpr int radix(int p) { return (REM && p == 0) ? (1 << REM) : 16; }
+    static constexpr int radix_fwd(int p) { return (REM && p == 0) ? (1 << REM) : 16; }
+    static constexpr int radix(int p) { return REV ? radix_fwd(NPASS - 1 - p) : radix_fwd(p); }
     static constexpr int span(int p) {          // product of the radices before pass p
         int n = 1;
         for (int i = 0; i < p; ++i) n *= radix(i);
@@ -94,71 +96,131 @@ template <int LOGN> struct FftPlan {
     }
 };
 
-// One Stockham pass of radix R over span NS (the product of the earlier radices):
-// butterfly j (k = j mod NS) takes in(j + r N/R), r < R, twiddles input r by
-// exp(-2 pi i r k / (NS R)), and writes DFT output q to out((j - k) R + k + q NS).
-// SYNC: a barrier between reading and writing (in-place passes through LDS).
-template <int N, int R, int NS, bool SYNC, int FT, typename In, typename Out>
-__device__ __forceinline__ void radix_pass(const float2* __restrict__ tw, In in, Out out) {
-    constexpr int NB = N / R;
-    constexpr int PER = (NB + FT - 1) / FT;
-    float2 v[PER][R];
+// In-register DIF DFT whose inputs R/2 .. R-1 are zero: the first stage only twiddles.
+template <int R>
+__device__ __forceinline__ void dft_dif_upper_zero(float2 (&x)[R]) {
+    if constexpr (R == 1) {
+        return;
+    } else {
+        constexpr int s = R / 2;
 #pragma unroll
-    for (int e = 0; e < PER; ++e) {
-        const int j = threadIdx.x + e * FT;
-        if (NB % FT == 0 || j < NB) {
-            const int k = j & (NS - 1);
+        for (int i = 0; i < s; ++i) x[i + s] = w16(x[i], i * (8 / s));
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[e][r] = in(j + r * NB);
-            if constexpr (NS > 1) {
+        for (int t = s / 2; t >= 1; t >>= 1) {
 #pragma unroll
-                for (int r = 1; r < R; ++r) v[e][r] = cmul(v[e][r], tw[r * k * (N / (NS * R))]);
+            for (int b = 0; b < R; b += 2 * t) {
+#pragma unroll
+                for (int i = 0; i < t; ++i) {
+                    const float2 a = x[b + i], c = x[b + i + t];
+                    x[b + i] = cadd(a, c);
+                    x[b + i + t] = w16(csub(a, c), i * (8 / t));
+                }
             }
-            dft_dif<R>(v[e]);
         }
     }
-    if constexpr (SYNC) __syncthreads();
+}
+
+// One Stockham pass of radix R over span NS (the product of the earlier radices):
+// butterfly j (k = j mod NS) takes in(j + r N/R), r < R, twiddles input r by
+// exp(-2 pi i r k / (NS R)) and computes its R-point DFT in v (outputs in bit-reversed
+// order); pass_store then writes DFT output q to out((j - k) R + k + q NS).  Thread t
+// owns butterflies t + e FT, e < PER.  UZ: inputs r >= R/2 are known to be zero (the
+// zero padding of the first pass) and are neither read nor added.
+template <int N, int R, int FT> struct Pass {
+    static constexpr int NB = N / R;
+    static constexpr int PER = (NB + FT - 1) / FT;
+    using Regs = float2[PER][R];
+    // the thread index, opaque to the compiler: index expressions of different passes
+    // are not merged (they would stay live across the passes and spill at N = 4096)
+    __device__ static int tid() {
+        int t = threadIdx.x;
+        asm volatile("" : "+v"(t));
+        return t;
+    }
+    __device__ static bool has(int t, int e) { return NB % FT == 0 || t + e * FT < NB; }
+    template <int NS, bool UZ, typename In>
+    __device__ static void load_dft(const float2* __restrict__ tw, In in, Regs& v) {
+        const int t = tid();
 #pragma unroll
-    for (int e = 0; e < PER; ++e) {
-        const int j = threadIdx.x + e * FT;
-        if (NB % FT == 0 || j < NB) {
+        for (int e = 0; e < PER; ++e) {
+            const int j = t + e * FT;
+            if (!has(t, e)) continue;
+            const int k = j & (NS - 1);
+            // the twiddles first, all in flight together, then the inputs
+            float2 w[R];
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) w[r] = tw[r * k * (N / (NS * R))];
+            }
+#pragma unroll
+            for (int r = 0; r < (UZ ? R / 2 : R); ++r) v[e][r] = in(j + r * NB);
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[e][r] = cmul(v[e][r], w[r]);
+            }
+            if constexpr (UZ) dft_dif_upper_zero<R>(v[e]);
+            else dft_dif<R>(v[e]);
+        }
+    }
+    template <int NS, typename Out>
+    __device__ static void store(Out out, const Regs& v) {
+        const int t = tid();
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int j = t + e * FT;
+            if (!has(t, e)) continue;
             const int k = j & (NS - 1);
             const int base = (j - k) * R + k;
 #pragma unroll
             for (int q = 0; q < R; ++q) out(base + q * NS, v[e][bitrev(q, R)]);
         }
     }
-}
+};
 
-// The transform from pass P on: pass P reads `in` (IN_LDS: from the LDS buffer), the
-// passes in between go through LDS, the last pass writes `out` (OUT_LDS: into the LDS
-// buffer, followed by a barrier).
-template <int LOGN, int P, bool IN_LDS, bool OUT_LDS, typename In, typename Out>
-__device__ __forceinline__ void fft_from(const float2* __restrict__ tw, float2* buf, In in, Out out) {
-    using PL = FftPlan<LOGN>;
-    constexpr int N = 1 << LOGN, R = PL::radix(P), NS = PL::span(P), FT = ft_of(LOGN);
-    if constexpr (P + 1 == PL::NPASS) {
-        radix_pass<N, R, NS, IN_LDS && OUT_LDS, FT>(tw, in, out);
-        if constexpr (OUT_LDS) __syncthreads();
-    } else {
-        radix_pass<N, R, NS, IN_LDS, FT>(tw, in, [buf](int i, float2 z) { buf[sw(i)] = z; });
+// Passes P .. LAST - 1 of plan PL through the LDS buffer, in place: each reads buf,
+// waits for every thread's reads, writes buf and waits again.
+template <typename PL, int N, int FT, int P, int LAST>
+__device__ __forceinline__ void lds_passes(const float2* __restrict__ tw, float2* buf) {
+    if constexpr (P < LAST) {
+        constexpr int R = PL::radix(P), NS = PL::span(P);
+        using PS = Pass<N, R, FT>;
+        typename PS::Regs v;
+        PS::template load_dft<NS, false>(tw, [buf](int i) { return buf[sw(i)]; }, v);
         __syncthreads();
-        fft_from<LOGN, P + 1, true, OUT_LDS>(tw, buf, [buf](int i) { return buf[sw(i)]; }, out);
+        PS::template store<NS>([buf](int i, float2 z) { buf[sw(i)] = z; }, v);
+        __syncthreads();
+        // keep the next pass's twiddle loads below the barrier (hoisting them above it
+        // raises the register pressure past 128 VGPRs at N = 4096)
+        __builtin_amdgcn_sched_barrier(0);
+        lds_passes<PL, N, FT, P + 1, LAST>(tw, buf);
     }
 }
 
-// MODE 0: ring float32; 1: ring bfloat16; 2: float32 staging [k][R][C] plus per-view
-// max |q| (for F16 quantisation); 3: plain float32 rows [R][C] (filter_rows hook).
-// At least 3 waves per SIMD at N = 4096 (the radix-16 passes fit 168 VGPRs without
-// spilling; 4 waves would spill), 4 below.
+// MODE 0: ring float32; 1: ring bfloat16; 2: ring float16 with a power-of-two scale per
+// view from the bound max|q| <= max|x| * gain * l1 (vmax: max |x| of the view's raw
+// rows, l1: sum_m |h(m)| of the tau = 1 response), stored in vscale; 3: plain float32
+// rows [R][C] (filter_rows hook); 4: adjoint (filter, then the cosine weight, views
+// [k][R][C] in, blocks [R][k][C] out).
+//
+// Both transforms are forward Stockham FFTs through the LDS buffer; the second runs its
+// passes in the reverse radix order, so that its first pass (radix 16, span 1) reads in
+// each thread exactly the indices j + r N/16 the last pass of the first transform
+// leaves in that thread's registers: the product with H and the conjugation are done
+// there, with no exchange through LDS.
 template <int LOGN, int MODE>
 __global__ __launch_bounds__(ft_of(LOGN))
-__attribute__((amdgpu_waves_per_eu(LOGN >= 13 ? 1 : LOGN == 12 ? 3 : 4))) void filter_kernel(
+__attribute__((amdgpu_waves_per_eu(LOGN >= 12 ? 4 : 5))) void filter_kernel(
     const float* __restrict__ block, int R, int k, int C, const FilterView* __restrict__ fv,
     const float* __restrict__ H, const float2* __restrict__ tw, void* __restrict__ dst, int pitch,
-    int64_t view_stride, unsigned* __restrict__ vmax) {
+    int64_t view_stride, const unsigned* __restrict__ vmax, float l1,
+    float* __restrict__ vscale) {
     constexpr int N = 1 << LOGN, FT = ft_of(LOGN);
-    __shared__ float2 buf[N];
+    using P1 = FftPlan<LOGN, false>;
+    using P2 = FftPlan<LOGN, true>;
+    constexpr int NP = P1::NPASS;
+    static_assert(P1::radix(NP - 1) == P2::radix(0), "fused passes share the radix");
+    __shared__ float2 lds[N];
+    float2* const buf = lds;                         // captured by the pass lambdas
     const int r0 = 2 * blockIdx.x, r1 = r0 + 1;
     const int v = blockIdx.y;
     const FilterView P = fv[v];
@@ -169,7 +231,8 @@ __attribute__((amdgpu_waves_per_eu(LOGN >= 13 ? 1 : LOGN == 12 ? 3 : 4))) void f
     const float* x1 = MODE == 4 ? block + (static_cast<int64_t>(v) * R + (has1 ? r1 : r0)) * C
                                 : block + (static_cast<int64_t>(has1 ? r1 : r0) * k + v) * C;
     const float b0 = P.b0 + r0 * P.db, b1 = P.b0 + r1 * P.db;
-    // transform 1 reads the (cosine-weighted, zero-padded) row pair from global memory
+    // transform 1 reads the (cosine-weighted) row pair from global memory; columns
+    // beyond C read as zero, and the upper half (>= N/2 >= C) is not read at all
     auto in_rows = [&](int c) -> float2 {
         float y0 = 0.0f, y1 = 0.0f;
         if (c < C) {
@@ -184,43 +247,95 @@ __attribute__((amdgpu_waves_per_eu(LOGN >= 13 ? 1 : LOGN == 12 ? 3 : 4))) void f
         }
         return make_float2(y0, y1);
     };
-    fft_from<LOGN, 0, false, true>(tw, buf, in_rows, [&](int i, float2 z) { buf[sw(i)] = z; });
-    // transform 2 starts from conj(Z) H and writes the filtered rows
-    auto in_h = [&](int c) -> float2 {
-        const float2 z = buf[sw(c)];
-        const float h = H[c];
-        return make_float2(z.x * h, -z.y * h);
-    };
+    auto to_lds = [buf](int i, float2 z) { buf[sw(i)] = z; };
+    constexpr int RF = P2::radix(0);                 // radix of the fused passes
+    using PF = Pass<N, RF, FT>;
+    typename PF::Regs vf;
+    if constexpr (NP == 1) {
+        PF::template load_dft<1, true>(tw, in_rows, vf);
+    } else {
+        {
+            constexpr int R0 = P1::radix(0);
+            using P0 = Pass<N, R0, FT>;
+            typename P0::Regs v0;
+            P0::template load_dft<1, true>(tw, in_rows, v0);
+            P0::template store<1>(to_lds, v0);
+            __syncthreads();
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        lds_passes<P1, N, FT, 1, NP - 1>(tw, buf);
+        PF::template load_dft<P1::span(NP - 1), false>(tw, [buf](int i) { return buf[sw(i)]; }, vf);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    // transform 1's output index j + q N/RF is in vf[e][bitrev(q)]: transform 2's first
+    // pass takes conj(Z) H at the same indices, r = q
+    const int tf = PF::tid();
+#pragma unroll
+    for (int e = 0; e < PF::PER; ++e) {
+        const int j = tf + e * FT;
+        if (!PF::has(tf, e)) continue;
+        float2 u[RF];
+#pragma unroll
+        for (int r = 0; r < RF; ++r) {
+            const float2 z = vf[e][bitrev(r, RF)];
+            const float h = H[j + r * (N / RF)];
+            u[r] = make_float2(z.x * h, -z.y * h);
+        }
+        dft_dif<RF>(u);
+#pragma unroll
+        for (int r = 0; r < RF; ++r) vf[e][r] = u[r];
+    }
     const float g = P.gain;
-    float lmax = 0.0f;
+    float inv_s = 1.0f;
+    if (MODE == 2) {
+        // f16: scale 2^(e - 15) with the bound < 2^e, so |q| / scale < 2^15
+        const float bound = __uint_as_float(vmax[v]) * g * l1;
+        float sc = 1.0f;
+        if (bound > 0.0f && isfinite(bound)) {
+            int ex;
+            frexpf(bound, &ex);
+            sc = ldexpf(1.0f, ex - 15);
+        }
+        inv_s = 1.0f / sc;
+        if (blockIdx.x == 0 && threadIdx.x == 0) vscale[P.slot] = sc;
+    }
+    // uniform row bases (one per row of the pair), indexed by the column
+    float* o0 = nullptr;
+    float* o1 = nullptr;
+    uint16_t* h0 = nullptr;
+    uint16_t* h1 = nullptr;
+    if (MODE == 0) {
+        o0 = static_cast<float*>(dst) + static_cast<int64_t>(P.slot) * view_stride +
+             static_cast<int64_t>(r0 + 1) * pitch + 1;
+        o1 = o0 + pitch;
+    } else if (MODE == 1 || MODE == 2) {
+        h0 = static_cast<uint16_t*>(dst) + static_cast<int64_t>(P.slot) * view_stride +
+             static_cast<int64_t>(r0 + 1) * pitch + 1;
+        h1 = h0 + pitch;
+    } else if (MODE == 3) {
+        o0 = static_cast<float*>(dst) + static_cast<int64_t>(r0) * C;
+        o1 = o0 + C;
+    } else {
+        o0 = static_cast<float*>(dst) + (static_cast<int64_t>(r0) * k + v) * C;
+        o1 = static_cast<float*>(dst) + (static_cast<int64_t>(has1 ? r1 : r0) * k + v) * C;
+    }
     auto out_rows = [&](int c, float2 z) {
         if (c >= C) return;
         const float y0 = z.x * g, y1 = -z.y * g;
-        if (MODE == 0) {
-            float* q = static_cast<float*>(dst) + static_cast<int64_t>(P.slot) * view_stride;
-            q[static_cast<int64_t>(r0 + 1) * pitch + c + 1] = y0;
-            if (has1) q[static_cast<int64_t>(r1 + 1) * pitch + c + 1] = y1;
+        if (MODE == 0 || MODE == 3) {
+            o0[c] = y0;
+            if (has1) o1[c] = y1;
         } else if (MODE == 1) {
-            uint16_t* q = static_cast<uint16_t*>(dst) + static_cast<int64_t>(P.slot) * view_stride;
             auto bf = [](float f) -> uint16_t {
                 uint32_t u = __float_as_uint(f);
                 u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even
                 return static_cast<uint16_t>(u >> 16);
             };
-            q[static_cast<int64_t>(r0 + 1) * pitch + c + 1] = bf(y0);
-            if (has1) q[static_cast<int64_t>(r1 + 1) * pitch + c + 1] = bf(y1);
+            h0[c] = bf(y0);
+            if (has1) h1[c] = bf(y1);
         } else if (MODE == 2) {
-            float* q = static_cast<float*>(dst) + static_cast<int64_t>(v) * R * C;
-            q[static_cast<int64_t>(r0) * C + c] = y0;
-            lmax = fmaxf(lmax, fabsf(y0));
-            if (has1) {
-                q[static_cast<int64_t>(r1) * C + c] = y1;
-                lmax = fmaxf(lmax, fabsf(y1));
-            }
-        } else if (MODE == 3) {
-            float* q = static_cast<float*>(dst);
-            q[static_cast<int64_t>(r0) * C + c] = y0;
-            if (has1) q[static_cast<int64_t>(r1) * C + c] = y1;
+            h0[c] = __half_as_ushort(__float2half_rn(y0 * inv_s));
+            if (has1) h1[c] = __half_as_ushort(__float2half_rn(y1 * inv_s));
         } else {
             // adjoint: the cosine weight after the (self-adjoint) filter, block layout
             float w0 = 1.0f, w1 = 1.0f;
@@ -230,28 +345,57 @@ __attribute__((amdgpu_waves_per_eu(LOGN >= 13 ? 1 : LOGN == 12 ? 3 : 4))) void f
                 w0 = P.sdd * __frsqrt_rn(fmaf(b0, b0, aa));
                 w1 = P.sdd * __frsqrt_rn(fmaf(b1, b1, aa));
             }
-            float* q = static_cast<float*>(dst);
-            q[(static_cast<int64_t>(r0) * k + v) * C + c] = y0 * w0;
-            if (has1) q[(static_cast<int64_t>(r1) * k + v) * C + c] = y1 * w1;
+            o0[c] = y0 * w0;
+            if (has1) o1[c] = y1 * w1;
         }
     };
-    fft_from<LOGN, 0, true, false>(tw, buf, in_h, out_rows);
-    if (MODE == 2) {
-        // workgroup max through LDS, then one atomic per workgroup
-        __shared__ float red[FT];
-        red[threadIdx.x] = lmax;
+    if constexpr (NP == 1) {
+        PF::template store<1>(out_rows, vf);
+    } else {
+        // the other threads may still read transform 1's last buffer
         __syncthreads();
-        for (int w = FT / 2; w > 0; w >>= 1) {
-            if (threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) atomicMax(vmax + v, __float_as_uint(red[0]));
+        PF::template store<1>(to_lds, vf);
+        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+        lds_passes<P2, N, FT, 1, NP - 1>(tw, buf);
+        constexpr int RL = P2::radix(NP - 1);
+        using PL = Pass<N, RL, FT>;
+        typename PL::Regs vl;
+        PL::template load_dft<P2::span(NP - 1), false>(tw, [buf](int i) { return buf[sw(i)]; }, vl);
+        PL::template store<P2::span(NP - 1)>(out_rows, vl);
     }
 }
 
-// F16 storage: power-of-two scale per view so that max |q| / scale <= 2^15, then
-// round to nearest half.  The scale is stored per slot and folded into the
-// back-projection weight.
+// Per-view max |x| of raw blocks [R][k][C] (the f16 scale bound of filter_kernel).
+// Workgroup (x, v) takes rows x, x + gridDim.x, ... of view v, so that each view gets
+// gridDim.x (at most 128) atomics: one atomic per row would serialise ~R atomics on one
+// address.
+__global__ __launch_bounds__(256) void input_max_kernel(const float* __restrict__ block, int R,
+                                                        int k, int C, unsigned* __restrict__ vmax) {
+    const int v = blockIdx.y;
+    float m = 0.0f;
+    for (int r = blockIdx.x; r < R; r += gridDim.x) {
+        const float* src = block + (static_cast<int64_t>(r) * k + v) * C;
+        int c = threadIdx.x;
+        // four independent loads in flight per thread
+        for (; c + 768 < C; c += 1024)
+            m = fmaxf(fmaxf(m, fmaxf(fabsf(src[c]), fabsf(src[c + 256]))),
+                      fmaxf(fabsf(src[c + 512]), fabsf(src[c + 768])));
+        for (; c < C; c += 256) m = fmaxf(m, fabsf(src[c]));
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    __shared__ float red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        atomicMax(vmax + v, __float_as_uint(m));
+    }
+}
+
+// F16 storage of already-filtered views: power-of-two scale per view so that
+// max |q| / scale <= 2^15, then round to nearest half.  The scale is stored per slot
+// and folded into the back-projection weight.
 __global__ void quantize_f16_kernel(const float* __restrict__ stage, int R, int C,
                                     const FilterView* __restrict__ fv,
                                     const unsigned* __restrict__ vmax, __half* __restrict__ ring,
@@ -331,11 +475,12 @@ __global__ void vmax_kernel(const float* __restrict__ q, int R, int C, unsigned*
 template <int MODE>
 int launch_mode(int logn, dim3 grid, hipStream_t s, const float* block, int R, int k, int C,
                 const FilterView* fv, const float* H, const float2* tw, void* dst, int pitch,
-                int64_t view_stride, unsigned* vmax) {
+                int64_t view_stride, const unsigned* vmax = nullptr, float l1 = 0.0f,
+                float* vscale = nullptr) {
 #define MR_FILTER_CASE(L)                                                                  \
     case L:                                                                                \
         hipLaunchKernelGGL((filter_kernel<L, MODE>), grid, dim3(ft_of(L)), 0, s, block, R, k, C, \
-                           fv, H, tw, dst, pitch, view_stride, vmax);                      \
+                           fv, H, tw, dst, pitch, view_stride, vmax, l1, vscale);          \
         break;
     switch (logn) {
         MR_FILTER_CASE(1) MR_FILTER_CASE(2) MR_FILTER_CASE(3) MR_FILTER_CASE(4) MR_FILTER_CASE(5)
@@ -359,30 +504,29 @@ int log2i(int N) {
 }  // namespace
 
 int launch_filter(Ctx* c, const float* d_block, int k, int n0, hipStream_t s,
-                  const FilterView* fv, float* fstage, unsigned* vmax) {
+                  const FilterView* fv, unsigned* vmax) {
     (void)n0;
     if (!fv) fv = c->d_fv;
-    if (!fstage) fstage = c->d_fstage;
     if (!vmax) vmax = c->d_vmax;
     const int logn = log2i(c->N);
     dim3 grid((c->R + 1) / 2, k);
     if (c->storage == MR_STORE_F32)
         return launch_mode<0>(logn, grid, s, d_block, c->R, k, c->C, fv, c->d_H, c->d_tw,
-                              c->ring, c->pitch, c->view_stride, nullptr);
+                              c->ring, c->pitch, c->view_stride);
     if (c->storage == MR_STORE_BF16)
         return launch_mode<1>(logn, grid, s, d_block, c->R, k, c->C, fv, c->d_H, c->d_tw,
-                              c->ring, c->pitch, c->view_stride, nullptr);
-    // F16: filter to float32 staging, then quantise with a per-view scale (the max
-    // buffer is cleared with a kernel so that the sequence can be graph-captured)
+                              c->ring, c->pitch, c->view_stride);
+    // F16: the per-view max |x| of the raw rows bounds the filtered view (max |q| <=
+    // max |x| gain l1, the FDK weight is at most 1), so its scale is known before the
+    // filter writes the halves; the max buffer is cleared with a kernel so that the
+    // sequence can be graph-captured
     hipLaunchKernelGGL(zero_u32_kernel, dim3((k + 255) / 256), dim3(256), 0, s, vmax, k);
-    int rc = launch_mode<2>(logn, grid, s, d_block, c->R, k, c->C, fv, c->d_H, c->d_tw,
-                            fstage, c->pitch, c->view_stride, vmax);
-    if (rc) return rc;
-    hipLaunchKernelGGL(quantize_f16_kernel, dim3(c->R, k), dim3(256), 0, s, fstage, c->R,
-                       c->C, fv, vmax, static_cast<__half*>(c->ring), c->pitch,
-                       c->view_stride, c->d_vscale);
+    hipLaunchKernelGGL(input_max_kernel, dim3(std::min(c->R, 128), k), dim3(256), 0, s, d_block,
+                       c->R, k, c->C, vmax);
     MR_HIP(hipGetLastError());
-    return MR_OK;
+    return launch_mode<2>(logn, grid, s, d_block, c->R, k, c->C, fv, c->d_H, c->d_tw, c->ring,
+                          c->pitch, c->view_stride, vmax, static_cast<float>(c->h_l1),
+                          c->d_vscale);
 }
 
 int launch_store(Ctx* c, const float* d_q, int k, hipStream_t s) {
@@ -407,7 +551,7 @@ int launch_store(Ctx* c, const float* d_q, int k, hipStream_t s) {
 int launch_filter_adjoint(Ctx* c, const float* d_gq, int k, float* d_graw, hipStream_t s) {
     dim3 grid((c->R + 1) / 2, k);
     return launch_mode<4>(log2i(c->N), grid, s, d_gq, c->R, k, c->C, c->d_fv, c->d_H, c->d_tw,
-                          d_graw, 0, 0, nullptr);
+                          d_graw, 0, 0);
 }
 
 int launch_stream_prep(Ctx* c, const void* d_meta, int k, int* d_nviews, hipStream_t s) {
@@ -432,8 +576,7 @@ int launch_filter_rows_plain(Ctx* c, const float* d_in, int rows, int n, float g
     MR_HIP(hipStreamSynchronize(s));
     MR_HIP(hipMemcpy(c->d_fv, &fv, sizeof(fv), hipMemcpyHostToDevice));
     dim3 grid((rows + 1) / 2, 1);
-    return launch_mode<3>(log2i(N), grid, s, d_in, rows, 1, n, c->d_fv, d_H, d_tw, d_out, 0, 0,
-                          nullptr);
+    return launch_mode<3>(log2i(N), grid, s, d_in, rows, 1, n, c->d_fv, d_H, d_tw, d_out, 0, 0);
 }
 
 }  // namespace mr

@@ -157,14 +157,16 @@ struct Ctx {
     ViewGeom* d_geom = nullptr;      // [n_total] per slot
     float* d_vscale = nullptr;       // [n_total] per slot
     float* d_H = nullptr;            // [N] ramp response for tau = 1 (windowed)
+    double h_l1 = 0.0;               // sum_m |h(m)| of that response's unnormalised inverse
+                                     // DFT (the f16 scale bound of the filter)
     float2* d_tw = nullptr;          // [N] twiddles exp(-2 pi i k / N)
     float* d_stage = nullptr;        // raw block staging
     size_t stage_bytes = 0;
     float* d_pre = nullptr;          // preprocessing ping-pong buffer
     size_t pre_bytes = 0;
-    float* d_fstage = nullptr;       // f32 filtered staging (f16 storage)
+    float* d_fstage = nullptr;       // f32 staging of a gradient (adjoint with a host output)
     size_t fstage_bytes = 0;
-    unsigned* d_vmax = nullptr;      // per-batch per-view max |q| (float bits)
+    unsigned* d_vmax = nullptr;      // per-batch per-view max |x| or |q| (float bits)
     size_t vmax_len = 0;
     FilterView* d_fv = nullptr;
     size_t fv_len = 0;
@@ -204,8 +206,7 @@ struct Plan {
     void* h_meta = nullptr;          // FilterView[kmax] + ViewGeom[kmax] + int slots[kmax]
     void* d_meta = nullptr;
     size_t meta_bytes = 0;
-    float* d_fstage = nullptr;       // f16 storage: f32 filtered staging [R][kmax][C]
-    unsigned* d_vmax = nullptr;      // f16 storage: per-view max |q| [kmax]
+    unsigned* d_vmax = nullptr;      // f16 storage: per-view max |x| [kmax]
     std::vector<std::pair<int, hipGraphExec_t>> graphs;   // per batch size
     double last_ms = 0.0;
     uint64_t geom_epoch = 0;         // context geometry the plan was made for
@@ -228,12 +229,11 @@ int ensure(void** p, size_t* have, size_t need);   // device buffer growth
 int ensure_host(void** p, size_t* have, size_t need);
 
 // ---------------------------------------------------------------- kernels (launchers)
-// fv, fstage, vmax: the batch's filter views, f32 staging [R][k][C] and per-view max
-// buffer [k] (f16 storage); null selects the context's own (d_fv, d_fstage, d_vmax).
-// A captured graph passes buffers it owns, since the context's may be reallocated.
+// fv, vmax: the batch's filter views and per-view max buffer [k] (f16 storage); null
+// selects the context's own (d_fv, d_vmax).  A captured graph passes buffers it owns,
+// since the context's may be reallocated.
 int launch_filter(Ctx* c, const float* d_block, int k, int n0, hipStream_t s,
-                  const FilterView* fv = nullptr, float* fstage = nullptr,
-                  unsigned* vmax = nullptr);
+                  const FilterView* fv = nullptr, unsigned* vmax = nullptr);
 int launch_store(Ctx* c, const float* d_q, int k, hipStream_t s);
 int launch_preprocess(Ctx* c, float* d_a, float* d_b, int R, int k, int C, double gaussian,
                       double tilt, float** d_res, hipStream_t s);
