@@ -79,6 +79,49 @@ def footprint_pixels(geom, pts, views):
     return float(np.mean(counts))
 
 
+def band_lines_per_load(geom, box, planes, views, max_tiles=64, seed=0):
+    """Mean number of distinct 128-byte lines that one 16-byte band-ring load instruction
+    of the default 16-bit slice kernel touches (64 lanes: 2 point rows x 32 point columns
+    of a 32 x 32 tile, the kernel's lane layout), on a sample of the request's tiles and
+    views: each point's band element (b, c0 + 1) and the next, b = floor((floor(row) + 1)
+    / 3), in the band copy's byte layout (pitch = roundup(C + 3, 64) elements of 8 bytes)."""
+    from minirecast import recon
+    from minirecast.recon import cone_constants
+    K = cone_constants(geom.vectors[views])
+    R, C = geom.rows, geom.cols
+    pitch = (C + 3 + 63) // 64 * 64
+    rng = np.random.default_rng(seed)
+    groups = []
+    for p in planes:
+        T = recon.plane_task(box, p['origin'], p['normal'], p['n'])
+        n = p['n']
+        nt = (n + 31) // 32
+        for tp in range(nt):
+            for ts in range(nt):
+                for w in range(16):                 # 2 workgroups x 4 waves x 2 point slots
+                    sub, wave, m = w >> 3, (w >> 1) & 3, w & 1
+                    pr = tp * 32 + sub * 16 + 2 * wave + 8 * m + np.repeat([0, 1], 32)
+                    sc = ts * 32 + np.tile(np.arange(32), 2)
+                    if pr.max() >= n or sc.max() >= n:
+                        continue
+                    groups.append(T['o'] + pr[:, None] * T['a'] + sc[:, None] * T['b'])
+    groups = np.array(groups)
+    if len(groups) > max_tiles * 16:
+        groups = groups[np.sort(rng.choice(len(groups), max_tiles * 16, replace=False))]
+    counts = []
+    for k in range(len(views)):
+        rel = groups - K['s'][k]
+        L = rel @ K['n'][k]
+        X = K['s'][k] + (K['sdd'][k] / L)[..., None] * rel - K['d'][k]
+        col = np.clip(X @ K['uu'][k] + (C - 1) / 2.0, -1.0, C)
+        row = np.clip(X @ K['vv'][k] + (R - 1) / 2.0, -1.0, R)
+        b = (np.floor(row).astype(np.int64) + 1) // 3
+        off = b * pitch * 8 + (np.floor(col).astype(np.int64) + 1) * 8
+        lines = np.sort(np.concatenate([off // 128, (off + 15) // 128], axis=1), axis=1)
+        counts.append(1 + (np.diff(lines, axis=1) != 0).sum(axis=1))
+    return float(np.mean(np.concatenate(counts)))
+
+
 def _pmc_pass(counter):
     """One rocprofv3 --pmc pass of a short f16 run of this benchmark (child process):
     the counter's mean per dispatch of the f16 cone tile kernel, the dispatches of one
@@ -469,6 +512,27 @@ def bench_filter(args):
           {"filter": res}, higher=False)
 
 
+def bench_l2_gather(args):
+    """The gather microbenchmark behind the back-projection roofline's L2 side: the rate
+    of scattered 16-byte loads (the band-ring element load) when the 64 lanes of each
+    load instruction touch L distinct 128-byte lines, for buffers from L2-resident (1 and
+    4 MiB) through the Infinity Cache (16, 128 MiB) to HBM (1 GiB)."""
+    from minirecast import _hip
+    ctx = _hip.Context(0)
+    sizes = [1 << 20, 1 << 22, 1 << 24, 1 << 27, 1 << 30]
+    lines = [1, 2, 4, 6, 7, 8, 16, 32, 64]
+    table = {}
+    for nb in sizes:
+        table[f"{nb >> 20} MiB"] = {str(L): round(ctx.l2_gather_bandwidth(L, nb)[0], 1)
+                                    for L in lines}
+        log(f"gather GB/s, buffer {nb >> 20} MiB: {table[f'{nb >> 20} MiB']}")
+    _line("GB/s of scattered 16-byte loads from an L2-resident buffer, 7 lines per load",
+          table["1 MiB"]["7"], "GB/s", None, 4, 1,
+          {"workload": "l2_gather: 16-byte buffer loads, L distinct 128-byte lines per "
+                       "64-lane load instruction, 8 loads in flight per thread, "
+                       "8 workgroups of 256 threads per CU"}, {"gather_gbps": table})
+
+
 def bench_diff_cone(args):
     """Differentiable back-projection (256^3-box phantom, 360 views, 256 x 256 detector,
     3 oblique 256^2 slices): forward + backward through the hand-written adjoint,
@@ -559,7 +623,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="full_cone",
                     choices=["full_cone", "full_volume", "stream_cone", "parallel_cor",
-                             "diff_cone", "filter"])
+                             "diff_cone", "filter", "l2_gather"])
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-check", action="store_true", help="skip the float64 oracle error")
@@ -580,7 +644,7 @@ def main():
     if args.workload != "full_cone":
         {"stream_cone": bench_stream_cone, "parallel_cor": bench_parallel_cor,
          "diff_cone": bench_diff_cone, "full_volume": bench_full_volume,
-         "filter": bench_filter}[args.workload](args)
+         "filter": bench_filter, "l2_gather": bench_l2_gather}[args.workload](args)
         return
     from minirecast import _hip, phantom, recon, workloads
     if args.calibrate_copy:
@@ -753,6 +817,23 @@ def main():
                     gather_bytes_l2=int(4 * esz * n_in * A))
     if traffic:
         roofline["traffic_detail"] = traffic
+    # the L2 side: every point-view is one 16-byte band-element load (16-bit forms); the
+    # gather microbenchmark measures how fast the L2s serve such loads with the kernel's
+    # number of distinct lines per load instruction (CPU emulation on a sample)
+    if head in ("f16", "bf16"):
+        lpl = band_lines_per_load(geom, box, w.planes, np.arange(0, A, max(1, A // 6)))
+        lo, hi = int(np.floor(lpl)), int(np.ceil(lpl))
+        g_lo, _ = ctx[head].l2_gather_bandwidth(lo)
+        g_hi, _ = ctx[head].l2_gather_bandwidth(hi) if hi != lo else (g_lo, 0.0)
+        gbps = g_lo + (g_hi - g_lo) * (lpl - lo)
+        load_bytes = 16 * n_in * A
+        t_l2 = load_bytes / (gbps * 1e9)
+        roofline["l2"] = dict(lines_per_load=round(lpl, 2), gather_gbps=round(gbps, 1),
+                              gather_gbps_at=[lo, round(g_lo, 1), hi, round(g_hi, 1)],
+                              load_bytes=int(load_bytes), t_l2_ms=round(t_l2 * 1e3, 4),
+                              t_hbm_ms=round(hbm_bytes / (HBM_PEAK_GBS * 1e9) * 1e3, 4),
+                              t_kernel_ms=round(kern_s * 1e3, 4),
+                              frac_l2=round(t_l2 / kern_s, 4))
     log(f"roofline: {roofline}")
 
     # ---------------------------------------------------------------- CPU baseline

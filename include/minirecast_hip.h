@@ -357,6 +357,16 @@ int mr_last_tile_kernel_ms(mr_ctx* ctx, double* ms);
  * only). */
 int mr_last_tile_kernel_launches(mr_ctx* ctx, int* n);
 
+/* L2 gather microbenchmark (no counterpart in the reference; measurement only): the rate
+ * *gbps (1e9 bytes/s of loaded data) at which the device serves scattered 16-byte loads
+ * from an L2-resident buffer of buffer_bytes (a power of two, 8 KiB..1 GiB), when the 64
+ * lanes of each load instruction touch lines_per_wave (1..64) distinct 128-byte lines;
+ * *ms (optional) the best of three timed runs after a warm-up.  It is the L2 side of the
+ * back-projection roofline: the band-ring kernel issues one such 16-byte load per
+ * point-view. */
+int mr_l2_gather_bandwidth(mr_ctx* ctx, int lines_per_wave, int64_t buffer_bytes, double* gbps,
+                           double* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -644,3 +644,23 @@ def test_served_replies_match_reference_fixtures_and_oracle(golden):
     want = np.asarray(orc.cor_scores(shifts, n=192))
     assert rep["best"] == float(shifts[int(np.argmax(want))])
     assert svc.recon.cor_shift == rep["best"]
+
+
+def test_warmup_runs_the_device_path(golden):
+    """warmup (recon.py:477-487) runs the 4-point back-projection on the device: a no-op
+    before any view arrives, then the same values as reconstruct_points at the origin,
+    which equal the float64 oracle's; there is no CPU backend to fall back to, so the
+    backend stays "hip" (a failure would raise)."""
+    from minirecast import recon
+    from minirecast.geometry import ConeVecGeometry
+    R, C = (int(x) for x in golden['cone_shape'])
+    V = golden['cone_vectors']
+    rec = recon.SliceReconstructor(None, ConeVecGeometry(V, R, C), verbose=False)
+    assert rec.warmup() is None and rec.n == 0
+    rec.add_projections(np.arange(len(V)), golden['cone_proj'])
+    assert rec.warmup() is None and rec.backend == "hip"
+    got = rec.reconstruct_points(np.zeros((4, 3)))
+    orc = O.OracleReconstructor("cone", R, C, _box_cone(V, R, C), vectors=V, store_f32=True)
+    orc.add_projections(np.arange(len(V)), golden['cone_proj'])
+    want = orc.reconstruct_points(np.zeros((4, 3)))
+    np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-7 * np.abs(want).max())
