@@ -21,6 +21,16 @@ int fft_length(int n) {
     return N;
 }
 
+// Widest detector row the filter transforms (FFT length 2^kFilterMaxLogN).
+constexpr int kMaxFilterCols = (1 << kFilterMaxLogN) / 2;
+
+int check_filter_cols(int n) {
+    MR_CHECK(n <= kMaxFilterCols, MR_EVALUE,
+             "detector rows of " + std::to_string(n) + " columns: at most " +
+                 std::to_string(kMaxFilterCols) + " are supported");
+    return MR_OK;
+}
+
 // In-place iterative radix-2 complex FFT in float64 (forward, e^{-2 pi i k n / N}).
 void fft64(std::vector<std::complex<double>>& a) {
     const size_t N = a.size();
@@ -83,7 +93,10 @@ void free_geometry(Ctx* c) {
     if (c->d_vscale) hipFree(c->d_vscale);
     if (c->d_H) hipFree(c->d_H);
     if (c->d_tw) hipFree(c->d_tw);
+    if (c->d_fs) hipFree(c->d_fs);
     c->ring = nullptr; c->d_geom = nullptr; c->d_vscale = nullptr; c->d_H = nullptr; c->d_tw = nullptr;
+    c->d_fs = nullptr;
+    c->fs_bytes = 0;
     if (c->pring) hipFree(c->pring);
     c->pring = nullptr;
     c->pring_bytes = 0;
@@ -102,10 +115,14 @@ void free_geometry(Ctx* c) {
     ++c->views_epoch;
 }
 
-// float32 device tables of the ramp response (tau = 1) and FFT twiddles for length N;
-// l1 (optional) = sum_m |h(m)|, h = the unnormalised inverse DFT of the response (real and
-// even, so a forward DFT gives it), in float64
+// float32 device tables of the ramp response (tau = 1) and FFT twiddles for length N
+// (followed, for the four-step path, by those for length N / 16); l1 (optional) =
+// sum_m |h(m)|, h = the unnormalised inverse DFT of the response (real and even, so a
+// forward DFT gives it), in float64
 int make_tables(int N, int window, float** d_H, float2** d_tw, double* l1 = nullptr) {
+    MR_CHECK(N >= 2 && N <= (1 << kFilterMaxLogN), MR_EVALUE,
+             "rows of at most " + std::to_string((1 << kFilterMaxLogN) / 2) +
+                 " detector columns are supported (FFT length " + std::to_string(N) + ")");
     std::vector<double> H;
     int rc = ramp_response_host(N / 2, 1.0, window, H);
     if (rc) return rc;
@@ -117,16 +134,21 @@ int make_tables(int N, int window, float** d_H, float2** d_tw, double* l1 = null
         *l1 = sum;
     }
     std::vector<float> Hf(N);
-    std::vector<float2> tw(N);
+    const int n_small = N > (1 << kFilterLdsLogN) ? N / 16 : 0;
+    std::vector<float2> tw(N + n_small);
+    auto twiddle = [](int k, int n) {
+        const double ang = -2.0 * M_PI * static_cast<double>(k) / static_cast<double>(n);
+        return make_float2(static_cast<float>(std::cos(ang)), static_cast<float>(std::sin(ang)));
+    };
     for (int k = 0; k < N; ++k) {
         Hf[k] = static_cast<float>(H[k]);
-        const double ang = -2.0 * M_PI * static_cast<double>(k) / static_cast<double>(N);
-        tw[k] = make_float2(static_cast<float>(std::cos(ang)), static_cast<float>(std::sin(ang)));
+        tw[k] = twiddle(k, N);
     }
+    for (int k = 0; k < n_small; ++k) tw[N + k] = twiddle(k, n_small);
     MR_HIP(hipMalloc(reinterpret_cast<void**>(d_H), sizeof(float) * N));
-    MR_HIP(hipMalloc(reinterpret_cast<void**>(d_tw), sizeof(float2) * N));
+    MR_HIP(hipMalloc(reinterpret_cast<void**>(d_tw), sizeof(float2) * tw.size()));
     MR_HIP(hipMemcpy(*d_H, Hf.data(), sizeof(float) * N, hipMemcpyHostToDevice));
-    MR_HIP(hipMemcpy(*d_tw, tw.data(), sizeof(float2) * N, hipMemcpyHostToDevice));
+    MR_HIP(hipMemcpy(*d_tw, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice));
     return MR_OK;
 }
 
@@ -163,6 +185,10 @@ int alloc_ring(Ctx* c, int n_total, int rows, int cols, int window, int storage)
     MR_HIP(hipMemcpy(c->d_vscale, ones.data(), sizeof(float) * n_total, hipMemcpyHostToDevice));
     int rc = make_tables(c->N, window, &c->d_H, &c->d_tw, &c->h_l1);
     if (rc) return rc;
+    // rows longer than one workgroup's LDS: the four-step scratch, fixed for the
+    // geometry's lifetime (captured stream graphs hold its address)
+    c->fs_bytes = fft_scratch_bytes(c->N, (rows + 1) / 2);
+    if (c->fs_bytes) MR_HIP(hipMalloc(reinterpret_cast<void**>(&c->d_fs), c->fs_bytes));
     MR_HIP(hipStreamSynchronize(c->s_ingest));
     c->received.assign(n_total, 0);
     c->n = 0;
@@ -281,6 +307,7 @@ int ramp_response_host(int n, double tau, int window, std::vector<double>& H) {
     if (n < 1) return fail(MR_EVALUE, "ramp_response: n must be >= 1");
     if (!valid_window(window)) return fail(MR_EVALUE, "unknown filter window " + std::to_string(window));
     if (!(tau > 0.0) || !std::isfinite(tau)) return fail(MR_EVALUE, "ramp_response: tau must be > 0");
+    if (n > (1 << 28)) return fail(MR_EVALUE, "ramp_response: n must be <= 2^28");
     const int N = fft_length(n);
     std::vector<std::complex<double>> h(N, 0.0);
     h[0] = 1.0 / (4.0 * tau * tau);
@@ -417,6 +444,8 @@ int mr_set_geometry_cone(mr_ctx* c, int n_total, int rows, int cols, const doubl
     MR_CHECK(vectors != nullptr, MR_EVALUE, "null vectors");
     MR_CHECK(valid_window(window), MR_EVALUE, "unknown filter window " + std::to_string(window));
     MR_CHECK(valid_storage(storage), MR_EVALUE, "unknown storage " + std::to_string(storage));
+    rc = check_filter_cols(cols);
+    if (rc) return rc;
     MR_HIP(hipSetDevice(c->device));
     MR_HIP(hipDeviceSynchronize());
     free_geometry(c);
@@ -482,6 +511,8 @@ int mr_set_geometry_parallel(mr_ctx* c, int n_total, int rows, int cols, const d
     MR_CHECK(pixel_u > 0 && pixel_v > 0, MR_EVALUE, "pixel sizes must be > 0");
     MR_CHECK(valid_window(window), MR_EVALUE, "unknown filter window " + std::to_string(window));
     MR_CHECK(valid_storage(storage), MR_EVALUE, "unknown storage " + std::to_string(storage));
+    rc = check_filter_cols(cols);
+    if (rc) return rc;
     MR_HIP(hipSetDevice(c->device));
     MR_HIP(hipDeviceSynchronize());
     free_geometry(c);
@@ -1630,6 +1661,8 @@ int mr_filter_rows(mr_ctx* c, const float* in, int rows, int n, double tau, int 
     MR_CHECK(tau > 0, MR_EVALUE, "tau must be > 0");
     if (rows == 0) return MR_OK;
     MR_CHECK(in && out, MR_EVALUE, "null input or output");
+    rc = check_filter_cols(n);
+    if (rc) return rc;
     MR_HIP(hipSetDevice(c->device));
     const int N = fft_length(n);
     float* d_H = nullptr;
@@ -1637,13 +1670,16 @@ int mr_filter_rows(mr_ctx* c, const float* in, int rows, int n, double tau, int 
     rc = make_tables(N, window, &d_H, &d_tw);
     if (rc) return rc;
     const size_t bytes = sizeof(float) * static_cast<size_t>(rows) * n;
+    const size_t fs_bytes = fft_scratch_bytes(N, (rows + 1) / 2);
     float *d_in = nullptr, *d_o = nullptr;
+    float2* d_fs = nullptr;
     hipError_t e1 = hipMalloc(reinterpret_cast<void**>(&d_in), bytes);
     hipError_t e2 = hipMalloc(reinterpret_cast<void**>(&d_o), bytes);
-    if (e1 == hipSuccess && e2 == hipSuccess) {
+    hipError_t e3 = fs_bytes ? hipMalloc(reinterpret_cast<void**>(&d_fs), fs_bytes) : hipSuccess;
+    if (e1 == hipSuccess && e2 == hipSuccess && e3 == hipSuccess) {
         hipMemcpy(d_in, in, bytes, hipMemcpyHostToDevice);
         rc = launch_filter_rows_plain(c, d_in, rows, n, static_cast<float>(1.0 / (N * tau)), d_H, d_tw,
-                                      N, d_o, c->s_req);
+                                      N, d_o, d_fs, fs_bytes, c->s_req);
         if (!rc) {
             hipError_t e = hipStreamSynchronize(c->s_req);
             if (e == hipSuccess) e = hipMemcpy(out, d_o, bytes, hipMemcpyDeviceToHost);
@@ -1654,6 +1690,7 @@ int mr_filter_rows(mr_ctx* c, const float* in, int rows, int n, double tau, int 
     }
     if (d_in) hipFree(d_in);
     if (d_o) hipFree(d_o);
+    if (d_fs) hipFree(d_fs);
     hipFree(d_H);
     hipFree(d_tw);
     return rc;

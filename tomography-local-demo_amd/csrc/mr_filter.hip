@@ -25,8 +25,14 @@ namespace mr {
 namespace {
 
 // Threads per workgroup: 256 (one radix-16 butterfly per thread per pass at N = 4096),
-// 512 for N = 8192.
-constexpr int ft_of(int logn) { return logn >= 13 ? 512 : 256; }
+// 512 for N = 8192, 1024 for N = 16384 (128 KB of LDS, one workgroup per CU).
+constexpr int ft_of(int logn) { return logn >= 14 ? 1024 : logn >= 13 ? 512 : 256; }
+
+// Longest transform one workgroup holds in LDS; longer rows (up to 16 times that) take the
+// four-step path.
+constexpr int kLdsMaxLogN = kFilterLdsLogN;
+constexpr int kMaxLogN = kFilterMaxLogN;
+static_assert(kMaxLogN == kLdsMaxLogN + 4, "four-step: 16-point column transforms");
 
 __device__ inline float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -196,44 +202,29 @@ __device__ __forceinline__ void lds_passes(const float2* __restrict__ tw, float2
     }
 }
 
-// MODE 0: ring float32; 1: ring bfloat16; 2: ring float16 with a power-of-two scale per
-// view from the bound max|q| <= max|x| * gain * l1 (vmax: max |x| of the view's raw
-// rows, l1: sum_m |h(m)| of the tau = 1 response), stored in vscale; 3: plain float32
-// rows [R][C] (filter_rows hook); 4: adjoint (filter, then the cosine weight, views
-// [k][R][C] in, blocks [R][k][C] out).
-//
-// Both transforms are forward Stockham FFTs through the LDS buffer; the second runs its
-// passes in the reverse radix order, so that its first pass (radix 16, span 1) reads in
-// each thread exactly the indices j + r N/16 the last pass of the first transform
-// leaves in that thread's registers: the product with H and the conjugation are done
-// there, with no exchange through LDS.
-template <int LOGN, int MODE>
-__global__ __launch_bounds__(ft_of(LOGN))
-__attribute__((amdgpu_waves_per_eu(LOGN >= 12 ? 4 : 5))) void filter_kernel(
-    const float* __restrict__ block, int R, int k, int C, const FilterView* __restrict__ fv,
-    const float* __restrict__ H, const float2* __restrict__ tw, void* __restrict__ dst, int pitch,
-    int64_t view_stride, const unsigned* __restrict__ vmax, float l1,
-    float* __restrict__ vscale) {
-    constexpr int N = 1 << LOGN, FT = ft_of(LOGN);
-    using P1 = FftPlan<LOGN, false>;
-    using P2 = FftPlan<LOGN, true>;
-    constexpr int NP = P1::NPASS;
-    static_assert(P1::radix(NP - 1) == P2::radix(0), "fused passes share the radix");
-    __shared__ float2 lds[N];
-    float2* const buf = lds;                         // captured by the pass lambdas
-    const int r0 = 2 * blockIdx.x, r1 = r0 + 1;
-    const int v = blockIdx.y;
-    const FilterView P = fv[v];
-    const bool has1 = r1 < R;
-    // MODE 4 (adjoint) reads views [k][R][C]; the others read blocks [R][k][C]
-    const float* x0 = MODE == 4 ? block + (static_cast<int64_t>(v) * R + r0) * C
-                                : block + (static_cast<int64_t>(r0) * k + v) * C;
-    const float* x1 = MODE == 4 ? block + (static_cast<int64_t>(v) * R + (has1 ? r1 : r0)) * C
-                                : block + (static_cast<int64_t>(has1 ? r1 : r0) * k + v) * C;
-    const float b0 = P.b0 + r0 * P.db, b1 = P.b0 + r1 * P.db;
-    // transform 1 reads the (cosine-weighted) row pair from global memory; columns
-    // beyond C read as zero, and the upper half (>= N/2 >= C) is not read at all
-    auto in_rows = [&](int c) -> float2 {
+// Input of transform 1: rows r0 and r0 + 1 of view v as the real and imaginary parts (the
+// second is zero past the last row), cone views cosine-weighted except in MODE 4; columns
+// >= C read as zero.  MODE 4 reads views [k][R][C], the others blocks [R][k][C].
+template <int MODE> struct RowsIn {
+    const float* x0;
+    const float* x1;
+    FilterView P;
+    float b0, b1;
+    int C;
+    bool has1;
+    __device__ RowsIn(const float* block, int R, int k, int C_, const FilterView& P_, int r0, int v)
+        : P(P_), C(C_) {
+        const int r1 = r0 + 1;
+        has1 = r1 < R;
+        const int rr = has1 ? r1 : r0;
+        x0 = MODE == 4 ? block + (static_cast<int64_t>(v) * R + r0) * C
+                       : block + (static_cast<int64_t>(r0) * k + v) * C;
+        x1 = MODE == 4 ? block + (static_cast<int64_t>(v) * R + rr) * C
+                       : block + (static_cast<int64_t>(rr) * k + v) * C;
+        b0 = P.b0 + r0 * P.db;
+        b1 = P.b0 + r1 * P.db;
+    }
+    __device__ float2 operator()(int c) const {
         float y0 = 0.0f, y1 = 0.0f;
         if (c < C) {
             y0 = x0[c];
@@ -246,7 +237,145 @@ __attribute__((amdgpu_waves_per_eu(LOGN >= 12 ? 4 : 5))) void filter_kernel(
             }
         }
         return make_float2(y0, y1);
-    };
+    }
+};
+
+// Output of transform 2 (its conjugate, times the gain, is the filtered row pair):
+// MODE 0 ring float32; 1 ring bfloat16; 2 ring float16 with a power-of-two scale per view
+// from the bound max|q| <= max|x| * gain * l1 (vmax: max |x| of the view's raw rows, l1:
+// sum_m |h(m)| of the tau = 1 response), stored in vscale by the thread given `first`;
+// 3 plain float32 rows [R][C]; 4 adjoint (the cosine weight after the self-adjoint
+// filter, blocks [R][k][C]).  Columns >= C are dropped.
+template <int MODE> struct RowsOut {
+    float* o0 = nullptr;
+    float* o1 = nullptr;
+    uint16_t* h0 = nullptr;
+    uint16_t* h1 = nullptr;
+    FilterView P;
+    float b0, b1, g, inv_s = 1.0f;
+    int C;
+    bool has1;
+    __device__ RowsOut(void* dst, int R, int k, int C_, int pitch, int64_t view_stride,
+                       const FilterView& P_, int r0, int v, const unsigned* vmax, float l1,
+                       float* vscale, bool first)
+        : P(P_), C(C_) {
+        const int r1 = r0 + 1;
+        has1 = r1 < R;
+        b0 = P.b0 + r0 * P.db;
+        b1 = P.b0 + r1 * P.db;
+        g = P.gain;
+        if (MODE == 2) {
+            // f16: scale 2^(e - 15) with the bound < 2^e, so |q| / scale < 2^15
+            const float bound = __uint_as_float(vmax[v]) * g * l1;
+            float sc = 1.0f;
+            if (bound > 0.0f && isfinite(bound)) {
+                int ex;
+                frexpf(bound, &ex);
+                sc = ldexpf(1.0f, ex - 15);
+            }
+            inv_s = 1.0f / sc;
+            if (first) vscale[P.slot] = sc;
+        }
+        // uniform row bases (one per row of the pair), indexed by the column
+        if (MODE == 0) {
+            o0 = static_cast<float*>(dst) + static_cast<int64_t>(P.slot) * view_stride +
+                 static_cast<int64_t>(r0 + 1) * pitch + 1;
+            o1 = o0 + pitch;
+        } else if (MODE == 1 || MODE == 2) {
+            h0 = static_cast<uint16_t*>(dst) + static_cast<int64_t>(P.slot) * view_stride +
+                 static_cast<int64_t>(r0 + 1) * pitch + 1;
+            h1 = h0 + pitch;
+        } else if (MODE == 3) {
+            o0 = static_cast<float*>(dst) + static_cast<int64_t>(r0) * C;
+            o1 = o0 + C;
+        } else {
+            o0 = static_cast<float*>(dst) + (static_cast<int64_t>(r0) * k + v) * C;
+            o1 = static_cast<float*>(dst) + (static_cast<int64_t>(has1 ? r1 : r0) * k + v) * C;
+        }
+    }
+    __device__ void operator()(int c, float2 z) const {
+        if (c >= C) return;
+        const float y0 = z.x * g, y1 = -z.y * g;
+        if (MODE == 0 || MODE == 3) {
+            o0[c] = y0;
+            if (has1) o1[c] = y1;
+        } else if (MODE == 1) {
+            auto bf = [](float f) -> uint16_t {
+                uint32_t u = __float_as_uint(f);
+                u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even
+                return static_cast<uint16_t>(u >> 16);
+            };
+            h0[c] = bf(y0);
+            if (has1) h1[c] = bf(y1);
+        } else if (MODE == 2) {
+            h0[c] = __half_as_ushort(__float2half_rn(y0 * inv_s));
+            if (has1) h1[c] = __half_as_ushort(__float2half_rn(y1 * inv_s));
+        } else {
+            float w0 = 1.0f, w1 = 1.0f;
+            if (P.cone) {
+                const float a = P.a0 + c * P.da;
+                const float aa = fmaf(a, a, P.sdd2);
+                w0 = P.sdd * __frsqrt_rn(fmaf(b0, b0, aa));
+                w1 = P.sdd * __frsqrt_rn(fmaf(b1, b1, aa));
+            }
+            o0[c] = y0 * w0;
+            if (has1) o1[c] = y1 * w1;
+        }
+    }
+};
+
+// Four-step transforms (rows longer than one workgroup's LDS holds): N = 16 N2, index
+// n = n1 N2 + n2, frequency k = k1 + 16 k2.  A slot is the N-point buffer of one row pair
+// of one view in the four-step scratch, row k1 of the slot at offset k1 N2.
+//
+// Middle step (MODE 5 of filter_kernel, N2 points): row k1 of a slot holds
+// Y[n2] = W_N^(n2 k1) sum_n1 z[n1 N2 + n2] W_16^(n1 k1); its N2-point transform is
+// Z[k1 + 16 k2]; transform 2 takes conj(Z) H at those indices, and its output at m2 times
+// W_N^(m2 k1) goes to the same row of the output scratch.
+struct SlotIn {
+    const float2* src;
+    __device__ float2 operator()(int i) const { return src[i]; }
+};
+struct SlotOut {
+    float2* dst;
+    const float2* twn;     // N-point twiddles
+    int k1;
+    __device__ void operator()(int m2, float2 z) const { dst[m2] = cmul(z, twn[m2 * k1]); }
+};
+
+// Grid (rows or slots, views): filter_kernel block (x, y) takes row pair x of view y
+// (MODE 0-4) or row x mod 16 of slot x / 16 (MODE 5, y = 0).
+//
+// Both transforms are forward Stockham FFTs through the LDS buffer; the second runs its
+// passes in the reverse radix order, so that its first pass (radix 16, span 1) reads in
+// each thread exactly the indices j + r N/16 the last pass of the first transform
+// leaves in that thread's registers: the product with H and the conjugation are done
+// there, with no exchange through LDS.  MODE 5: tw is the N2-point table, preceded by the
+// 16 N2-point one; H has 16 N2 points.
+template <int LOGN, int MODE>
+__global__ __launch_bounds__(ft_of(LOGN))
+__attribute__((amdgpu_waves_per_eu(LOGN >= 12 ? 4 : 5))) void filter_kernel(
+    const float* __restrict__ block, int R, int k, int C, const FilterView* __restrict__ fv,
+    const float* __restrict__ H, const float2* __restrict__ tw, void* __restrict__ dst, int pitch,
+    int64_t view_stride, const unsigned* __restrict__ vmax, float l1,
+    float* __restrict__ vscale) {
+    constexpr int N = 1 << LOGN, FT = ft_of(LOGN);
+    constexpr bool MID = MODE == 5;
+    using P1 = FftPlan<LOGN, false>;
+    using P2 = FftPlan<LOGN, true>;
+    constexpr int NP = P1::NPASS;
+    static_assert(P1::radix(NP - 1) == P2::radix(0), "fused passes share the radix");
+    static_assert(!MID || NP > 1, "the four-step middle transform uses the LDS passes");
+    __shared__ float2 lds[N];
+    float2* const buf = lds;                         // captured by the pass lambdas
+    const int k1 = MID ? static_cast<int>(blockIdx.x & 15) : 0;
+    const int64_t slot_row = (static_cast<int64_t>(blockIdx.x >> 4) * 16 + k1) * N;
+    auto in_rows = [&] {
+        if constexpr (MID)
+            return SlotIn{reinterpret_cast<const float2*>(block) + slot_row};
+        else
+            return RowsIn<MODE>(block, R, k, C, fv[blockIdx.y], 2 * blockIdx.x, blockIdx.y);
+    }();
     auto to_lds = [buf](int i, float2 z) { buf[sw(i)] = z; };
     constexpr int RF = P2::radix(0);                 // radix of the fused passes
     using PF = Pass<N, RF, FT>;
@@ -255,10 +384,12 @@ __attribute__((amdgpu_waves_per_eu(LOGN >= 12 ? 4 : 5))) void filter_kernel(
         PF::template load_dft<1, true>(tw, in_rows, vf);
     } else {
         {
+            // transform 1 reads the row pair from global memory; its upper half (>= N/2
+            // >= C) is zero padding and not read at all, except in the middle step
             constexpr int R0 = P1::radix(0);
             using P0 = Pass<N, R0, FT>;
             typename P0::Regs v0;
-            P0::template load_dft<1, true>(tw, in_rows, v0);
+            P0::template load_dft<1, !MID>(tw, in_rows, v0);
             P0::template store<1>(to_lds, v0);
             __syncthreads();
             __builtin_amdgcn_sched_barrier(0);
@@ -278,77 +409,21 @@ __attribute__((amdgpu_waves_per_eu(LOGN >= 12 ? 4 : 5))) void filter_kernel(
 #pragma unroll
         for (int r = 0; r < RF; ++r) {
             const float2 z = vf[e][bitrev(r, RF)];
-            const float h = H[j + r * (N / RF)];
+            const float h = MID ? H[k1 + 16 * (j + r * (N / RF))] : H[j + r * (N / RF)];
             u[r] = make_float2(z.x * h, -z.y * h);
         }
         dft_dif<RF>(u);
 #pragma unroll
         for (int r = 0; r < RF; ++r) vf[e][r] = u[r];
     }
-    const float g = P.gain;
-    float inv_s = 1.0f;
-    if (MODE == 2) {
-        // f16: scale 2^(e - 15) with the bound < 2^e, so |q| / scale < 2^15
-        const float bound = __uint_as_float(vmax[v]) * g * l1;
-        float sc = 1.0f;
-        if (bound > 0.0f && isfinite(bound)) {
-            int ex;
-            frexpf(bound, &ex);
-            sc = ldexpf(1.0f, ex - 15);
-        }
-        inv_s = 1.0f / sc;
-        if (blockIdx.x == 0 && threadIdx.x == 0) vscale[P.slot] = sc;
-    }
-    // uniform row bases (one per row of the pair), indexed by the column
-    float* o0 = nullptr;
-    float* o1 = nullptr;
-    uint16_t* h0 = nullptr;
-    uint16_t* h1 = nullptr;
-    if (MODE == 0) {
-        o0 = static_cast<float*>(dst) + static_cast<int64_t>(P.slot) * view_stride +
-             static_cast<int64_t>(r0 + 1) * pitch + 1;
-        o1 = o0 + pitch;
-    } else if (MODE == 1 || MODE == 2) {
-        h0 = static_cast<uint16_t*>(dst) + static_cast<int64_t>(P.slot) * view_stride +
-             static_cast<int64_t>(r0 + 1) * pitch + 1;
-        h1 = h0 + pitch;
-    } else if (MODE == 3) {
-        o0 = static_cast<float*>(dst) + static_cast<int64_t>(r0) * C;
-        o1 = o0 + C;
-    } else {
-        o0 = static_cast<float*>(dst) + (static_cast<int64_t>(r0) * k + v) * C;
-        o1 = static_cast<float*>(dst) + (static_cast<int64_t>(has1 ? r1 : r0) * k + v) * C;
-    }
-    auto out_rows = [&](int c, float2 z) {
-        if (c >= C) return;
-        const float y0 = z.x * g, y1 = -z.y * g;
-        if (MODE == 0 || MODE == 3) {
-            o0[c] = y0;
-            if (has1) o1[c] = y1;
-        } else if (MODE == 1) {
-            auto bf = [](float f) -> uint16_t {
-                uint32_t u = __float_as_uint(f);
-                u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even
-                return static_cast<uint16_t>(u >> 16);
-            };
-            h0[c] = bf(y0);
-            if (has1) h1[c] = bf(y1);
-        } else if (MODE == 2) {
-            h0[c] = __half_as_ushort(__float2half_rn(y0 * inv_s));
-            if (has1) h1[c] = __half_as_ushort(__float2half_rn(y1 * inv_s));
-        } else {
-            // adjoint: the cosine weight after the (self-adjoint) filter, block layout
-            float w0 = 1.0f, w1 = 1.0f;
-            if (P.cone) {
-                const float a = P.a0 + c * P.da;
-                const float aa = fmaf(a, a, P.sdd2);
-                w0 = P.sdd * __frsqrt_rn(fmaf(b0, b0, aa));
-                w1 = P.sdd * __frsqrt_rn(fmaf(b1, b1, aa));
-            }
-            o0[c] = y0 * w0;
-            if (has1) o1[c] = y1 * w1;
-        }
-    };
+    auto out_rows = [&] {
+        if constexpr (MID)
+            return SlotOut{static_cast<float2*>(dst) + slot_row, tw - 16 * N, k1};
+        else
+            return RowsOut<MODE>(dst, R, k, C, pitch, view_stride, fv[blockIdx.y],
+                                 2 * blockIdx.x, blockIdx.y, vmax, l1, vscale,
+                                 blockIdx.x == 0 && threadIdx.x == 0);
+    }();
     if constexpr (NP == 1) {
         PF::template store<1>(out_rows, vf);
     } else {
@@ -363,6 +438,47 @@ __attribute__((amdgpu_waves_per_eu(LOGN >= 12 ? 4 : 5))) void filter_kernel(
         typename PL::Regs vl;
         PL::template load_dft<P2::span(NP - 1), false>(tw, [buf](int i) { return buf[sw(i)]; }, vl);
         PL::template store<P2::span(NP - 1)>(out_rows, vl);
+    }
+}
+
+// First and last steps of the four-step filter (N = 16 n2len), one thread per column of
+// a slot, 16 points in registers.  Block (x, y, z): columns 256 x .., row pair p0 + y of
+// view v0 + z, slot z gridDim.y + y.
+//   OUT = false: the 16-point DFT over n1 of the input rows at columns n1 N2 + n2 (n1 >= 8
+//     is the zero padding: C <= N/2), times W_N^(n2 k1), to row k1 of the slot in fs;
+//   OUT = true: the 16-point DFT over k1 of column m2 of the slot in fs, output column
+//     m1 N2 + m2 of the row pair.
+// tw: the N-point twiddle table.
+template <int MODE, bool OUT>
+__global__ __launch_bounds__(256) void fourstep_cols_kernel(
+    const float* __restrict__ block, int R, int k, int C, const FilterView* __restrict__ fv,
+    const float2* __restrict__ tw, int n2len, float2* __restrict__ fs, void* __restrict__ dst,
+    int pitch, int64_t view_stride, const unsigned* __restrict__ vmax, float l1,
+    float* __restrict__ vscale, int p0, int v0) {
+    const int n2 = blockIdx.x * 256 + threadIdx.x;
+    const int r0 = 2 * (p0 + static_cast<int>(blockIdx.y));
+    const int v = v0 + static_cast<int>(blockIdx.z);
+    float2* slot = fs + (static_cast<int64_t>(blockIdx.z) * gridDim.y + blockIdx.y) * 16 * n2len;
+    float2 x[16];
+    if constexpr (!OUT) {
+        const RowsIn<MODE> in(block, R, k, C, fv[v], r0, v);
+#pragma unroll
+        for (int n1 = 0; n1 < 8; ++n1) x[n1] = in(n1 * n2len + n2);
+        dft_dif_upper_zero<16>(x);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int k1 = bitrev(q, 16);
+            slot[static_cast<int64_t>(k1) * n2len + n2] = k1 ? cmul(x[q], tw[n2 * k1]) : x[q];
+        }
+    } else {
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) x[k1] = slot[static_cast<int64_t>(k1) * n2len + n2];
+        dft_dif<16>(x);
+        const bool first = p0 == 0 && blockIdx.y == 0 && blockIdx.x == 0 && threadIdx.x == 0;
+        const RowsOut<MODE> out(dst, R, k, C, pitch, view_stride, fv[v], r0, v, vmax, l1, vscale,
+                                first);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) out(bitrev(q, 16) * n2len + n2, x[q]);
     }
 }
 
@@ -472,11 +588,64 @@ __global__ void vmax_kernel(const float* __restrict__ q, int R, int C, unsigned*
     if (threadIdx.x == 0) atomicMax(vmax + v, __float_as_uint(red[0]));
 }
 
+// Four-step filter of rows with N = 2^logn > 2^kLdsMaxLogN points: the row pairs of the
+// batch go through the scratch fs (two halves, each a whole number of N-point slots) in
+// chunks of slots: column step, middle transform pair, column step.  tw: the N-point
+// twiddle table followed by the N/16-point one.
+template <int MODE>
+int launch_fourstep(int logn, hipStream_t s, const float* block, int R, int k, int C,
+                    const FilterView* fv, const float* H, const float2* tw, void* dst, int pitch,
+                    int64_t view_stride, const unsigned* vmax, float l1, float* vscale,
+                    float2* fs, size_t fs_bytes) {
+    const int64_t N = int64_t{1} << logn;
+    const int n2len = static_cast<int>(N / 16);
+    const int64_t slots = static_cast<int64_t>(fs_bytes / 2 / (sizeof(float2) * N));
+    MR_CHECK(fs != nullptr && slots >= 1, MR_ERUNTIME, "four-step filter scratch missing");
+    const int npairs = (R + 1) / 2;
+    const int pc = static_cast<int>(std::min<int64_t>(npairs, slots));
+    const int kv = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(k, slots / pc)));
+    float2* fa = fs;
+    float2* fb = fs + slots * N;
+    const float2* tws = tw + N;
+    const int lm = logn - 4;
+    for (int v0 = 0; v0 < k; v0 += kv) {
+        const int nv = std::min(kv, k - v0);
+        for (int p0 = 0; p0 < npairs; p0 += pc) {
+            const int np = std::min(pc, npairs - p0);
+            const dim3 gc(n2len / 256, np, nv);
+            hipLaunchKernelGGL((fourstep_cols_kernel<MODE, false>), gc, dim3(256), 0, s, block, R, k,
+                               C, fv, tw, n2len, fa, dst, pitch, view_stride, vmax, l1, vscale, p0,
+                               v0);
+            const dim3 gm(16 * np * nv, 1);
+            const float* mid_in = reinterpret_cast<const float*>(fa);
+#define MR_MID_CASE(L)                                                                      \
+    case L:                                                                                 \
+        hipLaunchKernelGGL((filter_kernel<L, 5>), gm, dim3(ft_of(L)), 0, s, mid_in, 0, 0, 0, \
+                           fv, H, tws, fb, 0, 0, vmax, 0.0f, vscale);                         \
+        break;
+            switch (lm) {
+                MR_MID_CASE(11) MR_MID_CASE(12) MR_MID_CASE(13) MR_MID_CASE(14)
+            default:
+                return fail(MR_EVALUE, "four-step FFT length 2^" + std::to_string(logn));
+            }
+#undef MR_MID_CASE
+            hipLaunchKernelGGL((fourstep_cols_kernel<MODE, true>), gc, dim3(256), 0, s, block, R, k,
+                               C, fv, tw, n2len, fb, dst, pitch, view_stride, vmax, l1, vscale, p0,
+                               v0);
+        }
+    }
+    MR_HIP(hipGetLastError());
+    return MR_OK;
+}
+
 template <int MODE>
 int launch_mode(int logn, dim3 grid, hipStream_t s, const float* block, int R, int k, int C,
                 const FilterView* fv, const float* H, const float2* tw, void* dst, int pitch,
                 int64_t view_stride, const unsigned* vmax = nullptr, float l1 = 0.0f,
-                float* vscale = nullptr) {
+                float* vscale = nullptr, float2* fs = nullptr, size_t fs_bytes = 0) {
+    if (logn > kLdsMaxLogN && logn <= kMaxLogN)
+        return launch_fourstep<MODE>(logn, s, block, R, k, C, fv, H, tw, dst, pitch, view_stride,
+                                     vmax, l1, vscale, fs, fs_bytes);
 #define MR_FILTER_CASE(L)                                                                  \
     case L:                                                                                \
         hipLaunchKernelGGL((filter_kernel<L, MODE>), grid, dim3(ft_of(L)), 0, s, block, R, k, C, \
@@ -486,9 +655,10 @@ int launch_mode(int logn, dim3 grid, hipStream_t s, const float* block, int R, i
         MR_FILTER_CASE(1) MR_FILTER_CASE(2) MR_FILTER_CASE(3) MR_FILTER_CASE(4) MR_FILTER_CASE(5)
         MR_FILTER_CASE(6) MR_FILTER_CASE(7) MR_FILTER_CASE(8) MR_FILTER_CASE(9)
         MR_FILTER_CASE(10) MR_FILTER_CASE(11) MR_FILTER_CASE(12) MR_FILTER_CASE(13)
+        MR_FILTER_CASE(14)
     default:
         return fail(MR_EVALUE, "FFT length 2^" + std::to_string(logn) +
-                                   " outside the supported range 2..8192");
+                                   " outside the supported range 2..2^" + std::to_string(kMaxLogN));
     }
 #undef MR_FILTER_CASE
     MR_HIP(hipGetLastError());
@@ -512,10 +682,12 @@ int launch_filter(Ctx* c, const float* d_block, int k, int n0, hipStream_t s,
     dim3 grid((c->R + 1) / 2, k);
     if (c->storage == MR_STORE_F32)
         return launch_mode<0>(logn, grid, s, d_block, c->R, k, c->C, fv, c->d_H, c->d_tw,
-                              c->ring, c->pitch, c->view_stride);
+                              c->ring, c->pitch, c->view_stride, nullptr, 0.0f, nullptr, c->d_fs,
+                              c->fs_bytes);
     if (c->storage == MR_STORE_BF16)
         return launch_mode<1>(logn, grid, s, d_block, c->R, k, c->C, fv, c->d_H, c->d_tw,
-                              c->ring, c->pitch, c->view_stride);
+                              c->ring, c->pitch, c->view_stride, nullptr, 0.0f, nullptr, c->d_fs,
+                              c->fs_bytes);
     // F16: the per-view max |x| of the raw rows bounds the filtered view (max |q| <=
     // max |x| gain l1, the FDK weight is at most 1), so its scale is known before the
     // filter writes the halves; the max buffer is cleared with a kernel so that the
@@ -526,7 +698,7 @@ int launch_filter(Ctx* c, const float* d_block, int k, int n0, hipStream_t s,
     MR_HIP(hipGetLastError());
     return launch_mode<2>(logn, grid, s, d_block, c->R, k, c->C, fv, c->d_H, c->d_tw, c->ring,
                           c->pitch, c->view_stride, vmax, static_cast<float>(c->h_l1),
-                          c->d_vscale);
+                          c->d_vscale, c->d_fs, c->fs_bytes);
 }
 
 int launch_store(Ctx* c, const float* d_q, int k, hipStream_t s) {
@@ -551,7 +723,7 @@ int launch_store(Ctx* c, const float* d_q, int k, hipStream_t s) {
 int launch_filter_adjoint(Ctx* c, const float* d_gq, int k, float* d_graw, hipStream_t s) {
     dim3 grid((c->R + 1) / 2, k);
     return launch_mode<4>(log2i(c->N), grid, s, d_gq, c->R, k, c->C, c->d_fv, c->d_H, c->d_tw,
-                          d_graw, 0, 0);
+                          d_graw, 0, 0, nullptr, 0.0f, nullptr, c->d_fs, c->fs_bytes);
 }
 
 int launch_stream_prep(Ctx* c, const void* d_meta, int k, int* d_nviews, hipStream_t s) {
@@ -566,7 +738,7 @@ int launch_stream_prep(Ctx* c, const void* d_meta, int k, int* d_nviews, hipStre
 
 int launch_filter_rows_plain(Ctx* c, const float* d_in, int rows, int n, float gain,
                              const float* d_H, const float2* d_tw, int N, float* d_out,
-                             hipStream_t s) {
+                             float2* d_fs, size_t fs_bytes, hipStream_t s) {
     // one FilterView with no FDK weight: reuse the context's filter-view buffer
     FilterView fv{};
     fv.gain = gain;
@@ -576,7 +748,19 @@ int launch_filter_rows_plain(Ctx* c, const float* d_in, int rows, int n, float g
     MR_HIP(hipStreamSynchronize(s));
     MR_HIP(hipMemcpy(c->d_fv, &fv, sizeof(fv), hipMemcpyHostToDevice));
     dim3 grid((rows + 1) / 2, 1);
-    return launch_mode<3>(log2i(N), grid, s, d_in, rows, 1, n, c->d_fv, d_H, d_tw, d_out, 0, 0);
+    return launch_mode<3>(log2i(N), grid, s, d_in, rows, 1, n, c->d_fv, d_H, d_tw, d_out, 0, 0,
+                          nullptr, 0.0f, nullptr, d_fs, fs_bytes);
+}
+
+// Two halves of whole N-point slots: a slot per row pair, for up to 16 views of a batch,
+// at most 256 MiB a half (at least one slot).
+size_t fft_scratch_bytes(int N, int pairs) {
+    if (N <= (1 << kLdsMaxLogN)) return 0;
+    const size_t slot = sizeof(float2) * static_cast<size_t>(N);
+    size_t half = slot * static_cast<size_t>(std::max(pairs, 1)) * 16;
+    half = std::min(half, size_t{256} << 20);
+    half = std::max(slot, half / slot * slot);
+    return 2 * half;
 }
 
 }  // namespace mr

@@ -159,7 +159,10 @@ struct Ctx {
     float* d_H = nullptr;            // [N] ramp response for tau = 1 (windowed)
     double h_l1 = 0.0;               // sum_m |h(m)| of that response's unnormalised inverse
                                      // DFT (the f16 scale bound of the filter)
-    float2* d_tw = nullptr;          // [N] twiddles exp(-2 pi i k / N)
+    float2* d_tw = nullptr;          // [N] twiddles exp(-2 pi i k / N), then, when N >
+                                     // 2^kFilterLdsLogN, [N / 16] for N / 16
+    float2* d_fs = nullptr;          // four-step filter scratch (N > 2^kFilterLdsLogN)
+    size_t fs_bytes = 0;
     float* d_stage = nullptr;        // raw block staging
     size_t stage_bytes = 0;
     float* d_pre = nullptr;          // preprocessing ping-pong buffer
@@ -237,9 +240,16 @@ int launch_filter(Ctx* c, const float* d_block, int k, int n0, hipStream_t s,
 int launch_store(Ctx* c, const float* d_q, int k, hipStream_t s);
 int launch_preprocess(Ctx* c, float* d_a, float* d_b, int R, int k, int C, double gaussian,
                       double tilt, float** d_res, hipStream_t s);
+// d_fs, fs_bytes: the four-step scratch (fft_scratch_bytes) when N > 2^14, else unused
 int launch_filter_rows_plain(Ctx* c, const float* d_in, int rows, int n, float gain_tau,
                              const float* d_H, const float2* d_tw, int N, float* d_out,
-                             hipStream_t s);
+                             float2* d_fs, size_t fs_bytes, hipStream_t s);
+// FFT lengths: one workgroup's LDS holds 2^kFilterLdsLogN points; longer rows, up to
+// 2^kFilterMaxLogN, take the four-step path through a device scratch of
+// fft_scratch_bytes(N, pairs) bytes (0 when not needed)
+constexpr int kFilterLdsLogN = 14;
+constexpr int kFilterMaxLogN = 18;
+size_t fft_scratch_bytes(int N, int pairs);
 int launch_bp_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles,
                     int n_views, float scale, float* d_out, hipStream_t s,
                     const TileQueue* q = nullptr);
