@@ -439,6 +439,8 @@ int mr_set_geometry_cone(mr_ctx* c, int n_total, int rows, int cols, const doubl
                          int window, int storage) {
     int rc = check_ctx(c);
     if (rc) return rc;
+    // geometry and view-count changes exclude ingest and requests (ingest lock first)
+    std::lock_guard<std::mutex> li(c->ingest_mu);
     std::lock_guard<std::mutex> lk(c->mu);
     MR_CHECK(n_total >= 1 && rows >= 1 && cols >= 1, MR_EVALUE, "empty geometry");
     MR_CHECK(vectors != nullptr, MR_EVALUE, "null vectors");
@@ -505,6 +507,8 @@ int mr_set_geometry_parallel(mr_ctx* c, int n_total, int rows, int cols, const d
                              double pixel_u, double pixel_v, int window, int storage) {
     int rc = check_ctx(c);
     if (rc) return rc;
+    // geometry and view-count changes exclude ingest and requests (ingest lock first)
+    std::lock_guard<std::mutex> li(c->ingest_mu);
     std::lock_guard<std::mutex> lk(c->mu);
     MR_CHECK(n_total >= 1 && rows >= 1 && cols >= 1, MR_EVALUE, "empty geometry");
     MR_CHECK(angles != nullptr, MR_EVALUE, "null angles");
@@ -555,7 +559,9 @@ int mr_set_geometry_parallel(mr_ctx* c, int n_total, int rows, int cols, const d
     return MR_OK;
 }
 
-// Caller holds c->mu.  in_stage: the block is already in c->d_stage (device).
+// Caller holds c->ingest_mu (not c->mu): the batch is staged and filtered into the free
+// slots [n, n + k) of the ring and d_geom, which no request reads, and only the commit of
+// the view count takes c->mu.  in_stage: the block is already in c->d_stage (device).
 static int append_views_locked(mr_ctx* c, const int64_t* indices, int k, const float* block,
                                int on_device, int* n_out, bool filter, bool in_stage) {
     int rc = MR_OK;
@@ -614,6 +620,8 @@ static int append_views_locked(mr_ctx* c, const int64_t* indices, int k, const f
     rc = filter ? launch_filter(c, d_block, k, c->n, c->s_ingest) : launch_store(c, d_block, k, c->s_ingest);
     if (rc) return rc;
     MR_HIP(hipStreamSynchronize(c->s_ingest));
+    // the new views become visible to requests
+    std::lock_guard<std::mutex> lk(c->mu);
     c->n += k;
     if (n_out) *n_out = c->n;
     return MR_OK;
@@ -623,7 +631,7 @@ static int append_views(mr_ctx* c, const int64_t* indices, int k, const float* b
                         int on_device, int* n_out, bool filter) {
     int rc = check_ctx(c);
     if (rc) return rc;
-    std::lock_guard<std::mutex> lk(c->mu);
+    std::lock_guard<std::mutex> li(c->ingest_mu);
     return append_views_locked(c, indices, k, block, on_device, n_out, filter, false);
 }
 
@@ -636,7 +644,7 @@ int mr_set_raw_correction(mr_ctx* c, const float* dark, const float* scale, int 
                           int frame_cols, int sample, int flip, int row0, int col0) {
     int rc = check_ctx(c);
     if (rc) return rc;
-    std::lock_guard<std::mutex> lk(c->mu);
+    std::lock_guard<std::mutex> li(c->ingest_mu);
     MR_CHECK(c->kind != MR_KIND_NONE, MR_ERUNTIME, "no geometry set");
     MR_CHECK(dark && scale, MR_EVALUE, "null dark or scale");
     MR_CHECK(op == 0 || op == 1, MR_EVALUE, "op must be 0 (multiply) or 1 (divide)");
@@ -668,7 +676,7 @@ int mr_push_raw(mr_ctx* c, const int64_t* indices, int k, const uint16_t* frames
                 int frames_on_device, int* n_out) {
     int rc = check_ctx(c);
     if (rc) return rc;
-    std::lock_guard<std::mutex> lk(c->mu);
+    std::lock_guard<std::mutex> li(c->ingest_mu);
     MR_CHECK(c->raw.set, MR_ERUNTIME, "no raw correction set (mr_set_raw_correction)");
     MR_CHECK(k >= 0, MR_EVALUE, "negative view count");
     if (k == 0) {
@@ -705,7 +713,7 @@ int mr_set_preprocess(mr_ctx* c, double gaussian, double tilt) {
     int rc = check_ctx(c);
     if (rc) return rc;
     MR_CHECK(std::isfinite(gaussian) && std::isfinite(tilt), MR_EVALUE, "non-finite preprocessing");
-    std::lock_guard<std::mutex> lk(c->mu);
+    std::lock_guard<std::mutex> li(c->ingest_mu);
     c->gaussian = gaussian > 0.0 ? gaussian : 0.0;
     c->tilt = tilt;
     return MR_OK;
@@ -720,7 +728,7 @@ int mr_preprocess(mr_ctx* c, const float* in, int rows, int k, int cols, double 
     const size_t blk = sizeof(float) * static_cast<size_t>(rows) * k * cols;
     if (blk == 0) return MR_OK;
     MR_CHECK(in && out, MR_EVALUE, "null block");
-    std::lock_guard<std::mutex> lk(c->mu);
+    std::lock_guard<std::mutex> li(c->ingest_mu);
     MR_HIP(hipSetDevice(c->device));
     rc = ensure(reinterpret_cast<void**>(&c->d_stage), &c->stage_bytes, blk);
     if (rc) return rc;
@@ -1190,7 +1198,7 @@ int mr_filter_adjoint(mr_ctx* c, const int64_t* indices, int k, const float* gq,
                       float* graw, int graw_on_device) {
     int rc = check_ctx(c);
     if (rc) return rc;
-    std::lock_guard<std::mutex> lk(c->mu);
+    std::lock_guard<std::mutex> li(c->ingest_mu);
     MR_CHECK(c->kind != MR_KIND_NONE, MR_ERUNTIME, "no geometry set");
     MR_CHECK(k >= 0, MR_EVALUE, "negative view count");
     if (k == 0) return MR_OK;
@@ -1478,6 +1486,8 @@ int mr_plan_update(mr_plan* p, float* out_host) {
 int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block, float* out_host) {
     MR_CHECK(p != nullptr, MR_EVALUE, "null plan");
     Ctx* c = p->c;
+    // a streaming update filters (ingest buffers) and back-projects: both locks
+    std::lock_guard<std::mutex> li(c->ingest_mu);
     std::lock_guard<std::mutex> lk(c->mu);
     MR_CHECK(p->geom_epoch == c->geom_epoch, MR_ERUNTIME, "the plan was made for an earlier geometry");
     MR_CHECK(k >= 1 && indices && block, MR_EVALUE, "empty or null batch");
@@ -1601,6 +1611,8 @@ int mr_plan_last_ms(mr_plan* p, double* ms) {
 int mr_reset_views(mr_ctx* c) {
     int rc = check_ctx(c);
     if (rc) return rc;
+    // geometry and view-count changes exclude ingest and requests (ingest lock first)
+    std::lock_guard<std::mutex> li(c->ingest_mu);
     std::lock_guard<std::mutex> lk(c->mu);
     c->n = 0;
     c->pring_n = 0;
@@ -1655,7 +1667,7 @@ int mr_cor_scores(mr_ctx* c, const mr_task* tasks, int ntasks, int n_views, doub
 int mr_filter_rows(mr_ctx* c, const float* in, int rows, int n, double tau, int window, float* out) {
     int rc = check_ctx(c);
     if (rc) return rc;
-    std::lock_guard<std::mutex> lk(c->mu);
+    std::lock_guard<std::mutex> li(c->ingest_mu);
     MR_CHECK(rows >= 0 && n >= 1, MR_EVALUE, "bad row shape");
     MR_CHECK(valid_window(window), MR_EVALUE, "unknown filter window " + std::to_string(window));
     MR_CHECK(tau > 0, MR_EVALUE, "tau must be > 0");
@@ -1677,12 +1689,15 @@ int mr_filter_rows(mr_ctx* c, const float* in, int rows, int n, double tau, int 
     hipError_t e2 = hipMalloc(reinterpret_cast<void**>(&d_o), bytes);
     hipError_t e3 = fs_bytes ? hipMalloc(reinterpret_cast<void**>(&d_fs), fs_bytes) : hipSuccess;
     if (e1 == hipSuccess && e2 == hipSuccess && e3 == hipSuccess) {
-        hipMemcpy(d_in, in, bytes, hipMemcpyHostToDevice);
-        rc = launch_filter_rows_plain(c, d_in, rows, n, static_cast<float>(1.0 / (N * tau)), d_H, d_tw,
-                                      N, d_o, d_fs, fs_bytes, c->s_req);
+        // an ingest-side operation (the context's filter-view buffer): the ingest stream
+        hipError_t e = hipMemcpyAsync(d_in, in, bytes, hipMemcpyHostToDevice, c->s_ingest);
+        if (e == hipSuccess)
+            rc = launch_filter_rows_plain(c, d_in, rows, n, static_cast<float>(1.0 / (N * tau)), d_H,
+                                          d_tw, N, d_o, d_fs, fs_bytes, c->s_ingest);
         if (!rc) {
-            hipError_t e = hipStreamSynchronize(c->s_req);
-            if (e == hipSuccess) e = hipMemcpy(out, d_o, bytes, hipMemcpyDeviceToHost);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(out, d_o, bytes, hipMemcpyDeviceToHost, c->s_ingest);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->s_ingest);
             if (e != hipSuccess) rc = fail(MR_ERUNTIME, std::string("filter_rows: ") + hipGetErrorString(e));
         }
     } else {

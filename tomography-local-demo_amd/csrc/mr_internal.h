@@ -90,8 +90,15 @@ struct RawCorr {
     float* d_scale = nullptr;        // [R][C]
 };
 
+// Locks: ingest_mu serialises ingest (staging buffers d_stage, d_pre, d_raw, d_fv,
+// d_vmax, d_fs, the raw correction, the preprocessing parameters and s_ingest); mu guards
+// the stored-view count n, the ring copies, the request buffers and s_req.  An ingest
+// writes only ring and d_geom slots >= n, which no request reads, and takes mu just to
+// commit n; a request runs while a batch is being filtered.  Changes of the geometry or
+// of n take both, ingest_mu first.
 struct Ctx {
     int device = 0;
+    std::mutex ingest_mu;
     std::mutex mu;
     hipStream_t s_ingest = nullptr, s_req = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
