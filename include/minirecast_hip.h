@@ -215,6 +215,22 @@ int mr_clip_histogram_static(mr_ctx* ctx, const float* vol_device, int64_t n, do
 int mr_set_bp_config(mr_ctx* ctx, int kernel, int ppt, int layout, int depth);
 int mr_get_bp_config(mr_ctx* ctx, int* cfg);
 
+/* ---- device memory of a context (no counterpart in the reference, whose projections
+ * live in host memory)
+ *
+ * Kernels 6 and 7 read copies of the ring built on first use: the row-pair copy (twice
+ * the ring) and, for 16-bit storage, the band copy (4/3 of the ring).  They are caches:
+ * a copy is built only if the device keeps MINIRECAST_COPY_RESERVE_MB (default 1024) MiB
+ * free after it and both copies stay within the budget set here (bytes < 0: no budget,
+ * the default).  When freeing the copy a request does not read makes room, that copy is
+ * freed first; when nothing does, the request runs without the copy (kernel 6, then 7,
+ * then 4 on the ring itself; the work-queue loop on the ring), with the same results to
+ * float32 rounding.  Setting a budget frees the copies beyond it at once.
+ * mr_device_bytes: out[4] = {ring, band copy, row-pair copy, filter scratch} bytes held
+ * on the device now. */
+int mr_set_ring_copy_budget(mr_ctx* ctx, int64_t bytes);
+int mr_device_bytes(mr_ctx* ctx, int64_t* out);
+
 /* Autotuner: times every configuration on an evenly spaced sample of at most
  * max_tiles 32 x 32 tiles of `tasks` (reps timed runs each, after one warm-up, against
  * the first n_views stored views), keeps the fastest for this context and returns it

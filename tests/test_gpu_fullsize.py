@@ -6,8 +6,8 @@ Checked: the float64 oracle on 3 x 256 sampled in-box pixels (f32 <= 1e-5, f16 <
 bf16 <= 1e-3 relative L2) and on 512 sampled voxels of the 768 x 768 x 607 volume over
 the box (reconstruct_volume(maxdim=768), recon.py:442-470), stream == batch (interleaved arrival in batches of 40), exact linearity
 of the f32 path (projections x 2 give outputs x 2 bit for bit: scaling by 2 is exact in
-binary floating point), and the default f16 kernel (band ring) against the plain v1
-kernel on every output.
+binary floating point), and the default f16 kernel (band ring) against kernel 2 (the
+work-queue loop on the ring itself) on every output.
 """
 import numpy as np
 import pytest
@@ -115,7 +115,7 @@ def test_full_size_default_f16_kernel_vs_plain_kernel(full):
     ctx = full["ctx"]["f16"]
     cfg = ctx.bp_config()
     assert cfg["kernel"] == 7                 # the default for 16-bit storage
-    ctx.set_bp_config(kernel=1, ppt=4, layout=0, depth=2)
+    ctx.set_bp_config(kernel=2, ppt=4, layout=0, depth=2)
     try:
         plain = ctx.bp_tasks(full["T"], full["w"].geom.n_angles, full["out_len"])
     finally:

@@ -360,9 +360,9 @@ def test_storage_precision_vs_float64_oracle(small_cone):
     assert errs["f32"] < errs["f16"] < errs["bf16"]
 
 
-@pytest.mark.parametrize("variant", ["1", "2", "3", "4", "5", "6", "7"])
+@pytest.mark.parametrize("variant", ["2", "4", "5", "6", "7"])
 def test_bp_kernel_variants_vs_oracle(small_cone, variant, monkeypatch):
-    """Both tile-kernel inner loops (MINIRECAST_BP_VARIANT) against the float64 oracle,
+    """Every tile kernel (MINIRECAST_BP_VARIANT) against the float64 oracle,
     for every storage form, cone beam; and parallel beam on the reference fixture."""
     from minirecast import recon
     monkeypatch.setenv("MINIRECAST_BP_VARIANT", variant)
@@ -376,7 +376,7 @@ def test_bp_kernel_variants_vs_oracle(small_cone, variant, monkeypatch):
         assert rel_l2(img, ref) < tol, (variant, storage, rel_l2(img, ref))
 
 
-@pytest.mark.parametrize("variant", ["1", "2", "3", "4", "5", "6", "7"])
+@pytest.mark.parametrize("variant", ["2", "4", "5", "6", "7"])
 def test_bp_kernel_variants_parallel(golden, variant, monkeypatch):
     from minirecast import recon
     from minirecast.geometry import ParallelGeometry

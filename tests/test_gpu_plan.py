@@ -96,7 +96,7 @@ def test_plan_stream_graph_replays(workload, storage, monkeypatch):
     ("7", "2", "0", "2"), ("7", "2", "0", "3"), ("7", "1", "1", "3"), ("7", "4", "0", "2")])
 def test_bp_variants_full_check(workload, variant, ppt, layout, depth, monkeypatch):
     """Every tile-kernel configuration (kernel, points per thread, lane layout, views in
-    flight) against the plain v1 kernel on all outputs, for view counts that leave
+    flight) against kernel 2 on the f32 ring on all outputs, for view counts that leave
     every pipeline tail length (n, n - 1, 2, 1)."""
     from minirecast import _hip
     monkeypatch.setenv("MINIRECAST_BP_PPT", ppt)
@@ -105,7 +105,7 @@ def test_bp_variants_full_check(workload, variant, ppt, layout, depth, monkeypat
     w, proj, planes, tasks, out_len = workload
     T = np.array(tasks, dtype=_hip.TASK_DTYPE)
     A = w.geom.n_angles
-    ref_ctx = _ctx(w, "f32", "1", monkeypatch)
+    ref_ctx = _ctx(w, "f32", "2", monkeypatch)
     ref_ctx.push(np.arange(A), proj)
     refs = {nv: ref_ctx.bp_tasks(T, nv, out_len) for nv in (A, A - 1, 2, 1)}
     for storage, tol in (("f32", 1e-6), ("f16", 1e-3), ("bf16", 5e-3)):
@@ -133,14 +133,14 @@ def test_bp_variants_full_check(workload, variant, ppt, layout, depth, monkeypat
 
 def test_autotune_picks_a_configuration_and_keeps_parity(workload):
     """The autotuner times the configurations, keeps one, and the tuned context still
-    matches the plain v1 kernel; configurations are validated."""
+    matches kernel 2 on the ring; configurations are validated."""
     from minirecast import _hip, recon
     w, proj, planes, tasks, out_len = workload
     T = np.array(tasks, dtype=_hip.TASK_DTYPE)
     A = w.geom.n_angles
     ref_ctx = _hip.Context(0)
     ref_ctx.set_cone(w.geom.vectors, w.geom.rows, w.geom.cols, w.window)
-    ref_ctx.set_bp_config(kernel=1, ppt=4, layout=0, depth=2)
+    ref_ctx.set_bp_config(kernel=2, ppt=4, layout=0, depth=2)
     ref_ctx.push(np.arange(A), proj)
     ref = ref_ctx.bp_tasks(T, A, out_len)
     rec = recon.SliceReconstructor(proj, w.geom, window=w.window, verbose=False)
@@ -188,7 +188,7 @@ def test_pair_ring_follows_appends_and_resets(workload, variant, monkeypatch):
     w, proj, planes, tasks, out_len = workload
     T = np.array(tasks, dtype=_hip.TASK_DTYPE)
     A = w.geom.n_angles
-    ref_ctx = _ctx(w, "f16", "1", monkeypatch)
+    ref_ctx = _ctx(w, "f16", "2", monkeypatch)
     ctx = _ctx(w, "f16", variant, monkeypatch)
     h = A // 2
     for c in (ref_ctx, ctx):
@@ -206,7 +206,7 @@ def test_pair_ring_follows_appends_and_resets(workload, variant, monkeypatch):
 
 @pytest.mark.parametrize("rows", [1, 2, 4, 37, 38, 39])
 def test_band_ring_rows_and_borders(rows, monkeypatch):
-    """Kernel 7 (band ring) against kernel 1 on the same 16-bit ring, for detector heights
+    """Kernel 7 (band ring) against kernel 2 on the same 16-bit ring, for detector heights
     of every residue mod 3 (down to single-row detectors), cone and parallel beam: oblique
     slices, and a volume grid over 1.6x the box across and 4x along z whose outer points
     project off the detector (the zero border and clamps)."""
@@ -235,7 +235,7 @@ def test_band_ring_rows_and_borders(rows, monkeypatch):
         T = np.array(tasks, dtype=_hip.TASK_DTYPE)
         for storage in ("f16", "bf16"):
             outs = []
-            for variant in ("1", "7"):
+            for variant in ("2", "7"):
                 monkeypatch.setenv("MINIRECAST_BP_VARIANT", variant)
                 monkeypatch.setenv("MINIRECAST_BP_PPT", "2")
                 ctx = _hip.Context(0)
@@ -254,7 +254,7 @@ def test_band_ring_rows_and_borders(rows, monkeypatch):
 
 def test_autotune_views_per_pass(workload):
     """With every tile timed and a grid that fills the device, the autotuner also picks
-    the views per launch; the tuned context still matches the plain v1 kernel, and the
+    the views per launch; the tuned context still matches kernel 2 on the ring, and the
     setting is validated and reported."""
     from minirecast import _hip, recon
     w, proj, planes, tasks, out_len = workload
@@ -269,7 +269,7 @@ def test_autotune_views_per_pass(workload):
     T = np.array(T, dtype=_hip.TASK_DTYPE)
     ref_ctx = _hip.Context(0)
     ref_ctx.set_cone(w.geom.vectors, w.geom.rows, w.geom.cols, w.window, "f16")
-    ref_ctx.set_bp_config(kernel=1, ppt=4, layout=0, depth=2)
+    ref_ctx.set_bp_config(kernel=2, ppt=4, layout=0, depth=2)
     ref_ctx.push(np.arange(A), proj)
     ref = ref_ctx.bp_tasks(T, A, off)
     ctx = _hip.Context(0)
@@ -558,7 +558,7 @@ def test_oversized_slice_requests(workload, storage, beam, monkeypatch):
     """Slice requests whose records exceed the records buffer run the v2 loop in the
     default configuration (on the band copy for 16-bit rings, the row-pair copy for f32
     parallel beam), and the chunked records path in an explicit kernel-4/7 configuration;
-    both equal kernel 1.  The buffer is shrunk to 64 KB so that these small slices need
+    both equal kernel 2.  The buffer is shrunk to 64 KB so that these small slices need
     several chunks."""
     from minirecast import _hip, phantom, recon
     from minirecast.geometry import ParallelGeometry
@@ -581,7 +581,7 @@ def test_oversized_slice_requests(workload, storage, beam, monkeypatch):
     monkeypatch.setenv("MINIRECAST_BP_RECORD_KB", "64")
     outs, launches = {}, {}
     for name, variant in (("default", None), ("explicit", "7" if storage != "f32" else "4"),
-                          ("ref", "1")):
+                          ("ref", "2")):
         if variant is None:
             monkeypatch.delenv("MINIRECAST_BP_VARIANT", raising=False)
         else:

@@ -105,6 +105,8 @@ void free_geometry(Ctx* c) {
     c->bring = nullptr;
     c->bring_bytes = 0;
     c->bring_n = 0;
+    c->band_no_room = c->pair_no_room = false;
+    ++c->copies_epoch;
     // the raw-count correction is per detector frame: a new geometry drops it
     if (c->raw.d_dark) hipFree(c->raw.d_dark);
     if (c->raw.d_scale) hipFree(c->raw.d_scale);
@@ -378,7 +380,7 @@ int mr_ctx_create(int device, mr_ctx** out) {
                      std::getenv("MINIRECAST_BP_SYNC");
     if (const char* v = std::getenv("MINIRECAST_BP_VARIANT")) {
         const int k = std::atoi(v);
-        c->bp_variant = (k >= 1 && k <= 7) ? k : c->bp_variant;
+        c->bp_variant = (k == 2 || (k >= 4 && k <= 7)) ? k : c->bp_variant;
     }
     if (const char* v = std::getenv("MINIRECAST_BP_PPT")) {
         const int k = std::atoi(v);
@@ -395,6 +397,10 @@ int mr_ctx_create(int device, mr_ctx** out) {
     if (const char* v = std::getenv("MINIRECAST_BP_RECORD_KB")) {
         const long kb = std::atol(v);
         if (kb >= 1) c->rec_budget = static_cast<size_t>(kb) << 10;
+    }
+    if (const char* v = std::getenv("MINIRECAST_COPY_RESERVE_MB")) {
+        const long long mb = std::atoll(v);
+        if (mb >= 0) c->copy_reserve = static_cast<size_t>(mb) << 20;
     }
     if (const char* v = std::getenv("MINIRECAST_XCD_MAP")) {
         const int k = std::atoi(v);
@@ -826,8 +832,10 @@ const BpConfig kBpCandidates[] = {
     {2, 4, 0, 2}, {6, 2, 0, 2}, {6, 2, 0, 3}, {6, 1, 0, 3}, {6, 4, 0, 2}, {7, 2, 0, 2},
     {7, 1, 0, 2}, {7, 2, 0, 3}};
 
+bool valid_kernel(int variant) { return variant == 2 || (variant >= 4 && variant <= 7); }
+
 bool valid_config(int variant, int ppt, int layout, int depth) {
-    return (variant >= 1 && variant <= 7) && (ppt == 1 || ppt == 2 || ppt == 4) &&
+    return valid_kernel(variant) && (ppt == 1 || ppt == 2 || ppt == 4) &&
            (layout == 0 || layout == 1) && (depth == 2 || depth == 3);
 }
 }  // namespace
@@ -997,6 +1005,31 @@ int mr_set_bp_config(mr_ctx* c, int variant, int ppt, int layout, int depth) {
     c->bp_layout = layout;
     c->bp_depth = depth;
     c->bp_explicit = true;
+    return MR_OK;
+}
+
+int mr_set_ring_copy_budget(mr_ctx* c, int64_t bytes) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    MR_HIP(hipSetDevice(c->device));
+    c->copy_budget = bytes < 0 ? -1 : bytes;
+    trim_ring_copies(c, c->s_req);
+    return MR_OK;
+}
+
+int mr_device_bytes(mr_ctx* c, int64_t* out) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    MR_CHECK(out != nullptr, MR_EVALUE, "null output");
+    std::lock_guard<std::mutex> lk(c->mu);
+    const size_t ring = c->ring ? static_cast<size_t>(c->n_total) * c->view_stride *
+                                      (c->storage == MR_STORE_F32 ? 4 : 2)
+                                : 0;
+    out[0] = static_cast<int64_t>(ring);
+    out[1] = c->bring ? static_cast<int64_t>(c->bring_bytes) : 0;
+    out[2] = c->pring ? static_cast<int64_t>(c->pring_bytes) : 0;
+    out[3] = static_cast<int64_t>(c->fs_bytes);
     return MR_OK;
 }
 
@@ -1260,8 +1293,6 @@ int mr_plan_create(mr_ctx* c, const mr_task* tasks, int ntasks, int64_t out_len,
     MR_CHECK(c->kind != MR_KIND_NONE, MR_ERUNTIME, "no geometry set");
     MR_CHECK(ntasks >= 1 && tasks, MR_EVALUE, "a plan needs at least one task");
     MR_CHECK(out_len >= 1, MR_EVALUE, "a plan needs an output");
-    MR_CHECK(c->bp_variant == 2 || c->bp_variant >= 4, MR_ERUNTIME,
-             "plans run the persistent v2 back-projection kernel");
     rc = validate_tasks(tasks, ntasks, out_len);
     if (rc) return rc;
     MR_HIP(hipSetDevice(c->device));
@@ -1392,7 +1423,8 @@ int mr_plan_cor_scores(mr_plan* p, int n_views, double* scores) {
     int rc0 = prepare_persistent_bp(c, n_views, c->s_req);
     if (rc0) return rc0;
     const int rk = v2_ring_kind(c);
-    if (p->cor_exec && p->cor_ring != rk) {      // configuration changed since the capture
+    if (p->cor_exec && (p->cor_ring != rk || p->cor_copies != c->copies_epoch)) {
+        // configuration changed or a ring copy was dropped since the capture
         hipGraphExecDestroy(p->cor_exec);
         p->cor_exec = nullptr;
     }
@@ -1401,6 +1433,7 @@ int mr_plan_cor_scores(mr_plan* p, int n_views, double* scores) {
     MR_HIP(hipStreamSynchronize(c->s_req));
     if (!p->cor_exec) {
         p->cor_ring = rk;
+        p->cor_copies = c->copies_epoch;
         TileQueue q;
         q.head = p->d_counters + 1;
         q.count = p->d_counters;
@@ -1537,7 +1570,8 @@ int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block
     rc = prepare_persistent_bp(c, c->n, c->s_req);
     if (rc) return rc;
     const int rk = v2_ring_kind(c);
-    if (!p->graphs.empty() && p->graphs_ring != rk) {     // configuration changed
+    if (!p->graphs.empty() && (p->graphs_ring != rk || p->graphs_copies != c->copies_epoch)) {
+        // configuration changed or a ring copy was dropped since the capture
         for (auto& g : p->graphs) hipGraphExecDestroy(g.second);
         p->graphs.clear();
     }
@@ -1576,8 +1610,11 @@ int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block
         hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         hipGraphDestroy(graph);
         if (ei != hipSuccess) return fail(MR_ERUNTIME, std::string("graph instantiate: ") + hipGetErrorString(ei));
+        if (p->graphs.empty()) {
+            p->graphs_ring = rk;
+            p->graphs_copies = c->copies_epoch;
+        }
         p->graphs.emplace_back(k, exec);
-        p->graphs_ring = rk;
     }
     p->acc_n = -1;                     // until the replay is known to have completed
     MR_HIP(hipEventRecord(c->ev0, c->s_req));

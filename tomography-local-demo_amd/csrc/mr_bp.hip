@@ -119,143 +119,9 @@ __device__ void make_record(const ViewGeom* __restrict__ gp, float vs, const mr_
                         static_cast<float>(g.uvv));
 }
 
-template <bool CONE, typename ST>
-__global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel(
-    const ST* __restrict__ ring, int64_t view_stride, int pitch, int R, int C,
-    const ViewGeom* __restrict__ geom, const float* __restrict__ vscale,
-    const mr_task* __restrict__ tasks, const Tile* __restrict__ tiles, int n_views,
-    float scale, float* __restrict__ out) {
-    __shared__ float4 rec4[BP_VPASS * BP_REC / 4];
-    float* rec = reinterpret_cast<float*>(rec4);
-
-    const Tile tl = tiles[blockIdx.x];
-    const mr_task& T = tasks[tl.task];
-    const int t = threadIdx.x;
-    const int tx = t & (BP_TS - 1);
-    const int ty = t / BP_TS;                  // 0..7
-    constexpr int PSTEP = BP_THREADS / BP_TS;  // 8
-    constexpr int PPT = BP_TP / PSTEP;         // 4 points per thread
-
-    const int p0 = tl.tp * BP_TP, s0 = tl.ts * BP_TS;
-    double xt[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-        xt[d] = T.o[d] + tl.q * T.c[d] + p0 * T.a[d] + s0 * T.b[d];
-
-    const int s = s0 + tx;
-    bool valid[PPT], inside[PPT];
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) {
-        const int p = p0 + ty + PSTEP * m;
-        valid[m] = (p < T.np) && (s < T.ns);
-        inside[m] = valid[m] && (!T.mask_box || in_box(T, p, s));
-    }
-
-    const float ds = static_cast<float>(tx);
-    const float dp0 = static_cast<float>(ty);
-    float acc[PPT], accb[PPT];
-    float fr_par[PPT];
-    int roff_par[PPT];
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) { acc[m] = 0.0f; accb[m] = 0.0f; fr_par[m] = 0.0f; roff_par[m] = 0; }
-
-    for (int vb = 0; vb < n_views; vb += BP_VPASS) {
-        const int nv = min(BP_VPASS, n_views - vb);
-        __syncthreads();
-        if (t < nv)
-            make_record(geom + vb + t, vscale[vb + t], T, xt, R, C, pitch, CONE, rec + t * BP_REC);
-        __syncthreads();
-        if (!CONE && vb == 0) {
-            // parallel beam: the detector row of a point is the same for every view
-            const float4 r1 = rec4[0 * 4 + 1];
-            const float4 r0 = rec4[0 * 4 + 0];
-            const float4 r2 = rec4[0 * 4 + 2];
-            const float4 r3 = rec4[0 * 4 + 3];
-            const float V0 = r0.w, Va = r1.x, Vb = r1.y;
-            const float rlo = r2.w, rhi = r3.x;
-#pragma unroll
-            for (int m = 0; m < PPT; ++m) {
-                const float dp = dp0 + static_cast<float>(PSTEP * m);
-                float row = fmaf(dp, Vb, fmaf(ds, Va, V0));
-                row = __builtin_amdgcn_fmed3f(row, rlo, rhi);
-                const float rf = floorf(row);
-                fr_par[m] = row - rf;
-                roff_par[m] = static_cast<int>(rf) * pitch;
-            }
-        }
-        for (int v = 0; v < nv; ++v) {
-            const float4 r0 = rec4[v * 4 + 0];
-            const float4 r1 = rec4[v * 4 + 1];
-            const float4 r2 = rec4[v * 4 + 2];
-            const float4 r3 = rec4[v * 4 + 3];
-            const float U0 = r0.x, Ua = r0.y, Ub = r0.z;
-            const float clo = r2.y, chi = r2.z;
-            const int base = __float_as_int(r3.z);
-            const ST* __restrict__ q0 = ring + static_cast<int64_t>(vb + v) * view_stride;
-            const ST* __restrict__ q1 = q0 + pitch;
-            if (CONE) {
-                const float V0 = r0.w, Va = r1.x, Vb = r1.y;
-                const float W0 = r1.z, Wa = r1.w, Wb = r2.x;
-                const float rlo = r2.w, rhi = r3.x;
-                const float uh0 = fmaf(dp0, Ub, fmaf(ds, Ua, U0));
-                const float vh0 = fmaf(dp0, Vb, fmaf(ds, Va, V0));
-                const float wh0 = fmaf(dp0, Wb, fmaf(ds, Wa, W0));
-#pragma unroll
-                for (int m = 0; m < PPT; ++m) {
-                    const float dm = static_cast<float>(PSTEP * m);
-                    const float uh = fmaf(dm, Ub, uh0);
-                    const float vh = fmaf(dm, Vb, vh0);
-                    const float wh = fmaf(dm, Wb, wh0);
-                    const float rw = __builtin_amdgcn_rcpf(wh);
-                    float col = uh * rw, row = vh * rw;
-                    const float wt = rw * rw;
-                    col = __builtin_amdgcn_fmed3f(col, clo, chi);
-                    row = __builtin_amdgcn_fmed3f(row, rlo, rhi);
-                    const float cf = floorf(col), rf = floorf(row);
-                    const float fc = col - cf, fr = row - rf;
-                    const uint32_t idx = static_cast<uint32_t>(
-                        base + static_cast<int>(rf) * pitch + static_cast<int>(cf));
-                    const float a = Store<ST>::ld(q0, idx), b = Store<ST>::ld(q0, idx + 1);
-                    const float c = Store<ST>::ld(q1, idx), d = Store<ST>::ld(q1, idx + 1);
-                    const float top = fmaf(fc, b - a, a);
-                    const float bot = fmaf(fc, d - c, c);
-                    acc[m] = fmaf(wt, fmaf(fr, bot - top, top), acc[m]);
-                }
-            } else {
-                const float vs = r3.y;
-                const float uh0 = fmaf(dp0, Ub, fmaf(ds, Ua, U0));
-#pragma unroll
-                for (int m = 0; m < PPT; ++m) {
-                    const float dm = static_cast<float>(PSTEP * m);
-                    float col = fmaf(dm, Ub, uh0);
-                    col = __builtin_amdgcn_fmed3f(col, clo, chi);
-                    const float cf = floorf(col);
-                    const float fc = col - cf;
-                    const uint32_t idx =
-                        static_cast<uint32_t>(base + roff_par[m] + static_cast<int>(cf));
-                    const float a = Store<ST>::ld(q0, idx), b = Store<ST>::ld(q0, idx + 1);
-                    const float c = Store<ST>::ld(q1, idx), d = Store<ST>::ld(q1, idx + 1);
-                    acc[m] = fmaf(vs, fmaf(fc, b - a, a), acc[m]);
-                    accb[m] = fmaf(vs, fmaf(fc, d - c, c), accb[m]);
-                }
-            }
-        }
-    }
-
-    const int64_t obase = T.out_offset + static_cast<int64_t>(tl.q) * T.stride_q;
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) {
-        if (!valid[m]) continue;
-        const int p = p0 + ty + PSTEP * m;
-        float val = acc[m];
-        if (!CONE) val = fmaf(fr_par[m], accb[m] - acc[m], acc[m]);
-        out[obase + static_cast<int64_t>(p) * T.stride_p + static_cast<int64_t>(s) * T.stride_s] =
-            inside[m] ? val * scale : 0.0f;
-    }
-}
-
 // ---------------------------------------------------------------- v2 inner loop
-// Same tiles, records and arithmetic as bp_tiles_kernel; the gathers go through a
+// FP64 records of each (tile, view) in LDS, the FP32 interpolation of every point of the
+// tile in registers; the gathers go through a
 // buffer resource per view (base = the view's first element, range = the view), so
 // the per-point address is one 24-bit multiply-add and one shift-add of absolute
 // byte offsets, the second detector row comes from the SGPR offset, the +1 column
@@ -661,247 +527,6 @@ __global__ __launch_bounds__(BP_THREADS, 5) void bp_tiles_kernel2(
         out[oidx(m)] = inside[m] ? val * scale : 0.0f;
     }
     }   // tile loop
-}
-
-// ---------------------------------------------------------------- v3: LDS-staged taps
-// Oblique slices make the 64 lanes of one gather instruction fall on ~30 detector
-// rows, so direct gathers fetch mostly unused cache lines.  v3 stages, per (tile,
-// view), the detector bounding box of the tile's taps into LDS with 16-byte loads
-// (fully coalesced rows, one buffer resource per view so that reads past the view
-// return zero), then gathers the four taps from LDS.  A view whose box does not fit
-// the stage buffer falls back to the direct gathers of v2 (uniform branch).
-constexpr int V3_REC = 24;            // floats per (tile, view) record
-constexpr int V3_VPASS = 32;          // views per record pass
-constexpr int V3_STAGE_BYTES = 32768; // LDS stage buffer
-
-// The box: float64 projections of the 4 tile corners (a projective map sends the
-// tile's convex hull into the hull of the corner images), clamped like the inner loop,
-// widened by one pixel each side (FP32 inner-loop coordinates differ from float64 by
-// ~1e-5 px), rows/cols of the two taps, aligned to 16 bytes.
-template <int LE>
-__device__ void make_record3(const ViewGeom* __restrict__ gp, float vs, const mr_task& T,
-                             const double* xt, int R, int C, int pitch, bool cone,
-                             float* __restrict__ rec) {
-    RecRef q;
-    make_record(gp, vs, T, xt, R, C, pitch, cone, rec, &q);
-    // corner images of the tile, relative to the reference, then absolute
-    double cmin = 1e300, cmax = -1e300, rmin = 1e300, rmax = -1e300;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const double ds = (k & 1) ? BP_TS - 1 : 0, dp = (k & 2) ? BP_TP - 1 : 0;
-        const double w = q.W0 + ds * q.Wa + dp * q.Wb;
-        const double col = (q.U0 + ds * q.Ua + dp * q.Ub) / w + q.cref;
-        const double row = (q.V0 + ds * q.Va + dp * q.Vb) / w + q.rref;
-        cmin = fmin(cmin, col); cmax = fmax(cmax, col);
-        rmin = fmin(rmin, row); rmax = fmax(rmax, row);
-    }
-    // absolute range of the taps after the inner loop's clamp, one pixel margin
-    const double c_lo = fmax(floor(fmin(fmax(cmin, -1.0), (double)C)) - 1.0, -1.0);
-    const double c_hi = fmin(floor(fmin(fmax(cmax, -1.0), (double)C)) + 2.0, C + 1.0);
-    const double r_lo = fmax(floor(fmin(fmax(rmin, -1.0), (double)R)) - 1.0, -1.0);
-    const double r_hi = fmin(floor(fmin(fmax(rmax, -1.0), (double)R)) + 2.0, R + 1.0);
-    constexpr int ALIGN = 16 >> LE;                       // elements per 16 bytes
-    const int pc0 = (static_cast<int>(c_lo) + 1) & ~(ALIGN - 1);   // padded coordinates
-    const int pc1 = static_cast<int>(c_hi) + 1;
-    const int pr0 = static_cast<int>(r_lo) + 1;
-    const int pr1 = static_cast<int>(r_hi) + 1;
-    int nc = (pc1 - pc0 + ALIGN) & ~(ALIGN - 1);
-    int nr = pr1 - pr0 + 1;
-    const bool finite = cmin == cmin && cmax == cmax && rmin == rmin && rmax == rmax &&
-                        fabs(cmin) < 1e9 && fabs(cmax) < 1e9 && fabs(rmin) < 1e9 && fabs(rmax) < 1e9;
-    const int stride = nc + ALIGN;    // keeps 16-byte rows and shifts LDS banks per row
-    if (!finite || nr <= 0 || nc <= 0 || nr * stride * (1 << LE) > V3_STAGE_BYTES) nr = nc = 0;
-    int* ri = reinterpret_cast<int*>(rec);
-    ri[16] = pr0 - (static_cast<int>(q.rref) + 1);     // box row 0 relative to rref
-    ri[17] = pc0 - (static_cast<int>(q.cref) + 1);     // box col 0 relative to cref
-    ri[18] = nr;
-    ri[19] = nc;
-    ri[20] = pr0 * pitch + pc0;                        // element offset of the box origin
-    ri[21] = stride;
-    ri[22] = 0;
-    ri[23] = 0;
-}
-
-template <typename ST> struct Lds;
-template <> struct Lds<float> {
-    __device__ static inline void taps(const float* __restrict__ st, int i, int stride, float& a,
-                                       float& b, float& c, float& d) {
-        a = st[i]; b = st[i + 1]; c = st[i + stride]; d = st[i + stride + 1];
-    }
-};
-template <> struct Lds<__half> {
-    __device__ static inline void taps(const __half* __restrict__ st, int i, int stride, float& a,
-                                       float& b, float& c, float& d) {
-        a = __half2float(st[i]); b = __half2float(st[i + 1]);
-        c = __half2float(st[i + stride]); d = __half2float(st[i + stride + 1]);
-    }
-};
-template <> struct Lds<uint16_t> {
-    __device__ static inline void taps(const uint16_t* __restrict__ st, int i, int stride, float& a,
-                                       float& b, float& c, float& d) {
-        a = __uint_as_float(static_cast<uint32_t>(st[i]) << 16);
-        b = __uint_as_float(static_cast<uint32_t>(st[i + 1]) << 16);
-        c = __uint_as_float(static_cast<uint32_t>(st[i + stride]) << 16);
-        d = __uint_as_float(static_cast<uint32_t>(st[i + stride + 1]) << 16);
-    }
-};
-
-template <bool CONE, typename ST>
-__global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel3(
-    const ST* __restrict__ ring, int64_t view_stride, int pitch, int R, int C,
-    const ViewGeom* __restrict__ geom, const float* __restrict__ vscale,
-    const mr_task* __restrict__ tasks, const Tile* __restrict__ tiles, int n_views,
-    float scale, float* __restrict__ out) {
-    __shared__ float4 rec4[V3_VPASS * V3_REC / 4];
-    __shared__ float4 stage4[V3_STAGE_BYTES / 16];
-    float* rec = reinterpret_cast<float*>(rec4);
-    const ST* stage = reinterpret_cast<const ST*>(stage4);
-    constexpr int LE = Taps<ST>::LOG_ESZ;
-    constexpr int ALIGN = 16 >> LE;
-
-    const Tile tl = tiles[blockIdx.x];
-    const mr_task& T = tasks[tl.task];
-    const int t = threadIdx.x;
-    const int tx = t & (BP_TS - 1);
-    const int ty = t / BP_TS;
-    constexpr int PSTEP = BP_THREADS / BP_TS;
-    constexpr int PPT = BP_TP / PSTEP;
-
-    const int p0 = tl.tp * BP_TP, s0 = tl.ts * BP_TS;
-    double xt[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-        xt[d] = T.o[d] + tl.q * T.c[d] + p0 * T.a[d] + s0 * T.b[d];
-
-    const int s = s0 + tx;
-    bool valid[PPT], inside[PPT];
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) {
-        const int p = p0 + ty + PSTEP * m;
-        valid[m] = (p < T.np) && (s < T.ns);
-        inside[m] = valid[m] && (!T.mask_box || in_box(T, p, s));
-    }
-
-    const float ds = static_cast<float>(tx);
-    const float dp0 = static_cast<float>(ty);
-    const int pitchb = pitch << LE;
-    const int view_bytes = static_cast<int>(view_stride << LE);
-    float acc[PPT], accb[PPT], fr_par[PPT];
-    int rrow_par[PPT];
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) { acc[m] = 0.0f; accb[m] = 0.0f; fr_par[m] = 0.0f; rrow_par[m] = 0; }
-
-    for (int vb = 0; vb < n_views; vb += V3_VPASS) {
-        const int nv = min(V3_VPASS, n_views - vb);
-        __syncthreads();
-        if (t < nv)
-            make_record3<LE>(geom + vb + t, vscale[vb + t], T, xt, R, C, pitch, CONE,
-                             rec + t * V3_REC);
-        __syncthreads();
-        if (!CONE && vb == 0) {
-            const float4 r0 = rec4[0], r1 = rec4[1], r2 = rec4[2], r3 = rec4[3];
-#pragma unroll
-            for (int m = 0; m < PPT; ++m) {
-                const float dp = dp0 + static_cast<float>(PSTEP * m);
-                float row = fmaf(dp, r1.y, fmaf(ds, r1.x, r0.w));
-                row = __builtin_amdgcn_fmed3f(row, r2.w, r3.x);
-                fr_par[m] = __builtin_amdgcn_fractf(row);
-                rrow_par[m] = cvt_flr(row);
-            }
-        }
-        for (int v = 0; v < nv; ++v) {
-            const float* rv = rec + v * V3_REC;
-            const float4 r0 = rec4[v * (V3_REC / 4) + 0];
-            const float4 r1 = rec4[v * (V3_REC / 4) + 1];
-            const float4 r2 = rec4[v * (V3_REC / 4) + 2];
-            const float4 r3 = rec4[v * (V3_REC / 4) + 3];
-            const int4 bx = *reinterpret_cast<const int4*>(rv + 16);   // sr0, sc0, nr, nc
-            const int4 by = *reinterpret_cast<const int4*>(rv + 20);   // gofs, stride
-            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-                const_cast<ST*>(ring + static_cast<int64_t>(vb + v) * view_stride), 0, view_bytes,
-                0x00020000);
-            const bool staged = bx.z > 0;
-            if (staged) {
-                // stage the box: chunks of 16 bytes, rows of nc elements
-                const int cpr = bx.w / ALIGN;                 // chunks per row
-                const int total = bx.z * cpr;
-                const float inv = 1.0f / static_cast<float>(cpr);
-                __syncthreads();
-                for (int e = t; e < total; e += BP_THREADS) {
-                    const int r = static_cast<int>((static_cast<float>(e) + 0.5f) * inv);
-                    const int cc = e - r * cpr;
-                    const int goff = (by.x + r * pitch + cc * ALIGN) << LE;
-                    const float4 val = __builtin_bit_cast(
-                        float4, __builtin_amdgcn_raw_buffer_load_b128(rs, goff, 0, 0));
-                    stage4[(r * by.y + cc * ALIGN) / ALIGN] = val;
-                }
-                __syncthreads();
-            }
-            const int baseb = __float_as_int(r3.z) << LE;
-            if (CONE) {
-                const float uh0 = fmaf(dp0, r0.z, fmaf(ds, r0.y, r0.x));
-                const float vh0 = fmaf(dp0, r1.y, fmaf(ds, r1.x, r0.w));
-                const float wh0 = fmaf(dp0, r2.x, fmaf(ds, r1.w, r1.z));
-#pragma unroll
-                for (int m = 0; m < PPT; ++m) {
-                    const float dm = static_cast<float>(PSTEP * m);
-                    const float uh = m ? fmaf(dm, r0.z, uh0) : uh0;
-                    const float vh = m ? fmaf(dm, r1.y, vh0) : vh0;
-                    const float wh = m ? fmaf(dm, r2.x, wh0) : wh0;
-                    const float rw = __builtin_amdgcn_rcpf(wh);
-                    const float col = __builtin_amdgcn_fmed3f(uh * rw, r2.y, r2.z);
-                    const float row = __builtin_amdgcn_fmed3f(vh * rw, r2.w, r3.x);
-                    const float wt = rw * rw;
-                    const float fc = __builtin_amdgcn_fractf(col);
-                    const float fr = __builtin_amdgcn_fractf(row);
-                    const int ci = cvt_flr(col), ri = cvt_flr(row);
-                    float a, b, c, d;
-                    if (staged) {
-                        const int li = __mul24(ri - bx.x, by.y) + (ci - bx.y);
-                        Lds<ST>::taps(stage, li, by.y, a, b, c, d);
-                    } else {
-                        const int voff = __mul24(ri, pitchb) + ((ci << LE) + baseb);
-                        Taps<ST>::load(rs, voff, pitchb, a, b, c, d);
-                    }
-                    const float top = fmaf(fc, b - a, a);
-                    const float bot = fmaf(fc, d - c, c);
-                    acc[m] = fmaf(wt, fmaf(fr, bot - top, top), acc[m]);
-                }
-            } else {
-                const float vs = r3.y;
-                const float uh0 = fmaf(dp0, r0.z, fmaf(ds, r0.y, r0.x));
-#pragma unroll
-                for (int m = 0; m < PPT; ++m) {
-                    const float dm = static_cast<float>(PSTEP * m);
-                    const float col =
-                        __builtin_amdgcn_fmed3f(m ? fmaf(dm, r0.z, uh0) : uh0, r2.y, r2.z);
-                    const float fc = __builtin_amdgcn_fractf(col);
-                    const int ci = cvt_flr(col);
-                    float a, b, c, d;
-                    if (staged) {
-                        const int li = __mul24(rrow_par[m] - bx.x, by.y) + (ci - bx.y);
-                        Lds<ST>::taps(stage, li, by.y, a, b, c, d);
-                    } else {
-                        const int voff = __mul24(rrow_par[m], pitchb) + ((ci << LE) + baseb);
-                        Taps<ST>::load(rs, voff, pitchb, a, b, c, d);
-                    }
-                    acc[m] = fmaf(vs, fmaf(fc, b - a, a), acc[m]);
-                    accb[m] = fmaf(vs, fmaf(fc, d - c, c), accb[m]);
-                }
-            }
-        }
-    }
-
-    const int64_t obase = T.out_offset + static_cast<int64_t>(tl.q) * T.stride_q;
-#pragma unroll
-    for (int m = 0; m < PPT; ++m) {
-        if (!valid[m]) continue;
-        const int p = p0 + ty + PSTEP * m;
-        float val = acc[m];
-        if (!CONE) val = fmaf(fr_par[m], accb[m] - acc[m], acc[m]);
-        out[obase + static_cast<int64_t>(p) * T.stride_p + static_cast<int64_t>(s) * T.stride_s] =
-            inside[m] ? val * scale : 0.0f;
-    }
 }
 
 // ---------------------------------------------------------------- v4: precomputed records
@@ -1722,28 +1347,6 @@ int dispatch_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntil
                    float scale, float* d_out, hipStream_t s, const TileQueue& q) {
     // persistent launches size the grid to the resident capacity and pull tiles from q
     dim3 grid(q.head ? q.grid : ntiles), block(BP_THREADS);
-    if (c->bp_variant == 3) {
-        switch (c->storage) {
-        case MR_STORE_F32:
-            hipLaunchKernelGGL((bp_tiles_kernel3<CONE, float>), grid, block, 0, s,
-                               static_cast<const float*>(c->ring), c->view_stride, c->pitch, c->R,
-                               c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out);
-            break;
-        case MR_STORE_F16:
-            hipLaunchKernelGGL((bp_tiles_kernel3<CONE, __half>), grid, block, 0, s,
-                               static_cast<const __half*>(c->ring), c->view_stride, c->pitch, c->R,
-                               c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out);
-            break;
-        default:
-            hipLaunchKernelGGL((bp_tiles_kernel3<CONE, uint16_t>), grid, block, 0, s,
-                               static_cast<const uint16_t*>(c->ring), c->view_stride, c->pitch,
-                               c->R, c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale,
-                               d_out);
-            break;
-        }
-        MR_HIP(hipGetLastError());
-        return MR_OK;
-    }
     if (c->bp_variant == 2 && c->v2_ring == 1 && c->storage == MR_STORE_F32) {
         hipLaunchKernelGGL((bp_tiles_kernel2<CONE, float, 1>), grid, block, 0, s,
                            static_cast<const float*>(c->pring), pair_view_stride(c), c->pitch, c->R,
@@ -1770,7 +1373,9 @@ int dispatch_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntil
         MR_HIP(hipGetLastError());
         return MR_OK;
     }
-    if (c->bp_variant == 2) {
+    // kernel 2 on the ring itself: the v2 loop, and every launch not taken above (no
+    // views, one tile per workgroup)
+    {
         switch (c->storage) {
         case MR_STORE_F32:
             hipLaunchKernelGGL((bp_tiles_kernel2<CONE, float>), grid, block, 0, s,
@@ -1797,25 +1402,6 @@ int dispatch_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntil
         MR_HIP(hipGetLastError());
         return MR_OK;
     }
-    switch (c->storage) {
-    case MR_STORE_F32:
-        hipLaunchKernelGGL((bp_tiles_kernel<CONE, float>), grid, block, 0, s,
-                           static_cast<const float*>(c->ring), c->view_stride, c->pitch, c->R, c->C,
-                           c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out);
-        break;
-    case MR_STORE_F16:
-        hipLaunchKernelGGL((bp_tiles_kernel<CONE, __half>), grid, block, 0, s,
-                           static_cast<const __half*>(c->ring), c->view_stride, c->pitch, c->R,
-                           c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out);
-        break;
-    default:
-        hipLaunchKernelGGL((bp_tiles_kernel<CONE, uint16_t>), grid, block, 0, s,
-                           static_cast<const uint16_t*>(c->ring), c->view_stride, c->pitch, c->R,
-                           c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out);
-        break;
-    }
-    MR_HIP(hipGetLastError());
-    return MR_OK;
 }
 
 template <bool CONE>
@@ -1880,9 +1466,9 @@ int launch_bp_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int nti
             // of records in chunks and view blocks whose sums make a round trip through
             // the 105 MB output per launch (measured on that search: f16 38 against 54
             // ms, f32 parallel 38 against 82 ms)
-            const int rk = v2_ring_kind(c);
             const int rc0 = prepare_persistent_bp(c, n_views, s);
             if (rc0) return rc0;
+            const int rk = v2_ring_kind(c);
             const int sv = c->bp_variant;
             c->bp_variant = 2;
             c->v2_ring = rk;
@@ -1896,8 +1482,6 @@ int launch_bp_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int nti
         }
         return launch_bp_tiles_v4(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s);
     }
-    if (q && c->bp_variant != 2 && c->bp_variant < 4)
-        return fail(MR_ERUNTIME, "persistent launches need the v2 kernel");
     if (q) {                               // persistent launches always run the v2 loop
         const int saved = c->bp_variant;
         c->v2_ring = v2_ring_kind(c);
@@ -2113,17 +1697,102 @@ static void launch_v5_kernel(Ctx* c, const float* recs, const int* fit, const mr
                            d_tiles, n_views, scale, d_out, c->xcd_map != 0, fit);
 }
 
-// Kernel 6 reads a row-pair copy of the ring (twice its size), built on first use and
-// extended to the views stored since (slots below pring_n never change until the views
-// are reset).
+// Ring copies.  The row-pair copy (kernel 6, twice the ring) and the band copy (kernel 7,
+// 4/3 of a 16-bit ring) are caches of the ring, built on first use and extended to the
+// views stored since (slots below pring_n / bring_n never change until the views are
+// reset).  A copy is built only if the device keeps copy_reserve bytes free after it and
+// the context's copies stay within copy_budget; when only freeing the other copy (not
+// read by the request at hand) makes room, that copy is dropped first, and when nothing
+// does the request runs without it (kernel 6 -> 7 -> 4 on the ring, the v2 loop on the
+// ring).  A drop bumps copies_epoch, so graphs captured over the dropped copy's address
+// are captured again.
+constexpr int kNoRoom = 1;   // ensure_*_ring: not built for lack of memory (not an error)
+
+size_t copies_held(const Ctx* c) {
+    return (c->pring ? c->pring_bytes : 0) + (c->bring ? c->bring_bytes : 0);
+}
+
+namespace {
+
+// 0: room for `bytes` now; 1: room once the `freeable` bytes of the other copy are
+// freed; -1: no room
+int copy_room(const Ctx* c, size_t bytes, size_t freeable) {
+    int r = 0;
+    if (c->copy_budget >= 0) {
+        const size_t held = copies_held(c), budget = static_cast<size_t>(c->copy_budget);
+        if (held + bytes > budget) r = held - freeable + bytes <= budget ? 1 : -1;
+    }
+    size_t fr = 0, total = 0;
+    if (r >= 0 && hipMemGetInfo(&fr, &total) == hipSuccess) {
+        const size_t need = bytes + c->copy_reserve;
+        if (fr < need) r = fr + freeable >= need ? 1 : -1;
+    }
+    return r;
+}
+
+void drop_pair_ring(Ctx* c, hipStream_t s) {
+    if (!c->pring) return;
+    hipStreamSynchronize(s);
+    hipFree(c->pring);
+    c->pring = nullptr;
+    c->pring_bytes = 0;
+    c->pring_n = 0;
+    ++c->copies_epoch;
+}
+
+void drop_band_ring(Ctx* c, hipStream_t s) {
+    if (!c->bring) return;
+    hipStreamSynchronize(s);
+    hipFree(c->bring);
+    c->bring = nullptr;
+    c->bring_bytes = 0;
+    c->bring_n = 0;
+    ++c->copies_epoch;
+}
+
+// Allocates one copy of `bytes` at *dst (other: the other copy, dropped if that makes
+// room).  MR_OK, kNoRoom, or an error.
+int alloc_copy(Ctx* c, void** dst, size_t bytes, bool other_is_band, bool* no_room,
+               hipStream_t s) {
+    const size_t freeable = other_is_band ? (c->bring ? c->bring_bytes : 0)
+                                          : (c->pring ? c->pring_bytes : 0);
+    const int room = copy_room(c, bytes, freeable);
+    if (room < 0) {
+        *no_room = true;
+        return kNoRoom;
+    }
+    if (room == 1) other_is_band ? drop_band_ring(c, s) : drop_pair_ring(c, s);
+    const hipError_t e = hipMalloc(dst, bytes);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        *dst = nullptr;
+        *no_room = true;
+        return kNoRoom;
+    }
+    MR_HIP(e);
+    *no_room = false;
+    return MR_OK;
+}
+
+}  // namespace
+
+void trim_ring_copies(Ctx* c, hipStream_t s) {
+    if (c->copy_budget < 0) return;
+    if (copies_held(c) > static_cast<size_t>(c->copy_budget)) drop_pair_ring(c, s);
+    if (copies_held(c) > static_cast<size_t>(c->copy_budget)) drop_band_ring(c, s);
+}
+
 int ensure_pair_ring(Ctx* c, int n_views, hipStream_t s) {
     const size_t esz = c->storage == MR_STORE_F32 ? 4 : 2;
     const int64_t pvs = 2 * static_cast<int64_t>(c->R + 2) * c->pitch;
     if (!c->pring) {
-        c->pring_bytes = esz * static_cast<size_t>(pvs) * c->n_total;
-        MR_HIP(hipMalloc(&c->pring, c->pring_bytes));
+        const size_t bytes = esz * static_cast<size_t>(pvs) * c->n_total;
+        const int rc = alloc_copy(c, &c->pring, bytes, true, &c->pair_no_room, s);
+        if (rc) return rc;
+        c->pring_bytes = bytes;
         c->pring_n = 0;
     }
+    c->pair_no_room = false;
     const int v1 = n_views;
     for (int v0 = c->pring_n; v0 < v1; v0 += 4096) {
         const int nv = std::min(4096, v1 - v0);
@@ -2142,8 +1811,7 @@ int ensure_pair_ring(Ctx* c, int n_views, hipStream_t s) {
     return MR_OK;
 }
 
-// Kernel 7 (16-bit storage) reads the band ring: bands of 4 padded rows stepping 3 rows
-// (4/3 of the ring), built and extended like the row-pair ring.
+// The band copy: bands of 4 padded rows stepping 3 rows.
 int64_t band_view_stride(const Ctx* c) {
     return static_cast<int64_t>(band_count(c->R)) * c->pitch * 4;
 }
@@ -2151,10 +1819,13 @@ int64_t band_view_stride(const Ctx* c) {
 int ensure_band_ring(Ctx* c, int n_views, hipStream_t s) {
     const int64_t bvs = band_view_stride(c);
     if (!c->bring) {
-        c->bring_bytes = sizeof(uint16_t) * static_cast<size_t>(bvs) * c->n_total;
-        MR_HIP(hipMalloc(&c->bring, c->bring_bytes));
+        const size_t bytes = sizeof(uint16_t) * static_cast<size_t>(bvs) * c->n_total;
+        const int rc = alloc_copy(c, &c->bring, bytes, false, &c->band_no_room, s);
+        if (rc) return rc;
+        c->bring_bytes = bytes;
         c->bring_n = 0;
     }
+    c->band_no_room = false;
     for (int v0 = c->bring_n; v0 < n_views; v0 += 4096) {
         const int nv = std::min(4096, n_views - v0);
         const dim3 grid((c->pitch + 255) / 256, band_count(c->R), nv);
@@ -2182,16 +1853,22 @@ int64_t pair_view_stride(const Ctx* c) { return 2 * static_cast<int64_t>(c->R + 
 // row-pair copy for f32 parallel beam (its points keep one detector row, so one 16-byte
 // load brings both rows; measured on the parallel COR search: 38 against 73 ms; oblique
 // cone slices lose, 7.4 against 5.8 ms on kernel 4); else the ring itself.
+// A copy the last prepare_persistent_bp found no room for is not chosen.
 int v2_ring_kind(const Ctx* c) {
     const bool par = c->kind == MR_KIND_PARALLEL;
-    if ((c->bp_variant == 7 || (par && c->bp_variant == 6)) && band_fits(c)) return 2;
-    if (c->storage == MR_STORE_F32 && par && (c->bp_variant == 4 || c->bp_variant >= 6)) return 1;
+    if ((c->bp_variant == 7 || (par && c->bp_variant == 6)) && band_fits(c))
+        return c->band_no_room ? 0 : 2;
+    if (c->storage == MR_STORE_F32 && par && (c->bp_variant == 4 || c->bp_variant >= 6))
+        return c->pair_no_room ? 0 : 1;
     return 0;
 }
 
 int prepare_persistent_bp(Ctx* c, int n_views, hipStream_t s) {
+    c->band_no_room = c->pair_no_room = false;       // memory may have been freed since
     const int rk = v2_ring_kind(c);
-    return rk == 2 ? ensure_band_ring(c, n_views, s) : rk == 1 ? ensure_pair_ring(c, n_views, s) : MR_OK;
+    const int rc = rk == 2 ? ensure_band_ring(c, n_views, s)
+                           : rk == 1 ? ensure_pair_ring(c, n_views, s) : MR_OK;
+    return rc == kNoRoom ? MR_OK : rc;               // v2_ring_kind now answers 0
 }
 
 // In a captured streaming graph: row-pair copy of the k views just stored, ending at the
@@ -2228,18 +1905,23 @@ int launch_bp_tiles_v4(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int 
     }
     const bool cone = c->kind == MR_KIND_CONE;
     const bool v5 = cone && c->bp_variant == 5;
+    // the kernel the ring copies in memory allow: without room for the row-pair copy,
+    // kernel 7 (16-bit, band copy) or 4 (the ring); without room for the band copy, 4
+    auto run_as = [&](int variant) {
+        const int sv = c->bp_variant;
+        c->bp_variant = variant;
+        const int rc = launch_bp_tiles_v4(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s);
+        c->bp_variant = sv;
+        return rc;
+    };
     if (c->bp_variant == 6 || (c->bp_variant == 7 && c->storage == MR_STORE_F32)) {
-        int rc = ensure_pair_ring(c, n_views, s);
+        const int rc = ensure_pair_ring(c, n_views, s);
+        if (rc == kNoRoom) return run_as(c->storage != MR_STORE_F32 && band_fits(c) ? 7 : 4);
         if (rc) return rc;
     } else if (c->bp_variant == 7) {
-        if (!band_fits(c)) {
-            const int sv = c->bp_variant;
-            c->bp_variant = 6;
-            const int rc = launch_bp_tiles_v4(c, d_tasks, d_tiles, ntiles, n_views, scale, d_out, s);
-            c->bp_variant = sv;
-            return rc;
-        }
-        int rc = ensure_band_ring(c, n_views, s);
+        if (!band_fits(c)) return run_as(6);
+        const int rc = ensure_band_ring(c, n_views, s);
+        if (rc == kNoRoom) return run_as(4);
         if (rc) return rc;
     }
     const bool band = c->bp_variant == 7 && c->storage != MR_STORE_F32;   // band-ring records

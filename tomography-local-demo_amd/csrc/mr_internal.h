@@ -130,6 +130,13 @@ struct Ctx {
     void* bring = nullptr;           // band copy of a 16-bit ring (kernel 7)
     size_t bring_bytes = 0;
     int bring_n = 0;                 // views converted into bring
+    // ring copies are caches (ensure_pair_ring / ensure_band_ring): built only with
+    // copy_reserve bytes of device memory left after them and within copy_budget
+    // (-1: none) for both together; a copy dropped to make room bumps copies_epoch
+    size_t copy_reserve = size_t{1} << 30;   // MINIRECAST_COPY_RESERVE_MB
+    int64_t copy_budget = -1;
+    uint64_t copies_epoch = 0;
+    bool band_no_room = false, pair_no_room = false;   // the last attempt found no room
     uint16_t* d_raw = nullptr;       // staged raw frames
     size_t raw_bytes = 0;
     void* d_vol = nullptr;           // staged volume for the statistics entry points
@@ -225,8 +232,10 @@ struct Plan {
     float* d_acc = nullptr;
     int acc_n = -1;
     uint64_t acc_epoch = 0;
-    // ring copy the captured graphs read (v2_ring_kind; recaptured when that changes)
+    // ring copy the captured graphs read (v2_ring_kind) and the context's copies_epoch
+    // at their capture: recaptured when either changes
     int graphs_ring = 0, cor_ring = 0;
+    uint64_t graphs_copies = 0, cor_copies = 0;
     int cor_chunks = 0;           // grid width of the scoring kernels (largest chunk count)
     // rotation-centre scoring of the plan's images (one graph: persistent BP + scores)
     std::vector<mr_task> h_tasks;    // host copy of the tasks
@@ -284,6 +293,8 @@ int prepare_persistent_bp(Ctx* c, int n_views, hipStream_t s);
 int launch_band_convert_tail(Ctx* c, const int* d_n, int k, hipStream_t s);
 int launch_pair_convert_tail(Ctx* c, const int* d_n, int k, hipStream_t s);
 int ensure_pair_ring(Ctx* c, int n_views, hipStream_t s);
+size_t copies_held(const Ctx* c);    // device bytes of the resident ring copies
+void trim_ring_copies(Ctx* c, hipStream_t s);   // drops copies beyond copy_budget
 int launch_stream_prep(Ctx* c, const void* d_meta, int k, int* d_nviews, hipStream_t s);
 int persistent_grid(Ctx* c);
 int launch_bp_points(Ctx* c, const double* d_pts, int64_t M, double cor_shift, int n_views,

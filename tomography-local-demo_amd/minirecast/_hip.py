@@ -49,6 +49,8 @@ SIGNATURES = [
     ("mr_set_bp_views_per_pass", _c.c_int, [_P, _c.c_int]),
     ("mr_get_bp_views_per_pass", _c.c_int, [_P, _c.POINTER(_c.c_int)]),
     ("mr_get_bp_config", _c.c_int, [_P, _P]),
+    ("mr_set_ring_copy_budget", _c.c_int, [_P, _c.c_int64]),
+    ("mr_device_bytes", _c.c_int, [_P, _P]),
     ("mr_autotune", _c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_int64, _c.c_int, _c.c_int, _P,
                                _c.POINTER(_c.c_float)]),
     ("mr_preprocess", _c.c_int,
@@ -306,6 +308,20 @@ class Context:
         cfg = np.zeros(4, np.int32)
         check(self.lib.mr_get_bp_config(self.h, ptr(cfg)))
         return dict(zip(self.BP_CONFIG_KEYS, (int(x) for x in cfg)))
+
+    def set_ring_copy_budget(self, nbytes):
+        """Cap on the device bytes of the ring copies kernels 6 and 7 read (None or < 0:
+        no cap); copies beyond it are freed and requests run without them."""
+        check(self.lib.mr_set_ring_copy_budget(self.h, -1 if nbytes is None else int(nbytes)))
+
+    DEVICE_BYTES_KEYS = ("ring", "band_copy", "pair_copy", "filter_scratch")
+
+    def device_bytes(self):
+        """Device bytes held now by the ring, its band and row-pair copies and the
+        four-step filter scratch."""
+        out = np.zeros(4, np.int64)
+        check(self.lib.mr_device_bytes(self.h, ptr(out)))
+        return dict(zip(self.DEVICE_BYTES_KEYS, (int(x) for x in out)))
 
     def set_views_per_pass(self, views):
         """Views per launch of cone slice requests that fill the device (0: all)."""
