@@ -1732,8 +1732,8 @@ int copy_room(const Ctx* c, size_t bytes, size_t freeable) {
 
 void drop_pair_ring(Ctx* c, hipStream_t s) {
     if (!c->pring) return;
-    hipStreamSynchronize(s);
-    hipFree(c->pring);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(c->pring);
     c->pring = nullptr;
     c->pring_bytes = 0;
     c->pring_n = 0;
@@ -1742,8 +1742,8 @@ void drop_pair_ring(Ctx* c, hipStream_t s) {
 
 void drop_band_ring(Ctx* c, hipStream_t s) {
     if (!c->bring) return;
-    hipStreamSynchronize(s);
-    hipFree(c->bring);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(c->bring);
     c->bring = nullptr;
     c->bring_bytes = 0;
     c->bring_n = 0;

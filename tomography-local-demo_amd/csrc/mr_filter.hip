@@ -485,19 +485,58 @@ __global__ __launch_bounds__(256) void fourstep_cols_kernel(
 // Per-view max |x| of raw blocks [R][k][C] (the f16 scale bound of filter_kernel).
 // Workgroup (x, v) takes rows x, x + gridDim.x, ... of view v, so that each view gets
 // gridDim.x (at most 128) atomics: one atomic per row would serialise ~R atomics on one
-// address.
+// address.  A row is read as 16-byte loads from its first 16-byte boundary (two in flight
+// per thread), with the elements before it and after the last whole load read singly.
 __global__ __launch_bounds__(256) void input_max_kernel(const float* __restrict__ block, int R,
                                                         int k, int C, unsigned* __restrict__ vmax) {
     const int v = blockIdx.y;
     float m = 0.0f;
-    for (int r = blockIdx.x; r < R; r += gridDim.x) {
+    // every row starts on a 16-byte boundary: the workgroup's rows x, x + G, ... as one
+    // index space of (row, 16-byte load) pairs, four loads in flight per thread
+    const int c4 = C >> 2;
+    const int nr = static_cast<int>(blockIdx.x) < R
+                       ? (R - 1 - static_cast<int>(blockIdx.x)) / static_cast<int>(gridDim.x) + 1
+                       : 0;
+    const bool aligned = (C & 3) == 0 && (reinterpret_cast<uintptr_t>(block) & 15) == 0 &&
+                         static_cast<int64_t>(nr) * c4 < (int64_t{1} << 31) - 1024;
+    if (aligned) {
+        const int total = nr * c4;
+        auto at = [&](int j) {
+            const int rr = j / c4, i = j - rr * c4;
+            const int r = static_cast<int>(blockIdx.x) + rr * static_cast<int>(gridDim.x);
+            return reinterpret_cast<const float4*>(block + (static_cast<int64_t>(r) * k + v) * C)[i];
+        };
+        auto mx = [](float4 a) {
+            return fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(a.w)));
+        };
+        int j = threadIdx.x;
+        for (; j + 768 < total; j += 1024) {
+            const float4 a = at(j), b = at(j + 256), c = at(j + 512), d = at(j + 768);
+            m = fmaxf(m, fmaxf(fmaxf(mx(a), mx(b)), fmaxf(mx(c), mx(d))));
+        }
+        for (; j < total; j += 256) m = fmaxf(m, mx(at(j)));
+    }
+    for (int r = aligned ? R : static_cast<int>(blockIdx.x); r < R; r += gridDim.x) {
         const float* src = block + (static_cast<int64_t>(r) * k + v) * C;
-        int c = threadIdx.x;
-        // four independent loads in flight per thread
-        for (; c + 768 < C; c += 1024)
-            m = fmaxf(fmaxf(m, fmaxf(fabsf(src[c]), fabsf(src[c + 256]))),
-                      fmaxf(fabsf(src[c + 512]), fabsf(src[c + 768])));
-        for (; c < C; c += 256) m = fmaxf(m, fabsf(src[c]));
+        const int head = static_cast<int>(((16 - (reinterpret_cast<uintptr_t>(src) & 15)) & 15) >> 2);
+        const int h = head < C ? head : C;
+        const int nq = (C - h) >> 2;                   // whole 16-byte loads
+        const float4* q = reinterpret_cast<const float4*>(src + h);
+        int i = threadIdx.x;
+        for (; i + 256 < nq; i += 512) {
+            const float4 a = q[i], b = q[i + 256];
+            m = fmaxf(m, fmaxf(fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(a.w))),
+                               fmaxf(fmaxf(fabsf(b.x), fabsf(b.y)), fmaxf(fabsf(b.z), fabsf(b.w)))));
+        }
+        for (; i < nq; i += 256) {
+            const float4 a = q[i];
+            m = fmaxf(m, fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(a.w))));
+        }
+        // the elements before the first 16-byte boundary and after the last whole load
+        const int t = threadIdx.x;
+        if (t < h) m = fmaxf(m, fabsf(src[t]));
+        const int tail = h + (nq << 2);
+        if (t < C - tail) m = fmaxf(m, fabsf(src[tail + t]));
     }
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
     __shared__ float red[4];
