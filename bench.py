@@ -80,11 +80,13 @@ def footprint_pixels(geom, pts, views):
 
 
 def band_lines_per_load(geom, box, planes, views, max_tiles=64, seed=0):
-    """Mean number of distinct 128-byte lines that one 16-byte band-ring load instruction
-    of the default 16-bit slice kernel touches (64 lanes: 2 point rows x 32 point columns
-    of a 32 x 32 tile, the kernel's lane layout), on a sample of the request's tiles and
-    views: each point's band element (b, c0 + 1) and the next, b = floor((floor(row) + 1)
-    / 3), in the band copy's byte layout (pitch = roundup(C + 3, 64) elements of 8 bytes)."""
+    """(lines, quad lines): the mean number of distinct 128-byte lines that one 16-byte
+    band-ring load instruction of the default 16-bit slice kernel touches, and the sum
+    over its 16 lane quads (lanes 4q..4q+3) of the distinct lines each quad touches (64
+    lanes: 2 point rows x 32 point columns of a 32 x 32 tile, the kernel's lane layout),
+    on a sample of the request's tiles and views: each point's band element (b, c0 + 1)
+    and the next, b = floor((floor(row) + 1) / 3), in the band copy's byte layout
+    (pitch = roundup(C + 3, 64) elements of 8 bytes)."""
     from minirecast import recon
     from minirecast.recon import cone_constants
     K = cone_constants(geom.vectors[views])
@@ -108,7 +110,7 @@ def band_lines_per_load(geom, box, planes, views, max_tiles=64, seed=0):
     groups = np.array(groups)
     if len(groups) > max_tiles * 16:
         groups = groups[np.sort(rng.choice(len(groups), max_tiles * 16, replace=False))]
-    counts = []
+    counts, quads = [], []
     for k in range(len(views)):
         rel = groups - K['s'][k]
         L = rel @ K['n'][k]
@@ -117,9 +119,12 @@ def band_lines_per_load(geom, box, planes, views, max_tiles=64, seed=0):
         row = np.clip(X @ K['vv'][k] + (R - 1) / 2.0, -1.0, R)
         b = (np.floor(row).astype(np.int64) + 1) // 3
         off = b * pitch * 8 + (np.floor(col).astype(np.int64) + 1) * 8
-        lines = np.sort(np.concatenate([off // 128, (off + 15) // 128], axis=1), axis=1)
+        l0, l1 = off // 128, (off + 15) // 128
+        lines = np.sort(np.concatenate([l0, l1], axis=1), axis=1)
         counts.append(1 + (np.diff(lines, axis=1) != 0).sum(axis=1))
-    return float(np.mean(np.concatenate(counts)))
+        ql = np.sort(np.concatenate([l0.reshape(-1, 16, 4), l1.reshape(-1, 16, 4)], axis=2), axis=2)
+        quads.append((1 + (np.diff(ql, axis=2) != 0).sum(axis=2)).sum(axis=1))
+    return float(np.mean(np.concatenate(counts))), float(np.mean(np.concatenate(quads)))
 
 
 def _pmc_pass(counter):
@@ -526,11 +531,19 @@ def bench_l2_gather(args):
         table[f"{nb >> 20} MiB"] = {str(L): round(ctx.l2_gather_bandwidth(L, nb)[0], 1)
                                     for L in lines}
         log(f"gather GB/s, buffer {nb >> 20} MiB: {table[f'{nb >> 20} MiB']}")
+    # runs of g consecutive lanes on one line (L lines per instruction either way): the
+    # cost of a load instruction as a function of the lines each group of lanes touches
+    groups = {}
+    for g in (1, 2, 4, 8):
+        groups[str(g)] = {str(L): round(ctx.l2_gather_bandwidth(L, 1 << 20, g)[0], 1)
+                          for L in (1, 2, 4, 8, 16, 32) if L * g <= 64}
+        log(f"gather GB/s, 1 MiB, runs of {g} lanes per line: {groups[str(g)]}")
     _line("GB/s of scattered 16-byte loads from an L2-resident buffer, 7 lines per load",
           table["1 MiB"]["7"], "GB/s", None, 4, 1,
           {"workload": "l2_gather: 16-byte buffer loads, L distinct 128-byte lines per "
                        "64-lane load instruction, 8 loads in flight per thread, "
-                       "8 workgroups of 256 threads per CU"}, {"gather_gbps": table})
+                       "8 workgroups of 256 threads per CU"},
+          {"gather_gbps": table, "gather_gbps_lane_runs": groups})
 
 
 def bench_diff_cone(args):
@@ -817,19 +830,29 @@ def main():
                     gather_bytes_l2=int(4 * esz * n_in * A))
     if traffic:
         roofline["traffic_detail"] = traffic
-    # the L2 side: every point-view is one 16-byte band-element load (16-bit forms); the
-    # gather microbenchmark measures how fast the L2s serve such loads with the kernel's
-    # number of distinct lines per load instruction (CPU emulation on a sample)
+    # the L2 side: every point-view is one 16-byte band-element load (16-bit forms).  A
+    # load instruction's cost follows the distinct lines of each lane quad (the gather
+    # microbenchmark: runs of 4 lanes per line serve 32-36 TB/s at 1-8 lines per
+    # instruction, runs of 2 ~29 TB/s, every quad on 4 lines ~9.6 TB/s), so the rate is
+    # interpolated in the kernel's quad lines (CPU emulation on a sample) between those
+    # patterns at its number of lines per instruction
     if head in ("f16", "bf16"):
-        lpl = band_lines_per_load(geom, box, w.planes, np.arange(0, A, max(1, A // 6)))
-        lo, hi = int(np.floor(lpl)), int(np.ceil(lpl))
-        g_lo, _ = ctx[head].l2_gather_bandwidth(lo)
-        g_hi, _ = ctx[head].l2_gather_bandwidth(hi) if hi != lo else (g_lo, 0.0)
-        gbps = g_lo + (g_hi - g_lo) * (lpl - lo)
+        lpl, lq = band_lines_per_load(geom, box, w.planes, np.arange(0, A, max(1, A // 6)))
+        L = int(min(16, max(1, round(lpl))))
+        pts_lq = [(16.0, ctx[head].l2_gather_bandwidth(L, 1 << 20, 4)[0])]
+        if L <= 32:
+            pts_lq.append((32.0, ctx[head].l2_gather_bandwidth(L, 1 << 20, 2)[0]))
+        if L >= 4:
+            pts_lq.append((64.0, ctx[head].l2_gather_bandwidth(L, 1 << 20, 1)[0]))
+        xs = np.array([x for x, _ in pts_lq])
+        inv = np.array([1.0 / g for _, g in pts_lq])     # time per byte, linear in quad lines
+        gbps = 1.0 / float(np.interp(lq, xs, inv))
         load_bytes = 16 * n_in * A
         t_l2 = load_bytes / (gbps * 1e9)
-        roofline["l2"] = dict(lines_per_load=round(lpl, 2), gather_gbps=round(gbps, 1),
-                              gather_gbps_at=[lo, round(g_lo, 1), hi, round(g_hi, 1)],
+        roofline["l2"] = dict(lines_per_load=round(lpl, 2), quad_lines_per_load=round(lq, 2),
+                              gather_gbps=round(gbps, 1),
+                              gather_gbps_at={f"{int(x)} quad lines, {L} lines": round(g, 1)
+                                              for x, g in pts_lq},
                               load_bytes=int(load_bytes), t_l2_ms=round(t_l2 * 1e3, 4),
                               t_hbm_ms=round(hbm_bytes / (HBM_PEAK_GBS * 1e9) * 1e3, 4),
                               t_kernel_ms=round(kern_s * 1e3, 4),

@@ -377,11 +377,12 @@ int mr_last_tile_kernel_launches(mr_ctx* ctx, int* n);
  * *gbps (1e9 bytes/s of loaded data) at which the device serves scattered 16-byte loads
  * from an L2-resident buffer of buffer_bytes (a power of two, 8 KiB..1 GiB), when the 64
  * lanes of each load instruction touch lines_per_wave (1..64) distinct 128-byte lines;
- * *ms (optional) the best of three timed runs after a warm-up.  It is the L2 side of the
- * back-projection roofline: the band-ring kernel issues one such 16-byte load per
- * point-view. */
-int mr_l2_gather_bandwidth(mr_ctx* ctx, int lines_per_wave, int64_t buffer_bytes, double* gbps,
-                           double* ms);
+ * lane_group 1: consecutive lanes on different lines; 2, 4 or 8: runs of that many
+ * consecutive lanes on one line (lines_per_wave x lane_group <= 64); *ms (optional) the
+ * best of three timed runs after a warm-up.  It is the L2 side of the back-projection
+ * roofline: the band-ring kernel issues one such 16-byte load per point-view. */
+int mr_l2_gather_bandwidth(mr_ctx* ctx, int lines_per_wave, int lane_group,
+                           int64_t buffer_bytes, double* gbps, double* ms);
 
 #ifdef __cplusplus
 }

@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
     bound = {s[0] for s in _hip.SIGNATURES}
     assert set(names) == bound, set(names) ^ bound
-    assert lib.mr_abi_version() == 1
+    assert lib.mr_abi_version() == 2
 
 
 def test_task_layout_matches_header():

@@ -12,7 +12,7 @@ namespace mr {
 
 namespace {
 thread_local std::string g_err;
-constexpr int kAbiVersion = 1;
+constexpr int kAbiVersion = 2;
 
 int fft_length(int n) {
     // 2 ** ceil(log2(2 n)) (recon.py:37): the smallest power of two >= 2n

@@ -11,19 +11,23 @@ namespace {
 constexpr int kProbeThreads = 256;
 constexpr int kProbeLoads = 8;          // independent loads in flight per thread
 
-// Lane l of instruction i reads the 16-byte element (line_i(l mod L) * 8 + slot(l)) of a
+// Lane l of instruction i reads the 16-byte element (line_i(x(l)) * 8 + slot(l)) of a
 // buffer of nlines 128-byte lines (a power of two, small enough to stay in L2): line_i(x)
-// is a hash of (i, x), so consecutive instructions touch unrelated lines, and slot(l) =
-// (l / L) mod 8 spreads the 64 / L lanes of a line over its eight 16-byte elements.
+// is a hash of (i, x), so consecutive instructions touch unrelated lines.  Group g = 1:
+// x(l) = l mod L and slot(l) = (l / L) mod 8 (consecutive lanes on different lines, the
+// 64 / L lanes of a line over its eight 16-byte elements); g > 1: x(l) = (l / g) mod L
+// and slot(l) = l mod 8 (runs of g consecutive lanes on one line).  Either way the 64
+// lanes touch L distinct lines.
 __global__ __launch_bounds__(kProbeThreads) void l2_gather_kernel(const uint4* __restrict__ buf,
                                                                    int nlines, int lines_per_wave,
-                                                                   int iters,
+                                                                   int group, int iters,
                                                                    uint4* __restrict__ out) {
     const int lane = threadIdx.x & 63;
     const int wave = (blockIdx.x * kProbeThreads + threadIdx.x) >> 6;
     const uint32_t mask = static_cast<uint32_t>(nlines - 1);
-    const uint32_t x = static_cast<uint32_t>(lane % lines_per_wave);
-    const uint32_t slot = static_cast<uint32_t>((lane / lines_per_wave) & 7);
+    const uint32_t x = static_cast<uint32_t>(group == 1 ? lane % lines_per_wave
+                                                        : (lane / group) % lines_per_wave);
+    const uint32_t slot = static_cast<uint32_t>(group == 1 ? (lane / lines_per_wave) & 7 : lane & 7);
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<uint4*>(buf), 0, nlines * 128, 0x00020000);
     uint4 acc = make_uint4(0u, 0u, 0u, 0u);
@@ -56,11 +60,14 @@ using namespace mr;
 
 struct mr_ctx : Ctx {};
 
-extern "C" int mr_l2_gather_bandwidth(mr_ctx* c, int lines_per_wave, int64_t buffer_bytes,
-                                      double* gbps, double* ms) {
+extern "C" int mr_l2_gather_bandwidth(mr_ctx* c, int lines_per_wave, int lane_group,
+                                      int64_t buffer_bytes, double* gbps, double* ms) {
     if (!c) return fail(MR_EVALUE, "null context");
     MR_CHECK(gbps != nullptr, MR_EVALUE, "null output");
     MR_CHECK(lines_per_wave >= 1 && lines_per_wave <= 64, MR_EVALUE, "lines per wave in 1..64");
+    MR_CHECK((lane_group == 1 || lane_group == 2 || lane_group == 4 || lane_group == 8) &&
+                 lines_per_wave * lane_group <= 64,
+             MR_EVALUE, "lane group 1, 2, 4 or 8 with lines per wave x group <= 64");
     MR_CHECK(buffer_bytes >= 128 * 64 && buffer_bytes <= (int64_t{1} << 30) &&
                  (buffer_bytes & (buffer_bytes - 1)) == 0,
              MR_EVALUE, "buffer bytes: a power of two in 8 KiB..1 GiB");
@@ -81,7 +88,7 @@ extern "C" int mr_l2_gather_bandwidth(mr_ctx* c, int lines_per_wave, int64_t buf
     for (int rep = 0; rep < 4 && e == hipSuccess; ++rep) {     // first run warms the L2s
         hipEventRecord(c->ev0, c->s_req);
         hipLaunchKernelGGL(l2_gather_kernel, dim3(blocks), dim3(kProbeThreads), 0, c->s_req,
-                           static_cast<const uint4*>(d_buf), nlines, lines_per_wave, iters,
+                           static_cast<const uint4*>(d_buf), nlines, lines_per_wave, lane_group, iters,
                            static_cast<uint4*>(d_out));
         hipEventRecord(c->ev1, c->s_req);
         e = hipStreamSynchronize(c->s_req);

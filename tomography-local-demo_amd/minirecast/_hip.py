@@ -83,7 +83,7 @@ SIGNATURES = [
     ("mr_memcpy", _c.c_int, [_P, _P, _P, _c.c_int64, _c.c_int]),
     ("mr_synchronize", _c.c_int, [_P]),
     ("mr_last_kernel_ms", _c.c_int, [_P, _c.POINTER(_c.c_double)]),
-    ("mr_l2_gather_bandwidth", _c.c_int, [_P, _c.c_int, _c.c_int64, _c.POINTER(_c.c_double),
+    ("mr_l2_gather_bandwidth", _c.c_int, [_P, _c.c_int, _c.c_int, _c.c_int64, _c.POINTER(_c.c_double),
                                           _c.POINTER(_c.c_double)]),
 ]
 
@@ -471,13 +471,15 @@ class Context:
         check(self.lib.mr_last_tile_kernel_ms(self.h, ctypes.byref(ms)))
         return ms.value
 
-    def l2_gather_bandwidth(self, lines_per_wave, buffer_bytes=1 << 20):
+    def l2_gather_bandwidth(self, lines_per_wave, buffer_bytes=1 << 20, lane_group=1):
         """(GB/s, ms) of scattered 16-byte loads served from an L2-resident buffer, with
-        lines_per_wave distinct 128-byte lines per load instruction."""
+        lines_per_wave distinct 128-byte lines per load instruction (lane_group > 1: runs
+        of that many consecutive lanes on one line)."""
         g = ctypes.c_double(0.0)
         ms = ctypes.c_double(0.0)
-        check(self.lib.mr_l2_gather_bandwidth(self.h, int(lines_per_wave), int(buffer_bytes),
-                                              ctypes.byref(g), ctypes.byref(ms)))
+        check(self.lib.mr_l2_gather_bandwidth(self.h, int(lines_per_wave), int(lane_group),
+                                              int(buffer_bytes), ctypes.byref(g),
+                                              ctypes.byref(ms)))
         return g.value, ms.value
 
     def last_tile_launches(self):
