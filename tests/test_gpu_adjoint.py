@@ -185,3 +185,65 @@ def test_autograd_end_to_end(setup):
         mean_g += float((ds * wts).sum()) / M
     fd3 = (loss64(0.25 + h) - loss64(0.25 - h)) / (2 * h)
     assert abs(fd3 - mean_g) <= 1e-2 * abs(fd3), (fd3, mean_g)
+
+
+def test_adjoint_gather_and_scatter_paths_agree(setup):
+    """Requests of up to 4096 point tiles run the adjoint as a gather (one workgroup per
+    detector tile and view, sums in LDS), larger ones as a scatter with atomics.  A stack
+    of 65 planes of 256^2 points (4160 tiles) takes the scatter; its two halves (33 and
+    32 planes) take the gather; the adjoint is linear, so the halves sum to the whole
+    (two float32 evaluations of the same sums), and the dot-product test holds for
+    both."""
+    from minirecast import _hip, recon
+    g, box, planes, tasks, out_len, ctx = setup
+    print("[adjoint] gather vs scatter, 65 planes of 256^2", flush=True)
+    A, R, C = g.n_angles, g.rows, g.cols
+    rng = np.random.default_rng(5)
+    q = rng.normal(size=(A, R, C)).astype(np.float32)
+    ctx.reset_views()
+    ctx.store(np.arange(A), q)
+    h = 1.6 * float(np.max(box)) / 256
+    origin = -0.8 * np.asarray(box, float)
+    dims = (256, 256, 65)
+    full = recon.volume_task(origin, h, dims, 0, 65)
+    lo = recon.volume_task(origin, h, dims, 0, 33)
+    hi = recon.volume_task(origin, h, dims, 33, 65)
+    n = int(np.prod(dims))
+    w = rng.normal(size=n).astype(np.float32)
+    T = lambda t: np.array([t], dtype=_hip.TASK_DTYPE)
+    g_full = ctx.bp_tasks_adjoint(T(full), A, w, R, C)            # scatter
+    g_lo = ctx.bp_tasks_adjoint(T(lo), A, w, R, C)                # gather
+    g_hi = ctx.bp_tasks_adjoint(T(hi), A, w, R, C)                # gather
+    assert rel_l2(g_lo + g_hi, g_full) < 1e-5
+    for t, gq in ((full, g_full), (lo, g_lo)):
+        y = ctx.bp_tasks(T(t), A, n)
+        lhs = float(np.dot(y.astype(np.float64), w))
+        rhs = float(np.sum(q.astype(np.float64) * gq))
+        assert abs(lhs - rhs) <= 1e-5 * np.sqrt(np.sum(y.astype(float) ** 2) * np.sum(w.astype(float) ** 2))
+
+
+def test_adjoint_non_finite_weights_propagate(setup):
+    """A weight of +inf reaches every detector cell its point taps (as float sums do);
+    the workgroups that meet no such point keep their fixed-point sums, equal to the
+    float64 adjoint."""
+    g, box, planes, tasks, out_len, ctx = setup
+    A, R, C = g.n_angles, g.rows, g.cols
+    rng = np.random.default_rng(9)
+    ctx.reset_views()
+    ctx.store(np.arange(A), rng.normal(size=(A, R, C)).astype(np.float32))
+    pts, mask = _plane_points(box, planes)
+    w = rng.normal(size=out_len).astype(np.float32)
+    k = int(np.flatnonzero(mask)[len(np.flatnonzero(mask)) // 2])
+    w[k] = np.inf
+    gq = ctx.bp_tasks_adjoint(tasks, A, w, R, C)
+    K = O.cone_constants(g.vectors)
+    with np.errstate(invalid="ignore", over="ignore"):
+        want = O.bp_cone_adjoint(w.astype(np.float64)[mask], K['s'], K['d'], K['uu'], K['vv'],
+                                 K['n'], K['sdd'], K['sod'], pts[mask], np.pi / A, R, C)
+    bad = ~np.isfinite(want)
+    assert bad.any()
+    np.testing.assert_array_equal(~np.isfinite(gq), bad)
+    # views whose every cell is finite (the inf point projects off the detector in none of
+    # them would be rare): compare those cells
+    ok = np.isfinite(want)
+    assert rel_l2(gq[ok], want[ok]) < 1e-5

@@ -552,8 +552,8 @@ __global__ __launch_bounds__(256) void records_kernel(const ViewGeom* __restrict
     double xt[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) xt[d] = T.o[d] + tl.q * T.c[d] + p0 * T.a[d] + s0 * T.b[d];
-    make_record(geom + v, vscale[v], T, xt, R, C, pitch, cone != 0, recs + gid * BP_REC, nullptr,
-                band != 0);
+    make_record(geom + v, vscale ? vscale[v] : 1.0f, T, xt, R, C, pitch, cone != 0,
+                recs + gid * BP_REC, nullptr, band != 0);
 }
 
 // RING: 0 the ring itself, 1 the row-pair ring, 2 the band ring (16-bit forms).
@@ -1123,8 +1123,8 @@ __global__ __launch_bounds__(BP_THREADS) void bp_tiles_kernel5(
 template <bool CONE>
 __global__ __launch_bounds__(BP_THREADS) void bp_adjoint_kernel(
     int pitch, int R, int C, const ViewGeom* __restrict__ geom, const mr_task* __restrict__ tasks,
-    const Tile* __restrict__ tiles, int n_views, float scale, const float* __restrict__ g,
-    float* __restrict__ gq) {
+    const Tile* __restrict__ tiles, int n_views, int vch, float scale,
+    const float* __restrict__ g, float* __restrict__ gq) {
     __shared__ float4 rec4[BP_VPASS * BP_REC / 4];
     float* rec = reinterpret_cast<float*>(rec4);
     const Tile tl = tiles[blockIdx.x];
@@ -1149,8 +1149,10 @@ __global__ __launch_bounds__(BP_THREADS) void bp_adjoint_kernel(
     }
     const float ds = static_cast<float>(tx), dp0 = static_cast<float>(ty);
     const int64_t vstride = static_cast<int64_t>(R) * C;
-    for (int vb = 0; vb < n_views; vb += BP_VPASS) {
-        const int nv = min(BP_VPASS, n_views - vb);
+    // workgroup (tile, y) takes views [y vch, (y + 1) vch)
+    const int v_end = min(n_views, (static_cast<int>(blockIdx.y) + 1) * vch);
+    for (int vb = static_cast<int>(blockIdx.y) * vch; vb < v_end; vb += BP_VPASS) {
+        const int nv = min(BP_VPASS, v_end - vb);
         __syncthreads();
         if (t < nv) make_record(geom + vb + t, 1.0f, T, xt, R, C, pitch, CONE, rec + t * BP_REC);
         __syncthreads();
@@ -1197,6 +1199,228 @@ __global__ __launch_bounds__(BP_THREADS) void bp_adjoint_kernel(
                 }
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------- adjoint as a gather
+// The transpose of the tile back-projection without global atomics: a workgroup owns one
+// ADJ_TH x ADJ_TW tile of one view's detector, accumulates in LDS the bilinear taps that
+// fall into it from every point tile whose detector box meets it, and writes the tile
+// once.  Per point the arithmetic is that of the scatter kernel above (the same FP32
+// records, weights and clamps).  The sums are fixed point: an LDS float atomic add costs
+// ~190 cycles per wave instruction on gfx950 against ~6 for a 64-bit integer add, so each
+// tap is rounded to an integer multiple of 2^-k (k from a bound on every contribution the
+// workgroup can receive, the contributions below 2^30 in those units) and added as a
+// 64-bit integer, exact and independent of order; the rounding is at most 2^-31 of that
+// bound per tap.  A workgroup whose bound is not finite (non-finite weights) adds floats.
+constexpr int ADJ_TW = 64, ADJ_TH = 64;    // detector tile of a workgroup
+constexpr int ADJ_LIST = 1024;             // point tiles tested per round
+constexpr int kAdjGatherMaxTiles = 4096;   // larger requests: the scatter kernel
+
+// gm[tile][p][s] (32 x 32 per tile) = g of the point times scale, 0 for points outside
+// the task or the box; gmax[tile] = max |gm| over the tile (+inf if any is not finite).
+__global__ __launch_bounds__(BP_THREADS) void adj_weights_kernel(
+    const mr_task* __restrict__ tasks, const Tile* __restrict__ tiles, float scale,
+    const float* __restrict__ g, float* __restrict__ gm, float* __restrict__ gmax) {
+    const Tile tl = tiles[blockIdx.x];
+    const mr_task& T = tasks[tl.task];
+    const int t = threadIdx.x, tx = t & (BP_TS - 1), ty = t / BP_TS;
+    constexpr int PSTEP = BP_THREADS / BP_TS;
+    constexpr int PPT = BP_TP / PSTEP;
+    const int p0 = tl.tp * BP_TP, s0 = tl.ts * BP_TS;
+    const int s = s0 + tx;
+    const int64_t obase = T.out_offset + static_cast<int64_t>(tl.q) * T.stride_q;
+    float* out = gm + static_cast<int64_t>(blockIdx.x) * (BP_TP * BP_TS);
+    float m = 0.0f;
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        const int p = p0 + ty + PSTEP * k;
+        const bool ok = (p < T.np) && (s < T.ns) && (!T.mask_box || in_box(T, p, s));
+        const float x =
+            ok ? g[obase + static_cast<int64_t>(p) * T.stride_p + static_cast<int64_t>(s) * T.stride_s] * scale
+               : 0.0f;
+        out[(ty + PSTEP * k) * BP_TS + tx] = x;
+        m = isfinite(x) ? fmaxf(m, fabsf(x)) : INFINITY;
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    __shared__ float red[BP_THREADS / 64];
+    if ((t & 63) == 0) red[t >> 6] = m;
+    __syncthreads();
+    if (t == 0) {
+        float r = red[0];
+        for (int w = 1; w < BP_THREADS / 64; ++w) r = fmaxf(r, red[w]);
+        gmax[blockIdx.x] = r;
+    }
+}
+
+// Absolute detector columns and rows of one point of a tile from its (tile, view) record,
+// clamped as the back-projection clamps them; wt the FDK weight.
+template <bool CONE>
+__device__ __forceinline__ void adj_point(const float4& r0, const float4& r1, const float4& r2,
+                                          const float4& r3, float dp, float ds, float& col,
+                                          float& row, float& wt) {
+    const float uh = fmaf(dp, r0.z, fmaf(ds, r0.y, r0.x));
+    const float vh = fmaf(dp, r1.y, fmaf(ds, r1.x, r0.w));
+    if (CONE) {
+        const float wh = fmaf(dp, r2.x, fmaf(ds, r1.w, r1.z));
+        const float rw = __builtin_amdgcn_rcpf(wh);
+        col = uh * rw; row = vh * rw; wt = rw * rw;
+    } else {
+        col = uh; row = vh; wt = 1.0f;
+    }
+    col = __builtin_amdgcn_fmed3f(col, r2.y, r2.z);
+    row = __builtin_amdgcn_fmed3f(row, r2.w, r3.x);
+}
+
+// boxes[view][tile] = (c_lo, c_hi, r_lo, r_hi): the detector cells (absolute, inclusive)
+// the taps of the tile's points can reach in that view, from the images of its corners
+// (a projective map takes the tile to the convex hull of those), one cell of margin for
+// rounding; bound[view][tile] >= every tap weight the tile adds in that view (max |gm|
+// times the largest FDK weight 1/wh^2, at a corner since wh is affine over the tile).
+// The whole detector and an infinite bound when a corner lies at or behind the source.
+template <bool CONE>
+__global__ __launch_bounds__(256) void adj_boxes_kernel(const float* __restrict__ recs,
+                                                        const float* __restrict__ gmax,
+                                                        int ntiles, int n_views, int R, int C,
+                                                        int4* __restrict__ boxes,
+                                                        float* __restrict__ bound) {
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (gid >= static_cast<int64_t>(ntiles) * n_views) return;
+    const int ti = static_cast<int>(gid / n_views);
+    const int v = static_cast<int>(gid - static_cast<int64_t>(ti) * n_views);
+    const float4* r4 = reinterpret_cast<const float4*>(recs + gid * BP_REC);
+    const float4 r0 = r4[0], r1 = r4[1], r2 = r4[2], r3 = r4[3];
+    const int rref = static_cast<int>(-1.0f - r2.w);
+    const int cref = static_cast<int>(-1.0f - r2.y);
+    float cmin = 1e30f, cmax = -1e30f, rmin = 1e30f, rmax = -1e30f, whmin = 1e30f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float dp = (k & 1) ? static_cast<float>(BP_TP - 1) : 0.0f;
+        const float ds = (k & 2) ? static_cast<float>(BP_TS - 1) : 0.0f;
+        if (CONE) whmin = fminf(whmin, fmaf(dp, r2.x, fmaf(ds, r1.w, r1.z)));
+        float col, row, wt;
+        adj_point<CONE>(r0, r1, r2, r3, dp, ds, col, row, wt);
+        cmin = fminf(cmin, col); cmax = fmaxf(cmax, col);
+        rmin = fminf(rmin, row); rmax = fmaxf(rmax, row);
+    }
+    const bool ok = (!CONE || whmin > 0.0f) && cmin == cmin && cmax == cmax && rmin == rmin &&
+                    rmax == rmax;
+    int4 b;
+    float bw = INFINITY;
+    if (ok) {
+        b.x = cvt_flr(cmin) + cref - 1;
+        b.y = cvt_flr(cmax) + cref + 2;
+        b.z = cvt_flr(rmin) + rref - 1;
+        b.w = cvt_flr(rmax) + rref + 2;
+        const float rw = CONE ? 1.0f / whmin : 1.0f;
+        bw = gmax[ti] * (rw * rw) * 1.001f;          // margin for the FP32 rounding of wt
+    } else {
+        b = make_int4(0, C - 1, 0, R - 1);
+    }
+    const int64_t o = static_cast<int64_t>(v) * ntiles + ti;
+    boxes[o] = b;
+    bound[o] = bw;
+}
+
+__device__ __forceinline__ bool adj_meets(const int4& b, int X0, int Y0) {
+    return (b.x < X0 + ADJ_TW) & (b.y >= X0) & (b.z < Y0 + ADJ_TH) & (b.w >= Y0);
+}
+
+// Workgroup (v, y, x): detector cells rows [y ADJ_TH, ..), columns [x ADJ_TW, ..) of view v.
+template <bool CONE>
+__global__ __launch_bounds__(BP_THREADS) void adj_gather_kernel(
+    const float* __restrict__ recs, const int4* __restrict__ boxes,
+    const float* __restrict__ bound, const float* __restrict__ gm, int ntiles, int n_views,
+    int R, int C, float* __restrict__ gq) {
+    __shared__ unsigned long long acc[ADJ_TH * ADJ_TW];   // fixed point, or floats in place
+    __shared__ int list[ADJ_LIST];
+    __shared__ int nlist;
+    __shared__ unsigned bmax;
+    float* const accf = reinterpret_cast<float*>(acc);
+    const int v = blockIdx.x;
+    const int Y0 = blockIdx.y * ADJ_TH, X0 = blockIdx.z * ADJ_TW;
+    const int t = threadIdx.x, tx = t & (BP_TS - 1), ty = t / BP_TS;
+    constexpr int PSTEP = BP_THREADS / BP_TS;
+    constexpr int PPT = BP_TP / PSTEP;
+    const int4* bv = boxes + static_cast<int64_t>(v) * ntiles;
+    const float* bdv = bound + static_cast<int64_t>(v) * ntiles;
+    for (int i = t; i < ADJ_TH * ADJ_TW; i += BP_THREADS) acc[i] = 0ull;
+    if (t == 0) bmax = 0u;
+    __syncthreads();
+    // the largest contribution any tile meeting this detector tile can make
+    unsigned mb = 0u;
+    for (int i = t; i < ntiles; i += BP_THREADS) {
+        const int4 b = bv[i];
+        if (adj_meets(b, X0, Y0)) mb = max(mb, __float_as_uint(bdv[i]));   // bounds are >= 0
+    }
+    atomicMax(&bmax, mb);
+    __syncthreads();
+    const float B = __uint_as_float(bmax);
+    bool fixed = B < INFINITY;                        // false: non-finite bound (or NaN)
+    float fs = 1.0f;                                  // fixed-point scale 2^(30 - e), B < 2^e
+    if (fixed && B > 0.0f) {
+        int e;
+        frexpf(B, &e);
+        fixed = e >= -90 && e <= 150;                 // the scale stays a normal float
+        fs = fixed ? ldexpf(1.0f, 30 - e) : 1.0f;
+    }
+    for (int base = 0; base < ntiles; base += ADJ_LIST) {
+        if (t == 0) nlist = 0;
+        __syncthreads();
+        const int nb = min(ADJ_LIST, ntiles - base);
+        for (int i = t; i < nb; i += BP_THREADS) {
+            const int4 b = bv[base + i];
+            if (adj_meets(b, X0, Y0)) list[atomicAdd(&nlist, 1)] = base + i;
+        }
+        __syncthreads();
+        const int n = nlist;
+        for (int e = 0; e < n; ++e) {
+            const int ti = list[e];
+            const float4* r4 = reinterpret_cast<const float4*>(
+                recs + (static_cast<int64_t>(ti) * n_views + v) * BP_REC);
+            const float4 r0 = r4[0], r1 = r4[1], r2 = r4[2], r3 = r4[3];
+            const int rref = static_cast<int>(-1.0f - r2.w);
+            const int cref = static_cast<int>(-1.0f - r2.y);
+            const float* gw = gm + static_cast<int64_t>(ti) * (BP_TP * BP_TS);
+            float gp[PPT];
+#pragma unroll
+            for (int m = 0; m < PPT; ++m) gp[m] = gw[(ty + PSTEP * m) * BP_TS + tx];
+#pragma unroll
+            for (int m = 0; m < PPT; ++m) {
+                if (gp[m] == 0.0f) continue;
+                float col, row, wt;
+                adj_point<CONE>(r0, r1, r2, r3, static_cast<float>(ty + PSTEP * m),
+                                static_cast<float>(tx), col, row, wt);
+                const float fc = __builtin_amdgcn_fractf(col), fr = __builtin_amdgcn_fractf(row);
+                const int c0 = cvt_flr(col) + cref, rr0 = cvt_flr(row) + rref;   // absolute
+                const float sv = wt * gp[m];
+                const float w[4] = {(1.0f - fr) * (1.0f - fc) * sv, (1.0f - fr) * fc * sv,
+                                    fr * (1.0f - fc) * sv, fr * fc * sv};
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int rr = rr0 + (k >> 1), cc = c0 + (k & 1);
+                    const int ly = rr - Y0, lx = cc - X0;
+                    if (ly >= 0 && ly < ADJ_TH && lx >= 0 && lx < ADJ_TW && rr < R && cc < C) {
+                        if (fixed) {
+                            const long long q = __float2int_rn(w[k] * fs);
+                            atomicAdd(&acc[ly * ADJ_TW + lx], static_cast<unsigned long long>(q));
+                        } else {
+                            atomicAdd(&accf[ly * ADJ_TW + lx], w[k]);
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    float* gv = gq + static_cast<int64_t>(v) * R * C;
+    const double inv = 1.0 / static_cast<double>(fs);
+    for (int i = t; i < ADJ_TH * ADJ_TW; i += BP_THREADS) {
+        const int y = Y0 + i / ADJ_TW, x = X0 + i % ADJ_TW;
+        if (y < R && x < C)
+            gv[static_cast<int64_t>(y) * C + x] =
+                fixed ? static_cast<float>(static_cast<double>(static_cast<long long>(acc[i])) * inv)
+                      : accf[i];
     }
 }
 
@@ -1997,14 +2221,60 @@ int launch_bp_tiles_v4(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int 
 
 int launch_bp_adjoint(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles,
                       int n_views, float scale, const float* d_g, float* d_gq, hipStream_t s) {
-    if (ntiles <= 0) return MR_OK;
-    dim3 grid(ntiles), block(BP_THREADS);
+    if (ntiles <= 0 || n_views <= 0) return MR_OK;
+    const bool cone = c->kind == MR_KIND_CONE;
+    const size_t rec_bytes = sizeof(float) * BP_REC * static_cast<size_t>(ntiles) * n_views;
+    if (ntiles <= kAdjGatherMaxTiles && rec_bytes <= c->rec_budget) {
+        // the gather: records of every (tile, view), the masked weights, the detector box
+        // of every (view, tile), then one workgroup per detector tile and view
+        int rc = ensure(reinterpret_cast<void**>(&c->d_recs), &c->recs_bytes, rec_bytes);
+        if (rc) return rc;
+        auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+        const size_t gm_bytes = up(sizeof(float) * BP_TP * BP_TS * static_cast<size_t>(ntiles));
+        const size_t gmax_bytes = up(sizeof(float) * static_cast<size_t>(ntiles));
+        const size_t box_bytes = up(sizeof(int4) * static_cast<size_t>(ntiles) * n_views);
+        const size_t bnd_bytes = up(sizeof(float) * static_cast<size_t>(ntiles) * n_views);
+        rc = ensure(&c->d_adj, &c->adj_bytes, gm_bytes + gmax_bytes + box_bytes + bnd_bytes);
+        if (rc) return rc;
+        char* base = static_cast<char*>(c->d_adj);
+        float* gm = reinterpret_cast<float*>(base);
+        float* gmax = reinterpret_cast<float*>(base + gm_bytes);
+        int4* boxes = reinterpret_cast<int4*>(base + gm_bytes + gmax_bytes);
+        float* bound = reinterpret_cast<float*>(base + gm_bytes + gmax_bytes + box_bytes);
+        const int64_t nrec = static_cast<int64_t>(ntiles) * n_views;
+        const dim3 rgrid(static_cast<unsigned>((nrec + 255) / 256));
+        hipLaunchKernelGGL(records_kernel, rgrid, dim3(256), 0, s, c->d_geom, nullptr, d_tasks,
+                           d_tiles, ntiles, n_views, c->R, c->C, c->pitch, cone ? 1 : 0, c->d_recs, 0);
+        hipLaunchKernelGGL(adj_weights_kernel, dim3(ntiles), dim3(BP_THREADS), 0, s, d_tasks, d_tiles,
+                           scale, d_g, gm, gmax);
+        const dim3 ggrid(n_views, (c->R + ADJ_TH - 1) / ADJ_TH, (c->C + ADJ_TW - 1) / ADJ_TW);
+        if (cone) {
+            hipLaunchKernelGGL(adj_boxes_kernel<true>, rgrid, dim3(256), 0, s, c->d_recs, gmax,
+                               ntiles, n_views, c->R, c->C, boxes, bound);
+            hipLaunchKernelGGL(adj_gather_kernel<true>, ggrid, dim3(BP_THREADS), 0, s, c->d_recs,
+                               boxes, bound, gm, ntiles, n_views, c->R, c->C, d_gq);
+        } else {
+            hipLaunchKernelGGL(adj_boxes_kernel<false>, rgrid, dim3(256), 0, s, c->d_recs, gmax,
+                               ntiles, n_views, c->R, c->C, boxes, bound);
+            hipLaunchKernelGGL(adj_gather_kernel<false>, ggrid, dim3(BP_THREADS), 0, s, c->d_recs,
+                               boxes, bound, gm, ntiles, n_views, c->R, c->C, d_gq);
+        }
+        MR_HIP(hipGetLastError());
+        return MR_OK;
+    }
+    // many tiles (every detector workgroup would test every tile's box) or records beyond
+    // the buffer: the scatter, with the views split over workgroups too (about 16
+    // workgroups per CU): the scatters of different views go to different projections,
+    // so their atomics rarely meet
+    const int chunks = std::max(1, std::min(n_views, (16 * c->n_cus + ntiles - 1) / ntiles));
+    const int vch = (n_views + chunks - 1) / chunks;
+    dim3 grid(ntiles, (n_views + vch - 1) / vch), block(BP_THREADS);
     if (c->kind == MR_KIND_CONE)
         hipLaunchKernelGGL(bp_adjoint_kernel<true>, grid, block, 0, s, c->pitch, c->R, c->C,
-                           c->d_geom, d_tasks, d_tiles, n_views, scale, d_g, d_gq);
+                           c->d_geom, d_tasks, d_tiles, n_views, vch, scale, d_g, d_gq);
     else
         hipLaunchKernelGGL(bp_adjoint_kernel<false>, grid, block, 0, s, c->pitch, c->R, c->C,
-                           c->d_geom, d_tasks, d_tiles, n_views, scale, d_g, d_gq);
+                           c->d_geom, d_tasks, d_tiles, n_views, vch, scale, d_g, d_gq);
     MR_HIP(hipGetLastError());
     return MR_OK;
 }

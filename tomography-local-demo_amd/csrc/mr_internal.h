@@ -145,6 +145,8 @@ struct Ctx {
     size_t stat_bytes = 0;
     float* d_recs = nullptr;         // v4 per-(tile, view) records
     size_t recs_bytes = 0;
+    void* d_adj = nullptr;           // adjoint gather: masked weights + detector boxes
+    size_t adj_bytes = 0;
 
     int kind = MR_KIND_NONE;
     int n_total = 0, R = 0, C = 0;
