@@ -373,6 +373,13 @@ int mr_last_tile_kernel_ms(mr_ctx* ctx, double* ms);
  * only). */
 int mr_last_tile_kernel_launches(mr_ctx* ctx, int* n);
 
+/* The back-projection kernel of the last mr_bp_tasks: 2 (records in LDS, the work-queue
+ * loop), 4, 5, 6 or 7 (the plane-tile kernels), 8 (the z-column kernel of volume
+ * requests: a cone geometry whose detector column and depth do not depend on z, tasks
+ * stacking planes along z), 0 when nothing was launched.  No counterpart in the reference
+ * (introspection only). */
+int mr_last_bp_kernel(mr_ctx* ctx, int* kernel);
+
 /* L2 gather microbenchmark (no counterpart in the reference; measurement only): the rate
  * *gbps (1e9 bytes/s of loaded data) at which the device serves scattered 16-byte loads
  * from an L2-resident buffer of buffer_bytes (a power of two, 8 KiB..1 GiB), when the 64

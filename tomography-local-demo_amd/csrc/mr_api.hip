@@ -204,21 +204,11 @@ int validate_tasks(const mr_task* tasks, int ntasks, int64_t out_len) {
 
 enum TaskOp { OP_BP = 0, OP_ADJOINT = 1, OP_DSHIFT = 2 };
 
-// Uploads tasks + tiles and launches the tile kernel of `op` (device buffers), timing it.
-// For OP_BP and OP_DSHIFT every output element of every task is written: the tiles
-// culled by the box test are filled with zeros by their own small launch.
-int run_tasks(Ctx* c, const mr_task* tasks, int ntasks, int n_views, float* d_out,
-              int op = OP_BP, const float* d_g = nullptr) {
-    // the tile list depends only on the tasks (the culled-tile list and the block order
-    // on the operation and the XCD map): a request repeating one of the last few (a
-    // viewer re-requesting its slices as views arrive, one call per plane) reuses it
-    // instead of enumerating and sorting again (~0.1 ms on the host)
+// Copies the tasks and a tile list to the device request buffer (through the pinned
+// staging buffer, on the request stream).
+int upload_request(Ctx* c, const mr_task* tasks, int ntasks, const std::vector<Tile>& tiles,
+                   const mr_task** d_tasks, const Tile** d_tiles) {
     const size_t tb = sizeof(mr_task) * ntasks;
-    const int key_flags = (op == OP_ADJOINT ? 1 : 0) | (c->xcd_map << 1);
-    const TileCache& entry = cached_tiles(c->tile_cache, tasks, ntasks, key_flags,
-                                          op != OP_ADJOINT, c->xcd_map == 2);
-    const std::vector<Tile>& tiles = entry.tiles;
-    const size_t ne = entry.ne;
     const size_t tl = sizeof(Tile) * tiles.size();
     const size_t off_tiles = (tb + 255) / 256 * 256;
     const size_t need = off_tiles + tl + 256;
@@ -229,8 +219,72 @@ int run_tasks(Ctx* c, const mr_task* tasks, int ntasks, int n_views, float* d_ou
     std::memcpy(c->h_req, tasks, tb);
     if (tl) std::memcpy(static_cast<char*>(c->h_req) + off_tiles, tiles.data(), tl);
     MR_HIP(hipMemcpyAsync(c->d_req, c->h_req, off_tiles + tl, hipMemcpyHostToDevice, c->s_req));
-    const mr_task* d_tasks = static_cast<const mr_task*>(c->d_req);
-    const Tile* d_tiles = reinterpret_cast<const Tile*>(static_cast<char*>(c->d_req) + off_tiles);
+    *d_tasks = static_cast<const mr_task*>(c->d_req);
+    *d_tiles = reinterpret_cast<const Tile*>(static_cast<char*>(c->d_req) + off_tiles);
+    return MR_OK;
+}
+
+// Volume requests of a cone geometry whose views have no z term in P[0] and P[2]
+// (Ctx::zsep), in the default configuration, run the z-column kernel: every task a stack
+// of at least half a chunk of planes along z (c = (0, 0, h), no in-box mask), enough
+// column tiles to fill the device, and byte offsets of the ring (or its row-pair copy)
+// within the kernel's 32-bit and 24-bit products.
+bool zcol_request(const Ctx* c, const mr_task* tasks, int ntasks, int n_views) {
+    if (c->kind != MR_KIND_CONE || !c->zsep || c->bp_explicit || n_views <= 0) return false;
+    if (4 * static_cast<int64_t>(c->R + 3) * c->pitch >= (int64_t{1} << 31) ||
+        (static_cast<int64_t>(c->pitch) << 3) >= (int64_t{1} << 23))
+        return false;
+    int64_t ntiles = 0;
+    for (int i = 0; i < ntasks; ++i) {
+        const mr_task& T = tasks[i];
+        if (T.nq < ZQ / 2 || T.mask_box || T.c[0] != 0.0 || T.c[1] != 0.0 || T.c[2] == 0.0)
+            return false;
+        ntiles += static_cast<int64_t>((T.nq + ZQ - 1) / ZQ) * ((T.np + ZT - 1) / ZT) *
+                  ((T.ns + ZT - 1) / ZT);
+    }
+    return ntiles >= 4 * c->n_cus && ntiles < (int64_t{1} << 31);
+}
+
+// The z-column path of a volume request (zcol_request).
+int run_zcol(Ctx* c, const mr_task* tasks, int ntasks, int n_views, float* d_out) {
+    std::vector<Tile> tiles;
+    build_zcol_tiles(tasks, ntasks, tiles);
+    const mr_task* d_tasks = nullptr;
+    const Tile* d_tiles = nullptr;
+    int rc = upload_request(c, tasks, ntasks, tiles, &d_tasks, &d_tiles);
+    if (rc) return rc;
+    const float scale = static_cast<float>(M_PI / n_views);
+    c->tile_timed = false;
+    c->tile_launches = 0;
+    c->req_slices = false;
+    MR_HIP(hipEventRecord(c->ev0, c->s_req));
+    rc = launch_bp_zcol(c, d_tasks, ntasks, d_tiles, static_cast<int>(tiles.size()), n_views, scale,
+                        d_out, c->s_req);
+    if (rc) return rc;
+    MR_HIP(hipEventRecord(c->ev1, c->s_req));
+    return MR_OK;
+}
+
+// Uploads tasks + tiles and launches the tile kernel of `op` (device buffers), timing it.
+// For OP_BP and OP_DSHIFT every output element of every task is written: the tiles
+// culled by the box test are filled with zeros by their own small launch.
+int run_tasks(Ctx* c, const mr_task* tasks, int ntasks, int n_views, float* d_out,
+              int op = OP_BP, const float* d_g = nullptr) {
+    if (op == OP_BP && zcol_request(c, tasks, ntasks, n_views))
+        return run_zcol(c, tasks, ntasks, n_views, d_out);
+    // the tile list depends only on the tasks (the culled-tile list and the block order
+    // on the operation and the XCD map): a request repeating one of the last few (a
+    // viewer re-requesting its slices as views arrive, one call per plane) reuses it
+    // instead of enumerating and sorting again (~0.1 ms on the host)
+    const int key_flags = (op == OP_ADJOINT ? 1 : 0) | (c->xcd_map << 1);
+    const TileCache& entry = cached_tiles(c->tile_cache, tasks, ntasks, key_flags,
+                                          op != OP_ADJOINT, c->xcd_map == 2);
+    const std::vector<Tile>& tiles = entry.tiles;
+    const size_t ne = entry.ne;
+    const mr_task* d_tasks = nullptr;
+    const Tile* d_tiles = nullptr;
+    int rc = upload_request(c, tasks, ntasks, tiles, &d_tasks, &d_tiles);
+    if (rc) return rc;
     const float scale = n_views > 0 ? static_cast<float>(M_PI / n_views) : 0.0f;
     const int nt = static_cast<int>(tiles.size() - ne);
     if (ne) {
@@ -260,6 +314,8 @@ int finish_timing(Ctx* c) {
     c->last_kernel_ms = ms;
     c->last_tile_ms = ms;
     c->last_tile_launches = c->tile_launches;
+    c->last_kern = c->kern_used;
+    c->kern_used = 0;
     if (c->tile_timed) {
         MR_HIP(hipEventElapsedTime(&ms, c->evk0, c->evk1));
         c->last_tile_ms = ms;
@@ -505,6 +561,8 @@ int mr_set_geometry_cone(mr_ctx* c, int n_total, int rows, int cols, const doubl
         free_geometry(c);
         return rc;
     }
+    c->zsep = true;
+    for (const ViewGeom& g : c->geom_all) c->zsep = c->zsep && g.P[0][2] == 0.0 && g.P[2][2] == 0.0;
     c->kind = MR_KIND_CONE;
     return MR_OK;
 }
@@ -550,6 +608,7 @@ int mr_set_geometry_parallel(mr_ctx* c, int n_total, int rows, int cols, const d
         free_geometry(c);
         return rc;
     }
+    c->zsep = false;
     c->kind = MR_KIND_PARALLEL;
     // measured default for parallel beam: kernel 6, the row-pair copy (a point keeps one
     // detector row pair in every view), 4 points per thread, 2 views in flight.  A COR
@@ -1859,6 +1918,14 @@ int mr_last_tile_kernel_launches(mr_ctx* c, int* n) {
     if (rc) return rc;
     MR_CHECK(n != nullptr, MR_EVALUE, "null output");
     *n = c->last_tile_launches;
+    return MR_OK;
+}
+
+int mr_last_bp_kernel(mr_ctx* c, int* kernel) {
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    MR_CHECK(kernel != nullptr, MR_EVALUE, "null output");
+    *kernel = c->last_kern;
     return MR_OK;
 }
 

@@ -131,23 +131,43 @@ __device__ inline int cvt_flr(float x) {
     asm volatile("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
     return r;
 }
-// Row r (relative to a band start, |r| < 2^22) -> band b = floor(r / 3) and 16 (r - 3 b),
-// the shift that selects rows r, r + 1 of the band's four.  (r + 0.5) / 3 lies at least
-// 1/6 from an integer, so its float floor is exact.
-__device__ inline void band_split(int r, int& b, int& jsh16) {
-    b = cvt_flr(fmaf(static_cast<float>(r), 1.0f / 3.0f, 1.0f / 6.0f));
-    jsh16 = __mul24(b, -48) + (r << 4);
+// Band ring addressing.  Element (band b, column c) of a view holds the padded rows
+// 3b..3b+3 of column c (8 bytes, element c+1 follows); a band record's [14] is the byte
+// offset of (band k, reference column), its reference row placed so that padded row
+// ref + 1 = 3k.  A point r rows past the reference (r = floor(row)) and c columns past it
+// needs rows r, r+1 of band k + floor(r/3), i.e. rows j, j+1 (j = r mod 3) of element
+// (k + floor(r/3), c) and of element c+1: one 16-byte load at the element's 8-byte
+// aligned offset, then a 16j-bit shift of each column's 8 bytes (BandP).  With
+// q = floor((r + 3K)/3) = floor(r/3) + K (K = 2^20, so r + 3K >= 0 for every clamped
+// row), the byte offset is [14] + (q - K) pitchb + 8c, formed
+// modulo 2^32 by signed 24-bit multiply-adds (|c| < 2^23, q < 2^23, pitchb < 2^23 as
+// the band ring serves pitches up to 2^20 columns), and 16j = 16r - 48q modulo 64,
+// the bits a 64-bit shift reads (48K is a multiple of 64).
+constexpr int BAND_K = 1 << 20;
+// q = floor((r + 3K) / 3): 0x55555556 = (2^32 + 2) / 3, so the high word of the product
+// is exact for 0 <= r + 3K < 2^31
+__device__ inline int band_q(int r) {
+    return static_cast<int>(__umulhi(static_cast<uint32_t>(r + 3 * BAND_K), 0x55555556u));
 }
-// Byte offset b * pitchb + c * 8 + base of band-ring element (b, c) as two signed 24-bit
-// multiply-adds (b and c are relative to the record's reference and may be negative,
-// |b|, |c| < 2^23); the compiler would otherwise form a multiply, a shift and a
-// three-input add.
-__device__ inline int band_offset(int b, int c, int pitchb, int base) {
-    int x, r;
-    asm("v_mad_i32_i24 %0, %1, 8, %2" : "=v"(x) : "v"(c), "s"(base));
-    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(b), "s"(pitchb), "v"(x));
-    return r;
+// [14] - K pitchb, the per-view base of band_addr (wrapping 32-bit arithmetic)
+__device__ inline int band_base(int ref14, int pitchb) {
+    return static_cast<int>(static_cast<uint32_t>(ref14) -
+                            static_cast<uint32_t>(BAND_K) * static_cast<uint32_t>(pitchb));
 }
+// Byte offset of element (k + floor(r/3), c) for q = band_q(r), basek = band_base([14],
+// pitchb): two signed 24-bit multiply-adds (the compiler would otherwise form a
+// multiply, a shift and a three-input add).
+__device__ inline int band_addr(int q, int c, int pitchb, int basek) {
+    int x, o;
+    asm("v_mad_i32_i24 %0, %1, 8, %2" : "=v"(x) : "v"(c), "s"(basek));
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(o) : "v"(q), "s"(pitchb), "v"(x));
+    return o;
+}
+// 16 (r mod 3) modulo 64, the shift that selects rows r, r+1 of the element's four
+__device__ inline int band_shift(int q, int r) {
+    return __mul24(q, -48) + (r << 4);
+}
+
 
 template <typename ST> struct Taps;
 template <> struct Taps<float> {
@@ -301,10 +321,12 @@ template <> struct PairP<uint16_t> {
 // Taps from the band ring (16-bit forms): an element of band k holds the padded rows
 // 3k..3k+3 of one column (8 bytes), so the 2x2 stencil of every point lies in one band
 // and one buffer_load_dwordx4 fetches columns c and c+1; the two rows the point needs,
-// 3k+j and 3k+j+1, are picked at the lerp by a 16j-bit funnel shift.
+// 3k+j and 3k+j+1, are picked at the lerp by a 16j-bit funnel shift.  (A 12-byte load at
+// the 2-byte-aligned offset of the two rows would need no shift, but costs the texture
+// path far more: 5.6 against 3.2 ms for the full_cone f16 update.)
 template <typename ST> struct BandP {
     uint32_t d0, d1, d2, d3;           // (rows 0,1 | rows 2,3) of column c, then of c+1
-    uint32_t sh;                       // 16 j
+    uint32_t sh;                       // 16 j (modulo 64)
     __device__ inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int shift) {
         const uint4 x = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, 0, 0));
         d0 = x.x;
@@ -315,8 +337,8 @@ template <typename ST> struct BandP {
     }
     __device__ inline void lerp(float fc, float& top, float& bot) const {
         PairP<ST> p;
-        p.w0 = static_cast<uint32_t>(((static_cast<uint64_t>(d1) << 32) | d0) >> sh);
-        p.w1 = static_cast<uint32_t>(((static_cast<uint64_t>(d3) << 32) | d2) >> sh);
+        p.w0 = static_cast<uint32_t>(((static_cast<uint64_t>(d1) << 32) | d0) >> (sh & 63u));
+        p.w1 = static_cast<uint32_t>(((static_cast<uint64_t>(d3) << 32) | d2) >> (sh & 63u));
         p.lerp(fc, top, bot);
     }
 };
@@ -423,10 +445,9 @@ __global__ __launch_bounds__(BP_THREADS, 5) void bp_tiles_kernel2(
             row = __builtin_amdgcn_fmed3f(row, r2.w, r3.x);
             fr_par[m] = __builtin_amdgcn_fractf(row);
             if (BAND) {
-                int b, jsh16;
-                band_split(cvt_flr(row), b, jsh16);
-                roff_par[m] = __mul24(b, pitchb);
-                jsh_par[m] = jsh16;
+                const int r = cvt_flr(row), q = band_q(r);
+                roff_par[m] = __mul24(q, pitchb);
+                jsh_par[m] = band_shift(q, r);
             } else {
                 roff_par[m] = cvt_flr(row) * pitchb;
             }
@@ -446,7 +467,7 @@ __global__ __launch_bounds__(BP_THREADS, 5) void bp_tiles_kernel2(
             const float4 r1 = rec4[v * 4 + 1];
             const float4 r2 = rec4[v * 4 + 2];
             const float4 r3 = rec4[v * 4 + 3];
-            const int baseb = BAND ? __float_as_int(r3.z) : __float_as_int(r3.z) << ESH;
+            const int baseb = BAND ? band_base(__float_as_int(r3.z), pitchb) : __float_as_int(r3.z) << ESH;
             const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
                 const_cast<ST*>(ring + static_cast<int64_t>(vb + v) * view_stride), 0, view_bytes,
                 0x00020000);
@@ -468,10 +489,9 @@ __global__ __launch_bounds__(BP_THREADS, 5) void bp_tiles_kernel2(
                     const float fr = __builtin_amdgcn_fractf(row);
                     float top, bot;
                     if constexpr (BAND) {
-                        int b, jsh16;
-                        band_split(cvt_flr(row), b, jsh16);
+                        const int r = cvt_flr(row), q = band_q(r);
                         BandP<ST> tap;
-                        tap.load(rs, band_offset(b, cvt_flr(col), pitchb, baseb), jsh16);
+                        tap.load(rs, band_addr(q, cvt_flr(col), pitchb, baseb), band_shift(q, r));
                         tap.lerp(fc, top, bot);
                     } else if constexpr (RK == 1) {
                         PairP<ST> tap;
@@ -702,11 +722,11 @@ void bp_tiles_kernel4(
             row = __builtin_amdgcn_fmed3f(row, rt[11], rt[12]);
             frp[m] = __builtin_amdgcn_fractf(row);
             if (RING == 2) {
-                // band ring: band offset and 16 j of the row (the same for every view)
-                int b, jsh16;
-                band_split(cvt_flr(row), b, jsh16);
-                roffp[m] = __mul24(b, pitchb);
-                jsh[m] = jsh16;
+                // band ring: the row part of band_addr and the row shift (the same for
+                // every view)
+                const int r = cvt_flr(row), q = band_q(r);
+                roffp[m] = __mul24(q, pitchb);
+                jsh[m] = band_shift(q, r);
             } else {
                 roffp[m] = cvt_flr(row) * pitchb;
             }
@@ -726,7 +746,7 @@ void bp_tiles_kernel4(
         const float U0 = r[0], Ua = r[1], Ub = r[2];
         // band ring: [14] is the byte offset of the reference band and column
         const int ref14 = __float_as_int(r[14]);
-        const int baseb = RING == 2 ? ref14 : ref14 << SH;
+        const int baseb = RING == 2 ? band_base(ref14, pitchb) : ref14 << SH;
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<ST*>(ring + static_cast<int64_t>(v) * view_stride), 0, view_bytes, 0x00020000);
         const float uh0 = fmaf(dp0, Ub, fmaf(ds, Ua, U0));
@@ -746,9 +766,8 @@ void bp_tiles_kernel4(
                 st.fc[m] = __builtin_amdgcn_fractf(col);
                 st.fr[m] = __builtin_amdgcn_fractf(row);
                 if (RING == 2) {
-                    int b, jsh16;
-                    band_split(cvt_flr(row), b, jsh16);
-                    st.tap[m].load(rs, band_offset(b, cvt_flr(col), pitchb, baseb), jsh16);
+                    const int r = cvt_flr(row), q = band_q(r);
+                    st.tap[m].load(rs, band_addr(q, cvt_flr(col), pitchb, baseb), band_shift(q, r));
                 } else {
                     const int voff = __mul24(cvt_flr(row), pitchb) + ((cvt_flr(col) << SH) + baseb);
                     st.tap[m].load(rs, voff, pitchb);
@@ -818,6 +837,199 @@ void bp_tiles_kernel4(
         float val = acc[m];
         if (!CONE) val = fmaf(frp[m], accb[m] - acc[m], acc[m]);
         out[oidx(m)] = final_views ? (inside[m] ? val * scale : 0.0f) : val;
+    }
+}
+
+// ---------------------------------------------------------------- z columns (volumes)
+// When neither the detector column nor the depth of a point depends on z (P[0] and P[2]
+// have no z term: a circular orbit about the z axis with the detector's v axis along z,
+// as circular_cone_vectors builds it), every voxel of a column (x, y) of a volume has, in
+// each view, the same 1/w, the same column and so the same column fraction and FDK
+// weight; only its row moves, linearly in z.  The z-column kernel gives each thread one
+// column and a chunk of ZQ planes: per view it forms the reciprocal, the column, its
+// fraction and offset and the weight once, then per plane one FMA for the row, the clamp,
+// floor and fraction, the gather and the lerps.  The plane tile kernels redo the
+// projective division and the column per point.
+//
+// Records of a (column tile, plane chunk, view) are built in float64 by threads 0..63 of
+// the workgroup into LDS, one pass of 64 views at a time (as in the v2 loop), and read
+// back as VGPRs, so the FMAs of the per-column step take every coefficient from a VGPR.
+//   [0] U0 [1] Ua [2] Ub [3] V0 [4] Va [5] Vb [6] W0 [7] Wa [8] Wb: u_h, v_h, w_h at the
+//       tile's first column of the chunk's first plane and their steps along b (s) and
+//       a (p), relative to the integer reference (cref, rref) of the chunk's centre
+//   [9] [10] column clamp, [11] [12] row clamp (relative), [13] Vc: v_h per plane
+//   [14] + [15] element offset of the padded reference position: its row part (row *
+//       pitch) and its column (int bits)
+// The tile-independent part, per (task, view), comes from zcol_coef_kernel (ZC doubles:
+// u_h, v_h, w_h at the task origin and their steps along a and b, v_h along c), so the
+// record build holds few float64 values (the full build from the view's 3 x 4 form held
+// ~100 VGPRs, against ~70 for the main loop).
+// (ZT x ZT columns and ZQ planes per tile: mr_tiles.h)
+constexpr int ZG = 8;      // planes whose gathers are in flight together
+constexpr int ZC = 12;     // doubles per (task, view) coefficient set (10 used)
+static_assert(ZT * ZT == BP_THREADS, "one column per thread");
+
+// coef[(task * n_views + v) * ZC]: Uo Ua Ub Vo Va Vb Vc Wo Wa Wb of view slot v for task
+// `task`, with the task's COR shift (the point's u_h gains shift * w_h, v_h shift * u.vv *
+// w_h); a, b, c are the task's axes (p, s, plane), o its origin.
+__global__ __launch_bounds__(256) void zcol_coef_kernel(const ViewGeom* __restrict__ geom,
+                                                        const mr_task* __restrict__ tasks,
+                                                        int ntasks, int n_views,
+                                                        double* __restrict__ coef) {
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (gid >= static_cast<int64_t>(ntasks) * n_views) return;
+    const int ti = static_cast<int>(gid / n_views);
+    const int v = static_cast<int>(gid - static_cast<int64_t>(ti) * n_views);
+    const mr_task& T = tasks[ti];
+    const ViewGeom& g = geom[v];
+    const double sh = T.cor_shift, svv = sh * g.uvv;
+    const double Wo = dot3(g.P[2], T.o) + g.P[2][3], Wa = dot3(g.P[2], T.a), Wb = dot3(g.P[2], T.b);
+    double* k = coef + gid * ZC;
+    k[0] = dot3(g.P[0], T.o) + g.P[0][3] + sh * Wo;
+    k[1] = dot3(g.P[0], T.a) + sh * Wa;
+    k[2] = dot3(g.P[0], T.b) + sh * Wb;
+    k[3] = dot3(g.P[1], T.o) + g.P[1][3] + svv * Wo;
+    k[4] = dot3(g.P[1], T.a) + svv * Wa;
+    k[5] = dot3(g.P[1], T.b) + svv * Wb;
+    k[6] = dot3(g.P[1], T.c);          // P[0].c = P[2].c = 0 for this kernel
+    k[7] = Wo;
+    k[8] = Wa;
+    k[9] = Wb;
+}
+
+// Half `part` of the record of the tile whose first column is (p0, s0) on plane q0, from
+// its task's coefficient set k: part 0 the column half (u_h, w_h: [0..2], [6..10], [15]
+// the reference column's part of the element offset), part 1 the row half (v_h: [3..5],
+// [11..14], [14] the reference row's part).  Two threads build a record, so fewer float64
+// values are live at once in the kernel that holds the sums.  The integer reference only
+// needs to lie near the chunk's centre (coordinates relative to any integer are exact),
+// so it is chosen in float; the coefficients relative to it are formed in float64.
+__device__ void make_zrecord(const double* __restrict__ k, float vs, int part, int q0, int p0,
+                             int s0, int R, int C, int pitch, float* __restrict__ rec) {
+    const float hc = 0.5f * (ZT - 1), hq = 0.5f * (ZQ - 1);
+    const double W0 = k[7] + p0 * k[8] + s0 * k[9];
+    const float rwc = 1.0f / static_cast<float>(W0 + hc * (k[8] + k[9]));
+    // the weight 1/w_h^2 absorbs the view's storage scale
+    const double f = 1.0 / sqrt(static_cast<double>(vs));
+    if (part == 0) {
+        const double U0 = k[0] + p0 * k[1] + s0 * k[2];
+        float colc = static_cast<float>(U0 + hc * (k[1] + k[2])) * rwc + 0.5f * (C - 1);
+        colc = fminf(fmaxf(colc, -static_cast<float>(C)), 2.0f * C);   // NaN -> -C
+        const int cref = static_cast<int>(floorf(colc));
+        const double dc = 0.5 * (C - 1) - cref;
+        rec[0] = static_cast<float>((U0 + dc * W0) * f);
+        rec[1] = static_cast<float>((k[2] + dc * k[9]) * f);      // along b (s)
+        rec[2] = static_cast<float>((k[1] + dc * k[8]) * f);      // along a (p)
+        rec[6] = static_cast<float>(W0 * f);
+        rec[7] = static_cast<float>(k[9] * f);
+        rec[8] = static_cast<float>(k[8] * f);
+        rec[9] = static_cast<float>(-1 - cref);
+        rec[10] = static_cast<float>(C - cref);
+        rec[15] = __int_as_float(cref + 1);
+    } else {
+        const double V0 = k[3] + p0 * k[4] + s0 * k[5] + q0 * k[6];
+        float rowc = static_cast<float>(V0 + hc * (k[4] + k[5]) + hq * k[6]) * rwc + 0.5f * (R - 1);
+        rowc = fminf(fmaxf(rowc, -static_cast<float>(R)), 2.0f * R);
+        const int rref = static_cast<int>(floorf(rowc));
+        const double dr = 0.5 * (R - 1) - rref;
+        rec[3] = static_cast<float>((V0 + dr * W0) * f);
+        rec[4] = static_cast<float>((k[5] + dr * k[9]) * f);
+        rec[5] = static_cast<float>((k[4] + dr * k[8]) * f);
+        rec[11] = static_cast<float>(-1 - rref);
+        rec[12] = static_cast<float>(R - rref);
+        rec[13] = static_cast<float>(k[6] * f);
+        rec[14] = __int_as_float((rref + 1) * pitch);
+    }
+}
+
+// Tile tl: columns p = tl.tp * ZT + (t >> 4), s = tl.ts * ZT + (t & 15) of task tl.task,
+// planes tl.q .. tl.q + ZQ - 1 (the last chunk of a task may have fewer; the sums of the
+// planes past the task are formed and dropped).  RING 0: the ring (two row loads per
+// point), 1: the row-pair copy (one load of the four taps); view_stride: elements of ST
+// per view of that ring.
+template <typename ST, int RING>
+__global__ __launch_bounds__(BP_THREADS) void bp_zcol_kernel(
+    const ST* __restrict__ ring, int64_t view_stride, int pitch, int R, int C,
+    const double* __restrict__ coef, const float* __restrict__ vscale,
+    const mr_task* __restrict__ tasks, const Tile* __restrict__ tiles, int n_views, float scale,
+    float* __restrict__ out, int xcd) {
+    __shared__ float4 rec4[BP_VPASS * BP_REC / 4];
+    float* rec = reinterpret_cast<float*>(rec4);
+    const int lb = xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const Tile tl = tiles[lb];
+    const mr_task& T = tasks[tl.task];
+    const int t = threadIdx.x;
+    const int p = tl.tp * ZT + (t >> 4), s = tl.ts * ZT + (t & (ZT - 1));
+    const double* __restrict__ kt = coef + static_cast<int64_t>(tl.task) * n_views * ZC;
+    const float ds = static_cast<float>(t & (ZT - 1)), dp = static_cast<float>(t >> 4);
+    constexpr int LE = Taps<ST>::LOG_ESZ;
+    constexpr int SH = LE + (RING == 1 ? 1 : 0);     // log2 bytes per ring element
+    const int pitchb = pitch << SH;
+    const int view_bytes = static_cast<int>(view_stride << LE);
+    using Tap = typename std::conditional<RING == 1, PairP<ST>, Pair<ST>>::type;
+    float acc[ZQ];
+#pragma unroll
+    for (int i = 0; i < ZQ; ++i) acc[i] = 0.0f;
+    for (int vb = 0; vb < n_views; vb += BP_VPASS) {
+        const int nv = min(BP_VPASS, n_views - vb);
+        __syncthreads();
+        if (t < 2 * nv) {
+            const int v = t >> 1;
+            make_zrecord(kt + static_cast<int64_t>(vb + v) * ZC, vscale[vb + v], t & 1, tl.q,
+                         tl.tp * ZT, tl.ts * ZT, R, C, pitch, rec + v * BP_REC);
+        }
+        __syncthreads();
+        for (int v = 0; v < nv; ++v) {
+            const float4 r0 = rec4[v * 4 + 0], r1 = rec4[v * 4 + 1];
+            const float4 r2 = rec4[v * 4 + 2], r3 = rec4[v * 4 + 3];
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<ST*>(ring + static_cast<int64_t>(vb + v) * view_stride), 0, view_bytes,
+                0x00020000);
+            // once per column and view
+            const float uh = fmaf(dp, r0.z, fmaf(ds, r0.y, r0.x));
+            const float vh = fmaf(dp, r1.y, fmaf(ds, r1.x, r0.w));
+            const float wh = fmaf(dp, r2.x, fmaf(ds, r1.w, r1.z));
+            const float rw = __builtin_amdgcn_rcpf(wh);
+            const float col = __builtin_amdgcn_fmed3f(uh * rw, r2.y, r2.z);
+            const float fc = __builtin_amdgcn_fractf(col);
+            const int cofs = (cvt_flr(col) + __float_as_int(r3.z) + __float_as_int(r3.w)) << SH;
+            const float wt = rw * rw;
+            const float row0 = vh * rw, drow = r3.y * rw;
+            // per plane: the row, then ZG gathers in flight before their lerps.  The group's
+            // rows start from an opaque copy of row0 and a scheduling barrier ends the
+            // group, so the compiler neither forms every plane's row up front nor hoists
+            // the next group's gathers (either holds ZQ more VGPRs: fewer waves)
+#pragma unroll
+            for (int g = 0; g < ZQ; g += ZG) {
+                Tap tap[ZG];
+                float fr[ZG];
+                float rb = row0;
+                asm volatile("" : "+v"(rb));
+#pragma unroll
+                for (int i = 0; i < ZG; ++i) {
+                    const float row = __builtin_amdgcn_fmed3f(
+                        fmaf(static_cast<float>(g + i), drow, rb), r2.w, r3.x);
+                    fr[i] = __builtin_amdgcn_fractf(row);
+                    tap[i].load(rs, __mul24(cvt_flr(row), pitchb) + cofs, pitchb);
+                }
+#pragma unroll
+                for (int i = 0; i < ZG; ++i) {
+                    float top, bot;
+                    tap[i].lerp(fc, top, bot);
+                    acc[g + i] = fmaf(wt, fmaf(fr[i], bot - top, top), acc[g + i]);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+    }
+    if (p < T.np && s < T.ns) {
+        const int64_t ob = T.out_offset + static_cast<int64_t>(p) * T.stride_p +
+                           static_cast<int64_t>(s) * T.stride_s +
+                           static_cast<int64_t>(tl.q) * T.stride_q;
+        const int nq = T.nq - tl.q;
+#pragma unroll
+        for (int i = 0; i < ZQ; ++i)
+            if (i < nq) out[ob + static_cast<int64_t>(i) * T.stride_q] = acc[i] * scale;
     }
 }
 
@@ -1569,6 +1781,7 @@ __global__ void read_views_kernel(const ST* __restrict__ ring, int64_t view_stri
 template <bool CONE>
 int dispatch_tiles(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles, int n_views,
                    float scale, float* d_out, hipStream_t s, const TileQueue& q) {
+    c->kern_used = 2;
     // persistent launches size the grid to the resident capacity and pull tiles from q
     dim3 grid(q.head ? q.grid : ntiles), block(BP_THREADS);
     if (c->bp_variant == 2 && c->v2_ring == 1 && c->storage == MR_STORE_F32) {
@@ -2149,6 +2362,7 @@ int launch_bp_tiles_v4(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int 
         if (rc) return rc;
     }
     const bool band = c->bp_variant == 7 && c->storage != MR_STORE_F32;   // band-ring records
+    c->kern_used = c->bp_variant;
     const int rec_floats = v5 ? V5_REC : BP_REC;
     const size_t per_tile = sizeof(float) * rec_floats * static_cast<size_t>(n_views);
     int chunk = static_cast<int>(std::max<size_t>(1, c->rec_budget / per_tile));
@@ -2216,6 +2430,56 @@ int launch_bp_tiles_v4(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int 
             c->tile_timed = true;
         }
     }
+    return MR_OK;
+}
+
+// The z-column kernel over the tiles of build_zcol_tiles (chunks of ZQ planes x ZT x ZT
+// columns): on the row-pair copy for 16-bit storage when there is room for it (one 8-byte
+// load per point-view), else on the ring (two).  n_views > 0.
+int launch_bp_zcol(Ctx* c, const mr_task* d_tasks, int ntasks, const Tile* d_tiles, int ntiles,
+                   int n_views, float scale, float* d_out, hipStream_t s) {
+    if (ntiles <= 0 || n_views <= 0) return MR_OK;
+    bool paired = false;
+    if (c->storage != MR_STORE_F32) {
+        const int rc = ensure_pair_ring(c, n_views, s);
+        if (rc && rc != kNoRoom) return rc;
+        paired = rc != kNoRoom;
+    }
+    const int64_t pvs = pair_view_stride(c);
+    const int xcd = c->xcd_map != 0 ? 1 : 0;
+    // per (task, view) coefficients in the records buffer
+    const int64_t ncoef = static_cast<int64_t>(ntasks) * n_views;
+    int rc = ensure(reinterpret_cast<void**>(&c->d_recs), &c->recs_bytes,
+                    sizeof(double) * ZC * static_cast<size_t>(ncoef));
+    if (rc) return rc;
+    const double* d_coef = reinterpret_cast<const double*>(c->d_recs);
+    hipLaunchKernelGGL(zcol_coef_kernel, dim3(static_cast<unsigned>((ncoef + 255) / 256)), dim3(256),
+                       0, s, c->d_geom, d_tasks, ntasks, n_views, reinterpret_cast<double*>(c->d_recs));
+    MR_HIP(hipEventRecord(c->evk0, s));
+#define MR_ZC(ST, RK)                                                                          \
+    hipLaunchKernelGGL((bp_zcol_kernel<ST, RK>), dim3(ntiles), dim3(BP_THREADS), 0, s,          \
+                       static_cast<const ST*>(RK ? c->pring : c->ring), RK ? pvs : c->view_stride, \
+                       c->pitch, c->R, c->C, d_coef, c->d_vscale, d_tasks, d_tiles, n_views,      \
+                       scale, d_out, xcd)
+    switch (c->storage) {
+    case MR_STORE_F32:
+        MR_ZC(float, 0);
+        break;
+    case MR_STORE_F16:
+        if (paired) MR_ZC(__half, 1);
+        else MR_ZC(__half, 0);
+        break;
+    default:
+        if (paired) MR_ZC(uint16_t, 1);
+        else MR_ZC(uint16_t, 0);
+        break;
+    }
+#undef MR_ZC
+    MR_HIP(hipGetLastError());
+    MR_HIP(hipEventRecord(c->evk1, s));
+    c->tile_timed = true;
+    c->tile_launches = 1;
+    c->kern_used = 8;
     return MR_OK;
 }
 

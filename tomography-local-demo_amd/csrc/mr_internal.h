@@ -107,6 +107,9 @@ struct Ctx {
     bool tile_timed = false;
     int tile_launches = 0;           // tile-kernel dispatches of the request in flight
     int last_tile_launches = 0;      // ... of the last finished request
+    int kern_used = 0;               // back-projection kernel of the request in flight
+                                     // (2, 4-7; 8: z columns)
+    int last_kern = 0;               // ... of the last finished request
     double last_tile_ms = 0.0;
     int bp_variant = 4;              // back-projection inner loop (MINIRECAST_BP_VARIANT)
     int bp_ppt = 2;                  // v4/v5 points per thread (MINIRECAST_BP_PPT)
@@ -149,6 +152,9 @@ struct Ctx {
     size_t adj_bytes = 0;
 
     int kind = MR_KIND_NONE;
+    // cone geometry whose every view has P[0][2] = P[2][2] = 0 exactly: the detector column
+    // and depth of a point do not depend on z (volume requests can run the z-column kernel)
+    bool zsep = false;
     int n_total = 0, R = 0, C = 0;
     int window = MR_WINDOW_SHEPP_LOGAN, storage = MR_STORE_F32;
     int N = 0;                       // FFT length
@@ -306,6 +312,9 @@ int launch_bp_adjoint(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int n
                       int n_views, float scale, const float* d_g, float* d_gq, hipStream_t s);
 int launch_bp_dshift(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles, int n_views,
                      float scale, float* d_out, hipStream_t s);
+// the z-column kernel over z-column tiles (build_zcol_tiles) of a volume request; n_views > 0
+int launch_bp_zcol(Ctx* c, const mr_task* d_tasks, int ntasks, const Tile* d_tiles, int ntiles,
+                   int n_views, float scale, float* d_out, hipStream_t s);
 int launch_filter_adjoint(Ctx* c, const float* d_gq, int k, float* d_graw, hipStream_t s);
 // COR entropy scores of nimg images, image i = npix[i] floats from d_img + offset[i].
 // d_ws: a device workspace of cor_workspace_bytes(nimg) bytes (owned by the caller: a

@@ -39,6 +39,7 @@ SIGNATURES = [
     ("mr_push_raw", _c.c_int, [_P, _P, _c.c_int, _P, _c.c_int, _c.POINTER(_c.c_int)]),
     ("mr_last_tile_kernel_ms", _c.c_int, [_P, _c.POINTER(_c.c_double)]),
     ("mr_last_tile_kernel_launches", _c.c_int, [_P, _c.POINTER(_c.c_int)]),
+    ("mr_last_bp_kernel", _c.c_int, [_P, _c.POINTER(_c.c_int)]),
     ("mr_nonzero_percentiles", _c.c_int, [_P, _P, _c.c_int64, _c.c_int, _P, _c.c_int, _P,
                                           _c.POINTER(_c.c_int64)]),
     ("mr_clip_histogram", _c.c_int, [_P, _P, _c.c_int64, _c.c_int, _c.c_double, _c.c_double,
@@ -487,6 +488,13 @@ class Context:
         n = ctypes.c_int(0)
         check(self.lib.mr_last_tile_kernel_launches(self.h, ctypes.byref(n)))
         return n.value
+
+    def last_bp_kernel(self):
+        """Back-projection kernel of the last bp_tasks call: 2, 4-7 (plane tiles), 8 (z
+        columns of a volume request), 0 when nothing was launched."""
+        k = ctypes.c_int(0)
+        check(self.lib.mr_last_bp_kernel(self.h, ctypes.byref(k)))
+        return k.value
 
     # ---------------------------------------------------------------- device memory
     def alloc(self, nbytes):

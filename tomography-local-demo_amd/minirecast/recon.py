@@ -474,8 +474,9 @@ class SliceReconstructor:
 
     def reconstruct_volume(self, voxel=None, maxdim=None, chunk=262144, progress=None):
         """Full 3-D reconstruction of the box on a regular grid (recon.py:442-470),
-        in z-slabs of max(1, chunk // (nx*ny)) planes so that slice requests can
-        interleave.  Returns (volume [nx, ny, nz] float32, origin, voxel size)."""
+        in z-slabs of max(32, chunk // (nx*ny)) planes (one request and one progress
+        call each) so that slice requests can interleave.  Returns (volume [nx, ny, nz]
+        float32, origin, voxel size)."""
         if maxdim is not None:
             dims, origin, h = volume_grid(self.box, maxdim)
         else:
@@ -489,7 +490,11 @@ class SliceReconstructor:
             if progress is not None:
                 progress(1.0)
             return vol, np.asarray(origin, float), h
-        per_slab = max(1, chunk // (nx * ny))
+        # at least 32 planes per request: slabs of a few planes would keep a volume off the
+        # z-column kernel (planes stacked along z, one column per thread) and pay a
+        # request each; a 768^2 slab of 32 planes is ~20 ms of GPU time, short enough for
+        # slice requests to interleave
+        per_slab = max(32, chunk // (nx * ny))
         ctx = self._ctx
         dptr = ctx.alloc(vol.nbytes)
         try:
