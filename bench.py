@@ -264,7 +264,8 @@ def other_workloads(args):
 
 def bench_full_volume(args):
     """The 768^3 volume of the full_cone scan (5.44e11 updates), reconstructed into HBM
-    from the f16 ring in one request (z-planes of 32 x 32 tiles), then the Otsu
+    from the f16 ring in one request (the z-column kernel: 16 x 16 voxel columns x 16
+    planes per workgroup, each view's detector box staged through LDS), then the Otsu
     threshold of model.py computed on the device-resident volume."""
     from minirecast import _hip, model, phantom, recon, workloads
     w = workloads.full_cone()
@@ -310,7 +311,8 @@ def bench_full_volume(args):
     upd = nvox * A
     res = dict(dims=[int(d) for d in dims], voxel_mm=float(h), ms=ms,
                tile_kernel_ms=float(np.median(tile_ms)), gups=upd / (ms * 1e-3) / 1e9,
-               otsu_ms=otsu_ms, otsu_threshold=thr, nonzero_voxels=m, config=ctx.bp_config())
+               otsu_ms=otsu_ms, otsu_threshold=thr, nonzero_voxels=m, config=ctx.bp_config(),
+               kernel=ctx.last_bp_kernel())
     log(f"full_volume: {res}")
     _line("ms per 768^3 volume from 1200 views (f16 ring) + GUPS", round(res["gups"], 2),
           "GUPS", round(ms, 3), len(times), 1,

@@ -211,7 +211,10 @@ int mr_clip_histogram_static(mr_ctx* ctx, const float* vol_device, int64_t n, do
  * points, 2 views);
  * points per thread 1, 2 or 4; lane layout 0 (waves of 2 x 32 points) or 1 (4 x 16);
  * 2 or 3 views in flight (kernels 4, 6 and 7).
- * Until this is called the configuration follows the storage form of the geometry. */
+ * Until this is called the configuration follows the storage form of the geometry;
+ * kernel 0 (other arguments ignored) returns to that default.  Volume requests of a cone
+ * geometry whose detector column and depth do not depend on z run the z-column kernel
+ * (8 or 9, see mr_last_bp_kernel) in the default configuration only. */
 int mr_set_bp_config(mr_ctx* ctx, int kernel, int ppt, int layout, int depth);
 int mr_get_bp_config(mr_ctx* ctx, int* cfg);
 
@@ -374,10 +377,11 @@ int mr_last_tile_kernel_ms(mr_ctx* ctx, double* ms);
 int mr_last_tile_kernel_launches(mr_ctx* ctx, int* n);
 
 /* The back-projection kernel of the last mr_bp_tasks: 2 (records in LDS, the work-queue
- * loop), 4, 5, 6 or 7 (the plane-tile kernels), 8 (the z-column kernel of volume
+ * loop), 4, 5, 6 or 7 (the plane-tile kernels), 8 or 9 (the z-column kernel of volume
  * requests: a cone geometry whose detector column and depth do not depend on z, tasks
- * stacking planes along z), 0 when nothing was launched.  No counterpart in the reference
- * (introspection only). */
+ * stacking planes along z; 8 gathers from the ring, 9 stages each view's detector box of
+ * the 16-bit row-pair copy through LDS), 0 when nothing was launched.  No counterpart in
+ * the reference (introspection only). */
 int mr_last_bp_kernel(mr_ctx* ctx, int* kernel);
 
 /* L2 gather microbenchmark (no counterpart in the reference; measurement only): the rate

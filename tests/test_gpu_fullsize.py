@@ -3,8 +3,9 @@ oblique 1024^2 slices (1,985,962 in-box points), f32 and f16 rings in their defa
 back-projection configurations.
 
 Checked: the float64 oracle on 3 x 256 sampled in-box pixels (f32 <= 1e-5, f16 <= 1e-3,
-bf16 <= 1e-3 relative L2) and on 512 sampled voxels of the 768 x 768 x 607 volume over
-the box (reconstruct_volume(maxdim=768), recon.py:442-470), stream == batch (interleaved arrival in batches of 40), exact linearity
+bf16 <= 1e-3 relative L2), on 512 sampled voxels of the 768 x 768 x 607 volume over
+the box (reconstruct_volume(maxdim=768), recon.py:442-470; the z-column kernel, staged
+through LDS for f16) and on 256 of the 384^3 grid (boxes too large to stage), stream == batch (interleaved arrival in batches of 40), exact linearity
 of the f32 path (projections x 2 give outputs x 2 bit for bit: scaling by 2 is exact in
 binary floating point), and the default f16 kernel (band ring) against kernel 2 (the
 work-queue loop on the ring itself) on every output.
@@ -52,6 +53,12 @@ def full():
     vsel = np.sort(rng.choice(nvox, size=512, replace=False))
     ijk = np.stack(np.unravel_index(vsel, tuple(int(d) for d in dims)), axis=1)
     samp_pts.append(vorigin + vh * ijk)
+    # a 384^3 grid: voxels of 3.4 detector pixels, whose z-column boxes do not fit the LDS
+    # buffer of the staged kernel (its views are gathered from memory)
+    dims2, vorigin2, vh2 = recon.volume_grid(box, 384)
+    vsel2 = np.sort(rng.choice(int(np.prod(dims2)), size=256, replace=False))
+    ijk2 = np.stack(np.unravel_index(vsel2, tuple(int(d) for d in dims2)), axis=1)
+    samp_pts.append(vorigin2 + vh2 * ijk2)
     samp_pts = np.concatenate(samp_pts)
     acc64 = np.zeros(len(samp_pts))
     host = np.empty((R, batch, C), np.float32)
@@ -84,9 +91,11 @@ def full():
     T = np.array(tasks, dtype=_hip.TASK_DTYPE)
     out = {name: ctx[name].bp_tasks(T, A, off) for name in names}
     ref = acc64 * (np.pi / A)
+    n_vol = len(vsel)
     return dict(w=w, ctx=ctx, T=T, out_len=off, out=out, samp_idx=samp_idx,
-                ref=ref[:n_slice], vol_ref=ref[n_slice:], vsel=vsel, dims=dims,
-                vorigin=vorigin, vh=vh)
+                ref=ref[:n_slice], vol_ref=ref[n_slice:n_slice + n_vol], vsel=vsel, dims=dims,
+                vorigin=vorigin, vh=vh, vol2_ref=ref[n_slice + n_vol:], vsel2=vsel2,
+                dims2=dims2, vorigin2=vorigin2, vh2=vh2)
 
 
 def test_full_size_vs_float64_oracle(full):
@@ -119,7 +128,8 @@ def test_full_size_default_f16_kernel_vs_plain_kernel(full):
     try:
         plain = ctx.bp_tasks(full["T"], full["w"].geom.n_angles, full["out_len"])
     finally:
-        ctx.set_bp_config(**cfg)
+        ctx.reset_bp_config()
+    assert ctx.bp_config() == cfg
     got = full["out"]["f16"]
     np.testing.assert_array_equal(got == 0, plain == 0)
     assert rel_l2(got, plain) < 1e-6
@@ -144,5 +154,33 @@ def test_full_size_volume_vs_float64_oracle(full, name, tol):
         ctx.free(dptr)
     got = vol[full["vsel"]].astype(np.float64)
     err = rel_l2(got, full["vol_ref"])
-    print(f"{name}: volume {tuple(int(d) for d in dims)} rel-L2 {err:.2e} at {len(got)} voxels")
+    kern = ctx.last_bp_kernel()
+    print(f"{name}: volume {tuple(int(d) for d in dims)} rel-L2 {err:.2e} at {len(got)} voxels "
+          f"(kernel {kern})")
     assert np.isfinite(vol).all() and err < tol, (name, err)
+    # z columns: gathered from the ring (f32) or staged through LDS (16-bit)
+    assert kern == (8 if name == "f32" else 9), kern
+
+
+@pytest.mark.parametrize("name,tol", [("f32", 1e-5), ("f16", 1e-3)])
+def test_full_size_coarse_volume_vs_float64_oracle(full, name, tol):
+    """The 384^3 grid over the box (voxels of 3.4 detector pixels: the staged kernel's
+    boxes exceed its LDS buffer and every view is gathered from memory) at 256 sampled
+    voxels against the float64 oracle."""
+    from minirecast import recon
+    ctx = full["ctx"][name]
+    dims = full["dims2"]
+    nvox = int(np.prod(dims))
+    A = full["w"].geom.n_angles
+    T = recon.volume_task(full["vorigin2"], full["vh2"], dims, 0, int(dims[2]))
+    dptr = ctx.alloc(4 * nvox)
+    try:
+        ctx.bp_tasks_device(T[None], A, dptr, nvox)
+        vol = ctx.d2h(np.empty(nvox, np.float32), dptr)
+    finally:
+        ctx.free(dptr)
+    err = rel_l2(vol[full["vsel2"]].astype(np.float64), full["vol2_ref"])
+    kern = ctx.last_bp_kernel()
+    print(f"{name}: volume {tuple(int(d) for d in dims)} rel-L2 {err:.2e} (kernel {kern})")
+    assert np.isfinite(vol).all() and err < tol, (name, err)
+    assert kern == (8 if name == "f32" else 9), kern

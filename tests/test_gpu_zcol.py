@@ -1,10 +1,12 @@
-"""GPU tests of the z-column kernel (kernel 8) of volume requests.
+"""GPU tests of the z-column kernel (kernels 8 and 9) of volume requests.
 
 A cone geometry whose detector column and depth do not depend on z (a circular orbit about
 z with the detector's v axis along z) runs volume tasks (planes stacked along z) one voxel
 column per thread: the reciprocal, column, column fraction and weight once per column and
-view, the row per plane by one FMA.  Checked, for f32, f16 and bf16 rings:
-- default volume requests of such a geometry run kernel 8 (mr_last_bp_kernel);
+view, the row per plane by one FMA.  f32 rings gather from the ring (kernel 8); 16-bit
+rings stage each view's detector box of the row-pair copy through LDS (kernel 9).
+Checked, for f32, f16 and bf16 rings:
+- default volume requests of such a geometry run kernel 8 / 9 (mr_last_bp_kernel);
 - the volume equals the plane-tile kernels' (kernel 4 on the ring for f32, kernel 6 on
   the row-pair copy for 16-bit storage) to float32 rounding on every voxel;
 - sampled voxels against the float64 oracle (f32 <= 1e-5, 16-bit <= 1e-3 relative L2),
@@ -90,7 +92,7 @@ def test_zcol_volume_vs_tiles_and_oracle(zc, st, tol, shift):
     recon = zc["recon"]
     T = recon.volume_task(zc["origin"], zc["h"], DIMS, 0, DIMS[2], shift)[None]
     vol, kern = _volume(zc["ctx"][st, "dflt"], T, A, zc["nvox"])
-    assert kern == 8, kern
+    assert kern == (8 if st == "f32" else 9), kern
     tiles, kern_t = _volume(zc["ctx"][st, "tiles"], T, A, zc["nvox"])
     assert kern_t in (4, 6), kern_t
     assert np.isfinite(vol).all()
@@ -126,7 +128,7 @@ def test_zcol_not_used_without_z_separation_or_enough_tiles(zc):
     recon = zc["recon"]
     T = recon.volume_task(zc["origin"], zc["h"], DIMS, 0, DIMS[2])[None]
     vol, kern = _volume(zc["ctx"]["tilted"], T, A, zc["nvox"])
-    assert kern != 8, kern
+    assert kern not in (8, 9), kern
     err = rel_l2(vol[zc["sel"]].astype(np.float64), zc["ref"]["tilted"])
     print(f"tilted detector: kernel {kern}, vs float64 {err:.2e}")
     assert err < 1e-5, err
@@ -134,4 +136,4 @@ def test_zcol_not_used_without_z_separation_or_enough_tiles(zc):
     n = int(np.prod(small))
     _, kern = _volume(zc["ctx"]["f32", "dflt"], recon.volume_task(zc["origin"], zc["h"], small, 0, 64)[None],
                       A, n)
-    assert kern != 8, kern
+    assert kern not in (8, 9), kern

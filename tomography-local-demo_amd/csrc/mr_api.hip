@@ -154,6 +154,8 @@ int make_tables(int N, int window, float** d_H, float2** d_tw, double* l1 = null
     return MR_OK;
 }
 
+void apply_bp_defaults(Ctx* c, bool parallel);
+
 int alloc_ring(Ctx* c, int n_total, int rows, int cols, int window, int storage) {
     c->n_total = n_total;
     c->R = rows;
@@ -161,17 +163,7 @@ int alloc_ring(Ctx* c, int n_total, int rows, int cols, int window, int storage)
     c->window = window;
     c->storage = storage;
     c->N = fft_length(cols);
-    if (!c->bp_explicit) {
-        // measured defaults: 16-bit forms read the band ring (kernel 7, 2 points per
-        // thread, 3 views in flight); f32 reads the ring directly (kernel 4, 2 points per
-        // thread, 2 views in flight)
-        const bool f32 = storage == MR_STORE_F32;
-        c->bp_variant = f32 ? 4 : 7;
-        c->bp_ppt = 2;
-        c->bp_layout = 0;
-        c->bp_depth = f32 ? 2 : 3;
-        c->bp_sync = f32 ? 0 : 1;
-    }
+    if (!c->bp_explicit) apply_bp_defaults(c, false);
     // measured: cone updates that fill the device run fastest in launches of 32 (f32) or
     // 64 (16-bit) views
     if (!c->vchunk_explicit) c->bp_vchunk = storage == MR_STORE_F32 ? 32 : 64;
@@ -195,6 +187,30 @@ int alloc_ring(Ctx* c, int n_total, int rows, int cols, int window, int storage)
     c->received.assign(n_total, 0);
     c->n = 0;
     return MR_OK;
+}
+
+// The measured default back-projection configuration of a storage form and beam.
+// Cone: 16-bit forms read the band ring (kernel 7, 2 points per thread, 3 views in
+// flight, waves in lockstep per view); f32 reads the ring directly (kernel 4, 2 points,
+// 2 views).  Parallel: kernel 6, the row-pair copy (a point keeps one detector row pair
+// in every view), 4 points per thread, 2 views in flight; a COR search of 75 images of
+// up to 1024^2 over 1800 views with its records in memory: 38.7 ms (f32 and f16) against
+// 42.4 ms with 2 points, 74.5 ms for f32 on the ring (kernel 4) and 47 ms for f16 on the
+// band copy (kernel 7).
+void apply_bp_defaults(Ctx* c, bool parallel) {
+    const bool f32 = c->storage == MR_STORE_F32;
+    c->bp_layout = 0;
+    if (parallel) {
+        c->bp_variant = 6;
+        c->bp_ppt = 4;
+        c->bp_depth = 2;
+        c->bp_sync = 0;
+    } else {
+        c->bp_variant = f32 ? 4 : 7;
+        c->bp_ppt = 2;
+        c->bp_depth = f32 ? 2 : 3;
+        c->bp_sync = f32 ? 0 : 1;
+    }
 }
 
 int validate_tasks(const mr_task* tasks, int ntasks, int64_t out_len) {
@@ -247,19 +263,22 @@ bool zcol_request(const Ctx* c, const mr_task* tasks, int ntasks, int n_views) {
 
 // The z-column path of a volume request (zcol_request).
 int run_zcol(Ctx* c, const mr_task* tasks, int ntasks, int n_views, float* d_out) {
+    int form = ZCOL_RING;
+    int rc = zcol_form(c, n_views, c->s_req, &form);
+    if (rc) return rc;
     std::vector<Tile> tiles;
-    build_zcol_tiles(tasks, ntasks, tiles);
+    build_zcol_tiles(tasks, ntasks, form == ZCOL_STAGED ? ZQL : ZQ, tiles);
     const mr_task* d_tasks = nullptr;
     const Tile* d_tiles = nullptr;
-    int rc = upload_request(c, tasks, ntasks, tiles, &d_tasks, &d_tiles);
+    rc = upload_request(c, tasks, ntasks, tiles, &d_tasks, &d_tiles);
     if (rc) return rc;
     const float scale = static_cast<float>(M_PI / n_views);
     c->tile_timed = false;
     c->tile_launches = 0;
     c->req_slices = false;
     MR_HIP(hipEventRecord(c->ev0, c->s_req));
-    rc = launch_bp_zcol(c, d_tasks, ntasks, d_tiles, static_cast<int>(tiles.size()), n_views, scale,
-                        d_out, c->s_req);
+    rc = launch_bp_zcol(c, form, d_tasks, ntasks, d_tiles, static_cast<int>(tiles.size()), n_views,
+                        scale, d_out, c->s_req);
     if (rc) return rc;
     MR_HIP(hipEventRecord(c->ev1, c->s_req));
     return MR_OK;
@@ -610,17 +629,7 @@ int mr_set_geometry_parallel(mr_ctx* c, int n_total, int rows, int cols, const d
     }
     c->zsep = false;
     c->kind = MR_KIND_PARALLEL;
-    // measured default for parallel beam: kernel 6, the row-pair copy (a point keeps one
-    // detector row pair in every view), 4 points per thread, 2 views in flight.  A COR
-    // search of 75 images of up to 1024^2 over 1800 views with its records in memory:
-    // 38.7 ms (f32 and f16) against 42.4 ms with 2 points, 74.5 ms for f32 on the ring
-    // (kernel 4) and 47 ms for f16 on the band copy (kernel 7)
-    if (!c->bp_explicit) {
-        c->bp_variant = 6;
-        c->bp_ppt = 4;
-        c->bp_depth = 2;
-        c->bp_sync = 0;
-    }
+    if (!c->bp_explicit) apply_bp_defaults(c, true);
     return MR_OK;
 }
 
@@ -1057,6 +1066,12 @@ int mr_set_bp_config(mr_ctx* c, int variant, int ppt, int layout, int depth) {
     int rc = check_ctx(c);
     if (rc) return rc;
     std::lock_guard<std::mutex> lk(c->mu);
+    if (variant == 0) {
+        // back to the defaults of the geometry's storage form and beam
+        c->bp_explicit = false;
+        apply_bp_defaults(c, c->kind == MR_KIND_PARALLEL);
+        return MR_OK;
+    }
     MR_CHECK(valid_config(variant, ppt, layout, depth), MR_EVALUE,
              "invalid back-projection configuration");
     c->bp_variant = variant;

@@ -851,22 +851,27 @@ void bp_tiles_kernel4(
 // floor and fraction, the gather and the lerps.  The plane tile kernels redo the
 // projective division and the column per point.
 //
-// Records of a (column tile, plane chunk, view) are built in float64 by threads 0..63 of
-// the workgroup into LDS, one pass of 64 views at a time (as in the v2 loop), and read
-// back as VGPRs, so the FMAs of the per-column step take every coefficient from a VGPR.
+// Records of a (column tile, plane chunk, view) are built in float64 by threads of the
+// workgroup into LDS, one pass of 64 views at a time (as in the v2 loop), and read back as
+// VGPRs, so the FMAs of the per-column step take every coefficient from a VGPR.
 //   [0] U0 [1] Ua [2] Ub [3] V0 [4] Va [5] Vb [6] W0 [7] Wa [8] Wb: u_h, v_h, w_h at the
 //       tile's first column of the chunk's first plane and their steps along b (s) and
 //       a (p), relative to the integer reference (cref, rref) of the chunk's centre
 //   [9] [10] column clamp, [11] [12] row clamp (relative), [13] Vc: v_h per plane
-//   [14] + [15] element offset of the padded reference position: its row part (row *
-//       pitch) and its column (int bits)
+//   [14] rref + 1, [15] cref + 1: the padded reference position (int bits)
+//   [16] [17] [18] [19] (staged kernel) the detector box of the chunk's taps in the
+//       row-pair copy: first pair-row, pair-rows (0: the box does not fit the LDS
+//       buffer and the view is gathered from memory), first column (a multiple of 4),
+//       16-byte chunks per row (int bits)
 // The tile-independent part, per (task, view), comes from zcol_coef_kernel (ZC doubles:
 // u_h, v_h, w_h at the task origin and their steps along a and b, v_h along c), so the
 // record build holds few float64 values (the full build from the view's 3 x 4 form held
 // ~100 VGPRs, against ~70 for the main loop).
-// (ZT x ZT columns and ZQ planes per tile: mr_tiles.h)
-constexpr int ZG = 8;      // planes whose gathers are in flight together
-constexpr int ZC = 12;     // doubles per (task, view) coefficient set (10 used)
+// (ZT x ZT columns and ZQ or ZQL planes per tile: mr_tiles.h)
+constexpr int ZG = 8;          // planes whose gathers are in flight together
+constexpr int ZC = 12;         // doubles per (task, view) coefficient set (10 used)
+constexpr int ZREC = 20;       // floats per record
+constexpr int ZL_BUF = 16384;  // bytes of one staged box (two per workgroup)
 static_assert(ZT * ZT == BP_THREADS, "one column per thread");
 
 // coef[(task * n_views + v) * ZC]: Uo Ua Ub Vo Va Vb Vc Wo Wa Wb of view slot v for task
@@ -897,16 +902,66 @@ __global__ __launch_bounds__(256) void zcol_coef_kernel(const ViewGeom* __restri
     k[9] = Wb;
 }
 
-// Half `part` of the record of the tile whose first column is (p0, s0) on plane q0, from
-// its task's coefficient set k: part 0 the column half (u_h, w_h: [0..2], [6..10], [15]
-// the reference column's part of the element offset), part 1 the row half (v_h: [3..5],
-// [11..14], [14] the reference row's part).  Two threads build a record, so fewer float64
-// values are live at once in the kernel that holds the sums.  The integer reference only
-// needs to lie near the chunk's centre (coordinates relative to any integer are exact),
-// so it is chosen in float; the coefficients relative to it are formed in float64.
+// Part `part` of the record of the tile whose first column is (p0, s0) on plane q0 (NQ
+// planes), from its task's coefficient set k: part 0 the column half (u_h, w_h: [0..2],
+// [6..10], [15]), part 1 the row half (v_h: [3..5], [11..14]), part 2 the box ([16..19]).
+// Threads of the workgroup build the parts in parallel, so fewer float64 values are live
+// at once in the kernel that holds the sums.  The integer reference only needs to lie
+// near the chunk's centre (coordinates relative to any integer are exact), so it is
+// chosen in float; the coefficients relative to it are formed in float64.  The box is
+// conservative: the corners of the column tile x plane chunk, projected in float (within
+// ~1e-4 px of the kernel's relative coordinates; margin 1/16 px), clamped as the kernel
+// clamps, padded by the second tap; a projective map with w > 0 at every corner sends the
+// box of points into the hull of the corner images.
+template <int NQ>
 __device__ void make_zrecord(const double* __restrict__ k, float vs, int part, int q0, int p0,
                              int s0, int R, int C, int pitch, float* __restrict__ rec) {
-    const float hc = 0.5f * (ZT - 1), hq = 0.5f * (ZQ - 1);
+    const float hc = 0.5f * (ZT - 1), hq = 0.5f * (NQ - 1);
+    if (part == 2) {
+        const float Wo = static_cast<float>(k[7] + p0 * k[8] + s0 * k[9]);
+        const float Uo = static_cast<float>(k[0] + p0 * k[1] + s0 * k[2]);
+        const float Vo = static_cast<float>(k[3] + p0 * k[4] + s0 * k[5] + q0 * k[6]);
+        const float Wp = static_cast<float>(k[8]), Ws = static_cast<float>(k[9]);
+        const float Up = static_cast<float>(k[1]), Us = static_cast<float>(k[2]);
+        const float Vp = static_cast<float>(k[4]), Vs = static_cast<float>(k[5]);
+        const float Vq = static_cast<float>(k[6]) * (NQ - 1);
+        const float cc = 0.5f * (C - 1), cr = 0.5f * (R - 1), e = ZT - 1;
+        float cmin = 3.0e38f, cmax = -3.0e38f, rmin = 3.0e38f, rmax = -3.0e38f;
+        bool ok = true;
+#pragma unroll
+        for (int c4 = 0; c4 < 4; ++c4) {
+            const float a = (c4 & 1) ? e : 0.0f, b = (c4 & 2) ? e : 0.0f;
+            const float w = Wo + a * Wp + b * Ws;
+            ok = ok && w > 0.0f;
+            const float rw = 1.0f / w;
+            const float col = (Uo + a * Up + b * Us) * rw + cc;
+            const float v0 = Vo + a * Vp + b * Vs;
+            const float r0 = v0 * rw + cr, r1 = (v0 + Vq) * rw + cr;
+            cmin = fminf(cmin, col);
+            cmax = fmaxf(cmax, col);
+            rmin = fminf(rmin, fminf(r0, r1));
+            rmax = fmaxf(rmax, fmaxf(r0, r1));
+        }
+        const float m = 0.0625f;
+        cmin = fminf(fmaxf(cmin, -1.0f), static_cast<float>(C));
+        cmax = fminf(fmaxf(cmax, -1.0f), static_cast<float>(C));
+        rmin = fminf(fmaxf(rmin, -1.0f), static_cast<float>(R));
+        rmax = fminf(fmaxf(rmax, -1.0f), static_cast<float>(R));
+        // padded: a tap pair at detector column c covers padded columns c + 1, c + 2; the
+        // pair-row of detector row r is padded row r + 1
+        const int c0 = max(0, static_cast<int>(floorf(cmin - m)) + 1) & ~3;
+        const int c1 = min(C + 2, static_cast<int>(floorf(cmax + m)) + 2);
+        const int pr0 = max(0, static_cast<int>(floorf(rmin - m)) + 1);
+        const int pr1 = min(R + 1, static_cast<int>(floorf(rmax + m)) + 1);
+        const int nch = (c1 - c0) / 4 + 1;
+        const int nr = pr1 - pr0 + 1;
+        const bool fits = ok && nr > 0 && nr * nch * 16 <= ZL_BUF;
+        rec[16] = __int_as_float(pr0);
+        rec[17] = __int_as_float(fits ? nr : 0);
+        rec[18] = __int_as_float(c0);
+        rec[19] = __int_as_float(nch);
+        return;
+    }
     const double W0 = k[7] + p0 * k[8] + s0 * k[9];
     const float rwc = 1.0f / static_cast<float>(W0 + hc * (k[8] + k[9]));
     // the weight 1/w_h^2 absorbs the view's storage scale
@@ -938,22 +993,38 @@ __device__ void make_zrecord(const double* __restrict__ k, float vs, int part, i
         rec[11] = static_cast<float>(-1 - rref);
         rec[12] = static_cast<float>(R - rref);
         rec[13] = static_cast<float>(k[6] * f);
-        rec[14] = __int_as_float((rref + 1) * pitch);
+        rec[14] = __int_as_float(rref + 1);
     }
 }
 
 // Tile tl: columns p = tl.tp * ZT + (t >> 4), s = tl.ts * ZT + (t & 15) of task tl.task,
-// planes tl.q .. tl.q + ZQ - 1 (the last chunk of a task may have fewer; the sums of the
+// planes tl.q .. tl.q + NQ - 1 (the last chunk of a task may have fewer; the sums of the
 // planes past the task are formed and dropped).  RING 0: the ring (two row loads per
 // point), 1: the row-pair copy (one load of the four taps); view_stride: elements of ST
 // per view of that ring.
-template <typename ST, int RING>
+//
+// STAGE (16-bit storage, row-pair copy, NQ = ZQL): the gathers come from LDS instead of
+// the texture path, whose rate (one 64-lane load instruction per ~16-24 cycles per CU)
+// bounds the gather form on volumes.  Per view the workgroup copies the detector box of
+// its chunk (record [16..19]) from the row-pair copy into LDS with direct 16-byte
+// buffer-to-LDS loads, the next view's box into the other buffer while the current view
+// is interpolated (one barrier per view); each point then reads its four taps with one
+// ds_read2_b32.  A view whose box does not fit ZL_BUF is gathered from memory.
+// 16 bytes at byte offset voff of buffer rs to LDS at lds + 16 * lane (lds uniform over
+// the wave): buffer_load_dwordx4 ... lds, no VGPRs, counted by vmcnt.
+__device__ inline void lds_load16(__amdgpu_buffer_rsrc_t rs, uint32_t* lds, int voff) {
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, lds, 16, voff, 0, 0, 0);
+}
+
+template <typename ST, int RING, bool STAGE, int NQ>
 __global__ __launch_bounds__(BP_THREADS) void bp_zcol_kernel(
     const ST* __restrict__ ring, int64_t view_stride, int pitch, int R, int C,
     const double* __restrict__ coef, const float* __restrict__ vscale,
     const mr_task* __restrict__ tasks, const Tile* __restrict__ tiles, int n_views, float scale,
     float* __restrict__ out, int xcd) {
-    __shared__ float4 rec4[BP_VPASS * BP_REC / 4];
+    static_assert(!STAGE || (RING == 1 && sizeof(ST) == 2), "staged boxes of the 16-bit row-pair copy");
+    __shared__ float4 rec4[BP_VPASS * ZREC / 4];
+    __shared__ __attribute__((aligned(16))) uint32_t sbox[STAGE ? 2 * ZL_BUF / 4 : 1];
     float* rec = reinterpret_cast<float*>(rec4);
     const int lb = xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const Tile tl = tiles[lb];
@@ -967,24 +1038,56 @@ __global__ __launch_bounds__(BP_THREADS) void bp_zcol_kernel(
     const int pitchb = pitch << SH;
     const int view_bytes = static_cast<int>(view_stride << LE);
     using Tap = typename std::conditional<RING == 1, PairP<ST>, Pair<ST>>::type;
-    float acc[ZQ];
+    auto rsrc = [&](int v) {
+        return __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<ST*>(ring + static_cast<int64_t>(v) * view_stride), 0, view_bytes, 0x00020000);
+    };
+    // the box of view vb + v (record rv) into LDS buffer bsel: 16-byte chunk k of the box
+    // (row k / nch, chunk k mod nch) to LDS byte 16 k; each wave copies runs of 64 chunks
+    auto stage = [&](int vv, const float* rv, int bsel) {
+        if constexpr (STAGE) {
+            const int pr0 = __float_as_int(rv[16]), nr = __float_as_int(rv[17]);
+            const int c0 = __float_as_int(rv[18]), nch = __float_as_int(rv[19]);
+            if (nr == 0) return;
+            const int total = nr * nch;
+            const float inv = 1.0f / static_cast<float>(nch);
+            const __amdgpu_buffer_rsrc_t rs = rsrc(vv);
+            const int lane = t & 63;
+            for (int k0 = (t >> 6) * 64; k0 < total; k0 += BP_THREADS) {
+                const int k = k0 + lane;
+                if (k < total) {
+                    // floor(k / nch): (k + 0.5) / nch lies at least 1 / (2 nch) from an integer
+                    const int row = static_cast<int>((static_cast<float>(k) + 0.5f) * inv);
+                    const int cc = k - row * nch;
+                    const int voff = (((pr0 + row) * pitch + c0) << 2) + (cc << 4);
+                    lds_load16(rs, sbox + bsel * (ZL_BUF / 4) + k0 * 4, voff);
+                }
+            }
+        }
+    };
+    float acc[NQ];
 #pragma unroll
-    for (int i = 0; i < ZQ; ++i) acc[i] = 0.0f;
+    for (int i = 0; i < NQ; ++i) acc[i] = 0.0f;
     for (int vb = 0; vb < n_views; vb += BP_VPASS) {
         const int nv = min(BP_VPASS, n_views - vb);
         __syncthreads();
-        if (t < 2 * nv) {
-            const int v = t >> 1;
-            make_zrecord(kt + static_cast<int64_t>(vb + v) * ZC, vscale[vb + v], t & 1, tl.q,
-                         tl.tp * ZT, tl.ts * ZT, R, C, pitch, rec + v * BP_REC);
+        {
+            const int part = t >> 6, v = t & 63;
+            if (v < nv && part < (STAGE ? 3 : 2))
+                make_zrecord<NQ>(kt + static_cast<int64_t>(vb + v) * ZC, vscale[vb + v], part, tl.q,
+                                 tl.tp * ZT, tl.ts * ZT, R, C, pitch, rec + v * ZREC);
         }
         __syncthreads();
+        if (STAGE) {
+            stage(vb, rec, 0);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();
+        }
         for (int v = 0; v < nv; ++v) {
-            const float4 r0 = rec4[v * 4 + 0], r1 = rec4[v * 4 + 1];
-            const float4 r2 = rec4[v * 4 + 2], r3 = rec4[v * 4 + 3];
-            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-                const_cast<ST*>(ring + static_cast<int64_t>(vb + v) * view_stride), 0, view_bytes,
-                0x00020000);
+            const float* rv = rec + v * ZREC;
+            const float4 r0 = rec4[v * (ZREC / 4) + 0], r1 = rec4[v * (ZREC / 4) + 1];
+            const float4 r2 = rec4[v * (ZREC / 4) + 2], r3 = rec4[v * (ZREC / 4) + 3];
+            if (STAGE && v + 1 < nv) stage(vb + v + 1, rv + ZREC, (v + 1) & 1);
             // once per column and view
             const float uh = fmaf(dp, r0.z, fmaf(ds, r0.y, r0.x));
             const float vh = fmaf(dp, r1.y, fmaf(ds, r1.x, r0.w));
@@ -992,33 +1095,66 @@ __global__ __launch_bounds__(BP_THREADS) void bp_zcol_kernel(
             const float rw = __builtin_amdgcn_rcpf(wh);
             const float col = __builtin_amdgcn_fmed3f(uh * rw, r2.y, r2.z);
             const float fc = __builtin_amdgcn_fractf(col);
-            const int cofs = (cvt_flr(col) + __float_as_int(r3.z) + __float_as_int(r3.w)) << SH;
             const float wt = rw * rw;
             const float row0 = vh * rw, drow = r3.y * rw;
-            // per plane: the row, then ZG gathers in flight before their lerps.  The group's
-            // rows start from an opaque copy of row0 and a scheduling barrier ends the
-            // group, so the compiler neither forms every plane's row up front nor hoists
-            // the next group's gathers (either holds ZQ more VGPRs: fewer waves)
+            const int ci = cvt_flr(col);
+            const int rref1 = __float_as_int(r3.z), cref1 = __float_as_int(r3.w);
+            bool staged = false;
+            if constexpr (STAGE) staged = __float_as_int(rv[17]) != 0;
+            if constexpr (STAGE) if (staged) {
+                // LDS byte address of (pair-row rref1 + r, column cref1 + ci) in the box:
+                // r * SB + cb, SB = 16 nch bytes per box row
+                const int pr0 = __float_as_int(rv[16]), c0 = __float_as_int(rv[18]);
+                const int sb = __float_as_int(rv[19]) << 4;
+                const int cb = (cref1 + ci - c0) * 4 + (rref1 - pr0) * sb;
+                const char* bx = reinterpret_cast<const char*>(sbox) + (v & 1) * ZL_BUF;
 #pragma unroll
-            for (int g = 0; g < ZQ; g += ZG) {
-                Tap tap[ZG];
-                float fr[ZG];
-                float rb = row0;
-                asm volatile("" : "+v"(rb));
-#pragma unroll
-                for (int i = 0; i < ZG; ++i) {
+                for (int i = 0; i < NQ; ++i) {
                     const float row = __builtin_amdgcn_fmed3f(
-                        fmaf(static_cast<float>(g + i), drow, rb), r2.w, r3.x);
-                    fr[i] = __builtin_amdgcn_fractf(row);
-                    tap[i].load(rs, __mul24(cvt_flr(row), pitchb) + cofs, pitchb);
-                }
-#pragma unroll
-                for (int i = 0; i < ZG; ++i) {
+                        fmaf(static_cast<float>(i), drow, row0), r2.w, r3.x);
+                    const uint32_t* q = reinterpret_cast<const uint32_t*>(bx + __mul24(cvt_flr(row), sb) + cb);
+                    PairP<ST> tap;
+                    tap.w0 = q[0];
+                    tap.w1 = q[1];
                     float top, bot;
-                    tap[i].lerp(fc, top, bot);
-                    acc[g + i] = fmaf(wt, fmaf(fr[i], bot - top, top), acc[g + i]);
+                    tap.lerp(fc, top, bot);
+                    const float fr = __builtin_amdgcn_fractf(row);
+                    acc[i] = fmaf(wt, fmaf(fr, bot - top, top), acc[i]);
                 }
-                __builtin_amdgcn_sched_barrier(0);
+            }
+            if (!staged) {
+                const __amdgpu_buffer_rsrc_t rs = rsrc(vb + v);
+                const int cofs = (ci + __mul24(rref1, pitch) + cref1) << SH;
+                // per plane: the row, then ZG gathers in flight before their lerps.  The
+                // group's rows start from an opaque copy of row0 and a scheduling barrier
+                // ends the group, so the compiler neither forms every plane's row up front
+                // nor hoists the next group's gathers (either holds NQ more VGPRs)
+#pragma unroll
+                for (int g = 0; g < NQ; g += ZG) {
+                    Tap tap[ZG];
+                    float fr[ZG];
+                    float rb = row0;
+                    asm volatile("" : "+v"(rb));
+#pragma unroll
+                    for (int i = 0; i < ZG; ++i) {
+                        const float row = __builtin_amdgcn_fmed3f(
+                            fmaf(static_cast<float>(g + i), drow, rb), r2.w, r3.x);
+                        fr[i] = __builtin_amdgcn_fractf(row);
+                        tap[i].load(rs, __mul24(cvt_flr(row), pitchb) + cofs, pitchb);
+                    }
+#pragma unroll
+                    for (int i = 0; i < ZG; ++i) {
+                        float top, bot;
+                        tap[i].lerp(fc, top, bot);
+                        acc[g + i] = fmaf(wt, fmaf(fr[i], bot - top, top), acc[g + i]);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            if (STAGE) {
+                // the next view's box is in LDS, and every wave is done with this one
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                __syncthreads();
             }
         }
     }
@@ -1028,7 +1164,7 @@ __global__ __launch_bounds__(BP_THREADS) void bp_zcol_kernel(
                            static_cast<int64_t>(tl.q) * T.stride_q;
         const int nq = T.nq - tl.q;
 #pragma unroll
-        for (int i = 0; i < ZQ; ++i)
+        for (int i = 0; i < NQ; ++i)
             if (i < nq) out[ob + static_cast<int64_t>(i) * T.stride_q] = acc[i] * scale;
     }
 }
@@ -2433,18 +2569,24 @@ int launch_bp_tiles_v4(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int 
     return MR_OK;
 }
 
-// The z-column kernel over the tiles of build_zcol_tiles (chunks of ZQ planes x ZT x ZT
-// columns): on the row-pair copy for 16-bit storage when there is room for it (one 8-byte
-// load per point-view), else on the ring (two).  n_views > 0.
-int launch_bp_zcol(Ctx* c, const mr_task* d_tasks, int ntasks, const Tile* d_tiles, int ntiles,
-                   int n_views, float scale, float* d_out, hipStream_t s) {
+// The form of the z-column kernel a volume request of n_views runs: 16-bit storage with
+// room for the row-pair copy, ZCOL_STAGED (boxes of ZQL-plane chunks through LDS); else
+// ZCOL_RING (ZQ-plane chunks gathered from the ring).  Builds or extends the copy.
+int zcol_form(Ctx* c, int n_views, hipStream_t s, int* form) {
+    *form = ZCOL_RING;
+    if (c->storage == MR_STORE_F32) return MR_OK;
+    const int rc = ensure_pair_ring(c, n_views, s);
+    if (rc == kNoRoom) return MR_OK;
+    if (rc) return rc;
+    *form = ZCOL_STAGED;
+    return MR_OK;
+}
+
+// The z-column kernel over the tiles of build_zcol_tiles (chunks of ZQL planes for
+// ZCOL_STAGED, ZQ for ZCOL_RING, x ZT x ZT columns).  n_views > 0.
+int launch_bp_zcol(Ctx* c, int form, const mr_task* d_tasks, int ntasks, const Tile* d_tiles,
+                   int ntiles, int n_views, float scale, float* d_out, hipStream_t s) {
     if (ntiles <= 0 || n_views <= 0) return MR_OK;
-    bool paired = false;
-    if (c->storage != MR_STORE_F32) {
-        const int rc = ensure_pair_ring(c, n_views, s);
-        if (rc && rc != kNoRoom) return rc;
-        paired = rc != kNoRoom;
-    }
     const int64_t pvs = pair_view_stride(c);
     const int xcd = c->xcd_map != 0 ? 1 : 0;
     // per (task, view) coefficients in the records buffer
@@ -2456,22 +2598,23 @@ int launch_bp_zcol(Ctx* c, const mr_task* d_tasks, int ntasks, const Tile* d_til
     hipLaunchKernelGGL(zcol_coef_kernel, dim3(static_cast<unsigned>((ncoef + 255) / 256)), dim3(256),
                        0, s, c->d_geom, d_tasks, ntasks, n_views, reinterpret_cast<double*>(c->d_recs));
     MR_HIP(hipEventRecord(c->evk0, s));
-#define MR_ZC(ST, RK)                                                                          \
-    hipLaunchKernelGGL((bp_zcol_kernel<ST, RK>), dim3(ntiles), dim3(BP_THREADS), 0, s,          \
+#define MR_ZC(ST, RK, STG, NQ)                                                                 \
+    hipLaunchKernelGGL((bp_zcol_kernel<ST, RK, STG, NQ>), dim3(ntiles), dim3(BP_THREADS), 0, s,  \
                        static_cast<const ST*>(RK ? c->pring : c->ring), RK ? pvs : c->view_stride, \
                        c->pitch, c->R, c->C, d_coef, c->d_vscale, d_tasks, d_tiles, n_views,      \
                        scale, d_out, xcd)
+    const bool staged = form == ZCOL_STAGED;
     switch (c->storage) {
     case MR_STORE_F32:
-        MR_ZC(float, 0);
+        MR_ZC(float, 0, false, ZQ);
         break;
     case MR_STORE_F16:
-        if (paired) MR_ZC(__half, 1);
-        else MR_ZC(__half, 0);
+        if (staged) MR_ZC(__half, 1, true, ZQL);
+        else MR_ZC(__half, 0, false, ZQ);
         break;
     default:
-        if (paired) MR_ZC(uint16_t, 1);
-        else MR_ZC(uint16_t, 0);
+        if (staged) MR_ZC(uint16_t, 1, true, ZQL);
+        else MR_ZC(uint16_t, 0, false, ZQ);
         break;
     }
 #undef MR_ZC
@@ -2479,7 +2622,7 @@ int launch_bp_zcol(Ctx* c, const mr_task* d_tasks, int ntasks, const Tile* d_til
     MR_HIP(hipEventRecord(c->evk1, s));
     c->tile_timed = true;
     c->tile_launches = 1;
-    c->kern_used = 8;
+    c->kern_used = staged ? 9 : 8;
     return MR_OK;
 }
 

@@ -312,9 +312,13 @@ int launch_bp_adjoint(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int n
                       int n_views, float scale, const float* d_g, float* d_gq, hipStream_t s);
 int launch_bp_dshift(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles, int n_views,
                      float scale, float* d_out, hipStream_t s);
-// the z-column kernel over z-column tiles (build_zcol_tiles) of a volume request; n_views > 0
-int launch_bp_zcol(Ctx* c, const mr_task* d_tasks, int ntasks, const Tile* d_tiles, int ntiles,
-                   int n_views, float scale, float* d_out, hipStream_t s);
+// the z-column kernel of a volume request: zcol_form picks its form (and builds the ring
+// copy it reads), build_zcol_tiles(tasks, n, form == ZCOL_STAGED ? ZQL : ZQ) its tiles;
+// n_views > 0
+enum { ZCOL_RING = 0, ZCOL_STAGED = 1 };
+int zcol_form(Ctx* c, int n_views, hipStream_t s, int* form);
+int launch_bp_zcol(Ctx* c, int form, const mr_task* d_tasks, int ntasks, const Tile* d_tiles,
+                   int ntiles, int n_views, float scale, float* d_out, hipStream_t s);
 int launch_filter_adjoint(Ctx* c, const float* d_gq, int k, float* d_graw, hipStream_t s);
 // COR entropy scores of nimg images, image i = npix[i] floats from d_img + offset[i].
 // d_ws: a device workspace of cor_workspace_bytes(nimg) bytes (owned by the caller: a

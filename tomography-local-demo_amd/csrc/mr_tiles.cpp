@@ -121,11 +121,11 @@ const TileCache& cached_tiles(std::vector<TileCache>& cache, const mr_task* task
     return cache.front();
 }
 
-void build_zcol_tiles(const mr_task* tasks, int ntasks, std::vector<Tile>& tiles) {
+void build_zcol_tiles(const mr_task* tasks, int ntasks, int zq, std::vector<Tile>& tiles) {
     tiles.clear();
     for (int i = 0; i < ntasks; ++i) {
         const mr_task& T = tasks[i];
-        for (int q = 0; q < T.nq; q += ZQ)
+        for (int q = 0; q < T.nq; q += zq)
             for (int tp = 0; tp * ZT < T.np; ++tp)
                 for (int ts = 0; ts * ZT < T.ns; ++ts) tiles.push_back(Tile{i, q, tp, ts});
     }

@@ -18,9 +18,11 @@ namespace mr {
 constexpr int BP_TS = 32;
 constexpr int BP_TP = 32;
 
-// Z-column tiles of volume requests: ZT x ZT point columns (one per thread) of ZQ planes.
+// Z-column tiles of volume requests: ZT x ZT point columns (one per thread) of ZQ planes
+// (gathers from memory) or ZQL planes (taps staged through LDS: a box of fewer rows).
 constexpr int ZT = 16;
 constexpr int ZQ = 16;
+constexpr int ZQL = 16;
 
 // Tile work item of a back-projection launch: task, z-plane q, tile row tp, tile column ts
 // (z-column tiles: the chunk's first plane q and ZT x ZT column tile (tp, ts)).
@@ -49,9 +51,9 @@ bool tile_may_hit_box(const mr_task& T, int q, int p0, int p1, int s0, int s1);
 void build_tiles(const mr_task* tasks, int ntasks, std::vector<Tile>& tiles,
                  std::vector<Tile>* empty = nullptr);
 
-// Tiles of the z-column kernel: every chunk of ZQ planes (from plane 0) x every ZT x ZT
+// Tiles of the z-column kernel: every chunk of zq planes (from plane 0) x every ZT x ZT
 // column tile of every task, in task, chunk, row, column order.
-void build_zcol_tiles(const mr_task* tasks, int ntasks, std::vector<Tile>& tiles);
+void build_zcol_tiles(const mr_task* tasks, int ntasks, int zq, std::vector<Tile>& tiles);
 
 // Orders tiles by the height z of their centre (stable).
 void order_tiles_by_height(const mr_task* tasks, std::vector<Tile>& tiles);

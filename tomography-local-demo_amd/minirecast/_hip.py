@@ -305,6 +305,10 @@ class Context:
     def set_bp_config(self, kernel=4, ppt=2, layout=0, depth=2):
         check(self.lib.mr_set_bp_config(self.h, int(kernel), int(ppt), int(layout), int(depth)))
 
+    def reset_bp_config(self):
+        """Back to the default configuration of the geometry's storage form and beam."""
+        check(self.lib.mr_set_bp_config(self.h, 0, 0, 0, 0))
+
     def bp_config(self):
         cfg = np.zeros(4, np.int32)
         check(self.lib.mr_get_bp_config(self.h, ptr(cfg)))
@@ -490,8 +494,9 @@ class Context:
         return n.value
 
     def last_bp_kernel(self):
-        """Back-projection kernel of the last bp_tasks call: 2, 4-7 (plane tiles), 8 (z
-        columns of a volume request), 0 when nothing was launched."""
+        """Back-projection kernel of the last bp_tasks call: 2, 4-7 (plane tiles), 8 or 9
+        (z columns of a volume request, gathered from the ring / staged through LDS), 0
+        when nothing was launched."""
         k = ctypes.c_int(0)
         check(self.lib.mr_last_bp_kernel(self.h, ctypes.byref(k)))
         return k.value
