@@ -158,8 +158,8 @@ def test_full_size_volume_vs_float64_oracle(full, name, tol):
     print(f"{name}: volume {tuple(int(d) for d in dims)} rel-L2 {err:.2e} at {len(got)} voxels "
           f"(kernel {kern})")
     assert np.isfinite(vol).all() and err < tol, (name, err)
-    # z columns: gathered from the ring (f32) or staged through LDS (16-bit)
-    assert kern == (8 if name == "f32" else 9), kern
+    # z columns, staged through LDS
+    assert kern == 9, kern
 
 
 @pytest.mark.parametrize("name,tol", [("f32", 1e-5), ("f16", 1e-3)])
@@ -183,4 +183,4 @@ def test_full_size_coarse_volume_vs_float64_oracle(full, name, tol):
     kern = ctx.last_bp_kernel()
     print(f"{name}: volume {tuple(int(d) for d in dims)} rel-L2 {err:.2e} (kernel {kern})")
     assert np.isfinite(vol).all() and err < tol, (name, err)
-    assert kern == (8 if name == "f32" else 9), kern
+    assert kern == 9, kern

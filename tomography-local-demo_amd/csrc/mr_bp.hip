@@ -840,6 +840,28 @@ void bp_tiles_kernel4(
     }
 }
 
+// Four taps from an LDS box with rows strb bytes apart: f32 two dwords per row; 16-bit
+// formats one dword per row at 2-byte alignment.
+template <typename ST> struct LdsPair;
+template <> struct LdsPair<float> {
+    __device__ static inline Pair<float> read(const unsigned char* __restrict__ p, int strb) {
+        Pair<float> t;
+        t.a = *reinterpret_cast<const float*>(p);
+        t.b = *reinterpret_cast<const float*>(p + 4);
+        t.c = *reinterpret_cast<const float*>(p + strb);
+        t.d = *reinterpret_cast<const float*>(p + strb + 4);
+        return t;
+    }
+};
+template <typename ST> struct LdsPair {       // 16-bit formats: one dword per row
+    __device__ static inline Pair<ST> read(const unsigned char* __restrict__ p, int strb) {
+        Pair<ST> t;
+        __builtin_memcpy(&t.w0, p, 4);            // 2-byte aligned: unaligned LDS access
+        __builtin_memcpy(&t.w1, p + strb, 4);
+        return t;
+    }
+};
+
 // ---------------------------------------------------------------- z columns (volumes)
 // When neither the detector column nor the depth of a point depends on z (P[0] and P[2]
 // have no z term: a circular orbit about the z axis with the detector's v axis along z,
@@ -860,9 +882,9 @@ void bp_tiles_kernel4(
 //   [9] [10] column clamp, [11] [12] row clamp (relative), [13] Vc: v_h per plane
 //   [14] rref + 1, [15] cref + 1: the padded reference position (int bits)
 //   [16] [17] [18] [19] (staged kernel) the detector box of the chunk's taps in the
-//       row-pair copy: first pair-row, pair-rows (0: the box does not fit the LDS
-//       buffer and the view is gathered from memory), first column (a multiple of 4),
-//       16-byte chunks per row (int bits)
+//       ring it reads: first row, rows (0: the box does not fit the LDS buffer and the
+//       view is gathered from memory), first column (a multiple of the elements in 16
+//       bytes), 16-byte chunks per row (int bits)
 // The tile-independent part, per (task, view), comes from zcol_coef_kernel (ZC doubles:
 // u_h, v_h, w_h at the task origin and their steps along a and b, v_h along c), so the
 // record build holds few float64 values (the full build from the view's 3 x 4 form held
@@ -913,7 +935,10 @@ __global__ __launch_bounds__(256) void zcol_coef_kernel(const ViewGeom* __restri
 // ~1e-4 px of the kernel's relative coordinates; margin 1/16 px), clamped as the kernel
 // clamps, padded by the second tap; a projective map with w > 0 at every corner sends the
 // box of points into the hull of the corner images.
-template <int NQ>
+// PAIRED: the box rows are pair-rows of the row-pair copy (one per tap pair); else rows of
+// the ring itself (a tap pair needs rows r and r + 1).  LEZ: log2 bytes per element of
+// the staged ring (16-byte chunks of 16 >> LEZ elements).
+template <int NQ, bool PAIRED, int LEZ>
 __device__ void make_zrecord(const double* __restrict__ k, float vs, int part, int q0, int p0,
                              int s0, int R, int C, int pitch, float* __restrict__ rec) {
     const float hc = 0.5f * (ZT - 1), hq = 0.5f * (NQ - 1);
@@ -948,12 +973,14 @@ __device__ void make_zrecord(const double* __restrict__ k, float vs, int part, i
         rmin = fminf(fmaxf(rmin, -1.0f), static_cast<float>(R));
         rmax = fminf(fmaxf(rmax, -1.0f), static_cast<float>(R));
         // padded: a tap pair at detector column c covers padded columns c + 1, c + 2; the
-        // pair-row of detector row r is padded row r + 1
-        const int c0 = max(0, static_cast<int>(floorf(cmin - m)) + 1) & ~3;
+        // pair-row of detector row r is padded row r + 1 (the ring: rows r + 1, r + 2)
+        constexpr int CE = 16 >> LEZ;      // elements per 16-byte chunk
+        const int c0 = max(0, static_cast<int>(floorf(cmin - m)) + 1) & ~(CE - 1);
         const int c1 = min(C + 2, static_cast<int>(floorf(cmax + m)) + 2);
         const int pr0 = max(0, static_cast<int>(floorf(rmin - m)) + 1);
-        const int pr1 = min(R + 1, static_cast<int>(floorf(rmax + m)) + 1);
-        const int nch = (c1 - c0) / 4 + 1;
+        const int pr1 = PAIRED ? min(R + 1, static_cast<int>(floorf(rmax + m)) + 1)
+                               : min(R + 2, static_cast<int>(floorf(rmax + m)) + 2);
+        const int nch = (c1 - c0) / CE + 1;
         const int nr = pr1 - pr0 + 1;
         const bool fits = ok && nr > 0 && nr * nch * 16 <= ZL_BUF;
         rec[16] = __int_as_float(pr0);
@@ -1003,13 +1030,14 @@ __device__ void make_zrecord(const double* __restrict__ k, float vs, int part, i
 // point), 1: the row-pair copy (one load of the four taps); view_stride: elements of ST
 // per view of that ring.
 //
-// STAGE (16-bit storage, row-pair copy, NQ = ZQL): the gathers come from LDS instead of
-// the texture path, whose rate (one 64-lane load instruction per ~16-24 cycles per CU)
-// bounds the gather form on volumes.  Per view the workgroup copies the detector box of
-// its chunk (record [16..19]) from the row-pair copy into LDS with direct 16-byte
-// buffer-to-LDS loads, the next view's box into the other buffer while the current view
-// is interpolated (one barrier per view); each point then reads its four taps with one
-// ds_read2_b32.  A view whose box does not fit ZL_BUF is gathered from memory.
+// STAGE (NQ = ZQL): the gathers come from LDS instead of the texture path, whose rate (one
+// 64-lane load instruction per ~16-24 cycles per CU) bounds the gather form on volumes.
+// Per view the workgroup copies the detector box of its chunk (record [16..19]) from the
+// ring (or the 16-bit row-pair copy) into LDS with direct 16-byte buffer-to-LDS loads,
+// the next view's box into the other buffer while the current view is interpolated (one
+// barrier per view); each point then reads its taps from LDS: one read per row (the ring;
+// 16-bit rows at 2-byte alignment, which LDS serves unaligned) or one for both rows (the
+// row-pair copy).  A view whose box does not fit ZL_BUF is gathered from memory.
 // 16 bytes at byte offset voff of buffer rs to LDS at lds + 16 * lane (lds uniform over
 // the wave): buffer_load_dwordx4 ... lds, no VGPRs, counted by vmcnt.
 __device__ inline void lds_load16(__amdgpu_buffer_rsrc_t rs, uint32_t* lds, int voff) {
@@ -1022,7 +1050,7 @@ __global__ __launch_bounds__(BP_THREADS) void bp_zcol_kernel(
     const double* __restrict__ coef, const float* __restrict__ vscale,
     const mr_task* __restrict__ tasks, const Tile* __restrict__ tiles, int n_views, float scale,
     float* __restrict__ out, int xcd) {
-    static_assert(!STAGE || (RING == 1 && sizeof(ST) == 2), "staged boxes of the 16-bit row-pair copy");
+    static_assert(!STAGE || RING == 0 || sizeof(ST) == 2, "staged boxes: the ring or a 16-bit row-pair copy");
     __shared__ float4 rec4[BP_VPASS * ZREC / 4];
     __shared__ __attribute__((aligned(16))) uint32_t sbox[STAGE ? 2 * ZL_BUF / 4 : 1];
     float* rec = reinterpret_cast<float*>(rec4);
@@ -1059,7 +1087,7 @@ __global__ __launch_bounds__(BP_THREADS) void bp_zcol_kernel(
                     // floor(k / nch): (k + 0.5) / nch lies at least 1 / (2 nch) from an integer
                     const int row = static_cast<int>((static_cast<float>(k) + 0.5f) * inv);
                     const int cc = k - row * nch;
-                    const int voff = (((pr0 + row) * pitch + c0) << 2) + (cc << 4);
+                    const int voff = (((pr0 + row) * pitch + c0) << SH) + (cc << 4);
                     lds_load16(rs, sbox + bsel * (ZL_BUF / 4) + k0 * 4, voff);
                 }
             }
@@ -1074,8 +1102,9 @@ __global__ __launch_bounds__(BP_THREADS) void bp_zcol_kernel(
         {
             const int part = t >> 6, v = t & 63;
             if (v < nv && part < (STAGE ? 3 : 2))
-                make_zrecord<NQ>(kt + static_cast<int64_t>(vb + v) * ZC, vscale[vb + v], part, tl.q,
-                                 tl.tp * ZT, tl.ts * ZT, R, C, pitch, rec + v * ZREC);
+                make_zrecord<NQ, RING == 1, SH>(kt + static_cast<int64_t>(vb + v) * ZC, vscale[vb + v],
+                                                part, tl.q, tl.tp * ZT, tl.ts * ZT, R, C, pitch,
+                                                rec + v * ZREC);
         }
         __syncthreads();
         if (STAGE) {
@@ -1106,18 +1135,25 @@ __global__ __launch_bounds__(BP_THREADS) void bp_zcol_kernel(
                 // r * SB + cb, SB = 16 nch bytes per box row
                 const int pr0 = __float_as_int(rv[16]), c0 = __float_as_int(rv[18]);
                 const int sb = __float_as_int(rv[19]) << 4;
-                const int cb = (cref1 + ci - c0) * 4 + (rref1 - pr0) * sb;
+                const int cb = ((cref1 + ci - c0) << SH) + (rref1 - pr0) * sb;
                 const char* bx = reinterpret_cast<const char*>(sbox) + (v & 1) * ZL_BUF;
 #pragma unroll
                 for (int i = 0; i < NQ; ++i) {
                     const float row = __builtin_amdgcn_fmed3f(
                         fmaf(static_cast<float>(i), drow, row0), r2.w, r3.x);
                     const uint32_t* q = reinterpret_cast<const uint32_t*>(bx + __mul24(cvt_flr(row), sb) + cb);
-                    PairP<ST> tap;
-                    tap.w0 = q[0];
-                    tap.w1 = q[1];
                     float top, bot;
-                    tap.lerp(fc, top, bot);
+                    if constexpr (RING == 1) {
+                        PairP<ST> tap;             // (q(r, c) | q(r+1, c)), same for c + 1
+                        tap.w0 = q[0];
+                        tap.w1 = q[1];
+                        tap.lerp(fc, top, bot);
+                    } else {
+                        // the ring: row r, then row r + 1 one box row further
+                        const Pair<ST> tap =
+                            LdsPair<ST>::read(reinterpret_cast<const unsigned char*>(q), sb);
+                        tap.lerp(fc, top, bot);
+                    }
                     const float fr = __builtin_amdgcn_fractf(row);
                     acc[i] = fmaf(wt, fmaf(fr, bot - top, top), acc[i]);
                 }
@@ -1285,26 +1321,6 @@ template <typename ST, int K>
 struct BoxStage {
     uint4 val[K];
     int lds[K];          // LDS byte offset of chunk k, -1 when the chunk is outside the box
-};
-
-template <typename ST> struct LdsPair;
-template <> struct LdsPair<float> {
-    __device__ static inline Pair<float> read(const unsigned char* __restrict__ p, int strb) {
-        Pair<float> t;
-        t.a = *reinterpret_cast<const float*>(p);
-        t.b = *reinterpret_cast<const float*>(p + 4);
-        t.c = *reinterpret_cast<const float*>(p + strb);
-        t.d = *reinterpret_cast<const float*>(p + strb + 4);
-        return t;
-    }
-};
-template <typename ST> struct LdsPair {       // 16-bit formats: one dword per row
-    __device__ static inline Pair<ST> read(const unsigned char* __restrict__ p, int strb) {
-        Pair<ST> t;
-        __builtin_memcpy(&t.w0, p, 4);            // 2-byte aligned: unaligned LDS access
-        __builtin_memcpy(&t.w1, p + strb, 4);
-        return t;
-    }
 };
 
 template <typename ST, int PPT>
@@ -2569,12 +2585,14 @@ int launch_bp_tiles_v4(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int 
     return MR_OK;
 }
 
-// The form of the z-column kernel a volume request of n_views runs: 16-bit storage with
-// room for the row-pair copy, ZCOL_STAGED (boxes of ZQL-plane chunks through LDS); else
-// ZCOL_RING (ZQ-plane chunks gathered from the ring).  Builds or extends the copy.
+// The form of the z-column kernel a volume request of n_views runs: ZCOL_STAGED (boxes of
+// ZQL-plane chunks through LDS) for f32 rings (staged from the ring) and for 16-bit rings
+// with room for the row-pair copy (staged from it); else ZCOL_RING (ZQ-plane chunks
+// gathered from the ring).  Builds or extends the copy.
 int zcol_form(Ctx* c, int n_views, hipStream_t s, int* form) {
+    *form = ZCOL_STAGED;
+    if (c->storage == MR_STORE_F32) return MR_OK;      // staged from the ring itself
     *form = ZCOL_RING;
-    if (c->storage == MR_STORE_F32) return MR_OK;
     const int rc = ensure_pair_ring(c, n_views, s);
     if (rc == kNoRoom) return MR_OK;
     if (rc) return rc;
@@ -2606,7 +2624,8 @@ int launch_bp_zcol(Ctx* c, int form, const mr_task* d_tasks, int ntasks, const T
     const bool staged = form == ZCOL_STAGED;
     switch (c->storage) {
     case MR_STORE_F32:
-        MR_ZC(float, 0, false, ZQ);
+        if (staged) MR_ZC(float, 0, true, ZQL);
+        else MR_ZC(float, 0, false, ZQ);
         break;
     case MR_STORE_F16:
         if (staged) MR_ZC(__half, 1, true, ZQL);
