@@ -66,6 +66,58 @@ def test_f16_ingest_vs_oracle_and_f32(C):
     assert rel_l2(img16, img32) < 1e-3
 
 
+def _round_half_products(q):
+    """q rounded to half precision (11 significant bits, nearest even) per view, through
+    the power-of-two scale that puts each view's largest |q| in [2^14, 2^15)."""
+    out = np.empty_like(q)
+    for v in range(q.shape[0]):
+        m = float(np.max(np.abs(q[v])))
+        s = np.float32(2.0 ** (np.frexp(m)[1] - 15)) if m > 0 else np.float32(1.0)
+        out[v] = (q[v] / s).astype(np.float16).astype(np.float32) * s
+    return out
+
+
+def test_f16_provisional_scale_and_fixups():
+    """f16 ingest picks a provisional scale from about 96 sampled rows per view (here every
+    3rd of 200), records the exact max |q| of each filtered view and filters again the
+    views whose largest value would overflow a half or fall below 2^6.  Views: ordinary
+    data; zeros on the sampled rows and values ~1e-6 elsewhere (the provisional scale 1
+    would leave them subnormal); sampled rows ~1e-3 and one unsampled row of 1e4 (the
+    provisional scale would overflow); all zeros.  Every stored product equals the f32
+    ring's value rounded to half precision (any admissible power-of-two scale stores the
+    same products) wherever it is normal under every admissible scale, and the slices
+    equal the f32 ring's to f16 rounding."""
+    from minirecast import recon
+    from minirecast.geometry import circular_cone_vectors
+    R, A, C = 200, 6, 130
+    g = circular_cone_vectors(A, 400.0, 600.0, R, C, 0.15)
+    rng = np.random.default_rng(96)
+    block = rng.uniform(0.0, 3.0, size=(R, A, C)).astype(np.float32)
+    sampled = np.arange(R) % 3 == 0
+    block[sampled, 1, :] = 0.0
+    block[:, 1, :] *= 1e-6
+    block[:, 2, :] *= 1e-3
+    block[100, 2, :] = 1e4
+    block[:, 3, :] = 0.0
+    r32 = recon.SliceReconstructor(block, g, window="hann", verbose=False)
+    r16 = recon.SliceReconstructor(block, g, window="hann", verbose=False, storage="f16")
+    q32, q16 = r32.filtered_views(), r16.filtered_views()
+    assert np.isfinite(q16).all()
+    want = _round_half_products(q32)
+    for v in range(A):
+        m = float(np.max(np.abs(q32[v])))
+        if m == 0.0:
+            assert not q16[v].any(), v
+            continue
+        e = np.frexp(m)[1]
+        normal = np.abs(q32[v]) >= 2.0 ** (e - 20)
+        np.testing.assert_array_equal(q16[v][normal], want[v][normal], err_msg=f"view {v}")
+        assert np.max(np.abs(q16[v] - q32[v])) <= 2.0 ** (e - 11), v
+    img32, _ = r32.reconstruct_plane((0, 0, 0), (1, 0.5, 0.8), n=40)
+    img16, _ = r16.reconstruct_plane((0, 0, 0), (1, 0.5, 0.8), n=40)
+    assert rel_l2(img16, img32) < 1e-3
+
+
 def test_fdk_filter_block_vs_oracle():
     from minirecast import recon
     V = O.circular_cone_vectors(5, 400.0, 600.0, 0.3, det_shift_u=1.5)

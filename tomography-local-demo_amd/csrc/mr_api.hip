@@ -688,7 +688,8 @@ static int append_views_locked(mr_ctx* c, const int64_t* indices, int k, const f
     MR_HIP(hipMemcpyAsync(c->d_geom + c->n, gv.data(), sizeof(ViewGeom) * k, hipMemcpyHostToDevice,
                           c->s_ingest));
     if (c->storage == MR_STORE_F16) {
-        rc = ensure(reinterpret_cast<void**>(&c->d_vmax), &c->vmax_len, sizeof(unsigned) * k);
+        // per view: max |x| of (sampled) raw rows, then max |q| of the filtered view
+        rc = ensure(reinterpret_cast<void**>(&c->d_vmax), &c->vmax_len, sizeof(unsigned) * kF16VmaxPerView * k);
         if (rc) return rc;
     }
     rc = filter ? launch_filter(c, d_block, k, c->n, c->s_ingest) : launch_store(c, d_block, k, c->s_ingest);
@@ -1625,7 +1626,7 @@ int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block
         if (c->storage == MR_STORE_F16) {
             // the graphs' own per-view max buffer: the context's d_vmax is regrown (freed)
             // by other entry points, which would leave a graph on freed memory
-            MR_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_vmax), sizeof(unsigned) * k));
+            MR_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_vmax), sizeof(unsigned) * kF16VmaxPerView * k));
         }
         p->kmax = k;
     }

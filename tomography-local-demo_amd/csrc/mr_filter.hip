@@ -32,6 +32,8 @@ constexpr int ft_of(int logn) { return logn >= 14 ? 1024 : logn >= 13 ? 512 : 25
 // four-step path.
 constexpr int kLdsMaxLogN = kFilterLdsLogN;
 constexpr int kMaxLogN = kFilterMaxLogN;
+// rows per view whose max |x| gives the provisional f16 scale (every ceil(R / 96)-th row)
+constexpr int kF16SampleRows = 96;
 static_assert(kMaxLogN == kLdsMaxLogN + 4, "four-step: 16-point column transforms");
 
 __device__ inline float2 cmul(float2 a, float2 b) {
@@ -224,55 +226,122 @@ template <int MODE> struct RowsIn {
         b0 = P.b0 + r0 * P.db;
         b1 = P.b0 + r1 * P.db;
     }
+    // Branch-free: the column is clamped into the row (x1 is row r0 itself when there is no
+    // second row) and the value masked by a multiply, so that the loads of all the inputs a
+    // thread reads are issued together rather than one guarded pair at a time (a select of
+    // the loaded value is turned back into a branch around the load).  In-range values are
+    // multiplied by exactly 1; a non-finite input already poisons its whole row pair
+    // through the transform.
     __device__ float2 operator()(int c) const {
-        float y0 = 0.0f, y1 = 0.0f;
-        if (c < C) {
-            y0 = x0[c];
-            y1 = has1 ? x1[c] : 0.0f;
-            if (MODE != 4 && P.cone) {
-                const float a = P.a0 + c * P.da;
-                const float aa = fmaf(a, a, P.sdd2);
-                y0 *= P.sdd * __frsqrt_rn(fmaf(b0, b0, aa));
-                y1 *= P.sdd * __frsqrt_rn(fmaf(b1, b1, aa));
-            }
+        const bool in = c < C;
+        const int cc = in ? c : C - 1;
+        float y0 = x0[cc] * (in ? 1.0f : 0.0f);
+        float y1 = x1[cc] * (in && has1 ? 1.0f : 0.0f);
+        if (MODE != 4) {
+            const float a = P.a0 + cc * P.da;
+            const float aa = fmaf(a, a, P.sdd2);
+            const float w0 = P.sdd * __frsqrt_rn(fmaf(b0, b0, aa));
+            const float w1 = P.sdd * __frsqrt_rn(fmaf(b1, b1, aa));
+            y0 = P.cone ? y0 * w0 : y0;
+            y1 = P.cone ? y1 * w1 : y1;
         }
         return make_float2(y0, y1);
     }
 };
 
+// f16 storage: the power-of-two scale 2^(e - 15) of a view whose filtered values are
+// bounded by `bound` < 2^e, so |q| / scale < 2^15 (1 for a zero or non-finite bound).
+// Any power-of-two scale stores the same products half(q / scale) * scale while q / scale
+// stays within the normal half range, so the scale only has to keep the view's largest
+// value between 2^6 and the largest half.
+__device__ inline float f16_scale(float bound) {
+    float sc = 1.0f;
+    if (bound > 0.0f && isfinite(bound)) {
+        int ex;
+        frexpf(bound, &ex);
+        sc = ldexpf(1.0f, ex - 15);
+    }
+    return sc;
+}
+// The provisional scale of batch view v (MODE 6): the bound max|q| <= max|x| gain l1 from
+// the max |x| of sampled rows, kept per part of the sample (sample_max_kernel stores each
+// part's bound; rounding is monotone, so their max is the bound of the max).
+__device__ inline float f16_prov_scale(const unsigned* vmax, int v) {
+    const unsigned* p = vmax + kF16SampleParts * v;
+    unsigned m = p[0];
+#pragma unroll
+    for (int j = 1; j < kF16SampleParts; ++j) m = max(m, p[j]);
+    return f16_scale(__uint_as_float(m));
+}
+// Max |q| slot j of batch view v: wave w of workgroup x of the view raises slot (x waves +
+// w) mod kF16QmaxSlots in a MODE 6 pass.  Slots lie in separate 128-byte lines: atomics on
+// one line (or one address) per view would serialise its ~2R atomics.
+template <typename U> __device__ inline U* f16_qmax_slot(U* vmax, int k, int v, int j) {
+    return vmax + kF16SampleParts * k + (kF16QmaxSlots * v + j) * kF16QmaxStride;
+}
+// The exact max |q| of batch view v gathered by a MODE 6 pass.
+__device__ inline float f16_view_qmax(const unsigned* vmax, int k, int v) {
+    unsigned m = *f16_qmax_slot(vmax, k, v, 0);
+#pragma unroll
+    for (int j = 1; j < kF16QmaxSlots; ++j) m = max(m, *f16_qmax_slot(vmax, k, v, j));
+    return __uint_as_float(m);
+}
+// Largest of a wave's non-negative floats (their bits order as unsigned integers): the
+// max within each row of 16 lanes by DPP moves (quad permutations, half-row and row
+// mirrors), then across the four rows by lane reads.
+__device__ inline float wave_max_nonneg(float x) {
+    unsigned u = __float_as_uint(x);
+    u = max(u, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(u), 0xB1, 0xF, 0xF, false)));
+    u = max(u, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(u), 0x4E, 0xF, 0xF, false)));
+    u = max(u, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(u), 0x141, 0xF, 0xF, false)));
+    u = max(u, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(u), 0x140, 0xF, 0xF, false)));
+    const unsigned a = __builtin_amdgcn_readlane(static_cast<int>(u), 0);
+    const unsigned b = __builtin_amdgcn_readlane(static_cast<int>(u), 16);
+    const unsigned c = __builtin_amdgcn_readlane(static_cast<int>(u), 32);
+    const unsigned d = __builtin_amdgcn_readlane(static_cast<int>(u), 48);
+    return __uint_as_float(max(max(a, b), max(c, d)));
+}
+// Whether a view filtered with scale sc and largest |q| qmax is stored as it should be:
+// qmax / sc within [2^6, 65504] (no half overflows, the largest values keep their place
+// far above the subnormals), or a view of zeros.  NaN fails.
+__device__ inline bool f16_scale_ok(float qmax, float sc) {
+    const float r = qmax * (1.0f / sc);
+    return qmax == 0.0f || (r >= 64.0f && r <= 65504.0f);
+}
+
 // Output of transform 2 (its conjugate, times the gain, is the filtered row pair):
 // MODE 0 ring float32; 1 ring bfloat16; 2 ring float16 with a power-of-two scale per view
 // from the bound max|q| <= max|x| * gain * l1 (vmax: max |x| of the view's raw rows, l1:
 // sum_m |h(m)| of the tau = 1 response), stored in vscale by the thread given `first`;
-// 3 plain float32 rows [R][C]; 4 adjoint (the cosine weight after the self-adjoint
-// filter, blocks [R][k][C]).  Columns >= C are dropped.
+// 6 ring float16 with the provisional scale sc6 (f16_prov_scale: max |x| of sampled rows),
+// the largest |q| written kept in *qm; 7 ring float16 with the scale of the exact
+// max |q| of the view (f16_view_qmax, gathered by the MODE 6 pass); 3 plain float32 rows
+// [R][C]; 4 adjoint (the cosine weight after the self-adjoint filter, blocks [R][k][C]).
+// Columns >= C are dropped.
 template <int MODE> struct RowsOut {
+    static constexpr bool F16 = MODE == 2 || MODE == 6 || MODE == 7;
     float* o0 = nullptr;
     float* o1 = nullptr;
     uint16_t* h0 = nullptr;
     uint16_t* h1 = nullptr;
+    float* qm = nullptr;
     FilterView P;
     float b0, b1, g, inv_s = 1.0f;
     int C;
     bool has1;
     __device__ RowsOut(void* dst, int R, int k, int C_, int pitch, int64_t view_stride,
                        const FilterView& P_, int r0, int v, const unsigned* vmax, float l1,
-                       float* vscale, bool first)
+                       float* vscale, bool first, float sc6 = 1.0f)
         : P(P_), C(C_) {
         const int r1 = r0 + 1;
         has1 = r1 < R;
         b0 = P.b0 + r0 * P.db;
         b1 = P.b0 + r1 * P.db;
         g = P.gain;
-        if (MODE == 2) {
-            // f16: scale 2^(e - 15) with the bound < 2^e, so |q| / scale < 2^15
-            const float bound = __uint_as_float(vmax[v]) * g * l1;
-            float sc = 1.0f;
-            if (bound > 0.0f && isfinite(bound)) {
-                int ex;
-                frexpf(bound, &ex);
-                sc = ldexpf(1.0f, ex - 15);
-            }
+        if (F16) {
+            const float sc = MODE == 2   ? f16_scale(__uint_as_float(vmax[v]) * g * l1)
+                             : MODE == 6 ? sc6
+                                         : f16_scale(f16_view_qmax(vmax, k, v));
             inv_s = 1.0f / sc;
             if (first) vscale[P.slot] = sc;
         }
@@ -281,7 +350,7 @@ template <int MODE> struct RowsOut {
             o0 = static_cast<float*>(dst) + static_cast<int64_t>(P.slot) * view_stride +
                  static_cast<int64_t>(r0 + 1) * pitch + 1;
             o1 = o0 + pitch;
-        } else if (MODE == 1 || MODE == 2) {
+        } else if (MODE == 1 || F16) {
             h0 = static_cast<uint16_t*>(dst) + static_cast<int64_t>(P.slot) * view_stride +
                  static_cast<int64_t>(r0 + 1) * pitch + 1;
             h1 = h0 + pitch;
@@ -307,9 +376,12 @@ template <int MODE> struct RowsOut {
             };
             h0[c] = bf(y0);
             if (has1) h1[c] = bf(y1);
-        } else if (MODE == 2) {
+        } else if (F16) {
             h0[c] = __half_as_ushort(__float2half_rn(y0 * inv_s));
             if (has1) h1[c] = __half_as_ushort(__float2half_rn(y1 * inv_s));
+            // (a lone last row's partner is the filter of a zero row, rounding noise far
+            // below the row's own values: one max3 per pair)
+            if (MODE == 6) *qm = fmaxf(*qm, fmaxf(fabsf(y0), fabsf(y1)));
         } else {
             float w0 = 1.0f, w1 = 1.0f;
             if (P.cone) {
@@ -343,22 +415,21 @@ struct SlotOut {
     __device__ void operator()(int m2, float2 z) const { dst[m2] = cmul(z, twn[m2 * k1]); }
 };
 
-// Grid (rows or slots, views): filter_kernel block (x, y) takes row pair x of view y
-// (MODE 0-4) or row x mod 16 of slot x / 16 (MODE 5, y = 0).
-//
 // Both transforms are forward Stockham FFTs through the LDS buffer; the second runs its
 // passes in the reverse radix order, so that its first pass (radix 16, span 1) reads in
 // each thread exactly the indices j + r N/16 the last pass of the first transform
 // leaves in that thread's registers: the product with H and the conjugation are done
 // there, with no exchange through LDS.  MODE 5: tw is the N2-point table, preceded by the
 // 16 N2-point one; H has 16 N2 points.
+//
+// filter_view: row pair x of view v (MODE 0-4, 6, 7), or row x mod 16 of slot x / 16
+// (MODE 5).
 template <int LOGN, int MODE>
-__global__ __launch_bounds__(ft_of(LOGN))
-__attribute__((amdgpu_waves_per_eu(LOGN >= 12 ? 4 : 5))) void filter_kernel(
+__device__ __forceinline__ void filter_view(
     const float* __restrict__ block, int R, int k, int C, const FilterView* __restrict__ fv,
     const float* __restrict__ H, const float2* __restrict__ tw, void* __restrict__ dst, int pitch,
-    int64_t view_stride, const unsigned* __restrict__ vmax, float l1,
-    float* __restrict__ vscale) {
+    int64_t view_stride, unsigned* __restrict__ vmax, float l1, float* __restrict__ vscale,
+    int x, int v) {
     constexpr int N = 1 << LOGN, FT = ft_of(LOGN);
     constexpr bool MID = MODE == 5;
     using P1 = FftPlan<LOGN, false>;
@@ -366,15 +437,17 @@ __attribute__((amdgpu_waves_per_eu(LOGN >= 12 ? 4 : 5))) void filter_kernel(
     constexpr int NP = P1::NPASS;
     static_assert(P1::radix(NP - 1) == P2::radix(0), "fused passes share the radix");
     static_assert(!MID || NP > 1, "the four-step middle transform uses the LDS passes");
+    // MODE 6: the provisional scale first, its loads in flight with the row loads
+    const float sc6 = MODE == 6 ? f16_prov_scale(vmax, v) : 1.0f;
     __shared__ float2 lds[N];
     float2* const buf = lds;                         // captured by the pass lambdas
-    const int k1 = MID ? static_cast<int>(blockIdx.x & 15) : 0;
-    const int64_t slot_row = (static_cast<int64_t>(blockIdx.x >> 4) * 16 + k1) * N;
+    const int k1 = MID ? (x & 15) : 0;
+    const int64_t slot_row = (static_cast<int64_t>(x >> 4) * 16 + k1) * N;
     auto in_rows = [&] {
         if constexpr (MID)
             return SlotIn{reinterpret_cast<const float2*>(block) + slot_row};
         else
-            return RowsIn<MODE>(block, R, k, C, fv[blockIdx.y], 2 * blockIdx.x, blockIdx.y);
+            return RowsIn<MODE>(block, R, k, C, fv[v], 2 * x, v);
     }();
     auto to_lds = [buf](int i, float2 z) { buf[sw(i)] = z; };
     constexpr int RF = P2::radix(0);                 // radix of the fused passes
@@ -420,10 +493,11 @@ __attribute__((amdgpu_waves_per_eu(LOGN >= 12 ? 4 : 5))) void filter_kernel(
         if constexpr (MID)
             return SlotOut{static_cast<float2*>(dst) + slot_row, tw - 16 * N, k1};
         else
-            return RowsOut<MODE>(dst, R, k, C, pitch, view_stride, fv[blockIdx.y],
-                                 2 * blockIdx.x, blockIdx.y, vmax, l1, vscale,
-                                 blockIdx.x == 0 && threadIdx.x == 0);
+            return RowsOut<MODE>(dst, R, k, C, pitch, view_stride, fv[v], 2 * x, v, vmax, l1,
+                                 vscale, x == 0 && threadIdx.x == 0, sc6);
     }();
+    float qm = 0.0f;                                 // MODE 6: largest |q| this thread writes
+    if constexpr (MODE == 6) out_rows.qm = &qm;
     if constexpr (NP == 1) {
         PF::template store<1>(out_rows, vf);
     } else {
@@ -438,6 +512,45 @@ __attribute__((amdgpu_waves_per_eu(LOGN >= 12 ? 4 : 5))) void filter_kernel(
         typename PL::Regs vl;
         PL::template load_dft<P2::span(NP - 1), false>(tw, [buf](int i) { return buf[sw(i)]; }, vl);
         PL::template store<P2::span(NP - 1)>(out_rows, vl);
+    }
+    if constexpr (MODE == 6) {
+        // the wave's largest |q|, one atomic per wave (non-negative floats order as their
+        // bits; no barrier for a workgroup-wide reduction)
+        qm = wave_max_nonneg(qm);
+        if ((threadIdx.x & 63) == 0)
+            atomicMax(f16_qmax_slot(vmax, k, v,
+                                    (x * (FT / 64) + static_cast<int>(threadIdx.x >> 6)) % kF16QmaxSlots),
+                      __float_as_uint(qm));
+    }
+}
+
+// Grid (rows or slots, views) for MODE 0-6: block (x, y) filters row pair x of view y (or
+// row x mod 16 of slot x / 16, MODE 5).  MODE 7 (the fix-up pass after MODE 6): grid (row
+// pairs, 1); block x filters row pair x of every batch view whose provisional scale does
+// not fit, and skips the others (usually all) after reading their maxima.
+template <int LOGN, int MODE>
+__global__ __launch_bounds__(ft_of(LOGN))
+__attribute__((amdgpu_waves_per_eu(LOGN >= 12 ? 4 : 5))) void filter_kernel(
+    const float* __restrict__ block, int R, int k, int C, const FilterView* __restrict__ fv,
+    const float* __restrict__ H, const float2* __restrict__ tw, void* __restrict__ dst, int pitch,
+    int64_t view_stride, unsigned* __restrict__ vmax, float l1,
+    float* __restrict__ vscale) {
+    const int x = blockIdx.x;
+    if constexpr (MODE == 7) {
+        // lane j of every wave checks view v0 + j; the ballot is the same in every wave
+        for (int v0 = 0; v0 < k; v0 += 64) {
+            const int v = v0 + static_cast<int>(threadIdx.x & 63);
+            const bool bad = v < k && !f16_scale_ok(f16_view_qmax(vmax, k, v), f16_prov_scale(vmax, v));
+            for (uint64_t m = __ballot(bad); m; m &= m - 1) {
+                filter_view<LOGN, MODE>(block, R, k, C, fv, H, tw, dst, pitch, view_stride, vmax,
+                                        l1, vscale, x, v0 + __builtin_ctzll(m));
+                // the next view reuses the LDS buffer
+                __syncthreads();
+            }
+        }
+    } else {
+        filter_view<LOGN, MODE>(block, R, k, C, fv, H, tw, dst, pitch, view_stride, vmax, l1,
+                                vscale, x, blockIdx.y);
     }
 }
 
@@ -482,28 +595,25 @@ __global__ __launch_bounds__(256) void fourstep_cols_kernel(
     }
 }
 
-// Per-view max |x| of raw blocks [R][k][C] (the f16 scale bound of filter_kernel).
-// Workgroup (x, v) takes rows x, x + gridDim.x, ... of view v, so that each view gets
-// gridDim.x (at most 128) atomics: one atomic per row would serialise ~R atomics on one
-// address.  A row is read as 16-byte loads from its first 16-byte boundary (two in flight
-// per thread), with the elements before it and after the last whole load read singly.
-__global__ __launch_bounds__(256) void input_max_kernel(const float* __restrict__ block, int R,
-                                                        int k, int C, unsigned* __restrict__ vmax) {
-    const int v = blockIdx.y;
+// Max |x| over rows x rs, (x + G) rs, (x + 2G) rs, ... (below R) of view v of a raw block
+// [R][k][C], reduced over the 256-thread workgroup; the result is in thread 0.  A row is
+// read as 16-byte loads from its first 16-byte boundary (two in flight per thread), with
+// the elements before it and after the last whole load read singly.
+__device__ float rows_max_abs(const float* __restrict__ block, int R, int k, int C, int v, int x,
+                              int G, int rs) {
     float m = 0.0f;
-    // every row starts on a 16-byte boundary: the workgroup's rows x, x + G, ... as one
-    // index space of (row, 16-byte load) pairs, four loads in flight per thread
+    const int ns = (R + rs - 1) / rs;                // rows 0, rs, 2 rs, ...
+    // every row starts on a 16-byte boundary: the workgroup's rows as one index space of
+    // (row, 16-byte load) pairs, four loads in flight per thread
     const int c4 = C >> 2;
-    const int nr = static_cast<int>(blockIdx.x) < R
-                       ? (R - 1 - static_cast<int>(blockIdx.x)) / static_cast<int>(gridDim.x) + 1
-                       : 0;
+    const int nr = x < ns ? (ns - 1 - x) / G + 1 : 0;
     const bool aligned = (C & 3) == 0 && (reinterpret_cast<uintptr_t>(block) & 15) == 0 &&
                          static_cast<int64_t>(nr) * c4 < (int64_t{1} << 31) - 1024;
     if (aligned) {
         const int total = nr * c4;
         auto at = [&](int j) {
             const int rr = j / c4, i = j - rr * c4;
-            const int r = static_cast<int>(blockIdx.x) + rr * static_cast<int>(gridDim.x);
+            const int r = (x + rr * G) * rs;
             return reinterpret_cast<const float4*>(block + (static_cast<int64_t>(r) * k + v) * C)[i];
         };
         auto mx = [](float4 a) {
@@ -516,7 +626,8 @@ __global__ __launch_bounds__(256) void input_max_kernel(const float* __restrict_
         }
         for (; j < total; j += 256) m = fmaxf(m, mx(at(j)));
     }
-    for (int r = aligned ? R : static_cast<int>(blockIdx.x); r < R; r += gridDim.x) {
+    for (int ir = aligned ? ns : x; ir < ns; ir += G) {
+        const int r = ir * rs;
         const float* src = block + (static_cast<int64_t>(r) * k + v) * C;
         const int head = static_cast<int>(((16 - (reinterpret_cast<uintptr_t>(src) & 15)) & 15) >> 2);
         const int h = head < C ? head : C;
@@ -538,14 +649,37 @@ __global__ __launch_bounds__(256) void input_max_kernel(const float* __restrict_
         const int tail = h + (nq << 2);
         if (t < C - tail) m = fmaxf(m, fabsf(src[tail + t]));
     }
-    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    m = wave_max_nonneg(m);
     __shared__ float red[4];
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-        atomicMax(vmax + v, __float_as_uint(m));
-    }
+    return fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+}
+
+// Per-view max |x| of raw blocks [R][k][C] (the f16 scale bound of MODE 2) into vmax[v]
+// (cleared before).  Workgroup (x, v) takes rows x, x + gridDim.x, ... of view v, so that
+// each view gets gridDim.x (at most 128) atomics: one atomic per row would serialise ~R
+// atomics on one address.
+__global__ __launch_bounds__(256) void input_max_kernel(const float* __restrict__ block, int R,
+                                                        int k, int C, unsigned* __restrict__ vmax) {
+    const int v = blockIdx.y;
+    const float m = rows_max_abs(block, R, k, C, v, blockIdx.x, gridDim.x, 1);
+    if (threadIdx.x == 0) atomicMax(vmax + v, __float_as_uint(m));
+}
+
+// The sampled bound of the provisional f16 scale (MODE 6): grid (kF16SampleParts, k);
+// workgroup (x, v) writes the bound max|x| gain l1 from the max |x| of sampled rows x,
+// x + kF16SampleParts, ... (rows 0, rs, 2 rs, ...) of view v to vmax[kF16SampleParts v +
+// x], a plain store of every entry (0 for none), and workgroup (0, v) clears the view's
+// max |q| slots, which the MODE 6 pass raises.  No clearing launch, no atomics.
+__global__ __launch_bounds__(256) void sample_max_kernel(const float* __restrict__ block, int R,
+                                                         int k, int C, int rs,
+                                                         const FilterView* __restrict__ fv,
+                                                         float l1, unsigned* __restrict__ vmax) {
+    const int v = blockIdx.y, x = blockIdx.x;
+    const float m = rows_max_abs(block, R, k, C, v, x, kF16SampleParts, rs);
+    if (threadIdx.x == 0) vmax[kF16SampleParts * v + x] = __float_as_uint(m * fv[v].gain * l1);
+    if (x == 0 && threadIdx.x < kF16QmaxSlots) *f16_qmax_slot(vmax, k, v, threadIdx.x) = 0u;
 }
 
 // F16 storage of already-filtered views: power-of-two scale per view so that
@@ -634,7 +768,7 @@ __global__ void vmax_kernel(const float* __restrict__ q, int R, int C, unsigned*
 template <int MODE>
 int launch_fourstep(int logn, hipStream_t s, const float* block, int R, int k, int C,
                     const FilterView* fv, const float* H, const float2* tw, void* dst, int pitch,
-                    int64_t view_stride, const unsigned* vmax, float l1, float* vscale,
+                    int64_t view_stride, unsigned* vmax, float l1, float* vscale,
                     float2* fs, size_t fs_bytes) {
     const int64_t N = int64_t{1} << logn;
     const int n2len = static_cast<int>(N / 16);
@@ -680,11 +814,15 @@ int launch_fourstep(int logn, hipStream_t s, const float* block, int R, int k, i
 template <int MODE>
 int launch_mode(int logn, dim3 grid, hipStream_t s, const float* block, int R, int k, int C,
                 const FilterView* fv, const float* H, const float2* tw, void* dst, int pitch,
-                int64_t view_stride, const unsigned* vmax = nullptr, float l1 = 0.0f,
+                int64_t view_stride, unsigned* vmax = nullptr, float l1 = 0.0f,
                 float* vscale = nullptr, float2* fs = nullptr, size_t fs_bytes = 0) {
-    if (logn > kLdsMaxLogN && logn <= kMaxLogN)
-        return launch_fourstep<MODE>(logn, s, block, R, k, C, fv, H, tw, dst, pitch, view_stride,
-                                     vmax, l1, vscale, fs, fs_bytes);
+    if (logn > kLdsMaxLogN && logn <= kMaxLogN) {
+        if constexpr (MODE == 6 || MODE == 7)
+            return fail(MR_EVALUE, "provisional f16 scales are for one-workgroup rows only");
+        else
+            return launch_fourstep<MODE>(logn, s, block, R, k, C, fv, H, tw, dst, pitch,
+                                         view_stride, vmax, l1, vscale, fs, fs_bytes);
+    }
 #define MR_FILTER_CASE(L)                                                                  \
     case L:                                                                                \
         hipLaunchKernelGGL((filter_kernel<L, MODE>), grid, dim3(ft_of(L)), 0, s, block, R, k, C, \
@@ -727,17 +865,38 @@ int launch_filter(Ctx* c, const float* d_block, int k, int n0, hipStream_t s,
         return launch_mode<1>(logn, grid, s, d_block, c->R, k, c->C, fv, c->d_H, c->d_tw,
                               c->ring, c->pitch, c->view_stride, nullptr, 0.0f, nullptr, c->d_fs,
                               c->fs_bytes);
-    // F16: the per-view max |x| of the raw rows bounds the filtered view (max |q| <=
-    // max |x| gain l1, the FDK weight is at most 1), so its scale is known before the
-    // filter writes the halves; the max buffer is cleared with a kernel so that the
-    // sequence can be graph-captured
-    hipLaunchKernelGGL(zero_u32_kernel, dim3((k + 255) / 256), dim3(256), 0, s, vmax, k);
-    hipLaunchKernelGGL(input_max_kernel, dim3(std::min(c->R, 128), k), dim3(256), 0, s, d_block,
-                       c->R, k, c->C, vmax);
+    // F16: the scale of a view must be fixed before the filter writes its halves.  The max
+    // |x| of the raw rows bounds the filtered view (max |q| <= max |x| gain l1, the FDK
+    // weight is at most 1).  Rows up to 2^kLdsMaxLogN / 2 columns: the max of about 96
+    // sampled rows per view gives a provisional scale (sample_max_kernel, which also clears
+    // the max |q| slots), the filter (MODE 6) writes with it and gathers the exact max |q|
+    // of each view, and a second launch (MODE 7) filters again, with the scale of that max,
+    // only the views whose largest value would overflow a half or sit below 2^6; its
+    // workgroups read the maxima of every view and return.  Since any power-of-two scale
+    // stores the same products there, the stored views equal those of a scale from the
+    // full max, and the full read of the batch for the bound (0.11 ms of a 0.87 ms batch of
+    // 40 full-size views) is gone.  Wider rows (four-step transform) read the whole batch
+    // for the bound first (MODE 2, vmax[v] cleared by a kernel so that the sequence can be
+    // graph-captured).
+    const float l1 = static_cast<float>(c->h_l1);
+    if (logn > kLdsMaxLogN) {
+        hipLaunchKernelGGL(zero_u32_kernel, dim3((k + 255) / 256), dim3(256), 0, s, vmax, k);
+        hipLaunchKernelGGL(input_max_kernel, dim3(std::min(c->R, 128), k), dim3(256), 0, s, d_block,
+                           c->R, k, c->C, vmax);
+        MR_HIP(hipGetLastError());
+        return launch_fourstep<2>(logn, s, d_block, c->R, k, c->C, fv, c->d_H, c->d_tw, c->ring,
+                                  c->pitch, c->view_stride, vmax, l1, c->d_vscale, c->d_fs,
+                                  c->fs_bytes);
+    }
+    const int rs = (c->R + kF16SampleRows - 1) / kF16SampleRows;
+    hipLaunchKernelGGL(sample_max_kernel, dim3(kF16SampleParts, k), dim3(256), 0, s, d_block, c->R,
+                       k, c->C, rs, fv, l1, vmax);
     MR_HIP(hipGetLastError());
-    return launch_mode<2>(logn, grid, s, d_block, c->R, k, c->C, fv, c->d_H, c->d_tw, c->ring,
-                          c->pitch, c->view_stride, vmax, static_cast<float>(c->h_l1),
-                          c->d_vscale, c->d_fs, c->fs_bytes);
+    int rc = launch_mode<6>(logn, grid, s, d_block, c->R, k, c->C, fv, c->d_H, c->d_tw, c->ring,
+                            c->pitch, c->view_stride, vmax, l1, c->d_vscale);
+    if (rc) return rc;
+    return launch_mode<7>(logn, dim3(grid.x, 1), s, d_block, c->R, k, c->C, fv, c->d_H, c->d_tw,
+                          c->ring, c->pitch, c->view_stride, vmax, l1, c->d_vscale);
 }
 
 int launch_store(Ctx* c, const float* d_q, int k, hipStream_t s) {

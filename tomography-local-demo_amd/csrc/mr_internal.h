@@ -57,6 +57,13 @@ struct FilterView {
     int32_t pad[3];
 };
 static_assert(sizeof(FilterView) == 48, "FilterView layout");
+// f16 ingest: per batch view, the scale bound of sampled raw rows in kF16SampleParts parts and
+// the max |q| of the filtered view gathered over kF16QmaxSlots addresses in separate cache
+// lines (entries of the max buffer: kF16VmaxPerView per view)
+constexpr int kF16SampleParts = 16;
+constexpr int kF16QmaxSlots = 16;
+constexpr int kF16QmaxStride = 32;     // entries between slots: each slot in its own 128-byte line
+constexpr int kF16VmaxPerView = kF16SampleParts + kF16QmaxSlots * kF16QmaxStride;
 
 // Device work queue of a persistent back-projection launch.
 struct TileQueue {
@@ -191,7 +198,7 @@ struct Ctx {
     size_t pre_bytes = 0;
     float* d_fstage = nullptr;       // f32 staging of a gradient (adjoint with a host output)
     size_t fstage_bytes = 0;
-    unsigned* d_vmax = nullptr;      // per-batch per-view max |x| or |q| (float bits)
+    unsigned* d_vmax = nullptr;      // per-batch per-view max |x| or |q| (float bits) [kF16VmaxPerView k]
     size_t vmax_len = 0;
     FilterView* d_fv = nullptr;
     size_t fv_len = 0;
@@ -231,7 +238,7 @@ struct Plan {
     void* h_meta = nullptr;          // FilterView[kmax] + ViewGeom[kmax] + int slots[kmax]
     void* d_meta = nullptr;
     size_t meta_bytes = 0;
-    unsigned* d_vmax = nullptr;      // f16 storage: per-view max |x| [kmax]
+    unsigned* d_vmax = nullptr;      // f16 storage: per-view max |x|, max |q| [kF16VmaxPerView kmax]
     std::vector<std::pair<int, hipGraphExec_t>> graphs;   // per batch size
     double last_ms = 0.0;
     uint64_t geom_epoch = 0;         // context geometry the plan was made for
