@@ -3,6 +3,7 @@
 // offsets that overflow int64), tile enumeration against a brute-force box test, the
 // height order, and the request cache against fresh enumerations over a sequence of
 // more distinct requests than the cache holds.  Exit status 0: every check held.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -159,6 +160,129 @@ int main() {
     // an empty request
     const TileCache& none = cached_tiles(cache, nullptr, 0, 0, true, true);
     CHECK(none.tiles.empty() && none.ne == 0, "empty request has tiles");
+
+    // shift groups of rotation-centre searches: random axial planes, each with shifts on a
+    // grid of 1/q px (random gaps), every task in exactly one group slot at its shift,
+    // every tile of every group once, culled tiles without in-box points
+    std::uniform_int_distribution<int> nset(1, 3), qpick(0, 2), span(1, 60), keep(0, 3);
+    for (int r = 0; r < 200; ++r) {
+        std::vector<mr_task> tasks;
+        int64_t off = 0;
+        int qmax = 1;
+        const int ns_ = nset(rng);
+        for (int g = 0; g < ns_; ++g) {
+            int64_t dummy = 0;
+            mr_task base = random_task(rng, &dummy);
+            base.nq = 1;
+            base.a[2] = 0.0;
+            base.b[2] = 0.0;
+            base.c[0] = base.c[1] = base.c[2] = 0.0;
+            const int q = 1 << qpick(rng);
+            qmax = std::max(qmax, q);
+            const double s_lo = std::uniform_int_distribution<int>(-40, 40)(rng) / 4.0;
+            const int m = span(rng);
+            for (int k = 0; k < m; ++k) {
+                if (k > 0 && keep(rng) == 0) continue;       // a gap
+                mr_task T = base;
+                T.cor_shift = s_lo + static_cast<double>(k) / q;
+                T.out_offset = off;
+                off += static_cast<int64_t>(T.np) * T.ns;
+                tasks.push_back(T);
+            }
+        }
+        // shuffle the task order
+        std::shuffle(tasks.begin(), tasks.end(), rng);
+        CorPlan P;
+        const bool ok = cor_shift_groups(tasks.data(), static_cast<int>(tasks.size()), 1.0, P);
+        if (tasks.size() < 2) {
+            CHECK(!ok, "a single task grouped");
+            continue;
+        }
+        CHECK(ok, "grid shifts not grouped (request %d)", r);
+        if (!ok) continue;
+        CHECK(P.q >= 1 && P.q <= qmax, "grid %d beyond the finest used %d", P.q, qmax);
+        std::vector<int> used(tasks.size(), 0);
+        for (const CorGroup& G : P.groups) {
+            CHECK(G.set >= 0 && G.set < static_cast<int>(P.sets.size()), "group set index");
+            CHECK(G.ns >= 1 && G.ns <= 6 * P.q + 1, "group size %d for q %d", G.ns, P.q);
+            const mr_task& B = tasks[P.sets[G.set].task];
+            for (int k = 0; k < kCorMaxShifts; ++k) {
+                if (k >= G.ns) {
+                    CHECK(G.out[k] == -1, "slot past the group size used");
+                    continue;
+                }
+                if (G.out[k] < 0) continue;
+                int hit = -1;
+                for (size_t i = 0; i < tasks.size(); ++i)
+                    if (tasks[i].out_offset == G.out[k]) hit = static_cast<int>(i);
+                CHECK(hit >= 0, "group slot names no task");
+                if (hit < 0) continue;
+                ++used[hit];
+                const mr_task& T = tasks[hit];
+                CHECK(std::fabs(T.cor_shift - (G.s0 + static_cast<double>(k) / P.q)) < 1e-12,
+                      "slot %d holds shift %g, group s0 %g q %d", k, T.cor_shift, G.s0, P.q);
+                mr_task a = T, b = B;
+                a.cor_shift = b.cor_shift = 0.0;
+                a.out_offset = b.out_offset = 0;
+                CHECK(std::memcmp(&a, &b, sizeof(a)) == 0, "task grouped with another plane");
+            }
+        }
+        for (size_t i = 0; i < tasks.size(); ++i) CHECK(used[i] == 1, "task %zu in %d slots", i, used[i]);
+        std::vector<std::vector<int>> cover(P.groups.size());
+        for (size_t gi = 0; gi < P.groups.size(); ++gi) {
+            const mr_task& T = tasks[P.sets[P.groups[gi].set].task];
+            cover[gi].assign(static_cast<size_t>(((T.np + BP_TP - 1) / BP_TP) * ((T.ns + BP_TS - 1) / BP_TS)), 0);
+        }
+        size_t zeros = 0;
+        for (size_t i = 0; i < P.tiles.size(); ++i) {
+            const CorTile& t = P.tiles[i];
+            CHECK(t.group >= 0 && t.group < static_cast<int>(P.groups.size()), "tile group index");
+            const mr_task& T = tasks[P.sets[P.groups[t.group].set].task];
+            const int nts = (T.ns + BP_TS - 1) / BP_TS;
+            cover[t.group][static_cast<size_t>(t.tp * nts + t.ts)]++;
+            CHECK((t.zero != 0) == (i >= P.tiles.size() - P.ne), "culled tiles not at the end");
+            if (t.zero) {
+                ++zeros;
+                for (int p = t.tp * BP_TP; p < std::min((t.tp + 1) * BP_TP, T.np); ++p)
+                    for (int s = t.ts * BP_TS; s < std::min((t.ts + 1) * BP_TS, T.ns); ++s)
+                        CHECK(!point_in_box(T, 0, p, s), "culled group tile has an in-box point");
+            }
+        }
+        CHECK(zeros == P.ne, "culled tile count");
+        for (const auto& cv : cover)
+            for (int v : cv) CHECK(v == 1, "group tile covered %d times", v);
+    }
+    {
+        // refusals: off-grid shifts, a shift twice, an oblique or volume task, too few per group
+        int64_t dummy = 0;
+        mr_task B = random_task(rng, &dummy);
+        B.nq = 1;
+        B.a[2] = B.b[2] = 0.0;
+        std::vector<mr_task> t(5, B);
+        const double grid[5] = {0.0, 0.25, 0.5, 0.75, 1.0};
+        for (int i = 0; i < 5; ++i) {
+            t[i].cor_shift = grid[i];
+            t[i].out_offset = i * 1000000;
+        }
+        CorPlan P;
+        CHECK(cor_shift_groups(t.data(), 5, 4.0, P) && P.q == 4 && P.groups.size() == 1, "grid refused");
+        std::vector<mr_task> u = t;
+        u[2].cor_shift = 0.3;
+        CHECK(!cor_shift_groups(u.data(), 5, 1.0, P) && P.groups.empty(), "off-grid shift grouped");
+        u = t;
+        u[2].cor_shift = 0.25;
+        CHECK(!cor_shift_groups(u.data(), 5, 1.0, P), "a shift twice grouped");
+        u = t;
+        u[3].a[2] = 0.01;
+        CHECK(!cor_shift_groups(u.data(), 5, 1.0, P), "oblique plane grouped");
+        u = t;
+        u[0].nq = 2;
+        CHECK(!cor_shift_groups(u.data(), 5, 1.0, P), "volume task grouped");
+        CHECK(!cor_shift_groups(t.data(), 5, 6.0, P), "fewer tasks per group than asked accepted");
+        u = t;
+        u[4].cor_shift = 40.0;                               // 160 grid steps: a second group
+        CHECK(cor_shift_groups(u.data(), 5, 1.0, P) && P.groups.size() == 2, "far shift not split");
+    }
     if (failures) std::fprintf(stderr, "%d check(s) failed\n", failures);
     else std::printf("tiles_check: all checks passed\n");
     return failures ? 1 : 0;

@@ -284,6 +284,93 @@ int run_zcol(Ctx* c, const mr_task* tasks, int ntasks, int n_views, float* d_out
     return MR_OK;
 }
 
+// Requests of a parallel geometry in the default configuration whose tasks are axial
+// planes in groups differing only in their rotation-centre shift, the shifts on a grid of
+// 1/q px (q = 1, 2, 4) with at least 4 per group on average (a COR search), run the
+// shift-group kernel; its detector rows must stay within the 32-bit offsets of the kernel.
+bool cor_request(const Ctx* c, const mr_task* tasks, int ntasks, int n_views, int nvt,
+                 CorPlan& plan) {
+    if (c->kind != MR_KIND_PARALLEL || c->bp_explicit || n_views <= 0) return false;
+    if (static_cast<int64_t>(nvt) * cor_row_floats(c->C) * 4 >= (int64_t{1} << 31)) return false;
+    return cor_shift_groups(tasks, ntasks, 4.0, plan);
+}
+
+// Device layout of a shift-group request in one buffer: rows [nsets][nvt][lw] floats,
+// coefficients [nsets][nvt][4] doubles, sets, groups, tiles (256-byte aligned parts).
+struct CorLayout {
+    size_t rows = 0, coef = 0, sets = 0, groups = 0, tiles = 0, total = 0;
+};
+CorLayout cor_layout(const CorPlan& P, int nvt, int C) {
+    auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+    CorLayout L;
+    const size_t nsets = P.sets.size();
+    L.rows = 0;
+    L.coef = up(sizeof(float) * nsets * nvt * cor_row_floats(C));
+    L.sets = L.coef + up(sizeof(double) * 4 * nsets * nvt);
+    L.groups = L.sets + up(sizeof(CorRowSet) * nsets);
+    L.tiles = L.groups + up(sizeof(CorGroup) * P.groups.size());
+    L.total = L.tiles + up(sizeof(CorTile) * P.tiles.size());
+    return L;
+}
+
+// Copies the tables of P into `buf` (layout L) on stream s through the pinned request
+// staging buffer and fills a launch description (d_tasks: the tasks on the device).
+int cor_prepare(Ctx* c, const CorPlan& P, const CorLayout& L, void* buf, const mr_task* d_tasks,
+                int nvt, CorLaunch& out, hipStream_t s) {
+    const size_t nt = L.total - L.sets;
+    int rc = ensure_host(&c->h_req, &c->h_req_bytes, nt);
+    if (rc) return rc;
+    char* h = static_cast<char*>(c->h_req);
+    std::memcpy(h, P.sets.data(), sizeof(CorRowSet) * P.sets.size());
+    std::memcpy(h + (L.groups - L.sets), P.groups.data(), sizeof(CorGroup) * P.groups.size());
+    std::memcpy(h + (L.tiles - L.sets), P.tiles.data(), sizeof(CorTile) * P.tiles.size());
+    char* d = static_cast<char*>(buf);
+    MR_HIP(hipMemcpyAsync(d + L.sets, h, nt, hipMemcpyHostToDevice, s));
+    // the staging buffer is reused by the next request: wait for the copy
+    MR_HIP(hipStreamSynchronize(s));
+    out.q = P.q;
+    out.nsets = static_cast<int>(P.sets.size());
+    out.ntiles = static_cast<int>(P.tiles.size());
+    out.d_tasks = d_tasks;
+    out.d_sets = reinterpret_cast<const CorRowSet*>(d + L.sets);
+    out.d_groups = reinterpret_cast<const CorGroup*>(d + L.groups);
+    out.d_tiles = reinterpret_cast<const CorTile*>(d + L.tiles);
+    out.d_rows = reinterpret_cast<float*>(d + L.rows);
+    out.d_coef = reinterpret_cast<double*>(d + L.coef);
+    out.lw = cor_row_floats(c->C);
+    out.nvt = nvt;
+    return MR_OK;
+}
+
+// The shift-group path of a request (cor_request).
+int run_cor(Ctx* c, const mr_task* tasks, int ntasks, int n_views, const CorPlan& P, float* d_out) {
+    const CorLayout L = cor_layout(P, n_views, c->C);
+    int rc = ensure(&c->d_cor, &c->cor_bytes, L.total);
+    if (rc) return rc;
+    const size_t tb = sizeof(mr_task) * ntasks;
+    rc = ensure(&c->d_req, &c->req_bytes, tb);
+    if (rc) return rc;
+    rc = ensure_host(&c->h_req, &c->h_req_bytes, tb);
+    if (rc) return rc;
+    std::memcpy(c->h_req, tasks, tb);
+    MR_HIP(hipMemcpyAsync(c->d_req, c->h_req, tb, hipMemcpyHostToDevice, c->s_req));
+    MR_HIP(hipStreamSynchronize(c->s_req));
+    CorLaunch CL;
+    rc = cor_prepare(c, P, L, c->d_cor, static_cast<const mr_task*>(c->d_req), n_views, CL, c->s_req);
+    if (rc) return rc;
+    CL.n_views = n_views;
+    CL.d_out = d_out;
+    c->tile_timed = false;
+    c->tile_launches = 1;
+    c->req_slices = true;
+    MR_HIP(hipEventRecord(c->ev0, c->s_req));
+    rc = launch_cor_groups(c, CL, c->s_req);
+    if (rc) return rc;
+    MR_HIP(hipEventRecord(c->ev1, c->s_req));
+    c->kern_used = 10;
+    return MR_OK;
+}
+
 // Uploads tasks + tiles and launches the tile kernel of `op` (device buffers), timing it.
 // For OP_BP and OP_DSHIFT every output element of every task is written: the tiles
 // culled by the box test are filled with zeros by their own small launch.
@@ -291,6 +378,11 @@ int run_tasks(Ctx* c, const mr_task* tasks, int ntasks, int n_views, float* d_ou
               int op = OP_BP, const float* d_g = nullptr) {
     if (op == OP_BP && zcol_request(c, tasks, ntasks, n_views))
         return run_zcol(c, tasks, ntasks, n_views, d_out);
+    if (op == OP_BP) {
+        CorPlan P;
+        if (cor_request(c, tasks, ntasks, n_views, n_views, P))
+            return run_cor(c, tasks, ntasks, n_views, P, d_out);
+    }
     // the tile list depends only on the tasks (the culled-tile list and the block order
     // on the operation and the XCD map): a request repeating one of the last few (a
     // viewer re-requesting its slices as views arrive, one call per plane) reuses it
@@ -502,7 +594,8 @@ int mr_ctx_destroy(mr_ctx* c) {
     hipDeviceSynchronize();
     free_geometry(c);
     void* bufs[] = {c->d_stage, c->d_fstage, c->d_vmax, c->d_fv, c->d_req, c->d_out, c->d_pts, c->d_misc,
-                    c->d_pre, c->d_recs, c->d_raw, c->d_vol, c->d_stat, c->d_corws, c->d_adj};
+                    c->d_pre, c->d_recs, c->d_raw, c->d_vol, c->d_stat, c->d_corws, c->d_adj,
+                    c->d_cor};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (c->h_req) hipHostFree(c->h_req);
@@ -1352,7 +1445,7 @@ static void plan_free(Plan* p) {
     if (p->cor_exec) hipGraphExecDestroy(p->cor_exec);
     p->cor_exec = nullptr;
     void* dev[] = {p->d_tasks, p->d_toff, p->d_queue, p->d_counters, p->d_out, p->d_block, p->d_meta,
-                   p->d_vmax, p->d_scores, p->d_acc, p->d_corws};
+                   p->d_vmax, p->d_scores, p->d_acc, p->d_corws, p->d_cor};
     for (void* d : dev)
         if (d) hipFree(d);
     if (p->h_block) hipHostFree(p->h_block);
@@ -1495,18 +1588,54 @@ int mr_plan_cor_scores(mr_plan* p, int n_views, double* scores) {
         p->d_corws = ws;
         p->cor_chunks = chunks;
     }
-    int rc0 = prepare_persistent_bp(c, n_views, c->s_req);
-    if (rc0) return rc0;
-    const int rk = v2_ring_kind(c);
-    if (p->cor_exec && (p->cor_ring != rk || p->cor_copies != c->copies_epoch)) {
-        // configuration changed or a ring copy was dropped since the capture
+    // a COR search (shift groups) runs the shift-group kernel on the ring itself, rows for
+    // up to n_total views; other images the persistent v2 launch
+    CorPlan CP;
+    const bool groups = cor_request(c, p->h_tasks.data(), p->ntasks, n_views, c->n_total, CP);
+    int rk = -1;
+    if (!groups) {
+        int rc0 = prepare_persistent_bp(c, n_views, c->s_req);
+        if (rc0) return rc0;
+        rk = v2_ring_kind(c);
+    }
+    if (p->cor_exec && (p->cor_groups != groups ||
+                        (!groups && (p->cor_ring != rk || p->cor_copies != c->copies_epoch)))) {
+        // the path, the configuration or a ring copy changed since the capture
         hipGraphExecDestroy(p->cor_exec);
         p->cor_exec = nullptr;
     }
-    // the device view count of the persistent launch
+    // the device view count of the launch
     MR_HIP(hipMemcpyAsync(p->d_counters + 2, &n_views, sizeof(int), hipMemcpyHostToDevice, c->s_req));
     MR_HIP(hipStreamSynchronize(c->s_req));
+    if (!p->cor_exec && groups) {
+        p->cor_groups = true;
+        const CorLayout L = cor_layout(CP, c->n_total, c->C);
+        if (p->d_cor) hipFree(p->d_cor);
+        p->d_cor = nullptr;
+        MR_HIP(hipMalloc(&p->d_cor, L.total));
+        CorLaunch CL;
+        int r = cor_prepare(c, CP, L, p->d_cor, p->d_tasks, c->n_total, CL, c->s_req);
+        if (r) return r;
+        CL.d_nviews = p->d_counters + 2;
+        CL.d_out = p->d_out;
+        hipGraph_t graph = nullptr;
+        MR_HIP(hipStreamBeginCapture(c->s_req, hipStreamCaptureModeThreadLocal));
+        r = launch_cor_groups(c, CL, c->s_req);
+        if (!r) r = launch_cor_scores(p->d_out, p->ntasks, p->cor_chunks, p->d_scores, p->d_corws, c->s_req);
+        hipError_t ec = hipStreamEndCapture(c->s_req, &graph);
+        if (r || ec != hipSuccess) {
+            if (graph) hipGraphDestroy(graph);
+            return r ? r : fail(MR_ERUNTIME, std::string("graph capture: ") + hipGetErrorString(ec));
+        }
+        hipError_t ei = hipGraphInstantiate(&p->cor_exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            p->cor_exec = nullptr;
+            return fail(MR_ERUNTIME, std::string("graph instantiate: ") + hipGetErrorString(ei));
+        }
+    }
     if (!p->cor_exec) {
+        p->cor_groups = false;
         p->cor_ring = rk;
         p->cor_copies = c->copies_epoch;
         TileQueue q;
@@ -1534,6 +1663,7 @@ int mr_plan_cor_scores(mr_plan* p, int n_views, double* scores) {
     MR_HIP(hipEventRecord(c->ev0, c->s_req));
     MR_HIP(hipGraphLaunch(p->cor_exec, c->s_req));
     MR_HIP(hipEventRecord(c->ev1, c->s_req));
+    c->kern_used = p->cor_groups ? 10 : 2;
     MR_HIP(hipMemcpyAsync(scores, p->d_scores, sizeof(double) * p->ntasks, hipMemcpyDeviceToHost, c->s_req));
     int rc = finish_timing(c);
     p->last_ms = c->last_kernel_ms;

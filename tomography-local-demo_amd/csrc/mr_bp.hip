@@ -18,6 +18,7 @@
 // near 1000 px (ulp 6e-5 px), which would dominate the error budget.
 #include <type_traits>
 
+#include "mr_device.h"
 #include "mr_internal.h"
 
 namespace mr {
@@ -41,24 +42,6 @@ template <> struct Store<uint16_t> {   // bfloat16 bits
 
 __device__ inline double dot3(const double* a, const double* b) {
     return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
-}
-
-// In-box mask of recon.py:253-265 with the points built exactly as numpy does:
-// pts[p, s] = (mo + t_s * me1) + t_p * me2, t_x = (x - mhalf) * mstep,
-// each product and sum rounded separately (no contraction).
-__device__ bool in_box(const mr_task& T, int p, int s) {
-#pragma clang fp contract(off)
-    const double ts = (static_cast<double>(s) - T.mhalf) * T.mstep;
-    const double tp = (static_cast<double>(p) - T.mhalf) * T.mstep;
-    bool ok = true;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const double a = ts * T.me1[d];
-        const double b = tp * T.me2[d];
-        const double x = (T.mo[d] + a) + b;
-        ok = ok && (fabs(x) <= T.box[d]);
-    }
-    return ok;
 }
 
 // FP64 record of one (tile, view): coefficients of uh, vh, wh as affine functions of
@@ -126,11 +109,7 @@ __device__ void make_record(const ViewGeom* __restrict__ gp, float vs, const mr_
 // the per-point address is one 24-bit multiply-add and one shift-add of absolute
 // byte offsets, the second detector row comes from the SGPR offset, the +1 column
 // from the immediate offset, and an out-of-range offset can only read zero.
-__device__ inline int cvt_flr(float x) {
-    int r;
-    asm volatile("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
-    return r;
-}
+
 // Band ring addressing.  Element (band b, column c) of a view holds the padded rows
 // 3b..3b+3 of column c (8 bytes, element c+1 follows); a band record's [14] is the byte
 // offset of (band k, reference column), its reference row placed so that padded row
@@ -208,15 +187,6 @@ template <> struct Taps<uint16_t> {
     }
 };
 
-// Workgroup IDs are dealt round-robin to the 8 XCDs (ID mod 8), each with its own L2.
-// Remapping ID b to tile x * per + min(x, rem) + b / 8 (x = b mod 8) hands every XCD a
-// contiguous run of the tile list; neighbouring tiles on a slice then share the
-// detector cache lines at their borders inside one L2.  The map is a bijection on
-// [0, nb) for any nb.
-__device__ inline int xcd_block(int b, int nb) {
-    const int x = b & 7, k = b >> 3, per = nb >> 3, rem = nb & 7;
-    return x * per + (x < rem ? x : rem) + k;
-}
 
 // Taps of one point as loaded by kernels 2 and 4: f32 keeps four floats; the 16-bit formats keep
 // the two row dwords packed until the lerp, where f16 halves feed v_fma_mix_f32

@@ -115,7 +115,7 @@ struct Ctx {
     int tile_launches = 0;           // tile-kernel dispatches of the request in flight
     int last_tile_launches = 0;      // ... of the last finished request
     int kern_used = 0;               // back-projection kernel of the request in flight
-                                     // (2, 4-7; 8: z columns)
+                                     // (2, 4-7; 8, 9: z columns; 10: COR shift groups)
     int last_kern = 0;               // ... of the last finished request
     double last_tile_ms = 0.0;
     int bp_variant = 4;              // back-projection inner loop (MINIRECAST_BP_VARIANT)
@@ -216,6 +216,8 @@ struct Ctx {
     size_t misc_bytes = 0;
     void* d_corws = nullptr;         // COR scoring workspace
     size_t corws_bytes = 0;
+    void* d_cor = nullptr;           // COR shift groups: detector rows, coefficients, tables
+    size_t cor_bytes = 0;
 };
 
 // A compiled request: tasks and their device work queue, re-run on demand or after
@@ -257,6 +259,8 @@ struct Plan {
     double* d_scores = nullptr;
     void* d_corws = nullptr;         // scoring workspace of the captured graph
     hipGraphExec_t cor_exec = nullptr;
+    void* d_cor = nullptr;           // COR shift groups of the captured graph (cor_groups)
+    bool cor_groups = false;         // the captured graph runs the shift-group kernel
 };
 
 int ensure(void** p, size_t* have, size_t need);   // device buffer growth
@@ -323,6 +327,27 @@ int launch_bp_dshift(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int nt
 // copy it reads), build_zcol_tiles(tasks, n, form == ZCOL_STAGED ? ZQL : ZQ) its tiles;
 // n_views > 0
 enum { ZCOL_RING = 0, ZCOL_STAGED = 1 };
+// Shift groups of a rotation-centre search (parallel beam, cor_shift_groups): detector
+// rows per set and view, then the group kernel.  With d_nviews (graph replays) the view
+// count is read on the device and rows are prepared for up to nvt views.
+struct CorLaunch {
+    int q = 0;                       // shifts s0 + k / q
+    int nsets = 0, ntiles = 0;
+    const mr_task* d_tasks = nullptr;
+    const CorRowSet* d_sets = nullptr;
+    const CorGroup* d_groups = nullptr;
+    const CorTile* d_tiles = nullptr;
+    float* d_rows = nullptr;         // [nsets][nvt][lw]
+    double* d_coef = nullptr;        // [nsets][nvt][4]
+    int lw = 0, nvt = 0;             // floats per row (>= C + 2 kCorPad), views per set
+    const int* d_nviews = nullptr;
+    int n_views = 0;
+    float* d_out = nullptr;
+};
+int launch_cor_groups(Ctx* c, const CorLaunch& L, hipStream_t s);
+// floats per detector row of a COR shift set
+inline int cor_row_floats(int C) { return (C + 2 * kCorPad + 3) / 4 * 4; }
+
 int zcol_form(Ctx* c, int n_views, hipStream_t s, int* form);
 int launch_bp_zcol(Ctx* c, int form, const mr_task* d_tasks, int ntasks, const Tile* d_tiles,
                    int ntiles, int n_views, float scale, float* d_out, hipStream_t s);

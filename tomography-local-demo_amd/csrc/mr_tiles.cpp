@@ -2,7 +2,9 @@
 #include "mr_tiles.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
+#include <map>
 #include <utility>
 
 namespace mr {
@@ -129,6 +131,100 @@ void build_zcol_tiles(const mr_task* tasks, int ntasks, int zq, std::vector<Tile
             for (int tp = 0; tp * ZT < T.np; ++tp)
                 for (int ts = 0; ts * ZT < T.ns; ++ts) tiles.push_back(Tile{i, q, tp, ts});
     }
+}
+
+bool cor_shift_groups(const mr_task* tasks, int ntasks, double min_avg, CorPlan& plan) {
+    plan = CorPlan{};
+    if (ntasks < 2 || !tasks) return false;
+    // sets: the task bytes with the shift and the output offset cleared
+    std::map<std::vector<uint8_t>, int> key_set;
+    std::vector<std::vector<int>> members;
+    for (int i = 0; i < ntasks; ++i) {
+        const mr_task& T = tasks[i];
+        if (T.nq != 1 || T.np < 1 || T.ns < 1 || T.a[2] != 0.0 || T.b[2] != 0.0 ||
+            !std::isfinite(T.cor_shift))
+            return false;
+        mr_task K = T;
+        K.cor_shift = 0.0;
+        K.out_offset = 0;
+        const uint8_t* kb = reinterpret_cast<const uint8_t*>(&K);
+        std::vector<uint8_t> key(kb, kb + sizeof(K));
+        auto it = key_set.find(key);
+        if (it == key_set.end()) {
+            it = key_set.emplace(std::move(key), static_cast<int>(members.size())).first;
+            members.emplace_back();
+        }
+        members[it->second].push_back(i);
+    }
+    // per set: shifts in increasing order and their grid steps m for each q
+    std::vector<std::vector<std::pair<double, int>>> sorted(members.size());
+    for (size_t g = 0; g < members.size(); ++g) {
+        for (int i : members[g]) sorted[g].push_back({tasks[i].cor_shift, i});
+        std::sort(sorted[g].begin(), sorted[g].end());
+    }
+    auto steps = [&](size_t g, int q, std::vector<int64_t>& m) {
+        m.clear();
+        const double s_min = sorted[g].front().first;
+        for (const auto& e : sorted[g]) {
+            const double x = (e.first - s_min) * q;
+            const double r = std::nearbyint(x);
+            if (std::fabs(x - r) > 1e-9 * std::max(1.0, std::fabs(x)) || r > 1e9) return false;
+            const int64_t mi = static_cast<int64_t>(r);
+            if (!m.empty() && mi <= m.back()) return false;      // the same shift twice
+            m.push_back(mi);
+        }
+        return true;
+    };
+    int q = 0;
+    std::vector<int64_t> m;
+    for (int cand : {1, 2, 4}) {
+        bool ok = true;
+        for (size_t g = 0; g < sorted.size() && ok; ++g) ok = steps(g, cand, m);
+        if (ok) {
+            q = cand;
+            break;
+        }
+    }
+    if (!q) return false;
+    CorPlan P;
+    P.q = q;
+    for (size_t g = 0; g < sorted.size(); ++g) {
+        steps(g, q, m);
+        const double s_min = sorted[g].front().first;
+        P.sets.push_back(CorRowSet{sorted[g].front().second, {0, 0, 0}});
+        for (size_t j = 0; j < m.size();) {
+            CorGroup G;
+            std::memset(&G, 0, sizeof(G));
+            G.set = static_cast<int32_t>(g);
+            G.s0 = s_min + static_cast<double>(m[j]) / q;
+            for (int k = 0; k < kCorMaxShifts; ++k) G.out[k] = -1;
+            const int64_t m0 = m[j];
+            for (; j < m.size() && m[j] - m0 <= 6 * q; ++j) {
+                const int k = static_cast<int>(m[j] - m0);
+                G.out[k] = tasks[sorted[g][j].second].out_offset;
+                G.ns = k + 1;
+            }
+            P.groups.push_back(G);
+        }
+    }
+    if (static_cast<double>(ntasks) < min_avg * static_cast<double>(P.groups.size())) return false;
+    std::vector<CorTile> empty;
+    for (size_t gi = 0; gi < P.groups.size(); ++gi) {
+        const mr_task& T = tasks[P.sets[P.groups[gi].set].task];
+        const int ntp = (T.np + BP_TP - 1) / BP_TP, nts = (T.ns + BP_TS - 1) / BP_TS;
+        for (int tp = 0; tp < ntp; ++tp)
+            for (int ts = 0; ts < nts; ++ts) {
+                const int p0 = tp * BP_TP, s0 = ts * BP_TS;
+                const int p1 = std::min(p0 + BP_TP, T.np) - 1;
+                const int s1 = std::min(s0 + BP_TS, T.ns) - 1;
+                const bool hit = tile_may_hit_box(T, 0, p0, p1, s0, s1);
+                (hit ? P.tiles : empty).push_back(CorTile{static_cast<int32_t>(gi), tp, ts, hit ? 0 : 1});
+            }
+    }
+    P.ne = empty.size();
+    P.tiles.insert(P.tiles.end(), empty.begin(), empty.end());
+    plan = std::move(P);
+    return true;
 }
 
 }  // namespace mr

@@ -55,6 +55,45 @@ void build_tiles(const mr_task* tasks, int ntasks, std::vector<Tile>& tiles,
 // column tile of every task, in task, chunk, row, column order.
 void build_zcol_tiles(const mr_task* tasks, int ntasks, int zq, std::vector<Tile>& tiles);
 
+// ---- shift groups of rotation-centre searches (parallel beam, axial planes)
+// A group: tasks that differ only in cor_shift and out_offset, with shifts s0 + k / q
+// (q = 1, 2 or 4, k < 6 q + 1): their columns lie within 6 px of the first one, so the
+// kCorWin detector values from floor(col + s0) on hold every tap of every shift.
+constexpr int kCorWin = 8;              // detector values per point and view
+constexpr int kCorMaxShifts = 25;       // 6 q + 1 for q = 4
+constexpr int kCorPad = 16;             // zero columns on each side of a set's detector rows
+
+// A set of tasks equal but for cor_shift and out_offset: one detector row per view.
+struct CorRowSet {
+    int32_t task;                       // a task of the set (plane, mask, strides)
+    int32_t pad[3];
+};
+struct CorGroup {
+    int32_t set;
+    int32_t ns;                         // outputs k < ns: shift s0 + k / q
+    int32_t pad[2];
+    double s0;
+    int64_t out[kCorMaxShifts];         // out_offset of the task with shift s0 + k / q; -1 none
+};
+// 32 x 32 tile (tp, ts) of a group; zero: no point of the tile lies in the box.
+struct CorTile {
+    int32_t group, tp, ts, zero;
+};
+struct CorPlan {
+    int q = 0;
+    std::vector<CorRowSet> sets;
+    std::vector<CorGroup> groups;
+    std::vector<CorTile> tiles;          // tiles that may hit the box, then the others
+    size_t ne = 0;                       // tiles with zero = 1 (at the end)
+};
+
+// Fills `plan` and returns true when every task is an axial plane (nq = 1, a[2] = b[2] =
+// 0, finite shift) and the tasks fall into sets whose shifts lie on one grid s0 + m / q
+// with q = 1, 2 or 4 (the smallest that fits every set; no shift twice in a set), split
+// into groups spanning at most 6 q grid steps, with at least min_avg tasks per group on
+// average.  Otherwise returns false and leaves `plan` empty.
+bool cor_shift_groups(const mr_task* tasks, int ntasks, double min_avg, CorPlan& plan);
+
 // Orders tiles by the height z of their centre (stable).
 void order_tiles_by_height(const mr_task* tasks, std::vector<Tile>& tiles);
 
