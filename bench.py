@@ -414,7 +414,10 @@ def bench_parallel_cor(args):
     for the float32 and float16 rings (per-view scale) as one request per slice size, as
     one replayed graph per slice size (persistent launch + scoring), as one request of all
     75 trial images, and as one graph of all 75 (one persistent launch over the tiles of
-    the three sizes + scoring; float32 is the line's value)."""
+    the three sizes + scoring; float32 is the line's value).  The trial images of a slice
+    differ only by their shift on a 0.25 px grid, so every form runs the shift-group
+    kernel (kernel 10: one window of detector values per point and view serves all 25
+    shifts of a slice)."""
     from minirecast import _hip, phantom, recon, workloads
     w = workloads.parallel_cor()
     g = w.geom
@@ -476,6 +479,7 @@ def bench_parallel_cor(args):
             t0 = time.perf_counter()
             gsc = one.cor_scores(rec.n)
             gtimes.append((time.perf_counter() - t0) * 1e3)
+        kernel = ctx.last_bp_kernel()
         one.close()
         cuts = np.cumsum([len(T) for T in groups])[:-1]
         best = lambda x: [float(shifts[int(np.argmax(y))]) for y in np.split(x, cuts)]
@@ -489,7 +493,7 @@ def bench_parallel_cor(args):
                               one_request_best_shift_per_slice=best(osc),
                               one_graph_ms_per_search=float(np.median(gtimes)),
                               one_graph_gups=upd / (float(np.median(gtimes)) * 1e-3) / 1e9,
-                              one_graph_best_shift_per_slice=best(gsc))
+                              one_graph_best_shift_per_slice=best(gsc), kernel=kernel)
         del rec, ctx, plans, one                         # frees the context's rings
     res = dict(forms, trial_images=25 * 3, updates=upd)
     log(f"parallel_cor: {res}")

@@ -214,7 +214,8 @@ int mr_clip_histogram_static(mr_ctx* ctx, const float* vol_device, int64_t n, do
  * Until this is called the configuration follows the storage form of the geometry;
  * kernel 0 (other arguments ignored) returns to that default.  Volume requests of a cone
  * geometry whose detector column and depth do not depend on z run the z-column kernel
- * (8 or 9, see mr_last_bp_kernel) in the default configuration only. */
+ * (8 or 9, see mr_last_bp_kernel), and rotation-centre searches of a parallel geometry
+ * the shift-group kernel (10), in the default configuration only. */
 int mr_set_bp_config(mr_ctx* ctx, int kernel, int ppt, int layout, int depth);
 int mr_get_bp_config(mr_ctx* ctx, int* cfg);
 
@@ -255,7 +256,9 @@ int mr_get_bp_views_per_pass(mr_ctx* ctx, int* views);
  * (nq = 1; out_offset/strides are ignored; sizes may differ, so the trial images of
  * several slices can be scored in one call): it is back-projected into a device buffer
  * and scored on the device as sum p ln p of a 128-bin histogram of its non-zero values
- * (np.histogram range min..max).  scores: host [ntasks] float64. */
+ * (np.histogram range min..max).  scores: host [ntasks] float64.  With a parallel
+ * geometry, trial images of axial slices whose shifts lie on a grid of 1, 1/2 or 1/4 px
+ * are back-projected by the shift-group kernel (kernel 10, mr_last_bp_kernel). */
 int mr_cor_scores(mr_ctx* ctx, const mr_task* tasks, int ntasks, int n_views, double* scores);
 
 /* ---- differentiation (no counterpart in the reference: the transposes of the maps above)
@@ -323,9 +326,12 @@ int mr_plan_update(mr_plan* plan, float* out_host);
 /* Rotation-centre scores (recon.py:498-513) of a plan whose tasks are trial images:
  * single planes of contiguous rows (nq = 1, stride_p = ns, stride_s = 1), any sizes (one
  * plan can hold the searches of several slices).  Back-projects them from the first
- * n_views stored views (< 0: all) in the plan's persistent launch and scores each image
- * (histogram entropy of its non-zero values, as mr_cor_scores) on the device, both in one
- * HIP graph captured on the first call and replayed after; scores[ntasks]. */
+ * n_views stored views (< 0: all) in the plan's persistent launch, or with the
+ * shift-group kernel when mr_cor_scores would use it (the plan keeps detector rows for
+ * every view the ring holds), and scores each image (histogram entropy of its non-zero
+ * values, as mr_cor_scores) on the device, both in one HIP graph captured on the first
+ * call and replayed after (captured again when the kernel choice changes);
+ * scores[ntasks]. */
 int mr_plan_cor_scores(mr_plan* plan, int n_views, double* scores);
 
 /* Device pointer and length of the plan's output. */
@@ -376,12 +382,16 @@ int mr_last_tile_kernel_ms(mr_ctx* ctx, double* ms);
  * only). */
 int mr_last_tile_kernel_launches(mr_ctx* ctx, int* n);
 
-/* The back-projection kernel of the last mr_bp_tasks: 2 (records in LDS, the work-queue
- * loop), 4, 5, 6 or 7 (the plane-tile kernels), 8 or 9 (the z-column kernel of volume
- * requests: a cone geometry whose detector column and depth do not depend on z, tasks
- * stacking planes along z; 8 gathers from the ring, 9 stages each view's detector box of
- * the 16-bit row-pair copy through LDS), 0 when nothing was launched.  No counterpart in
- * the reference (introspection only). */
+/* The back-projection kernel of the last mr_bp_tasks, mr_cor_scores or
+ * mr_plan_cor_scores: 2 (records in LDS, the work-queue loop), 4, 5, 6 or 7 (the
+ * plane-tile kernels), 8 or 9 (the z-column kernel of volume requests: a cone geometry
+ * whose detector column and depth do not depend on z, tasks stacking planes along z; 8
+ * gathers from the ring, 9 stages each view's detector box of the 16-bit row-pair copy
+ * through LDS), 10 (the shift-group kernel of rotation-centre searches: a parallel
+ * geometry, axial planes that differ only in cor_shift, the shifts on a grid of 1, 1/2 or
+ * 1/4 px, at least 4 per plane on average), 0 when nothing was launched.  Kernels 8-10
+ * run in the default configuration only.  No counterpart in the reference
+ * (introspection only). */
 int mr_last_bp_kernel(mr_ctx* ctx, int* kernel);
 
 /* L2 gather microbenchmark (no counterpart in the reference; measurement only): the rate
