@@ -308,7 +308,8 @@ int mr_plan_run(mr_plan* plan, int n_views, float* out_host);
 /* Streaming update (recon.py:346-376 followed by the plan's requests): ingests k views
  * (indices, host block [rows][k][cols]) and brings the plan's output up to all stored
  * views, as one HIP graph captured per batch size and replayed for every later batch of
- * that size.  The plan keeps the unscaled FP32 sums of its points and adds only the
+ * that size.  The block is copied to pinned memory in chunks by three host threads the
+ * plan keeps (and the calling thread) and each chunk uploaded as soon as it is complete.  The plan keeps the unscaled FP32 sums of its points and adds only the
  * views not yet summed (the same bits as a full pass); they restart after
  * mr_reset_views.  MR_ERUNTIME once the context's geometry has been set again (as for
  * every plan call).  Copies the output to out_host when it is not NULL. */

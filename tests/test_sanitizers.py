@@ -1,8 +1,9 @@
 """CPU tests under AddressSanitizer and UndefinedBehaviorSanitizer: the library's
 host-only request preparation (csrc/mr_tiles.cpp: task validation, tile enumeration,
-height order, the request cache) and the C oracle (oracle/oracle_c.c), each built
-from source with g++/gcc -fsanitize=address,undefined by this test and run on inputs
-that reach their edge cases (tests/sanitize/*)."""
+height order, the request cache, COR shift groups), its host copy pool
+(csrc/mr_hostpool.cpp, also under ThreadSanitizer) and the C oracle
+(oracle/oracle_c.c), each built from source with g++/gcc -fsanitize=... by this test
+and run on inputs that reach their edge cases (tests/sanitize/*)."""
 import os
 import shutil
 import subprocess
@@ -33,6 +34,17 @@ def test_tile_preparation_under_sanitizers(tmp_path):
     exe = str(tmp_path / "tiles_check")
     out = _run(["g++", "-std=c++17", *FLAGS, "-I", CSRC, os.path.join(SAN, "tiles_check.cpp"),
                 os.path.join(CSRC, "mr_tiles.cpp"), "-o", exe], exe, tmp_path)
+    assert "all checks passed" in out
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+@pytest.mark.parametrize("san", ["address,undefined", "thread"])
+def test_host_copy_pool_under_sanitizers(tmp_path, san):
+    exe = str(tmp_path / "hostpool_check")
+    flags = [f if not f.startswith("-fsanitize=") else "-fsanitize=" + san for f in FLAGS]
+    out = _run(["g++", "-std=c++17", *flags, "-pthread", "-I", CSRC,
+                os.path.join(SAN, "hostpool_check.cpp"), os.path.join(CSRC, "mr_hostpool.cpp"),
+                "-o", exe], exe, tmp_path)
     assert "all checks passed" in out
 
 

@@ -1761,8 +1761,27 @@ int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block
         p->kmax = k;
     }
     int rc = 0;
-    // stage the batch and its metadata in pinned memory (fixed addresses)
-    std::memcpy(p->h_block, block, blk);
+    // the batch goes to the device in chunks through pinned memory: host threads copy
+    // the chunks, and each is sent on the request stream as soon as it is complete, so the
+    // host copy of the later chunks overlaps the transfer (the previous call has
+    // synchronised the stream, so the staging buffer is free); its metadata is staged too
+    // and sent by the graph
+    {
+        auto send = [&](size_t o, size_t len) {
+            const hipError_t e = hipMemcpyAsync(reinterpret_cast<char*>(p->d_block) + o,
+                                                reinterpret_cast<const char*>(p->h_block) + o, len,
+                                                hipMemcpyHostToDevice, c->s_req);
+            return e == hipSuccess ? MR_OK : fail(MR_ERUNTIME, std::string("batch upload: ") + hipGetErrorString(e));
+        };
+        if (blk >= (size_t{1} << 20)) {
+            if (!p->copy_pool) p->copy_pool.reset(new HostCopyPool(3));
+            rc = p->copy_pool->copy(p->h_block, block, blk, 8, send);
+        } else {
+            std::memcpy(p->h_block, block, blk);
+            rc = send(0, blk);
+        }
+        if (rc) return rc;
+    }
     FilterView* fv = static_cast<FilterView*>(p->h_meta);
     ViewGeom* gv = reinterpret_cast<ViewGeom*>(static_cast<char*>(p->h_meta) + sizeof(FilterView) * k);
     for (int j = 0; j < k; ++j) {
@@ -1797,9 +1816,8 @@ int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block
         q.accum_half = p->out_len;
         hipGraph_t graph = nullptr;
         MR_HIP(hipStreamBeginCapture(c->s_req, hipStreamCaptureModeThreadLocal));
-        hipError_t e1 = hipMemcpyAsync(p->d_block, p->h_block, blk, hipMemcpyHostToDevice, c->s_req);
         hipError_t e2 = hipMemcpyAsync(p->d_meta, p->h_meta, meta, hipMemcpyHostToDevice, c->s_req);
-        int r = (e1 != hipSuccess || e2 != hipSuccess) ? MR_ERUNTIME : MR_OK;
+        int r = e2 != hipSuccess ? MR_ERUNTIME : MR_OK;
         if (!r) r = launch_stream_prep(c, p->d_meta, k, p->d_counters + 2, c->s_req);
         if (!r) r = launch_filter(c, p->d_block, k, 0, c->s_req, static_cast<const FilterView*>(p->d_meta),
                                   p->d_vmax);

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/minirecast_hip.h"
+#include "mr_hostpool.h"
 #include "mr_tiles.h"
 
 namespace mr {
@@ -261,6 +262,7 @@ struct Plan {
     hipGraphExec_t cor_exec = nullptr;
     void* d_cor = nullptr;           // COR shift groups of the captured graph (cor_groups)
     bool cor_groups = false;         // the captured graph runs the shift-group kernel
+    std::unique_ptr<HostCopyPool> copy_pool;   // staging copies of streamed batches
 };
 
 int ensure(void** p, size_t* have, size_t need);   // device buffer growth
