@@ -157,6 +157,23 @@ int main() {
         CHECK(cache.size() <= kTileCacheEntries, "cache grew beyond its entries");
         CHECK(&got == &cache.front(), "entry not moved to the front");
     }
+    {
+        // the same planes at another shift and output place: the same entry, not a new one
+        std::vector<mr_task> a = reqs[0], b = reqs[0];
+        for (mr_task& T : b) {
+            T.cor_shift += 0.75;
+            T.out_offset += 12345;
+            T.stride_p += 1;
+        }
+        const TileCache& ea = cached_tiles(cache, a.data(), int(a.size()), 0, true, true);
+        const std::vector<Tile> tiles_a = ea.tiles;
+        const size_t n_before = cache.size();
+        const TileCache& eb = cached_tiles(cache, b.data(), int(b.size()), 0, true, true);
+        CHECK(&eb == &cache.front() && cache.size() == n_before, "shifted request missed the cache");
+        CHECK(eb.tiles.size() == tiles_a.size() &&
+                  std::memcmp(eb.tiles.data(), tiles_a.data(), sizeof(Tile) * tiles_a.size()) == 0,
+              "shifted request changed the tiles");
+    }
     // an empty request
     const TileCache& none = cached_tiles(cache, nullptr, 0, 0, true, true);
     CHECK(none.tiles.empty() && none.ne == 0, "empty request has tiles");

@@ -113,6 +113,32 @@ def test_plane_task_frame_reproduces_slice_points():
     np.testing.assert_array_equal(np.all(np.abs(exact) <= T['box'], axis=-1), mask)
 
 
+def test_plane_task_records_are_kept_and_copied():
+    """plane_task keeps the records of recent arguments: a repeated request gives the same
+    bytes as a fresh build (with its own shift and output offset), changing a returned
+    record does not reach later ones, and more distinct planes than are kept still give
+    the right records."""
+    from minirecast import recon
+    box = np.array([9.6, 9.6, 7.3])
+    args = ((0.5, 1, -1), (1, 0.5, 0.8), 21)
+    kw = dict(zoom=1.3, center=(1, 0, 0))
+    T1 = recon.plane_task(box, *args, cor_shift=0.5, out_offset=7, **kw)
+    T1['mask_box'] = 0
+    T1['o'][0] = 99.0
+    T2 = recon.plane_task(box, *args, cor_shift=-1.25, out_offset=11, **kw)
+    fresh = recon._plane_task_frame(box, args[0], args[1], args[2], None, kw['zoom'], kw['center'])
+    fresh['cor_shift'], fresh['out_offset'] = -1.25, 11
+    assert T2.tobytes() == fresh.tobytes()
+    assert T2['mask_box'] == 1 and T2['o'][0] != 99.0
+    for k in range(recon._PLANE_TASKS_MAX + 5):
+        normal = (1.0, 0.01 * k, 0.8)
+        got = recon.plane_task(box, (0, 0, 0), normal, 8 + k % 3)
+        want = recon._plane_task_frame(box, (0, 0, 0), normal, 8 + k % 3, None, 1.0, None)
+        want['cor_shift'], want['out_offset'] = 0.0, 0
+        assert got.tobytes() == want.tobytes(), k
+    assert len(recon._PLANE_TASKS) <= recon._PLANE_TASKS_MAX
+
+
 def test_geometry_contract_matches_oracle():
     from minirecast.geometry import ConeVecGeometry, ParallelGeometry, circular_cone_vectors
     g = circular_cone_vectors(17, 300.0, 600.0, 32, 48, 1.5, det_shift_u=2.0)

@@ -100,7 +100,16 @@ void order_tiles_by_height(const mr_task* tasks, std::vector<Tile>& tiles) {
 const TileCache& cached_tiles(std::vector<TileCache>& cache, const mr_task* tasks, int ntasks,
                               int flags, bool keep_empty, bool by_height) {
     const size_t tb = sizeof(mr_task) * static_cast<size_t>(ntasks > 0 ? ntasks : 0);
-    const uint8_t* tk = reinterpret_cast<const uint8_t*>(tasks);
+    // the key: the task bytes without the fields the tiles do not depend on (the shift,
+    // the output offset and strides), so that requests of the same planes at another
+    // rotation-centre shift or output place reuse the entry
+    std::vector<mr_task> keyed(tasks, tasks + (ntasks > 0 ? ntasks : 0));
+    for (mr_task& T : keyed) {
+        T.cor_shift = 0.0;
+        T.out_offset = 0;
+        T.stride_q = T.stride_p = T.stride_s = 0;
+    }
+    const uint8_t* tk = reinterpret_cast<const uint8_t*>(keyed.data());
     int hit = -1;
     for (int i = 0; i < static_cast<int>(cache.size()) && hit < 0; ++i)
         if (cache[i].flags == flags && cache[i].tasks.size() == tb &&

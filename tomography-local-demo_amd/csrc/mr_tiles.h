@@ -98,7 +98,8 @@ bool cor_shift_groups(const mr_task* tasks, int ntasks, double min_avg, CorPlan&
 void order_tiles_by_height(const mr_task* tasks, std::vector<Tile>& tiles);
 
 // The tile list of a request from the cache (most recent first, at most
-// kTileCacheEntries): a hit moves the entry to the front; a miss enumerates the tiles
+// kTileCacheEntries; keyed by the tasks without their cor_shift, out_offset and strides,
+// which the tiles do not depend on): a hit moves the entry to the front; a miss enumerates the tiles
 // (keeping the culled ones after them when keep_empty), orders them by height when
 // by_height, and puts the new entry in front, dropping the oldest.  flags is part of
 // the key.  The returned entry stays valid until the next call on the same cache.

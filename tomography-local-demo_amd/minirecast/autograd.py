@@ -39,14 +39,19 @@ class FDKSlices:
         self.out_len = int(sum(n * n for n in self.sizes))
         self.device = torch.device("cuda", device)
         self._loaded = None
-
-    def tasks(self, shift):
+        # the planes' task records (the shift enters only their cor_shift field): built
+        # once, since plane_task takes ~0.2 ms of host time per plane
         out, off = [], 0
         for p, n in zip(self.planes, self.sizes):
             out.append(recon.plane_task(self.box, p["origin"], p["normal"], n, p.get("size"),
-                                        p.get("zoom", 1.0), p.get("center"), float(shift), off))
+                                        p.get("zoom", 1.0), p.get("center"), 0.0, off))
             off += n * n
-        return np.array(out, dtype=_hip.TASK_DTYPE)
+        self._tasks = np.array(out, dtype=_hip.TASK_DTYPE)
+
+    def tasks(self, shift):
+        T = self._tasks.copy()
+        T['cor_shift'] = float(shift)
+        return T
 
     def _check(self, proj):
         g = self.geom
