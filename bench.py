@@ -263,7 +263,7 @@ def other_workloads(args):
 
 
 def bench_full_volume(args):
-    """The 768^3 volume of the full_cone scan (5.44e11 updates), reconstructed into HBM
+    """The 768 x 768 x 607 volume over the full_cone scan's box (4.3e11 updates), reconstructed into HBM
     from the f16 ring in one request (the z-column kernel: 16 x 16 voxel columns x 16
     planes per workgroup, each view's detector box staged through LDS), then the Otsu
     threshold of model.py computed on the device-resident volume."""
