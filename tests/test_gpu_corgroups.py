@@ -14,7 +14,9 @@ shift's lerp comes from them.  Checked, for f32 and f16 rings:
   that are not multiples of the tile, a plane above the detector (every tap zero), a view
   count below the stored views;
 - a plan of the trial images scores them on the device (one graph) like numpy's
-  histogram entropy of the plan's own images (estimate_cor's formula, recon.py:509-511).
+  histogram entropy of the plan's own images (estimate_cor's formula, recon.py:509-511),
+  also after the stored views were reset and fewer pushed again;
+- a bf16 ring; a request mixing a search with an oblique plane runs the plane tiles.
 """
 import numpy as np
 import pytest
@@ -167,3 +169,53 @@ def test_cor_groups_plan_scores(cg, st):
     offs = np.concatenate([[0], np.cumsum([t['np'] * t['ns'] for t in T])])
     host = np.array([entropy_score(imgs[offs[i]:offs[i + 1]]) for i in range(len(T))])
     np.testing.assert_allclose(scores, host, rtol=1e-12, atol=0)
+
+
+def test_cor_groups_bf16_and_mixed_requests(cg):
+    """A bf16 ring through the shift groups; a request mixing a shift search with an
+    oblique plane runs the plane tiles for all of it (one kernel per request)."""
+    from minirecast import _hip, recon
+    g = cg["g"]
+    c16 = _hip.Context(0)
+    c16.set_parallel(g.angles, R, C, PU, PV, WINDOW, "bf16")
+    c16.push(np.arange(A), cg["proj"])
+    T, out_len = _tasks(cg["box"], PLANES[:1], SHIFTS)
+    got = c16.bp_tasks(T, A, out_len)
+    assert c16.last_bp_kernel() == 10
+    c16.set_bp_config(6, 4, 0, 2)
+    ref = c16.bp_tasks(T, A, out_len)
+    _close(got, ref)
+    # mixed: the search plus one oblique plane
+    ob = recon.plane_task(cg["box"], (0.0, 0.0, 0.0), (0.2, 0.1, 1.0), 48, out_offset=out_len)
+    M = np.concatenate([T, np.array([ob], dtype=_hip.TASK_DTYPE)])
+    got, ref, kern = _both(cg, "f32", M, A, out_len + 48 * 48)
+    assert kern != 10
+    _close(got, ref)
+
+
+def test_cor_groups_plan_after_reset(cg):
+    """A COR plan replayed after the stored views were reset and fewer pushed again:
+    its graph reads the device view count and rebuilds the detector rows, equal to a
+    request over the new views."""
+    from minirecast import _hip
+    g = cg["g"]
+    ctx = _hip.Context(0)
+    ctx.set_parallel(g.angles, R, C, PU, PV, WINDOW, "f32")
+    ctx.push(np.arange(A), cg["proj"])
+    T, out_len = _tasks(cg["box"], PLANES[:1], SHIFTS)
+    plan = _hip.Plan(ctx, T, out_len)
+    try:
+        s_all = plan.cor_scores(A)
+        ctx.reset_views()
+        half = np.arange(0, A, 2)
+        ctx.push(half, np.ascontiguousarray(cg["proj"][:, half, :]))
+        s_half = plan.cor_scores(len(half))
+        assert ctx.last_bp_kernel() == 10
+        imgs = ctx.d2h(np.empty(out_len, np.float32), plan.output_ptr())
+    finally:
+        plan.close()
+    one = ctx.bp_tasks(T, len(half), out_len)
+    assert rel_l2(imgs.astype(np.float64), one.astype(np.float64)) < 1e-7
+    assert not np.array_equal(s_all, s_half)
+    host = np.array([entropy_score(imgs[i * 100 * 100:(i + 1) * 100 * 100]) for i in range(len(T))])
+    np.testing.assert_allclose(s_half, host, rtol=1e-12, atol=0)
