@@ -1761,27 +1761,6 @@ int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block
         p->kmax = k;
     }
     int rc = 0;
-    // the batch goes to the device in chunks through pinned memory: host threads copy
-    // the chunks, and each is sent on the request stream as soon as it is complete, so the
-    // host copy of the later chunks overlaps the transfer (the previous call has
-    // synchronised the stream, so the staging buffer is free); its metadata is staged too
-    // and sent by the graph
-    {
-        auto send = [&](size_t o, size_t len) {
-            const hipError_t e = hipMemcpyAsync(reinterpret_cast<char*>(p->d_block) + o,
-                                                reinterpret_cast<const char*>(p->h_block) + o, len,
-                                                hipMemcpyHostToDevice, c->s_req);
-            return e == hipSuccess ? MR_OK : fail(MR_ERUNTIME, std::string("batch upload: ") + hipGetErrorString(e));
-        };
-        if (blk >= (size_t{1} << 20)) {
-            if (!p->copy_pool) p->copy_pool.reset(new HostCopyPool(3));
-            rc = p->copy_pool->copy(p->h_block, block, blk, 8, send);
-        } else {
-            std::memcpy(p->h_block, block, blk);
-            rc = send(0, blk);
-        }
-        if (rc) return rc;
-    }
     FilterView* fv = static_cast<FilterView*>(p->h_meta);
     ViewGeom* gv = reinterpret_cast<ViewGeom*>(static_cast<char*>(p->h_meta) + sizeof(FilterView) * k);
     for (int j = 0; j < k; ++j) {
@@ -1838,6 +1817,29 @@ int mr_plan_stream(mr_plan* p, const int64_t* indices, int k, const float* block
             p->graphs_copies = c->copies_epoch;
         }
         p->graphs.emplace_back(k, exec);
+    }
+    // the batch goes to the device in chunks through pinned memory: host threads copy
+    // the chunks, and each is sent on the request stream as soon as it is complete, so the
+    // host copy of the later chunks overlaps the transfer (the previous call has
+    // synchronised the stream, so the staging buffer is free).  Everything that waits on
+    // the stream (the running-sum counters, a capture) is done before, so the graph launch
+    // queues behind the transfer instead of waiting for it; the batch's metadata is sent
+    // by the graph
+    {
+        auto send = [&](size_t o, size_t len) {
+            const hipError_t e = hipMemcpyAsync(reinterpret_cast<char*>(p->d_block) + o,
+                                                reinterpret_cast<const char*>(p->h_block) + o, len,
+                                                hipMemcpyHostToDevice, c->s_req);
+            return e == hipSuccess ? MR_OK : fail(MR_ERUNTIME, std::string("batch upload: ") + hipGetErrorString(e));
+        };
+        if (blk >= (size_t{1} << 20)) {
+            if (!p->copy_pool) p->copy_pool.reset(new HostCopyPool(3));
+            rc = p->copy_pool->copy(p->h_block, block, blk, 8, send);
+        } else {
+            std::memcpy(p->h_block, block, blk);
+            rc = send(0, blk);
+        }
+        if (rc) return rc;
     }
     p->acc_n = -1;                     // until the replay is known to have completed
     MR_HIP(hipEventRecord(c->ev0, c->s_req));
