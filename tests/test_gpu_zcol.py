@@ -120,7 +120,10 @@ def test_zcol_volume_vs_tiles_and_oracle(zc, st, tol, shift):
     err = rel_l2(vol[zc["sel"]].astype(np.float64), zc["ref"][shift])
     print(f"{st} shift {shift}: z columns vs plane tiles rel-L2 {d:.2e} (max {dmax:.2e}), "
           f"vs float64 {err:.2e}")
-    assert d < 1e-6 and dmax < 1e-5, (d, dmax)
+    # f16: the staged kernel lerps in packed half arithmetic (rounded to 2^-11 relative,
+    # the order of the storage rounding), the plane tiles in float32
+    dtol, mtol = (2e-4, 5e-4) if st == "f16" else (1e-6, 1e-5)
+    assert d < dtol and dmax < mtol, (d, dmax)
     assert err < tol, err
 
 
@@ -145,8 +148,8 @@ def test_zcol_slabs_and_partial_views(zc):
 
 def test_zcol_ring_form_equals_staged_form(zc):
     """Without room for the row-pair copy (a copy budget of 0) a 16-bit volume gathers
-    its taps from the ring (kernel 8): the same taps and formulas as the staged form,
-    equal to float32 rounding."""
+    its taps from the ring (kernel 8): the same taps as the staged form, whose f16 lerps
+    are rounded to half precision (kernel 8 lerps in float32), so equal to that rounding."""
     recon = zc["recon"]
     T = recon.volume_task(zc["origin"], zc["h"], DIMS, 0, DIMS[2], 0.75)[None]
     ctx = zc["ctx"]["f16", "dflt"]
@@ -158,8 +161,10 @@ def test_zcol_ring_form_equals_staged_form(zc):
     finally:
         ctx.set_ring_copy_budget(-1)
     assert kern == 8, kern
-    assert rel_l2(ring.astype(np.float64), staged.astype(np.float64)) < 1e-6
-    assert float(np.max(np.abs(ring - staged))) <= 1e-6 * float(np.max(np.abs(staged)))
+    d = rel_l2(ring.astype(np.float64), staged.astype(np.float64))
+    dmax = float(np.max(np.abs(ring - staged))) / float(np.max(np.abs(staged)))
+    print(f"f16 gathered vs staged: rel-L2 {d:.2e}, max {dmax:.2e}")
+    assert d < 2e-4 and dmax < 5e-4, (d, dmax)
 
 
 @pytest.mark.parametrize("st", ["f32", "f16"])

@@ -273,6 +273,20 @@ template <> struct PairP<__half> {
         bot = fmaf(fc, half_hi(w1), fmaf(-fc, half_hi(w0), half_hi(w0)));
     }
 };
+// Bilinear value of f16 row-pair taps in packed half arithmetic: the column lerps of both
+// rows in one v_pk_fma_f16 (after one v_pk_add_f16 for the differences), the row
+// difference as one f16 subtraction, the row lerp in f32 from the f16 operands
+// (v_fma_mix_f32): 4 VALU against 6 with f32 lerps.  The lerps round to f16 (relative
+// 2^-11, the order of the storage rounding itself); fc2 is the column fraction as f16 in
+// both halves.
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+__device__ inline float pair_bilerp_f16(uint32_t w0, uint32_t w1, half2v fc2, float fr) {
+    const half2v x = __builtin_bit_cast(half2v, w0);      // (q(r, c), q(r+1, c))
+    const half2v y = __builtin_bit_cast(half2v, w1);      // (q(r, c+1), q(r+1, c+1))
+    const half2v tb = __builtin_elementwise_fma(fc2, y - x, x);   // (top, bottom)
+    const _Float16 d = tb.y - tb.x;
+    return fmaf(fr, static_cast<float>(d), static_cast<float>(tb.x));
+}
 template <> struct PairP<uint16_t> {
     uint32_t w0, w1;
     __device__ inline void load(__amdgpu_buffer_rsrc_t rs, int voff, int) {
@@ -1044,21 +1058,39 @@ __global__ __launch_bounds__(BP_THREADS) void bp_zcol_kernel(
     // (row k / nch, chunk k mod nch) to LDS byte 16 k; each wave copies runs of 64 chunks
     auto stage = [&](int vv, const float* rv, int bsel) {
         if constexpr (STAGE) {
-            const int pr0 = __float_as_int(rv[16]), nr = __float_as_int(rv[17]);
-            const int c0 = __float_as_int(rv[18]), nch = __float_as_int(rv[19]);
+            // the box descriptor is the same for the whole workgroup: scalar registers
+            const int pr0 = __builtin_amdgcn_readfirstlane(__float_as_int(rv[16]));
+            const int nr = __builtin_amdgcn_readfirstlane(__float_as_int(rv[17]));
+            const int c0 = __builtin_amdgcn_readfirstlane(__float_as_int(rv[18]));
+            const int nch = __builtin_amdgcn_readfirstlane(__float_as_int(rv[19]));
             if (nr == 0) return;
             const int total = nr * nch;
-            const float inv = 1.0f / static_cast<float>(nch);
             const __amdgpu_buffer_rsrc_t rs = rsrc(vv);
-            const int lane = t & 63;
-            for (int k0 = (t >> 6) * 64; k0 < total; k0 += BP_THREADS) {
-                const int k = k0 + lane;
-                if (k < total) {
-                    // floor(k / nch): (k + 0.5) / nch lies at least 1 / (2 nch) from an integer
-                    const int row = static_cast<int>((static_cast<float>(k) + 0.5f) * inv);
-                    const int cc = k - row * nch;
-                    const int voff = (((pr0 + row) * pitch + c0) << SH) + (cc << 4);
-                    lds_load16(rs, sbox + bsel * (ZL_BUF / 4) + k0 * 4, voff);
+            // chunk k = t + 256 j: its row k / nch and chunk cc = k mod nch, the first by one
+            // division, then steps of 256 chunks = dq rows and dr < nch chunks, one row more
+            // when cc passes nch.  floor((x + 0.5) * rcp(nch)) is exact for x <= 256: the
+            // quotient lies at least 1 / (2 nch) from an integer, and the reciprocal's error
+            // (1 ulp) moves it by under 2^-14 (a staged box has nch <= ZL_BUF / 16 = 1024)
+            const float inv = __builtin_amdgcn_rcpf(static_cast<float>(nch));
+            const int row = static_cast<int>((static_cast<float>(t) + 0.5f) * inv);
+            int cc = t - row * nch;
+            int voff = (((pr0 + row) * pitch + c0) << SH) + (cc << 4);
+            const int dq = __builtin_amdgcn_readfirstlane(
+                static_cast<int>((static_cast<float>(BP_THREADS) + 0.5f) * inv));
+            const int dr = BP_THREADS - dq * nch;
+            const int dvoff = dq * pitchb + (dr << 4);
+            const int wrap = pitchb - (nch << 4);
+            const int k0w = __builtin_amdgcn_readfirstlane(t & ~63);   // the wave's first chunk
+            const int lim = total - (t & 63);
+            uint32_t* dst = sbox + bsel * (ZL_BUF / 4) + k0w * 4;
+            for (int k0 = k0w; k0 < total; k0 += BP_THREADS) {
+                if (k0 < lim) lds_load16(rs, dst, voff);
+                dst += BP_THREADS * 4;
+                cc += dr;
+                voff += dvoff;
+                if (cc >= nch) {
+                    cc -= nch;
+                    voff += wrap;
                 }
             }
         }
@@ -1099,34 +1131,52 @@ __global__ __launch_bounds__(BP_THREADS) void bp_zcol_kernel(
             const int ci = cvt_flr(col);
             const int rref1 = __float_as_int(r3.z), cref1 = __float_as_int(r3.w);
             bool staged = false;
-            if constexpr (STAGE) staged = __float_as_int(rv[17]) != 0;
+            if constexpr (STAGE) staged = __builtin_amdgcn_readfirstlane(__float_as_int(rv[17])) != 0;
             if constexpr (STAGE) if (staged) {
                 // LDS byte address of (pair-row rref1 + r, column cref1 + ci) in the box:
                 // r * SB + cb, SB = 16 nch bytes per box row
-                const int pr0 = __float_as_int(rv[16]), c0 = __float_as_int(rv[18]);
-                const int sb = __float_as_int(rv[19]) << 4;
+                const int pr0 = __builtin_amdgcn_readfirstlane(__float_as_int(rv[16]));
+                const int c0 = __builtin_amdgcn_readfirstlane(__float_as_int(rv[18]));
+                const int sb = __builtin_amdgcn_readfirstlane(__float_as_int(rv[19])) << 4;
                 const int cb = ((cref1 + ci - c0) << SH) + (rref1 - pr0) * sb;
                 const char* bx = reinterpret_cast<const char*>(sbox) + (v & 1) * ZL_BUF;
+                constexpr bool PK = RING == 1 && std::is_same<ST, __half>::value;
+                const _Float16 fch = static_cast<_Float16>(fc);
+                const half2v fc2 = {fch, fch};
+                // the planes' rows fma(i, drow, row0) are monotone in i, so when the first
+                // and the last of every lane lie in the clamp range, all do and the clamp
+                // (the identity there) is left out
+                const float rl = fmaf(static_cast<float>(NQ - 1), drow, row0);
+                const bool in_range = fminf(row0, rl) >= r2.w && fmaxf(row0, rl) <= r3.x;
+                auto planes = [&](auto clamp) {
 #pragma unroll
-                for (int i = 0; i < NQ; ++i) {
-                    const float row = __builtin_amdgcn_fmed3f(
-                        fmaf(static_cast<float>(i), drow, row0), r2.w, r3.x);
-                    const uint32_t* q = reinterpret_cast<const uint32_t*>(bx + __mul24(cvt_flr(row), sb) + cb);
-                    float top, bot;
-                    if constexpr (RING == 1) {
-                        PairP<ST> tap;             // (q(r, c) | q(r+1, c)), same for c + 1
-                        tap.w0 = q[0];
-                        tap.w1 = q[1];
-                        tap.lerp(fc, top, bot);
-                    } else {
-                        // the ring: row r, then row r + 1 one box row further
-                        const Pair<ST> tap =
-                            LdsPair<ST>::read(reinterpret_cast<const unsigned char*>(q), sb);
-                        tap.lerp(fc, top, bot);
+                    for (int i = 0; i < NQ; ++i) {
+                        float row = fmaf(static_cast<float>(i), drow, row0);
+                        if constexpr (decltype(clamp)::value) row = __builtin_amdgcn_fmed3f(row, r2.w, r3.x);
+                        const uint32_t* q = reinterpret_cast<const uint32_t*>(bx + __mul24(cvt_flr(row), sb) + cb);
+                        const float fr = __builtin_amdgcn_fractf(row);
+                        if constexpr (PK) {
+                            // f16 row-pair copy: packed column lerps
+                            acc[i] = fmaf(wt, pair_bilerp_f16(q[0], q[1], fc2, fr), acc[i]);
+                        } else {
+                            float top, bot;
+                            if constexpr (RING == 1) {
+                                PairP<ST> tap;     // (q(r, c) | q(r+1, c)), same for c + 1
+                                tap.w0 = q[0];
+                                tap.w1 = q[1];
+                                tap.lerp(fc, top, bot);
+                            } else {
+                                // the ring: row r, then row r + 1 one box row further
+                                const Pair<ST> tap =
+                                    LdsPair<ST>::read(reinterpret_cast<const unsigned char*>(q), sb);
+                                tap.lerp(fc, top, bot);
+                            }
+                            acc[i] = fmaf(wt, fmaf(fr, bot - top, top), acc[i]);
+                        }
                     }
-                    const float fr = __builtin_amdgcn_fractf(row);
-                    acc[i] = fmaf(wt, fmaf(fr, bot - top, top), acc[i]);
-                }
+                };
+                if (__all(in_range)) planes(std::false_type{});
+                else planes(std::true_type{});
             }
             if (!staged) {
                 const __amdgpu_buffer_rsrc_t rs = rsrc(vb + v);
