@@ -1131,6 +1131,10 @@ __global__ __launch_bounds__(BP_THREADS) void bp_zcol_kernel(
             const int ci = cvt_flr(col);
             const int rref1 = __float_as_int(r3.z), cref1 = __float_as_int(r3.w);
             bool staged = false;
+            // staged kernels: each path leaves the planes' interpolated values in val and the
+            // sums are updated once after both (so the sums have one definition per view
+            // and are not copied where the paths meet)
+            float val[STAGE ? NQ : 1];
             if constexpr (STAGE) staged = __builtin_amdgcn_readfirstlane(__float_as_int(rv[17])) != 0;
             if constexpr (STAGE) if (staged) {
                 // LDS byte address of (pair-row rref1 + r, column cref1 + ci) in the box:
@@ -1157,7 +1161,7 @@ __global__ __launch_bounds__(BP_THREADS) void bp_zcol_kernel(
                         const float fr = __builtin_amdgcn_fractf(row);
                         if constexpr (PK) {
                             // f16 row-pair copy: packed column lerps
-                            acc[i] = fmaf(wt, pair_bilerp_f16(q[0], q[1], fc2, fr), acc[i]);
+                            val[i] = pair_bilerp_f16(q[0], q[1], fc2, fr);
                         } else {
                             float top, bot;
                             if constexpr (RING == 1) {
@@ -1171,7 +1175,7 @@ __global__ __launch_bounds__(BP_THREADS) void bp_zcol_kernel(
                                     LdsPair<ST>::read(reinterpret_cast<const unsigned char*>(q), sb);
                                 tap.lerp(fc, top, bot);
                             }
-                            acc[i] = fmaf(wt, fmaf(fr, bot - top, top), acc[i]);
+                            val[i] = fmaf(fr, bot - top, top);
                         }
                     }
                 };
@@ -1202,10 +1206,15 @@ __global__ __launch_bounds__(BP_THREADS) void bp_zcol_kernel(
                     for (int i = 0; i < ZG; ++i) {
                         float top, bot;
                         tap[i].lerp(fc, top, bot);
-                        acc[g + i] = fmaf(wt, fmaf(fr[i], bot - top, top), acc[g + i]);
+                        if constexpr (STAGE) val[g + i] = fmaf(fr[i], bot - top, top);
+                        else acc[g + i] = fmaf(wt, fmaf(fr[i], bot - top, top), acc[g + i]);
                     }
                     __builtin_amdgcn_sched_barrier(0);
                 }
+            }
+            if constexpr (STAGE) {
+#pragma unroll
+                for (int i = 0; i < NQ; ++i) acc[i] = fmaf(wt, val[i], acc[i]);
             }
             if (STAGE) {
                 // the next view's box is in LDS, and every wave is done with this one
