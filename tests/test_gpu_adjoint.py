@@ -187,6 +187,35 @@ def test_autograd_end_to_end(setup):
     assert abs(fd3 - mean_g) <= 1e-2 * abs(fd3), (fd3, mean_g)
 
 
+def test_new_projection_tensor_in_reused_memory_is_filtered(setup):
+    """A projection tensor freed and a new one allocated in its memory (PyTorch's caching
+    allocator hands the block back; the new tensor's version counter starts at 0 again)
+    must be filtered anew: the operator is linear, so projections times 2 give slices
+    times 2."""
+    import torch
+    from minirecast import autograd, phantom
+    g, box, planes, tasks, out_len, _ = setup
+    op = autograd.FDKSlices(g, planes, "shepp-logan")
+    ph = phantom.random_ellipsoids(5, 6, box, (0.2, 1.0))
+    base = torch.tensor(phantom.project_cone(ph, g), device="cuda")
+    with torch.no_grad():
+        p1 = base.clone()
+        addr = p1.data_ptr()
+        y1 = autograd.fdk_slices(p1, 0.0, op).clone()
+        del p1
+        p2 = base * 2.0                    # a new tensor, version 0
+        reused = p2.data_ptr() == addr
+        y2 = autograd.fdk_slices(p2, 0.0, op)
+    print(f"second tensor in the first one's memory: {reused}")
+    err = rel_l2(y2.cpu().numpy().astype(np.float64), 2.0 * y1.cpu().numpy().astype(np.float64))
+    assert err < 1e-6, (reused, err)
+    # the same tensor changed in place is filtered again as well
+    with torch.no_grad():
+        p2.mul_(0.5)
+        y3 = autograd.fdk_slices(p2, 0.0, op)
+    assert rel_l2(y3.cpu().numpy().astype(np.float64), y1.cpu().numpy().astype(np.float64)) < 1e-6
+
+
 def test_adjoint_gather_and_scatter_paths_agree(setup):
     """Requests of up to 4096 point tiles run the adjoint as a gather (one workgroup per
     detector tile and view, sums in LDS), larger ones as a scatter with atomics.  A stack

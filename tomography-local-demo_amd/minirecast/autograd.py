@@ -12,6 +12,8 @@ Forward and backward both run in libminirecast_hip.so:
     dL/ds = sum_i g_i dy_i/ds  (mr_bp_tasks_dshift)
 PyTorch only provides the tensors and the autograd graph.
 """
+import weakref
+
 import numpy as np
 import torch
 
@@ -38,7 +40,7 @@ class FDKSlices:
         self.sizes = [int(p.get("n", 256)) for p in self.planes]
         self.out_len = int(sum(n * n for n in self.sizes))
         self.device = torch.device("cuda", device)
-        self._loaded = None
+        self._loaded = None             # (weak reference to the loaded tensor, its version)
         # the planes' task records (the shift enters only their cor_shift field): built
         # once, since plane_task takes ~0.2 ms of host time per plane
         out, off = [], 0
@@ -61,16 +63,21 @@ class FDKSlices:
             raise ValueError("projections must be on the GPU")
 
     def load(self, proj):
-        """Filter proj into the device ring (skipped if this exact tensor is loaded)."""
+        """Filter proj into the device ring, skipped when this same tensor object, unchanged
+        since (same version counter), is the one loaded.  The address alone does not
+        identify the data: a new tensor can take a freed tensor's memory with the same
+        version number."""
         self._check(proj)
-        key = (proj.data_ptr(), proj._version)
-        if self._loaded == key:
-            return
-        proj = proj.contiguous()
-        torch.cuda.synchronize(proj.device)
+        if self._loaded is not None:
+            ref, version = self._loaded
+            if ref() is proj and proj._version == version:
+                return
+        self._loaded = None
+        src = proj.contiguous()
+        torch.cuda.synchronize(src.device)
         self.ctx.reset_views()
-        self.ctx.push_device(np.arange(self.geom.n_angles), proj.data_ptr())
-        self._loaded = key
+        self.ctx.push_device(np.arange(self.geom.n_angles), src.data_ptr())
+        self._loaded = (weakref.ref(proj), proj._version)
 
     def forward(self, proj, shift):
         self.load(proj)
