@@ -142,6 +142,40 @@ def test_dshift_vs_oracle_and_finite_difference(setup):
     assert abs(fd64 - an64) > 3e-2 * abs(fd64)      # the kinks matter at this step
 
 
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-4), ("f16", 1e-3)])
+def test_dshift_split_over_views_vs_oracle(setup, storage, tol):
+    """A request of few tiles over at least 32 views splits its views into parts summed
+    by separate workgroups (the part sums then added in part order): 96 views here, and
+    31 views (one part), each against the float64 analytic derivative."""
+    from minirecast import _hip, phantom, recon
+    from minirecast.geometry import circular_cone_vectors
+    _, _, planes, _, _, _ = setup
+    g = circular_cone_vectors(96, 400.0, 600.0, 20, 32, 0.6, det_shift_u=0.3)
+    box = g.volume_box()
+    tasks, off = [], 0
+    for p in planes:
+        tasks.append(recon.plane_task(box, p['origin'], p['normal'], p['n'], cor_shift=0.37,
+                                      out_offset=off))
+        off += p['n'] ** 2
+    T = np.array(tasks, dtype=_hip.TASK_DTYPE)
+    ctx = _hip.Context(0)
+    ctx.set_cone(g.vectors, g.rows, g.cols, "shepp-logan", storage)
+    ph = phantom.random_ellipsoids(6, 6, box, (0.2, 1.0))
+    ctx.push(np.arange(g.n_angles), phantom.project_cone(ph, g))
+    pts, mask = _plane_points(box, planes)
+    K = O.cone_constants(g.vectors)
+    for nv in (96, 31):
+        d = ctx.bp_tasks_dshift(T, nv, off)
+        q = ctx.read_views(0, nv, g.rows, g.cols).astype(np.float64)
+        Kv = {k: v[:nv] for k, v in K.items()}
+        want = np.zeros(off)
+        want[mask] = O.bp_cone_dshift(q, Kv['s'], Kv['d'], Kv['u'], Kv['uu'], Kv['vv'], Kv['n'],
+                                      Kv['sdd'], Kv['sod'], pts[mask], np.pi / nv, 0.37)
+        err = rel_l2(d, want)
+        print(f"{storage} {nv} views: dy/ds rel-L2 {err:.2e}")
+        assert np.all(d[~mask] == 0) and err < tol, (nv, err)
+
+
 def test_autograd_end_to_end(setup):
     import torch
     from minirecast import autograd, phantom

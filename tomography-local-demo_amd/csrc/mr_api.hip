@@ -594,7 +594,7 @@ int mr_ctx_destroy(mr_ctx* c) {
     hipDeviceSynchronize();
     free_geometry(c);
     void* bufs[] = {c->d_stage, c->d_fstage, c->d_vmax, c->d_fv, c->d_req, c->d_out, c->d_pts, c->d_misc,
-                    c->d_pre, c->d_recs, c->d_raw, c->d_vol, c->d_stat, c->d_corws, c->d_adj,
+                    c->d_pre, c->d_recs, c->d_raw, c->d_vol, c->d_stat, c->d_corws, c->d_adj, c->d_part,
                     c->d_cor};
     for (void* b : bufs)
         if (b) hipFree(b);

@@ -1821,12 +1821,17 @@ __global__ __launch_bounds__(BP_THREADS) void adj_gather_kernel(
 // s * (u . vv) to the row of every tap (recon.py:420), so per view the derivative of
 // the interpolant is dbil/dcol + (u . vv) dbil/drow, zero where the coordinate is
 // clamped outside the detector.
+//
+// part == nullptr: one workgroup per tile (grid.x) over all views, scaled sums into out.
+// Else workgroup (tile, k = blockIdx.y) sums views [k vper, (k + 1) vper) and leaves the
+// unscaled sums of its tile's 32 x 32 points in part[k][tile] for part_sum_kernel: small
+// requests (few tiles) then fill the device.
 template <bool CONE, typename ST>
 __global__ __launch_bounds__(BP_THREADS) void bp_dshift_kernel(
     const ST* __restrict__ ring, int64_t view_stride, int pitch, int R, int C,
     const ViewGeom* __restrict__ geom, const float* __restrict__ vscale,
     const mr_task* __restrict__ tasks, const Tile* __restrict__ tiles, int n_views, float scale,
-    float* __restrict__ out) {
+    float* __restrict__ out, int vper, float* __restrict__ part) {
     __shared__ float4 rec4[BP_VPASS * BP_REC / 4];
     float* rec = reinterpret_cast<float*>(rec4);
     constexpr int LE = Taps<ST>::LOG_ESZ;
@@ -1853,8 +1858,10 @@ __global__ __launch_bounds__(BP_THREADS) void bp_dshift_kernel(
     float acc[PPT];
 #pragma unroll
     for (int m = 0; m < PPT; ++m) acc[m] = 0.0f;
-    for (int vb = 0; vb < n_views; vb += BP_VPASS) {
-        const int nv = min(BP_VPASS, n_views - vb);
+    const int v0 = part ? static_cast<int>(blockIdx.y) * vper : 0;
+    const int v1 = part ? min(n_views, v0 + vper) : n_views;
+    for (int vb = v0; vb < v1; vb += BP_VPASS) {
+        const int nv = min(BP_VPASS, v1 - vb);
         __syncthreads();
         if (t < nv)
             make_record(geom + vb + t, vscale[vb + t], T, xt, R, C, pitch, CONE, rec + t * BP_REC);
@@ -1891,6 +1898,13 @@ __global__ __launch_bounds__(BP_THREADS) void bp_dshift_kernel(
                 acc[m] = fmaf(wt, fmaf(uvv, ddr, ddc), acc[m]);
             }
         }
+    }
+    if (part) {
+        float* pt = part + (static_cast<int64_t>(blockIdx.y) * gridDim.x + blockIdx.x) * (BP_TP * BP_TS);
+#pragma unroll
+        for (int m = 0; m < PPT; ++m)
+            if (valid[m]) pt[(ty + PSTEP * m) * BP_TS + tx] = acc[m];
+        return;
     }
     const int64_t obase = T.out_offset + static_cast<int64_t>(tl.q) * T.stride_q;
 #pragma unroll
@@ -2734,28 +2748,79 @@ int launch_bp_adjoint(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int n
     return MR_OK;
 }
 
+// Sums of the view parts of a split launch, in part order: out = scale * sum over k of
+// part[k][tile] for every point of every tile (0 outside a masked plane's box).
+__global__ __launch_bounds__(256) void part_sum_kernel(const mr_task* __restrict__ tasks,
+                                                      const Tile* __restrict__ tiles,
+                                                      const float* __restrict__ part, int nparts,
+                                                      float scale, float* __restrict__ out) {
+    const Tile tl = tiles[blockIdx.x];
+    const mr_task& T = tasks[tl.task];
+    const int64_t obase = T.out_offset + static_cast<int64_t>(tl.q) * T.stride_q;
+    const int64_t pstride = static_cast<int64_t>(gridDim.x) * (BP_TP * BP_TS);
+    for (int i = threadIdx.x; i < BP_TP * BP_TS; i += 256) {
+        const int p = tl.tp * BP_TP + i / BP_TS, s = tl.ts * BP_TS + i % BP_TS;
+        if (p >= T.np || s >= T.ns) continue;
+        const float* src = part + static_cast<int64_t>(blockIdx.x) * (BP_TP * BP_TS) + i;
+        float sum = 0.0f;
+        for (int k = 0; k < nparts; ++k) sum += src[k * pstride];
+        const bool inside = !T.mask_box || in_box(T, p, s);
+        out[obase + static_cast<int64_t>(p) * T.stride_p + static_cast<int64_t>(s) * T.stride_s] =
+            inside ? sum * scale : 0.0f;
+    }
+}
+
+// Split launches keep their part sums within this many bytes (else one part).
+constexpr size_t kPartBudget = size_t{256} << 20;
+
 template <bool CONE>
 static int dshift_dispatch(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles,
                            int n_views, float scale, float* d_out, hipStream_t s) {
-    dim3 grid(ntiles), block(BP_THREADS);
+    // a request of fewer tiles than the device holds workgroups splits its views into parts
+    // of at least 16 views, up to about one workgroup per resident slot
+    int parts = 1;
+    const int slots = 8 * c->n_cus;
+    if (ntiles < slots && n_views >= 32) {
+        parts = std::max(1, std::min((slots + ntiles - 1) / ntiles, n_views / 16));
+        const size_t need = sizeof(float) * BP_TP * BP_TS * static_cast<size_t>(ntiles) * parts;
+        if (need > kPartBudget) parts = 1;
+        if (parts > 1) {
+            const int rc = ensure(&c->d_part, &c->part_bytes, need);
+            if (rc) return rc;
+        }
+    }
+    int vper = n_views;
+    if (parts > 1) {
+        vper = (n_views + parts - 1) / parts;
+        parts = (n_views + vper - 1) / vper;             // no empty part
+    }
+    float* part = parts > 1 ? static_cast<float*>(c->d_part) : nullptr;
+    dim3 grid(ntiles, parts), block(BP_THREADS);
     switch (c->storage) {
     case MR_STORE_F32:
         hipLaunchKernelGGL((bp_dshift_kernel<CONE, float>), grid, block, 0, s,
                            static_cast<const float*>(c->ring), c->view_stride, c->pitch, c->R, c->C,
-                           c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out);
+                           c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out, vper, part);
         break;
     case MR_STORE_F16:
         hipLaunchKernelGGL((bp_dshift_kernel<CONE, __half>), grid, block, 0, s,
                            static_cast<const __half*>(c->ring), c->view_stride, c->pitch, c->R,
-                           c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out);
+                           c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out,
+                           vper, part);
         break;
     default:
         hipLaunchKernelGGL((bp_dshift_kernel<CONE, uint16_t>), grid, block, 0, s,
                            static_cast<const uint16_t*>(c->ring), c->view_stride, c->pitch, c->R,
-                           c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out);
+                           c->C, c->d_geom, c->d_vscale, d_tasks, d_tiles, n_views, scale, d_out,
+                           vper, part);
         break;
     }
     MR_HIP(hipGetLastError());
+    if (part) {
+        hipLaunchKernelGGL(part_sum_kernel, dim3(ntiles), dim3(256), 0, s, d_tasks, d_tiles, part,
+                           parts, scale, d_out);
+        MR_HIP(hipGetLastError());
+    }
     return MR_OK;
 }
 

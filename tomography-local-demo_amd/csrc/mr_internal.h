@@ -158,6 +158,8 @@ struct Ctx {
     size_t recs_bytes = 0;
     void* d_adj = nullptr;           // adjoint gather: masked weights + detector boxes
     size_t adj_bytes = 0;
+    void* d_part = nullptr;          // per-part sums of view-split launches (shift derivative)
+    size_t part_bytes = 0;
 
     int kind = MR_KIND_NONE;
     // cone geometry whose every view has P[0][2] = P[2][2] = 0 exactly: the detector column
