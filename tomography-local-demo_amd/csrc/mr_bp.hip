@@ -645,18 +645,30 @@ template <int PPT, int LAY> struct Map4 {
 // Cone beam, views [vb, ve) of n_views: a launch with vb > 0 resumes from the unscaled
 // FP32 sums the previous launch left in `out`, and one with ve < n_views leaves them
 // there, so an update split into several launches equals the single launch bit for bit.
+// part != nullptr (cone beam, small grids): the grid is `parts` copies of the tile grid;
+// copy k sums views [k kv, (k + 1) kv) from zero and leaves the unscaled sums of its
+// tile's points in part[k][tile] (32 x 32 floats), which part_sum_kernel adds in part
+// order.
 template <bool CONE, typename ST, int PPT, int LAY, int DEPTH, int RING, bool SYNC = false>
 __global__ __launch_bounds__(BP_THREADS, (sizeof(ST) == 2 && ((DEPTH == 2 && (RING < 2 || PPT < 4)) || (RING == 2 && PPT == 2))) ? 8 : 1)
 void bp_tiles_kernel4(
     const ST* __restrict__ ring, int64_t view_stride, int pitch,
     const float* __restrict__ recs, const mr_task* __restrict__ tasks,
     const Tile* __restrict__ tiles, int n_views, float scale, float* __restrict__ out, int xcd,
-    int vb, int ve) {
+    int vb, int ve, float* __restrict__ part, int parts, int kv) {
     using M = Map4<PPT, LAY>;
     constexpr int LE = Taps<ST>::LOG_ESZ;
     constexpr int SUBS = M::SUBS;                    // workgroups per 32-row tile
     static_assert(SUBS * 8 * PPT == BP_TP, "tile shape");
-    const int lb = xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    int lb = xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    int pk = 0;                                      // part of a split launch
+    const int nbw = static_cast<int>(gridDim.x) / parts;
+    if (CONE && part) {
+        pk = lb / nbw;
+        lb -= pk * nbw;
+        vb = pk * kv;
+        ve = min(n_views, vb + kv);
+    }
     const int tile = lb / SUBS;
     const int sub = lb - tile * SUBS;
     const Tile tl = tiles[tile];
@@ -691,7 +703,7 @@ void bp_tiles_kernel4(
         return obase + static_cast<int64_t>(p0 + pt + M::dp(m)) * T.stride_p +
                static_cast<int64_t>(s0 + st0 + M::ds(m)) * T.stride_s;
     };
-    if (CONE && vb > 0) {
+    if (CONE && vb > 0 && !part) {
 #pragma unroll
         for (int m = 0; m < PPT; ++m)
             if (valid[m]) acc[m] = out[oidx(m)];
@@ -814,6 +826,14 @@ void bp_tiles_kernel4(
             if (v + j < ve) consume(S[j]);
     }
 
+    if (CONE && part) {
+        // the point (pt + dp(m), st0 + ds(m)) of the tile, row-major
+        float* dst = part + (static_cast<int64_t>(pk) * (nbw / SUBS) + tile) * (BP_TP * BP_TS);
+#pragma unroll
+        for (int m = 0; m < PPT; ++m)
+            if (valid[m]) dst[(pt + M::dp(m)) * BP_TS + st0 + M::ds(m)] = acc[m];
+        return;
+    }
     const bool final_views = !CONE || ve >= n_views;
 #pragma unroll
     for (int m = 0; m < PPT; ++m) {
@@ -2250,11 +2270,57 @@ int launch_bp_points(Ctx* c, const double* d_pts, int64_t M, double cor_shift, i
                : dispatch_points<false>(c, d_pts, M, cor_shift, n_views, scale, d_out, s);
 }
 
+// Sums of the view parts of a split launch, in part order: out = scale * sum over k of
+// part[k][tile] for every point of every tile (0 outside a masked plane's box).
+__global__ __launch_bounds__(256) void part_sum_kernel(const mr_task* __restrict__ tasks,
+                                                      const Tile* __restrict__ tiles,
+                                                      const float* __restrict__ part, int nparts,
+                                                      float scale, float* __restrict__ out) {
+    const Tile tl = tiles[blockIdx.x];
+    const mr_task& T = tasks[tl.task];
+    const int64_t obase = T.out_offset + static_cast<int64_t>(tl.q) * T.stride_q;
+    const int64_t pstride = static_cast<int64_t>(gridDim.x) * (BP_TP * BP_TS);
+    for (int i = threadIdx.x; i < BP_TP * BP_TS; i += 256) {
+        const int p = tl.tp * BP_TP + i / BP_TS, s = tl.ts * BP_TS + i % BP_TS;
+        if (p >= T.np || s >= T.ns) continue;
+        const float* src = part + static_cast<int64_t>(blockIdx.x) * (BP_TP * BP_TS) + i;
+        float sum = 0.0f;
+        for (int k = 0; k < nparts; ++k) sum += src[k * pstride];
+        const bool inside = !T.mask_box || in_box(T, p, s);
+        out[obase + static_cast<int64_t>(p) * T.stride_p + static_cast<int64_t>(s) * T.stride_s] =
+            inside ? sum * scale : 0.0f;
+    }
+}
+
+// Split launches keep their part sums within this many bytes (else one part).
+constexpr size_t kPartBudget = size_t{256} << 20;
+
 template <bool CONE, typename ST, int PPT, int LAY, int DEPTH, int RING, bool SYNC>
 static void launch_k4(Ctx* c, const ST* ring, int64_t vs, const float* recs, const mr_task* d_tasks,
                       const Tile* d_tiles, int nt, int n_views, float scale, float* d_out,
                       hipStream_t s) {
     const int nb = Map4<PPT, LAY>::SUBS * nt;
+    // cone beam, grids of at most two workgroups per CU (a few small slices): one launch of
+    // ceil(n_views / bp_vchunk) copies of the grid, copy k summing views [k bp_vchunk,
+    // (k + 1) bp_vchunk), and the part sums added in part order, so that the device is
+    // filled; every such request uses the same partition of the views
+    if (CONE && c->bp_vchunk > 0 && !c->vchunk_any && n_views >= 2 * c->bp_vchunk &&
+        nb <= 2 * c->n_cus) {
+        const int kv = c->bp_vchunk;
+        const int parts = (n_views + kv - 1) / kv;
+        const size_t need = sizeof(float) * BP_TP * BP_TS * static_cast<size_t>(nt) * parts;
+        if (need <= kPartBudget && ensure(&c->d_part, &c->part_bytes, need) == MR_OK) {
+            float* part = static_cast<float*>(c->d_part);
+            hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, DEPTH, RING, SYNC>), dim3(nb * parts),
+                               dim3(BP_THREADS), 0, s, ring, vs, c->pitch, recs, d_tasks, d_tiles,
+                               n_views, scale, d_out, c->xcd_map != 0 ? 1 : 0, 0, n_views, part,
+                               parts, kv);
+            hipLaunchKernelGGL(part_sum_kernel, dim3(nt), dim3(256), 0, s, d_tasks, d_tiles, part,
+                               parts, scale, d_out);
+            ++c->tile_launches;
+            return;
+        }
+    }
     // cone beam, slice requests whose grid fills the device: one launch per block of
     // bp_vchunk views, so that all tiles read the same few views at a time and share
     // detector lines in the L2 (-22% L2-miss traffic at 64 views) at the cost of a round
@@ -2269,7 +2335,7 @@ static void launch_k4(Ctx* c, const ST* ring, int64_t vs, const float* recs, con
         const int ve = std::min(n_views, vb + k);
         hipLaunchKernelGGL((bp_tiles_kernel4<CONE, ST, PPT, LAY, DEPTH, RING, SYNC>), dim3(nb),
                            dim3(BP_THREADS), 0, s, ring, vs, c->pitch, recs, d_tasks, d_tiles,
-                           n_views, scale, d_out, c->xcd_map != 0 ? 1 : 0, vb, ve);
+                           n_views, scale, d_out, c->xcd_map != 0 ? 1 : 0, vb, ve, nullptr, 1, 0);
         ++c->tile_launches;
         vb = ve;
     } while (vb < n_views);
@@ -2747,31 +2813,6 @@ int launch_bp_adjoint(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int n
     MR_HIP(hipGetLastError());
     return MR_OK;
 }
-
-// Sums of the view parts of a split launch, in part order: out = scale * sum over k of
-// part[k][tile] for every point of every tile (0 outside a masked plane's box).
-__global__ __launch_bounds__(256) void part_sum_kernel(const mr_task* __restrict__ tasks,
-                                                      const Tile* __restrict__ tiles,
-                                                      const float* __restrict__ part, int nparts,
-                                                      float scale, float* __restrict__ out) {
-    const Tile tl = tiles[blockIdx.x];
-    const mr_task& T = tasks[tl.task];
-    const int64_t obase = T.out_offset + static_cast<int64_t>(tl.q) * T.stride_q;
-    const int64_t pstride = static_cast<int64_t>(gridDim.x) * (BP_TP * BP_TS);
-    for (int i = threadIdx.x; i < BP_TP * BP_TS; i += 256) {
-        const int p = tl.tp * BP_TP + i / BP_TS, s = tl.ts * BP_TS + i % BP_TS;
-        if (p >= T.np || s >= T.ns) continue;
-        const float* src = part + static_cast<int64_t>(blockIdx.x) * (BP_TP * BP_TS) + i;
-        float sum = 0.0f;
-        for (int k = 0; k < nparts; ++k) sum += src[k * pstride];
-        const bool inside = !T.mask_box || in_box(T, p, s);
-        out[obase + static_cast<int64_t>(p) * T.stride_p + static_cast<int64_t>(s) * T.stride_s] =
-            inside ? sum * scale : 0.0f;
-    }
-}
-
-// Split launches keep their part sums within this many bytes (else one part).
-constexpr size_t kPartBudget = size_t{256} << 20;
 
 template <bool CONE>
 static int dshift_dispatch(Ctx* c, const mr_task* d_tasks, const Tile* d_tiles, int ntiles,
