@@ -15,11 +15,12 @@ Reported (one JSON line on stdout):
   projections, oracle/liboracle_exact.so).
   roofline   for the back-projection tile kernel: compulsory HBM bytes (union of the 2x2
              tap footprints over all requested points, per view, x bytes per sample,
-             plus the output) / the duration of one update's tile-kernel launches (HIP
-             events around them on their stream; an update that fills the device runs
-             as one launch per block of 64 views, so per launch this is the same rate),
-             against 8 TB/s; traffic = measured HBM bytes per launch (mean per tile-kernel
-             dispatch; traffic_per_update = x the launches of one update) from two
+             plus the output) / the duration of one update's tile-kernel launch and the
+             part-sum pass that adds its view parts (HIP events around them on their
+             stream; one launch per update unless the part sums exceed their budget,
+             then one launch per block of 64 views), against 8 TB/s; traffic = measured
+             HBM bytes per launch (tile kernel and part-sum pass; traffic_per_update = x
+             the launches of one update) from two
              rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) of a short run of the same
              f16 update, each scaled by the counter's reading for a 1 GiB device copy
              (null when rocprofv3 is unavailable or --no-traffic).
@@ -129,9 +130,10 @@ def band_lines_per_load(geom, box, planes, views, max_tiles=64, seed=0):
 
 def _pmc_pass(counter):
     """One rocprofv3 --pmc pass of a short f16 run of this benchmark (child process):
-    the counter's mean per dispatch of the f16 cone tile kernel, the dispatches of one
-    update (which the child reports), and the value for the 1 GiB copy run first by the
-    child."""
+    the counter per tile launch of the f16 cone update (its tile-kernel dispatches and the
+    part-sum pass that adds the view parts of a split launch, over the updates of the
+    run, divided by the launches of one update), the launches of one update (which the
+    child reports), and the value for the 1 GiB copy run first by the child."""
     out = tempfile.mkdtemp(prefix="mr_pmc_")
     cmd = ["rocprofv3", "--pmc", counter, "-d", out, "-o", "run", "--output-format", "csv",
            "--", sys.executable, "-u", os.path.abspath(__file__), "--pmc-child", "--steps", "2",
@@ -142,7 +144,7 @@ def _pmc_pass(counter):
         files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
         if not files:
             return None
-        bp, cp = [], []
+        bp, ps, cp = [], [], []
         with open(files[0]) as fh:
             for row in csv.DictReader(fh):
                 if row.get("Counter_Name") != counter:
@@ -151,17 +153,22 @@ def _pmc_pass(counter):
                 if ("bp_tiles_kernel4<true, __half" in name or
                         "bp_tiles_kernel4ILb1E6__half" in name):
                     bp.append(val)
+                elif "part_sum_kernel" in name:
+                    ps.append(val)
                 elif "copyBuffer" in name or "copy" in name.lower():
                     cp.append(val)
         if not bp or not cp:
             return None
-        # one update = `launches` dispatches of the tile kernel (blocks of views)
+        # one update = `launches` dispatches of the tile kernel (one split launch, or one
+        # per block of views) and, for a split launch, one part-sum pass
         launches = 1
         with open(os.path.join(out, "child.log")) as fh:
             for ln in fh:
                 if "tile launches per update:" in ln:
                     launches = int(ln.rsplit(":", 1)[1])
-        return float(np.mean(bp)), launches, float(max(cp))
+        updates = len(bp) / launches
+        per_update = (sum(bp) + sum(ps)) / updates
+        return per_update / launches, launches, float(max(cp))
     except (OSError, subprocess.SubprocessError, ValueError, KeyError):
         return None
     finally:
@@ -872,7 +879,7 @@ def main():
     tl = int(traffic["fetch"] + traffic["write"]) if traffic else None
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=tl,
-                    kernel=f"bp_tiles_kernel4 (cone, {head})",
+                    kernel=f"bp_tiles_kernel4 + part_sum_kernel (cone, {head})",
                     launches_per_update=launches,
                     algorithmic_bytes_per_launch=int(hbm_bytes / launches),
                     algorithmic_bytes=int(hbm_bytes),

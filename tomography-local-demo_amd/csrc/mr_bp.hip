@@ -2292,20 +2292,23 @@ __global__ __launch_bounds__(256) void part_sum_kernel(const mr_task* __restrict
     }
 }
 
-// Split launches keep their part sums within this many bytes (else one part).
-constexpr size_t kPartBudget = size_t{256} << 20;
+// Split launches keep their part sums within this many bytes (else one part, or for the
+// tile kernels the view-block launches).
+constexpr size_t kPartBudget = size_t{1} << 30;
 
 template <bool CONE, typename ST, int PPT, int LAY, int DEPTH, int RING, bool SYNC>
 static void launch_k4(Ctx* c, const ST* ring, int64_t vs, const float* recs, const mr_task* d_tasks,
                       const Tile* d_tiles, int nt, int n_views, float scale, float* d_out,
                       hipStream_t s) {
     const int nb = Map4<PPT, LAY>::SUBS * nt;
-    // cone beam, grids of at most two workgroups per CU (a few small slices): one launch of
-    // ceil(n_views / bp_vchunk) copies of the grid, copy k summing views [k bp_vchunk,
-    // (k + 1) bp_vchunk), and the part sums added in part order, so that the device is
-    // filled; every such request uses the same partition of the views
-    if (CONE && c->bp_vchunk > 0 && !c->vchunk_any && n_views >= 2 * c->bp_vchunk &&
-        nb <= 2 * c->n_cus) {
+    // cone beam: one launch of ceil(n_views / bp_vchunk) copies of the grid, copy k
+    // summing views [k bp_vchunk, (k + 1) bp_vchunk) of every tile, then the part sums
+    // added in part order.  Small requests fill the device this way, and large ones run
+    // faster than one launch per view block with the sums carried through `out` (full_cone
+    // 3 x 1024^2 at 1200 views, f16: 2.9 against 3.4 ms; 3 x 128^2: 0.88 against 1.34 ms).
+    // Every request whose part sums fit kPartBudget uses the same partition of the views,
+    // so a plane gets the same values alone or within any batch.
+    if (CONE && c->bp_vchunk > 0 && !c->vchunk_any && n_views >= 2 * c->bp_vchunk) {
         const int kv = c->bp_vchunk;
         const int parts = (n_views + kv - 1) / kv;
         const size_t need = sizeof(float) * BP_TP * BP_TS * static_cast<size_t>(nt) * parts;
@@ -2321,12 +2324,12 @@ static void launch_k4(Ctx* c, const ST* ring, int64_t vs, const float* recs, con
             return;
         }
     }
-    // cone beam, slice requests whose grid fills the device: one launch per block of
-    // bp_vchunk views, so that all tiles read the same few views at a time and share
-    // detector lines in the L2 (-22% L2-miss traffic at 64 views) at the cost of a round
-    // trip of the sums through `out` per launch.  Smaller grids lose more to the launches
-    // than they gain; volume requests (stacks of planes, whose tiles share lines plane to
-    // plane already) would only pay the round trips.
+    // cone beam, requests beyond the part budget (or MINIRECAST_BP_VCHUNK_ANY): for slice
+    // requests whose grid fills the device, one launch per block of bp_vchunk views, so
+    // that all tiles read the same few views at a time and share detector lines in the L2,
+    // at the cost of a round trip of the sums through `out` per launch.  Smaller grids lose
+    // more to the launches than they gain; volume requests (stacks of planes, whose tiles
+    // share lines plane to plane already) would only pay the round trips.
     const bool split = CONE && c->bp_vchunk > 0 &&
                        (c->vchunk_any || (c->req_slices && nb >= 8 * c->n_cus));
     const int k = split ? c->bp_vchunk : std::max(n_views, 1);
