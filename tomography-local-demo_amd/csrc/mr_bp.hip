@@ -2271,24 +2271,47 @@ int launch_bp_points(Ctx* c, const double* d_pts, int64_t M, double cor_shift, i
 }
 
 // Sums of the view parts of a split launch, in part order: out = scale * sum over k of
-// part[k][tile] for every point of every tile (0 outside a masked plane's box).
+// part[k][tile] for every point of every tile (0 outside a masked plane's box).  A thread
+// adds 4 consecutive points of a tile row with 16-byte loads, four parts' loads issued
+// before their additions (the additions stay in part order).
 __global__ __launch_bounds__(256) void part_sum_kernel(const mr_task* __restrict__ tasks,
                                                       const Tile* __restrict__ tiles,
                                                       const float* __restrict__ part, int nparts,
                                                       float scale, float* __restrict__ out) {
+    static_assert(BP_TP * BP_TS == 4 * 256 && BP_TS % 4 == 0, "4 points per thread");
     const Tile tl = tiles[blockIdx.x];
     const mr_task& T = tasks[tl.task];
-    const int64_t obase = T.out_offset + static_cast<int64_t>(tl.q) * T.stride_q;
-    const int64_t pstride = static_cast<int64_t>(gridDim.x) * (BP_TP * BP_TS);
-    for (int i = threadIdx.x; i < BP_TP * BP_TS; i += 256) {
-        const int p = tl.tp * BP_TP + i / BP_TS, s = tl.ts * BP_TS + i % BP_TS;
-        if (p >= T.np || s >= T.ns) continue;
-        const float* src = part + static_cast<int64_t>(blockIdx.x) * (BP_TP * BP_TS) + i;
-        float sum = 0.0f;
-        for (int k = 0; k < nparts; ++k) sum += src[k * pstride];
-        const bool inside = !T.mask_box || in_box(T, p, s);
-        out[obase + static_cast<int64_t>(p) * T.stride_p + static_cast<int64_t>(s) * T.stride_s] =
-            inside ? sum * scale : 0.0f;
+    const int i = 4 * static_cast<int>(threadIdx.x);
+    const int p = tl.tp * BP_TP + i / BP_TS, s0 = tl.ts * BP_TS + i % BP_TS;
+    if (p >= T.np || s0 >= T.ns) return;
+    const float4* __restrict__ src =
+        reinterpret_cast<const float4*>(part + static_cast<int64_t>(blockIdx.x) * (BP_TP * BP_TS) + i);
+    const int64_t ps = static_cast<int64_t>(gridDim.x) * (BP_TP * BP_TS / 4);   // float4s per part
+    float4 sum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    auto add = [&](const float4 a) {
+        sum.x += a.x;
+        sum.y += a.y;
+        sum.z += a.z;
+        sum.w += a.w;
+    };
+    int k = 0;
+    for (; k + 4 <= nparts; k += 4) {
+        const float4 a = src[k * ps], b = src[(k + 1) * ps], c = src[(k + 2) * ps], d = src[(k + 3) * ps];
+        add(a);
+        add(b);
+        add(c);
+        add(d);
+    }
+    for (; k < nparts; ++k) add(src[k * ps]);
+    const float v[4] = {sum.x, sum.y, sum.z, sum.w};
+    const int64_t ob = T.out_offset + static_cast<int64_t>(tl.q) * T.stride_q + static_cast<int64_t>(p) * T.stride_p;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int s = s0 + j;
+        if (s < T.ns) {
+            const bool inside = !T.mask_box || in_box(T, p, s);
+            out[ob + static_cast<int64_t>(s) * T.stride_s] = inside ? v[j] * scale : 0.0f;
+        }
     }
 }
 
