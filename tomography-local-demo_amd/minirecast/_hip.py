@@ -329,7 +329,8 @@ class Context:
         return dict(zip(self.DEVICE_BYTES_KEYS, (int(x) for x in out)))
 
     def set_views_per_pass(self, views):
-        """Views per launch of cone slice requests that fill the device (0: all)."""
+        """Views per part of cone requests on kernels 4/6/7 (each part summed in view
+        order, the parts added in part order; 0: one sequential sum)."""
         check(self.lib.mr_set_bp_views_per_pass(self.h, int(views)))
 
     def views_per_pass(self):
