@@ -120,6 +120,25 @@ def test_full_size_linearity_is_exact(full):
     np.testing.assert_array_equal(full["out"]["f32_x2"], 2.0 * full["out"]["f32"])
 
 
+def test_full_size_plane_alone_equals_plane_in_batch(full):
+    """Cone updates on the tile kernels sum fixed parts of the views (64 for 16-bit rings,
+    32 for f32) and add the parts in order, in one launch whatever the request's size, so
+    each plane alone (one third of the grid) equals its values within the three-plane
+    batch bit for bit."""
+    from minirecast import _hip
+    A = full["w"].geom.n_angles
+    T, out_len = full["T"], full["out_len"]
+    for name in ("f32", "f16"):
+        ctx = full["ctx"][name]
+        batch = full["out"][name]
+        for t in T:
+            n2 = int(t['np']) * int(t['ns'])
+            lo = int(t['out_offset'])
+            one = ctx.bp_tasks(np.array([t], dtype=_hip.TASK_DTYPE), A, out_len)
+            np.testing.assert_array_equal(one[lo:lo + n2], batch[lo:lo + n2], err_msg=name)
+        assert ctx.last_tile_launches() == 1
+
+
 def test_full_size_default_f16_kernel_vs_plain_kernel(full):
     ctx = full["ctx"]["f16"]
     cfg = ctx.bp_config()
