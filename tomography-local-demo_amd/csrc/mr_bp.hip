@@ -645,10 +645,10 @@ template <int PPT, int LAY> struct Map4 {
 // Cone beam, views [vb, ve) of n_views: a launch with vb > 0 resumes from the unscaled
 // FP32 sums the previous launch left in `out`, and one with ve < n_views leaves them
 // there, so an update split into several launches equals the single launch bit for bit.
-// part != nullptr (cone beam, small grids): the grid is `parts` copies of the tile grid;
-// copy k sums views [k kv, (k + 1) kv) from zero and leaves the unscaled sums of its
-// tile's points in part[k][tile] (32 x 32 floats), which part_sum_kernel adds in part
-// order.
+// part != nullptr: the grid is `parts` copies of the tile grid; copy k sums views
+// [k kv, (k + 1) kv) from zero and leaves the unscaled sums of its tile's points in
+// part[k][tile] (32 x 32 floats), which part_sum_kernel adds in part order (parallel
+// beam: each part's row-lerped sum, the row of a point being the same in every view).
 template <bool CONE, typename ST, int PPT, int LAY, int DEPTH, int RING, bool SYNC = false>
 __global__ __launch_bounds__(BP_THREADS, (sizeof(ST) == 2 && ((DEPTH == 2 && (RING < 2 || PPT < 4)) || (RING == 2 && PPT == 2))) ? 8 : 1)
 void bp_tiles_kernel4(
@@ -663,7 +663,7 @@ void bp_tiles_kernel4(
     int lb = xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     int pk = 0;                                      // part of a split launch
     const int nbw = static_cast<int>(gridDim.x) / parts;
-    if (CONE && part) {
+    if (part) {
         pk = lb / nbw;
         lb -= pk * nbw;
         vb = pk * kv;
@@ -826,12 +826,14 @@ void bp_tiles_kernel4(
             if (v + j < ve) consume(S[j]);
     }
 
-    if (CONE && part) {
+    if (part) {
         // the point (pt + dp(m), st0 + ds(m)) of the tile, row-major
         float* dst = part + (static_cast<int64_t>(pk) * (nbw / SUBS) + tile) * (BP_TP * BP_TS);
 #pragma unroll
         for (int m = 0; m < PPT; ++m)
-            if (valid[m]) dst[(pt + M::dp(m)) * BP_TS + st0 + M::ds(m)] = acc[m];
+            if (valid[m])
+                dst[(pt + M::dp(m)) * BP_TS + st0 + M::ds(m)] =
+                    CONE ? acc[m] : fmaf(frp[m], accb[m] - acc[m], acc[m]);
         return;
     }
     const bool final_views = !CONE || ve >= n_views;
@@ -2324,14 +2326,14 @@ static void launch_k4(Ctx* c, const ST* ring, int64_t vs, const float* recs, con
                       const Tile* d_tiles, int nt, int n_views, float scale, float* d_out,
                       hipStream_t s) {
     const int nb = Map4<PPT, LAY>::SUBS * nt;
-    // cone beam: one launch of ceil(n_views / bp_vchunk) copies of the grid, copy k
+    // one launch of ceil(n_views / bp_vchunk) copies of the grid, copy k
     // summing views [k bp_vchunk, (k + 1) bp_vchunk) of every tile, then the part sums
     // added in part order.  Small requests fill the device this way, and large ones run
     // faster than one launch per view block with the sums carried through `out` (full_cone
     // 3 x 1024^2 at 1200 views, f16: 2.9 against 3.4 ms; 3 x 128^2: 0.88 against 1.34 ms).
     // Every request whose part sums fit kPartBudget uses the same partition of the views,
     // so a plane gets the same values alone or within any batch.
-    if (CONE && c->bp_vchunk > 0 && !c->vchunk_any && n_views >= 2 * c->bp_vchunk) {
+    if (c->bp_vchunk > 0 && !c->vchunk_any && n_views >= 2 * c->bp_vchunk) {
         const int kv = c->bp_vchunk;
         const int parts = (n_views + kv - 1) / kv;
         const size_t need = sizeof(float) * BP_TP * BP_TS * static_cast<size_t>(nt) * parts;
