@@ -245,7 +245,7 @@ int mr_device_bytes(mr_ctx* ctx, int64_t* out);
 int mr_autotune(mr_ctx* ctx, const mr_task* tasks, int ntasks, int n_views, int64_t out_len,
                 int max_tiles, int reps, int* cfg, float* best_ms);
 
-/* Views per part of kernels 4, 6 and 7 for cone-beam requests: the request runs as one
+/* Views per part of kernels 4, 6 and 7 (cone and parallel beam): a request runs as one
  * launch over view parts of this many views (each part summed in view order, then the
  * parts added in part order), or, when the part sums would exceed 1 GiB, as one launch per
  * block of this many views for single-plane tasks whose grid fills the device (>= 8
@@ -379,8 +379,8 @@ int mr_last_kernel_ms(mr_ctx* ctx, double* ms);
  * mr_last_kernel_ms). */
 int mr_last_tile_kernel_ms(mr_ctx* ctx, double* ms);
 
-/* Dispatches of kernels 4/6/7 within the last mr_bp_tasks: 1 for a cone-beam request
- * run over view parts in one launch (mr_set_bp_views_per_pass), or one per block of views
+/* Dispatches of kernels 4/6/7 within the last mr_bp_tasks: 1 for a request run over view
+ * parts in one launch (mr_set_bp_views_per_pass), or one per block of views
  * when its part sums exceed their budget and it is a request of single-plane tasks whose
  * grid fills the device.  0 for the other kernels.  No counterpart in the reference
  * (measurement only). */

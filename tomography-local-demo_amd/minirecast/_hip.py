@@ -329,7 +329,7 @@ class Context:
         return dict(zip(self.DEVICE_BYTES_KEYS, (int(x) for x in out)))
 
     def set_views_per_pass(self, views):
-        """Views per part of cone requests on kernels 4/6/7 (each part summed in view
+        """Views per part of requests on kernels 4/6/7 (each part summed in view
         order, the parts added in part order; 0: one sequential sum)."""
         check(self.lib.mr_set_bp_views_per_pass(self.h, int(views)))
 
